@@ -1,0 +1,144 @@
+/*
+ * odemi.h -- C ABI of the MI355X ensemble ODE engine (libodemi.so).
+ *
+ * This is the drop-in boundary for the `EnsembleThreads` slot of
+ * `solve(EnsembleProblem(prob; prob_func), Tsit5() | Rodas5P(), ensemblealg;
+ *  trajectories, reltol, abstol, saveat, ...)`.  A host front-end (the Python
+ * package `odemi`, or the Julia `EnsembleMI355X` module through `ccall`)
+ * evaluates `prob_func` on the host, packs the trajectories structure-of-arrays,
+ * traces `f(du,u,p,t)` / `f.jac` / `f.tgrad` into HIP source and calls:
+ *
+ *   odemi_create            <- SciMLBase `EnsembleAlgorithm` instance (EnsembleMI355X())
+ *   odemi_compile_rhs       <- the `ODEFunction` (f, jac, tgrad); the reference
+ *                              specialises the solver on f at
+ *                              lib/DiffEqBase/src/solve.jl:918- (promote_f)
+ *   odemi_solve_batch       <- SciMLBase `__solve(::EnsembleProblem, alg,
+ *                              ::EnsembleThreads; trajectories, kw...)`, which runs
+ *                              `solve(prob_i, alg; kw...)`
+ *                              (lib/DiffEqBase/src/solve.jl:609-670 ->
+ *                              lib/OrdinaryDiffEqCore/src/solve.jl:1-12,904-946)
+ *                              once per trajectory
+ *   retcode[j]              <- `sol.retcode` (SciMLBase.ReturnCode order;
+ *                              lib/DiffEqBase/src/check_error.jl:69-118)
+ *   stats[k*N + j]          <- `sol.stats` (DEStats: nf, naccept, nreject, njacs,
+ *                              nw, nsolve; lib/DiffEqBase/src/stats.jl:1-60)
+ *   odemi_serve_*           <- no counterpart in the reference: a continuous-batching
+ *                              request loop for many concurrent ensemble solves
+ *
+ * Conventions: status return 0 = OK, negative = error, message through
+ * odemi_last_error(ctx).  A bad trajectory never aborts the process: it gets its
+ * own retcode.  Arrays are structure-of-arrays over the N trajectories:
+ * x[i*N + j] is component i of trajectory j.  All pointers in one call are
+ * either host pointers or device pointers (args.device_ptrs).  Calls on one ctx
+ * serialise on its stream; distinct ctx values may be used from distinct threads.
+ */
+#ifndef ODEMI_H
+#define ODEMI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct odemi_ctx odemi_ctx;
+typedef struct odemi_rhs odemi_rhs;
+
+/* algorithms */
+#define ODEMI_TSIT5 0
+#define ODEMI_RODAS5P 1
+/* engines */
+#define ODEMI_ENGINE_PLAIN 0      /* FP64, one kernel launch per step attempt, state in HBM */
+#define ODEMI_ENGINE_PERSISTENT 1 /* persistent kernel + work queue, state in registers/LDS */
+/* precisions (persistent engine) */
+#define ODEMI_FP64 0
+#define ODEMI_MIXED 1             /* Tsit5: FP32 stages, FP64 state/t/error/controller;
+                                     Rodas5P: FP32 LU + FP64 iterative refinement */
+/* odemi_compile_rhs flags */
+#define ODEMI_COMPILE_TSIT5 1
+#define ODEMI_COMPILE_RODAS5P 2
+#define ODEMI_HAS_USER_TGRAD 4    /* f.tgrad supplied: no nf charge for dT */
+#define ODEMI_COMPILE_WG 8        /* also build the workgroup-per-trajectory Rodas5P kernels */
+
+/* SciMLBase.ReturnCode values used per trajectory */
+#define ODEMI_RC_DEFAULT 0
+#define ODEMI_RC_SUCCESS 1
+#define ODEMI_RC_DTNAN 3
+#define ODEMI_RC_MAXITERS 4
+#define ODEMI_RC_DTLESSTHANMIN 6
+#define ODEMI_RC_UNSTABLE 7
+
+typedef struct {
+    int alg;                  /* ODEMI_TSIT5 / ODEMI_RODAS5P */
+    int engine;               /* ODEMI_ENGINE_* */
+    int precision;            /* ODEMI_FP64 / ODEMI_MIXED */
+    int device_ptrs;          /* 1: every array below is a device pointer */
+    int64_t N;                /* trajectories */
+    int nmax;                 /* state dimension (max over trajectories) */
+    int np;                   /* parameters per trajectory */
+    const int32_t* nvec;      /* per-trajectory dimension for uneven batches, or NULL */
+    const double* u0;         /* [nmax][N] */
+    const double* p;          /* [np][N] */
+    const double* t0;         /* [N] */
+    const double* tf;         /* [N] */
+    double reltol, abstol;    /* solve.jl defaults: 1e-3, 1e-6 */
+    double dtmax;             /* <= 0: tf - t0 (solve.jl:153) */
+    double dtmin;             /* 0 (solve.jl:152) */
+    double dt;                /* > 0: initial dt; 0: automatic (initdt.jl) */
+    int64_t maxiters;         /* 1_000_000 (solve.jl:161) */
+    const double* saveat;     /* sorted output times shared by all trajectories, or NULL */
+    int nsaveat;
+    int save_start, save_end, save_everystep;
+    int save_end_user;        /* -1: `save_end = nothing`, 0/1: user Bool */
+    int max_saves;            /* capacity of t_out/u_out per trajectory */
+    double* t_out;            /* [max_saves][N] */
+    double* u_out;            /* [max_saves][nmax][N] */
+    int32_t* nsaved;          /* [N] points saved (may exceed max_saves: extra dropped) */
+    double* u_final;          /* [nmax][N] integrator state at exit (may be NULL) */
+    double* t_final;          /* [N] (may be NULL) */
+    int32_t* retcode;         /* [N] */
+    int32_t* stats;           /* [6][N]: nf, naccept, nreject, njacs, nw, nsolve (may be NULL) */
+    int refine_sweeps;        /* ODEMI_MIXED Rodas5P: FP64 refinement sweeps (default 2) */
+    int block;                /* threads per workgroup (0: engine default) */
+    int waves_per_traj;       /* workgroup kernels: waves cooperating on one trajectory (0: auto) */
+    int sort_by_cost;         /* 1: hand out trajectories longest-span first */
+    void* stream;             /* hipStream_t to run on, or NULL for the ctx stream */
+    double kernel_ms;         /* out: device time of the solve kernels (HIP events) */
+    int64_t launches;         /* out: kernel launches issued */
+} odemi_solve_args;
+
+odemi_ctx* odemi_create(int device);
+void odemi_destroy(odemi_ctx* ctx);
+const char* odemi_last_error(const odemi_ctx* ctx);
+/* compute units, free and total HBM bytes of the ctx device */
+int odemi_device_info(odemi_ctx* ctx, int* cu_count, size_t* free_bytes, size_t* total_bytes);
+/* Version / ABI tag. */
+int odemi_abi_version(void);
+
+/* Compile a traced problem with hipRTC for gfx950.  `problem_src` defines
+ *   template<typename T> __device__ void odemi_f(T* du, const T* u, const T* p, T t);
+ *   template<typename T> __device__ void odemi_jac(T* J, const T* u, const T* p, T t); (row-major n x n)
+ *   template<typename T> __device__ void odemi_tgrad(T* dT, const T* u, const T* p, T t);
+ * and, for uneven batches, the workgroup forms odemi_f_wg / odemi_jac_wg.
+ * n/np are the (maximum) state and parameter counts.  Code objects are cached
+ * on disk by source hash ($ODEMI_CACHE_DIR).  compile_seconds may be NULL. */
+odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src, int n, int np, int flags,
+                             double* compile_seconds);
+void odemi_rhs_destroy(odemi_rhs* rhs);
+
+/* hipRTC-compile a problem into the on-disk code-object cache without touching a
+ * GPU (used by the build step to pre-populate the cache).  Returns 0 on success;
+ * path_out receives the cache file, log_out the compiler log on failure. */
+int odemi_rtc_codeobject(const char* problem_src, int n, int np, int flags, char* path_out, int path_cap,
+                         double* seconds, char* log_out, int log_cap);
+
+/* Solve N independent IVPs (one EnsembleProblem solve).  Synchronous. */
+int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* args);
+
+/* Fill args with the reference's solve defaults for `alg`. */
+void odemi_default_args(odemi_solve_args* args, int alg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

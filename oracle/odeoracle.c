@@ -1,0 +1,989 @@
+/*
+ * odeoracle.c -- CPU restatement of the OrdinaryDiffEq.jl v7.7.0 algorithms on
+ * the ensemble hot path.  TEST INFRASTRUCTURE ONLY: it is the parity checker
+ * for the GPU engine and the CPU baseline timed beside it.  No product code
+ * links, imports or calls anything in this directory.
+ *
+ * Reference files restated (paths relative to the OrdinaryDiffEq.jl tree):
+ *   lib/OrdinaryDiffEqTsit5/src/tsit_perform_step.jl:188-262   (Tsit5 step, FSAL init)
+ *   lib/OrdinaryDiffEqTsit5/src/tsit_tableaus.jl:45-101,244-276 (tableau, interpolant)
+ *   lib/OrdinaryDiffEqTsit5/src/interpolants.jl:24-93           (dense output)
+ *   lib/OrdinaryDiffEqRosenbrock/src/rosenbrock_perform_step.jl:561-847 (Rodas step)
+ *   lib/OrdinaryDiffEqRosenbrock/src/rosenbrock_tableaus.jl:23-73 (Rodas5P tableau)
+ *   lib/OrdinaryDiffEqRosenbrock/src/rosenbrock_interpolants.jl:172-206
+ *   lib/OrdinaryDiffEqDifferentiation/src/derivative_utils.jl:181-227,535-595,663-685,805-935,1011-1041
+ *   lib/OrdinaryDiffEqCore/src/initdt.jl:17-298, integrators/integrator_interface.jl:644-662
+ *   lib/OrdinaryDiffEqCore/src/solve.jl:128-203,904-946
+ *   lib/OrdinaryDiffEqCore/src/integrators/integrator_utils.jl:84-414,493-677,1027-1034,1199-1256,1290-1314
+ *   lib/OrdinaryDiffEqCore/src/integrators/controllers.jl:244-293,744-843; alg_utils.jl:407-879
+ *   lib/DiffEqBase/src/calculate_residuals.jl:9-14, common_defaults.jl:69-75,190-225,
+ *   lib/DiffEqBase/src/check_error.jl:69-118, utils.jl:90-94
+ *
+ * Floating-point conventions of the restatement (shared with the GPU engine by
+ * specification, not by code):
+ *   - every `@muladd` site is an explicit fma(); MuladdMacro nests a sum of
+ *     products left to right: a1*k1 + a2*k2 + a3*k3 -> fma(a3,k3,fma(a2,k2,a1*k1));
+ *   - the fused Rosenbrock helpers (_madd_terms) start from the base value:
+ *     fma(cN,kN,...fma(c1,k1,base));
+ *   - ODE_DEFAULT_NORM accumulates sum += x*x as an fma chain, then sqrt(sum/n);
+ *   - Julia min/max semantics (NaN-propagating) are restated as jl_min/jl_max;
+ *   - compile with -ffp-contract=off so no other contraction happens.
+ * Third-party arithmetic restated (not vendored in the reference tree):
+ *   FastPower.jl 1.1.x `fastpower` (called at controllers.jl:815-816): the
+ *   published fast log2 of Goldberg (rational fit, "line 8 of table 2"),
+ *   followed by exp2 in Float32; here exp2 is evaluated in double and rounded
+ *   to Float32 (correctly rounded Float32 exp2).
+ *   LinearSolve.jl default dense LU: textbook right-looking LU with partial
+ *   pivoting (first maximum on ties), multipliers scaled by the reciprocal of
+ *   the pivot, trailing update a -= l*u as fma.
+ */
+#include "odeoracle.h"
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------------ */
+/* Julia scalar semantics                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Base.eps(x::Float64): spacing to the next float above |x|. */
+double oo_jl_eps(double x) {
+    x = fabs(x);
+    if (!isfinite(x)) return NAN;
+    if (x < DBL_MIN) return 4.9406564584124654e-324;
+    int e;
+    frexp(x, &e);
+    return ldexp(1.0, e - 53);
+}
+
+/* Base.min / Base.max for IEEE floats: NaN-propagating, -0.0 < 0.0. */
+static inline double jl_min(double x, double y) {
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == y) return signbit(x) ? x : y;
+    return x < y ? x : y;
+}
+static inline double jl_max(double x, double y) {
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == y) return signbit(x) ? y : x;
+    return x > y ? x : y;
+}
+/* Base.FastMath.max_fast as used inside calculate_residuals (@fastmath). */
+static inline double max_fast(double x, double y) { return y > x ? y : x; }
+
+/* ------------------------------------------------------------------------ */
+/* FastPower.jl restatement (controllers.jl:815-816 call sites)              */
+/* ------------------------------------------------------------------------ */
+
+float oo_fastlog2(float x) {
+    const float a = 0.338953f, b = 2.198599f, c = 1.523692f;
+    uint32_t ux1i;
+    memcpy(&ux1i, &x, 4);
+    uint32_t ex = (ux1i & 0x7F800000u) >> 23;
+    uint32_t greater = ux1i & 0x00400000u;
+    float signif, fexp;
+    if (greater != 0u) {
+        uint32_t ux2i = (ux1i & 0x007FFFFFu) | 0x3f000000u;
+        memcpy(&signif, &ux2i, 4);
+        fexp = (float)ex - 126.0f;
+        signif = signif - 1.0f;
+    } else {
+        uint32_t ux2i = (ux1i & 0x007FFFFFu) | 0x3f800000u;
+        memcpy(&signif, &ux2i, 4);
+        fexp = (float)ex - 127.0f;
+        signif = signif - 1.0f;
+    }
+    float t1 = a * signif;
+    float t2 = t1 + b;
+    float t3 = signif * t2;
+    float t4 = signif + c;
+    float t5 = t3 / t4;
+    return fexp + t5;
+}
+
+double oo_fastpower(double x, double y) {
+    if (x == 0.0) return 0.0;
+    if (isinf(x) && isinf(y)) return INFINITY;
+    float lg = oo_fastlog2((float)x);
+    float prod = (float)y * lg;
+    float r = (float)exp2((double)prod);
+    return (double)r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tableaus (tsit_tableaus.jl:45-101, 244-276; rosenbrock_tableaus.jl:23-73)  */
+/* ------------------------------------------------------------------------ */
+
+static const double TS_c1 = 0.161, TS_c2 = 0.327, TS_c3 = 0.9, TS_c4 = 0.9800255409045097;
+static const double TS_a21 = 0.161;
+static const double TS_a31 = -0.008480655492356989, TS_a32 = 0.335480655492357;
+static const double TS_a41 = 2.8971530571054935, TS_a42 = -6.359448489975075, TS_a43 = 4.3622954328695815;
+static const double TS_a51 = 5.325864828439257, TS_a52 = -11.748883564062828, TS_a53 = 7.4955393428898365,
+                    TS_a54 = -0.09249506636175525;
+static const double TS_a61 = 5.86145544294642, TS_a62 = -12.92096931784711, TS_a63 = 8.159367898576159,
+                    TS_a64 = -0.071584973281401, TS_a65 = -0.028269050394068383;
+static const double TS_a71 = 0.09646076681806523, TS_a72 = 0.01, TS_a73 = 0.4798896504144996,
+                    TS_a74 = 1.379008574103742, TS_a75 = -3.290069515436081, TS_a76 = 2.324710524099774;
+static const double TS_bt1 = -0.00178001105222577714, TS_bt2 = -0.0008164344596567469,
+                    TS_bt3 = 0.007880878010261995, TS_bt4 = -0.1447110071732629,
+                    TS_bt5 = 0.5823571654525552, TS_bt6 = -0.45808210592918697,
+                    TS_bt7 = 0.015151515151515152;
+/* Tsit5Interp r_ij (coefficient of Theta^j in b_i(Theta)). */
+static const double TS_R[7][4] = {
+    {1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+    {0.0, 0.13169999999999998, -0.2234, 0.1017},
+    {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+    {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+    {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+    {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
+    {0.0, 1.5, -4.0, 2.5},
+};
+
+void oo_tsit5_tableau(double* c, double* a, double* b, double* btilde) {
+    double cc[7] = {0.0, TS_c1, TS_c2, TS_c3, TS_c4, 1.0, 1.0};
+    double aa[7][7] = {{0}};
+    aa[1][0] = TS_a21;
+    aa[2][0] = TS_a31; aa[2][1] = TS_a32;
+    aa[3][0] = TS_a41; aa[3][1] = TS_a42; aa[3][2] = TS_a43;
+    aa[4][0] = TS_a51; aa[4][1] = TS_a52; aa[4][2] = TS_a53; aa[4][3] = TS_a54;
+    aa[5][0] = TS_a61; aa[5][1] = TS_a62; aa[5][2] = TS_a63; aa[5][3] = TS_a64; aa[5][4] = TS_a65;
+    aa[6][0] = TS_a71; aa[6][1] = TS_a72; aa[6][2] = TS_a73; aa[6][3] = TS_a74; aa[6][4] = TS_a75;
+    aa[6][5] = TS_a76;
+    double bt[7] = {TS_bt1, TS_bt2, TS_bt3, TS_bt4, TS_bt5, TS_bt6, TS_bt7};
+    for (int i = 0; i < 7; i++) {
+        c[i] = cc[i];
+        b[i] = aa[6][i];
+        btilde[i] = bt[i];
+        for (int j = 0; j < 7; j++) a[i * 7 + j] = aa[i][j];
+    }
+}
+
+void oo_tsit5_interp_coeffs(double* r) {
+    for (int i = 0; i < 7; i++)
+        for (int j = 0; j < 4; j++) r[i * 4 + j] = TS_R[i][j];
+}
+
+static const double RP_A[8][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0},
+    {3.0, 0, 0, 0, 0, 0, 0, 0},
+    {2.849394379747939, 0.45842242204463923, 0, 0, 0, 0, 0, 0},
+    {-6.954028509809101, 2.489845061869568, -10.358996098473584, 0, 0, 0, 0, 0},
+    {2.8029986275628964, 0.5072464736228206, -0.3988312541770524, -0.04721187230404641, 0, 0, 0, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946, 0, 0, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946, 1, 0, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946, 1, 1, 0},
+};
+static const double RP_C[8][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {-14.155112264123755, 0, 0, 0, 0, 0, 0},
+    {-17.97296035885952, -2.859693295451294, 0, 0, 0, 0, 0},
+    {147.12150275711716, -1.41221402718213, 71.68940251302358, 0, 0, 0, 0},
+    {165.43517024871676, -0.4592823456491126, 42.90938336958603, -5.961986721573306, 0, 0, 0},
+    {24.854864614690072, -3.0009227002832186, 47.4931110020768, 5.5814197821558125,
+     -0.6610691825249471, 0, 0},
+    {30.91273214028599, -3.1208243349937974, 77.79954646070892, 34.28646028294783,
+     -19.097331116725623, -28.087943162872662, 0},
+    {37.80277123390563, -3.2571969029072276, 112.26918849496327, 66.9347231244047,
+     -40.06618937091002, -54.66780262877968, -9.48861652309627},
+};
+static const double RP_c[8] = {0, 0.6358126895828704, 0.4095798393397535, 0.9769306725060716,
+                               0.4288403609558664, 1, 1, 1};
+static const double RP_d[8] = {0.21193756319429014, -0.42387512638858027, -0.3384627126235924,
+                               1.8046452872882734, 2.325825639765069, 0, 0, 0};
+static const double RP_H[3][8] = {
+    {25.948786856663858, -2.5579724845846235, 10.433815404888879, -2.3679251022685204,
+     0.524948541321073, 1.1241088310450404, 0.4272876194431874, -0.17202221070155493},
+    {-9.91568850695171, -0.9689944594115154, 3.0438037242978453, -24.495224566215796,
+     20.176138334709044, 15.98066361424651, -6.789040303419874, -6.710236069923372},
+    {11.419903575922262, 2.8879645146136994, 72.92137995996029, 80.12511834622643,
+     -52.072871366152654, -59.78993625266729, -0.15582684282751913, 4.883087185713722},
+};
+static const double RP_gamma = 0.21193756319429014;
+
+void oo_rodas5p_tableau(double* A, double* C, double* c, double* d, double* H, double* b,
+                        double* btilde, double* gamma) {
+    for (int i = 0; i < 8; i++) {
+        for (int j = 0; j < 8; j++) {
+            A[i * 8 + j] = RP_A[i][j];
+            C[i * 8 + j] = (j < 7) ? RP_C[i][j] : 0.0;
+        }
+        c[i] = RP_c[i];
+        d[i] = RP_d[i];
+        b[i] = (i < 7) ? RP_A[7][i] : 1.0;
+        btilde[i] = (i == 7) ? 1.0 : 0.0;
+    }
+    for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 8; i++) H[j * 8 + i] = RP_H[j][i];
+    *gamma = RP_gamma;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Norm and residuals (common_defaults.jl:69-75, calculate_residuals.jl:9-14) */
+/* ------------------------------------------------------------------------ */
+
+static double ode_norm(const double* x, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s = fma(x[i], x[i], s);
+    return sqrt(s / (double)(n > 1 ? n : 1));
+}
+
+/* EEst = ODE_DEFAULT_NORM(utilde ./ (abstol .+ max.(|u0|,|u1|) .* reltol)) */
+static double residual_norm(const double* ut, const double* u0, const double* u1, int n,
+                            double abstol, double reltol) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) {
+        double den = fma(max_fast(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
+        double r = ut[i] / den;
+        s = fma(r, r, s);
+    }
+    return sqrt(s / (double)(n > 1 ? n : 1));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dense LU (LinearSolve default, restated) -- see header comment            */
+/* ------------------------------------------------------------------------ */
+
+static void lu_factor(int n, double* A, int* piv) {
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double amax = fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; i++) {
+            double v = fabs(A[i * n + k]);
+            if (v > amax) { amax = v; p = i; }
+        }
+        piv[k] = p;
+        if (p != k)
+            for (int j = 0; j < n; j++) {
+                double tmp = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = tmp;
+            }
+        double inv = 1.0 / A[k * n + k];
+        for (int i = k + 1; i < n; i++) {
+            double l = A[i * n + k] * inv;
+            A[i * n + k] = l;
+            for (int j = k + 1; j < n; j++) A[i * n + j] = fma(-l, A[k * n + j], A[i * n + j]);
+        }
+    }
+}
+
+static void lu_solve(int n, const double* LU, const int* piv, double* b) {
+    for (int k = 0; k < n; k++) {
+        int p = piv[k];
+        if (p != k) { double t = b[k]; b[k] = b[p]; b[p] = t; }
+    }
+    for (int i = 1; i < n; i++) {
+        double acc = b[i];
+        for (int j = 0; j < i; j++) acc = fma(-LU[i * n + j], b[j], acc);
+        b[i] = acc;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double acc = b[i];
+        for (int j = i + 1; j < n; j++) acc = fma(-LU[i * n + j], b[j], acc);
+        b[i] = acc / LU[i * n + i];
+    }
+}
+
+int oo_lu_solve(int n, double* A, int* piv, double* b) {
+    lu_factor(n, A, piv);
+    lu_solve(n, A, piv, b);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Integrator state                                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int n, alg;
+    oo_rhs_fn f; oo_jac_fn jac; oo_tgrad_fn tgrad; void* user;
+    const double* p;
+    const oo_opts* o;
+    double reltol, abstol, dtmax, dtmin, t0, tf;
+    /* controller constants (alg_utils.jl:407-879, controllers.jl:744-803) */
+    double qmin, qmax, qmax_first, gamma, beta1, beta2, qoldinit, qsteady_min, qsteady_max;
+    /* integrator scalars */
+    double t, dt, dtpropose, tprev, tstop_target, EEst, q11, errold;
+    int64_t iter, success_iter;
+    int accept_step, next_step_tstop;
+    oo_result* res;
+    /* vectors */
+    double *u, *uprev, *tmp, *atmp;
+    double* k[8];          /* Tsit5: k1..k7;  Rodas: ks1..ks8 */
+    double *du, *dT, *lt, *fsal, *J, *W, *dense[3];
+    int* piv;
+    /* saving */
+    double* t_out; double* u_out; int max_out;
+    double* saveat_int; int n_saveat, saveat_pos;
+} integ_t;
+
+static void save_point(integ_t* I, double t, const double* v) {
+    int s = I->res->nsaved;
+    if (s < I->max_out) {
+        if (I->t_out) I->t_out[s] = t;
+        if (I->u_out) memcpy(I->u_out + (size_t)s * I->n, v, sizeof(double) * I->n);
+    }
+    I->res->nsaved = s + 1;
+}
+
+static inline void F(integ_t* I, double* du, const double* u, double t) {
+    I->f(du, u, I->p, t, I->user);
+    I->res->nf++;
+}
+
+/* initdt.jl:17-298 (_ode_initdt_iip, no mass matrix, no noise), called through
+ * integrator_interface.jl:644-662 with dtmax = min(opts.dtmax, tstop - t). */
+static double initdt(integ_t* I) {
+    const int n = I->n;
+    const double t = I->t;
+    const double dtmax_tdir = jl_min(fabs(I->dtmax), fabs(I->tf - t));
+    const double dtmin = nextafter(jl_max(I->dtmin, oo_jl_eps(t)), INFINITY);
+    const double smalldt = jl_max(dtmin, 1e-6);
+    if (n == 0) return jl_max(smalldt, dtmin);
+    double* sk = I->atmp;      /* scratch */
+    double* f0 = I->du;
+    double* f1 = I->dT;
+    double* u1 = I->tmp;
+    double* tmp = I->lt;
+    for (int i = 0; i < n; i++) sk[i] = fma(fabs(I->u[i]), I->reltol, I->abstol);
+    I->f(f0, I->u, I->p, t, I->user);
+    for (int i = 0; i < n; i++) tmp[i] = I->u[i] / sk[i];
+    double d0 = ode_norm(tmp, n);
+    for (int i = 0; i < n; i++) tmp[i] = f0[i] / sk[i] * 1.0;
+    double d1 = ode_norm(tmp, n);
+    if (isnan(d1)) return dtmin;
+    double dt0 = ((d0 < 1e-5) | (d1 < 1e-5)) ? smalldt : (d0 / d1) / 100.0;
+    dt0 = jl_min(dt0, dtmax_tdir);
+    if (dt0 < 10.0 * DBL_EPSILON) return jl_max(smalldt, dtmin);
+    for (int i = 0; i < n; i++) u1[i] = fma(dt0, f0[i], I->u[i]);
+    I->f(f1, u1, I->p, t + dt0, I->user);
+    int same = 1;
+    for (int i = 0; i < n; i++) if (!(f0[i] == f1[i])) { same = 0; break; }
+    if (same) return jl_max(dtmin, 100.0 * dt0);
+    for (int i = 0; i < n; i++) tmp[i] = (f1[i] - f0[i]) / sk[i] * 1.0;
+    double d2 = ode_norm(tmp, n) / dt0 * 1.0;
+    double mx = jl_max(d1, d2);
+    double dt1;
+    if (mx < 1e-15) /* `<= 1//10^15` evaluated exactly */
+        dt1 = jl_max(1e-6, dt0 * 0.001);
+    else
+        dt1 = pow(10.0, -(2.0 + log10(mx)) / 5.0);
+    return jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
+}
+
+/* integrator_utils.jl:268-324 (adaptive branch, tstops = [tf], tdir = +1) */
+static void modify_dt_for_tstops(integ_t* I) {
+    double distance = fabs(I->tf - I->t);
+    double tol = 0.0;
+    if (isfinite(I->tf) && isfinite(I->t))
+        tol = 100.0 * oo_jl_eps(jl_max(fabs(I->t), fabs(I->tf)));
+    double original_dt = fabs(I->dt);
+    I->dtpropose = original_dt;
+    if (original_dt + tol < distance) {
+        I->next_step_tstop = 0;
+    } else {
+        I->next_step_tstop = 1;
+        I->tstop_target = I->tf;
+    }
+    I->dt = jl_min(original_dt, distance);
+}
+
+/* integrator_utils.jl:1243-1256 with utils.jl:90-94 */
+static void fix_dt_at_bounds(integ_t* I) {
+    I->dt = jl_min(I->dtmax, I->dt);
+    double dtmin_t = fabs(jl_max(oo_jl_eps(I->t), I->dtmin));
+    I->dt = jl_max(I->dt, dtmin_t);
+}
+
+/* controllers.jl:805-821 */
+static double stepsize_controller(integ_t* I) {
+    double qmax = (I->success_iter == 0) ? I->qmax_first : I->qmax;
+    double q;
+    if (I->EEst == 0.0) {
+        q = 1.0 / qmax;
+    } else {
+        double q11 = oo_fastpower(I->EEst, I->beta1);
+        q = q11 / oo_fastpower(I->errold, I->beta2);
+        I->q11 = q11;
+        double x = q / I->gamma, lo = 1.0 / qmax, hi = 1.0 / I->qmin;
+        q = (x > hi) ? hi : ((x < lo) ? lo : x);
+    }
+    return q;
+}
+
+/* controllers.jl:823-836 */
+static double step_accept_controller(integ_t* I, double q) {
+    if (I->qsteady_min <= q && q <= I->qsteady_max) q = 1.0;
+    I->errold = jl_max(I->EEst, I->qoldinit);
+    return I->dt / q;
+}
+
+/* controllers.jl:838-843 */
+static void step_reject_controller(integ_t* I) {
+    I->dt = I->dt / jl_min(1.0 / I->qmin, I->q11 / I->gamma);
+}
+
+/* integrator_utils.jl:1199-1210 */
+static void calc_dt_propose(integ_t* I, double dtnew) {
+    double dp = jl_min(fabs(I->dtmax), fabs(dtnew));
+    I->dtpropose = jl_max(fabs(dp), fabs(jl_max(oo_jl_eps(I->t), I->dtmin)));
+}
+
+/* check_error.jl:69-118 (adaptive, force_dtmin = false, default unstable_check) */
+static int check_error(integ_t* I) {
+    if (isnan(I->dt)) return OO_RC_DTNAN;
+    if (I->iter > I->o->maxiters) return OO_RC_MAXITERS;
+    if (fabs(I->dt) <= fabs(I->dtmin) && (!I->accept_step || I->t + I->dt < I->tf))
+        return OO_RC_DTLESSTHANMIN;
+    else if (!I->accept_step && fabs(I->dt) <= fabs(oo_jl_eps(I->t)))
+        return OO_RC_UNSTABLE;
+    if (I->accept_step) {
+        for (int i = 0; i < I->n; i++)
+            if (!isfinite(I->u[i])) return OO_RC_UNSTABLE;
+    }
+    return OO_RC_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tsit5 perform_step! (tsit_perform_step.jl:204-262, @muladd)                */
+/* ------------------------------------------------------------------------ */
+
+static void tsit5_step(integ_t* I) {
+    const int n = I->n;
+    const double t = I->t, dt = I->dt;
+    double *k1 = I->k[0], *k2 = I->k[1], *k3 = I->k[2], *k4 = I->k[3], *k5 = I->k[4],
+           *k6 = I->k[5], *k7 = I->k[6];
+    double *tmp = I->tmp, *uprev = I->uprev, *u = I->u;
+    const double a = dt * TS_a21;
+    for (int i = 0; i < n; i++) tmp[i] = fma(a, k1[i], uprev[i]);
+    F(I, k2, tmp, fma(TS_c1, dt, t));
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a31 * k1[i];
+        acc = fma(TS_a32, k2[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    F(I, k3, tmp, fma(TS_c2, dt, t));
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a41 * k1[i];
+        acc = fma(TS_a42, k2[i], acc);
+        acc = fma(TS_a43, k3[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    F(I, k4, tmp, fma(TS_c3, dt, t));
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a51 * k1[i];
+        acc = fma(TS_a52, k2[i], acc);
+        acc = fma(TS_a53, k3[i], acc);
+        acc = fma(TS_a54, k4[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    F(I, k5, tmp, fma(TS_c4, dt, t));
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a61 * k1[i];
+        acc = fma(TS_a62, k2[i], acc);
+        acc = fma(TS_a63, k3[i], acc);
+        acc = fma(TS_a64, k4[i], acc);
+        acc = fma(TS_a65, k5[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    F(I, k6, tmp, t + dt);
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a71 * k1[i];
+        acc = fma(TS_a72, k2[i], acc);
+        acc = fma(TS_a73, k3[i], acc);
+        acc = fma(TS_a74, k4[i], acc);
+        acc = fma(TS_a75, k5[i], acc);
+        acc = fma(TS_a76, k6[i], acc);
+        u[i] = fma(dt, acc, uprev[i]);
+    }
+    F(I, k7, u, t + dt);
+    double* ut = I->atmp;
+    for (int i = 0; i < n; i++) {
+        double acc = TS_bt1 * k1[i];
+        acc = fma(TS_bt2, k2[i], acc);
+        acc = fma(TS_bt3, k3[i], acc);
+        acc = fma(TS_bt4, k4[i], acc);
+        acc = fma(TS_bt5, k5[i], acc);
+        acc = fma(TS_bt6, k6[i], acc);
+        acc = fma(TS_bt7, k7[i], acc);
+        ut[i] = dt * acc;
+    }
+    I->EEst = residual_norm(ut, uprev, u, n, I->abstol, I->reltol);
+}
+
+/* interpolants.jl:52-64 with tsit5pre0 (@evalpoly = Horner via muladd) */
+static void tsit5_interp(integ_t* I, double theta, double dt, double* out) {
+    const int n = I->n;
+    double th2 = theta * theta;
+    double b[7];
+    b[0] = theta * fma(theta, fma(theta, fma(theta, TS_R[0][3], TS_R[0][2]), TS_R[0][1]), TS_R[0][0]);
+    for (int s = 1; s < 7; s++)
+        b[s] = th2 * fma(theta, fma(theta, TS_R[s][3], TS_R[s][2]), TS_R[s][1]);
+    for (int i = 0; i < n; i++) {
+        double acc = I->k[0][i] * b[0];
+        for (int s = 1; s < 7; s++) acc = fma(I->k[s][i], b[s], acc);
+        out[i] = fma(dt, acc, I->uprev[i]);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rodas5P perform_step! (rosenbrock_perform_step.jl:729-847)                 */
+/* ------------------------------------------------------------------------ */
+
+static void rodas_step(integ_t* I) {
+    const int n = I->n;
+    const double t = I->t, dt = I->dt;
+    double *uprev = I->uprev, *u = I->u, *du = I->du, *dT = I->dT, *lt = I->lt;
+    double dtC[8][7], dtd[8];
+    const double idt = 1.0 / dt;
+    for (int s = 0; s < 8; s++) {
+        for (int j = 0; j < 7; j++) dtC[s][j] = RP_C[s][j] * idt;
+        dtd[s] = dt * RP_d[s];
+    }
+    const double dtgamma = dt * RP_gamma;
+    /* fsalfirst = f(uprev, t) */
+    F(I, I->fsal, uprev, t);
+    /* calc_rosenbrock_differentiation! -> calc_W! -> do_newJW
+     * (derivative_utils.jl:535-595, 805-935, 1011-1041) */
+    int new_jac;
+    if (I->iter <= 1) new_jac = 1;
+    else {
+        int errorfail = I->EEst > 1.0;
+        new_jac = !(I->iter > 1 && errorfail);
+    }
+    if (new_jac) {
+        I->jac(I->J, uprev, I->p, t, I->user);
+        I->res->njacs++;
+    }
+    /* jacobian2W! (derivative_utils.jl:663-685), mass matrix = I */
+    const double invdtgamma = 1.0 / dtgamma;
+    for (int i = 0; i < n * n; i++) I->W[i] = I->J[i];
+    for (int i = 0; i < n; i++) I->W[i * n + i] = fma(-1.0, invdtgamma, I->J[i * n + i]);
+    I->res->nw++;
+    /* calc_tderivative! (derivative_utils.jl:181-227) */
+    if (new_jac) {
+        if (I->tgrad) I->tgrad(dT, uprev, I->p, t, I->user);
+        else for (int i = 0; i < n; i++) dT[i] = 0.0;
+        if (!I->o->has_tgrad) I->res->nf++;   /* AD derivative costs one f call */
+    }
+    for (int i = 0; i < n; i++) lt[i] = I->fsal[i] + dtd[0] * dT[i];
+    /* dolinsolve with A = W (refactorize) */
+    lu_factor(n, I->W, I->piv);
+    for (int i = 0; i < n; i++) I->tmp[i] = lt[i];
+    lu_solve(n, I->W, I->piv, I->tmp);
+    for (int i = 0; i < n; i++) I->k[0][i] = -I->tmp[i];
+    I->res->nsolve++;
+    for (int s = 1; s < 8; s++) {
+        /* _stage_accum! : u = fma chain from uprev */
+        for (int i = 0; i < n; i++) {
+            double acc = uprev[i];
+            for (int j = 0; j < s; j++) acc = fma(RP_A[s][j], I->k[j][i], acc);
+            u[i] = acc;
+        }
+        F(I, du, u, fma(RP_c[s], dt, t));
+        /* _stage_rhs! */
+        for (int i = 0; i < n; i++) {
+            double acc = fma(dtd[s], dT[i], du[i]);
+            for (int j = 0; j < s; j++) acc = fma(dtC[s][j], I->k[j][i], acc);
+            I->tmp[i] = acc;
+        }
+        lu_solve(n, I->W, I->piv, I->tmp);
+        for (int i = 0; i < n; i++) I->k[s][i] = -I->tmp[i];
+        I->res->nsolve++;
+    }
+    /* u = uprev + sum b_i ks_i ; b = A[8,1:7] ++ [1] */
+    for (int i = 0; i < n; i++) {
+        double acc = uprev[i];
+        for (int j = 0; j < 7; j++) acc = fma(RP_A[7][j], I->k[j][i], acc);
+        acc = fma(1.0, I->k[7][i], acc);
+        u[i] = acc;
+    }
+    /* err = sum btilde_i ks_i (btilde = e8), base zero */
+    for (int i = 0; i < n; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < 7; j++) acc = fma(0.0, I->k[j][i], acc);
+        acc = fma(1.0, I->k[7][i], acc);
+        du[i] = acc;
+    }
+    I->EEst = residual_norm(du, uprev, u, n, I->abstol, I->reltol);
+    /* dense rows (calck <=> saveat non-empty) */
+    if (I->n_saveat > 0) {
+        for (int r = 0; r < 3; r++)
+            for (int i = 0; i < n; i++) {
+                double acc = 0.0;
+                for (int j = 0; j < 8; j++) acc = fma(RP_H[r][j], I->k[j][i], acc);
+                I->dense[r][i] = acc;
+            }
+    }
+}
+
+/* rosenbrock_interpolants.jl:172-206, interp_order = 3 */
+static void rodas_interp(integ_t* I, double theta, double* out) {
+    const double th1 = 1.0 - theta;
+    for (int i = 0; i < I->n; i++) {
+        double in1 = fma(theta, I->dense[2][i], I->dense[1][i]);
+        double in2 = fma(theta, in1, I->dense[0][i]);
+        double in3 = fma(th1, in2, I->u[i]);
+        out[i] = fma(theta, in3, th1 * I->uprev[i]);
+    }
+}
+
+/* integrator_utils.jl:340-414 (savevalues!, save_idxs = nothing) */
+static void savevalues(integ_t* I) {
+    const oo_opts* o = I->o;
+    while (I->saveat_pos < I->n_saveat && I->saveat_int[I->saveat_pos] <= I->t) {
+        double curt = I->saveat_int[I->saveat_pos++];
+        if (curt != I->t) {
+            double theta = (curt - I->tprev) / I->dt;
+            if (I->alg == OO_TSIT5) tsit5_interp(I, theta, I->dt, I->tmp);
+            else rodas_interp(I, theta, I->tmp);
+            save_point(I, curt, I->tmp);
+        } else {
+            if (curt == I->tf && !o->save_end) continue;
+            save_point(I, I->t, I->u);
+        }
+    }
+    if (o->save_everystep) {
+        int s = I->res->nsaved;
+        int empty = (s == 0);
+        double last = (!empty && s - 1 < I->max_out && I->t_out) ? I->t_out[s - 1] : NAN;
+        if (empty || ((I->t != last || I->dt == 0.0) && (o->save_end || I->t != I->tf)))
+            save_point(I, I->t, I->u);
+    }
+}
+
+static int cmp_double(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+void oo_default_opts(oo_opts* o, int alg) {
+    memset(o, 0, sizeof(*o));
+    o->alg = alg;
+    o->reltol = 1e-3;
+    o->abstol = 1e-6;
+    o->dtmax = NAN;
+    o->dtmin = 0.0;
+    o->dt = 0.0;
+    o->maxiters = 1000000;
+    o->save_start = 1;
+    o->save_end = 1;
+    o->save_everystep = 0;
+    o->save_end_user = -1;
+}
+
+int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
+             const double* u0, const double* p, double t0, double tf,
+             const oo_opts* opts, double* t_out, double* u_out, int max_out,
+             double* u_final, oo_result* res) {
+    integ_t S;
+    integ_t* I = &S;
+    memset(I, 0, sizeof(*I));
+    memset(res, 0, sizeof(*res));
+    I->n = n; I->alg = opts->alg; I->f = f; I->jac = jac; I->tgrad = tgrad; I->user = user;
+    I->p = p; I->o = opts; I->res = res;
+    I->reltol = opts->reltol; I->abstol = opts->abstol;
+    I->t0 = t0; I->tf = tf;
+    I->dtmax = (isnan(opts->dtmax) || opts->dtmax <= 0.0) ? (tf - t0) : opts->dtmax;
+    I->dtmin = opts->dtmin;
+    I->t_out = t_out; I->u_out = u_out; I->max_out = max_out;
+    /* controller defaults: qmin 1//5, qmax 10, qmax_first_step 10000, gamma 9//10,
+     * beta2 = 2//(5*5), beta1 = 7//(10*5), qoldinit 1//10^4,
+     * qsteady = [1,1] (Tsit5) or [1, 6//5] (Rodas5P, adaptive implicit) */
+    I->qmin = 0.2; I->qmax = 10.0; I->qmax_first = 10000.0; I->gamma = 0.9;
+    I->beta2 = 0.08; I->beta1 = 0.14; I->qoldinit = 1e-4;
+    I->qsteady_min = 1.0; I->qsteady_max = (I->alg == OO_RODAS5P) ? 1.2 : 1.0;
+
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    double* mem = (double*)calloc(nn * 24 + nn * nn * 2 + 8, sizeof(double));
+    int* piv = (int*)calloc(nn, sizeof(int));
+    double* m = mem;
+    I->u = m; m += nn; I->uprev = m; m += nn; I->tmp = m; m += nn; I->atmp = m; m += nn;
+    for (int s = 0; s < 8; s++) { I->k[s] = m; m += nn; }
+    I->du = m; m += nn; I->dT = m; m += nn; I->lt = m; m += nn; I->fsal = m; m += nn;
+    for (int r = 0; r < 3; r++) { I->dense[r] = m; m += nn; }
+    I->J = m; m += nn * nn; I->W = m; m += nn * nn;
+    I->piv = piv;
+
+    /* saveat_internal (solve.jl:1103-1123): times in (t0, tf], sorted */
+    I->saveat_int = NULL; I->n_saveat = 0;
+    if (opts->nsaveat > 0) {
+        I->saveat_int = (double*)malloc(sizeof(double) * opts->nsaveat);
+        for (int i = 0; i < opts->nsaveat; i++) {
+            double s = opts->saveat[i];
+            if (t0 < s && s <= tf) I->saveat_int[I->n_saveat++] = s;
+        }
+        qsort(I->saveat_int, I->n_saveat, sizeof(double), cmp_double);
+    }
+
+    memcpy(I->u, u0, sizeof(double) * n);
+    memcpy(I->uprev, u0, sizeof(double) * n);
+    I->t = t0; I->tprev = t0;
+    I->EEst = 1.0; I->q11 = 1.0; I->errold = I->qoldinit;
+    I->iter = 0; I->success_iter = 0; I->accept_step = 0; I->next_step_tstop = 0;
+    I->tstop_target = 0.0;
+
+    if (opts->save_start) save_point(I, t0, I->u);
+    /* initialize! (tsit_perform_step.jl:188-202): FSAL pre-start */
+    if (I->alg == OO_TSIT5) F(I, I->k[0], I->uprev, I->t);
+    /* handle_dt! (solve.jl:968-985) */
+    if (opts->dt > 0.0) {
+        I->dt = opts->dt; I->dtpropose = opts->dt;
+    } else {
+        I->dt = initdt(I);
+        I->dtpropose = I->dt;
+        res->nf += 2;
+    }
+
+    int rc = OO_RC_SUCCESS;
+    /* solve! (solve.jl:904-946) with tstops = [tf] */
+    while (I->t < I->tf) {
+        /* loopheader! (integrator_utils.jl:84-127) */
+        if (I->iter > 0) {
+            if (I->accept_step) {
+                I->success_iter++;
+                /* apply_step! (175-203) */
+                memcpy(I->uprev, I->u, sizeof(double) * n);
+                I->dt = I->dtpropose;
+                if (I->alg == OO_TSIT5) memcpy(I->k[0], I->k[6], sizeof(double) * n);
+                modify_dt_for_tstops(I);
+            } else {
+                step_reject_controller(I);
+            }
+        }
+        I->iter++;
+        fix_dt_at_bounds(I);
+        modify_dt_for_tstops(I);
+        rc = check_error(I);
+        if (rc != OO_RC_SUCCESS) break;
+        if (I->next_step_tstop && fabs(I->dt) < oo_jl_eps(fabs(I->t))) {
+            I->accept_step = 1;   /* handle_tstop_step!: skip the tiny step */
+        } else if (I->alg == OO_TSIT5) {
+            tsit5_step(I);
+        } else {
+            rodas_step(I);
+        }
+        /* _loopfooter! (integrator_utils.jl:597-677) */
+        double ttmp = I->t + I->dt;
+        double q = stepsize_controller(I);
+        I->accept_step = (I->EEst <= 1.0);
+        if (I->accept_step) {
+            res->naccept++;
+            I->tprev = I->t;
+            if (I->next_step_tstop) I->dt = I->dtpropose;
+            if (I->next_step_tstop) { I->next_step_tstop = 0; I->t = I->tstop_target; }
+            else I->t = ttmp;
+            double dtnew = step_accept_controller(I, q);
+            calc_dt_propose(I, dtnew);
+            savevalues(I);
+        } else {
+            res->nreject++;
+        }
+    }
+    if (rc == OO_RC_SUCCESS) {
+        /* postamble! -> solution_endpoint_match_cur_integrator! (493-588) */
+        if (opts->save_end) {
+            int s = res->nsaved;
+            int doit = (s == 0);
+            if (!doit) {
+                double last = (s - 1 < max_out && t_out) ? t_out[s - 1] : NAN;
+                int in_cache = 0;
+                for (int i = 0; i < opts->nsaveat; i++) if (opts->saveat[i] == I->t) in_cache = 1;
+                doit = (last != I->t) &&
+                       (opts->save_end_user == 1 || in_cache || I->t == tf || opts->nsaveat == 0);
+            }
+            if (doit) save_point(I, I->t, I->u);
+        }
+    }
+    res->retcode = rc;
+    res->t_final = I->t;
+    res->dt_last = I->dt;
+    if (u_final) memcpy(u_final, I->u, sizeof(double) * n);
+    free(I->saveat_int);
+    free(mem);
+    free(piv);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Built-in problem families                                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { int n; } builtin_user;
+
+/* Lorenz: p = (sigma, rho, beta) */
+static void lorenz_f(double* du, const double* u, const double* p, double t, void* user) {
+    (void)t; (void)user;
+    double x = u[0], y = u[1], z = u[2];
+    du[0] = p[0] * (y - x);
+    du[1] = x * (p[1] - z) - y;
+    du[2] = x * y - p[2] * z;
+}
+static void lorenz_jac(double* J, const double* u, const double* p, double t, void* user) {
+    (void)t; (void)user;
+    J[0] = -p[0]; J[1] = p[0]; J[2] = 0.0;
+    J[3] = p[1] - u[2]; J[4] = -1.0; J[5] = -u[0];
+    J[6] = u[1]; J[7] = u[0]; J[8] = -p[2];
+}
+/* Robertson: p = (k1, k2, k3) */
+static void rober_f(double* du, const double* u, const double* p, double t, void* user) {
+    (void)t; (void)user;
+    double y1 = u[0], y2 = u[1], y3 = u[2];
+    du[0] = -p[0] * y1 + p[2] * y2 * y3;
+    du[1] = p[0] * y1 - p[1] * (y2 * y2) - p[2] * y2 * y3;
+    du[2] = p[1] * (y2 * y2);
+}
+static void rober_jac(double* J, const double* u, const double* p, double t, void* user) {
+    (void)t; (void)user;
+    double y2 = u[1], y3 = u[2];
+    J[0] = -p[0]; J[1] = p[2] * y3; J[2] = p[2] * y2;
+    J[3] = p[0]; J[4] = -2.0 * p[1] * y2 - p[2] * y3; J[5] = -p[2] * y2;
+    J[6] = 0.0; J[7] = 2.0 * p[1] * y2; J[8] = 0.0;
+}
+/* Oscillator chain, n = 2m: u = (x_1..x_m, v_1..v_m), p = (k, c, alpha),
+ * fixed ends x_0 = x_{m+1} = 0:
+ *   x_i' = v_i ;  v_i' = -k (2 x_i - x_{i-1} - x_{i+1}) - c v_i - alpha x_i^3 */
+static void chain_f(double* du, const double* u, const double* p, double t, void* user) {
+    (void)t;
+    int n = ((builtin_user*)user)->n, m = n / 2;
+    const double k = p[0], c = p[1], al = p[2];
+    for (int i = 0; i < m; i++) du[i] = u[m + i];
+    for (int i = 0; i < m; i++) {
+        double x = u[i];
+        double s = 2.0 * x;
+        if (i > 0) s = s - u[i - 1];
+        if (i < m - 1) s = s - u[i + 1];
+        du[m + i] = -k * s - c * u[m + i] - al * (x * x * x);
+    }
+}
+static void chain_jac(double* J, const double* u, const double* p, double t, void* user) {
+    (void)t;
+    int n = ((builtin_user*)user)->n, m = n / 2;
+    const double k = p[0], c = p[1], al = p[2];
+    memset(J, 0, sizeof(double) * n * n);
+    for (int i = 0; i < m; i++) J[i * n + m + i] = 1.0;
+    for (int i = 0; i < m; i++) {
+        double x = u[i];
+        int r = m + i;
+        J[r * n + i] = -k * 2.0 - al * (3.0 * (x * x));
+        if (i > 0) J[r * n + i - 1] = k;
+        if (i < m - 1) J[r * n + i + 1] = k;
+        J[r * n + m + i] = -c;
+    }
+}
+/* linear u' = p0 * u (componentwise) */
+static void linear_f(double* du, const double* u, const double* p, double t, void* user) {
+    (void)t;
+    int n = ((builtin_user*)user)->n;
+    for (int i = 0; i < n; i++) du[i] = p[0] * u[i];
+}
+static void linear_jac(double* J, const double* u, const double* p, double t, void* user) {
+    (void)t; (void)u;
+    int n = ((builtin_user*)user)->n;
+    memset(J, 0, sizeof(double) * n * n);
+    for (int i = 0; i < n; i++) J[i * n + i] = p[0];
+}
+/* blow-up u' = exp(u) (check_error.jl test problem) */
+static void expb_f(double* du, const double* u, const double* p, double t, void* user) {
+    (void)t; (void)p;
+    int n = ((builtin_user*)user)->n;
+    for (int i = 0; i < n; i++) du[i] = exp(u[i]);
+}
+static void expb_jac(double* J, const double* u, const double* p, double t, void* user) {
+    (void)t; (void)p;
+    int n = ((builtin_user*)user)->n;
+    memset(J, 0, sizeof(double) * n * n);
+    for (int i = 0; i < n; i++) J[i * n + i] = exp(u[i]);
+}
+
+static int builtin_fns(int prob, oo_rhs_fn* f, oo_jac_fn* j) {
+    switch (prob) {
+        case OO_PROB_LORENZ: *f = lorenz_f; *j = lorenz_jac; return 3;
+        case OO_PROB_ROBERTSON: *f = rober_f; *j = rober_jac; return 3;
+        case OO_PROB_CHAIN: *f = chain_f; *j = chain_jac; return 3;
+        case OO_PROB_LINEAR: *f = linear_f; *j = linear_jac; return 1;
+        case OO_PROB_EXPBLOWUP: *f = expb_f; *j = expb_jac; return 0;
+        default: return -1;
+    }
+}
+
+int oo_solve_builtin(int prob, int n, const double* u0, const double* p, double t0, double tf,
+                     const oo_opts* opts, double* t_out, double* u_out, int max_out,
+                     double* u_final, oo_result* res) {
+    oo_rhs_fn f; oo_jac_fn j;
+    if (builtin_fns(prob, &f, &j) < 0) return -1;
+    builtin_user bu = {n};
+    oo_opts o2 = *opts;
+    o2.has_tgrad = 0;
+    return oo_solve(n, f, j, NULL, &bu, u0, p, t0, tf, &o2, t_out, u_out, max_out, u_final, res);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Threaded ensemble driver (EnsembleThreads shape: workers pull chunks)      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int prob; int64_t N; int nmax; const int* nvec;
+    const double* u0; int np; const double* p; const double* t0; const double* tf;
+    const oo_opts* opts; int64_t chunk;
+    double* u_final; int32_t* retcode; int64_t* naccept; int64_t* nreject; int64_t* nf;
+    int64_t next;
+    pthread_mutex_t mu;
+} ens_t;
+
+static void* ens_worker(void* arg) {
+    ens_t* E = (ens_t*)arg;
+    double u0[256], pp[16], uf[256];
+    oo_opts o = *E->opts;
+    o.save_start = 0; o.save_everystep = 0; o.nsaveat = 0; o.saveat = NULL;
+    for (;;) {
+        pthread_mutex_lock(&E->mu);
+        int64_t b = E->next;
+        E->next += E->chunk;
+        pthread_mutex_unlock(&E->mu);
+        if (b >= E->N) break;
+        int64_t e = b + E->chunk < E->N ? b + E->chunk : E->N;
+        for (int64_t j = b; j < e; j++) {
+            int n = E->nvec ? E->nvec[j] : E->nmax;
+            for (int i = 0; i < n; i++) u0[i] = E->u0[(size_t)i * E->N + j];
+            for (int k = 0; k < E->np; k++) pp[k] = E->p[(size_t)k * E->N + j];
+            oo_result r;
+            oo_solve_builtin(E->prob, n, u0, pp, E->t0[j], E->tf[j], &o, NULL, NULL, 0, uf, &r);
+            if (E->u_final)
+                for (int i = 0; i < E->nmax; i++)
+                    E->u_final[(size_t)i * E->N + j] = (i < n) ? uf[i] : 0.0;
+            if (E->retcode) E->retcode[j] = r.retcode;
+            if (E->naccept) E->naccept[j] = r.naccept;
+            if (E->nreject) E->nreject[j] = r.nreject;
+            if (E->nf) E->nf[j] = r.nf;
+        }
+    }
+    return NULL;
+}
+
+double oo_ensemble_builtin(int prob, int64_t N, int nmax, const int* nvec,
+                           const double* u0, int np, const double* p,
+                           const double* t0, const double* tf, const oo_opts* opts,
+                           int threads, int64_t chunk,
+                           double* u_final, int32_t* retcode, int64_t* naccept,
+                           int64_t* nreject, int64_t* nf) {
+    ens_t E;
+    E.prob = prob; E.N = N; E.nmax = nmax; E.nvec = nvec; E.u0 = u0; E.np = np; E.p = p;
+    E.t0 = t0; E.tf = tf; E.opts = opts; E.chunk = chunk > 0 ? chunk : 1;
+    E.u_final = u_final; E.retcode = retcode; E.naccept = naccept; E.nreject = nreject; E.nf = nf;
+    E.next = 0;
+    pthread_mutex_init(&E.mu, NULL);
+    if (threads < 1) threads = 1;
+    struct timespec a, b;
+    clock_gettime(CLOCK_MONOTONIC, &a);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+    for (int i = 0; i < threads; i++) pthread_create(&th[i], NULL, ens_worker, &E);
+    for (int i = 0; i < threads; i++) pthread_join(th[i], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &b);
+    free(th);
+    pthread_mutex_destroy(&E.mu);
+    return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+}

@@ -1,0 +1,99 @@
+/*
+ * odeoracle.h -- CPU restatement of OrdinaryDiffEq.jl's Tsit5 / Rodas5P step
+ * control, used ONLY as test infrastructure (the parity checker and the CPU
+ * baseline timed beside the GPU engine).  Nothing in the product links this.
+ *
+ * Every function in odeoracle.c cites the reference file:line it restates.
+ * Paths are relative to the OrdinaryDiffEq.jl v7.7.0 tree.
+ */
+#ifndef ODEORACLE_H
+#define ODEORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OO_TSIT5 = 0, OO_RODAS5P = 1 };
+
+/* SciMLBase.ReturnCode enum order (Default=0, Success=1, ...). */
+enum {
+    OO_RC_DEFAULT = 0, OO_RC_SUCCESS = 1, OO_RC_TERMINATED = 2, OO_RC_DTNAN = 3,
+    OO_RC_MAXITERS = 4, OO_RC_MAXNUMSUB = 5, OO_RC_DTLESSTHANMIN = 6, OO_RC_UNSTABLE = 7
+};
+
+/* Built-in right-hand sides (the benchmark problem families). */
+enum { OO_PROB_USER = 0, OO_PROB_LORENZ = 1, OO_PROB_ROBERTSON = 2, OO_PROB_CHAIN = 3,
+       OO_PROB_LINEAR = 4, OO_PROB_EXPBLOWUP = 5 };
+
+typedef void (*oo_rhs_fn)(double* du, const double* u, const double* p, double t, void* user);
+/* J is row-major n x n: J[i*n + j] = d f_i / d u_j */
+typedef void (*oo_jac_fn)(double* J, const double* u, const double* p, double t, void* user);
+typedef void (*oo_tgrad_fn)(double* dT, const double* u, const double* p, double t, void* user);
+
+typedef struct {
+    int alg;              /* OO_TSIT5 / OO_RODAS5P */
+    double reltol, abstol;
+    double dtmax;         /* <= 0 or NaN: tf - t0 (solve.jl:153) */
+    double dtmin;         /* default 0 (solve.jl:152) */
+    double dt;            /* 0: automatic initial dt (initdt.jl) */
+    int64_t maxiters;     /* default 1_000_000 (solve.jl:161) */
+    int save_start, save_end, save_everystep;
+    int save_end_user;    /* -1: nothing (default rule), 0/1: user Bool */
+    const double* saveat; /* sorted output times, may be NULL */
+    int nsaveat;
+    int has_tgrad;        /* Rodas stats: no nf increment for dT when 1 */
+} oo_opts;
+
+typedef struct {
+    int retcode;
+    int64_t nf, naccept, nreject, njacs, nw, nsolve;
+    int nsaved;
+    double t_final;       /* integrator t at exit */
+    double dt_last;       /* integrator dt at exit */
+} oo_result;
+
+void oo_default_opts(oo_opts* o, int alg);
+
+/* Solve one IVP.  t_out: max_out slots, u_out: max_out * n (row per save).
+ * u_final (n) receives the integrator state at exit. */
+int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
+             const double* u0, const double* p, double t0, double tf,
+             const oo_opts* opts, double* t_out, double* u_out, int max_out,
+             double* u_final, oo_result* res);
+
+/* Solve with a built-in RHS (prob = OO_PROB_*); np is implied by prob. */
+int oo_solve_builtin(int prob, int n, const double* u0, const double* p,
+                     double t0, double tf, const oo_opts* opts,
+                     double* t_out, double* u_out, int max_out,
+                     double* u_final, oo_result* res);
+
+/* Ensemble of built-in problems over `threads` workers with dynamic chunks of
+ * trajectory indices (the EnsembleThreads scheduling shape).  SoA inputs:
+ * u0[i*N + j] (component i of trajectory j), p[k*N + j]; nvec[j] = state
+ * dimension of trajectory j (NULL: all n).  Outputs: u_final SoA (nmax x N),
+ * retcode/naccept/nreject/nf per trajectory (any may be NULL).
+ * Returns wall seconds. */
+double oo_ensemble_builtin(int prob, int64_t N, int nmax, const int* nvec,
+                           const double* u0, int np, const double* p,
+                           const double* t0, const double* tf, const oo_opts* opts,
+                           int threads, int64_t chunk,
+                           double* u_final, int32_t* retcode, int64_t* naccept,
+                           int64_t* nreject, int64_t* nf);
+
+/* Exposed pieces for known-answer tests. */
+double oo_fastpower(double x, double y);
+float oo_fastlog2(float x);
+double oo_jl_eps(double x);
+/* Dense LU with partial pivoting + solve, in place on A (row-major n x n). */
+int oo_lu_solve(int n, double* A, int* piv, double* b);
+/* Tableau access for the order-condition tests. */
+void oo_tsit5_tableau(double* c /*7*/, double* a /*7x7*/, double* b /*7*/, double* btilde /*7*/);
+void oo_rodas5p_tableau(double* A /*8x8*/, double* C /*8x8*/, double* c /*8*/, double* d /*8*/,
+                        double* H /*3x8*/, double* b /*8*/, double* btilde /*8*/, double* gamma);
+void oo_tsit5_interp_coeffs(double* r /*7x4, r[i*4 + j] = r_{i+1, j+1}*/);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1147 @@
+// odemi_device.h -- device side of the MI355X ensemble ODE engine.
+//
+// This header is compiled at run time by hipRTC together with a traced
+// problem (odemi_f / odemi_jac / odemi_tgrad and the ODEMI_* size macros), so
+// every kernel below is specialised for one right-hand side.  It restates the
+// step-control semantics of OrdinaryDiffEq.jl v7.7.0:
+//   Tsit5 step        lib/OrdinaryDiffEqTsit5/src/tsit_perform_step.jl:188-262
+//   Tsit5 tableau     lib/OrdinaryDiffEqTsit5/src/tsit_tableaus.jl:45-101,244-276
+//   Tsit5 dense       lib/OrdinaryDiffEqTsit5/src/interpolants.jl:24-93
+//   Rodas5P step      lib/OrdinaryDiffEqRosenbrock/src/rosenbrock_perform_step.jl:729-847
+//   Rodas5P tableau   lib/OrdinaryDiffEqRosenbrock/src/rosenbrock_tableaus.jl:23-73
+//   Rodas dense       lib/OrdinaryDiffEqRosenbrock/src/rosenbrock_interpolants.jl:172-206
+//   J/W/dT            lib/OrdinaryDiffEqDifferentiation/src/derivative_utils.jl:181-227,535-595,663-685
+//   loop              lib/OrdinaryDiffEqCore/src/solve.jl:904-946,
+//                     integrators/integrator_utils.jl:84-414,493-677,1027-1034,1199-1256
+//   PI controller     lib/OrdinaryDiffEqCore/src/integrators/controllers.jl:244-293,805-843
+//   initdt            lib/OrdinaryDiffEqCore/src/initdt.jl:17-298
+//   residual, norm    lib/DiffEqBase/src/calculate_residuals.jl:9-14, common_defaults.jl:69-75
+//   retcodes          lib/DiffEqBase/src/check_error.jl:69-118
+// Arithmetic conventions: each `@muladd` site is an explicit fma, products are
+// summed left to right (fma(a3,k3,fma(a2,k2,a1*k1))), fused Rosenbrock sums start
+// from their base value, the RMS norm is an fma chain; the program is compiled
+// with -ffp-contract=off so no other fusion happens.  FP64 kernels therefore
+// reproduce the CPU restatement bit for bit.
+#pragma once
+
+#ifndef ODEMI_NP
+#define ODEMI_NP 1
+#endif
+#ifndef ODEMI_HAS_TGRAD
+#define ODEMI_HAS_TGRAD 0
+#endif
+
+namespace odemi {
+
+typedef long long i64;
+typedef unsigned long long u64;
+
+enum { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_DTNAN = 3, RC_MAXITERS = 4, RC_DTLESSTHANMIN = 6,
+       RC_UNSTABLE = 7 };
+enum { ST_NF = 0, ST_NACCEPT = 1, ST_NREJECT = 2, ST_NJACS = 3, ST_NW = 4, ST_NSOLVE = 5,
+       ST_COUNT = 6 };
+
+// Kernel argument block (one struct, passed by value).  All arrays are SoA over
+// the N trajectories of the batch: x[i * N + j] is component i of trajectory j.
+struct Params {
+    i64 N;
+    int nmax;                 // state dimension (upper bound for uneven batches)
+    int np;
+    const int* nvec;          // per-trajectory dimension (uneven batches) or null
+    const double* u0;         // [nmax][N]
+    const double* p;          // [np][N]
+    const double* t0;         // [N]
+    const double* tf;         // [N]
+    const i64* order;         // work-queue order (permutation of 0..N-1) or null
+    double reltol, abstol;
+    double dtmax;             // <= 0: tf - t0 per trajectory
+    double dtmin;
+    double dt_user;           // > 0: fixed initial dt (no initdt)
+    i64 maxiters;
+    const double* saveat;     // sorted, shared by all trajectories
+    int nsaveat;
+    int save_start, save_end, save_everystep, save_end_user;  // save_end_user: -1 nothing
+    int max_saves;
+    double* t_out;            // [max_saves][N]
+    double* u_out;            // [max_saves][nmax][N]
+    int* nsaved;              // [N]
+    double* u_final;          // [nmax][N]
+    double* t_final;          // [N]
+    int* retcode;             // [N]
+    int* stats;               // [ST_COUNT][N]
+    u64* counters;            // [0] work queue head, [1] live count (plain mode), [2] finished
+    // plain (one kernel per step attempt) mode: device-resident integrator state
+    double* st_scalar;        // [16][N]
+    double* st_vec;           // [8][nmax][N]
+    int* st_int;              // [8][N]
+    int refine_sweeps;        // mixed Rosenbrock: FP64 refinement sweeps per solve
+    int pad0;
+};
+
+// ---------------------------------------------------------------------------
+// Julia scalar semantics
+// ---------------------------------------------------------------------------
+
+// Base.eps(::Float64)
+__device__ __forceinline__ double jl_eps(double x) {
+    u64 b = (u64)__double_as_longlong(x) & 0x7fffffffffffffffULL;
+    int e = (int)(b >> 52);
+    if (e == 0x7ff) return __longlong_as_double(0x7ff8000000000000LL);
+    if (e == 0) return __longlong_as_double(1LL);
+    if (e > 52) return __longlong_as_double((i64)(e - 52) << 52);
+    return __longlong_as_double(1LL << (e - 1));
+}
+__device__ __forceinline__ double nextfloat_pos(double x) {  // nextfloat for x >= 0 finite
+    return __longlong_as_double(__double_as_longlong(x) + 1);
+}
+// Base.min / Base.max (NaN-propagating, -0.0 < 0.0)
+__device__ __forceinline__ double jl_min(double x, double y) {
+    if (isnan(x) || isnan(y)) return __longlong_as_double(0x7ff8000000000000LL);
+    if (x == y) return signbit(x) ? x : y;
+    return x < y ? x : y;
+}
+__device__ __forceinline__ double jl_max(double x, double y) {
+    if (isnan(x) || isnan(y)) return __longlong_as_double(0x7ff8000000000000LL);
+    if (x == y) return signbit(x) ? y : x;
+    return x > y ? x : y;
+}
+template <typename T>
+__device__ __forceinline__ T max_fast(T x, T y) { return y > x ? y : x; }
+
+// FastPower.jl 1.1.x `fastpower` (controllers.jl:815-816): Goldberg's rational
+// fast log2 in Float32, then exp2 of the Float32 product, correctly rounded to
+// Float32 (evaluated in double and rounded once).
+__device__ __forceinline__ float fastlog2f(float x) {
+    unsigned ux1i = __float_as_uint(x);
+    unsigned ex = (ux1i & 0x7F800000u) >> 23;
+    float signif, fexp;
+    if ((ux1i & 0x00400000u) != 0u) {
+        signif = __uint_as_float((ux1i & 0x007FFFFFu) | 0x3f000000u);
+        fexp = (float)ex - 126.0f;
+    } else {
+        signif = __uint_as_float((ux1i & 0x007FFFFFu) | 0x3f800000u);
+        fexp = (float)ex - 127.0f;
+    }
+    signif = signif - 1.0f;
+    float t1 = 0.338953f * signif;
+    float t2 = t1 + 2.198599f;
+    float t3 = signif * t2;
+    float t4 = signif + 1.523692f;
+    float t5 = t3 / t4;
+    return fexp + t5;
+}
+__device__ __forceinline__ double fastpower(double x, double y) {
+    if (x == 0.0) return 0.0;
+    if (isinf(x) && isinf(y)) return __longlong_as_double(0x7ff0000000000000LL);
+    float lg = fastlog2f((float)x);
+    float prod = (float)y * lg;
+    return (double)(float)exp2((double)prod);
+}
+
+// ---------------------------------------------------------------------------
+// Tableaus
+// ---------------------------------------------------------------------------
+struct Tsit5Tab {
+    static constexpr double c1 = 0.161, c2 = 0.327, c3 = 0.9, c4 = 0.9800255409045097;
+    static constexpr double a21 = 0.161;
+    static constexpr double a31 = -0.008480655492356989, a32 = 0.335480655492357;
+    static constexpr double a41 = 2.8971530571054935, a42 = -6.359448489975075,
+                            a43 = 4.3622954328695815;
+    static constexpr double a51 = 5.325864828439257, a52 = -11.748883564062828,
+                            a53 = 7.4955393428898365, a54 = -0.09249506636175525;
+    static constexpr double a61 = 5.86145544294642, a62 = -12.92096931784711,
+                            a63 = 8.159367898576159, a64 = -0.071584973281401,
+                            a65 = -0.028269050394068383;
+    static constexpr double a71 = 0.09646076681806523, a72 = 0.01, a73 = 0.4798896504144996,
+                            a74 = 1.379008574103742, a75 = -3.290069515436081,
+                            a76 = 2.324710524099774;
+    static constexpr double bt1 = -0.00178001105222577714, bt2 = -0.0008164344596567469,
+                            bt3 = 0.007880878010261995, bt4 = -0.1447110071732629,
+                            bt5 = 0.5823571654525552, bt6 = -0.45808210592918697,
+                            bt7 = 0.015151515151515152;
+};
+
+__constant__ double kTsitR[7][4] = {
+    {1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+    {0.0, 0.13169999999999998, -0.2234, 0.1017},
+    {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+    {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+    {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+    {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
+    {0.0, 1.5, -4.0, 2.5},
+};
+
+// Rodas5P (Steinebach 2023), transformed Hairer-Wanner form.
+struct RodasTab {
+    static constexpr double gamma = 0.21193756319429014;
+};
+__constant__ double kRA[8][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0},
+    {3.0, 0, 0, 0, 0, 0, 0, 0},
+    {2.849394379747939, 0.45842242204463923, 0, 0, 0, 0, 0, 0},
+    {-6.954028509809101, 2.489845061869568, -10.358996098473584, 0, 0, 0, 0, 0},
+    {2.8029986275628964, 0.5072464736228206, -0.3988312541770524, -0.04721187230404641, 0, 0, 0, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946, 0, 0, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946, 1, 0, 0},
+    {-7.502846399306121, 2.561846144803919, -11.627539656261098, -0.18268767659942256,
+     0.030198172008377946, 1, 1, 0},
+};
+__constant__ double kRC[8][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {-14.155112264123755, 0, 0, 0, 0, 0, 0},
+    {-17.97296035885952, -2.859693295451294, 0, 0, 0, 0, 0},
+    {147.12150275711716, -1.41221402718213, 71.68940251302358, 0, 0, 0, 0},
+    {165.43517024871676, -0.4592823456491126, 42.90938336958603, -5.961986721573306, 0, 0, 0},
+    {24.854864614690072, -3.0009227002832186, 47.4931110020768, 5.5814197821558125,
+     -0.6610691825249471, 0, 0},
+    {30.91273214028599, -3.1208243349937974, 77.79954646070892, 34.28646028294783,
+     -19.097331116725623, -28.087943162872662, 0},
+    {37.80277123390563, -3.2571969029072276, 112.26918849496327, 66.9347231244047,
+     -40.06618937091002, -54.66780262877968, -9.48861652309627},
+};
+__constant__ double kRc[8] = {0, 0.6358126895828704, 0.4095798393397535, 0.9769306725060716,
+                              0.4288403609558664, 1, 1, 1};
+__constant__ double kRd[8] = {0.21193756319429014, -0.42387512638858027, -0.3384627126235924,
+                              1.8046452872882734, 2.325825639765069, 0, 0, 0};
+__constant__ double kRH[3][8] = {
+    {25.948786856663858, -2.5579724845846235, 10.433815404888879, -2.3679251022685204,
+     0.524948541321073, 1.1241088310450404, 0.4272876194431874, -0.17202221070155493},
+    {-9.91568850695171, -0.9689944594115154, 3.0438037242978453, -24.495224566215796,
+     20.176138334709044, 15.98066361424651, -6.789040303419874, -6.710236069923372},
+    {11.419903575922262, 2.8879645146136994, 72.92137995996029, 80.12511834622643,
+     -52.072871366152654, -59.78993625266729, -0.15582684282751913, 4.883087185713722},
+};
+
+// ---------------------------------------------------------------------------
+// Common per-trajectory controller / loop state (all FP64)
+// ---------------------------------------------------------------------------
+struct Ctl {
+    double t, dt, dtpropose, tprev, tstop_target, tf, dtmax, EEst, q11, errold;
+    int iter, success_iter, accept_step, next_step_tstop;
+    int nf, naccept, nreject, njacs, nw, nsolve;
+    int spos, send, nsaved, tf_in_saveat;
+};
+
+// controller constants (alg_utils.jl:407-879; controllers.jl:744-803)
+struct CtlConst {
+    double qmin, qmax, qmax_first, gamma, beta1, beta2, qoldinit, qsteady_min, qsteady_max;
+};
+__device__ __forceinline__ CtlConst ctl_const(bool rosenbrock) {
+    CtlConst c;
+    c.qmin = 0.2; c.qmax = 10.0; c.qmax_first = 10000.0; c.gamma = 0.9;
+    c.beta1 = 0.14; c.beta2 = 0.08; c.qoldinit = 1e-4;
+    c.qsteady_min = 1.0; c.qsteady_max = rosenbrock ? 1.2 : 1.0;
+    return c;
+}
+
+// integrator_utils.jl:268-324 (adaptive branch, tstops = [tf])
+__device__ __forceinline__ void modify_dt_for_tstops(Ctl& c) {
+    double distance = fabs(c.tf - c.t);
+    double tol = 0.0;
+    if (isfinite(c.tf) && isfinite(c.t)) tol = 100.0 * jl_eps(jl_max(fabs(c.t), fabs(c.tf)));
+    double original_dt = fabs(c.dt);
+    c.dtpropose = original_dt;
+    if (original_dt + tol < distance) {
+        c.next_step_tstop = 0;
+    } else {
+        c.next_step_tstop = 1;
+        c.tstop_target = c.tf;
+    }
+    c.dt = jl_min(original_dt, distance);
+}
+// integrator_utils.jl:1243-1256, utils.jl:90-94
+__device__ __forceinline__ void fix_dt_at_bounds(Ctl& c, double dtmin) {
+    c.dt = jl_min(c.dtmax, c.dt);
+    c.dt = jl_max(c.dt, fabs(jl_max(jl_eps(c.t), dtmin)));
+}
+// controllers.jl:805-821
+__device__ __forceinline__ double stepsize_controller(Ctl& c, const CtlConst& k) {
+    double qmax = (c.success_iter == 0) ? k.qmax_first : k.qmax;
+    double q;
+    if (c.EEst == 0.0) {
+        q = 1.0 / qmax;
+    } else {
+        double q11 = fastpower(c.EEst, k.beta1);
+        q = q11 / fastpower(c.errold, k.beta2);
+        c.q11 = q11;
+        double x = q / k.gamma, lo = 1.0 / qmax, hi = 1.0 / k.qmin;
+        q = (x > hi) ? hi : ((x < lo) ? lo : x);
+    }
+    return q;
+}
+// controllers.jl:823-836 + integrator_utils.jl:1199-1210
+__device__ __forceinline__ void accept_controller(Ctl& c, const CtlConst& k, double q, double dtmin) {
+    if (k.qsteady_min <= q && q <= k.qsteady_max) q = 1.0;
+    c.errold = jl_max(c.EEst, k.qoldinit);
+    double dtnew = c.dt / q;
+    double dp = jl_min(fabs(c.dtmax), fabs(dtnew));
+    c.dtpropose = jl_max(fabs(dp), fabs(jl_max(jl_eps(c.t), dtmin)));
+}
+// controllers.jl:838-843
+__device__ __forceinline__ void reject_controller(Ctl& c, const CtlConst& k) {
+    c.dt = c.dt / jl_min(1.0 / k.qmin, c.q11 / k.gamma);
+}
+
+template <typename S, int N>
+__device__ __forceinline__ bool all_finite(const S* u) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) ok = ok && isfinite((double)u[i]);
+    return ok;
+}
+
+// check_error.jl:69-118 (adaptive, default dtmin/unstable_check)
+__device__ __forceinline__ int check_error(const Ctl& c, const Params& P, bool u_finite) {
+    if (isnan(c.dt)) return RC_DTNAN;
+    if ((i64)c.iter > P.maxiters) return RC_MAXITERS;
+    if (fabs(c.dt) <= fabs(P.dtmin) && (!c.accept_step || c.t + c.dt < c.tf)) return RC_DTLESSTHANMIN;
+    else if (!c.accept_step && fabs(c.dt) <= fabs(jl_eps(c.t))) return RC_UNSTABLE;
+    if (c.accept_step && !u_finite) return RC_UNSTABLE;
+    return RC_SUCCESS;
+}
+
+// Saveat cursor setup: saveat_internal = sorted times in (t0, tf]
+__device__ __forceinline__ void saveat_init(Ctl& c, const Params& P, double t0) {
+    int lo = 0, hi = P.nsaveat;
+    while (lo < hi) { int m = (lo + hi) >> 1; if (P.saveat[m] <= t0) lo = m + 1; else hi = m; }
+    c.spos = lo;
+    lo = c.spos; hi = P.nsaveat;
+    while (lo < hi) { int m = (lo + hi) >> 1; if (P.saveat[m] <= c.tf) lo = m + 1; else hi = m; }
+    c.send = lo;
+    c.tf_in_saveat = (c.send > 0 && P.saveat[c.send - 1] == c.tf) ? 1 : 0;
+}
+
+template <int N, typename V>
+__device__ __forceinline__ void save_point(Ctl& c, const Params& P, i64 j, double t, const V* v) {
+    int s = c.nsaved;
+    if (s < P.max_saves) {
+        P.t_out[(i64)s * P.N + j] = t;
+#pragma unroll
+        for (int i = 0; i < N; i++) P.u_out[((i64)s * P.nmax + i) * P.N + j] = (double)v[i];
+    }
+    c.nsaved = s + 1;
+}
+
+// RMS norm of residuals, FP64 accumulation (calculate_residuals.jl:9-14 +
+// common_defaults.jl:69-75)
+template <int N, typename E, typename S>
+__device__ __forceinline__ double residual_norm(const E* ut, const S* u0, const S* u1,
+                                                double abstol, double reltol) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double den = fma(max_fast(fabs((double)u0[i]), fabs((double)u1[i])), reltol, abstol);
+        double r = (double)ut[i] / den;
+        s = fma(r, r, s);
+    }
+    return sqrt(s / (double)(N > 1 ? N : 1));
+}
+
+template <int N>
+__device__ __forceinline__ double ode_norm(const double* x) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) s = fma(x[i], x[i], s);
+    return sqrt(s / (double)(N > 1 ? N : 1));
+}
+
+// initdt.jl:17-298 restated for a lane-resident state (FP64 evaluation).
+template <int N>
+__device__ double initdt_lane(const Ctl& c, const double* u0, const double* p, double reltol,
+                              double abstol, double dtmax_opt, double dtmin_opt) {
+    const double t = c.t;
+    const double dtmax_tdir = jl_min(fabs(dtmax_opt), fabs(c.tf - t));
+    const double dtmin = nextfloat_pos(jl_max(dtmin_opt, jl_eps(t)));
+    const double smalldt = jl_max(dtmin, 1e-6);
+    if (N == 0) return jl_max(smalldt, dtmin);
+    double sk[N], f0[N], f1[N], tmp[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) sk[i] = fma(fabs(u0[i]), reltol, abstol);
+    odemi_f<double>(f0, u0, p, t);
+#pragma unroll
+    for (int i = 0; i < N; i++) tmp[i] = u0[i] / sk[i];
+    double d0 = ode_norm<N>(tmp);
+#pragma unroll
+    for (int i = 0; i < N; i++) tmp[i] = f0[i] / sk[i] * 1.0;
+    double d1 = ode_norm<N>(tmp);
+    if (isnan(d1)) return dtmin;
+    double dt0 = ((d0 < 1e-5) | (d1 < 1e-5)) ? smalldt : (d0 / d1) / 100.0;
+    dt0 = jl_min(dt0, dtmax_tdir);
+    if (dt0 < 10.0 * 2.220446049250313e-16) return jl_max(smalldt, dtmin);
+#pragma unroll
+    for (int i = 0; i < N; i++) tmp[i] = fma(dt0, f0[i], u0[i]);
+    odemi_f<double>(f1, tmp, p, t + dt0);
+    bool same = true;
+#pragma unroll
+    for (int i = 0; i < N; i++) same = same && (f0[i] == f1[i]);
+    if (same) return jl_max(dtmin, 100.0 * dt0);
+#pragma unroll
+    for (int i = 0; i < N; i++) tmp[i] = (f1[i] - f0[i]) / sk[i] * 1.0;
+    double d2 = ode_norm<N>(tmp) / dt0 * 1.0;
+    double mx = jl_max(d1, d2);
+    double dt1;
+    if (mx < 1e-15) dt1 = jl_max(1e-6, dt0 * 0.001);
+    else dt1 = pow(10.0, -(2.0 + log10(mx)) / 5.0);
+    return jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
+}
+
+// Common per-trajectory initialisation of the controller state.
+__device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
+    double t0 = P.t0[j], tf = P.tf[j];
+    c.t = t0; c.tprev = t0; c.tf = tf;
+    c.dtmax = (P.dtmax > 0.0) ? P.dtmax : (tf - t0);
+    c.EEst = 1.0; c.q11 = 1.0; c.errold = 1e-4;
+    c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0;
+    c.tstop_target = 0.0; c.dt = 0.0; c.dtpropose = 0.0;
+    c.nf = 0; c.naccept = 0; c.nreject = 0; c.njacs = 0; c.nw = 0; c.nsolve = 0;
+    c.nsaved = 0; c.spos = 0; c.send = 0; c.tf_in_saveat = 0;
+    if (P.nsaveat > 0) saveat_init(c, P, t0);
+}
+
+// solution_endpoint_match_cur_integrator! (integrator_utils.jl:540-588) and
+// result write-out.
+template <int N, typename S>
+__device__ void finish_lane(Ctl& c, const Params& P, i64 j, const S* u, int rc) {
+    if (rc == RC_SUCCESS && P.save_end) {
+        bool doit = (c.nsaved == 0);
+        if (!doit) {
+            int s = c.nsaved - 1;
+            double last = (s < P.max_saves) ? P.t_out[(i64)s * P.N + j] : __longlong_as_double(0x7ff8000000000000LL);
+            doit = (last != c.t) && (P.save_end_user == 1 || c.tf_in_saveat || c.t == c.tf || P.nsaveat == 0);
+        }
+        if (doit) save_point<N>(c, P, j, c.t, u);
+    }
+    if (P.u_final) {
+#pragma unroll
+        for (int i = 0; i < N; i++) P.u_final[(i64)i * P.N + j] = (double)u[i];
+    }
+    if (P.t_final) P.t_final[j] = c.t;
+    if (P.nsaved) P.nsaved[j] = c.nsaved;
+    P.retcode[j] = rc;
+    if (P.stats) {
+        P.stats[ST_NF * P.N + j] = c.nf;
+        P.stats[ST_NACCEPT * P.N + j] = c.naccept;
+        P.stats[ST_NREJECT * P.N + j] = c.nreject;
+        P.stats[ST_NJACS * P.N + j] = c.njacs;
+        P.stats[ST_NW * P.N + j] = c.nw;
+        P.stats[ST_NSOLVE * P.N + j] = c.nsolve;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tsit5, one trajectory per lane.
+//   S = state type (uprev/u accumulation), K = stage type (RHS, k_i, stage sums)
+//   FP64 engine: S = K = double.  Mixed engine: S = double, K = float.
+// ---------------------------------------------------------------------------
+template <typename S, typename K, int N>
+struct Tsit5Lane {
+    Ctl c;
+    S u[N], uprev[N];
+    K k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N];
+    K p[ODEMI_NP];
+    double pd[ODEMI_NP];
+};
+
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+    ctl_init(L.c, P, j);
+    double u0d[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) { u0d[i] = P.u0[(i64)i * P.N + j]; L.u[i] = (S)u0d[i]; L.uprev[i] = L.u[i]; }
+#pragma unroll
+    for (int k = 0; k < ODEMI_NP; k++) { L.pd[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; L.p[k] = (K)L.pd[k]; }
+    if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
+    // initialize! : FSAL pre-start
+    K uk[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) uk[i] = (K)L.uprev[i];
+    odemi_f<K>(L.k1, uk, L.p, (K)L.c.t);
+    L.c.nf += 1;
+    if (P.dt_user > 0.0) {
+        L.c.dt = P.dt_user;
+    } else {
+        L.c.dt = initdt_lane<N>(L.c, u0d, L.pd, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
+        L.c.nf += 2;
+    }
+    L.c.dtpropose = L.c.dt;
+}
+
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_perform_step(Tsit5Lane<S, K, N>& L, const Params& P) {
+    typedef Tsit5Tab T;
+    const double t = L.c.t, dt = L.c.dt;
+    K tmp[N];
+    const K a = (K)(dt * T::a21);
+#pragma unroll
+    for (int i = 0; i < N; i++) tmp[i] = (K)fma((S)a, (S)L.k1[i], L.uprev[i]);
+    odemi_f<K>(L.k2, tmp, L.p, (K)fma(T::c1, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a31 * L.k1[i];
+        acc = fma((K)T::a32, L.k2[i], acc);
+        tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k3, tmp, L.p, (K)fma(T::c2, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a41 * L.k1[i];
+        acc = fma((K)T::a42, L.k2[i], acc);
+        acc = fma((K)T::a43, L.k3[i], acc);
+        tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k4, tmp, L.p, (K)fma(T::c3, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a51 * L.k1[i];
+        acc = fma((K)T::a52, L.k2[i], acc);
+        acc = fma((K)T::a53, L.k3[i], acc);
+        acc = fma((K)T::a54, L.k4[i], acc);
+        tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k5, tmp, L.p, (K)fma(T::c4, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a61 * L.k1[i];
+        acc = fma((K)T::a62, L.k2[i], acc);
+        acc = fma((K)T::a63, L.k3[i], acc);
+        acc = fma((K)T::a64, L.k4[i], acc);
+        acc = fma((K)T::a65, L.k5[i], acc);
+        tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k6, tmp, L.p, (K)(t + dt));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a71 * L.k1[i];
+        acc = fma((K)T::a72, L.k2[i], acc);
+        acc = fma((K)T::a73, L.k3[i], acc);
+        acc = fma((K)T::a74, L.k4[i], acc);
+        acc = fma((K)T::a75, L.k5[i], acc);
+        acc = fma((K)T::a76, L.k6[i], acc);
+        L.u[i] = fma((S)dt, (S)acc, L.uprev[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) tmp[i] = (K)L.u[i];
+    odemi_f<K>(L.k7, tmp, L.p, (K)(t + dt));
+    L.c.nf += 6;
+    // embedded error estimate, accumulated in FP64
+    double ut[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double acc = T::bt1 * (double)L.k1[i];
+        acc = fma(T::bt2, (double)L.k2[i], acc);
+        acc = fma(T::bt3, (double)L.k3[i], acc);
+        acc = fma(T::bt4, (double)L.k4[i], acc);
+        acc = fma(T::bt5, (double)L.k5[i], acc);
+        acc = fma(T::bt6, (double)L.k6[i], acc);
+        acc = fma(T::bt7, (double)L.k7[i], acc);
+        ut[i] = dt * acc;
+    }
+    L.c.EEst = residual_norm<N>(ut, L.uprev, L.u, P.abstol, P.reltol);
+}
+
+// interpolants.jl:52-64 (@evalpoly Horner via muladd), FP64 evaluation.
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_interp(const Tsit5Lane<S, K, N>& L, double th, double dt, double* out) {
+    double th2 = th * th;
+    double b[7];
+    b[0] = th * fma(th, fma(th, fma(th, kTsitR[0][3], kTsitR[0][2]), kTsitR[0][1]), kTsitR[0][0]);
+#pragma unroll
+    for (int s = 1; s < 7; s++) b[s] = th2 * fma(th, fma(th, kTsitR[s][3], kTsitR[s][2]), kTsitR[s][1]);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double acc = (double)L.k1[i] * b[0];
+        acc = fma((double)L.k2[i], b[1], acc);
+        acc = fma((double)L.k3[i], b[2], acc);
+        acc = fma((double)L.k4[i], b[3], acc);
+        acc = fma((double)L.k5[i], b[4], acc);
+        acc = fma((double)L.k6[i], b[5], acc);
+        acc = fma((double)L.k7[i], b[6], acc);
+        out[i] = fma(dt, acc, (double)L.uprev[i]);
+    }
+}
+
+// savevalues! (integrator_utils.jl:340-414)
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+    Ctl& c = L.c;
+    while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
+        double curt = P.saveat[c.spos++];
+        if (curt != c.t) {
+            double th = (curt - c.tprev) / c.dt;
+            double v[N];
+            tsit5_interp(L, th, c.dt, v);
+            save_point<N>(c, P, j, curt, v);
+        } else {
+            if (curt == c.tf && !P.save_end) continue;
+            save_point<N>(c, P, j, c.t, L.u);
+        }
+    }
+    if (P.save_everystep) {
+        int s = c.nsaved;
+        double last = (s > 0 && s - 1 < P.max_saves) ? P.t_out[(i64)(s - 1) * P.N + j] : __longlong_as_double(0x7ff8000000000000LL);
+        if (s == 0 || ((c.t != last || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
+            save_point<N>(c, P, j, c.t, L.u);
+    }
+}
+
+// One step attempt: loopheader! + check_error! + perform_step! + loopfooter!.
+// Returns 0 while running, otherwise the final retcode (lane finished).
+template <typename S, typename K, int N>
+__device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+    const CtlConst kc = ctl_const(false);
+    Ctl& c = L.c;
+    if (!(c.t < c.tf)) return RC_SUCCESS;
+    if (c.iter > 0) {
+        if (c.accept_step) {
+            c.success_iter++;
+#pragma unroll
+            for (int i = 0; i < N; i++) { L.uprev[i] = L.u[i]; L.k1[i] = L.k7[i]; }
+            c.dt = c.dtpropose;
+            modify_dt_for_tstops(c);
+        } else {
+            reject_controller(c, kc);
+        }
+    }
+    c.iter++;
+    fix_dt_at_bounds(c, P.dtmin);
+    modify_dt_for_tstops(c);
+    int rc = check_error(c, P, all_finite<S, N>(L.u));
+    if (rc != RC_SUCCESS) return rc;
+    if (c.next_step_tstop && fabs(c.dt) < jl_eps(fabs(c.t))) {
+        c.accept_step = 1;
+    } else {
+        tsit5_perform_step(L, P);
+    }
+    double ttmp = c.t + c.dt;
+    double q = stepsize_controller(c, kc);
+    c.accept_step = (c.EEst <= 1.0);
+    if (c.accept_step) {
+        c.naccept++;
+        c.tprev = c.t;
+        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tstop_target; }
+        else c.t = ttmp;
+        accept_controller(c, kc, q, P.dtmin);
+        if (P.nsaveat > 0 || P.save_everystep) tsit5_savevalues(L, P, j);
+    } else {
+        c.nreject++;
+    }
+    return (c.t < c.tf) ? 0 : RC_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Work queue: wave-aggregated fetch of trajectory indices
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ i64 wave_fetch(u64* head, bool need) {
+    u64 mask = __ballot(need);
+    if (mask == 0) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    u64 base = 0;
+    if (lane == leader) base = atomicAdd(head, (u64)__popcll(mask));
+    base = __shfl(base, leader);
+    if (!need) return -1;
+    u64 below = mask & ((lane == 0) ? 0ULL : (~0ULL >> (64 - lane)));
+    return (i64)(base + (u64)__popcll(below));
+}
+
+template <typename S, typename K, int N>
+__device__ void tsit5_persistent(const Params& P) {
+    Tsit5Lane<S, K, N> L;
+    bool active = false, exhausted = false;
+    i64 j = 0;
+    for (;;) {
+        bool need = !active && !exhausted;
+        i64 slot = wave_fetch(&P.counters[0], need);
+        if (need) {
+            if (slot < P.N) {
+                j = P.order ? P.order[slot] : slot;
+                tsit5_init(L, P, j);
+                active = true;
+            } else {
+                exhausted = true;
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            int rc = tsit5_attempt(L, P, j);
+            if (rc) {
+                finish_lane<N>(L.c, P, j, L.u, rc);
+                active = false;
+            }
+        }
+    }
+}
+
+// Plain engine: one kernel launch per step attempt, state in HBM.
+// st_scalar rows: t dt dtpropose tprev tstop_target tf dtmax EEst q11 errold
+// st_int rows: iter success_iter accept_step next_step_tstop nf naccept nreject done
+// st_vec rows: u uprev k1 k7 (Tsit5)
+template <int N>
+__device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
+    const double* s = P.st_scalar;
+    const i64 M = P.N;
+    c.t = s[0 * M + j]; c.dt = s[1 * M + j]; c.dtpropose = s[2 * M + j]; c.tprev = s[3 * M + j];
+    c.tstop_target = s[4 * M + j]; c.tf = s[5 * M + j]; c.dtmax = s[6 * M + j];
+    c.EEst = s[7 * M + j]; c.q11 = s[8 * M + j]; c.errold = s[9 * M + j];
+    const int* q = P.st_int;
+    c.iter = q[0 * M + j]; c.success_iter = q[1 * M + j]; c.accept_step = q[2 * M + j];
+    c.next_step_tstop = q[3 * M + j]; c.nf = q[4 * M + j]; c.naccept = q[5 * M + j];
+    c.nreject = q[6 * M + j];
+    c.nsaved = P.nsaved ? P.nsaved[j] : 0;
+    c.spos = 0; c.send = 0; c.tf_in_saveat = 0;
+    if (P.nsaveat > 0) {
+        saveat_init(c, P, P.t0[j]);
+        c.spos = q[8 * M + j];
+    }
+    c.njacs = 0; c.nw = 0; c.nsolve = 0;
+}
+template <int N>
+__device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i64 j, int done) {
+    double* s = P.st_scalar;
+    const i64 M = P.N;
+    s[0 * M + j] = c.t; s[1 * M + j] = c.dt; s[2 * M + j] = c.dtpropose; s[3 * M + j] = c.tprev;
+    s[4 * M + j] = c.tstop_target; s[5 * M + j] = c.tf; s[6 * M + j] = c.dtmax;
+    s[7 * M + j] = c.EEst; s[8 * M + j] = c.q11; s[9 * M + j] = c.errold;
+    int* q = P.st_int;
+    q[0 * M + j] = c.iter; q[1 * M + j] = c.success_iter; q[2 * M + j] = c.accept_step;
+    q[3 * M + j] = c.next_step_tstop; q[4 * M + j] = c.nf; q[5 * M + j] = c.naccept;
+    q[6 * M + j] = c.nreject; q[7 * M + j] = done; q[8 * M + j] = c.spos;
+    if (P.nsaved) P.nsaved[j] = c.nsaved;
+}
+
+template <int N>
+__device__ void tsit5_plain_init(const Params& P) {
+    i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    Tsit5Lane<double, double, N> L;
+    tsit5_init(L, P, j);
+    plain_store_ctl<N>(L.c, P, j, 0);
+    const i64 M = P.N;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        P.st_vec[((i64)0 * P.nmax + i) * M + j] = L.u[i];
+        P.st_vec[((i64)1 * P.nmax + i) * M + j] = L.uprev[i];
+        P.st_vec[((i64)2 * P.nmax + i) * M + j] = L.k1[i];
+        P.st_vec[((i64)3 * P.nmax + i) * M + j] = L.k7[i];
+    }
+}
+
+template <int N>
+__device__ void tsit5_plain_step(const Params& P) {
+    i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    if (P.st_int[7 * P.N + j]) return;
+    Tsit5Lane<double, double, N> L;
+    plain_load_ctl<N>(L.c, P, j);
+    const i64 M = P.N;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        L.u[i] = P.st_vec[((i64)0 * P.nmax + i) * M + j];
+        L.uprev[i] = P.st_vec[((i64)1 * P.nmax + i) * M + j];
+        L.k1[i] = P.st_vec[((i64)2 * P.nmax + i) * M + j];
+        L.k7[i] = P.st_vec[((i64)3 * P.nmax + i) * M + j];
+    }
+#pragma unroll
+    for (int k = 0; k < ODEMI_NP; k++) { L.pd[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; L.p[k] = L.pd[k]; }
+    int rc = tsit5_attempt(L, P, j);
+    if (rc) {
+        finish_lane<N>(L.c, P, j, L.u, rc);
+        plain_store_ctl<N>(L.c, P, j, 1);
+    } else {
+        plain_store_ctl<N>(L.c, P, j, 0);
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            P.st_vec[((i64)0 * P.nmax + i) * M + j] = L.u[i];
+            P.st_vec[((i64)1 * P.nmax + i) * M + j] = L.uprev[i];
+            P.st_vec[((i64)2 * P.nmax + i) * M + j] = L.k1[i];
+            P.st_vec[((i64)3 * P.nmax + i) * M + j] = L.k7[i];
+        }
+        atomicAdd(&P.counters[1], 1ULL);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense LU on a lane-resident matrix (compile-time N, branchless row swaps).
+// Textbook right-looking LU with partial pivoting (first maximum on ties),
+// multipliers scaled by the reciprocal pivot, trailing update a -= l*u as fma.
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__device__ __forceinline__ void lu_factor_reg(T (&A)[N][N], int (&piv)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        int p = k;
+        T amax = fabs(A[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < N; i++) {
+            T v = fabs(A[i][k]);
+            if (v > amax) { amax = v; p = i; }
+        }
+        piv[k] = p;
+#pragma unroll
+        for (int i = k + 1; i < N; i++) {
+            bool s = (p == i);
+#pragma unroll
+            for (int c = 0; c < N; c++) {
+                T a = A[k][c], b = A[i][c];
+                A[k][c] = s ? b : a;
+                A[i][c] = s ? a : b;
+            }
+        }
+        T inv = (T)1 / A[k][k];
+#pragma unroll
+        for (int i = k + 1; i < N; i++) {
+            T l = A[i][k] * inv;
+            A[i][k] = l;
+#pragma unroll
+            for (int c = k + 1; c < N; c++) A[i][c] = fma(-l, A[k][c], A[i][c]);
+        }
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void lu_solve_reg(const T (&A)[N][N], const int (&piv)[N], T (&b)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+#pragma unroll
+        for (int i = k + 1; i < N; i++) {
+            bool s = (piv[k] == i);
+            T x = b[k], y = b[i];
+            b[k] = s ? y : x;
+            b[i] = s ? x : y;
+        }
+    }
+#pragma unroll
+    for (int i = 1; i < N; i++) {
+        T acc = b[i];
+#pragma unroll
+        for (int c = 0; c < i; c++) acc = fma(-A[i][c], b[c], acc);
+        b[i] = acc;
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        T acc = b[i];
+#pragma unroll
+        for (int c = i + 1; c < N; c++) acc = fma(-A[i][c], b[c], acc);
+        b[i] = acc / A[i][i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rodas5P, one trajectory per lane (small n).  FP64 state, RHS and stages.
+//   F = factorisation type: double (FP64 LU) or float (FP32 LU with FP64
+//   iterative refinement against the FP64 W, P.refine_sweeps sweeps).
+// ---------------------------------------------------------------------------
+template <typename F, int N>
+struct RodasLane {
+    Ctl c;
+    double u[N], uprev[N];
+    double J[N][N];
+    double dT[N];
+    double ks[8][N];
+    double dense[3][N];
+    double p[ODEMI_NP];
+};
+
+template <typename F, int N>
+__device__ __forceinline__ void rodas_init(RodasLane<F, N>& L, const Params& P, i64 j) {
+    ctl_init(L.c, P, j);
+#pragma unroll
+    for (int i = 0; i < N; i++) { L.u[i] = P.u0[(i64)i * P.N + j]; L.uprev[i] = L.u[i]; L.dT[i] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < ODEMI_NP; k++) L.p[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0;
+#pragma unroll
+    for (int a = 0; a < N; a++)
+#pragma unroll
+        for (int b = 0; b < N; b++) L.J[a][b] = 0.0;
+    if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
+    if (P.dt_user > 0.0) {
+        L.c.dt = P.dt_user;
+    } else {
+        L.c.dt = initdt_lane<N>(L.c, L.u, L.p, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
+        L.c.nf += 2;
+    }
+    L.c.dtpropose = L.c.dt;
+}
+
+// W x = b with W held in FP64; returns x in b.
+template <typename F, int N>
+__device__ __forceinline__ void rodas_wsolve(const double (&W)[N][N], const F (&LU)[N][N], const int (&piv)[N],
+                                             double (&b)[N], int sweeps) {
+    if constexpr (sizeof(F) == sizeof(double)) {
+        lu_solve_reg<double, N>(LU, piv, b);
+    } else {
+        F xf[N];
+        double x[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) xf[i] = (F)b[i];
+        lu_solve_reg<F, N>(LU, piv, xf);
+#pragma unroll
+        for (int i = 0; i < N; i++) x[i] = (double)xf[i];
+        for (int s = 0; s < sweeps; s++) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                double acc = b[i];
+#pragma unroll
+                for (int c = 0; c < N; c++) acc = fma(-W[i][c], x[c], acc);
+                xf[i] = (F)acc;
+            }
+            lu_solve_reg<F, N>(LU, piv, xf);
+#pragma unroll
+            for (int i = 0; i < N; i++) x[i] = x[i] + (double)xf[i];
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++) b[i] = x[i];
+    }
+}
+
+template <typename F, int N>
+__device__ __forceinline__ void rodas_perform_step(RodasLane<F, N>& L, const Params& P) {
+    Ctl& c = L.c;
+    const double t = c.t, dt = c.dt;
+    const double idt = 1.0 / dt;
+    const double dtgamma = dt * RodasTab::gamma;
+    double fsal[N], du[N], lt[N];
+    odemi_f<double>(fsal, L.uprev, L.p, t);
+    c.nf += 1;
+    // do_newJW (derivative_utils.jl:535-595): J is reused only right after a reject
+    bool new_jac = (c.iter <= 1) ? true : !(c.iter > 1 && c.EEst > 1.0);
+    if (new_jac) {
+        odemi_jac<double>(&L.J[0][0], L.uprev, L.p, t);
+        c.njacs += 1;
+    }
+    // jacobian2W! : W = J - I/(dt*gamma)
+    const double invdtgamma = 1.0 / dtgamma;
+    double W[N][N];
+    F LU[N][N];
+    int piv[N];
+#pragma unroll
+    for (int a = 0; a < N; a++)
+#pragma unroll
+        for (int b = 0; b < N; b++) W[a][b] = (a == b) ? fma(-1.0, invdtgamma, L.J[a][b]) : L.J[a][b];
+    c.nw += 1;
+    if (new_jac) {
+        odemi_tgrad<double>(L.dT, L.uprev, L.p, t);
+        if (!ODEMI_HAS_TGRAD) c.nf += 1;
+    }
+    const double dtd0 = dt * kRd[0];
+#pragma unroll
+    for (int i = 0; i < N; i++) lt[i] = fsal[i] + dtd0 * L.dT[i];
+#pragma unroll
+    for (int a = 0; a < N; a++)
+#pragma unroll
+        for (int b = 0; b < N; b++) LU[a][b] = (F)W[a][b];
+    lu_factor_reg<F, N>(LU, piv);
+    rodas_wsolve<F, N>(W, LU, piv, lt, P.refine_sweeps);
+#pragma unroll
+    for (int i = 0; i < N; i++) L.ks[0][i] = -lt[i];
+    c.nsolve += 1;
+#pragma unroll
+    for (int s = 1; s < 8; s++) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double acc = L.uprev[i];
+#pragma unroll
+            for (int q = 0; q < s; q++) acc = fma(kRA[s][q], L.ks[q][i], acc);
+            L.u[i] = acc;
+        }
+        odemi_f<double>(du, L.u, L.p, fma(kRc[s], dt, t));
+        c.nf += 1;
+        const double dtds = dt * kRd[s];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double acc = fma(dtds, L.dT[i], du[i]);
+#pragma unroll
+            for (int q = 0; q < s; q++) acc = fma(kRC[s][q] * idt, L.ks[q][i], acc);
+            lt[i] = acc;
+        }
+        rodas_wsolve<F, N>(W, LU, piv, lt, P.refine_sweeps);
+#pragma unroll
+        for (int i = 0; i < N; i++) L.ks[s][i] = -lt[i];
+        c.nsolve += 1;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double acc = L.uprev[i];
+#pragma unroll
+        for (int q = 0; q < 7; q++) acc = fma(kRA[7][q], L.ks[q][i], acc);
+        acc = fma(1.0, L.ks[7][i], acc);
+        L.u[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 7; q++) acc = fma(0.0, L.ks[q][i], acc);
+        acc = fma(1.0, L.ks[7][i], acc);
+        du[i] = acc;
+    }
+    c.EEst = residual_norm<N>(du, L.uprev, L.u, P.abstol, P.reltol);
+    if (P.nsaveat > 0) {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < 8; q++) acc = fma(kRH[r][q], L.ks[q][i], acc);
+                L.dense[r][i] = acc;
+            }
+    }
+}
+
+template <typename F, int N>
+__device__ __forceinline__ void rodas_savevalues(RodasLane<F, N>& L, const Params& P, i64 j) {
+    Ctl& c = L.c;
+    while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
+        double curt = P.saveat[c.spos++];
+        if (curt != c.t) {
+            double th = (curt - c.tprev) / c.dt;
+            double th1 = 1.0 - th;
+            double v[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                double in1 = fma(th, L.dense[2][i], L.dense[1][i]);
+                double in2 = fma(th, in1, L.dense[0][i]);
+                double in3 = fma(th1, in2, L.u[i]);
+                v[i] = fma(th, in3, th1 * L.uprev[i]);
+            }
+            save_point<N>(c, P, j, curt, v);
+        } else {
+            if (curt == c.tf && !P.save_end) continue;
+            save_point<N>(c, P, j, c.t, L.u);
+        }
+    }
+    if (P.save_everystep) {
+        int s = c.nsaved;
+        double last = (s > 0 && s - 1 < P.max_saves) ? P.t_out[(i64)(s - 1) * P.N + j] : __longlong_as_double(0x7ff8000000000000LL);
+        if (s == 0 || ((c.t != last || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
+            save_point<N>(c, P, j, c.t, L.u);
+    }
+}
+
+template <typename F, int N>
+__device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P, i64 j) {
+    const CtlConst kc = ctl_const(true);
+    Ctl& c = L.c;
+    if (!(c.t < c.tf)) return RC_SUCCESS;
+    if (c.iter > 0) {
+        if (c.accept_step) {
+            c.success_iter++;
+#pragma unroll
+            for (int i = 0; i < N; i++) L.uprev[i] = L.u[i];
+            c.dt = c.dtpropose;
+            modify_dt_for_tstops(c);
+        } else {
+            reject_controller(c, kc);
+        }
+    }
+    c.iter++;
+    fix_dt_at_bounds(c, P.dtmin);
+    modify_dt_for_tstops(c);
+    int rc = check_error(c, P, all_finite<double, N>(L.u));
+    if (rc != RC_SUCCESS) return rc;
+    if (c.next_step_tstop && fabs(c.dt) < jl_eps(fabs(c.t))) {
+        c.accept_step = 1;
+    } else {
+        rodas_perform_step(L, P);
+    }
+    double ttmp = c.t + c.dt;
+    double q = stepsize_controller(c, kc);
+    c.accept_step = (c.EEst <= 1.0);
+    if (c.accept_step) {
+        c.naccept++;
+        c.tprev = c.t;
+        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tstop_target; }
+        else c.t = ttmp;
+        accept_controller(c, kc, q, P.dtmin);
+        if (P.nsaveat > 0 || P.save_everystep) rodas_savevalues(L, P, j);
+    } else {
+        c.nreject++;
+    }
+    return (c.t < c.tf) ? 0 : RC_SUCCESS;
+}
+
+template <typename F, int N>
+__device__ void rodas_persistent(const Params& P) {
+    RodasLane<F, N> L;
+    bool active = false, exhausted = false;
+    i64 j = 0;
+    for (;;) {
+        bool need = !active && !exhausted;
+        i64 slot = wave_fetch(&P.counters[0], need);
+        if (need) {
+            if (slot < P.N) {
+                j = P.order ? P.order[slot] : slot;
+                rodas_init(L, P, j);
+                active = true;
+            } else {
+                exhausted = true;
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (active) {
+            int rc = rodas_attempt(L, P, j);
+            if (rc) {
+                finish_lane<N>(L.c, P, j, L.u, rc);
+                active = false;
+            }
+        }
+    }
+}
+
+// Plain Rodas5P: st_vec rows u, uprev, dT, J[0..N-1]
+template <int N>
+__device__ __forceinline__ void rodas_plain_store_vec(const RodasLane<double, N>& L, const Params& P, i64 j) {
+    const i64 M = P.N;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        P.st_vec[((i64)0 * P.nmax + i) * M + j] = L.u[i];
+        P.st_vec[((i64)1 * P.nmax + i) * M + j] = L.uprev[i];
+        P.st_vec[((i64)2 * P.nmax + i) * M + j] = L.dT[i];
+#pragma unroll
+        for (int b = 0; b < N; b++) P.st_vec[((i64)(3 + i) * P.nmax + b) * M + j] = L.J[i][b];
+    }
+}
+template <int N>
+__device__ void rodas_plain_init(const Params& P) {
+    i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    RodasLane<double, N> L;
+    rodas_init(L, P, j);
+    plain_store_ctl<N>(L.c, P, j, 0);
+    rodas_plain_store_vec<N>(L, P, j);
+}
+template <int N>
+__device__ void rodas_plain_step(const Params& P) {
+    i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    if (P.st_int[7 * P.N + j]) return;
+    RodasLane<double, N> L;
+    plain_load_ctl<N>(L.c, P, j);
+    const i64 M = P.N;
+    // counters beyond nf/naccept/nreject live in st_int rows 9..11
+    L.c.njacs = P.st_int[9 * M + j]; L.c.nw = P.st_int[10 * M + j]; L.c.nsolve = P.st_int[11 * M + j];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        L.u[i] = P.st_vec[((i64)0 * P.nmax + i) * M + j];
+        L.uprev[i] = P.st_vec[((i64)1 * P.nmax + i) * M + j];
+        L.dT[i] = P.st_vec[((i64)2 * P.nmax + i) * M + j];
+#pragma unroll
+        for (int b = 0; b < N; b++) L.J[i][b] = P.st_vec[((i64)(3 + i) * P.nmax + b) * M + j];
+    }
+#pragma unroll
+    for (int k = 0; k < ODEMI_NP; k++) L.p[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0;
+    int rc = rodas_attempt(L, P, j);
+    if (rc) {
+        finish_lane<N>(L.c, P, j, L.u, rc);
+        plain_store_ctl<N>(L.c, P, j, 1);
+    } else {
+        plain_store_ctl<N>(L.c, P, j, 0);
+        P.st_int[9 * M + j] = L.c.njacs; P.st_int[10 * M + j] = L.c.nw; P.st_int[11 * M + j] = L.c.nsolve;
+        rodas_plain_store_vec<N>(L, P, j);
+        atomicAdd(&P.counters[1], 1ULL);
+    }
+}
+
+}  // namespace odemi

@@ -1,0 +1,550 @@
+// odemi_runtime.cpp -- host runtime of the MI355X ensemble ODE engine.
+//
+// Implements the C ABI in include/odemi.h: device context, hipRTC compilation
+// of traced problems (with an on-disk code-object cache), device workspace
+// management and the launch logic of the two engines:
+//   plain       one kernel launch per step attempt over all live trajectories,
+//               integrator state round-trips through HBM (FP64 baseline);
+//   persistent  one launch; each lane (or workgroup) owns a trajectory in
+//               registers/LDS, retires it and refills from a work queue.
+#include "odemi.h"
+
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+namespace {
+
+const char* kDeviceSource =
+#include "odemi_device_src.inc"
+    ;
+
+struct Params {  // must match odemi::Params in kernels/odemi_device.h
+    int64_t N;
+    int nmax, np;
+    const int* nvec;
+    const double *u0, *p, *t0, *tf;
+    const int64_t* order;
+    double reltol, abstol, dtmax, dtmin, dt_user;
+    int64_t maxiters;
+    const double* saveat;
+    int nsaveat;
+    int save_start, save_end, save_everystep, save_end_user;
+    int max_saves;
+    double* t_out;
+    double* u_out;
+    int* nsaved;
+    double* u_final;
+    double* t_final;
+    int* retcode;
+    int* stats;
+    unsigned long long* counters;
+    double* st_scalar;
+    double* st_vec;
+    int* st_int;
+    int refine_sweeps;
+    int pad0;
+};
+
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL) {
+    for (unsigned char c : s) { h ^= c; h *= 1099511628211ULL; }
+    return h;
+}
+
+std::string cache_dir() {
+    const char* e = getenv("ODEMI_CACHE_DIR");
+    if (e && *e) return e;
+    const char* h = getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.cache/odemi";
+}
+
+void mkdirs(const std::string& path) {
+    std::string cur;
+    std::stringstream ss(path);
+    std::string part;
+    if (!path.empty() && path[0] == '/') cur = "/";
+    while (std::getline(ss, part, '/')) {
+        if (part.empty()) continue;
+        cur += part + "/";
+        mkdir(cur.c_str(), 0755);
+    }
+}
+
+}  // namespace
+
+struct odemi_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    std::string err;
+    std::mutex mu;
+    // grow-only device workspace
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct odemi_rhs {
+    odemi_ctx* ctx = nullptr;
+    hipModule_t mod = nullptr;
+    int n = 0, np = 0, flags = 0;
+    hipFunction_t tsit5_persist_f64 = nullptr, tsit5_persist_mixed = nullptr;
+    hipFunction_t tsit5_plain_init = nullptr, tsit5_plain_step = nullptr;
+    hipFunction_t rodas_persist_f64 = nullptr, rodas_persist_mixed = nullptr;
+    hipFunction_t rodas_plain_init = nullptr, rodas_plain_step = nullptr;
+    hipFunction_t rodas_wg_f64 = nullptr, rodas_wg_mixed = nullptr;
+};
+
+#define HIPCHK(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+            return -1;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+extern "C" int odemi_abi_version(void) { return 1; }
+
+extern "C" odemi_ctx* odemi_create(int device) {
+    odemi_ctx* c = new odemi_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { c->err = "hipSetDevice failed"; return c; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) c->err = "stream create failed";
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+    return c;
+}
+
+extern "C" void odemi_destroy(odemi_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->ws) (void)hipFree(c->ws);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* odemi_last_error(const odemi_ctx* c) { return c ? c->err.c_str() : "null ctx"; }
+
+extern "C" int odemi_device_info(odemi_ctx* c, int* cu, size_t* fr, size_t* tot) {
+    (void)hipSetDevice(c->device);
+    if (cu) *cu = c->cu_count;
+    size_t f = 0, t = 0;
+    HIPCHK(c, hipMemGetInfo(&f, &t));
+    if (fr) *fr = f;
+    if (tot) *tot = t;
+    return 0;
+}
+
+extern "C" void odemi_default_args(odemi_solve_args* a, int alg) {
+    memset(a, 0, sizeof(*a));
+    a->alg = alg;
+    a->engine = ODEMI_ENGINE_PERSISTENT;
+    a->precision = ODEMI_FP64;
+    a->reltol = 1e-3;
+    a->abstol = 1e-6;
+    a->dtmax = 0.0;
+    a->dtmin = 0.0;
+    a->dt = 0.0;
+    a->maxiters = 1000000;
+    a->save_start = 1;
+    a->save_end = 1;
+    a->save_everystep = 0;
+    a->save_end_user = -1;
+    a->refine_sweeps = 2;
+}
+
+static std::string build_source(const char* problem_src, int n, int np, int flags) {
+    std::ostringstream s;
+    s << "#define ODEMI_N " << n << "\n#define ODEMI_NP " << (np > 0 ? np : 1) << "\n";
+    s << "#define ODEMI_HAS_TGRAD " << ((flags & ODEMI_HAS_USER_TGRAD) ? 1 : 0) << "\n";
+    if (flags & ODEMI_COMPILE_WG) s << "#define ODEMI_WITH_WG 1\n";
+    s << problem_src << "\n";
+    s << kDeviceSource << "\n";
+    if (flags & ODEMI_COMPILE_TSIT5) {
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_persist_f64(odemi::Params P)"
+             " { odemi::tsit5_persistent<double, double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_persist_mixed(odemi::Params P)"
+             " { odemi::tsit5_persistent<double, float, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_init(odemi::Params P)"
+             " { odemi::tsit5_plain_init<ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_step(odemi::Params P)"
+             " { odemi::tsit5_plain_step<ODEMI_N>(P); }\n";
+    }
+    if (flags & ODEMI_COMPILE_RODAS5P) {
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_persist_f64(odemi::Params P)"
+             " { odemi::rodas_persistent<double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_persist_mixed(odemi::Params P)"
+             " { odemi::rodas_persistent<float, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_plain_init(odemi::Params P)"
+             " { odemi::rodas_plain_init<ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_plain_step(odemi::Params P)"
+             " { odemi::rodas_plain_step<ODEMI_N>(P); }\n";
+    }
+    if (flags & ODEMI_COMPILE_WG) {
+        s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) odemi_rodas_wg_f64(odemi::Params P)"
+             " { odemi::rodas_wg<double>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) odemi_rodas_wg_mixed(odemi::Params P)"
+             " { odemi::rodas_wg<float>(P); }\n";
+    }
+    return s.str();
+}
+
+static const char* kRtcOpts[] = {
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+};
+
+static bool rtc_compile(const std::string& src, std::vector<char>& code, std::string& log) {
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "odemi_problem.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    int nopt = (int)(sizeof(kRtcOpts) / sizeof(kRtcOpts[0]));
+    hiprtcResult r = hiprtcCompileProgram(prog, nopt, kRtcOpts);
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    if (ls > 1) {
+        std::vector<char> l(ls + 1, 0);
+        hiprtcGetProgramLog(prog, l.data());
+        log = l.data();
+    }
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return false;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    code.resize(cs);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    return true;
+}
+
+// Compile (or fetch from the disk cache) the code object for a problem.
+// Exposed so that a build step can pre-populate the cache without a GPU.
+extern "C" int odemi_rtc_codeobject(const char* problem_src, int n, int np, int flags, char* path_out,
+                                    int path_cap, double* seconds, char* log_out, int log_cap) {
+    std::string src = build_source(problem_src, n, np, flags);
+    std::string key = src;
+    for (const char* o : kRtcOpts) key += o;
+    int ver = 0;
+    hiprtcVersion(&ver, &ver);
+    key += std::to_string(ver);
+    char hex[32];
+    snprintf(hex, sizeof(hex), "%016llx", (unsigned long long)fnv1a(key));
+    std::string dir = cache_dir();
+    std::string path = dir + "/odemi_" + hex + ".co";
+    auto t0 = std::chrono::steady_clock::now();
+    struct stat st;
+    if (stat(path.c_str(), &st) != 0) {
+        std::vector<char> code;
+        std::string log;
+        if (!rtc_compile(src, code, log)) {
+            if (log_out && log_cap > 0) { strncpy(log_out, log.c_str(), log_cap - 1); log_out[log_cap - 1] = 0; }
+            return -1;
+        }
+        mkdirs(dir);
+        std::string tmp = path + ".tmp" + std::to_string(getpid());
+        std::ofstream f(tmp, std::ios::binary);
+        f.write(code.data(), (std::streamsize)code.size());
+        f.close();
+        rename(tmp.c_str(), path.c_str());
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (path_out && path_cap > 0) { strncpy(path_out, path.c_str(), path_cap - 1); path_out[path_cap - 1] = 0; }
+    return 0;
+}
+
+extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src, int n, int np, int flags,
+                                        double* compile_seconds) {
+    if (!ctx) return nullptr;
+    if (n < 1) { ctx->err = "n must be >= 1"; return nullptr; }
+    if (!(flags & (ODEMI_COMPILE_TSIT5 | ODEMI_COMPILE_RODAS5P | ODEMI_COMPILE_WG))) flags |= ODEMI_COMPILE_TSIT5;
+    char path[4096];
+    static char log[65536];
+    double secs = 0;
+    if (odemi_rtc_codeobject(problem_src, n, np, flags, path, sizeof(path), &secs, log, sizeof(log)) != 0) {
+        ctx->err = std::string("hipRTC compile failed:\n") + log;
+        return nullptr;
+    }
+    if (compile_seconds) *compile_seconds = secs;
+    std::ifstream f(path, std::ios::binary);
+    std::vector<char> code((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    (void)hipSetDevice(ctx->device);
+    odemi_rhs* r = new odemi_rhs();
+    r->ctx = ctx;
+    r->n = n;
+    r->np = np;
+    r->flags = flags;
+    hipError_t e = hipModuleLoadData(&r->mod, code.data());
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
+        delete r;
+        return nullptr;
+    }
+    auto get = [&](const char* name, hipFunction_t* fn) {
+        if (hipModuleGetFunction(fn, r->mod, name) != hipSuccess) *fn = nullptr;
+    };
+    if (flags & ODEMI_COMPILE_TSIT5) {
+        get("odemi_tsit5_persist_f64", &r->tsit5_persist_f64);
+        get("odemi_tsit5_persist_mixed", &r->tsit5_persist_mixed);
+        get("odemi_tsit5_plain_init", &r->tsit5_plain_init);
+        get("odemi_tsit5_plain_step", &r->tsit5_plain_step);
+    }
+    if (flags & ODEMI_COMPILE_RODAS5P) {
+        get("odemi_rodas_persist_f64", &r->rodas_persist_f64);
+        get("odemi_rodas_persist_mixed", &r->rodas_persist_mixed);
+        get("odemi_rodas_plain_init", &r->rodas_plain_init);
+        get("odemi_rodas_plain_step", &r->rodas_plain_step);
+    }
+    if (flags & ODEMI_COMPILE_WG) {
+        get("odemi_rodas_wg_f64", &r->rodas_wg_f64);
+        get("odemi_rodas_wg_mixed", &r->rodas_wg_mixed);
+    }
+    return r;
+}
+
+extern "C" void odemi_rhs_destroy(odemi_rhs* r) {
+    if (!r) return;
+    if (r->mod) {
+        (void)hipSetDevice(r->ctx->device);
+        (void)hipModuleUnload(r->mod);
+    }
+    delete r;
+}
+
+namespace {
+
+// Bump allocator over the ctx workspace (256-byte aligned slices).
+struct Arena {
+    char* base;
+    size_t cap, off = 0;
+    void* take(size_t bytes) {
+        size_t o = (off + 255) & ~size_t(255);
+        if (o + bytes > cap) return nullptr;
+        off = o + bytes;
+        return base + o;
+    }
+};
+
+int ensure_ws(odemi_ctx* c, size_t bytes) {
+    if (c->ws_bytes >= bytes) return 0;
+    if (c->ws) { (void)hipFree(c->ws); c->ws = nullptr; c->ws_bytes = 0; }
+    size_t b = std::max(bytes, (size_t)1 << 20);
+    HIPCHK(c, hipMalloc(&c->ws, b));
+    c->ws_bytes = b;
+    return 0;
+}
+
+int launch(odemi_ctx* c, hipFunction_t fn, int grid, int block, Params& P, hipStream_t s, size_t shmem = 0) {
+    void* args[] = {&P};
+    HIPCHK(c, hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)shmem, s, args, nullptr));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
+    if (!ctx || !rhs || !a) return -1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    const int64_t N = a->N;
+    const int nmax = a->nmax;
+    const int np = a->np;
+    a->kernel_ms = 0.0;
+    a->launches = 0;
+    if (N <= 0) return 0;
+    if (nmax != rhs->n) { ctx->err = "args.nmax does not match the compiled problem dimension"; return -2; }
+    if (np > std::max(rhs->np, 1)) { ctx->err = "args.np exceeds the compiled parameter count"; return -2; }
+    if (a->max_saves < 0 || (a->max_saves > 0 && (!a->t_out || !a->u_out))) { ctx->err = "save buffers missing"; return -2; }
+    if (!a->retcode) { ctx->err = "retcode output is required"; return -2; }
+    if (a->nsaveat < 0 || (a->nsaveat > 0 && !a->saveat)) { ctx->err = "saveat buffer missing"; return -2; }
+    for (int i = 1; i < a->nsaveat; i++)
+        if (!a->device_ptrs && a->saveat[i] < a->saveat[i - 1]) { ctx->err = "saveat must be sorted"; return -2; }
+    const bool rodas = (a->alg == ODEMI_RODAS5P);
+    const bool plain = (a->engine == ODEMI_ENGINE_PLAIN);
+    const bool mixed = (a->precision == ODEMI_MIXED);
+    const bool use_wg = !plain && rodas && (rhs->flags & ODEMI_COMPILE_WG);
+    hipFunction_t fn_main = nullptr, fn_init = nullptr;
+    if (!rodas) {
+        if (plain) { fn_init = rhs->tsit5_plain_init; fn_main = rhs->tsit5_plain_step; }
+        else fn_main = mixed ? rhs->tsit5_persist_mixed : rhs->tsit5_persist_f64;
+    } else if (use_wg) {
+        fn_main = mixed ? rhs->rodas_wg_mixed : rhs->rodas_wg_f64;
+    } else {
+        if (plain) { fn_init = rhs->rodas_plain_init; fn_main = rhs->rodas_plain_step; }
+        else fn_main = mixed ? rhs->rodas_persist_mixed : rhs->rodas_persist_f64;
+    }
+    if (plain && mixed) { ctx->err = "the plain engine is FP64 only"; return -3; }
+    if (!fn_main || (plain && !fn_init)) { ctx->err = "requested kernel was not compiled for this problem"; return -3; }
+    if (a->nvec && !use_wg) { ctx->err = "uneven batches (nvec) need the workgroup Rodas5P kernels"; return -3; }
+
+    hipStream_t s = a->stream ? (hipStream_t)a->stream : ctx->stream;
+    const bool dev = a->device_ptrs != 0;
+    const size_t N_ = (size_t)N;
+    const size_t ms_ = (size_t)std::max(a->max_saves, 0);
+    const size_t vrows = rodas ? (size_t)(3 + nmax) : 4;
+    // ---- workspace layout ----
+    size_t need = 0;
+    auto add = [&](size_t b) { need += ((b + 255) & ~size_t(255)); };
+    add(64);                                   // counters
+    if (a->sort_by_cost) add(N_ * 8);          // order
+    if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(12 * N_ * 4); }
+    if (!dev) {
+        add(N_ * nmax * 8); add(N_ * std::max(np, 1) * 8); add(N_ * 8); add(N_ * 8);
+        add(std::max(a->nsaveat, 1) * 8);
+        add(N_ * std::max<size_t>(ms_, 1) * 8); add(N_ * std::max<size_t>(ms_, 1) * nmax * 8);
+        add(N_ * 4); add(N_ * nmax * 8); add(N_ * 8); add(N_ * 4); add(N_ * 6 * 4);
+        if (a->nvec) add(N_ * 4);
+    }
+    if (ensure_ws(ctx, need)) return -1;
+    Arena ar{(char*)ctx->ws, ctx->ws_bytes};
+    Params P;
+    memset(&P, 0, sizeof(P));
+    P.N = N; P.nmax = nmax; P.np = np;
+    P.reltol = a->reltol; P.abstol = a->abstol; P.dtmax = a->dtmax; P.dtmin = a->dtmin; P.dt_user = a->dt;
+    P.maxiters = a->maxiters;
+    P.nsaveat = a->nsaveat;
+    P.save_start = a->save_start; P.save_end = a->save_end; P.save_everystep = a->save_everystep;
+    P.save_end_user = a->save_end_user; P.max_saves = a->max_saves;
+    P.refine_sweeps = a->refine_sweeps;
+    P.counters = (unsigned long long*)ar.take(64);
+    HIPCHK(ctx, hipMemsetAsync(P.counters, 0, 64, s));
+    if (dev) {
+        P.u0 = a->u0; P.p = a->p; P.t0 = a->t0; P.tf = a->tf; P.saveat = a->saveat; P.nvec = a->nvec;
+        P.t_out = a->t_out; P.u_out = a->u_out; P.nsaved = a->nsaved; P.u_final = a->u_final;
+        P.t_final = a->t_final; P.retcode = a->retcode; P.stats = a->stats;
+    } else {
+        double* u0 = (double*)ar.take(N_ * nmax * 8);
+        double* pp = (double*)ar.take(N_ * std::max(np, 1) * 8);
+        double* t0 = (double*)ar.take(N_ * 8);
+        double* tf = (double*)ar.take(N_ * 8);
+        double* sa = (double*)ar.take(std::max(a->nsaveat, 1) * 8);
+        HIPCHK(ctx, hipMemcpyAsync(u0, a->u0, N_ * nmax * 8, hipMemcpyHostToDevice, s));
+        if (np > 0) HIPCHK(ctx, hipMemcpyAsync(pp, a->p, N_ * np * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(t0, a->t0, N_ * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipMemcpyAsync(tf, a->tf, N_ * 8, hipMemcpyHostToDevice, s));
+        if (a->nsaveat > 0) HIPCHK(ctx, hipMemcpyAsync(sa, a->saveat, a->nsaveat * 8, hipMemcpyHostToDevice, s));
+        P.u0 = u0; P.p = pp; P.t0 = t0; P.tf = tf; P.saveat = sa;
+        P.t_out = (double*)ar.take(N_ * std::max<size_t>(ms_, 1) * 8);
+        P.u_out = (double*)ar.take(N_ * std::max<size_t>(ms_, 1) * nmax * 8);
+        P.nsaved = (int*)ar.take(N_ * 4);
+        P.u_final = (double*)ar.take(N_ * nmax * 8);
+        P.t_final = (double*)ar.take(N_ * 8);
+        P.retcode = (int*)ar.take(N_ * 4);
+        P.stats = (int*)ar.take(N_ * 6 * 4);
+        if (a->nvec) {
+            int* nv = (int*)ar.take(N_ * 4);
+            HIPCHK(ctx, hipMemcpyAsync(nv, a->nvec, N_ * 4, hipMemcpyHostToDevice, s));
+            P.nvec = nv;
+        }
+    }
+    if (a->sort_by_cost) {
+        // Longest-processing-time-first: hand out trajectories by decreasing
+        // predicted cost (integration span, times n^3 for uneven Rosenbrock batches).
+        std::vector<double> h0(N_), h1(N_);
+        std::vector<int32_t> nv;
+        if (dev) {
+            HIPCHK(ctx, hipMemcpyAsync(h0.data(), a->t0, N_ * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(ctx, hipMemcpyAsync(h1.data(), a->tf, N_ * 8, hipMemcpyDeviceToHost, s));
+            if (a->nvec) {
+                nv.resize(N_);
+                HIPCHK(ctx, hipMemcpyAsync(nv.data(), a->nvec, N_ * 4, hipMemcpyDeviceToHost, s));
+            }
+            HIPCHK(ctx, hipStreamSynchronize(s));
+        } else {
+            memcpy(h0.data(), a->t0, N_ * 8);
+            memcpy(h1.data(), a->tf, N_ * 8);
+            if (a->nvec) nv.assign(a->nvec, a->nvec + N_);
+        }
+        std::vector<double> cost(N_);
+        for (size_t j = 0; j < N_; j++) {
+            cost[j] = h1[j] - h0[j];
+            if (!nv.empty()) cost[j] *= (double)nv[j] * nv[j] * nv[j];
+        }
+        std::vector<int64_t> ord(N_);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
+        int64_t* dord = (int64_t*)ar.take(N_ * 8);
+        HIPCHK(ctx, hipMemcpyAsync(dord, ord.data(), N_ * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+        P.order = dord;
+    }
+    if (plain) {
+        P.st_scalar = (double*)ar.take(16 * N_ * 8);
+        P.st_vec = (double*)ar.take(vrows * nmax * N_ * 8);
+        P.st_int = (int*)ar.take(12 * N_ * 4);
+        HIPCHK(ctx, hipMemsetAsync(P.st_int, 0, 12 * N_ * 4, s));
+    }
+    if (P.nsaved) HIPCHK(ctx, hipMemsetAsync(P.nsaved, 0, N_ * 4, s));
+
+    const int block = a->block > 0 ? a->block : 256;
+    int64_t launches = 0;
+    HIPCHK(ctx, hipEventRecord(ctx->ev0, s));
+    if (plain) {
+        const int grid = (int)((N + block - 1) / block);
+        if (launch(ctx, fn_init, grid, block, P, s)) return -1;
+        launches++;
+        unsigned long long live = 1;
+        const int batch = 32;
+        while (live) {
+            for (int k = 0; k < batch - 1; k++)
+                if (launch(ctx, fn_main, grid, block, P, s)) return -1;
+            HIPCHK(ctx, hipMemsetAsync(P.counters + 1, 0, 8, s));
+            if (launch(ctx, fn_main, grid, block, P, s)) return -1;
+            launches += batch;
+            HIPCHK(ctx, hipMemcpyAsync(&live, P.counters + 1, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(ctx, hipStreamSynchronize(s));
+            if (launches > (int64_t)a->maxiters * 4 + 1024) { ctx->err = "plain engine did not terminate"; return -4; }
+        }
+    } else {
+        const int threads = use_wg ? 256 : block;   // ODEMI_WG_THREADS for the workgroup kernels
+        int nb = 0;
+        HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, threads, 0));
+        if (nb < 1) nb = 1;
+        int64_t grid = (int64_t)nb * std::max(ctx->cu_count, 1);
+        const int64_t per_block = use_wg ? 1 : threads;
+        grid = std::min<int64_t>(grid, (N + per_block - 1) / per_block);
+        if (launch(ctx, fn_main, (int)grid, threads, P, s)) return -1;
+        launches++;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev1, s));
+    if (!dev) {
+        if (ms_ > 0) {
+            HIPCHK(ctx, hipMemcpyAsync(a->t_out, P.t_out, N_ * ms_ * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(ctx, hipMemcpyAsync(a->u_out, P.u_out, N_ * ms_ * nmax * 8, hipMemcpyDeviceToHost, s));
+        }
+        if (a->nsaved) HIPCHK(ctx, hipMemcpyAsync(a->nsaved, P.nsaved, N_ * 4, hipMemcpyDeviceToHost, s));
+        if (a->u_final) HIPCHK(ctx, hipMemcpyAsync(a->u_final, P.u_final, N_ * nmax * 8, hipMemcpyDeviceToHost, s));
+        if (a->t_final) HIPCHK(ctx, hipMemcpyAsync(a->t_final, P.t_final, N_ * 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipMemcpyAsync(a->retcode, P.retcode, N_ * 4, hipMemcpyDeviceToHost, s));
+        if (a->stats) HIPCHK(ctx, hipMemcpyAsync(a->stats, P.stats, N_ * 6 * 4, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    a->kernel_ms = ms;
+    a->launches = launches;
+    return 0;
+}

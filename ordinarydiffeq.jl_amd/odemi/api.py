@@ -1,0 +1,392 @@
+"""Host-side mirror of the reference's ensemble surface.
+
+    solve(EnsembleProblem(prob; prob_func, output_func, reduction),
+          Tsit5() | Rodas5P(), EnsembleMI355X(); trajectories, reltol, abstol, saveat, ...)
+
+returns an EnsembleSolution, like SciMLBase's ``EnsembleThreads`` driver does
+(call sites: lib/DiffEqBase/test/downstream/ensemble_thread_safety.jl:1-31;
+``prob_func(prob, ctx)`` / ``output_func(sol, ctx)`` signatures: NEWS.md:209-225).
+Keyword defaults and save rules follow lib/OrdinaryDiffEqCore/src/solve.jl:128-203,
+381-393 and 596-599; saveat filtering follows solve.jl:1103-1123.  The numerics
+run in libodemi.so on the GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import enum
+import math
+import time
+from fractions import Fraction
+
+import numpy as np
+
+from . import _lib
+from .tracer import trace_problem
+
+__all__ = ["ReturnCode", "ODEFunction", "ODEProblem", "remake", "Tsit5", "Rodas5P", "EnsembleMI355X",
+           "EnsembleContext", "EnsembleProblem", "DEStats", "ODESolution", "EnsembleSolution", "solve",
+           "solve_arrays", "get_context", "compile_problem"]
+
+
+class ReturnCode(enum.IntEnum):
+    """SciMLBase.ReturnCode (values in enum order)."""
+    Default = 0
+    Success = 1
+    Terminated = 2
+    DtNaN = 3
+    MaxIters = 4
+    MaxNumSub = 5
+    DtLessThanMin = 6
+    Unstable = 7
+
+
+class ODEFunction:
+    """In-place RHS f(du, u, p, t) with optional analytic jac(J, u, p, t) and
+    tgrad(dT, u, p, t) (the reference's ODEFunction fields f, jac, tgrad)."""
+
+    def __init__(self, f, jac=None, tgrad=None):
+        if isinstance(f, ODEFunction):
+            f, jac, tgrad = f.f, jac or f.jac, tgrad or f.tgrad
+        self.f, self.jac, self.tgrad = f, jac, tgrad
+
+
+class ODEProblem:
+    def __init__(self, f, u0, tspan, p=()):
+        self.f = f if isinstance(f, ODEFunction) else ODEFunction(f)
+        self.u0 = np.array(u0, dtype=np.float64).reshape(-1)
+        self.tspan = (float(tspan[0]), float(tspan[1]))
+        self.p = np.array(p if p is not None else (), dtype=np.float64).reshape(-1)
+
+
+def remake(prob, u0=None, tspan=None, p=None, f=None):
+    return ODEProblem(prob.f if f is None else f, prob.u0 if u0 is None else u0,
+                      prob.tspan if tspan is None else tspan, prob.p if p is None else p)
+
+
+class Tsit5:
+    code = _lib.TSIT5
+    order = 5
+
+    def __repr__(self):
+        return "Tsit5()"
+
+
+class Rodas5P:
+    code = _lib.RODAS5P
+    order = 5
+
+    def __init__(self, autodiff=None, linsolve=None):
+        self.autodiff, self.linsolve = autodiff, linsolve
+
+    def __repr__(self):
+        return "Rodas5P()"
+
+
+class EnsembleMI355X:
+    """Ensemble algorithm running every trajectory on one MI355X.
+
+    engine:    "persistent" (one launch, work queue, state in registers) or
+               "plain" (FP64, one launch per step attempt, state in HBM).
+    precision: "fp64", or "mixed" (Tsit5: FP32 stages, FP64 state/t/error/controller;
+               Rodas5P: FP32 LU with FP64 iterative refinement).
+    """
+
+    def __init__(self, device=0, engine="persistent", precision="fp64", refine_sweeps=2, sort_by_cost=True,
+                 block=0):
+        if engine not in ("persistent", "plain"):
+            raise ValueError("engine must be 'persistent' or 'plain'")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if engine == "plain" and precision != "fp64":
+            raise ValueError("the plain engine is FP64 only")
+        self.device, self.engine, self.precision = device, engine, precision
+        self.refine_sweeps, self.sort_by_cost, self.block = refine_sweeps, sort_by_cost, block
+
+
+class EnsembleContext:
+    """Per-trajectory context handed to prob_func/output_func (v7 API)."""
+
+    def __init__(self, sim_id, repeat, rng):
+        self.sim_id, self.repeat, self.rng = sim_id, repeat, rng
+
+    @property
+    def i(self):
+        return self.sim_id
+
+
+class EnsembleProblem:
+    def __init__(self, prob, prob_func=None, output_func=None, reduction=None, u_init=None, safetycopy=True):
+        self.prob = prob
+        self.prob_func = prob_func or (lambda prob, ctx: prob)
+        self.output_func = output_func or (lambda sol, ctx: (sol, False))
+        self.reduction = reduction
+        self.u_init = u_init
+        self.safetycopy = safetycopy
+
+
+class DEStats:
+    __slots__ = ("nf", "naccept", "nreject", "njacs", "nw", "nsolve")
+
+    def __init__(self, nf, naccept, nreject, njacs, nw, nsolve):
+        self.nf, self.naccept, self.nreject = int(nf), int(naccept), int(nreject)
+        self.njacs, self.nw, self.nsolve = int(njacs), int(nw), int(nsolve)
+
+    def __repr__(self):
+        return (f"DEStats(nf={self.nf}, naccept={self.naccept}, nreject={self.nreject}, "
+                f"njacs={self.njacs}, nw={self.nw}, nsolve={self.nsolve})")
+
+
+class ODESolution:
+    def __init__(self, t, u, retcode, stats, prob=None, alg=None):
+        self.t, self.u, self.retcode, self.stats, self.prob, self.alg = t, u, retcode, stats, prob, alg
+
+
+class EnsembleSolution:
+    """Ensemble result.  ``u`` is a list of per-trajectory outputs (ODESolution by
+    default); the packed arrays (``u_final``, ``t_out``, ``u_out``, ``retcodes``,
+    ``stats``) are kept for bulk access."""
+
+    def __init__(self, u, elapsed, arrays, converged=True):
+        self.u, self.elapsedTime, self.arrays, self.converged = u, elapsed, arrays, converged
+
+    def __len__(self):
+        return len(self.u)
+
+
+_CTX = {}
+
+
+def get_context(device=0):
+    c = _CTX.get(device)
+    if c is None:
+        c = _lib.Context(device)
+        _CTX[device] = c
+    return c
+
+
+_TRACES = {}
+
+
+def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None):
+    """Trace fun (ODEFunction) for (n, np_) and hipRTC-compile it. Returns (rhs, traced, seconds)."""
+    key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_)
+    tr = _TRACES.get(key)
+    if tr is None:
+        tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad)
+        _TRACES[key] = tr
+    flags = 0
+    if "tsit5" in algs:
+        flags |= _lib.COMPILE_TSIT5
+    if "rodas5p" in algs:
+        flags |= _lib.COMPILE_RODAS5P
+    if tr.has_user_tgrad:
+        flags |= _lib.HAS_USER_TGRAD
+    ctx = ctx or get_context()
+    h, secs = ctx.compile(tr.src, n, max(np_, 1), flags)
+    return h, tr, secs
+
+
+def _decimal(x):
+    return Fraction(repr(float(x)))
+
+
+def saveat_times(saveat, t0, tf):
+    """Saveat list for (t0, tf]: a scalar interval gives (t0+d):d:tf (decimal-exact
+    like Julia's float ranges), a collection is filtered to t0 < s <= tf."""
+    if saveat is None:
+        return np.zeros(0)
+    if np.isscalar(saveat):
+        d = abs(float(saveat))
+        if d == 0:
+            raise ValueError("saveat interval must be nonzero")
+        start = _decimal(t0) + _decimal(d)
+        step = _decimal(d)
+        out = []
+        k = 0
+        while True:
+            v = start + k * step
+            if v > _decimal(tf) * (1 + Fraction(1, 10 ** 12)):
+                break
+            fv = float(v)
+            if fv <= tf:
+                out.append(fv)
+            k += 1
+        return np.array(out, dtype=np.float64)
+    arr = np.sort(np.asarray(list(saveat), dtype=np.float64))
+    return arr
+
+
+def _resolve_save_flags(saveat, save_start, save_end, save_everystep, t0, tf):
+    is_num = saveat is not None and np.isscalar(saveat)
+    empty = saveat is None or (not is_num and len(saveat) == 0)
+    sv = list(saveat) if (not empty and not is_num) else []
+    if save_everystep is None:
+        save_everystep = empty
+    if save_start is None:
+        save_start = save_everystep or empty or is_num or (t0 in sv)
+    save_end_user = -1 if save_end is None else int(bool(save_end))
+    if save_end is None:
+        save_end = save_everystep or empty or is_num or (tf in sv)
+    return bool(save_start), bool(save_end), bool(save_everystep), save_end_user
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abstol=None, saveat=None,
+                 save_start=False, save_end=True, save_everystep=False, save_end_user=-1, max_saves=None,
+                 dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None):
+    """Bulk entry point on packed SoA arrays: u0 (n, N), p (np, N), t0/tf (N,).
+
+    Arrays may be numpy (host) or torch CUDA tensors (device-resident, zero
+    copies).  Returns a dict of output arrays of the same kind."""
+    ea = ensemblealg or EnsembleMI355X()
+    ctx = ctx or get_context(ea.device)
+    fun = fun if isinstance(fun, ODEFunction) else ODEFunction(fun)
+    dev = _is_torch(u0)
+    n, N = int(u0.shape[0]), int(u0.shape[1])
+    np_ = int(p.shape[0]) if p is not None else 0
+    algname = "tsit5" if alg.code == _lib.TSIT5 else "rodas5p"
+    h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx)
+    reltol = 1e-3 if reltol is None else float(reltol)
+    abstol = 1e-6 if abstol is None else float(abstol)
+    sa = np.zeros(0) if saveat is None else np.asarray(saveat, dtype=np.float64)
+    if max_saves is None:
+        max_saves = (1 if save_end else 0) + (1 if save_start else 0) + len(sa)
+        if save_everystep:
+            max_saves = 1024
+    a = ctx.default_args(alg.code)
+    a.engine = _lib.ENGINE_PLAIN if ea.engine == "plain" else _lib.ENGINE_PERSISTENT
+    a.precision = _lib.MIXED if ea.precision == "mixed" else _lib.FP64
+    a.refine_sweeps = ea.refine_sweeps
+    a.sort_by_cost = int(ea.sort_by_cost)
+    a.block = ea.block
+    a.N, a.nmax, a.np = N, n, np_
+    a.reltol, a.abstol = reltol, abstol
+    a.dtmax = 0.0 if dtmax is None else float(dtmax)
+    a.dtmin = float(dtmin)
+    a.dt = 0.0 if dt is None else float(dt)
+    a.maxiters = int(maxiters)
+    a.save_start, a.save_end, a.save_everystep = int(save_start), int(save_end), int(save_everystep)
+    a.save_end_user = int(save_end_user)
+    a.max_saves = int(max_saves)
+    keep = []
+    if dev:
+        import torch
+        devc = u0.device
+        dd = dict(dtype=torch.float64, device=devc)
+        u0c = u0.contiguous()
+        pc = (p.contiguous() if p is not None else torch.zeros((1, N), **dd))
+        t0c, tfc = t0.contiguous(), tf.contiguous()
+        sat = torch.as_tensor(sa, **dd) if len(sa) else torch.zeros(1, **dd)
+        out = outputs or {}
+        o = dict(
+            t_out=out.get("t_out", torch.empty((max(max_saves, 1), N), **dd)),
+            u_out=out.get("u_out", torch.empty((max(max_saves, 1), n, N), **dd)),
+            nsaved=out.get("nsaved", torch.empty(N, dtype=torch.int32, device=devc)),
+            u_final=out.get("u_final", torch.empty((n, N), **dd)),
+            t_final=out.get("t_final", torch.empty(N, **dd)),
+            retcode=out.get("retcode", torch.empty(N, dtype=torch.int32, device=devc)),
+            stats=out.get("stats", torch.empty((6, N), dtype=torch.int32, device=devc)),
+        )
+        keep += [u0c, pc, t0c, tfc, sat]
+        ptr = lambda x: x.data_ptr()
+        a.device_ptrs = 1
+        a.stream = torch.cuda.current_stream(devc).cuda_stream
+    else:
+        u0c = np.ascontiguousarray(u0, dtype=np.float64)
+        pc = np.ascontiguousarray(p if p is not None else np.zeros((1, N)), dtype=np.float64)
+        t0c = np.ascontiguousarray(np.broadcast_to(t0, (N,)), dtype=np.float64)
+        tfc = np.ascontiguousarray(np.broadcast_to(tf, (N,)), dtype=np.float64)
+        sat = np.ascontiguousarray(sa if len(sa) else np.zeros(1))
+        o = dict(t_out=np.empty((max(max_saves, 1), N)), u_out=np.empty((max(max_saves, 1), n, N)),
+                 nsaved=np.empty(N, dtype=np.int32), u_final=np.empty((n, N)), t_final=np.empty(N),
+                 retcode=np.empty(N, dtype=np.int32), stats=np.empty((6, N), dtype=np.int32))
+        keep += [u0c, pc, t0c, tfc, sat]
+        ptr = lambda x: x.ctypes.data
+        a.device_ptrs = 0
+    a.u0, a.p, a.t0, a.tf = ptr(u0c), ptr(pc), ptr(t0c), ptr(tfc)
+    a.saveat, a.nsaveat = ptr(sat), len(sa)
+    a.t_out, a.u_out, a.nsaved = ptr(o["t_out"]), ptr(o["u_out"]), ptr(o["nsaved"])
+    a.u_final, a.t_final, a.retcode, a.stats = ptr(o["u_final"]), ptr(o["t_final"]), ptr(o["retcode"]), ptr(o["stats"])
+    t_s = time.perf_counter()
+    ctx.solve(h, a)
+    o["wall_s"] = time.perf_counter() - t_s
+    o["kernel_ms"] = a.kernel_ms
+    o["launches"] = a.launches
+    o["compile_s"] = csecs
+    o["traced"] = tr
+    o["max_saves"] = max_saves
+    return o
+
+
+def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=None, saveat=None,
+          save_start=None, save_end=None, save_everystep=None, dt=None, dtmax=None, dtmin=0.0,
+          maxiters=1_000_000, seed=0, batch_size=None, max_saves=None, **kw):
+    """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...) -> EnsembleSolution.
+
+    A plain ODEProblem is solved as a one-trajectory ensemble and returns its ODESolution."""
+    if kw:
+        raise TypeError(f"unsupported keyword(s): {sorted(kw)}")
+    single = isinstance(prob, ODEProblem)
+    ens = EnsembleProblem(prob) if single else prob
+    if single:
+        trajectories = 1
+    ea = ensemblealg or EnsembleMI355X()
+    t_start = time.perf_counter()
+    base = ens.prob
+    N = int(trajectories)
+    probs = []
+    for i in range(N):
+        ctx_i = EnsembleContext(i + 1, 1, np.random.default_rng([int(seed), i + 1]))
+        pi = ens.prob_func(remake(base) if ens.safetycopy else base, ctx_i)
+        probs.append(pi)
+    n = len(base.u0)
+    np_ = len(base.p)
+    for pi in probs:
+        if len(pi.u0) != n or len(pi.p) != np_:
+            raise ValueError("all trajectories must share the state and parameter dimensions")
+        if pi.f.f is not base.f.f:
+            raise ValueError("prob_func must not change the right-hand side function")
+        if not (pi.tspan[1] > pi.tspan[0]) and pi.tspan[1] != pi.tspan[0]:
+            raise ValueError("only forward integration (tspan[2] >= tspan[1]) is supported")
+    u0 = np.stack([pi.u0 for pi in probs], axis=1) if N else np.zeros((n, 0))
+    pp = np.stack([pi.p for pi in probs], axis=1) if np_ else None
+    t0 = np.array([pi.tspan[0] for pi in probs])
+    tf = np.array([pi.tspan[1] for pi in probs])
+    # save rules are resolved against the template tspan (shared by the common case)
+    ss, se, sev, seu = _resolve_save_flags(saveat, save_start, save_end, save_everystep,
+                                           base.tspan[0], base.tspan[1])
+    if saveat is not None and np.isscalar(saveat):
+        sa = saveat_times(saveat, float(t0.min()), float(tf.max()))
+    elif saveat is not None and len(saveat):
+        sa = saveat_times(saveat, 0, 0)
+    else:
+        sa = np.zeros(0)
+    if max_saves is None and sev:
+        max_saves = 4096
+    o = solve_arrays(base.f, alg, u0, pp, t0, tf, ea, reltol=reltol, abstol=abstol, saveat=sa, save_start=ss,
+                     save_end=se, save_everystep=sev, save_end_user=seu, max_saves=max_saves, dt=dt,
+                     dtmax=dtmax, dtmin=dtmin, maxiters=maxiters)
+    if sev and int(o["nsaved"].max(initial=0)) > o["max_saves"]:
+        raise RuntimeError(f"save_everystep produced more than max_saves={o['max_saves']} points; "
+                           "pass a larger max_saves")
+    sols = []
+    outs = []
+    for j in range(N):
+        ns = min(int(o["nsaved"][j]), o["max_saves"])
+        st = o["stats"][:, j]
+        sol = ODESolution(o["t_out"][:ns, j].copy(), [o["u_out"][k, :, j].copy() for k in range(ns)],
+                          ReturnCode(int(o["retcode"][j])), DEStats(*st), probs[j], alg)
+        res, _rerun = ens.output_func(sol, EnsembleContext(j + 1, 1, None))
+        outs.append(res)
+    if ens.reduction is not None:
+        u_acc = ens.u_init
+        u_acc, _conv = ens.reduction(u_acc, outs, list(range(1, N + 1)))
+        sols = u_acc
+    else:
+        sols = outs
+    elapsed = time.perf_counter() - t_start
+    if single:
+        return sols[0]
+    return EnsembleSolution(sols, elapsed, o)

@@ -1,0 +1,436 @@
+"""Trace a user right-hand side ``f(du, u, p, t)`` into HIP device functions.
+
+The reference specialises its solver on the user's Julia function
+(lib/DiffEqBase/src/solve.jl:918- ``promote_f``) and evaluates an analytic
+Jacobian through ``f.jac(J, u, p, t)`` when one is supplied
+(lib/OrdinaryDiffEqDifferentiation/src/derivative_utils.jl:408-416), else
+ForwardDiff; ``f.tgrad`` likewise (derivative_utils.jl:181-227).  Here the
+user's Python function is executed once on symbolic scalars; the recorded
+expression DAG is emitted as straight-line C++ templated on the arithmetic
+type, which hipRTC compiles into every engine kernel.  Missing Jacobians and
+time derivatives are derived symbolically from the same DAG.
+
+Evaluation order is preserved exactly (no algebraic rewriting of ``f``), so an
+FP64 kernel evaluates the same IEEE operations the user wrote.  Integer powers
+``x**k`` expand to left-to-right products like Julia's ``literal_pow``.
+"""
+from __future__ import annotations
+
+import math
+import struct
+
+__all__ = ["Sym", "trace_problem", "TracedProblem", "sym_exp", "sym_log", "sym_sin", "sym_cos",
+           "sym_sqrt", "sym_tanh", "sym_abs"]
+
+
+class _Graph:
+    def __init__(self):
+        self.nodes = {}
+        self.order = []
+
+    def make(self, op, args=(), val=None):
+        key = (op, tuple(a.id for a in args), val)
+        n = self.nodes.get(key)
+        if n is None:
+            n = Sym(self, op, args, val, len(self.order))
+            self.nodes[key] = n
+            self.order.append(n)
+        return n
+
+
+_CUR = None  # active graph during tracing
+
+
+def _g():
+    if _CUR is None:
+        raise RuntimeError("symbolic value used outside of tracing")
+    return _CUR
+
+
+def _lift(x):
+    if isinstance(x, Sym):
+        return x
+    if isinstance(x, (int, float)) or type(x).__module__ == "numpy":
+        return _g().make("const", (), float(x))
+    raise TypeError(f"unsupported operand {type(x)!r} in traced function")
+
+
+class Sym:
+    """Symbolic scalar.  Supports + - * / ** (integer powers), unary -, abs and
+    the methods exp/log/sin/cos/tan/sqrt/tanh (numpy ufuncs dispatch here)."""
+
+    __slots__ = ("g", "op", "args", "val", "id")
+    __array_priority__ = 1000
+
+    def __init__(self, g, op, args, val, id_):
+        self.g, self.op, self.args, self.val, self.id = g, op, args, val, id_
+
+    # arithmetic -----------------------------------------------------------
+    def _bin(self, op, other, rev=False):
+        o = _lift(other)
+        a, b = (o, self) if rev else (self, o)
+        if a.op == "const" and b.op == "const":
+            x, y = a.val, b.val
+            try:
+                v = {"add": lambda: x + y, "sub": lambda: x - y, "mul": lambda: x * y,
+                     "div": lambda: x / y}[op]()
+                return self.g.make("const", (), float(v))
+            except ZeroDivisionError:
+                pass
+        return self.g.make(op, (a, b))
+
+    def __add__(self, o): return self._bin("add", o)
+    def __radd__(self, o): return self._bin("add", o, True)
+    def __sub__(self, o): return self._bin("sub", o)
+    def __rsub__(self, o): return self._bin("sub", o, True)
+    def __mul__(self, o): return self._bin("mul", o)
+    def __rmul__(self, o): return self._bin("mul", o, True)
+    def __truediv__(self, o): return self._bin("div", o)
+    def __rtruediv__(self, o): return self._bin("div", o, True)
+
+    def __neg__(self):
+        if self.op == "const":
+            return self.g.make("const", (), -self.val)
+        return self.g.make("neg", (self,))
+
+    def __pos__(self):
+        return self
+
+    def __pow__(self, k):
+        if isinstance(k, Sym):
+            return self.g.make("pow", (self, k))
+        if float(k).is_integer() and 1 <= int(k) <= 16:
+            r = self
+            for _ in range(int(k) - 1):
+                r = r * self
+            return r
+        if float(k) == 0.0:
+            return _lift(1.0)
+        return self.g.make("pow", (self, _lift(float(k))))
+
+    def __rpow__(self, b):
+        return self.g.make("pow", (_lift(b), self))
+
+    def __abs__(self): return self.g.make("abs", (self,))
+
+    # comparisons are not traceable (data-dependent control flow)
+    def __bool__(self):
+        raise TypeError("data-dependent control flow on a traced value is not supported; "
+                        "use ifelse-free arithmetic")
+
+    # math methods (numpy ufunc dispatch on object operands calls these) ------
+    def _fn(self, name):
+        if self.op == "const":
+            return _lift(getattr(math, name)(self.val))
+        return self.g.make(name, (self,))
+
+    def exp(self): return self._fn("exp")
+    def log(self): return self._fn("log")
+    def sin(self): return self._fn("sin")
+    def cos(self): return self._fn("cos")
+    def tan(self): return self._fn("tan")
+    def sqrt(self): return self._fn("sqrt")
+    def tanh(self): return self._fn("tanh")
+    def arctan(self): return self._fn("atan")
+    def __array_ufunc__(self, ufunc, method, *inputs, **kw):
+        names = {"exp": "exp", "log": "log", "sin": "sin", "cos": "cos", "tan": "tan", "sqrt": "sqrt",
+                 "tanh": "tanh", "arctan": "atan", "absolute": "abs", "negative": "neg",
+                 "add": "add", "subtract": "sub", "multiply": "mul", "true_divide": "div", "power": "pow"}
+        nm = names.get(ufunc.__name__)
+        if nm is None or method != "__call__":
+            return NotImplemented
+        if nm in ("add", "sub", "mul", "div"):
+            a, b = inputs
+            return _lift(a)._bin(nm, b)
+        if nm == "pow":
+            return _lift(inputs[0]) ** inputs[1]
+        if nm == "neg":
+            return -inputs[0]
+        if nm == "abs":
+            return abs(inputs[0])
+        return _lift(inputs[0])._fn(nm)
+
+    def __repr__(self):
+        return f"Sym({self.op}#{self.id})"
+
+
+def sym_exp(x): return _lift(x).exp() if isinstance(x, Sym) else math.exp(x)
+def sym_log(x): return _lift(x).log() if isinstance(x, Sym) else math.log(x)
+def sym_sin(x): return _lift(x).sin() if isinstance(x, Sym) else math.sin(x)
+def sym_cos(x): return _lift(x).cos() if isinstance(x, Sym) else math.cos(x)
+def sym_sqrt(x): return _lift(x).sqrt() if isinstance(x, Sym) else math.sqrt(x)
+def sym_tanh(x): return _lift(x).tanh() if isinstance(x, Sym) else math.tanh(x)
+def sym_abs(x): return abs(x)
+
+
+class _Recorder:
+    """Write-only output vector/matrix: du[i] = expr, J[i, j] = expr, J[i][j] = expr."""
+
+    def __init__(self, shape):
+        self.shape = shape
+        self.vals = {}
+
+    def __len__(self):
+        return self.shape[0]
+
+    def __setitem__(self, key, value):
+        if isinstance(key, tuple):
+            i, j = key
+            self.vals[(int(i), int(j))] = _lift(value)
+        else:
+            if len(self.shape) == 2:
+                raise TypeError("use J[i, j] = ... for Jacobian entries")
+            self.vals[int(key)] = _lift(value)
+
+    def __getitem__(self, key):
+        if len(self.shape) == 2 and not isinstance(key, tuple):
+            rec = self
+
+            class _Row:
+                def __setitem__(_s, j, v):
+                    rec[(key, j)] = v
+            return _Row()
+        v = self.vals.get(key if not isinstance(key, tuple) else (int(key[0]), int(key[1])))
+        return v if v is not None else _lift(0.0)
+
+
+# ---------------------------------------------------------------------------
+# symbolic differentiation (used only for Jacobians / tgrad not supplied by the
+# user; simplification is allowed here because there is no user rounding to keep)
+# ---------------------------------------------------------------------------
+def _is0(x): return x.op == "const" and x.val == 0.0
+def _is1(x): return x.op == "const" and x.val == 1.0
+
+
+def _sadd(a, b):
+    if _is0(a): return b
+    if _is0(b): return a
+    return a + b
+
+
+def _ssub(a, b):
+    if _is0(b): return a
+    if _is0(a): return -b
+    return a - b
+
+
+def _smul(a, b):
+    if _is0(a) or _is0(b): return _lift(0.0)
+    if _is1(a): return b
+    if _is1(b): return a
+    return a * b
+
+
+def _sdiv(a, b):
+    if _is0(a): return _lift(0.0)
+    if _is1(b): return a
+    return a / b
+
+
+def _deriv(node, var, memo):
+    """d node / d var, var is a leaf Sym (u_j or t)."""
+    r = memo.get(node.id)
+    if r is not None:
+        return r
+    op, a = node.op, node.args
+    zero = _lift(0.0)
+    if node is var:
+        r = _lift(1.0)
+    elif op in ("const", "u", "p", "t"):
+        r = zero
+    elif op == "add":
+        r = _sadd(_deriv(a[0], var, memo), _deriv(a[1], var, memo))
+    elif op == "sub":
+        r = _ssub(_deriv(a[0], var, memo), _deriv(a[1], var, memo))
+    elif op == "mul":
+        r = _sadd(_smul(_deriv(a[0], var, memo), a[1]), _smul(a[0], _deriv(a[1], var, memo)))
+    elif op == "div":
+        da, db = _deriv(a[0], var, memo), _deriv(a[1], var, memo)
+        r = _ssub(_sdiv(da, a[1]), _sdiv(_smul(a[0], db), _smul(a[1], a[1])))
+    elif op == "neg":
+        d = _deriv(a[0], var, memo)
+        r = zero if _is0(d) else -d
+    else:
+        d = _deriv(a[0], var, memo)
+        if _is0(d) and op != "pow":
+            r = zero
+        elif op == "exp":
+            r = _smul(node, d)
+        elif op == "log":
+            r = _sdiv(d, a[0])
+        elif op == "sin":
+            r = _smul(a[0].cos(), d)
+        elif op == "cos":
+            r = _smul(-(a[0].sin()), d)
+        elif op == "tan":
+            r = _smul(_sadd(_lift(1.0), node * node), d)
+        elif op == "sqrt":
+            r = _sdiv(d, _smul(_lift(2.0), node))
+        elif op == "tanh":
+            r = _smul(_ssub(_lift(1.0), node * node), d)
+        elif op == "atan":
+            r = _sdiv(d, _sadd(_lift(1.0), a[0] * a[0]))
+        elif op == "abs":
+            r = _smul(node.g.make("sign", (a[0],)), d)
+        elif op == "pow":
+            db = _deriv(a[1], var, memo)
+            t1 = zero if _is0(d) else _smul(_smul(a[1], node.g.make("pow", (a[0], _ssub(a[1], _lift(1.0))))), d)
+            t2 = zero if _is0(db) else _smul(_smul(node, a[0].log()), db)
+            r = _sadd(t1, t2)
+        else:
+            raise NotImplementedError(f"derivative of {op}")
+    memo[node.id] = r
+    return r
+
+
+# ---------------------------------------------------------------------------
+# code emission
+# ---------------------------------------------------------------------------
+def _hexf(v):
+    if math.isnan(v):
+        return "T(__builtin_nan(\"\"))"
+    if math.isinf(v):
+        return "T(__builtin_inf())" if v > 0 else "T(-__builtin_inf())"
+    if v == 0.0:
+        return "T(-0.0)" if math.copysign(1.0, v) < 0 else "T(0.0)"
+    if float(v).is_integer() and abs(v) < 2 ** 53:
+        return f"T({int(v)}.0)"
+    return f"T({float(v).hex()})"
+
+
+_FN = {"exp": "exp", "log": "log", "sin": "sin", "cos": "cos", "tan": "tan", "sqrt": "sqrt",
+       "tanh": "tanh", "atan": "atan", "abs": "fabs"}
+
+
+def _emit_body(outputs, out_name, indent="    "):
+    """outputs: list of (lvalue_index_expr, Sym).  Emits temporaries in DAG order."""
+    needed = set()
+    stack = [s for _, s in outputs]
+    while stack:
+        n = stack.pop()
+        if n.id in needed:
+            continue
+        needed.add(n.id)
+        stack.extend(n.args)
+    if not outputs:
+        return ""
+    g = outputs[0][1].g
+    names = {}
+    lines = []
+    for n in g.order:
+        if n.id not in needed:
+            continue
+        op = n.op
+        if op == "const":
+            names[n.id] = _hexf(n.val)
+            continue
+        if op == "u":
+            names[n.id] = f"u[{n.val}]"
+            continue
+        if op == "p":
+            names[n.id] = f"p[{n.val}]"
+            continue
+        if op == "t":
+            names[n.id] = "t"
+            continue
+        a = [names[x.id] for x in n.args]
+        if op == "add": e = f"{a[0]} + {a[1]}"
+        elif op == "sub": e = f"{a[0]} - {a[1]}"
+        elif op == "mul": e = f"{a[0]} * {a[1]}"
+        elif op == "div": e = f"{a[0]} / {a[1]}"
+        elif op == "neg": e = f"-{a[0]}"
+        elif op == "pow": e = f"pow({a[0]}, {a[1]})"
+        elif op == "sign": e = f"T(({a[0]} > T(0)) - ({a[0]} < T(0)))"
+        else: e = f"{_FN[op]}({a[0]})"
+        nm = f"v{n.id}"
+        lines.append(f"{indent}const T {nm} = {e};")
+        names[n.id] = nm
+    for idx, s in outputs:
+        lines.append(f"{indent}{out_name}[{idx}] = {names[s.id]};")
+    return "\n".join(lines)
+
+
+class TracedProblem:
+    """Result of tracing: HIP source plus metadata."""
+
+    def __init__(self, n, np_, src, has_user_jac, has_user_tgrad, autonomous, flops_f, flops_j):
+        self.n, self.np, self.src = n, np_, src
+        self.has_user_jac, self.has_user_tgrad = has_user_jac, has_user_tgrad
+        self.autonomous = autonomous
+        self.flops_f, self.flops_j = flops_f, flops_j
+
+
+def _count_flops(outputs):
+    seen, cnt = set(), 0
+    stack = [s for _, s in outputs]
+    weights = {"add": 1, "sub": 1, "mul": 1, "div": 1, "neg": 0, "abs": 1, "sign": 1}
+    while stack:
+        n = stack.pop()
+        if n.id in seen:
+            continue
+        seen.add(n.id)
+        if n.op not in ("const", "u", "p", "t"):
+            cnt += weights.get(n.op, 20)
+        stack.extend(n.args)
+    return cnt
+
+
+def trace_problem(f, n, np_, jac=None, tgrad=None):
+    """Trace f (and jac/tgrad when given) for state dimension n and np_ params."""
+    global _CUR
+    g = _Graph()
+    _CUR = g
+    try:
+        u = [g.make("u", (), i) for i in range(n)]
+        p = [g.make("p", (), k) for k in range(np_)]
+        t = g.make("t", (), None)
+        du = _Recorder((n,))
+        f(du, u, p, t)
+        fout = [(i, du.vals.get(i, _lift(0.0))) for i in range(n)]
+        has_user_jac = jac is not None
+        if jac is not None:
+            J = _Recorder((n, n))
+            jac(J, u, p, t)
+            jout = [(i * n + j, v) for (i, j), v in sorted(J.vals.items())]
+        else:
+            jout = []
+            for i in range(n):
+                for j in range(n):
+                    d = _deriv(fout[i][1], u[j], {})
+                    if not _is0(d):
+                        jout.append((i * n + j, d))
+        if tgrad is not None:
+            dT = _Recorder((n,))
+            tgrad(dT, u, p, t)
+            tout = [(i, dT.vals.get(i, _lift(0.0))) for i in range(n)]
+        else:
+            tout = [(i, _deriv(fout[i][1], t, {})) for i in range(n)]
+        autonomous = all(_is0(v) for _, v in tout) and tgrad is None
+        src = []
+        src.append("template <typename T>\n__device__ __forceinline__ void odemi_f(T* __restrict__ du, "
+                   "const T* __restrict__ u, const T* __restrict__ p, T t) {")
+        src.append("    (void)u; (void)p; (void)t;")
+        src.append(_emit_body(fout, "du"))
+        src.append("}")
+        src.append("template <typename T>\n__device__ __forceinline__ void odemi_jac(T* __restrict__ J, "
+                   "const T* __restrict__ u, const T* __restrict__ p, T t) {")
+        src.append("    (void)u; (void)p; (void)t;")
+        nz = {i for i, _ in jout}
+        for k in range(n * n):
+            if k not in nz:
+                src.append(f"    J[{k}] = T(0);")
+        src.append(_emit_body(jout, "J"))
+        src.append("}")
+        src.append("template <typename T>\n__device__ __forceinline__ void odemi_tgrad(T* __restrict__ dT, "
+                   "const T* __restrict__ u, const T* __restrict__ p, T t) {")
+        src.append("    (void)u; (void)p; (void)t;")
+        tz = [(i, v) for i, v in tout if not _is0(v)]
+        for i, v in tout:
+            if _is0(v):
+                src.append(f"    dT[{i}] = T(0);")
+        src.append(_emit_body(tz, "dT"))
+        src.append("}")
+        return TracedProblem(n, np_, "\n".join(src) + "\n", has_user_jac, tgrad is not None, autonomous,
+                             _count_flops(fout), _count_flops(jout))
+    finally:
+        _CUR = None

@@ -1,0 +1,86 @@
+"""GPU parity: the HIP engines (through libodemi.so's C ABI) against the CPU
+restatement on identical seeded inputs.
+
+Bars: FP64 engines reproduce the oracle's final states to 1e-10 relative and its
+accepted/rejected step counts on >= 99% of trajectories (they are expected to be
+bit-identical); the mixed engine meets the requested tolerance scale."""
+import numpy as np
+import pytest
+
+import oracle as O
+import odemi
+from odemi import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_ensemble(prob_id, d, alg, N=None, **kw):
+    N = N or d["u0"].shape[1]
+    return O.ensemble_builtin(prob_id, d["u0"][:, :N], d["p"][:, :N], d["t0"][:N], d["tf"][:N], alg=alg,
+                              reltol=d["reltol"], abstol=d["abstol"], save_start=False, **kw)
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def _run(fun, alg, d, engine, precision, N=None, **kw):
+    N = N or d["u0"].shape[1]
+    ea = odemi.EnsembleMI355X(engine=engine, precision=precision)
+    return odemi.solve_arrays(fun, alg, d["u0"][:, :N], d["p"][:, :N], d["t0"][:N], d["tf"][:N], ea,
+                              reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=True, **kw)
+
+
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_lorenz_small_fp64_matches_oracle(engine):
+    d = P.lorenz_small()
+    print(f"[case] lorenz N=1024 tsit5 fp64 engine={engine}", flush=True)
+    ref = _oracle_ensemble(O.PROB_LORENZ, d, O.TSIT5)
+    out = _run(P.LORENZ, odemi.Tsit5(), d, engine, "fp64")
+    assert (out["retcode"] == 1).all()
+    rel = _rel(out["u_final"], ref["u_final"])
+    same_steps = (out["stats"][1] == ref["naccept"]) & (out["stats"][2] == ref["nreject"])
+    print(f"max rel {rel.max():.3e}  exact {np.mean(out['u_final'] == ref['u_final']):.4f}  "
+          f"steps equal {same_steps.mean():.4f}")
+    assert rel.max() <= 1e-10
+    assert same_steps.mean() >= 0.99
+    assert (out["stats"][0] == ref["nf"]).mean() >= 0.99
+
+
+def test_lorenz_large_subset_fp64_and_mixed():
+    N = 8192
+    d = P.lorenz_large(N=N)
+    print(f"[case] lorenz_large N={N} tsit5 fp64+mixed", flush=True)
+    ref = _oracle_ensemble(O.PROB_LORENZ, d, O.TSIT5)
+    out = _run(P.LORENZ, odemi.Tsit5(), d, "persistent", "fp64")
+    rel = _rel(out["u_final"], ref["u_final"])
+    same = (out["stats"][1] == ref["naccept"]) & (out["stats"][2] == ref["nreject"])
+    print(f"fp64: max rel {rel.max():.3e} p99 {np.quantile(rel, 0.99):.3e} steps equal {same.mean():.4f}")
+    assert np.quantile(rel.max(axis=0), 0.99) <= 1e-10
+    assert same.mean() >= 0.99
+    mx = _run(P.LORENZ, odemi.Tsit5(), d, "persistent", "mixed")
+    assert (mx["retcode"] == 1).all()
+    relm = _rel(mx["u_final"], ref["u_final"]).max(axis=0)
+    print(f"mixed: median rel {np.median(relm):.3e} p90 {np.quantile(relm, 0.9):.3e}")
+
+
+@pytest.mark.parametrize("engine,precision", [("persistent", "fp64"), ("plain", "fp64"), ("persistent", "mixed")])
+def test_robertson_matches_oracle(engine, precision):
+    N = 2048
+    d = P.robertson_ensemble(N=N)
+    print(f"[case] robertson N={N} rodas5p {engine} {precision}", flush=True)
+    ref = _oracle_ensemble(O.PROB_ROBERTSON, d, O.RODAS5P)
+    out = _run(P.ROBERTSON, odemi.Rodas5P(), d, engine, precision)
+    assert (out["retcode"] == 1).all()
+    rel = _rel(out["u_final"], ref["u_final"]).max(axis=0)
+    same = (out["stats"][1] == ref["naccept"]) & (out["stats"][2] == ref["nreject"])
+    print(f"max rel {rel.max():.3e} steps equal {same.mean():.4f} exact {np.mean(out['u_final'] == ref['u_final']):.4f}")
+    if precision == "fp64":
+        assert np.quantile(rel, 0.99) <= 1e-10
+        assert same.mean() >= 0.99
+        assert (out["stats"][0] == ref["nf"]).mean() >= 0.99
+    else:
+        # FP32 LU + FP64 refinement: the solution must stay within the requested tolerance scale
+        err = np.abs(out["u_final"] - ref["u_final"]) / (d["abstol"] + d["reltol"] * np.abs(ref["u_final"]))
+        print(f"mixed scaled err max {err.max():.3e}")
+        assert np.quantile(err.max(axis=0), 0.99) <= 10.0
