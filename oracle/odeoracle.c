@@ -31,8 +31,9 @@
  * Third-party arithmetic restated (not vendored in the reference tree):
  *   FastPower.jl 1.1.x `fastpower` (called at controllers.jl:815-816): the
  *   published fast log2 of Goldberg (rational fit, "line 8 of table 2"),
- *   followed by exp2 in Float32; here exp2 is evaluated in double and rounded
- *   to Float32 (correctly rounded Float32 exp2).
+ *   followed by exp2 in Float32; here exp2 is evaluated in double (oo_det_exp)
+ *   and rounded to Float32.  initdt's 10^x and log10 use the same oo_det_log /
+ *   oo_det_exp, so the GPU engine can reproduce every bit.
  *   LinearSolve.jl default dense LU: textbook right-looking LU with partial
  *   pivoting (first maximum on ties), multipliers scaled by the reciprocal of
  *   the pivot, trailing update a -= l*u as fma.
@@ -74,6 +75,57 @@ static inline double jl_max(double x, double y) {
 static inline double max_fast(double x, double y) { return y > x ? y : x; }
 
 /* ------------------------------------------------------------------------ */
+/* Deterministic log / exp (fdlibm-style reduction + minimax polynomials).   */
+/* The GPU engine evaluates the same algorithm, so initdt's 10^x / log10 and */
+/* fastpower's exp2 agree bit for bit between the two.  Accuracy ~1 ulp.     */
+/* ------------------------------------------------------------------------ */
+static const double DL_Lg1 = 6.666666666666735130e-01, DL_Lg2 = 3.999999999940941908e-01,
+                    DL_Lg3 = 2.857142874366239149e-01, DL_Lg4 = 2.222219843214978396e-01,
+                    DL_Lg5 = 1.818357216161805012e-01, DL_Lg6 = 1.531383769920937332e-01,
+                    DL_Lg7 = 1.479819860511658591e-01;
+static const double DL_ln2_hi = 6.93147180369123816490e-01, DL_ln2_lo = 1.90821492927058770002e-10;
+static const double DE_P1 = 1.66666666666666019037e-01, DE_P2 = -2.77777777770155933842e-03,
+                    DE_P3 = 6.61375632143793436117e-05, DE_P4 = -1.65339022054652515390e-06,
+                    DE_P5 = 4.13813679705723846039e-08, DE_invln2 = 1.44269504088896338700e+00;
+static const double D_LN10 = 2.302585092994046, D_LN2 = 0.6931471805599453;
+
+double oo_det_log(double x) {
+    if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
+    if (isinf(x)) return x;
+    int e = 0;
+    if (x < DBL_MIN) { x *= 18014398509481984.0; e = -54; }
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    e += (int)((b >> 52) & 0x7ff) - 1023;
+    b = (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+    double m;
+    memcpy(&m, &b, 8);
+    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s, w = z * z;
+    double t1 = w * (DL_Lg2 + w * (DL_Lg4 + w * DL_Lg6));
+    double t2 = z * (DL_Lg1 + w * (DL_Lg3 + w * (DL_Lg5 + w * DL_Lg7)));
+    double R = t2 + t1;
+    double hfsq = 0.5 * f * f;
+    double dk = (double)e;
+    return dk * DL_ln2_hi - ((hfsq - (s * (hfsq + R) + dk * DL_ln2_lo)) - f);
+}
+
+double oo_det_exp(double x) {
+    if (isnan(x)) return x;
+    if (x > 709.782712893384) return INFINITY;
+    if (x < -745.1332191019411) return 0.0;
+    double k = rint(x * DE_invln2);
+    double hi = x - k * DL_ln2_hi, lo = k * DL_ln2_lo;
+    double r = hi - lo;
+    double t = r * r;
+    double c = r - t * (DE_P1 + t * (DE_P2 + t * (DE_P3 + t * (DE_P4 + t * DE_P5))));
+    double y = 1.0 - ((lo - (r * c) / (2.0 - c)) - hi);
+    return ldexp(y, (int)k);
+}
+
+/* ------------------------------------------------------------------------ */
 /* FastPower.jl restatement (controllers.jl:815-816 call sites)              */
 /* ------------------------------------------------------------------------ */
 
@@ -108,7 +160,7 @@ double oo_fastpower(double x, double y) {
     if (isinf(x) && isinf(y)) return INFINITY;
     float lg = oo_fastlog2((float)x);
     float prod = (float)y * lg;
-    float r = (float)exp2((double)prod);
+    float r = (float)oo_det_exp((double)prod * D_LN2);
     return (double)r;
 }
 
@@ -370,7 +422,7 @@ static double initdt(integ_t* I) {
     if (mx < 1e-15) /* `<= 1//10^15` evaluated exactly */
         dt1 = jl_max(1e-6, dt0 * 0.001);
     else
-        dt1 = pow(10.0, -(2.0 + log10(mx)) / 5.0);
+        dt1 = oo_det_exp(-(2.0 + oo_det_log(mx) / D_LN10) / 5.0 * D_LN10);
     return jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
 }
 

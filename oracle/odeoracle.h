@@ -85,6 +85,8 @@ double oo_ensemble_builtin(int prob, int64_t N, int nmax, const int* nvec,
 double oo_fastpower(double x, double y);
 float oo_fastlog2(float x);
 double oo_jl_eps(double x);
+double oo_det_log(double x);
+double oo_det_exp(double x);
 /* Dense LU with partial pivoting + solve, in place on A (row-major n x n). */
 int oo_lu_solve(int n, double* A, int* piv, double* b);
 /* Tableau access for the order-condition tests. */

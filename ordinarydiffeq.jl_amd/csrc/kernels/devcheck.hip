@@ -20,8 +20,8 @@ __device__ __forceinline__ void odemi_tgrad(T* __restrict__ dT, const T* __restr
     dT[0] = T(0); dT[1] = T(0); dT[2] = T(0);
 }
 #include "odemi_device.h"
-extern "C" __global__ void __launch_bounds__(256) k1(odemi::Params P) { odemi::tsit5_persistent<double, double, 3>(P); }
-extern "C" __global__ void __launch_bounds__(256) k2(odemi::Params P) { odemi::tsit5_persistent<double, float, 3>(P); }
+extern "C" __global__ void __launch_bounds__(256) k1(odemi::Params P) { odemi::tsit5_persistent<false, double, double, 3>(P); }
+extern "C" __global__ void __launch_bounds__(256) k2(odemi::Params P) { odemi::tsit5_persistent<false, double, float, 3>(P); }
 extern "C" __global__ void __launch_bounds__(256) k3(odemi::Params P) { odemi::tsit5_plain_init<3>(P); }
 extern "C" __global__ void __launch_bounds__(256) k4(odemi::Params P) { odemi::tsit5_plain_step<3>(P); }
 extern "C" __global__ void __launch_bounds__(256) k5(odemi::Params P) { odemi::rodas_persistent<double, 3>(P); }

@@ -52,7 +52,7 @@ struct Params {
     const double* p;          // [np][N]
     const double* t0;         // [N]
     const double* tf;         // [N]
-    const i64* order;         // work-queue order (permutation of 0..N-1) or null
+    const int* order;         // work-queue order (permutation of 0..N-1) or null
     double reltol, abstol;
     double dtmax;             // <= 0: tf - t0 per trajectory
     double dtmin;
@@ -108,9 +108,55 @@ __device__ __forceinline__ double jl_max(double x, double y) {
 template <typename T>
 __device__ __forceinline__ T max_fast(T x, T y) { return y > x ? y : x; }
 
+// Deterministic log / exp (fdlibm-style argument reduction + minimax
+// polynomials, ~1 ulp).  Used wherever the controller needs an elementary
+// function, so the CPU restatement (same algorithm) agrees bit for bit.
+__device__ __forceinline__ double det_log(double x) {
+    if (!(x > 0.0)) return x == 0.0 ? -__longlong_as_double(0x7ff0000000000000LL)
+                                    : __longlong_as_double(0x7ff8000000000000LL);
+    if (isinf(x)) return x;
+    int e = 0;
+    if (x < 2.2250738585072014e-308) { x *= 18014398509481984.0; e = -54; }
+    u64 b = (u64)__double_as_longlong(x);
+    e += (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((i64)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+    const double Lg1 = 6.666666666666735130e-01, Lg2 = 3.999999999940941908e-01,
+                 Lg3 = 2.857142874366239149e-01, Lg4 = 2.222219843214978396e-01,
+                 Lg5 = 1.818357216161805012e-01, Lg6 = 1.531383769920937332e-01,
+                 Lg7 = 1.479819860511658591e-01;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s, w = z * z;
+    double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+    double t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+    double R = t2 + t1;
+    double hfsq = 0.5 * f * f;
+    double dk = (double)e;
+    return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
+__device__ __forceinline__ double det_exp(double x) {
+    if (isnan(x)) return x;
+    if (x > 709.782712893384) return __longlong_as_double(0x7ff0000000000000LL);
+    if (x < -745.1332191019411) return 0.0;
+    const double P1 = 1.66666666666666019037e-01, P2 = -2.77777777770155933842e-03,
+                 P3 = 6.61375632143793436117e-05, P4 = -1.65339022054652515390e-06,
+                 P5 = 4.13813679705723846039e-08, invln2 = 1.44269504088896338700e+00;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    double k = rint(x * invln2);
+    double hi = x - k * ln2_hi, lo = k * ln2_lo;
+    double r = hi - lo;
+    double t = r * r;
+    double c = r - t * (P1 + t * (P2 + t * (P3 + t * (P4 + t * P5))));
+    double y = 1.0 - ((lo - (r * c) / (2.0 - c)) - hi);
+    return ldexp(y, (int)k);
+}
+constexpr double kLN10 = 2.302585092994046, kLN2 = 0.6931471805599453;
+
 // FastPower.jl 1.1.x `fastpower` (controllers.jl:815-816): Goldberg's rational
-// fast log2 in Float32, then exp2 of the Float32 product, correctly rounded to
-// Float32 (evaluated in double and rounded once).
+// fast log2 in Float32, then exp2 of the Float32 product evaluated in double
+// (det_exp) and rounded once to Float32.
 __device__ __forceinline__ float fastlog2f(float x) {
     unsigned ux1i = __float_as_uint(x);
     unsigned ex = (ux1i & 0x7F800000u) >> 23;
@@ -135,7 +181,7 @@ __device__ __forceinline__ double fastpower(double x, double y) {
     if (isinf(x) && isinf(y)) return __longlong_as_double(0x7ff0000000000000LL);
     float lg = fastlog2f((float)x);
     float prod = (float)y * lg;
-    return (double)(float)exp2((double)prod);
+    return (double)(float)det_exp((double)prod * kLN2);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,7 +264,7 @@ __constant__ double kRH[3][8] = {
 // Common per-trajectory controller / loop state (all FP64)
 // ---------------------------------------------------------------------------
 struct Ctl {
-    double t, dt, dtpropose, tprev, tstop_target, tf, dtmax, EEst, q11, errold;
+    double t, dt, dtpropose, tprev, tf, dtmax, EEst, q11, errold;
     int iter, success_iter, accept_step, next_step_tstop;
     int nf, naccept, nreject, njacs, nw, nsolve;
     int spos, send, nsaved, tf_in_saveat;
@@ -243,12 +289,8 @@ __device__ __forceinline__ void modify_dt_for_tstops(Ctl& c) {
     if (isfinite(c.tf) && isfinite(c.t)) tol = 100.0 * jl_eps(jl_max(fabs(c.t), fabs(c.tf)));
     double original_dt = fabs(c.dt);
     c.dtpropose = original_dt;
-    if (original_dt + tol < distance) {
-        c.next_step_tstop = 0;
-    } else {
-        c.next_step_tstop = 1;
-        c.tstop_target = c.tf;
-    }
+    // the only tstop is tf, so the tstop target is always c.tf
+    c.next_step_tstop = (original_dt + tol < distance) ? 0 : 1;
     c.dt = jl_min(original_dt, distance);
 }
 // integrator_utils.jl:1243-1256, utils.jl:90-94
@@ -383,7 +425,7 @@ __device__ double initdt_lane(const Ctl& c, const double* u0, const double* p, d
     double mx = jl_max(d1, d2);
     double dt1;
     if (mx < 1e-15) dt1 = jl_max(1e-6, dt0 * 0.001);
-    else dt1 = pow(10.0, -(2.0 + log10(mx)) / 5.0);
+    else dt1 = det_exp(-(2.0 + det_log(mx) / kLN10) / 5.0 * kLN10);
     return jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
 }
 
@@ -394,7 +436,7 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     c.dtmax = (P.dtmax > 0.0) ? P.dtmax : (tf - t0);
     c.EEst = 1.0; c.q11 = 1.0; c.errold = 1e-4;
     c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0;
-    c.tstop_target = 0.0; c.dt = 0.0; c.dtpropose = 0.0;
+    c.dt = 0.0; c.dtpropose = 0.0;
     c.nf = 0; c.naccept = 0; c.nreject = 0; c.njacs = 0; c.nw = 0; c.nsolve = 0;
     c.nsaved = 0; c.spos = 0; c.send = 0; c.tf_in_saveat = 0;
     if (P.nsaveat > 0) saveat_init(c, P, t0);
@@ -441,17 +483,16 @@ struct Tsit5Lane {
     S u[N], uprev[N];
     K k1[N], k2[N], k3[N], k4[N], k5[N], k6[N], k7[N];
     K p[ODEMI_NP];
-    double pd[ODEMI_NP];
 };
 
 template <typename S, typename K, int N>
 __device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
     ctl_init(L.c, P, j);
-    double u0d[N];
+    double u0d[N], pd[ODEMI_NP];
 #pragma unroll
     for (int i = 0; i < N; i++) { u0d[i] = P.u0[(i64)i * P.N + j]; L.u[i] = (S)u0d[i]; L.uprev[i] = L.u[i]; }
 #pragma unroll
-    for (int k = 0; k < ODEMI_NP; k++) { L.pd[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; L.p[k] = (K)L.pd[k]; }
+    for (int k = 0; k < ODEMI_NP; k++) { pd[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; L.p[k] = (K)pd[k]; }
     if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
     // initialize! : FSAL pre-start
     K uk[N];
@@ -462,7 +503,7 @@ __device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& 
     if (P.dt_user > 0.0) {
         L.c.dt = P.dt_user;
     } else {
-        L.c.dt = initdt_lane<N>(L.c, u0d, L.pd, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
+        L.c.dt = initdt_lane<N>(L.c, u0d, pd, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
         L.c.nf += 2;
     }
     L.c.dtpropose = L.c.dt;
@@ -473,15 +514,20 @@ __device__ __forceinline__ void tsit5_perform_step(Tsit5Lane<S, K, N>& L, const 
     typedef Tsit5Tab T;
     const double t = L.c.t, dt = L.c.dt;
     K tmp[N];
+    double ea[N];  // embedded error sum bt1*k1 + ... + bt7*k7, accumulated in FP64 as stages appear
     const K a = (K)(dt * T::a21);
 #pragma unroll
-    for (int i = 0; i < N; i++) tmp[i] = (K)fma((S)a, (S)L.k1[i], L.uprev[i]);
+    for (int i = 0; i < N; i++) {
+        tmp[i] = (K)fma((S)a, (S)L.k1[i], L.uprev[i]);
+        ea[i] = T::bt1 * (double)L.k1[i];
+    }
     odemi_f<K>(L.k2, tmp, L.p, (K)fma(T::c1, dt, t));
 #pragma unroll
     for (int i = 0; i < N; i++) {
         K acc = (K)T::a31 * L.k1[i];
         acc = fma((K)T::a32, L.k2[i], acc);
         tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+        ea[i] = fma(T::bt2, (double)L.k2[i], ea[i]);
     }
     odemi_f<K>(L.k3, tmp, L.p, (K)fma(T::c2, dt, t));
 #pragma unroll
@@ -490,6 +536,7 @@ __device__ __forceinline__ void tsit5_perform_step(Tsit5Lane<S, K, N>& L, const 
         acc = fma((K)T::a42, L.k2[i], acc);
         acc = fma((K)T::a43, L.k3[i], acc);
         tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+        ea[i] = fma(T::bt3, (double)L.k3[i], ea[i]);
     }
     odemi_f<K>(L.k4, tmp, L.p, (K)fma(T::c3, dt, t));
 #pragma unroll
@@ -499,6 +546,7 @@ __device__ __forceinline__ void tsit5_perform_step(Tsit5Lane<S, K, N>& L, const 
         acc = fma((K)T::a53, L.k3[i], acc);
         acc = fma((K)T::a54, L.k4[i], acc);
         tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+        ea[i] = fma(T::bt4, (double)L.k4[i], ea[i]);
     }
     odemi_f<K>(L.k5, tmp, L.p, (K)fma(T::c4, dt, t));
 #pragma unroll
@@ -509,6 +557,7 @@ __device__ __forceinline__ void tsit5_perform_step(Tsit5Lane<S, K, N>& L, const 
         acc = fma((K)T::a64, L.k4[i], acc);
         acc = fma((K)T::a65, L.k5[i], acc);
         tmp[i] = (K)fma((S)dt, (S)acc, L.uprev[i]);
+        ea[i] = fma(T::bt5, (double)L.k5[i], ea[i]);
     }
     odemi_f<K>(L.k6, tmp, L.p, (K)(t + dt));
 #pragma unroll
@@ -520,25 +569,15 @@ __device__ __forceinline__ void tsit5_perform_step(Tsit5Lane<S, K, N>& L, const 
         acc = fma((K)T::a75, L.k5[i], acc);
         acc = fma((K)T::a76, L.k6[i], acc);
         L.u[i] = fma((S)dt, (S)acc, L.uprev[i]);
+        ea[i] = fma(T::bt6, (double)L.k6[i], ea[i]);
     }
 #pragma unroll
     for (int i = 0; i < N; i++) tmp[i] = (K)L.u[i];
     odemi_f<K>(L.k7, tmp, L.p, (K)(t + dt));
     L.c.nf += 6;
-    // embedded error estimate, accumulated in FP64
-    double ut[N];
 #pragma unroll
-    for (int i = 0; i < N; i++) {
-        double acc = T::bt1 * (double)L.k1[i];
-        acc = fma(T::bt2, (double)L.k2[i], acc);
-        acc = fma(T::bt3, (double)L.k3[i], acc);
-        acc = fma(T::bt4, (double)L.k4[i], acc);
-        acc = fma(T::bt5, (double)L.k5[i], acc);
-        acc = fma(T::bt6, (double)L.k6[i], acc);
-        acc = fma(T::bt7, (double)L.k7[i], acc);
-        ut[i] = dt * acc;
-    }
-    L.c.EEst = residual_norm<N>(ut, L.uprev, L.u, P.abstol, P.reltol);
+    for (int i = 0; i < N; i++) ea[i] = dt * fma(T::bt7, (double)L.k7[i], ea[i]);
+    L.c.EEst = residual_norm<N>(ea, L.uprev, L.u, P.abstol, P.reltol);
 }
 
 // interpolants.jl:52-64 (@evalpoly Horner via muladd), FP64 evaluation.
@@ -588,7 +627,7 @@ __device__ __forceinline__ void tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Pa
 
 // One step attempt: loopheader! + check_error! + perform_step! + loopfooter!.
 // Returns 0 while running, otherwise the final retcode (lane finished).
-template <typename S, typename K, int N>
+template <bool SAVES, typename S, typename K, int N>
 __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
     const CtlConst kc = ctl_const(false);
     Ctl& c = L.c;
@@ -620,10 +659,10 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;
-        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tstop_target; }
+        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
         else c.t = ttmp;
         accept_controller(c, kc, q, P.dtmin);
-        if (P.nsaveat > 0 || P.save_everystep) tsit5_savevalues(L, P, j);
+        if (SAVES && (P.nsaveat > 0 || P.save_everystep)) tsit5_savevalues(L, P, j);
     } else {
         c.nreject++;
     }
@@ -646,7 +685,7 @@ __device__ __forceinline__ i64 wave_fetch(u64* head, bool need) {
     return (i64)(base + (u64)__popcll(below));
 }
 
-template <typename S, typename K, int N>
+template <bool SAVES, typename S, typename K, int N>
 __device__ void tsit5_persistent(const Params& P) {
     Tsit5Lane<S, K, N> L;
     bool active = false, exhausted = false;
@@ -665,7 +704,7 @@ __device__ void tsit5_persistent(const Params& P) {
         }
         if (__ballot(active) == 0) break;
         if (active) {
-            int rc = tsit5_attempt(L, P, j);
+            int rc = tsit5_attempt<SAVES>(L, P, j);
             if (rc) {
                 finish_lane<N>(L.c, P, j, L.u, rc);
                 active = false;
@@ -675,7 +714,7 @@ __device__ void tsit5_persistent(const Params& P) {
 }
 
 // Plain engine: one kernel launch per step attempt, state in HBM.
-// st_scalar rows: t dt dtpropose tprev tstop_target tf dtmax EEst q11 errold
+// st_scalar rows: t dt dtpropose tprev (unused) tf dtmax EEst q11 errold
 // st_int rows: iter success_iter accept_step next_step_tstop nf naccept nreject done
 // st_vec rows: u uprev k1 k7 (Tsit5)
 template <int N>
@@ -683,7 +722,7 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
     const double* s = P.st_scalar;
     const i64 M = P.N;
     c.t = s[0 * M + j]; c.dt = s[1 * M + j]; c.dtpropose = s[2 * M + j]; c.tprev = s[3 * M + j];
-    c.tstop_target = s[4 * M + j]; c.tf = s[5 * M + j]; c.dtmax = s[6 * M + j];
+    c.tf = s[5 * M + j]; c.dtmax = s[6 * M + j];
     c.EEst = s[7 * M + j]; c.q11 = s[8 * M + j]; c.errold = s[9 * M + j];
     const int* q = P.st_int;
     c.iter = q[0 * M + j]; c.success_iter = q[1 * M + j]; c.accept_step = q[2 * M + j];
@@ -702,7 +741,7 @@ __device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i
     double* s = P.st_scalar;
     const i64 M = P.N;
     s[0 * M + j] = c.t; s[1 * M + j] = c.dt; s[2 * M + j] = c.dtpropose; s[3 * M + j] = c.tprev;
-    s[4 * M + j] = c.tstop_target; s[5 * M + j] = c.tf; s[6 * M + j] = c.dtmax;
+    s[5 * M + j] = c.tf; s[6 * M + j] = c.dtmax;
     s[7 * M + j] = c.EEst; s[8 * M + j] = c.q11; s[9 * M + j] = c.errold;
     int* q = P.st_int;
     q[0 * M + j] = c.iter; q[1 * M + j] = c.success_iter; q[2 * M + j] = c.accept_step;
@@ -744,8 +783,8 @@ __device__ void tsit5_plain_step(const Params& P) {
         L.k7[i] = P.st_vec[((i64)3 * P.nmax + i) * M + j];
     }
 #pragma unroll
-    for (int k = 0; k < ODEMI_NP; k++) { L.pd[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; L.p[k] = L.pd[k]; }
-    int rc = tsit5_attempt(L, P, j);
+    for (int k = 0; k < ODEMI_NP; k++) L.p[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0;
+    int rc = tsit5_attempt<true>(L, P, j);
     if (rc) {
         finish_lane<N>(L.c, P, j, L.u, rc);
         plain_store_ctl<N>(L.c, P, j, 1);
@@ -1052,7 +1091,7 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;
-        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tstop_target; }
+        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
         else c.t = ttmp;
         accept_controller(c, kc, q, P.dtmin);
         if (P.nsaveat > 0 || P.save_everystep) rodas_savevalues(L, P, j);

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -37,7 +38,7 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int nmax, np;
     const int* nvec;
     const double *u0, *p, *t0, *tf;
-    const int64_t* order;
+    const int* order;
     double reltol, abstol, dtmax, dtmin, dt_user;
     int64_t maxiters;
     const double* saveat;
@@ -58,6 +59,17 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int refine_sweeps;
     int pad0;
 };
+
+// Longest-predicted-cost-first work order: key_j = (tf_j - t0_j) * n_j^3 as FP32.
+__global__ void __launch_bounds__(256) cost_keys_kernel(const double* t0, const double* tf, const int* nvec,
+                                                       int64_t N, float* keys, int* idx) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    double c = tf[j] - t0[j];
+    if (nvec) { double n = (double)nvec[j]; c *= n * n * n; }
+    keys[j] = (c > 0.0) ? (float)c : 0.0f;
+    idx[j] = (int)j;
+}
 
 uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL) {
     for (unsigned char c : s) { h ^= c; h *= 1099511628211ULL; }
@@ -102,6 +114,7 @@ struct odemi_rhs {
     hipModule_t mod = nullptr;
     int n = 0, np = 0, flags = 0;
     hipFunction_t tsit5_persist_f64 = nullptr, tsit5_persist_mixed = nullptr;
+    hipFunction_t tsit5_persist_f64_sv = nullptr, tsit5_persist_mixed_sv = nullptr;
     hipFunction_t tsit5_plain_init = nullptr, tsit5_plain_step = nullptr;
     hipFunction_t rodas_persist_f64 = nullptr, rodas_persist_mixed = nullptr;
     hipFunction_t rodas_plain_init = nullptr, rodas_plain_step = nullptr;
@@ -175,14 +188,22 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     std::ostringstream s;
     s << "#define ODEMI_N " << n << "\n#define ODEMI_NP " << (np > 0 ? np : 1) << "\n";
     s << "#define ODEMI_HAS_TGRAD " << ((flags & ODEMI_HAS_USER_TGRAD) ? 1 : 0) << "\n";
+    // minimum waves per SIMD for the persistent lane kernels (occupancy vs spills)
+    const char* mw = getenv("ODEMI_MIN_WAVES");
+    s << "#define ODEMI_MIN_WAVES " << ((mw && *mw) ? atoi(mw) : 4) << "\n";
     if (flags & ODEMI_COMPILE_WG) s << "#define ODEMI_WITH_WG 1\n";
     s << problem_src << "\n";
     s << kDeviceSource << "\n";
     if (flags & ODEMI_COMPILE_TSIT5) {
-        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_persist_f64(odemi::Params P)"
-             " { odemi::tsit5_persistent<double, double, ODEMI_N>(P); }\n";
-        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_persist_mixed(odemi::Params P)"
-             " { odemi::tsit5_persistent<double, float, ODEMI_N>(P); }\n";
+        // persistent kernels: final-state-only (no save path) and saving variants
+        const char* v[4][3] = {{"odemi_tsit5_persist_f64", "false", "double"},
+                               {"odemi_tsit5_persist_f64_sv", "true", "double"},
+                               {"odemi_tsit5_persist_mixed", "false", "float"},
+                               {"odemi_tsit5_persist_mixed_sv", "true", "float"}};
+        for (auto& k : v)
+            s << "extern \"C\" __global__ void __launch_bounds__(256, ODEMI_MIN_WAVES) " << k[0]
+              << "(odemi::Params P) { odemi::tsit5_persistent<" << k[1] << ", double, " << k[2]
+              << ", ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_init(odemi::Params P)"
              " { odemi::tsit5_plain_init<ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_step(odemi::Params P)"
@@ -309,6 +330,8 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
     if (flags & ODEMI_COMPILE_TSIT5) {
         get("odemi_tsit5_persist_f64", &r->tsit5_persist_f64);
         get("odemi_tsit5_persist_mixed", &r->tsit5_persist_mixed);
+        get("odemi_tsit5_persist_f64_sv", &r->tsit5_persist_f64_sv);
+        get("odemi_tsit5_persist_mixed_sv", &r->tsit5_persist_mixed_sv);
         get("odemi_tsit5_plain_init", &r->tsit5_plain_init);
         get("odemi_tsit5_plain_step", &r->tsit5_plain_step);
     }
@@ -389,6 +412,8 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     hipFunction_t fn_main = nullptr, fn_init = nullptr;
     if (!rodas) {
         if (plain) { fn_init = rhs->tsit5_plain_init; fn_main = rhs->tsit5_plain_step; }
+        else if (a->nsaveat > 0 || a->save_everystep)
+            fn_main = mixed ? rhs->tsit5_persist_mixed_sv : rhs->tsit5_persist_f64_sv;
         else fn_main = mixed ? rhs->tsit5_persist_mixed : rhs->tsit5_persist_f64;
     } else if (use_wg) {
         fn_main = mixed ? rhs->rodas_wg_mixed : rhs->rodas_wg_f64;
@@ -409,7 +434,13 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     size_t need = 0;
     auto add = [&](size_t b) { need += ((b + 255) & ~size_t(255)); };
     add(64);                                   // counters
-    if (a->sort_by_cost) add(N_ * 8);          // order
+    size_t sort_tmp = 0;
+    if (a->sort_by_cost && !plain) {
+        if (N > 0x7fffffffLL) { ctx->err = "N too large for the work-queue order"; return -2; }
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_tmp, (float*)nullptr, (float*)nullptr,
+                                                                 (int*)nullptr, (int*)nullptr, (int)N, 0, 32, s));
+        add(N_ * 16); add(sort_tmp + 1024);
+    }
     if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(12 * N_ * 4); }
     if (!dev) {
         add(N_ * nmax * 8); add(N_ * std::max(np, 1) * 8); add(N_ * 8); add(N_ * 8);
@@ -460,36 +491,18 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
             P.nvec = nv;
         }
     }
-    if (a->sort_by_cost) {
-        // Longest-processing-time-first: hand out trajectories by decreasing
-        // predicted cost (integration span, times n^3 for uneven Rosenbrock batches).
-        std::vector<double> h0(N_), h1(N_);
-        std::vector<int32_t> nv;
-        if (dev) {
-            HIPCHK(ctx, hipMemcpyAsync(h0.data(), a->t0, N_ * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(ctx, hipMemcpyAsync(h1.data(), a->tf, N_ * 8, hipMemcpyDeviceToHost, s));
-            if (a->nvec) {
-                nv.resize(N_);
-                HIPCHK(ctx, hipMemcpyAsync(nv.data(), a->nvec, N_ * 4, hipMemcpyDeviceToHost, s));
-            }
-            HIPCHK(ctx, hipStreamSynchronize(s));
-        } else {
-            memcpy(h0.data(), a->t0, N_ * 8);
-            memcpy(h1.data(), a->tf, N_ * 8);
-            if (a->nvec) nv.assign(a->nvec, a->nvec + N_);
-        }
-        std::vector<double> cost(N_);
-        for (size_t j = 0; j < N_; j++) {
-            cost[j] = h1[j] - h0[j];
-            if (!nv.empty()) cost[j] *= (double)nv[j] * nv[j] * nv[j];
-        }
-        std::vector<int64_t> ord(N_);
-        std::iota(ord.begin(), ord.end(), 0);
-        std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return cost[x] > cost[y]; });
-        int64_t* dord = (int64_t*)ar.take(N_ * 8);
-        HIPCHK(ctx, hipMemcpyAsync(dord, ord.data(), N_ * 8, hipMemcpyHostToDevice, s));
-        HIPCHK(ctx, hipStreamSynchronize(s));
-        P.order = dord;
+    if (a->sort_by_cost && !plain) {
+        float* keys = (float*)ar.take(N_ * 4);
+        float* keys2 = (float*)ar.take(N_ * 4);
+        int* idx = (int*)ar.take(N_ * 4);
+        int* ord = (int*)ar.take(N_ * 4);
+        void* tmp = ar.take(sort_tmp);
+        const int g = (int)((N + 255) / 256);
+        hipLaunchKernelGGL(cost_keys_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, keys, idx);
+        HIPCHK(ctx, hipGetLastError());
+        size_t tb = sort_tmp;
+        HIPCHK(ctx, hipcub::DeviceRadixSort::SortPairsDescending(tmp, tb, keys, keys2, idx, ord, (int)N, 0, 32, s));
+        P.order = ord;
     }
     if (plain) {
         P.st_scalar = (double*)ar.take(16 * N_ * 8);

@@ -45,6 +45,8 @@ def test_lorenz_small_fp64_matches_oracle(engine):
     assert rel.max() <= 1e-10
     assert same_steps.mean() >= 0.99
     assert (out["stats"][0] == ref["nf"]).mean() >= 0.99
+    # same operation order and the same elementary-function algorithm: bit-identical
+    assert np.mean(np.all(out["u_final"] == ref["u_final"], axis=0)) >= 0.99
 
 
 def test_lorenz_large_subset_fp64_and_mixed():
