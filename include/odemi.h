@@ -84,7 +84,7 @@ typedef struct {
     double reltol, abstol;    /* solve.jl defaults: 1e-3, 1e-6 */
     double dtmax;             /* <= 0: tf - t0 (solve.jl:153) */
     double dtmin;             /* 0 (solve.jl:152) */
-    double dt;                /* > 0: initial dt; 0: automatic (initdt.jl) */
+    double dt;                /* != 0: initial dt; 0: automatic (initdt.jl) */
     int64_t maxiters;         /* 1_000_000 (solve.jl:161) */
     const double* saveat;     /* sorted output times shared by all trajectories, or NULL */
     int nsaveat;

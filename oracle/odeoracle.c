@@ -358,7 +358,7 @@ typedef struct {
     /* controller constants (alg_utils.jl:407-879, controllers.jl:744-803) */
     double qmin, qmax, qmax_first, gamma, beta1, beta2, qoldinit, qsteady_min, qsteady_max;
     /* integrator scalars */
-    double t, dt, dtpropose, tprev, tstop_target, EEst, q11, errold;
+    double t, dt, dtpropose, tprev, tstop_target, EEst, q11, errold, dtcache;
     int64_t iter, success_iter;
     int accept_step, next_step_tstop;
     oo_result* res;
@@ -432,6 +432,12 @@ static void modify_dt_for_tstops(integ_t* I) {
     double tol = 0.0;
     if (isfinite(I->tf) && isfinite(I->t))
         tol = 100.0 * oo_jl_eps(jl_max(fabs(I->t), fabs(I->tf)));
+    if (!I->o->adaptive) {   /* dtchangeable branch: always try dtcache */
+        I->next_step_tstop = !(fabs(I->dtcache) + tol < distance);
+        if (I->next_step_tstop) I->tstop_target = I->tf;
+        I->dt = jl_min(fabs(I->dtcache), distance);
+        return;
+    }
     double original_dt = fabs(I->dt);
     I->dtpropose = original_dt;
     if (original_dt + tol < distance) {
@@ -488,10 +494,12 @@ static void calc_dt_propose(integ_t* I, double dtnew) {
 static int check_error(integ_t* I) {
     if (isnan(I->dt)) return OO_RC_DTNAN;
     if (I->iter > I->o->maxiters) return OO_RC_MAXITERS;
-    if (fabs(I->dt) <= fabs(I->dtmin) && (!I->accept_step || I->t + I->dt < I->tf))
-        return OO_RC_DTLESSTHANMIN;
-    else if (!I->accept_step && fabs(I->dt) <= fabs(oo_jl_eps(I->t)))
-        return OO_RC_UNSTABLE;
+    if (I->o->adaptive) {
+        if (fabs(I->dt) <= fabs(I->dtmin) && (!I->accept_step || I->t + I->dt < I->tf))
+            return OO_RC_DTLESSTHANMIN;
+        else if (!I->accept_step && fabs(I->dt) <= fabs(oo_jl_eps(I->t)))
+            return OO_RC_UNSTABLE;
+    }
     if (I->accept_step) {
         for (int i = 0; i < I->n; i++)
             if (!isfinite(I->u[i])) return OO_RC_UNSTABLE;
@@ -725,6 +733,7 @@ void oo_default_opts(oo_opts* o, int alg) {
     o->save_end = 1;
     o->save_everystep = 0;
     o->save_end_user = -1;
+    o->adaptive = 1;
 }
 
 int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
@@ -782,13 +791,15 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     /* initialize! (tsit_perform_step.jl:188-202): FSAL pre-start */
     if (I->alg == OO_TSIT5) F(I, I->k[0], I->uprev, I->t);
     /* handle_dt! (solve.jl:968-985) */
-    if (opts->dt > 0.0) {
+    I->dtcache = opts->dt;
+    if (opts->dt != 0.0) {   /* user dt (NaN included: check_error reports DtNaN) */
         I->dt = opts->dt; I->dtpropose = opts->dt;
     } else {
         I->dt = initdt(I);
         I->dtpropose = I->dt;
         res->nf += 2;
     }
+    res->dt_init = I->dt;
 
     int rc = OO_RC_SUCCESS;
     /* solve! (solve.jl:904-946) with tstops = [tf] */
@@ -820,6 +831,16 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
         }
         /* _loopfooter! (integrator_utils.jl:597-677) */
         double ttmp = I->t + I->dt;
+        if (!opts->adaptive) {   /* non-adaptive branch: every step is accepted */
+            res->naccept++;
+            I->tprev = I->t;
+            if (I->next_step_tstop) { I->next_step_tstop = 0; I->t = I->tstop_target; }
+            else I->t = ttmp;
+            I->accept_step = 1;
+            I->dtpropose = I->dt;
+            savevalues(I);
+            continue;
+        }
         double q = stepsize_controller(I);
         I->accept_step = (I->EEst <= 1.0);
         if (I->accept_step) {

@@ -43,6 +43,7 @@ typedef struct {
     const double* saveat; /* sorted output times, may be NULL */
     int nsaveat;
     int has_tgrad;        /* Rodas stats: no nf increment for dT when 1 */
+    int adaptive;         /* 0: fixed steps of opts.dt (solve.jl `adaptive = false`) */
 } oo_opts;
 
 typedef struct {
@@ -51,6 +52,7 @@ typedef struct {
     int nsaved;
     double t_final;       /* integrator t at exit */
     double dt_last;       /* integrator dt at exit */
+    double dt_init;       /* dt after handle_dt! (automatic initial step) */
 } oo_result;
 
 void oo_default_opts(oo_opts* o, int alg);

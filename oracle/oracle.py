@@ -27,14 +27,14 @@ class Opts(ctypes.Structure):
                 ("maxiters", ctypes.c_int64), ("save_start", ctypes.c_int), ("save_end", ctypes.c_int),
                 ("save_everystep", ctypes.c_int), ("save_end_user", ctypes.c_int),
                 ("saveat", ctypes.POINTER(ctypes.c_double)), ("nsaveat", ctypes.c_int),
-                ("has_tgrad", ctypes.c_int)]
+                ("has_tgrad", ctypes.c_int), ("adaptive", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("retcode", ctypes.c_int), ("nf", ctypes.c_int64), ("naccept", ctypes.c_int64),
                 ("nreject", ctypes.c_int64), ("njacs", ctypes.c_int64), ("nw", ctypes.c_int64),
                 ("nsolve", ctypes.c_int64), ("nsaved", ctypes.c_int), ("t_final", ctypes.c_double),
-                ("dt_last", ctypes.c_double)]
+                ("dt_last", ctypes.c_double), ("dt_init", ctypes.c_double)]
 
 
 def build():
@@ -68,6 +68,10 @@ def lib():
         L.oo_fastlog2.restype = ctypes.c_float
         L.oo_jl_eps.argtypes = [d]
         L.oo_jl_eps.restype = d
+        L.oo_det_log.argtypes = [d]
+        L.oo_det_log.restype = d
+        L.oo_det_exp.argtypes = [d]
+        L.oo_det_exp.restype = d
         L.oo_lu_solve.argtypes = [ctypes.c_int, P(d), P(ctypes.c_int), P(d)]
         L.oo_tsit5_tableau.argtypes = [P(d)] * 4
         L.oo_rodas5p_tableau.argtypes = [P(d)] * 8
@@ -82,7 +86,7 @@ def _dp(a):
 
 def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0, maxiters=1000000,
               save_start=True, save_end=True, save_everystep=False, save_end_user=None, saveat=None,
-              has_tgrad=False):
+              has_tgrad=False, adaptive=True):
     o = Opts()
     lib().oo_default_opts(ctypes.byref(o), alg)
     o.reltol, o.abstol = reltol, abstol
@@ -95,6 +99,7 @@ def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0
         keep = np.ascontiguousarray(saveat, dtype=np.float64)
         o.saveat, o.nsaveat = _dp(keep), len(keep)
     o.has_tgrad = int(has_tgrad)
+    o.adaptive = int(adaptive)
     return o, keep
 
 
@@ -142,7 +147,7 @@ def solve(f, u0, p, tspan, jac=None, tgrad=None, max_out=4096, **kw):
     ns = min(res.nsaved, max_out)
     return dict(t=t_out[:ns].copy(), u=u_out[:ns, :n].copy(), u_final=uf[:n].copy(), retcode=res.retcode,
                 nf=res.nf, naccept=res.naccept, nreject=res.nreject, njacs=res.njacs, nw=res.nw,
-                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved)
+                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved, dt_init=res.dt_init)
 
 
 def solve_builtin(prob, u0, p, tspan, max_out=4096, **kw):
@@ -159,7 +164,7 @@ def solve_builtin(prob, u0, p, tspan, max_out=4096, **kw):
     ns = min(res.nsaved, max_out)
     return dict(t=t_out[:ns].copy(), u=u_out[:ns, :n].copy(), u_final=uf[:n].copy(), retcode=res.retcode,
                 nf=res.nf, naccept=res.naccept, nreject=res.nreject, njacs=res.njacs, nw=res.nw,
-                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved)
+                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved, dt_init=res.dt_init)
 
 
 def ensemble_builtin(prob, u0_soa, p_soa, t0, tf, nvec=None, threads=None, chunk=64, **kw):
@@ -232,3 +237,11 @@ def tsit5_interp_coeffs():
     r = np.zeros(28)
     lib().oo_tsit5_interp_coeffs(_dp(r))
     return r.reshape(7, 4)
+
+
+def det_log(x):
+    return lib().oo_det_log(x)
+
+
+def det_exp(x):
+    return lib().oo_det_exp(x)

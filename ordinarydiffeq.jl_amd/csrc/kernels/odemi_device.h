@@ -500,7 +500,7 @@ __device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& 
     for (int i = 0; i < N; i++) uk[i] = (K)L.uprev[i];
     odemi_f<K>(L.k1, uk, L.p, (K)L.c.t);
     L.c.nf += 1;
-    if (P.dt_user > 0.0) {
+    if (P.dt_user != 0.0) {  // user dt (a NaN dt is reported as DtNaN)
         L.c.dt = P.dt_user;
     } else {
         L.c.dt = initdt_lane<N>(L.c, u0d, pd, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
@@ -894,7 +894,7 @@ __device__ __forceinline__ void rodas_init(RodasLane<F, N>& L, const Params& P, 
 #pragma unroll
         for (int b = 0; b < N; b++) L.J[a][b] = 0.0;
     if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
-    if (P.dt_user > 0.0) {
+    if (P.dt_user != 0.0) {  // user dt (a NaN dt is reported as DtNaN)
         L.c.dt = P.dt_user;
     } else {
         L.c.dt = initdt_lane<N>(L.c, L.u, L.p, P.reltol, P.abstol, L.c.dtmax, P.dtmin);

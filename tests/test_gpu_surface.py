@@ -1,0 +1,159 @@
+"""GPU tests of the solve surface (through libodemi.so's C ABI) against the CPU
+restatement: saveat / save_start / save_end / save_everystep, dense output of
+both methods, return codes, stats, user dt, and the EnsembleProblem hooks."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import odemi
+from odemi import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _f2(du, u, p, t):
+    du[0] = -odemi.sym_cos(u[0]) * u[0]
+
+
+def _f2_py(du, u, p, t):
+    du[0] = -math.cos(u[0]) * u[0]
+
+
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()])
+def test_saveat_rules_match_reference_lists(alg):
+    """test/InterfaceI/ode_saveat_tests.jl:246-267 on the GPU engine."""
+    print(f"[case] saveat rules {alg}", flush=True)
+    prob = odemi.ODEProblem(_f2, [10.0], (0.0, 0.4))
+    sa = [0.0, 0.1, 0.2, 0.3, 0.4]
+    assert list(odemi.solve(prob, alg, saveat=sa).t) == sa
+    assert list(odemi.solve(prob, alg, saveat=sa, save_start=True, save_end=True).t) == sa
+    assert list(odemi.solve(prob, alg, saveat=sa, save_start=False, save_end=False).t) == [0.1, 0.2, 0.3]
+    assert list(odemi.solve(prob, alg, saveat=[0.2]).t) == [0.2]
+    assert list(odemi.solve(prob, alg, saveat=[0.2], save_start=True, save_end=True).t) == [0.0, 0.2, 0.4]
+    ts = odemi.solve(prob, alg).t
+    assert 0.0 in ts and 0.4 in ts
+    ts = odemi.solve(prob, alg, save_start=False, save_end=False).t
+    assert 0.0 not in ts and 0.4 not in ts
+    # values agree with the oracle (cos differs by at most an ulp between libms)
+    sol = odemi.solve(prob, alg, saveat=sa)
+    ref = O.solve(_f2_py, [10.0], [0.0], (0.0, 0.4), alg=alg.code, saveat=sa, save_start=True, save_end=True,
+                  jac=lambda J, u, p, t: J.__setitem__((0, 0), math.sin(u[0]) * u[0] - math.cos(u[0])))
+    assert np.allclose(np.array(sol.u)[:, 0], ref["u"][:, 0], rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("alg,prob_id,fun,dgen", [
+    (odemi.Tsit5(), O.PROB_LORENZ, P.LORENZ, lambda: P.lorenz_large(N=512)),
+    (odemi.Rodas5P(), O.PROB_ROBERTSON, P.ROBERTSON, lambda: P.robertson_ensemble(N=256)),
+])
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_ensemble_saveat_bit_identical(alg, prob_id, fun, dgen, engine):
+    d = dgen()
+    N = d["u0"].shape[1]
+    sa = np.linspace(0.05, 0.95, 7) * (1.0 if alg.code == O.TSIT5 else 1e4)
+    print(f"[case] ensemble saveat {alg} {engine} N={N}", flush=True)
+    out = odemi.solve_arrays(fun, alg, d["u0"], d["p"], d["t0"], d["tf"], odemi.EnsembleMI355X(engine=engine),
+                             reltol=d["reltol"], abstol=d["abstol"], saveat=sa, save_start=True, save_end=True)
+    for j in range(0, N, 37):
+        r = O.solve_builtin(prob_id, d["u0"][:, j], d["p"][:, j], (d["t0"][j], d["tf"][j]), alg=alg.code,
+                            reltol=d["reltol"], abstol=d["abstol"], saveat=sa, save_start=True, save_end=True)
+        ns = int(out["nsaved"][j])
+        assert ns == r["nsaved"], (j, ns, r["nsaved"])
+        assert np.array_equal(out["t_out"][:ns, j], r["t"])
+        assert np.array_equal(out["u_out"][:ns, :, j], r["u"])
+        assert int(out["retcode"][j]) == r["retcode"]
+
+
+@pytest.mark.parametrize("alg,prob_id,fun,dgen", [
+    (odemi.Tsit5(), O.PROB_LORENZ, P.LORENZ, lambda: P.lorenz_small(N=64)),
+    (odemi.Rodas5P(), O.PROB_ROBERTSON, P.ROBERTSON, lambda: P.robertson_ensemble(N=64)),
+])
+def test_save_everystep_bit_identical(alg, prob_id, fun, dgen):
+    d = dgen()
+    print(f"[case] save_everystep {alg}", flush=True)
+    out = odemi.solve_arrays(fun, alg, d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"], abstol=d["abstol"],
+                             save_everystep=True, save_start=True, save_end=True, max_saves=2048)
+    for j in range(0, 64, 9):
+        r = O.solve_builtin(prob_id, d["u0"][:, j], d["p"][:, j], (d["t0"][j], d["tf"][j]), alg=alg.code,
+                            reltol=d["reltol"], abstol=d["abstol"], save_everystep=True, max_out=2048)
+        ns = int(out["nsaved"][j])
+        assert ns == r["nsaved"]
+        assert np.array_equal(out["t_out"][:ns, j], r["t"])
+        assert np.array_equal(out["u_out"][:ns, :, j], r["u"])
+
+
+def _expb(du, u, p, t):
+    du[0] = odemi.sym_exp(u[0])
+
+
+def test_return_codes():
+    print("[case] return codes", flush=True)
+    for alg in (odemi.Tsit5(), odemi.Rodas5P()):
+        sol = odemi.solve(odemi.ODEProblem(_expb, [0.0], (0.0, 10.0)), alg, reltol=1e-8, abstol=1e-8)
+        assert sol.retcode in (odemi.ReturnCode.MaxIters, odemi.ReturnCode.Unstable)
+        ref = O.solve_builtin(O.PROB_EXPBLOWUP, [0.0], [0.0], (0.0, 10.0), alg=alg.code, reltol=1e-8, abstol=1e-8)
+        assert int(sol.retcode) == ref["retcode"]
+    lor = odemi.ODEProblem(P.LORENZ, [1.0, 0.0, 0.0], (0.0, 10.0), [10.0, 28.0, 8 / 3])
+    sol = odemi.solve(lor, odemi.Tsit5(), maxiters=5)
+    assert sol.retcode == odemi.ReturnCode.MaxIters and sol.stats.naccept + sol.stats.nreject == 5
+    sol = odemi.solve(odemi.remake(lor, u0=[float("nan"), 0.0, 0.0]), odemi.Tsit5())
+    assert sol.retcode == odemi.ReturnCode.Unstable
+    sol = odemi.solve(lor, odemi.Tsit5(), dt=float("nan"))
+    assert sol.retcode == odemi.ReturnCode.DtNaN
+
+
+def test_stats_and_user_dt_match_oracle():
+    print("[case] stats + user dt", flush=True)
+    d = P.lorenz_small(N=128)
+    for alg, pid, fun in ((odemi.Tsit5(), O.PROB_LORENZ, P.LORENZ), (odemi.Rodas5P(), O.PROB_LORENZ, P.LORENZ)):
+        out = odemi.solve_arrays(fun, alg, d["u0"], d["p"], d["t0"], d["tf"], reltol=1e-6, abstol=1e-6, dt=1e-3)
+        for j in range(0, 128, 17):
+            r = O.solve_builtin(pid, d["u0"][:, j], d["p"][:, j], (0.0, 1.0), alg=alg.code, reltol=1e-6,
+                                abstol=1e-6, dt=1e-3, save_start=False)
+            st = out["stats"][:, j]
+            assert (st[0], st[1], st[2], st[3], st[4], st[5]) == (r["nf"], r["naccept"], r["nreject"], r["njacs"],
+                                                                   r["nw"], r["nsolve"])
+            assert np.array_equal(out["u_final"][:, j], r["u_final"])
+
+
+def test_ensemble_problem_hooks():
+    """prob_func(prob, ctx) sets the initial conditions (ensemble_thread_safety.jl);
+    output_func and reduction see every trajectory."""
+    print("[case] EnsembleProblem hooks", flush=True)
+
+    def f(du, u, p, t):
+        du[0] = 1.01 * u[0]
+        du[1] = 0.0 * u[1]
+
+    prob = odemi.ODEProblem(f, [0.0, 0.0], (0.0, 1.0))
+    ics = np.linspace(0, 1, 100)
+
+    def prob_func(prob, ctx):
+        return odemi.remake(prob, u0=[ics[ctx.sim_id - 1], 0.0])
+
+    sim = odemi.solve(odemi.EnsembleProblem(prob, prob_func=prob_func), odemi.Tsit5(), odemi.EnsembleMI355X(),
+                      trajectories=100)
+    assert len(sim.u) == 100
+    assert np.array_equal([s.u[0][0] for s in sim.u], ics)           # get_timepoint(sim, 0)
+    assert all(s.retcode == odemi.ReturnCode.Success for s in sim.u)
+    assert np.allclose([s.u[-1][0] for s in sim.u], ics * math.exp(1.01), rtol=1e-3)
+    red = odemi.solve(odemi.EnsembleProblem(prob, prob_func=prob_func,
+                                            output_func=lambda sol, ctx: (sol.u[-1][0], False),
+                                            reduction=lambda u, data, I: ((u or 0.0) + sum(data), False)),
+                      odemi.Tsit5(), odemi.EnsembleMI355X(), trajectories=100, saveat=[1.0])
+    assert red.u == pytest.approx(float(np.sum(ics)) * math.exp(1.01), rel=1e-3)
+
+
+def test_mixed_precision_saveat_within_tolerance():
+    d = P.lorenz_small(N=256)
+    sa = np.linspace(0.1, 0.9, 9)
+    print("[case] mixed saveat", flush=True)
+    o64 = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"], reltol=1e-6, abstol=1e-6,
+                             saveat=sa)
+    om = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(precision="mixed"), reltol=1e-6, abstol=1e-6, saveat=sa)
+    assert np.array_equal(o64["nsaved"], om["nsaved"])
+    ns = int(o64["nsaved"].min())
+    err = np.abs(om["u_out"][:ns] - o64["u_out"][:ns]) / (1e-6 + 1e-6 * np.abs(o64["u_out"][:ns]))
+    assert np.quantile(err, 0.99) < 20.0

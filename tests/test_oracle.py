@@ -1,0 +1,419 @@
+"""Pins the CPU restatement (oracle/) before it is trusted as the parity target.
+
+Each check is chosen so that a plausible transcription mistake (a dropped
+term, a wrong sign or index, a transposed operand) fails it:
+  * tableaus: Butcher / Rosenbrock order conditions in mpmath at 50 digits;
+  * one step on u' = lambda u equals the method's stability function, which
+    must agree with exp(z) to the method order and (Rodas5P) vanish at -inf;
+  * fixed-step convergence order 5 +- 0.2 (the reference's own test:
+    lib/OrdinaryDiffEqRosenbrock/test/ode_rosenbrock_tests.jl:195-215);
+  * reference known answers: saveat/save_start/save_end time lists
+    (test/InterfaceI/ode_saveat_tests.jl:246-267), qmax_first_step growth and
+    u(100) = exp(-100) (test/InterfaceI/controllers.jl:29-73), the exp(u)
+    blow-up return codes (test/Integrators_I/check_error.jl:1-44), nf accounting
+    (test/InterfaceIII/stats_tests.jl:1-40), empty-state dt = 1e-6
+    (test/Regression_I/ode_adaptive_tests.jl:111-118);
+  * the textbook Hairer-Norsett-Wanner initial step recomputed in mpmath;
+  * invariants: Robertson mass conservation; agreement with scipy's DOP853 /
+    Radau at tight tolerance; LU against numpy; Julia eps/min semantics.
+"""
+import math
+from fractions import Fraction
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+import oracle as O
+
+mp.mp.dps = 50
+
+
+# ---------------------------------------------------------------------------
+# rooted trees and Butcher order conditions
+# ---------------------------------------------------------------------------
+def _trees(order):
+    """All rooted trees with `order` nodes, as sorted tuples of children."""
+    if order == 1:
+        return [()]
+    out = set()
+
+    def partitions(n, maxpart):
+        if n == 0:
+            yield []
+            return
+        for k in range(min(n, maxpart), 0, -1):
+            for rest in partitions(n - k, k):
+                yield [k] + rest
+
+    for parts in partitions(order - 1, order - 1):
+        lists = [_trees(k) for k in parts]
+
+        def combos(i):
+            if i == len(lists):
+                yield []
+                return
+            for t in lists[i]:
+                for r in combos(i + 1):
+                    yield [t] + r
+
+        for ch in combos(0):
+            out.add(tuple(sorted(ch)))
+    return sorted(out)
+
+
+def _gamma(t):
+    n = 1 + sum(_size(c) for c in t)
+    g = n
+    for c in t:
+        g *= _gamma(c)
+    return g
+
+
+def _size(t):
+    return 1 + sum(_size(c) for c in t)
+
+
+def _stage_weights(t, A, s):
+    """Vector Phi_i(t) = prod_children (A Phi(child))_i."""
+    v = [mp.mpf(1)] * s
+    for c in t:
+        w = _stage_weights(c, A, s)
+        aw = [mp.fsum(A[i][j] * w[j] for j in range(s)) for i in range(s)]
+        v = [v[i] * aw[i] for i in range(s)]
+    return v
+
+
+def _order_residuals(A, b, p):
+    s = len(b)
+    res = []
+    for order in range(1, p + 1):
+        for t in _trees(order):
+            phi = _stage_weights(t, A, s)
+            res.append(abs(mp.fsum(b[i] * phi[i] for i in range(s)) - mp.mpf(1) / _gamma(t)))
+    return res
+
+
+def test_tree_counts():
+    assert [len(_trees(k)) for k in range(1, 6)] == [1, 1, 2, 4, 9]
+
+
+def test_tsit5_tableau_order_conditions():
+    c, a, b, bt = O.tsit5_tableau()
+    A = [[mp.mpf(float(x)) for x in row] for row in a]
+    B = [mp.mpf(float(x)) for x in b]
+    # row sums: c_i = sum_j a_ij
+    for i in range(7):
+        assert abs(mp.fsum(A[i]) - mp.mpf(float(c[i]))) < 1e-15
+    # FSAL: last row of A is b, b_7 = 0
+    assert b[6] == 0.0 and np.all(a[6] == b)
+    # 5th order solution (17 conditions through order 5) to double rounding
+    assert max(_order_residuals(A, B, 5)) < 1e-13
+    # embedded 4th order: bhat = b - btilde satisfies order 4 but not 5
+    Bh = [B[i] - mp.mpf(float(bt[i])) for i in range(7)]
+    assert max(_order_residuals(A, Bh, 4)) < 1e-13
+    assert max(_order_residuals(A, Bh, 5)) > 1e-6
+
+
+def _tsit5_stability(z):
+    c, a, b, _ = O.tsit5_tableau()
+    A = mp.matrix([[mp.mpf(float(x)) for x in row] for row in a])
+    B = mp.matrix([[mp.mpf(float(x)) for x in b]])
+    one = mp.matrix([[1]] * 7)
+    M = mp.eye(7) - z * A
+    return 1 + z * (B * mp.lu_solve(M, one))[0]
+
+
+def test_tsit5_stability_polynomial_matches_exp_to_order5():
+    # R(z) - e^z = O(z^6): check the Taylor coefficients through z^5
+    for z in (mp.mpf("1e-2"), mp.mpf("-2e-2")):
+        assert abs(_tsit5_stability(z) - mp.e ** z) < 5 * abs(z) ** 6
+
+
+@pytest.mark.parametrize("lam,h", [(-1.0, 0.1), (-3.0, 0.05), (0.7, 0.2)])
+def test_tsit5_single_step_is_stability_polynomial(lam, h):
+    # one fixed step of u' = lam u
+    r = O.solve_builtin(O.PROB_LINEAR, [1.0], [lam], (0.0, h), dt=h, adaptive=False, save_start=False)
+    assert r["naccept"] == 1
+    exact = _tsit5_stability(mp.mpf(lam) * mp.mpf(h))
+    assert abs(r["u_final"][0] - float(exact)) < 4e-16
+
+
+def _rodas_stability(z):
+    """R(z) of the transformed Rosenbrock scheme on u' = lam u from (A, C, gamma, b)."""
+    T = O.rodas5p_tableau()
+    A = [[mp.mpf(float(x)) for x in row] for row in T["A"]]
+    C = [[mp.mpf(float(x)) for x in row] for row in T["C"]]
+    g = mp.mpf(float(T["gamma"]))
+    b = [mp.mpf(float(x)) for x in T["b"]]
+    # with h = 1: (1/gamma - z) k_s = z (1 + sum_j A_sj k_j) + sum_j C_sj k_j
+    k = []
+    for s in range(8):
+        rhs = z * (1 + mp.fsum(A[s][j] * k[j] for j in range(s))) + mp.fsum(C[s][j] * k[j] for j in range(s))
+        k.append(rhs / (1 / g - z))
+    return 1 + mp.fsum(b[s] * k[s] for s in range(8))
+
+
+def test_rodas5p_stability_function():
+    # order 5: R(z) = e^z + O(z^6)
+    for z in (mp.mpf("1e-2"), mp.mpf("-1e-2")):
+        assert abs(_rodas_stability(z) - mp.e ** z) < 50 * abs(z) ** 6
+    # L-stable: R(-inf) = 0
+    assert abs(_rodas_stability(mp.mpf("-1e12"))) < 1e-9
+
+
+@pytest.mark.parametrize("lam,h", [(-1.0, 0.1), (-50.0, 0.1), (1.01, 0.25)])
+def test_rodas5p_single_step_is_stability_function(lam, h):
+    r = O.solve_builtin(O.PROB_LINEAR, [1.0], [lam], (0.0, h), alg=O.RODAS5P, dt=h, adaptive=False,
+                        save_start=False)
+    exact = _rodas_stability(mp.mpf(lam) * mp.mpf(h))
+    assert abs(r["u_final"][0] - float(exact)) < 1e-14 * max(1.0, abs(float(exact)))
+
+
+def test_tsit5_convergence_order_5_linear():
+    """test_convergence on prob_ode_linear (u' = 1.01 u, u0 = 1/2), fixed dt = 2^-k: order 5 +- 0.2."""
+    dts = [0.5 ** k for k in (4, 3, 2, 1)]
+    errs = []
+    for dt in dts:
+        r = O.solve_builtin(O.PROB_LINEAR, [0.5], [1.01], (0.0, 1.0), dt=dt, adaptive=False, save_start=False)
+        errs.append(abs(r["u_final"][0] - 0.5 * math.exp(1.01)))
+    slope = np.polyfit(np.log(dts), np.log(errs), 1)[0]
+    assert abs(slope - 5.0) < 0.2, slope
+
+
+def _nl_f(du, u, p, t):
+    du[0] = -u[0] * u[0] + math.sin(t)
+
+
+def _nl_j(J, u, p, t):
+    J[0, 0] = -2.0 * u[0]
+
+
+def _nl_tg(dT, u, p, t):
+    dT[0] = math.cos(t)
+
+
+@pytest.mark.parametrize("alg", [O.TSIT5, O.RODAS5P])
+def test_convergence_nonlinear_nonautonomous(alg):
+    """u' = -u^2 + sin t exercises the nonlinear stage terms, J and dT/dt (Rodas):
+    a wrong coefficient, sign or index drops the observed order to <= 3."""
+    ref = O.solve(_nl_f, [1.0], [0.0], (0.0, 2.0), jac=_nl_j, tgrad=_nl_tg, reltol=1e-13, abstol=1e-14,
+                  save_start=False)["u_final"][0]
+    dts = [0.5 ** k for k in (6, 5, 4, 3)]
+    errs = [abs(O.solve(_nl_f, [1.0], [0.0], (0.0, 2.0), jac=_nl_j, tgrad=_nl_tg, alg=alg, dt=dt,
+                        adaptive=False, save_start=False)["u_final"][0] - ref) for dt in dts]
+    slope = np.polyfit(np.log(dts), np.log(errs), 1)[0]
+    assert slope > 4.6, slope
+
+
+def test_rodas5p_convergence_linear():
+    dts = [0.5 ** k for k in (4, 3, 2, 1)]
+    errs = [abs(O.solve_builtin(O.PROB_LINEAR, [0.5], [1.01], (0.0, 1.0), alg=O.RODAS5P, dt=dt, adaptive=False,
+                                save_start=False)["u_final"][0] - 0.5 * math.exp(1.01)) for dt in dts]
+    assert np.polyfit(np.log(dts), np.log(errs), 1)[0] > 4.8
+
+
+def test_rodas5p_adaptive_linear_few_steps():
+    # ode_rosenbrock_tests.jl: the adaptive solve of prob_ode_linear needs < 20 steps
+    r = O.solve_builtin(O.PROB_LINEAR, [0.5], [1.01], (0.0, 1.0), alg=O.RODAS5P, save_everystep=True)
+    assert r["retcode"] == 1 and r["nsaved"] < 20
+
+
+# ---------------------------------------------------------------------------
+# step control known answers
+# ---------------------------------------------------------------------------
+def test_qmax_first_step_then_qmax():
+    """u' = 0: EEst == 0 every step, so q = 1/qmax: the first step grows dt by
+    qmax_first_step = 1e4, later ones by qmax = 10 (controllers.jl:288-293, 805-821)."""
+    for alg in (O.TSIT5, O.RODAS5P):
+        r = O.solve_builtin(O.PROB_LINEAR, [1.0], [0.0], (0.0, 100.0), alg=alg, save_everystep=True)
+        # initdt: d1 = 0 -> dt0 = 1e-6, f0 == f1 -> 100*dt0 = 1e-4
+        assert r["dt_init"] == pytest.approx(1e-4, rel=1e-15)
+        t = r["t"]
+        assert t[0] == 0.0 and t[1] == pytest.approx(1e-4, rel=1e-15)
+        assert t[2] - t[1] == pytest.approx(1.0, rel=1e-12)      # x 1e4
+        assert t[3] - t[2] == pytest.approx(10.0, rel=1e-12)     # x 10
+        assert t[-1] == 100.0 and len(t) == 5                    # dtmax = 100, clipped at tf
+
+
+@pytest.mark.parametrize("alg", [O.TSIT5, O.RODAS5P])
+def test_exponential_decay_to_100(alg):
+    r = O.solve_builtin(O.PROB_LINEAR, [1.0], [-1.0], (0.0, 100.0), alg=alg)
+    assert r["retcode"] == 1
+    assert abs(r["u_final"][0] - math.exp(-100.0)) < 1e-6
+
+
+def test_empty_state_initial_dt():
+    r = O.solve(lambda du, u, p, t: None, np.zeros(0), [0.0], (0.0, 1.0))
+    assert r["dt_init"] == pytest.approx(1e-6)
+
+
+def _hnw_initdt(f, u0, p, t0, tf, reltol, abstol, order=5):
+    """Textbook Hairer-Norsett-Wanner starting step (Solving ODEs I, II.4) in mpmath."""
+    u0 = [mp.mpf(x) for x in u0]
+    n = len(u0)
+    sk = [abstol + abs(x) * reltol for x in u0]
+    nrm = lambda v: mp.sqrt(mp.fsum(x * x for x in v) / n)
+    f0 = np.zeros(n)
+    f(f0, np.array([float(x) for x in u0]), p, t0)
+    d0 = nrm([u0[i] / sk[i] for i in range(n)])
+    d1 = nrm([mp.mpf(f0[i]) / sk[i] for i in range(n)])
+    h0 = mp.mpf("1e-6") if (d0 < 1e-5 or d1 < 1e-5) else d0 / d1 / 100
+    h0 = min(h0, mp.mpf(tf - t0))
+    u1 = np.array([float(u0[i] + h0 * f0[i]) for i in range(n)])
+    f1 = np.zeros(n)
+    f(f1, u1, p, float(t0 + h0))
+    d2 = nrm([(mp.mpf(f1[i]) - f0[i]) / sk[i] for i in range(n)]) / h0
+    m = max(d1, d2)
+    h1 = max(mp.mpf("1e-6"), h0 * mp.mpf("1e-3")) if m <= 1e-15 else mp.power(10, -(2 + mp.log10(m)) / order)
+    return float(min(100 * h0, h1, tf - t0))
+
+
+def _lorenz(du, u, p, t):
+    du[0] = p[0] * (u[1] - u[0])
+    du[1] = u[0] * (p[1] - u[2]) - u[1]
+    du[2] = u[0] * u[1] - p[2] * u[2]
+
+
+@pytest.mark.parametrize("u0,p,tol", [([1.0, 0.0, 0.0], [10.0, 28.0, 8 / 3], 1e-6),
+                                      ([0.3, -0.2, 0.9], [9.0, 14.0, 8 / 3], 1e-8),
+                                      ([1.0, 0.0, 0.0], [10.0, 0.5, 8 / 3], 1e-3)])
+def test_initdt_matches_textbook(u0, p, tol):
+    r = O.solve_builtin(O.PROB_LORENZ, u0, p, (0.0, 10.0), reltol=tol, abstol=tol)
+    ref = _hnw_initdt(_lorenz, u0, p, 0.0, 10.0, tol, tol)
+    assert r["dt_init"] == pytest.approx(ref, rel=1e-12)
+
+
+def test_saveat_save_start_save_end_rules():
+    """test/InterfaceI/ode_saveat_tests.jl:246-267 (resolved save flags, f = -cos(u) u)."""
+    f2 = lambda du, u, p, t: du.__setitem__(0, -math.cos(u[0]) * u[0])
+    kw = dict(tspan=(0.0, 0.4), p=[0.0])
+    sa = [0.0, 0.1, 0.2, 0.3, 0.4]
+    # defaults: t0 and tf are in saveat -> save_start = save_end = true
+    r = O.solve(f2, [10.0], saveat=sa, save_start=True, save_end=True, **kw)
+    assert list(r["t"]) == [0.0, 0.1, 0.2, 0.3, 0.4]
+    r = O.solve(f2, [10.0], saveat=sa, save_start=False, save_end=False, **kw)
+    assert list(r["t"]) == [0.1, 0.2, 0.3]
+    r = O.solve(f2, [10.0], saveat=[0.2], save_start=False, save_end=False, **kw)
+    assert list(r["t"]) == [0.2]
+    r = O.solve(f2, [10.0], saveat=[0.2], save_start=True, save_end=True, save_end_user=True, **kw)
+    assert list(r["t"]) == [0.0, 0.2, 0.4]
+    r = O.solve(f2, [10.0], save_everystep=True, **kw)
+    assert r["t"][0] == 0.0 and r["t"][-1] == 0.4
+    r = O.solve(f2, [10.0], save_everystep=True, save_start=False, save_end=False, **kw)
+    assert 0.0 not in r["t"] and 0.4 not in r["t"]
+
+
+def test_saveat_interpolant_accuracy():
+    # dense output (Tsit5 free 4th-order interpolant, Rodas 3rd-order H-rows) vs exp
+    sa = list(np.linspace(0.05, 0.95, 19))
+    for alg, tol in ((O.TSIT5, 1e-6), (O.RODAS5P, 1e-5)):
+        r = O.solve_builtin(O.PROB_LINEAR, [1.0], [-2.0], (0.0, 1.0), alg=alg, saveat=sa, reltol=1e-8,
+                            abstol=1e-10, save_start=False, save_end=False)
+        assert np.allclose(r["t"], sa)
+        assert np.max(np.abs(r["u"][:, 0] - np.exp(-2.0 * r["t"]))) < tol
+
+
+def test_exp_blowup_retcode():
+    """u' = exp(u), u0 = 0 on (0, 10): explodes at t = 1 -> MaxIters or Unstable."""
+    r = O.solve_builtin(O.PROB_EXPBLOWUP, [0.0], [0.0], (0.0, 10.0), reltol=1e-8, abstol=1e-8)
+    assert O.RC_NAMES[r["retcode"]] in ("MaxIters", "Unstable")
+    r = O.solve_builtin(O.PROB_EXPBLOWUP, [0.0], [0.0], (0.0, 10.0), alg=O.RODAS5P, reltol=1e-8, abstol=1e-8)
+    assert O.RC_NAMES[r["retcode"]] in ("MaxIters", "Unstable")
+
+
+def test_maxiters_and_nan_retcodes():
+    r = O.solve_builtin(O.PROB_LORENZ, [1.0, 0.0, 0.0], [10.0, 28.0, 8 / 3], (0.0, 10.0), maxiters=5)
+    assert O.RC_NAMES[r["retcode"]] == "MaxIters" and r["naccept"] + r["nreject"] == 5
+    # NaN in the state: initdt returns dtmin (initdt.jl "First function call produced NaNs"),
+    # every attempt is rejected until dt <= eps(t) -> Unstable
+    r = O.solve_builtin(O.PROB_LORENZ, [float("nan"), 0.0, 0.0], [10.0, 28.0, 8 / 3], (0.0, 1.0))
+    assert O.RC_NAMES[r["retcode"]] == "Unstable"
+    # a NaN step size is caught first by check_error -> DtNaN
+    r = O.solve_builtin(O.PROB_LORENZ, [1.0, 0.0, 0.0], [10.0, 28.0, 8 / 3], (0.0, 1.0), dt=float("nan"))
+    assert O.RC_NAMES[r["retcode"]] == "DtNaN"
+
+
+def test_stats_nf_accounting():
+    """Tsit5: nf = 1 (FSAL) + 2 (initdt) + 6 per attempt; Rodas5P with analytic J and
+    AD time derivative: nf = 2 + 8 per attempt + 1 per new J (stats_tests.jl)."""
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        u0 = rng.uniform(-1, 1, 3)
+        p = [rng.uniform(8, 12), rng.uniform(0, 28), 8 / 3]
+        r = O.solve_builtin(O.PROB_LORENZ, u0, p, (0.0, 3.0), reltol=1e-6, abstol=1e-6)
+        assert r["nf"] == 3 + 6 * (r["naccept"] + r["nreject"])
+        r = O.solve_builtin(O.PROB_LORENZ, u0, p, (0.0, 3.0), alg=O.RODAS5P, reltol=1e-6, abstol=1e-6)
+        att = r["naccept"] + r["nreject"]
+        assert r["nf"] == 2 + 8 * att + r["njacs"]
+        assert r["nw"] == att and r["nsolve"] == 8 * att
+        # every rejected attempt is followed by one attempt that reuses J
+        assert r["njacs"] == att - r["nreject"]
+
+
+def test_rodas_reuses_jacobian_after_reject():
+    # do_newJW: J is recomputed every attempt except right after a rejection
+    r = O.solve_builtin(O.PROB_ROBERTSON, [1.0, 0.0, 0.0], [0.04, 3e7, 1e4], (0.0, 1e5), alg=O.RODAS5P,
+                        reltol=1e-6, abstol=1e-10)
+    assert r["nreject"] > 0
+    assert r["njacs"] == r["naccept"]
+
+
+def test_robertson_conservation_and_radau():
+    from scipy.integrate import solve_ivp
+    k = [0.04, 3e7, 1e4]
+    r = O.solve_builtin(O.PROB_ROBERTSON, [1.0, 0.0, 0.0], k, (0.0, 1e5), alg=O.RODAS5P, reltol=1e-6,
+                        abstol=1e-10)
+    assert abs(sum(r["u_final"]) - 1.0) < 1e-8
+    ref = solve_ivp(lambda t, u: [-k[0] * u[0] + k[2] * u[1] * u[2],
+                                  k[0] * u[0] - k[1] * u[1] ** 2 - k[2] * u[1] * u[2], k[1] * u[1] ** 2],
+                    (0, 1e5), [1, 0, 0], method="Radau", rtol=1e-12, atol=1e-16)
+    err = np.abs(r["u_final"] - ref.y[:, -1]) / (1e-10 + 1e-6 * np.abs(ref.y[:, -1]))
+    assert err.max() < 50
+
+
+def test_tsit5_tight_tolerance_vs_dop853():
+    from scipy.integrate import solve_ivp
+    p = [10.0, 28.0, 8 / 3]
+    r = O.solve_builtin(O.PROB_LORENZ, [1.0, 0.0, 0.0], p, (0.0, 1.0), reltol=1e-11, abstol=1e-11)
+    ref = solve_ivp(lambda t, u: [p[0] * (u[1] - u[0]), u[0] * (p[1] - u[2]) - u[1], u[0] * u[1] - p[2] * u[2]],
+                    (0, 1), [1, 0, 0], method="DOP853", rtol=1e-13, atol=1e-14)
+    assert np.max(np.abs(r["u_final"] - ref.y[:, -1]) / np.abs(ref.y[:, -1])) < 1e-8
+
+
+def test_lu_matches_numpy():
+    rng = np.random.default_rng(7)
+    for n in (1, 2, 3, 5, 8, 17):
+        A = rng.normal(size=(n, n))
+        b = rng.normal(size=n)
+        x = O.lu_solve(A, b)
+        assert np.allclose(A @ x, b, rtol=1e-10, atol=1e-10)
+    # pivoting: a zero leading entry must still solve
+    A = np.array([[0.0, 1.0], [1.0, 0.0]])
+    assert np.allclose(O.lu_solve(A, [2.0, 3.0]), [3.0, 2.0])
+
+
+def test_julia_scalar_semantics():
+    for x in (1.0, 0.0, 1e300, 3.5e-310, 0.1, 123456.789, 2.0 ** -1022):
+        assert O.jl_eps(x) == np.spacing(abs(x)) or (x == 0.0 and O.jl_eps(x) == 5e-324)
+    assert math.isnan(O.jl_eps(float("inf")))
+    # fast log2: exact at powers of two, within 2e-3 of log2 elsewhere
+    for x in (1.0, 2.0, 0.5, 1024.0):
+        assert O.fastlog2(x) == math.log2(x)
+    xs = np.exp(np.linspace(-20, 20, 2001))
+    assert max(abs(O.fastlog2(float(x)) - math.log2(x)) for x in xs) < 2e-3
+    # fastpower: FastPower semantics (0 -> 0), Float32 result
+    assert O.fastpower(0.0, 0.14) == 0.0
+    assert O.fastpower(1.0, 0.14) == 1.0
+    v = O.fastpower(0.37, 0.14)
+    assert v == float(np.float32(v)) and abs(v - 0.37 ** 0.14) < 1e-3
+
+
+def test_det_log_exp_accuracy():
+    rng = np.random.default_rng(11)
+    xs = np.exp(rng.uniform(-700, 700, 20000))
+    ys = rng.uniform(-700, 700, 20000)
+    el = max(abs(O.det_log(float(x)) - math.log(x)) / max(abs(math.log(x)), 1e-300) for x in xs)
+    ee = max(abs(O.det_exp(float(y)) - math.exp(y)) / math.exp(y) for y in ys)
+    assert el < 3e-16 and ee < 3e-16
+    assert O.det_exp(0.0) == 1.0 and O.det_log(1.0) == 0.0
