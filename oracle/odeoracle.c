@@ -36,7 +36,8 @@
  *   oo_det_exp, so the GPU engine can reproduce every bit.
  *   LinearSolve.jl default dense LU: textbook right-looking LU with partial
  *   pivoting (first maximum on ties), multipliers scaled by the reciprocal of
- *   the pivot, trailing update a -= l*u as fma.
+ *   the pivot, trailing update a -= l*u as fma; triangular solves in LAPACK's
+ *   column order (dtrsv).
  */
 #include "odeoracle.h"
 #include <math.h>
@@ -332,10 +333,11 @@ static void lu_solve(int n, const double* LU, const int* piv, double* b) {
         for (int j = 0; j < i; j++) acc = fma(-LU[i * n + j], b[j], acc);
         b[i] = acc;
     }
-    for (int i = n - 1; i >= 0; i--) {
-        double acc = b[i];
-        for (int j = i + 1; j < n; j++) acc = fma(-LU[i * n + j], b[j], acc);
-        b[i] = acc / LU[i * n + i];
+    /* back substitution in column order (LAPACK dtrsv 'U','N','N'):
+     * x_j = x_j / U_jj, then x_i -= U_ij x_j for i < j, j = n-1 .. 0 */
+    for (int j = n - 1; j >= 0; j--) {
+        b[j] = b[j] / LU[j * n + j];
+        for (int i = 0; i < j; i++) b[i] = fma(-LU[i * n + j], b[j], b[i]);
     }
 }
 

@@ -804,7 +804,8 @@ __device__ void tsit5_plain_step(const Params& P) {
 // ---------------------------------------------------------------------------
 // Dense LU on a lane-resident matrix (compile-time N, branchless row swaps).
 // Textbook right-looking LU with partial pivoting (first maximum on ties),
-// multipliers scaled by the reciprocal pivot, trailing update a -= l*u as fma.
+// multipliers scaled by the reciprocal pivot, trailing update a -= l*u as fma,
+// triangular solves in LAPACK's column order.
 // ---------------------------------------------------------------------------
 template <typename T, int N>
 __device__ __forceinline__ void lu_factor_reg(T (&A)[N][N], int (&piv)[N]) {
@@ -857,12 +858,12 @@ __device__ __forceinline__ void lu_solve_reg(const T (&A)[N][N], const int (&piv
         for (int c = 0; c < i; c++) acc = fma(-A[i][c], b[c], acc);
         b[i] = acc;
     }
+    // back substitution in column order (LAPACK dtrsv 'U','N','N')
 #pragma unroll
-    for (int i = N - 1; i >= 0; i--) {
-        T acc = b[i];
+    for (int c = N - 1; c >= 0; c--) {
+        b[c] = b[c] / A[c][c];
 #pragma unroll
-        for (int c = i + 1; c < N; c++) acc = fma(-A[i][c], b[c], acc);
-        b[i] = acc / A[i][i];
+        for (int i = 0; i < c; i++) b[i] = fma(-A[i][c], b[c], b[i]);
     }
 }
 
@@ -1184,3 +1185,7 @@ __device__ void rodas_plain_step(const Params& P) {
 }
 
 }  // namespace odemi
+
+#if defined(ODEMI_WITH_WG) && !defined(ODEMI_RTC)
+#include "odemi_wg.h"
+#endif

@@ -1,0 +1,526 @@
+// odemi_wg.h -- Rodas5P with one workgroup per trajectory, for systems too
+// large for a lane (n up to 256) and for batches of uneven dimension (nvec).
+//
+// Same step-control semantics as the lane kernels (see odemi_device.h) and the
+// same arithmetic order as the CPU restatement: every element of the LU and of
+// the triangular solves sees the same fma sequence as the sequential code, and
+// the norms are accumulated by one thread in index order, so the FP64 variant
+// is bit-identical to the restatement.
+//
+// LDS layout (dynamic, sized by the host from nmax):
+//   W    F[nmax][ld], ld = nmax + 1       J, then W = J - I/(dt*gamma), then its LU
+//   vec  double[NV][nmax]                 u, uprev, fsal, du, dT, ks[8] (+ dense[3]);
+//   pd   double[32]                       parameters (FP64)
+//   fsc  float[nmax + 32]                 FP32 copies of uprev and p (mixed variant)
+//   piv  int[nmax]
+//   red  double[32] + int[32]             reductions / broadcasts
+// FP64 variant: W is FP64 (132 KB at n = 128, one workgroup per CU).
+// Mixed variant: W is FP32 and every stage solve gets `refine_sweeps` FP64
+// iterative-refinement sweeps whose residual b - (J x - x/(dt*gamma)) uses the
+// traced Jacobian-vector product in FP64, so J is never stored in FP64 and the
+// refined solution is that of the FP64 system (about 80 KB at n = 128, two
+// workgroups per CU).
+#pragma once
+#ifdef ODEMI_WITH_WG
+
+#ifndef ODEMI_WG_THREADS
+#define ODEMI_WG_THREADS 256
+#endif
+#define ODEMI_WG_ROWS ((ODEMI_N + 63) / 64)
+
+namespace odemi {
+
+__host__ __device__ inline int wg_nvec(bool saves) { return 13 + (saves ? 3 : 0); }
+constexpr int kWGMaxNP = 32;
+
+struct WGPtrs {
+    void* W;
+    double *u, *uprev, *fsal, *du, *dT, *lt, *ks, *dense;
+    double* pd;
+    float* fsc;
+    float* pf;
+    int* piv;
+    double* red;
+    int* ired;
+    int ld;
+};
+
+template <typename F>
+__device__ __forceinline__ WGPtrs wg_layout(unsigned char* base, int nmax, bool saves) {
+    WGPtrs L;
+    L.ld = nmax + 1;
+    size_t off = 0;
+    L.W = base;
+    off += ((size_t)nmax * L.ld * sizeof(F) + 15) & ~(size_t)15;
+    double* v = (double*)(base + off);
+    L.u = v; L.uprev = v + nmax; L.fsal = v + 2 * nmax; L.du = v + 3 * nmax; L.dT = v + 4 * nmax;
+    L.ks = v + 5 * nmax; L.lt = L.ks; L.dense = v + 13 * nmax;
+    off += (size_t)wg_nvec(saves) * nmax * 8;
+    L.pd = (double*)(base + off);
+    off += (size_t)kWGMaxNP * 8;
+    L.fsc = (float*)(base + off);
+    L.pf = L.fsc + nmax;
+    off += (size_t)(nmax + 32) * 4;
+    L.piv = (int*)(base + off);
+    off += (size_t)nmax * 4;
+    off = (off + 15) & ~(size_t)15;
+    L.red = (double*)(base + off);
+    L.ired = (int*)(L.red + 32);
+    return L;
+}
+
+// Block-wide argmax of v over threads (larger value wins, ties -> smaller index).
+__device__ __forceinline__ int wg_argmax(double v, int idx, double* red, int* ired) {
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    for (int off = 32; off >= 1; off >>= 1) {
+        double ov = __shfl_xor(v, off);
+        int oi = __shfl_xor(idx, off);
+        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    if (lane == 0) { red[w] = v; ired[w] = idx; }
+    __syncthreads();
+    double bv = red[0];
+    int bi = ired[0];
+    const int nw = blockDim.x >> 6;
+    for (int q = 1; q < nw; q++) {
+        double ov = red[q];
+        int oi = ired[q];
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    return bi;
+}
+
+// In-place LU with partial pivoting of the n x n leading block of A (stride ld).
+template <typename F>
+__device__ void wg_lu(F* A, int ld, int n, int* piv, double* red, int* ired) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    for (int k = 0; k < n; k++) {
+        // pivot: first maximum of |A[i][k]|, i >= k (a NaN diagonal keeps p = k)
+        double v = -1.0;
+        int idx = 0x7fffffff;
+        for (int i = k + tid; i < n; i += NT) {
+            double a = fabs((double)A[i * ld + k]);
+            if (!isnan(a) && (a > v || (a == v && i < idx))) { v = a; idx = i; }
+        }
+        int p = wg_argmax(v, idx, red, ired);
+        if (isnan((double)A[k * ld + k]) || p == 0x7fffffff) p = k;
+        if (tid == 0) piv[k] = p;
+        if (p != k) {
+            for (int c = tid; c < n; c += NT) {
+                F a = A[k * ld + c];
+                A[k * ld + c] = A[p * ld + c];
+                A[p * ld + c] = a;
+            }
+        }
+        __syncthreads();
+        const F inv = (F)1 / A[k * ld + k];
+        for (int i = k + 1 + tid; i < n; i += NT) A[i * ld + k] = A[i * ld + k] * inv;
+        __syncthreads();
+        const int m = n - k - 1;
+        const int cnt = m * m;
+        for (int e = tid; e < cnt; e += NT) {
+            const int i = k + 1 + e / m, c = k + 1 + e % m;
+            A[i * ld + c] = fma(-A[i * ld + k], A[k * ld + c], A[i * ld + c]);
+        }
+        __syncthreads();
+    }
+}
+
+// x <- A^{-1} x using the LU of wg_lu.  x is FP64 in LDS; the substitutions run
+// in precision F on wave 0 (rows held in registers, the solved entry broadcast
+// with readlane), in LAPACK's column order.  All threads must call.
+template <typename F>
+__device__ void wg_lu_solve(const F* A, int ld, int n, const int* piv, double* x) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        if (lane == 0)
+            for (int k = 0; k < n; k++) {
+                int p = piv[k];
+                if (p != k) { double t = x[k]; x[k] = x[p]; x[p] = t; }
+            }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        F b[ODEMI_WG_ROWS];
+#pragma unroll
+        for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+            int i = lane + 64 * r;
+            b[r] = (i < n) ? (F)x[i] : (F)0;
+        }
+        // forward, unit lower triangle
+        for (int c = 0; c < n - 1; c++) {
+            F bc = b[0];
+#pragma unroll
+            for (int r = 1; r < ODEMI_WG_ROWS; r++) if ((c >> 6) == r) bc = b[r];
+            F xc = __shfl(bc, c & 63);
+#pragma unroll
+            for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+                int i = lane + 64 * r;
+                if (i > c && i < n) b[r] = fma(-A[i * ld + c], xc, b[r]);
+            }
+        }
+        // backward, upper triangle
+        for (int c = n - 1; c >= 0; c--) {
+            F bc = b[0];
+#pragma unroll
+            for (int r = 1; r < ODEMI_WG_ROWS; r++) if ((c >> 6) == r) bc = b[r];
+            F xc = __shfl(bc, c & 63) / A[c * ld + c];
+#pragma unroll
+            for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+                int i = lane + 64 * r;
+                if (i == c) b[r] = xc;
+                else if (i < c) b[r] = fma(-A[i * ld + c], xc, b[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+            int i = lane + 64 * r;
+            if (i < n) x[i] = (double)b[r];
+        }
+    }
+    __syncthreads();
+}
+
+// Stage solve W x = b (b in `x` on entry; x must be visible to all threads).
+// FP64: one LU solve.  Mixed: FP32 LU solve plus refinement sweeps with the
+// FP64 residual r = b - J x + sigma x (J x from the traced Jacobian-vector
+// product at uprev).  Scratch: rb (copy of b), r, d -- three free vectors.
+template <typename F>
+__device__ void wg_wsolve(const WGPtrs& S, int n, double* x, double* rb, double* r, double* d,
+                          const double* uprev, const double* p, double t, double sigma, int sweeps) {
+    const F* A = (const F*)S.W;
+    if constexpr (sizeof(F) == sizeof(double)) {
+        wg_lu_solve<F>(A, S.ld, n, S.piv, x);
+    } else {
+        const int tid = threadIdx.x, NT = blockDim.x;
+        for (int i = tid; i < n; i += NT) rb[i] = x[i];
+        __syncthreads();
+        wg_lu_solve<F>(A, S.ld, n, S.piv, x);
+        for (int sw = 0; sw < sweeps; sw++) {
+            odemi_jvp_wg<double>(r, x, uprev, p, t, n, tid, NT);
+            __syncthreads();
+            for (int i = tid; i < n; i += NT) d[i] = fma(sigma, x[i], rb[i] - r[i]);
+            __syncthreads();
+            wg_lu_solve<F>(A, S.ld, n, S.piv, d);
+            for (int i = tid; i < n; i += NT) x[i] = x[i] + d[i];
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ void wg_save_point(Ctl& c, double& tlast, const Params& P, i64 j, double t,
+                                              const double* v, int n) {
+    const int s = c.nsaved;
+    if (s < P.max_saves) {
+        if (threadIdx.x == 0) P.t_out[(i64)s * P.N + j] = t;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) P.u_out[((i64)s * P.nmax + i) * P.N + j] = v[i];
+    }
+    c.nsaved = s + 1;
+    tlast = t;
+}
+
+// Index-order fma-chain norms, evaluated by one thread (bit-identical to the
+// sequential restatement).
+__device__ __forceinline__ double wg_residual_norm(const double* ut, const double* u0, const double* u1, int n,
+                                                   double abstol, double reltol) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) {
+        double den = fma(max_fast(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
+        double r = ut[i] / den;
+        s = fma(r, r, s);
+    }
+    return sqrt(s / (double)(n > 1 ? n : 1));
+}
+__device__ __forceinline__ double wg_norm_div(const double* a, const double* sk, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) { double x = a[i] / sk[i] * 1.0; s = fma(x, x, s); }
+    return sqrt(s / (double)(n > 1 ? n : 1));
+}
+__device__ __forceinline__ double wg_norm_diff_div(const double* a, const double* b, const double* sk, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) { double x = (a[i] - b[i]) / sk[i] * 1.0; s = fma(x, x, s); }
+    return sqrt(s / (double)(n > 1 ? n : 1));
+}
+
+// initdt.jl:17-298, cooperative: RHS evaluations by the whole workgroup, the
+// scalar logic and norms on thread 0, result broadcast.  Uses ks[0], du, dT,
+// fsal as scratch (sk, f0, f1, u1).
+__device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, const double* p, int n,
+                            double reltol, double abstol, double dtmin_opt) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const double t = c.t;
+    const double dtmax_tdir = jl_min(fabs(c.dtmax), fabs(c.tf - t));
+    const double dtmin = nextfloat_pos(jl_max(dtmin_opt, jl_eps(t)));
+    const double smalldt = jl_max(dtmin, 1e-6);
+    double* sk = S.lt;
+    double* f0 = S.du;
+    double* f1 = S.dT;
+    double* u1 = S.fsal;
+    double* red = S.red;
+    for (int i = tid; i < n; i += NT) sk[i] = fma(fabs(u0[i]), reltol, abstol);
+    odemi_f_wg<double>(f0, u0, p, t, n, tid, NT);
+    __syncthreads();
+    if (tid == 0) {
+        double d0 = wg_norm_div(u0, sk, n);
+        double d1 = wg_norm_div(f0, sk, n);
+        double res = 0.0, need = 0.0, dt0 = 0.0;
+        if (isnan(d1)) {
+            res = dtmin;
+        } else {
+            dt0 = ((d0 < 1e-5) | (d1 < 1e-5)) ? smalldt : (d0 / d1) / 100.0;
+            dt0 = jl_min(dt0, dtmax_tdir);
+            if (dt0 < 10.0 * 2.220446049250313e-16) res = jl_max(smalldt, dtmin);
+            else need = 1.0;
+        }
+        red[20] = res; red[21] = need; red[22] = dt0; red[23] = d1;
+    }
+    __syncthreads();
+    double res = red[20];
+    const bool need = red[21] != 0.0;
+    const double dt0 = red[22];
+    __syncthreads();
+    if (!need) return res;
+    for (int i = tid; i < n; i += NT) u1[i] = fma(dt0, f0[i], u0[i]);
+    __syncthreads();
+    odemi_f_wg<double>(f1, u1, p, t + dt0, n, tid, NT);
+    __syncthreads();
+    if (tid == 0) {
+        bool same = true;
+        for (int i = 0; i < n; i++) same = same && (f0[i] == f1[i]);
+        double r;
+        if (same) {
+            r = jl_max(dtmin, 100.0 * dt0);
+        } else {
+            double d1 = red[23];
+            double d2 = wg_norm_diff_div(f1, f0, sk, n) / dt0 * 1.0;
+            double mx = jl_max(d1, d2);
+            double dt1;
+            if (mx < 1e-15) dt1 = jl_max(1e-6, dt0 * 0.001);
+            else dt1 = det_exp(-(2.0 + det_log(mx) / kLN10) / 5.0 * kLN10);
+            r = jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
+        }
+        red[20] = r;
+    }
+    __syncthreads();
+    res = red[20];
+    __syncthreads();
+    return res;
+}
+
+template <typename F, bool SAVES>
+__device__ void rodas_wg_impl(const Params& P) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES);
+    const CtlConst kc = ctl_const(true);
+    double* u = S.u;
+    double* uprev = S.uprev;
+    double* fsal = S.fsal;
+    double* du = S.du;
+    double* dT = S.dT;
+    double* ks = S.ks;
+    double* dense = S.dense;
+    double* red = S.red;
+    const int nmax = P.nmax, ld = S.ld;
+    F* W = (F*)S.W;
+    for (;;) {
+        if (tid == 0) ((i64*)red)[30] = (i64)atomicAdd(&P.counters[0], 1ULL);
+        __syncthreads();
+        const i64 slot = ((i64*)red)[30];
+        __syncthreads();
+        if (slot >= P.N) break;
+        const i64 j = P.order ? (i64)P.order[slot] : slot;
+        const int n = P.nvec ? P.nvec[j] : nmax;
+        double* pp = S.pd;
+        float* pf = S.pf;
+        for (int k = tid; k < ODEMI_NP; k += NT) { pp[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; pf[k] = (float)pp[k]; }
+        for (int i = tid; i < n; i += NT) { double v = P.u0[(i64)i * P.N + j]; u[i] = v; uprev[i] = v; dT[i] = 0.0; }
+        Ctl c;
+        ctl_init(c, P, j);
+        double tlast = 0.0;
+        __syncthreads();
+        if (P.save_start) wg_save_point(c, tlast, P, j, c.t, u, n);
+        if (P.dt_user != 0.0) {
+            c.dt = P.dt_user;
+        } else {
+            c.dt = wg_initdt(S, c, u, pp, n, P.reltol, P.abstol, P.dtmin);
+            c.nf += 2;
+        }
+        c.dtpropose = c.dt;
+        int rc = RC_SUCCESS;
+        while (c.t < c.tf) {
+            // ---- loopheader! ----
+            if (c.iter > 0) {
+                if (c.accept_step) {
+                    c.success_iter++;
+                    for (int i = tid; i < n; i += NT) uprev[i] = u[i];
+                    c.dt = c.dtpropose;
+                    modify_dt_for_tstops(c);
+                } else {
+                    reject_controller(c, kc);
+                }
+            }
+            c.iter++;
+            fix_dt_at_bounds(c, P.dtmin);
+            modify_dt_for_tstops(c);
+            int bad = 0;
+            for (int i = tid; i < n; i += NT) bad |= !isfinite(u[i]);
+            const bool finite = !__syncthreads_or(bad);
+            rc = check_error(c, P, finite);
+            if (rc != RC_SUCCESS) break;
+            if (c.next_step_tstop && fabs(c.dt) < jl_eps(fabs(c.t))) {
+                c.accept_step = 1;
+            } else {
+                // ---- perform_step! (rosenbrock_perform_step.jl:729-847) ----
+                const double t = c.t, dt = c.dt;
+                const double idt = 1.0 / dt;
+                const double dtgamma = dt * RodasTab::gamma;
+                const double sigma = 1.0 / dtgamma;
+                odemi_f_wg<double>(fsal, uprev, pp, t, n, tid, NT);
+                c.nf += 1;
+                const bool new_jac = (c.iter <= 1) ? true : !(c.iter > 1 && c.EEst > 1.0);
+                for (int e = tid; e < n * ld; e += NT) W[e] = (F)0;
+                if (sizeof(F) != sizeof(double))
+                    for (int i = tid; i < n; i += NT) S.fsc[i] = (float)uprev[i];
+                __syncthreads();
+                if constexpr (sizeof(F) == sizeof(double)) {
+                    odemi_jac_wg<double>((double*)W, ld, uprev, pp, t, n, tid, NT);
+                } else {
+                    odemi_jac_wg<float>((float*)W, ld, S.fsc, pf, (float)t, n, tid, NT);
+                }
+                if (new_jac) {
+                    c.njacs += 1;
+                    for (int i = tid; i < n; i += NT) dT[i] = 0.0;
+                }
+                __syncthreads();
+                for (int i = tid; i < n; i += NT)
+                    W[i * ld + i] = (F)fma(-1.0, sigma, (double)W[i * ld + i]);
+                if (new_jac) {
+                    odemi_tgrad_wg<double>(dT, uprev, pp, t, n, tid, NT);
+                    if (!ODEMI_HAS_TGRAD) c.nf += 1;
+                }
+                c.nw += 1;
+                __syncthreads();
+                const double dtd0 = dt * kRd[0];
+                for (int i = tid; i < n; i += NT) ks[i] = fsal[i] + dtd0 * dT[i];
+                wg_lu<F>(W, ld, n, S.piv, red, S.ired);
+                // rb = fsal, r = du, d = u are free during the stage-1 solve
+                wg_wsolve<F>(S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                for (int i = tid; i < n; i += NT) ks[i] = -ks[i];
+                c.nsolve += 1;
+                __syncthreads();
+                for (int s = 1; s < 8; s++) {
+                    for (int i = tid; i < n; i += NT) {
+                        double acc = uprev[i];
+                        for (int q = 0; q < s; q++) acc = fma(kRA[s][q], ks[q * nmax + i], acc);
+                        u[i] = acc;
+                    }
+                    __syncthreads();
+                    odemi_f_wg<double>(du, u, pp, fma(kRc[s], dt, t), n, tid, NT);
+                    c.nf += 1;
+                    __syncthreads();
+                    const double dtds = dt * kRd[s];
+                    double* x = ks + s * nmax;
+                    for (int i = tid; i < n; i += NT) {
+                        double acc = fma(dtds, dT[i], du[i]);
+                        for (int q = 0; q < s; q++) acc = fma(kRC[s][q] * idt, ks[q * nmax + i], acc);
+                        x[i] = acc;
+                    }
+                    __syncthreads();
+                    wg_wsolve<F>(S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                    for (int i = tid; i < n; i += NT) x[i] = -x[i];
+                    c.nsolve += 1;
+                    __syncthreads();
+                }
+                for (int i = tid; i < n; i += NT) {
+                    double acc = uprev[i];
+                    for (int q = 0; q < 7; q++) acc = fma(kRA[7][q], ks[q * nmax + i], acc);
+                    acc = fma(1.0, ks[7 * nmax + i], acc);
+                    u[i] = acc;
+                    double e = 0.0;
+                    for (int q = 0; q < 7; q++) e = fma(0.0, ks[q * nmax + i], e);
+                    e = fma(1.0, ks[7 * nmax + i], e);
+                    du[i] = e;
+                }
+                if (SAVES && P.nsaveat > 0) {
+                    for (int r = 0; r < 3; r++)
+                        for (int i = tid; i < n; i += NT) {
+                            double acc = 0.0;
+                            for (int q = 0; q < 8; q++) acc = fma(kRH[r][q], ks[q * nmax + i], acc);
+                            dense[r * nmax + i] = acc;
+                        }
+                }
+                __syncthreads();
+                if (tid == 0) red[24] = wg_residual_norm(du, uprev, u, n, P.abstol, P.reltol);
+                __syncthreads();
+                c.EEst = red[24];
+            }
+            // ---- loopfooter! ----
+            const double ttmp = c.t + c.dt;
+            const double q = stepsize_controller(c, kc);
+            c.accept_step = (c.EEst <= 1.0);
+            if (c.accept_step) {
+                c.naccept++;
+                c.tprev = c.t;
+                if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
+                else c.t = ttmp;
+                accept_controller(c, kc, q, P.dtmin);
+                if (SAVES) {
+                    while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
+                        const double curt = P.saveat[c.spos++];
+                        if (curt != c.t) {
+                            const double th = (curt - c.tprev) / c.dt, th1 = 1.0 - th;
+                            const int sidx = c.nsaved;
+                            if (sidx < P.max_saves) {
+                                if (tid == 0) P.t_out[(i64)sidx * P.N + j] = curt;
+                                for (int i = tid; i < n; i += NT) {
+                                    double in1 = fma(th, dense[2 * nmax + i], dense[nmax + i]);
+                                    double in2 = fma(th, in1, dense[i]);
+                                    double in3 = fma(th1, in2, u[i]);
+                                    P.u_out[((i64)sidx * P.nmax + i) * P.N + j] = fma(th, in3, th1 * uprev[i]);
+                                }
+                            }
+                            c.nsaved = sidx + 1;
+                            tlast = curt;
+                        } else {
+                            if (curt == c.tf && !P.save_end) continue;
+                            wg_save_point(c, tlast, P, j, c.t, u, n);
+                        }
+                    }
+                    if (P.save_everystep) {
+                        if (c.nsaved == 0 || ((c.t != tlast || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
+                            wg_save_point(c, tlast, P, j, c.t, u, n);
+                    }
+                }
+            } else {
+                c.nreject++;
+            }
+        }
+        // ---- postamble! / result write-out ----
+        if (rc == RC_SUCCESS && P.save_end) {
+            bool doit = (c.nsaved == 0);
+            if (!doit)
+                doit = (tlast != c.t) && (P.save_end_user == 1 || c.tf_in_saveat || c.t == c.tf || P.nsaveat == 0);
+            if (doit) wg_save_point(c, tlast, P, j, c.t, u, n);
+        }
+        if (P.u_final)
+            for (int i = tid; i < nmax; i += NT) P.u_final[(i64)i * P.N + j] = (i < n) ? u[i] : 0.0;
+        if (tid == 0) {
+            if (P.t_final) P.t_final[j] = c.t;
+            if (P.nsaved) P.nsaved[j] = c.nsaved;
+            P.retcode[j] = rc;
+            if (P.stats) {
+                P.stats[ST_NF * P.N + j] = c.nf;
+                P.stats[ST_NACCEPT * P.N + j] = c.naccept;
+                P.stats[ST_NREJECT * P.N + j] = c.nreject;
+                P.stats[ST_NJACS * P.N + j] = c.njacs;
+                P.stats[ST_NW * P.N + j] = c.nw;
+                P.stats[ST_NSOLVE * P.N + j] = c.nsolve;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace odemi
+
+#endif  // ODEMI_WITH_WG

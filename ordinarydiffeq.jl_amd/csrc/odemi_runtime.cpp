@@ -119,6 +119,7 @@ struct odemi_rhs {
     hipFunction_t rodas_persist_f64 = nullptr, rodas_persist_mixed = nullptr;
     hipFunction_t rodas_plain_init = nullptr, rodas_plain_step = nullptr;
     hipFunction_t rodas_wg_f64 = nullptr, rodas_wg_mixed = nullptr;
+    hipFunction_t rodas_wg_f64_sv = nullptr, rodas_wg_mixed_sv = nullptr;
 };
 
 #define HIPCHK(ctx, call)                                                                        \
@@ -220,10 +221,11 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
              " { odemi::rodas_plain_step<ODEMI_N>(P); }\n";
     }
     if (flags & ODEMI_COMPILE_WG) {
-        s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) odemi_rodas_wg_f64(odemi::Params P)"
-             " { odemi::rodas_wg<double>(P); }\n";
-        s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) odemi_rodas_wg_mixed(odemi::Params P)"
-             " { odemi::rodas_wg<float>(P); }\n";
+        const char* v[4][3] = {{"odemi_rodas_wg_f64", "double", "false"}, {"odemi_rodas_wg_f64_sv", "double", "true"},
+                               {"odemi_rodas_wg_mixed", "float", "false"}, {"odemi_rodas_wg_mixed_sv", "float", "true"}};
+        for (auto& k : v)
+            s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) " << k[0]
+              << "(odemi::Params P) { odemi::rodas_wg_impl<" << k[1] << ", " << k[2] << ">(P); }\n";
     }
     return s.str();
 }
@@ -344,6 +346,8 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
     if (flags & ODEMI_COMPILE_WG) {
         get("odemi_rodas_wg_f64", &r->rodas_wg_f64);
         get("odemi_rodas_wg_mixed", &r->rodas_wg_mixed);
+        get("odemi_rodas_wg_f64_sv", &r->rodas_wg_f64_sv);
+        get("odemi_rodas_wg_mixed_sv", &r->rodas_wg_mixed_sv);
     }
     return r;
 }
@@ -400,6 +404,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     if (N <= 0) return 0;
     if (nmax != rhs->n) { ctx->err = "args.nmax does not match the compiled problem dimension"; return -2; }
     if (np > std::max(rhs->np, 1)) { ctx->err = "args.np exceeds the compiled parameter count"; return -2; }
+    if ((rhs->flags & ODEMI_COMPILE_WG) && np > 32) { ctx->err = "workgroup kernels take at most 32 parameters"; return -2; }
     if (a->max_saves < 0 || (a->max_saves > 0 && (!a->t_out || !a->u_out))) { ctx->err = "save buffers missing"; return -2; }
     if (!a->retcode) { ctx->err = "retcode output is required"; return -2; }
     if (a->nsaveat < 0 || (a->nsaveat > 0 && !a->saveat)) { ctx->err = "saveat buffer missing"; return -2; }
@@ -416,7 +421,8 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
             fn_main = mixed ? rhs->tsit5_persist_mixed_sv : rhs->tsit5_persist_f64_sv;
         else fn_main = mixed ? rhs->tsit5_persist_mixed : rhs->tsit5_persist_f64;
     } else if (use_wg) {
-        fn_main = mixed ? rhs->rodas_wg_mixed : rhs->rodas_wg_f64;
+        if (a->nsaveat > 0 || a->save_everystep) fn_main = mixed ? rhs->rodas_wg_mixed_sv : rhs->rodas_wg_f64_sv;
+        else fn_main = mixed ? rhs->rodas_wg_mixed : rhs->rodas_wg_f64;
     } else {
         if (plain) { fn_init = rhs->rodas_plain_init; fn_main = rhs->rodas_plain_step; }
         else fn_main = mixed ? rhs->rodas_persist_mixed : rhs->rodas_persist_f64;
@@ -533,13 +539,25 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         }
     } else {
         const int threads = use_wg ? 256 : block;   // ODEMI_WG_THREADS for the workgroup kernels
+        size_t shmem = 0;
+        if (use_wg) {
+            // must match odemi::wg_layout in kernels/odemi_wg.h
+            const size_t ld = (size_t)nmax + 1, fb = mixed ? 4 : 8;
+            const bool saves = a->nsaveat > 0 || a->save_everystep;
+            shmem = ((size_t)nmax * ld * fb + 15) & ~(size_t)15;
+            shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
+            shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 4;
+            shmem = (shmem + 15) & ~(size_t)15;
+            shmem += 32 * 8 + 32 * 4;
+            if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
+        }
         int nb = 0;
-        HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, threads, 0));
+        HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, threads, shmem));
         if (nb < 1) nb = 1;
         int64_t grid = (int64_t)nb * std::max(ctx->cu_count, 1);
         const int64_t per_block = use_wg ? 1 : threads;
         grid = std::min<int64_t>(grid, (N + per_block - 1) / per_block);
-        if (launch(ctx, fn_main, (int)grid, threads, P, s)) return -1;
+        if (launch(ctx, fn_main, (int)grid, threads, P, s, shmem)) return -1;
         launches++;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, s));

@@ -20,7 +20,7 @@ from fractions import Fraction
 import numpy as np
 
 from . import _lib
-from .tracer import trace_problem
+from .tracer import trace_problem, trace_problem_wg
 
 __all__ = ["ReturnCode", "ODEFunction", "ODEProblem", "remake", "Tsit5", "Rodas5P", "EnsembleMI355X",
            "EnsembleContext", "EnsembleProblem", "DEStats", "ODESolution", "EnsembleSolution", "solve",
@@ -166,18 +166,41 @@ def get_context(device=0):
 _TRACES = {}
 
 
-def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None):
-    """Trace fun (ODEFunction) for (n, np_) and hipRTC-compile it. Returns (rhs, traced, seconds)."""
-    key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_)
+LANE_MAX_RODAS = 4      # Rodas5P: lane-per-trajectory kernel up to this n, workgroup kernel above
+LANE_MAX_TSIT5 = 16
+
+_LANE_STUBS = """template <typename T> __device__ __forceinline__ void odemi_f(T* du, const T* u, const T* p, T t) {}
+template <typename T> __device__ __forceinline__ void odemi_jac(T* J, const T* u, const T* p, T t) {}
+template <typename T> __device__ __forceinline__ void odemi_tgrad(T* dT, const T* u, const T* p, T t) {}
+"""
+
+
+def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None):
+    """Trace fun (ODEFunction) for state size n (or the sizes `ns` of an uneven
+    batch, n = max) and hipRTC-compile it.  Returns (rhs, traced, seconds)."""
+    wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
+    if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
+        raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
+    if ns is not None and "tsit5" in algs:
+        raise ValueError("uneven-size batches (nvec) are supported with Rodas5P")
+    key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_, tuple(ns) if ns is not None else None, wg)
     tr = _TRACES.get(key)
     if tr is None:
-        tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad)
+        if wg:
+            tr = trace_problem_wg(fun.f, list(ns) if ns is not None else [n], np_, jac=fun.jac, tgrad=fun.tgrad)
+            tr.src = _LANE_STUBS + tr.src
+            tr.has_user_tgrad = fun.tgrad is not None
+        else:
+            tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad)
         _TRACES[key] = tr
     flags = 0
-    if "tsit5" in algs:
-        flags |= _lib.COMPILE_TSIT5
-    if "rodas5p" in algs:
-        flags |= _lib.COMPILE_RODAS5P
+    if wg:
+        flags |= _lib.COMPILE_WG
+    else:
+        if "tsit5" in algs:
+            flags |= _lib.COMPILE_TSIT5
+        if "rodas5p" in algs:
+            flags |= _lib.COMPILE_RODAS5P
     if tr.has_user_tgrad:
         flags |= _lib.HAS_USER_TGRAD
     ctx = ctx or get_context()
@@ -235,8 +258,9 @@ def _is_torch(x):
 
 def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abstol=None, saveat=None,
                  save_start=False, save_end=True, save_everystep=False, save_end_user=-1, max_saves=None,
-                 dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None):
-    """Bulk entry point on packed SoA arrays: u0 (n, N), p (np, N), t0/tf (N,).
+                 dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None, nvec=None):
+    """Bulk entry point on packed SoA arrays: u0 (nmax, N), p (np, N), t0/tf (N,);
+    nvec (N,) gives per-trajectory dimensions for uneven batches (Rodas5P).
 
     Arrays may be numpy (host) or torch CUDA tensors (device-resident, zero
     copies).  Returns a dict of output arrays of the same kind."""
@@ -247,7 +271,13 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
     n, N = int(u0.shape[0]), int(u0.shape[1])
     np_ = int(p.shape[0]) if p is not None else 0
     algname = "tsit5" if alg.code == _lib.TSIT5 else "rodas5p"
-    h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx)
+    ns = None
+    if nvec is not None:
+        nv_host = nvec.cpu().numpy() if _is_torch(nvec) else np.asarray(nvec)
+        ns = sorted(set(int(x) for x in nv_host))
+        if ns[-1] > n or ns[0] < 1:
+            raise ValueError("nvec entries must lie in [1, u0.shape[0]]")
+    h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx, ns=ns)
     reltol = 1e-3 if reltol is None else float(reltol)
     abstol = 1e-6 if abstol is None else float(abstol)
     sa = np.zeros(0) if saveat is None else np.asarray(saveat, dtype=np.float64)
@@ -290,6 +320,10 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
             stats=out.get("stats", torch.empty((6, N), dtype=torch.int32, device=devc)),
         )
         keep += [u0c, pc, t0c, tfc, sat]
+        if nvec is not None:
+            nvc = torch.as_tensor(nvec, dtype=torch.int32, device=devc).contiguous()
+            keep.append(nvc)
+            a.nvec = nvc.data_ptr()
         ptr = lambda x: x.data_ptr()
         a.device_ptrs = 1
         a.stream = torch.cuda.current_stream(devc).cuda_stream
@@ -303,6 +337,10 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
                  nsaved=np.empty(N, dtype=np.int32), u_final=np.empty((n, N)), t_final=np.empty(N),
                  retcode=np.empty(N, dtype=np.int32), stats=np.empty((6, N), dtype=np.int32))
         keep += [u0c, pc, t0c, tfc, sat]
+        if nvec is not None:
+            nvc = np.ascontiguousarray(nvec, dtype=np.int32)
+            keep.append(nvc)
+            a.nvec = nvc.ctypes.data
         ptr = lambda x: x.ctypes.data
         a.device_ptrs = 0
     a.u0, a.p, a.t0, a.tf = ptr(u0c), ptr(pc), ptr(t0c), ptr(tfc)

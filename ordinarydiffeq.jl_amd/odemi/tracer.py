@@ -434,3 +434,188 @@ def trace_problem(f, n, np_, jac=None, tgrad=None):
                              _count_flops(fout), _count_flops(jout))
     finally:
         _CUR = None
+
+
+# ---------------------------------------------------------------------------
+# Workgroup forms for uneven batches: one cooperative function per state size,
+# dispatched by a switch on n.  Outputs whose expressions have the same shape
+# relative to their row index are "rerolled" into thread-strided loops, so a
+# 64-spring chain becomes a handful of loops instead of 128 serial statements.
+# Each output keeps its own operation order, so values are identical to the
+# straight-line form.
+# ---------------------------------------------------------------------------
+def _rel_sig(node, row, memo):
+    """Shape of `node` with u/v leaf indices made relative to `row`."""
+    r = memo.get(node.id)
+    if r is not None:
+        return r
+    op = node.op
+    if op in ("u", "v"):
+        r = f"{op}[{node.val - row}]"
+    elif op == "p":
+        r = f"p[{node.val}]"
+    elif op == "t":
+        r = "t"
+    elif op == "const":
+        r = repr(float(node.val))
+    else:
+        r = op + "(" + ",".join(_rel_sig(a, row, memo) for a in node.args) + ")"
+    memo[node.id] = r
+    return r
+
+
+def _group_runs(entries):
+    """entries: list of (row, col_or_None, expr).  Returns runs
+    [(row0, dcol, count, expr_of_first)] of same-shape outputs with row stepping by 1."""
+    groups = {}
+    for row, col, e in entries:
+        if e.op == "const" and e.val == 0.0 and col is not None:
+            continue  # Jacobian zeros are pre-filled
+        key = (None if col is None else col - row, _rel_sig(e, row, {}))
+        groups.setdefault(key, []).append((row, col, e))
+    runs = []
+    for key, items in groups.items():
+        items.sort(key=lambda x: x[0])
+        start = 0
+        for i in range(1, len(items) + 1):
+            if i == len(items) or items[i][0] != items[i - 1][0] + 1:
+                r0, c0, e0 = items[start]
+                runs.append((r0, None if c0 is None else c0 - r0, i - start, e0))
+                start = i
+    runs.sort(key=lambda r: (-r[2], r[0]))
+    return runs
+
+
+def _emit_rel_body(root, row0, out_expr, indent):
+    """Temporaries of one expression with u/v leaves indexed relative to `o`."""
+    needed = set()
+    stack = [root]
+    while stack:
+        n = stack.pop()
+        if n.id in needed:
+            continue
+        needed.add(n.id)
+        stack.extend(n.args)
+    names, lines = {}, []
+    for n in root.g.order:
+        if n.id not in needed:
+            continue
+        op = n.op
+        if op == "const":
+            names[n.id] = _hexf(n.val)
+            continue
+        if op in ("u", "v"):
+            off = n.val - row0
+            names[n.id] = f"{op}[o + ({off})]" if off else f"{op}[o]"
+            continue
+        if op == "p":
+            names[n.id] = f"p[{n.val}]"
+            continue
+        if op == "t":
+            names[n.id] = "t"
+            continue
+        a = [names[x.id] for x in n.args]
+        if op == "add": e = f"{a[0]} + {a[1]}"
+        elif op == "sub": e = f"{a[0]} - {a[1]}"
+        elif op == "mul": e = f"{a[0]} * {a[1]}"
+        elif op == "div": e = f"{a[0]} / {a[1]}"
+        elif op == "neg": e = f"-{a[0]}"
+        elif op == "pow": e = f"pow({a[0]}, {a[1]})"
+        elif op == "sign": e = f"T(({a[0]} > T(0)) - ({a[0]} < T(0)))"
+        else: e = f"{_FN[op]}({a[0]})"
+        lines.append(f"{indent}const T v{n.id} = {e};")
+        names[n.id] = f"v{n.id}"
+    lines.append(f"{indent}{out_expr} = {names[root.id]};")
+    return "\n".join(lines)
+
+
+def _emit_runs(runs, store):
+    """store(dcol) -> C lvalue template using `o` (row index)."""
+    out = []
+    for r0, dcol, cnt, e in runs:
+        out.append(f"        for (int q = tid; q < {cnt}; q += nt) {{")
+        out.append(f"            const int o = {r0} + q;")
+        out.append(_emit_rel_body(e, r0, store(dcol), "            "))
+        out.append("        }")
+    return "\n".join(out)
+
+
+class TracedProblemWG:
+    def __init__(self, ns, np_, src, flops_f, flops_j, nnz):
+        self.ns, self.np, self.src = ns, np_, src
+        self.flops_f, self.flops_j, self.nnz = flops_f, flops_j, nnz
+
+
+def trace_problem_wg(f, ns, np_, jac=None, tgrad=None):
+    """Workgroup forms for every state size in `ns` (distinct, sorted)."""
+    global _CUR
+    ns = sorted(set(int(n) for n in ns))
+    fcases, jcases, tcases, vcases = [], [], [], []
+    flops_f, flops_j, nnz = {}, {}, {}
+    for n in ns:
+        g = _Graph()
+        _CUR = g
+        try:
+            u = [g.make("u", (), i) for i in range(n)]
+            v = [g.make("v", (), i) for i in range(n)]
+            p = [g.make("p", (), k) for k in range(np_)]
+            t = g.make("t", (), None)
+            du = _Recorder((n,))
+            f(du, u, p, t)
+            fout = [(i, du.vals.get(i, _lift(0.0))) for i in range(n)]
+            if jac is not None:
+                J = _Recorder((n, n))
+                jac(J, u, p, t)
+                jent = [(i, j, e) for (i, j), e in J.vals.items()]
+            else:
+                jent = []
+                for i in range(n):
+                    for j in range(n):
+                        d = _deriv(fout[i][1], u[j], {})
+                        if not _is0(d):
+                            jent.append((i, j, d))
+            if tgrad is not None:
+                dT = _Recorder((n,))
+                tgrad(dT, u, p, t)
+                tout = [(i, dT.vals.get(i, _lift(0.0))) for i in range(n)]
+            else:
+                tout = [(i, _deriv(fout[i][1], t, {})) for i in range(n)]
+            # J v, summed over each row's nonzeros in increasing column order
+            rows = {}
+            for i, j, e in jent:
+                if not (e.op == "const" and e.val == 0.0):
+                    rows.setdefault(i, []).append((j, e))
+            vout = []
+            for i in range(n):
+                acc = None
+                for j, e in sorted(rows.get(i, []), key=lambda x: x[0]):
+                    term = e * v[j]
+                    acc = term if acc is None else acc + term
+                vout.append((i, acc if acc is not None else _lift(0.0)))
+            flops_f[n] = _count_flops(fout)
+            flops_j[n] = _count_flops([(0, e) for _, _, e in jent])
+            nnz[n] = len(rows and [1 for i in rows for _ in rows[i]])
+            fr = _group_runs([(i, None, e) for i, e in fout])
+            jr = _group_runs(jent)
+            tr = _group_runs([(i, None, e) for i, e in tout if not _is0(e)])
+            vr = _group_runs([(i, None, e) for i, e in vout])
+            fcases.append(f"    case {n}: {{\n{_emit_runs(fr, lambda d: 'du[o]')}\n        break;\n    }}")
+            jcases.append(f"    case {n}: {{\n{_emit_runs(jr, lambda d: f'J[o * ld + o + ({d})]')}\n        break;\n    }}")
+            tcases.append(f"    case {n}: {{\n{_emit_runs(tr, lambda d: 'dT[o]')}\n        break;\n    }}")
+            vcases.append(f"    case {n}: {{\n{_emit_runs(vr, lambda d: 'Jv[o]')}\n        break;\n    }}")
+        finally:
+            _CUR = None
+    hdr = "template <typename T>\n__device__ __noinline__ void "
+    args = "const T* __restrict__ u, const T* __restrict__ p, T t, int n, int tid, int nt"
+    src = "\n".join([
+        hdr + f"odemi_f_wg(T* __restrict__ du, {args}) {{\n    (void)u; (void)p; (void)t;\n    switch (n) {{\n"
+        + "\n".join(fcases) + "\n    default: break;\n    }\n}",
+        hdr + f"odemi_jac_wg(T* __restrict__ J, int ld, {args}) {{\n    (void)u; (void)p; (void)t;\n    switch (n) {{\n"
+        + "\n".join(jcases) + "\n    default: break;\n    }\n}",
+        hdr + f"odemi_tgrad_wg(T* __restrict__ dT, {args}) {{\n    (void)u; (void)p; (void)t;\n    switch (n) {{\n"
+        + "\n".join(tcases) + "\n    default: break;\n    }\n}",
+        hdr + "odemi_jvp_wg(T* __restrict__ Jv, const T* __restrict__ v, " + args
+        + ") {\n    (void)u; (void)v; (void)p; (void)t;\n    switch (n) {\n"
+        + "\n".join(vcases) + "\n    default: break;\n    }\n}",
+    ]) + "\n"
+    return TracedProblemWG(ns, np_, src, flops_f, flops_j, nnz)

@@ -1,0 +1,83 @@
+"""GPU parity of the workgroup-per-trajectory Rodas5P kernel (uneven batches,
+n up to 128) against the CPU restatement, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+import odemi
+from odemi import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _chain(N, m_lo=4, m_hi=12, seed=4):
+    d = P.chain_ensemble(N=N, seed=seed, m_lo=m_lo, m_hi=m_hi)
+    d["tf"] = np.full(N, 1.0)
+    return d
+
+
+def _oracle(d, **kw):
+    return O.ensemble_builtin(O.PROB_CHAIN, d["u0"], d["p"], d["t0"], d["tf"], nvec=d["nvec"], alg=O.RODAS5P,
+                              reltol=d["reltol"], abstol=d["abstol"], save_start=False, **kw)
+
+
+def test_chain_uneven_fp64_bit_identical():
+    d = _chain(256)
+    print("[case] chain uneven n 8..24 wg fp64", flush=True)
+    ref = _oracle(d)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"])
+    assert (out["retcode"] == 1).all()
+    same = (out["stats"][1] == ref["naccept"]) & (out["stats"][2] == ref["nreject"])
+    exact = np.all(out["u_final"] == ref["u_final"], axis=0)
+    print(f"steps equal {same.mean():.4f} exact {exact.mean():.4f}")
+    assert same.mean() >= 0.99 and exact.mean() >= 0.99
+    assert (out["stats"][0] == ref["nf"]).all()
+
+
+def test_chain_large_n_fp64_and_mixed():
+    d = _chain(64, m_lo=48, m_hi=64, seed=9)
+    print("[case] chain n 96..128 wg fp64 + mixed", flush=True)
+    ref = _oracle(d)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"])
+    rel = np.abs(out["u_final"] - ref["u_final"]) / np.maximum(np.abs(ref["u_final"]), 1e-300)
+    same = (out["stats"][1] == ref["naccept"]) & (out["stats"][2] == ref["nreject"])
+    print(f"fp64 steps equal {same.mean():.4f} exact {np.mean(np.all(out['u_final'] == ref['u_final'], axis=0)):.4f}")
+    assert same.mean() >= 0.99
+    assert np.quantile(rel.max(axis=0), 0.99) <= 1e-10
+    mx = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(precision="mixed"), reltol=d["reltol"], abstol=d["abstol"],
+                            nvec=d["nvec"])
+    assert (mx["retcode"] == 1).all()
+    err = np.abs(mx["u_final"] - ref["u_final"]) / (d["abstol"] + d["reltol"] * np.abs(ref["u_final"]))
+    print(f"mixed scaled err p99 {np.quantile(err.max(axis=0), 0.99):.3e} max {err.max():.3e}")
+    assert np.quantile(err.max(axis=0), 0.99) <= 10.0
+
+
+def test_wg_kernel_robertson_matches_lane_kernel():
+    """n = 3 through the workgroup kernel (nvec given) equals the lane kernel bit for bit."""
+    d = P.robertson_ensemble(N=128)
+    print("[case] robertson via wg kernel", flush=True)
+    lane = odemi.solve_arrays(P.ROBERTSON, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                              abstol=d["abstol"])
+    wg = odemi.solve_arrays(P.ROBERTSON, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                            abstol=d["abstol"], nvec=np.full(128, 3, dtype=np.int32))
+    assert np.array_equal(lane["u_final"], wg["u_final"])
+    assert np.array_equal(lane["stats"], wg["stats"])
+
+
+def test_wg_saveat_bit_identical():
+    d = _chain(32)
+    sa = np.linspace(0.1, 0.9, 5)
+    print("[case] chain wg saveat", flush=True)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"], saveat=sa, save_start=True, save_end=True)
+    for j in range(0, 32, 5):
+        n = int(d["nvec"][j])
+        r = O.solve_builtin(O.PROB_CHAIN, d["u0"][:n, j], d["p"][:, j], (0.0, 1.0), alg=O.RODAS5P,
+                            reltol=d["reltol"], abstol=d["abstol"], saveat=sa, save_start=True, save_end=True)
+        ns = int(out["nsaved"][j])
+        assert ns == r["nsaved"]
+        assert np.array_equal(out["t_out"][:ns, j], r["t"])
+        assert np.array_equal(out["u_out"][:ns, :n, j], r["u"])
