@@ -35,12 +35,116 @@ def tsit5_flops_init(F_f, n):
     return 3 * F_f + 12 * n
 
 
+def rodas_flops_per_attempt(F_f, F_J, n, saves=False):
+    """Algorithmic flops of one Rodas5P step attempt: 8 RHS + J + LU + 8 solves +
+    stage/update vectors + controller (dense rows +48n when saving)."""
+    return 8 * F_f + F_J + (2 * n ** 3 / 3 - n ** 2 / 2 - n / 6) + 16 * n ** 2 + 175 * n + 60 + (48 * n if saves else 0)
+
+
+def _timed(run, steps, warmup, torch):
+    out = None
+    for _ in range(warmup):
+        out = run()
+    torch.cuda.synchronize()
+    times, kms = [], []
+    for s in range(steps):
+        torch.cuda.synchronize()
+        t_a = time.perf_counter()
+        out = run()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t_a)
+        kms.append(out["kernel_ms"])
+        print(f"[bench] step {s} {times[-1]*1e3:.2f} ms (kernel {kms[-1]:.2f} ms)", file=sys.stderr, flush=True)
+    return out, float(np.median(times)) * 1e3, float(np.median(kms))
+
+
+def bench_rodas(args, workload):
+    """Robertson (262,144 trajectories, lane kernel) or the uneven oscillator-chain
+    batch (16,384 trajectories, n = 8..128, workgroup kernel): FP64 LU engine vs
+    FP32 LU + FP64 refinement engine, CPU restatement on a bounded sample."""
+    import torch
+    import odemi
+    from odemi import problems as P
+    dev = torch.device("cuda:0")
+    if workload == "robertson":
+        N = args.N or 262144
+        d = P.robertson_ensemble(N=N)
+        fun, pid = P.ROBERTSON, 1
+        nvec = None
+        desc = ("Robertson ensemble, 262,144 trajectories, Rodas5P, k=(0.04,3e7,1e4)*LogU[0.5,2], u0=(1,0,0), "
+                "tspan (0,1e5), reltol 1e-6, abstol 1e-10, analytic 3x3 J")
+    else:
+        N = args.N or 16384
+        d = P.chain_ensemble(N=N)
+        fun, pid = P.CHAIN, 3
+        nvec = d["nvec"]
+        desc = ("Uneven oscillator chains, 16,384 trajectories, n=2m, m~U{4..64}, k~U[1,1e4], c~U[0.01,1], "
+                "alpha~U[0,1], x0~N(0,0.1^2), tspan (0,10), Rodas5P, reltol=abstol=1e-6, one launch")
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()
+    u0, p, t0, tf = T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"])
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=True, nvec=nvec)
+    mixed = odemi.EnsembleMI355X(precision="mixed")
+    fp64 = odemi.EnsembleMI355X(precision="fp64")
+    run = lambda ea: odemi.solve_arrays(fun, odemi.Rodas5P(), u0, p, t0, tf, ea, **kw)
+    out, ms, kms = _timed(lambda: run(mixed), args.steps, args.warmup, torch)
+    st = out["stats"].cpu().numpy()
+    att = (st[1] + st[2]).astype(np.float64)
+    tr = out["traced"]
+    if nvec is None:
+        fl = att * rodas_flops_per_attempt(tr.flops_f, tr.flops_j, 3)
+        nrow = 3
+    else:
+        fl = np.array([att[j] * rodas_flops_per_attempt(tr.flops_f[int(nvec[j])], tr.flops_j[int(nvec[j])],
+                                                         int(nvec[j])) for j in range(N)])
+        nrow = int(np.max(nvec))
+    flops = float(fl.sum())
+    res = {
+        "metric": "ensemble trajectories/s at reltol 1e-6 (Rodas5P), FP32 LU + FP64 refinement engine",
+        "value": N / (ms / 1e3), "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 LU + f64 refinement, f64 state/RHS/controller", "data": "synthetic (seeded PCG64)",
+        "config": {"workload": desc, "trajectories": N,
+                   "parallelism": "one MI355X, " + ("lane per trajectory" if nvec is None else
+                                                    "workgroup per trajectory") + ", work queue"},
+        "step_attempts": int(att.sum()), "success_fraction": float((out["retcode"].cpu().numpy() == 1).mean()),
+        "roofline": {"bound": "valu-fp64" if nvec is None else "lds/valu-fp32", "achieved": flops / (kms / 1e3) / 1e12,
+                     "peak": PEAK_FP64_TFLOPS if nvec is None else PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": flops / (kms / 1e3) / 1e12 / (PEAK_FP64_TFLOPS if nvec is None else PEAK_FP32_TFLOPS),
+                     "traffic": None, "kernel_ms": kms,
+                     "kernel": "odemi_rodas_persist_mixed" if nvec is None else "odemi_rodas_wg_mixed"},
+    }
+    o64, ms64, kms64 = _timed(lambda: run(fp64), 1, 0, torch)
+    res["persistent_fp64"] = {"trajectories_per_s": N / (ms64 / 1e3), "ms": ms64, "kernel_ms": kms64}
+    u64 = o64["u_final"].cpu().numpy()
+    um = out["u_final"].cpu().numpy()
+    err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
+    res["mixed_vs_fp64_scaled_err"] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
+                                       "max": float(err.max())}
+    if workload == "robertson" and not args.no_plain:
+        plain = odemi.EnsembleMI355X(engine="plain")
+        op, msp, _ = _timed(lambda: run(plain), 1, 0, torch)
+        res["plain_fp64"] = {"trajectories_per_s": N / (msp / 1e3), "ms": msp, "launches": int(op["launches"])}
+        res["batched_over_plain"] = (N / (ms / 1e3)) / (N / (msp / 1e3))
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        S = min(args.cpu_sample if workload == "robertson" else 256, N)
+        print(f"[bench] cpu baseline {S} trajectories on {threads} threads", file=sys.stderr, flush=True)
+        r = O.ensemble_builtin(pid, d["u0"][:, :S], d["p"][:, :S], d["t0"][:S], d["tf"][:S],
+                               nvec=None if nvec is None else nvec[:S], alg=O.RODAS5P, reltol=d["reltol"],
+                               abstol=d["abstol"], save_start=False, threads=threads, chunk=1 if nvec is not None else 64)
+        res["cpu_baseline"] = {"value": S / r["seconds"], "unit": "trajectories/s", "cores": threads, "kind": "port",
+                               "sample": f"first {S} trajectories of the same batch, {r['seconds']:.2f} s wall"}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--N", type=int, default=1048576)
+    ap.add_argument("--N", type=int, default=0)
+    ap.add_argument("--workload", default="lorenz", choices=["lorenz", "robertson", "chain"])
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1048576)
@@ -49,6 +153,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload != "lorenz":
+        return bench_rodas(args, args.workload)
+    args.N = args.N or 1048576
     import torch
     import odemi
     from odemi import problems as P

@@ -12,7 +12,7 @@
 //   vec  double[NV][nmax]                 u, uprev, fsal, du, dT, ks[8] (+ dense[3]);
 //   pd   double[32]                       parameters (FP64)
 //   fsc  float[nmax + 32]                 FP32 copies of uprev and p (mixed variant)
-//   piv  int[nmax]
+//   piv  int[nmax], perm int[nmax]   pivots and the equivalent row gather
 //   red  double[32] + int[32]             reductions / broadcasts
 // FP64 variant: W is FP64 (132 KB at n = 128, one workgroup per CU).
 // Mixed variant: W is FP32 and every stage solve gets `refine_sweeps` FP64
@@ -40,6 +40,7 @@ struct WGPtrs {
     float* fsc;
     float* pf;
     int* piv;
+    int* perm;   // row gather: after the LU, (P b)[i] = b[perm[i]]
     double* red;
     int* ired;
     int ld;
@@ -62,6 +63,8 @@ __device__ __forceinline__ WGPtrs wg_layout(unsigned char* base, int nmax, bool 
     L.pf = L.fsc + nmax;
     off += (size_t)(nmax + 32) * 4;
     L.piv = (int*)(base + off);
+    off += (size_t)nmax * 4;
+    L.perm = (int*)(base + off);
     off += (size_t)nmax * 4;
     off = (off + 15) & ~(size_t)15;
     L.red = (double*)(base + off);
@@ -90,10 +93,23 @@ __device__ __forceinline__ int wg_argmax(double v, int idx, double* red, int* ir
     return bi;
 }
 
+// Broadcast a value held by lane `src` (wave-uniform) to the whole wave.
+__device__ __forceinline__ float wave_bcast(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double wave_bcast(double v, int src) {
+    i64 b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((i64)hi << 32) | (i64)(unsigned)lo);
+}
+
 // In-place LU with partial pivoting of the n x n leading block of A (stride ld).
+#define ODEMI_WG_CB ((ODEMI_N + 15) / 16)
 template <typename F>
-__device__ void wg_lu(F* A, int ld, int n, int* piv, double* red, int* ired) {
+__device__ void wg_lu(F* A, int ld, int n, int* piv, int* perm, double* red, int* ired) {
     const int tid = threadIdx.x, NT = blockDim.x;
+    for (int i = tid; i < n; i += NT) perm[i] = i;
     for (int k = 0; k < n; k++) {
         // pivot: first maximum of |A[i][k]|, i >= k (a NaN diagonal keeps p = k)
         double v = -1.0;
@@ -104,7 +120,10 @@ __device__ void wg_lu(F* A, int ld, int n, int* piv, double* red, int* ired) {
         }
         int p = wg_argmax(v, idx, red, ired);
         if (isnan((double)A[k * ld + k]) || p == 0x7fffffff) p = k;
-        if (tid == 0) piv[k] = p;
+        if (tid == 0) {
+            piv[k] = p;
+            int t = perm[k]; perm[k] = perm[p]; perm[p] = t;
+        }
         if (p != k) {
             for (int c = tid; c < n; c += NT) {
                 F a = A[k * ld + c];
@@ -113,14 +132,33 @@ __device__ void wg_lu(F* A, int ld, int n, int* piv, double* red, int* ired) {
             }
         }
         __syncthreads();
+        // multipliers and trailing update on a 16 x (NT/16) thread grid.  The 16
+        // threads of one row are lanes of one wave: they all read the unscaled
+        // A[i][k] before lane tj == 0 stores the multiplier.  Each thread keeps
+        // its slice of the pivot row in registers.
         const F inv = (F)1 / A[k * ld + k];
-        for (int i = k + 1 + tid; i < n; i += NT) A[i * ld + k] = A[i * ld + k] * inv;
-        __syncthreads();
-        const int m = n - k - 1;
-        const int cnt = m * m;
-        for (int e = tid; e < cnt; e += NT) {
-            const int i = k + 1 + e / m, c = k + 1 + e % m;
-            A[i * ld + c] = fma(-A[i * ld + k], A[k * ld + c], A[i * ld + c]);
+        const int ti = tid >> 4, tj = tid & 15, TI = NT >> 4;
+        F pr[ODEMI_WG_CB];
+#pragma unroll
+        for (int b = 0; b < ODEMI_WG_CB; b++) {
+            const int c = k + 1 + tj + 16 * b;
+            pr[b] = (c < n) ? A[k * ld + c] : (F)0;
+        }
+        for (int i = k + 1 + ti; i < n; i += TI) {
+            F* row = A + i * ld;
+            const F l = row[k] * inv;
+            F a[ODEMI_WG_CB];
+#pragma unroll
+            for (int b = 0; b < ODEMI_WG_CB; b++) {
+                const int c = k + 1 + tj + 16 * b;
+                a[b] = (c < n) ? row[c] : (F)0;
+            }
+#pragma unroll
+            for (int b = 0; b < ODEMI_WG_CB; b++) {
+                const int c = k + 1 + tj + 16 * b;
+                if (c < n) row[c] = fma(-l, pr[b], a[b]);
+            }
+            if (tj == 0) row[k] = l;
         }
         __syncthreads();
     }
@@ -130,46 +168,72 @@ __device__ void wg_lu(F* A, int ld, int n, int* piv, double* red, int* ired) {
 // in precision F on wave 0 (rows held in registers, the solved entry broadcast
 // with readlane), in LAPACK's column order.  All threads must call.
 template <typename F>
-__device__ void wg_lu_solve(const F* A, int ld, int n, const int* piv, double* x) {
+__device__ void wg_lu_solve(const F* A, int ld, int n, const int* perm, double* x) {
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        if (lane == 0)
-            for (int k = 0; k < n; k++) {
-                int p = piv[k];
-                if (p != k) { double t = x[k]; x[k] = x[p]; x[p] = t; }
-            }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
+        // row interchanges of the LU applied as one gather (same result as the
+        // sequential swaps b[k] <-> b[piv[k]], k = 0..n-1)
         F b[ODEMI_WG_ROWS];
 #pragma unroll
         for (int r = 0; r < ODEMI_WG_ROWS; r++) {
             int i = lane + 64 * r;
-            b[r] = (i < n) ? (F)x[i] : (F)0;
+            b[r] = (i < n) ? (F)x[perm[i]] : (F)0;
         }
-        // forward, unit lower triangle
-        for (int c = 0; c < n - 1; c++) {
-            F bc = b[0];
+        // forward, unit lower triangle (column order); the solved entry is
+        // broadcast with readlane (uniform lane), the factor column is read
+        // one step ahead so the dependent chain is readlane + fma
+        F Lc[ODEMI_WG_ROWS], Ln[ODEMI_WG_ROWS];
 #pragma unroll
-            for (int r = 1; r < ODEMI_WG_ROWS; r++) if ((c >> 6) == r) bc = b[r];
-            F xc = __shfl(bc, c & 63);
+        for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+            int i = lane + 64 * r;
+            Lc[r] = (i < n && n > 1) ? A[i * ld + 0] : (F)0;
+        }
+        for (int c = 0; c < n - 1; c++) {
 #pragma unroll
             for (int r = 0; r < ODEMI_WG_ROWS; r++) {
                 int i = lane + 64 * r;
-                if (i > c && i < n) b[r] = fma(-A[i * ld + c], xc, b[r]);
+                Ln[r] = (i < n && c + 1 < n - 1) ? A[i * ld + c + 1] : (F)0;
             }
-        }
-        // backward, upper triangle
-        for (int c = n - 1; c >= 0; c--) {
             F bc = b[0];
 #pragma unroll
             for (int r = 1; r < ODEMI_WG_ROWS; r++) if ((c >> 6) == r) bc = b[r];
-            F xc = __shfl(bc, c & 63) / A[c * ld + c];
+            const F xc = wave_bcast(bc, c & 63);
+#pragma unroll
+            for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+                int i = lane + 64 * r;
+                if (i > c && i < n) b[r] = fma(-Lc[r], xc, b[r]);
+                Lc[r] = Ln[r];
+            }
+        }
+        // backward, upper triangle (column order)
+        F Uc[ODEMI_WG_ROWS], Un[ODEMI_WG_ROWS];
+        F dc = A[(n - 1) * ld + (n - 1)], dn = (F)1;
+#pragma unroll
+        for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+            int i = lane + 64 * r;
+            Uc[r] = (i < n) ? A[i * ld + (n - 1)] : (F)0;
+        }
+        for (int c = n - 1; c >= 0; c--) {
+            if (c > 0) {
+                dn = A[(c - 1) * ld + (c - 1)];
+#pragma unroll
+                for (int r = 0; r < ODEMI_WG_ROWS; r++) {
+                    int i = lane + 64 * r;
+                    Un[r] = (i < n) ? A[i * ld + (c - 1)] : (F)0;
+                }
+            }
+            F bc = b[0];
+#pragma unroll
+            for (int r = 1; r < ODEMI_WG_ROWS; r++) if ((c >> 6) == r) bc = b[r];
+            const F xc = wave_bcast(bc, c & 63) / dc;
 #pragma unroll
             for (int r = 0; r < ODEMI_WG_ROWS; r++) {
                 int i = lane + 64 * r;
                 if (i == c) b[r] = xc;
-                else if (i < c) b[r] = fma(-A[i * ld + c], xc, b[r]);
+                else if (i < c) b[r] = fma(-Uc[r], xc, b[r]);
+                Uc[r] = Un[r];
             }
+            dc = dn;
         }
 #pragma unroll
         for (int r = 0; r < ODEMI_WG_ROWS; r++) {
@@ -189,18 +253,18 @@ __device__ void wg_wsolve(const WGPtrs& S, int n, double* x, double* rb, double*
                           const double* uprev, const double* p, double t, double sigma, int sweeps) {
     const F* A = (const F*)S.W;
     if constexpr (sizeof(F) == sizeof(double)) {
-        wg_lu_solve<F>(A, S.ld, n, S.piv, x);
+        wg_lu_solve<F>(A, S.ld, n, S.perm, x);
     } else {
         const int tid = threadIdx.x, NT = blockDim.x;
         for (int i = tid; i < n; i += NT) rb[i] = x[i];
         __syncthreads();
-        wg_lu_solve<F>(A, S.ld, n, S.piv, x);
+        wg_lu_solve<F>(A, S.ld, n, S.perm, x);
         for (int sw = 0; sw < sweeps; sw++) {
             odemi_jvp_wg<double>(r, x, uprev, p, t, n, tid, NT);
             __syncthreads();
             for (int i = tid; i < n; i += NT) d[i] = fma(sigma, x[i], rb[i] - r[i]);
             __syncthreads();
-            wg_lu_solve<F>(A, S.ld, n, S.piv, d);
+            wg_lu_solve<F>(A, S.ld, n, S.perm, d);
             for (int i = tid; i < n; i += NT) x[i] = x[i] + d[i];
             __syncthreads();
         }
@@ -402,7 +466,7 @@ __device__ void rodas_wg_impl(const Params& P) {
                 __syncthreads();
                 const double dtd0 = dt * kRd[0];
                 for (int i = tid; i < n; i += NT) ks[i] = fsal[i] + dtd0 * dT[i];
-                wg_lu<F>(W, ld, n, S.piv, red, S.ired);
+                wg_lu<F>(W, ld, n, S.piv, S.perm, red, S.ired);
                 // rb = fsal, r = du, d = u are free during the stage-1 solve
                 wg_wsolve<F>(S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
                 for (int i = tid; i < n; i += NT) ks[i] = -ks[i];

@@ -546,7 +546,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
             const bool saves = a->nsaveat > 0 || a->save_everystep;
             shmem = ((size_t)nmax * ld * fb + 15) & ~(size_t)15;
             shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
-            shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 4;
+            shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 8;  // fsc + pf, piv + perm
             shmem = (shmem + 15) & ~(size_t)15;
             shmem += 32 * 8 + 32 * 4;
             if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
