@@ -138,6 +138,47 @@ int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* args);
 /* Fill args with the reference's solve defaults for `alg`. */
 void odemi_default_args(odemi_solve_args* args, int alg);
 
+/* ---------------------------------------------------------------------------
+ * Serving (no counterpart in the reference): continuous batching of many
+ * concurrent ensemble requests over a device-resident slot pool.  A worker
+ * thread admits arriving requests into free slots (admission bounded by the
+ * pool, i.e. by the HBM budget), keeps every trajectory's integrator state
+ * (t, dt, dtpropose, u, uprev, FSAL stage / J, dT, q11, errold, counters) on
+ * the device between ticks, advances all active trajectories together, and
+ * copies a request's results to the caller's buffers when its last trajectory
+ * finishes.  A trajectory's result does not depend on what else is in flight.
+ * Requests share the server's options (tolerances, final state only).
+ * ------------------------------------------------------------------------- */
+typedef struct odemi_server odemi_server;
+typedef struct {
+    int alg;                  /* ODEMI_TSIT5 / ODEMI_RODAS5P (lane kernels) */
+    int precision;            /* ODEMI_FP64 / ODEMI_MIXED */
+    double reltol, abstol, dtmax, dtmin;
+    int64_t maxiters;
+    int64_t max_slots;        /* slot pool size (trajectories in flight) */
+    size_t hbm_budget_bytes;  /* > 0: cap the pool so it fits this many bytes */
+    int tick_attempts;        /* step attempts per trajectory per tick */
+    int refine_sweeps;
+} odemi_server_opts;
+
+void odemi_server_default_opts(odemi_server_opts* opts, int alg);
+odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, const odemi_server_opts* opts);
+void odemi_server_destroy(odemi_server* server);
+/* Submit N trajectories (host SoA inputs, copied before return).  Outputs are
+ * written at completion: u_final [n][N], retcode [N], stats [6][N] (stats may be
+ * NULL).  Returns a request id (> 0) or a negative error. */
+int64_t odemi_submit(odemi_server* server, int64_t N, const double* u0, const double* p, const double* t0,
+                     const double* tf, double* u_final, int32_t* retcode, int32_t* stats);
+/* 1 done, 0 pending or running, < 0 error */
+int odemi_poll(odemi_server* server, int64_t request);
+/* Block until done (timeout_s <= 0: no timeout).  1 done, 0 timed out, < 0 error. */
+int odemi_wait(odemi_server* server, int64_t request, double timeout_s);
+/* steady-clock seconds of submission, admission and completion */
+int odemi_request_times(odemi_server* server, int64_t request, double* t_submit, double* t_admit, double* t_done);
+int odemi_server_info(odemi_server* server, int64_t* slots, int64_t* ticks, int64_t* pending, int64_t* active,
+                      int64_t* finished);
+const char* odemi_server_last_error(odemi_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,7 +75,11 @@ struct Params {
     double* st_vec;           // [8][nmax][N]
     int* st_int;              // [8][N]
     int refine_sweeps;        // mixed Rosenbrock: FP64 refinement sweeps per solve
-    int pad0;
+    int tick_attempts;        // serving: step attempts per slot per tick
+    // serving (continuous batching): slot pool bookkeeping
+    i64 slot0, nslots;        // init kernel: slot range to initialise
+    const int* slot_req;      // [N] request table index of each slot
+    u64* req_done;            // [R] finished trajectories per request
 };
 
 // ---------------------------------------------------------------------------
@@ -1131,8 +1135,8 @@ __device__ void rodas_persistent(const Params& P) {
 }
 
 // Plain Rodas5P: st_vec rows u, uprev, dT, J[0..N-1]
-template <int N>
-__device__ __forceinline__ void rodas_plain_store_vec(const RodasLane<double, N>& L, const Params& P, i64 j) {
+template <typename F, int N>
+__device__ __forceinline__ void rodas_plain_store_vec(const RodasLane<F, N>& L, const Params& P, i64 j) {
     const i64 M = P.N;
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -1150,7 +1154,7 @@ __device__ void rodas_plain_init(const Params& P) {
     RodasLane<double, N> L;
     rodas_init(L, P, j);
     plain_store_ctl<N>(L.c, P, j, 0);
-    rodas_plain_store_vec<N>(L, P, j);
+    rodas_plain_store_vec(L, P, j);
 }
 template <int N>
 __device__ void rodas_plain_step(const Params& P) {
@@ -1179,11 +1183,118 @@ __device__ void rodas_plain_step(const Params& P) {
     } else {
         plain_store_ctl<N>(L.c, P, j, 0);
         P.st_int[9 * M + j] = L.c.njacs; P.st_int[10 * M + j] = L.c.nw; P.st_int[11 * M + j] = L.c.nsolve;
-        rodas_plain_store_vec<N>(L, P, j);
+        rodas_plain_store_vec(L, P, j);
         atomicAdd(&P.counters[1], 1ULL);
     }
 }
 
+}  // namespace odemi
+
+namespace odemi {
+// ---------------------------------------------------------------------------
+// Serving (continuous batching).  The slot pool holds device-resident
+// integrator state between ticks in the plain-engine layout (st_scalar,
+// st_int, st_vec); st_int row 7 is the slot state (0 free, 1 active).  The
+// init kernels admit a contiguous slot range, the tick kernels advance every
+// active slot by up to P.tick_attempts step attempts with the state in
+// registers.  Stage values are stored as FP64 (exact for FP32 stages), so a
+// trajectory's result does not depend on slot placement or tick boundaries.
+// ---------------------------------------------------------------------------
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_store_state(const Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+    const i64 M = P.N;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        P.st_vec[((i64)0 * P.nmax + i) * M + j] = (double)L.u[i];
+        P.st_vec[((i64)1 * P.nmax + i) * M + j] = (double)L.uprev[i];
+        P.st_vec[((i64)2 * P.nmax + i) * M + j] = (double)L.k1[i];
+        P.st_vec[((i64)3 * P.nmax + i) * M + j] = (double)L.k7[i];
+    }
+}
+template <typename S, typename K, int N>
+__device__ void tsit5_serve_init(const Params& P) {
+    i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P.nslots) return;
+    const i64 j = P.slot0 + q;
+    Tsit5Lane<S, K, N> L;
+    tsit5_init(L, P, j);
+    plain_store_ctl<N>(L.c, P, j, 1);
+    tsit5_store_state(L, P, j);
+}
+template <typename S, typename K, int N>
+__device__ void tsit5_serve_tick(const Params& P) {
+    const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    if (P.st_int[7 * P.N + j] != 1) return;
+    Tsit5Lane<S, K, N> L;
+    plain_load_ctl<N>(L.c, P, j);
+    const i64 M = P.N;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        L.u[i] = (S)P.st_vec[((i64)0 * P.nmax + i) * M + j];
+        L.uprev[i] = (S)P.st_vec[((i64)1 * P.nmax + i) * M + j];
+        L.k1[i] = (K)P.st_vec[((i64)2 * P.nmax + i) * M + j];
+        L.k7[i] = (K)P.st_vec[((i64)3 * P.nmax + i) * M + j];
+    }
+#pragma unroll
+    for (int k = 0; k < ODEMI_NP; k++) L.p[k] = (K)((k < P.np) ? P.p[(i64)k * P.N + j] : 0.0);
+    for (int a = 0; a < P.tick_attempts; a++) {
+        int rc = tsit5_attempt<false>(L, P, j);
+        if (rc) {
+            finish_lane<N>(L.c, P, j, L.u, rc);
+            plain_store_ctl<N>(L.c, P, j, 0);
+            atomicAdd(&P.req_done[P.slot_req[j]], 1ULL);
+            return;
+        }
+    }
+    plain_store_ctl<N>(L.c, P, j, 1);
+    tsit5_store_state(L, P, j);
+}
+
+template <typename F, int N>
+__device__ void rodas_serve_init(const Params& P) {
+    i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P.nslots) return;
+    const i64 j = P.slot0 + q;
+    RodasLane<F, N> L;
+    rodas_init(L, P, j);
+    plain_store_ctl<N>(L.c, P, j, 1);
+    const i64 M = P.N;
+    P.st_int[9 * M + j] = 0; P.st_int[10 * M + j] = 0; P.st_int[11 * M + j] = 0;
+    rodas_plain_store_vec(L, P, j);
+}
+template <typename F, int N>
+__device__ void rodas_serve_tick(const Params& P) {
+    const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    if (P.st_int[7 * P.N + j] != 1) return;
+    RodasLane<F, N> L;
+    plain_load_ctl<N>(L.c, P, j);
+    const i64 M = P.N;
+    L.c.njacs = P.st_int[9 * M + j]; L.c.nw = P.st_int[10 * M + j]; L.c.nsolve = P.st_int[11 * M + j];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        L.u[i] = P.st_vec[((i64)0 * P.nmax + i) * M + j];
+        L.uprev[i] = P.st_vec[((i64)1 * P.nmax + i) * M + j];
+        L.dT[i] = P.st_vec[((i64)2 * P.nmax + i) * M + j];
+#pragma unroll
+        for (int b = 0; b < N; b++) L.J[i][b] = P.st_vec[((i64)(3 + i) * P.nmax + b) * M + j];
+    }
+#pragma unroll
+    for (int k = 0; k < ODEMI_NP; k++) L.p[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0;
+    for (int a = 0; a < P.tick_attempts; a++) {
+        int rc = rodas_attempt(L, P, j);
+        if (rc) {
+            finish_lane<N>(L.c, P, j, L.u, rc);
+            plain_store_ctl<N>(L.c, P, j, 0);
+            atomicAdd(&P.req_done[P.slot_req[j]], 1ULL);
+            return;
+        }
+    }
+    plain_store_ctl<N>(L.c, P, j, 1);
+    P.st_int[9 * M + j] = L.c.njacs; P.st_int[10 * M + j] = L.c.nw; P.st_int[11 * M + j] = L.c.nsolve;
+    rodas_plain_store_vec(L, P, j);
+}
 }  // namespace odemi
 
 #if defined(ODEMI_WITH_WG) && !defined(ODEMI_RTC)

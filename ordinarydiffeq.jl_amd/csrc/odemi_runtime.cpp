@@ -19,7 +19,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <condition_variable>
+#include <deque>
+#include <map>
 #include <mutex>
+#include <thread>
 #include <numeric>
 #include <sstream>
 #include <string>
@@ -57,7 +61,10 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     double* st_vec;
     int* st_int;
     int refine_sweeps;
-    int pad0;
+    int tick_attempts;
+    int64_t slot0, nslots;
+    const int* slot_req;
+    unsigned long long* req_done;
 };
 
 // Longest-predicted-cost-first work order: key_j = (tf_j - t0_j) * n_j^3 as FP32.
@@ -120,6 +127,7 @@ struct odemi_rhs {
     hipFunction_t rodas_plain_init = nullptr, rodas_plain_step = nullptr;
     hipFunction_t rodas_wg_f64 = nullptr, rodas_wg_mixed = nullptr;
     hipFunction_t rodas_wg_f64_sv = nullptr, rodas_wg_mixed_sv = nullptr;
+    hipFunction_t serve_init[2][2] = {}, serve_tick[2][2] = {};  // [alg][fp64, mixed]
 };
 
 #define HIPCHK(ctx, call)                                                                        \
@@ -207,6 +215,14 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
               << ", ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_init(odemi::Params P)"
              " { odemi::tsit5_plain_init<ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_init_f64(odemi::Params P)"
+             " { odemi::tsit5_serve_init<double, double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_init_mixed(odemi::Params P)"
+             " { odemi::tsit5_serve_init<double, float, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_tick_f64(odemi::Params P)"
+             " { odemi::tsit5_serve_tick<double, double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_tick_mixed(odemi::Params P)"
+             " { odemi::tsit5_serve_tick<double, float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_step(odemi::Params P)"
              " { odemi::tsit5_plain_step<ODEMI_N>(P); }\n";
     }
@@ -217,6 +233,14 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
              " { odemi::rodas_persistent<float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_plain_init(odemi::Params P)"
              " { odemi::rodas_plain_init<ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_serve_init_f64(odemi::Params P)"
+             " { odemi::rodas_serve_init<double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_serve_init_mixed(odemi::Params P)"
+             " { odemi::rodas_serve_init<float, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_serve_tick_f64(odemi::Params P)"
+             " { odemi::rodas_serve_tick<double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_serve_tick_mixed(odemi::Params P)"
+             " { odemi::rodas_serve_tick<float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_plain_step(odemi::Params P)"
              " { odemi::rodas_plain_step<ODEMI_N>(P); }\n";
     }
@@ -334,6 +358,10 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         get("odemi_tsit5_persist_mixed", &r->tsit5_persist_mixed);
         get("odemi_tsit5_persist_f64_sv", &r->tsit5_persist_f64_sv);
         get("odemi_tsit5_persist_mixed_sv", &r->tsit5_persist_mixed_sv);
+        get("odemi_tsit5_serve_init_f64", &r->serve_init[ODEMI_TSIT5][0]);
+        get("odemi_tsit5_serve_init_mixed", &r->serve_init[ODEMI_TSIT5][1]);
+        get("odemi_tsit5_serve_tick_f64", &r->serve_tick[ODEMI_TSIT5][0]);
+        get("odemi_tsit5_serve_tick_mixed", &r->serve_tick[ODEMI_TSIT5][1]);
         get("odemi_tsit5_plain_init", &r->tsit5_plain_init);
         get("odemi_tsit5_plain_step", &r->tsit5_plain_step);
     }
@@ -342,6 +370,10 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         get("odemi_rodas_persist_mixed", &r->rodas_persist_mixed);
         get("odemi_rodas_plain_init", &r->rodas_plain_init);
         get("odemi_rodas_plain_step", &r->rodas_plain_step);
+        get("odemi_rodas_serve_init_f64", &r->serve_init[ODEMI_RODAS5P][0]);
+        get("odemi_rodas_serve_init_mixed", &r->serve_init[ODEMI_RODAS5P][1]);
+        get("odemi_rodas_serve_tick_f64", &r->serve_tick[ODEMI_RODAS5P][0]);
+        get("odemi_rodas_serve_tick_mixed", &r->serve_tick[ODEMI_RODAS5P][1]);
     }
     if (flags & ODEMI_COMPILE_WG) {
         get("odemi_rodas_wg_f64", &r->rodas_wg_f64);
@@ -578,4 +610,357 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     a->kernel_ms = ms;
     a->launches = launches;
     return 0;
+}
+
+// ===========================================================================
+// Serving: continuous batching over a device-resident slot pool
+// ===========================================================================
+namespace {
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    int64_t id = 0, N = 0;
+    std::vector<double> u0, p, t0, tf;   // host copies of the inputs (SoA, N columns)
+    std::vector<int> sreq;               // slot -> request-table index for the admitted range
+    double* u_final = nullptr;           // caller's output buffers
+    int32_t* retcode = nullptr;
+    int32_t* stats = nullptr;
+    int64_t slot0 = -1;
+    int rt = -1;                         // request-table index on the device
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int state = 0;                       // 0 pending, 1 active, 2 done, <0 error
+};
+
+// First-fit allocator of contiguous slot ranges.
+struct RangeAlloc {
+    std::map<int64_t, int64_t> free_;    // start -> length
+    int64_t high = 0;                    // high-water mark (grid extent)
+    explicit RangeAlloc(int64_t n = 0) { if (n > 0) free_[0] = n; }
+    int64_t take(int64_t n) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                int64_t s = it->first, len = it->second;
+                free_.erase(it);
+                if (len > n) free_[s + n] = len - n;
+                high = std::max(high, s + n);
+                return s;
+            }
+        }
+        return -1;
+    }
+    void give(int64_t s, int64_t n) {
+        auto it = free_.emplace(s, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); it = pv; }
+        }
+        // shrink the high-water mark when the tail is free
+        auto last = std::prev(free_.end());
+        if (last->first + last->second >= high) high = std::min(high, last->first);
+    }
+};
+}  // namespace
+
+struct odemi_server {
+    odemi_ctx* ctx = nullptr;
+    odemi_rhs* rhs = nullptr;
+    odemi_server_opts o;
+    int64_t S = 0;                       // slots
+    int R = 4096;                        // request table entries
+    int rows = 4;
+    hipStream_t stream = nullptr;
+    void* mem = nullptr;
+    Params P;
+    std::vector<unsigned long long> done_h;
+    RangeAlloc alloc;
+    std::vector<int> rt_free;
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::deque<Request*> pending;
+    std::vector<Request*> active;
+    std::map<int64_t, Request*> all;
+    int64_t next_id = 1;
+    bool stop = false;
+    std::thread worker;
+    std::string err;
+    int64_t ticks = 0, admitted = 0, finished = 0;
+};
+
+static void server_loop(odemi_server* S);
+
+extern "C" void odemi_server_default_opts(odemi_server_opts* o, int alg) {
+    memset(o, 0, sizeof(*o));
+    o->alg = alg;
+    o->precision = ODEMI_FP64;
+    o->reltol = 1e-3;
+    o->abstol = 1e-6;
+    o->maxiters = 1000000;
+    o->tick_attempts = 64;
+    o->max_slots = 1 << 20;
+    o->refine_sweeps = 2;
+}
+
+extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, const odemi_server_opts* opts) {
+    if (!ctx || !rhs || !opts) return nullptr;
+    if (opts->alg < 0 || opts->alg > 1 || !rhs->serve_tick[opts->alg][opts->precision == ODEMI_MIXED ? 1 : 0]) {
+        ctx->err = "the compiled problem has no serving kernels for this algorithm";
+        return nullptr;
+    }
+    (void)hipSetDevice(ctx->device);
+    odemi_server* S = new odemi_server();
+    S->ctx = ctx;
+    S->rhs = rhs;
+    S->o = *opts;
+    const int n = rhs->n, np = std::max(rhs->np, 1);
+    S->rows = (opts->alg == ODEMI_RODAS5P) ? 3 + n : 4;
+    // bytes per slot: inputs, state, outputs
+    const size_t per_slot = (size_t)(n + np + 2) * 8 + 16 * 8 + (size_t)S->rows * n * 8 + 12 * 4 +
+                            (size_t)n * 8 + 8 + 4 + 4 + 6 * 4 + 4;
+    int64_t slots = opts->max_slots > 0 ? opts->max_slots : (1 << 20);
+    if (opts->hbm_budget_bytes > 0) slots = std::min<int64_t>(slots, (int64_t)(opts->hbm_budget_bytes / per_slot));
+    if (slots < 1) { ctx->err = "HBM budget too small for one slot"; delete S; return nullptr; }
+    S->S = slots;
+    const size_t Sz = (size_t)slots;
+    size_t bytes = 0;
+    auto al = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~size_t(255); return o; };
+    size_t o_u0 = al(Sz * n * 8), o_p = al(Sz * np * 8), o_t0 = al(Sz * 8), o_tf = al(Sz * 8);
+    size_t o_sc = al(16 * Sz * 8), o_vec = al((size_t)S->rows * n * Sz * 8), o_int = al(12 * Sz * 4);
+    size_t o_uf = al(Sz * n * 8), o_tfin = al(Sz * 8), o_rc = al(Sz * 4), o_st = al(6 * Sz * 4), o_ns = al(Sz * 4);
+    size_t o_sreq = al(Sz * 4), o_done = al((size_t)S->R * 8), o_cnt = al(64);
+    if (hipMalloc(&S->mem, bytes) != hipSuccess) { ctx->err = "serving pool allocation failed"; delete S; return nullptr; }
+    char* b = (char*)S->mem;
+    (void)hipMemset(b + o_int, 0, 12 * Sz * 4);
+    (void)hipMemset(b + o_done, 0, (size_t)S->R * 8);
+    (void)hipMemset(b + o_cnt, 0, 64);
+    Params& P = S->P;
+    memset(&P, 0, sizeof(P));
+    P.N = slots; P.nmax = n; P.np = rhs->np;
+    P.u0 = (double*)(b + o_u0); P.p = (double*)(b + o_p); P.t0 = (double*)(b + o_t0); P.tf = (double*)(b + o_tf);
+    P.reltol = opts->reltol; P.abstol = opts->abstol; P.dtmax = opts->dtmax; P.dtmin = opts->dtmin; P.dt_user = 0.0;
+    P.maxiters = opts->maxiters;
+    P.save_start = 0; P.save_end = 0; P.save_everystep = 0; P.save_end_user = -1; P.max_saves = 0;
+    P.st_scalar = (double*)(b + o_sc); P.st_vec = (double*)(b + o_vec); P.st_int = (int*)(b + o_int);
+    P.u_final = (double*)(b + o_uf); P.t_final = (double*)(b + o_tfin); P.retcode = (int*)(b + o_rc);
+    P.stats = (int*)(b + o_st); P.nsaved = (int*)(b + o_ns);
+    P.slot_req = (int*)(b + o_sreq); P.req_done = (unsigned long long*)(b + o_done);
+    P.counters = (unsigned long long*)(b + o_cnt);
+    P.refine_sweeps = opts->refine_sweeps;
+    P.tick_attempts = opts->tick_attempts > 0 ? opts->tick_attempts : 64;
+    S->done_h.assign(S->R, 0);
+    S->alloc = RangeAlloc(slots);
+    for (int i = S->R - 1; i >= 0; i--) S->rt_free.push_back(i);
+    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->err = "stream create failed"; (void)hipFree(S->mem); delete S; return nullptr;
+    }
+    S->worker = std::thread(server_loop, S);
+    return S;
+}
+
+extern "C" void odemi_server_destroy(odemi_server* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    (void)hipSetDevice(S->ctx->device);
+    (void)hipStreamSynchronize(S->stream);
+    (void)hipStreamDestroy(S->stream);
+    (void)hipFree(S->mem);
+    for (auto& kv : S->all) delete kv.second;
+    delete S;
+}
+
+extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const double* u0, const double* p, const double* t0,
+                                const double* tf, double* u_final, int32_t* retcode, int32_t* stats) {
+    if (!S || N <= 0 || !u0 || !t0 || !tf || !retcode) return -1;
+    if (S->rhs->np > 0 && !p) { S->err = "the problem has parameters but p is NULL"; return -1; }
+    if (N > S->S) { S->err = "request larger than the slot pool"; return -2; }
+    const int n = S->rhs->n, np = S->rhs->np;
+    Request* r = new Request();
+    r->N = N;
+    r->u0.assign(u0, u0 + (size_t)n * N);
+    if (np > 0 && p) r->p.assign(p, p + (size_t)np * N); else r->p.assign((size_t)std::max(np, 1) * N, 0.0);
+    r->t0.assign(t0, t0 + N);
+    r->tf.assign(tf, tf + N);
+    r->u_final = u_final; r->retcode = retcode; r->stats = stats;
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->id = S->next_id++;
+        S->all[r->id] = r;
+        S->pending.push_back(r);
+    }
+    S->cv.notify_all();
+    return r->id;
+}
+
+extern "C" int odemi_poll(odemi_server* S, int64_t id) {
+    if (!S) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return -1;
+    return it->second->state == 2 ? 1 : (it->second->state < 0 ? it->second->state : 0);
+}
+
+extern "C" int odemi_wait(odemi_server* S, int64_t id, double timeout_s) {
+    if (!S) return -1;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return -1;
+    Request* r = it->second;
+    auto pred = [&] { return r->state == 2 || r->state < 0; };
+    if (timeout_s > 0) {
+        if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) return 0;
+    } else {
+        S->cv_done.wait(lk, pred);
+    }
+    return r->state == 2 ? 1 : r->state;
+}
+
+extern "C" int odemi_request_times(odemi_server* S, int64_t id, double* t_submit, double* t_admit, double* t_done) {
+    if (!S) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return -1;
+    if (t_submit) *t_submit = it->second->t_submit;
+    if (t_admit) *t_admit = it->second->t_admit;
+    if (t_done) *t_done = it->second->t_done;
+    return 0;
+}
+
+extern "C" int odemi_server_info(odemi_server* S, int64_t* slots, int64_t* ticks, int64_t* pending, int64_t* active,
+                                 int64_t* finished) {
+    if (!S) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (slots) *slots = S->S;
+    if (ticks) *ticks = S->ticks;
+    if (pending) *pending = (int64_t)S->pending.size();
+    if (active) *active = (int64_t)S->active.size();
+    if (finished) *finished = S->finished;
+    return 0;
+}
+
+extern "C" const char* odemi_server_last_error(odemi_server* S) { return S ? S->err.c_str() : "null server"; }
+
+static int server_fail(odemi_server* S, const char* what, hipError_t e) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->err = std::string(what) + ": " + hipGetErrorString(e);
+    for (auto* r : S->active) r->state = -3;
+    for (auto* r : S->pending) r->state = -3;
+    S->active.clear();
+    S->pending.clear();
+    S->stop = true;
+    S->cv_done.notify_all();
+    return -1;
+}
+
+#define SCHK(call)                                            \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) { server_fail(S, #call, e_); return; } \
+    } while (0)
+
+static void server_loop(odemi_server* S) {
+    (void)hipSetDevice(S->ctx->device);
+    const int n = S->rhs->n, np = std::max(S->rhs->np, 1);
+    const int pi = S->o.precision == ODEMI_MIXED ? 1 : 0;
+    hipFunction_t f_init = S->rhs->serve_init[S->o.alg][pi], f_tick = S->rhs->serve_tick[S->o.alg][pi];
+    const size_t Sz = (size_t)S->S;
+    for (;;) {
+        std::vector<Request*> admit;
+        int64_t high = 0;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [&] { return S->stop || !S->pending.empty() || !S->active.empty(); });
+            if (S->stop && S->active.empty() && S->pending.empty()) break;
+            if (S->stop) break;
+            // admission: FIFO while a contiguous slot range and a request-table entry are free
+            while (!S->pending.empty() && !S->rt_free.empty()) {
+                Request* r = S->pending.front();
+                int64_t s0 = S->alloc.take(r->N);
+                if (s0 < 0) break;
+                S->pending.pop_front();
+                r->slot0 = s0;
+                r->rt = S->rt_free.back();
+                S->rt_free.pop_back();
+                r->state = 1;
+                r->t_admit = now_s();
+                S->active.push_back(r);
+                admit.push_back(r);
+                S->admitted++;
+            }
+            high = S->alloc.high;
+        }
+        for (Request* r : admit) {
+            const size_t N = (size_t)r->N, s0 = (size_t)r->slot0;
+            SCHK(hipMemcpy2DAsync((double*)S->P.u0 + s0, Sz * 8, r->u0.data(), N * 8, N * 8, n, hipMemcpyHostToDevice, S->stream));
+            SCHK(hipMemcpy2DAsync((double*)S->P.p + s0, Sz * 8, r->p.data(), N * 8, N * 8, np, hipMemcpyHostToDevice, S->stream));
+            SCHK(hipMemcpyAsync((double*)S->P.t0 + s0, r->t0.data(), N * 8, hipMemcpyHostToDevice, S->stream));
+            SCHK(hipMemcpyAsync((double*)S->P.tf + s0, r->tf.data(), N * 8, hipMemcpyHostToDevice, S->stream));
+            r->sreq.assign(N, r->rt);
+            SCHK(hipMemcpyAsync((int*)S->P.slot_req + s0, r->sreq.data(), N * 4, hipMemcpyHostToDevice, S->stream));
+            SCHK(hipMemsetAsync(S->P.req_done + r->rt, 0, 8, S->stream));
+            SCHK(hipMemsetAsync((int*)S->P.nsaved + s0, 0, N * 4, S->stream));
+            Params Pi = S->P;
+            Pi.slot0 = (int64_t)s0;
+            Pi.nslots = (int64_t)N;
+            void* args[] = {&Pi};
+            SCHK(hipModuleLaunchKernel(f_init, (unsigned)((N + 255) / 256), 1, 1, 256, 1, 1, 0, S->stream, args, nullptr));
+            // the host input vectors live in the Request until it finishes, so no wait here
+        }
+        if (high > 0) {
+            Params Pt = S->P;
+            void* args[] = {&Pt};
+            SCHK(hipModuleLaunchKernel(f_tick, (unsigned)((high + 255) / 256), 1, 1, 256, 1, 1, 0, S->stream, args, nullptr));
+        }
+        SCHK(hipMemcpyAsync(S->done_h.data(), S->P.req_done, (size_t)S->R * 8, hipMemcpyDeviceToHost, S->stream));
+        SCHK(hipStreamSynchronize(S->stream));
+        std::vector<Request*> fin;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->ticks++;
+            for (size_t i = 0; i < S->active.size();) {
+                Request* r = S->active[i];
+                if ((int64_t)S->done_h[r->rt] >= r->N) {
+                    fin.push_back(r);
+                    S->active[i] = S->active.back();
+                    S->active.pop_back();
+                } else {
+                    i++;
+                }
+            }
+        }
+        for (Request* r : fin) {
+            const size_t N = (size_t)r->N, s0 = (size_t)r->slot0;
+            if (r->u_final)
+                SCHK(hipMemcpy2DAsync(r->u_final, N * 8, (double*)S->P.u_final + s0, Sz * 8, N * 8, n, hipMemcpyDeviceToHost, S->stream));
+            SCHK(hipMemcpyAsync(r->retcode, (int*)S->P.retcode + s0, N * 4, hipMemcpyDeviceToHost, S->stream));
+            if (r->stats)
+                SCHK(hipMemcpy2DAsync(r->stats, N * 4, (int*)S->P.stats + s0, Sz * 4, N * 4, 6, hipMemcpyDeviceToHost, S->stream));
+        }
+        if (!fin.empty()) SCHK(hipStreamSynchronize(S->stream));
+        if (!fin.empty()) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            const double t = now_s();
+            for (Request* r : fin) {
+                S->alloc.give(r->slot0, r->N);
+                S->rt_free.push_back(r->rt);
+                r->t_done = t;
+                r->state = 2;
+                std::vector<double>().swap(r->u0);
+                std::vector<double>().swap(r->p);
+                std::vector<double>().swap(r->t0);
+                std::vector<double>().swap(r->tf);
+                std::vector<int>().swap(r->sreq);
+                S->finished++;
+            }
+            S->cv_done.notify_all();
+        }
+    }
 }

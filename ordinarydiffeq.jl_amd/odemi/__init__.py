@@ -3,9 +3,10 @@ reference's EnsembleProblem surface.  See api.py for the user entry points."""
 from .api import (DEStats, EnsembleContext, EnsembleMI355X, EnsembleProblem, EnsembleSolution, ODEFunction,
                   ODEProblem, ODESolution, ReturnCode, Rodas5P, Tsit5, compile_problem, get_context, remake,
                   solve, solve_arrays)
+from .serving import EnsembleServer, Request
 from .tracer import Sym, sym_cos, sym_exp, sym_log, sym_sin, sym_sqrt, sym_tanh, trace_problem
 
 __all__ = ["DEStats", "EnsembleContext", "EnsembleMI355X", "EnsembleProblem", "EnsembleSolution", "ODEFunction",
            "ODEProblem", "ODESolution", "ReturnCode", "Rodas5P", "Tsit5", "compile_problem", "get_context", "remake",
            "solve", "solve_arrays", "Sym", "sym_cos", "sym_exp", "sym_log", "sym_sin", "sym_sqrt", "sym_tanh",
-           "trace_problem"]
+           "trace_problem", "EnsembleServer", "Request"]

@@ -39,9 +39,21 @@ class SolveArgs(ctypes.Structure):
     ]
 
 
+class ServerOpts(ctypes.Structure):
+    """Mirror of odemi_server_opts (include/odemi.h)."""
+    _fields_ = [
+        ("alg", c_i), ("precision", c_i),
+        ("reltol", c_d), ("abstol", c_d), ("dtmax", c_d), ("dtmin", c_d),
+        ("maxiters", c_i64), ("max_slots", c_i64), ("hbm_budget_bytes", ctypes.c_size_t),
+        ("tick_attempts", c_i), ("refine_sweeps", c_i),
+    ]
+
+
 EXPORTS = ["odemi_create", "odemi_destroy", "odemi_last_error", "odemi_device_info", "odemi_abi_version",
            "odemi_compile_rhs", "odemi_rhs_destroy", "odemi_solve_batch", "odemi_default_args",
-           "odemi_rtc_codeobject"]
+           "odemi_rtc_codeobject", "odemi_server_default_opts", "odemi_server_create", "odemi_server_destroy",
+           "odemi_submit", "odemi_poll", "odemi_wait", "odemi_request_times", "odemi_server_info",
+           "odemi_server_last_error"]
 
 _lib = None
 
@@ -71,6 +83,22 @@ def load():
     L.odemi_rtc_codeobject.argtypes = [ctypes.c_char_p, c_i, c_i, c_i, ctypes.c_char_p, c_i, P(c_d),
                                        ctypes.c_char_p, c_i]
     L.odemi_rtc_codeobject.restype = c_i
+    L.odemi_server_default_opts.argtypes = [P(ServerOpts), c_i]
+    L.odemi_server_create.argtypes = [c_vp, c_vp, P(ServerOpts)]
+    L.odemi_server_create.restype = c_vp
+    L.odemi_server_destroy.argtypes = [c_vp]
+    L.odemi_submit.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
+    L.odemi_submit.restype = c_i64
+    L.odemi_poll.argtypes = [c_vp, c_i64]
+    L.odemi_poll.restype = c_i
+    L.odemi_wait.argtypes = [c_vp, c_i64, c_d]
+    L.odemi_wait.restype = c_i
+    L.odemi_request_times.argtypes = [c_vp, c_i64, P(c_d), P(c_d), P(c_d)]
+    L.odemi_request_times.restype = c_i
+    L.odemi_server_info.argtypes = [c_vp] + [P(c_i64)] * 5
+    L.odemi_server_info.restype = c_i
+    L.odemi_server_last_error.argtypes = [c_vp]
+    L.odemi_server_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 

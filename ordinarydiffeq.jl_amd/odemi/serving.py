@@ -1,0 +1,140 @@
+"""Continuous-batching ensemble server (odemi_server_* in include/odemi.h).
+
+Many independent callers submit small ensembles (a request = N trajectories of
+one compiled problem); the native worker thread admits them into a
+device-resident slot pool as slots free up, advances every active trajectory
+together in ticks of a few step attempts, and hands each request its final
+states as soon as its last trajectory finishes.  The pool size is the
+admission bound: it is set from an HBM budget (bytes) or a slot count.
+
+The reference has no serving layer; the per-trajectory semantics are the
+reference's `solve` with save_everystep=false, save_start=false (final state
+only), identical to odemi.solve_arrays on the same inputs.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .api import ODEFunction, compile_problem, get_context
+
+
+class Request:
+    """Handle of one submitted ensemble; outputs are filled at completion."""
+
+    def __init__(self, server, rid, N, u_final, retcode, stats, keep):
+        self.server, self.id, self.N = server, rid, N
+        self.u_final, self.retcode, self.stats = u_final, retcode, stats
+        self._keep = keep
+
+    def done(self):
+        rc = self.server._L.odemi_poll(self.server._handle(), self.id)
+        if rc < 0:
+            raise RuntimeError(f"request {self.id} failed: {self.server.error()}")
+        if rc == 1:
+            self.server._live.pop(self.id, None)
+        return rc == 1
+
+    def wait(self, timeout=None):
+        rc = self.server._L.odemi_wait(self.server._handle(), self.id, 0.0 if timeout is None else float(timeout))
+        if rc < 0:
+            raise RuntimeError(f"request {self.id} failed: {self.server.error()}")
+        if rc == 1:
+            self.server._live.pop(self.id, None)
+        return rc == 1
+
+    def times(self):
+        """(t_submit, t_admit, t_done) on the steady clock, seconds."""
+        a, b, c = _lib.c_d(), _lib.c_d(), _lib.c_d()
+        if self.server._L.odemi_request_times(self.server._handle(), self.id, ctypes.byref(a), ctypes.byref(b),
+                                              ctypes.byref(c)) != 0:
+            raise RuntimeError(f"unknown request {self.id}")
+        return a.value, b.value, c.value
+
+    def latency(self):
+        s, _, d = self.times()
+        return d - s
+
+
+class EnsembleServer:
+    """Serve one problem type (ODEFunction + state size + algorithm)."""
+
+    def __init__(self, fun, n, np_, alg, *, precision="fp64", reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0,
+                 maxiters=1_000_000, max_slots=1 << 20, hbm_budget_bytes=0, tick_attempts=64, refine_sweeps=2,
+                 ctx=None):
+        fun = fun if isinstance(fun, ODEFunction) else ODEFunction(fun)
+        self.ctx = ctx or get_context()
+        algname = "tsit5" if alg.code == _lib.TSIT5 else "rodas5p"
+        self._rhs, _, _ = compile_problem(fun, n, np_, algs=(algname,), ctx=self.ctx)
+        self.n, self.np = n, np_
+        self._L = self.ctx.L
+        o = _lib.ServerOpts()
+        self._L.odemi_server_default_opts(ctypes.byref(o), alg.code)
+        o.precision = _lib.MIXED if precision == "mixed" else _lib.FP64
+        o.reltol, o.abstol = float(reltol), float(abstol)
+        o.dtmax = 0.0 if dtmax is None else float(dtmax)
+        o.dtmin = float(dtmin)
+        o.maxiters = int(maxiters)
+        o.max_slots = int(max_slots)
+        o.hbm_budget_bytes = int(hbm_budget_bytes)
+        o.tick_attempts = int(tick_attempts)
+        o.refine_sweeps = int(refine_sweeps)
+        self._h = self._L.odemi_server_create(self.ctx.h, self._rhs, ctypes.byref(o))
+        if not self._h:
+            raise RuntimeError("odemi_server_create failed: " + self.ctx.error())
+        self._live = {}   # request id -> output buffers the worker writes at completion
+
+    def _handle(self):
+        if not self._h:
+            raise RuntimeError("the server is closed")
+        return self._h
+
+    def error(self):
+        return self._L.odemi_server_last_error(self._handle()).decode(errors="replace")
+
+    def submit(self, u0, p, t0, tf, stats=True):
+        """u0 (n, N), p (np, N) or None, t0/tf (N,) or scalars.  Inputs are
+        copied before return; returns a Request."""
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        if u0.ndim == 1:
+            u0 = u0[:, None]
+        n, N = u0.shape
+        if n != self.n:
+            raise ValueError(f"state size {n} != served size {self.n}")
+        t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, dtype=np.float64), (N,)))
+        tf = np.ascontiguousarray(np.broadcast_to(np.asarray(tf, dtype=np.float64), (N,)))
+        pp = None
+        if self.np > 0:
+            pp = np.ascontiguousarray(p, dtype=np.float64).reshape(self.np, N)
+        uf = np.empty((n, N), dtype=np.float64)
+        rc = np.zeros(N, dtype=np.int32)
+        st = np.zeros((6, N), dtype=np.int32) if stats else None
+        ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
+        rid = self._L.odemi_submit(self._handle(), N, ptr(u0), ptr(pp), ptr(t0), ptr(tf), ptr(uf), ptr(rc), ptr(st))
+        if rid <= 0:
+            raise RuntimeError(f"odemi_submit failed ({rid}): {self.error()}")
+        self._live[rid] = (uf, rc, st)
+        return Request(self, rid, N, uf, rc, st, (uf, rc, st))
+
+    def info(self):
+        v = [_lib.c_i64() for _ in range(5)]
+        self._L.odemi_server_info(self._handle(), *[ctypes.byref(x) for x in v])
+        return dict(zip(("slots", "ticks", "pending", "active", "finished"), (x.value for x in v)))
+
+    def close(self):
+        if self._h:
+            self._L.odemi_server_destroy(self._h)   # joins the worker: no writes after this
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,117 @@
+"""GPU tests of the continuous-batching server (odemi_server_* through the C ABI):
+a request's result is the same alone and in a mix of concurrent requests, and
+the same as the batch engine and the CPU restatement on the same inputs;
+admission under a slot budget smaller than the offered load."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import odemi
+from odemi import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _cols(d, idx):
+    return (d["u0"][:, idx], d["p"][:, idx], d["t0"][idx], d["tf"][idx])
+
+
+CASES = [
+    ("tsit5-fp64", odemi.Tsit5(), "fp64", P.LORENZ, O.PROB_LORENZ, lambda: P.lorenz_large(N=768)),
+    ("tsit5-mixed", odemi.Tsit5(), "mixed", P.LORENZ, O.PROB_LORENZ, lambda: P.lorenz_large(N=768)),
+    ("rodas-fp64", odemi.Rodas5P(), "fp64", P.ROBERTSON, O.PROB_ROBERTSON, lambda: P.robertson_ensemble(N=768)),
+    ("rodas-mixed", odemi.Rodas5P(), "mixed", P.ROBERTSON, O.PROB_ROBERTSON, lambda: P.robertson_ensemble(N=768)),
+]
+
+
+@pytest.mark.parametrize("name,alg,prec,fun,pid,dgen", CASES, ids=[c[0] for c in CASES])
+def test_alone_equals_in_mix(name, alg, prec, fun, pid, dgen):
+    d = dgen()
+    n, N = d["u0"].shape
+    print(f"[case] serving alone vs mix {name} N={N}", flush=True)
+    probe = np.arange(40, 104)                       # the request under test: 64 trajectories
+    kw = dict(precision=prec, reltol=d["reltol"], abstol=d["abstol"], tick_attempts=16)
+    with odemi.EnsembleServer(fun, n, d["p"].shape[0], alg, **kw) as srv:
+        alone = srv.submit(*_cols(d, probe))
+        assert alone.wait(timeout=120)
+        # the same trajectories again, now among concurrent requests of ragged sizes
+        reqs, k = [], 0
+        others = np.setdiff1d(np.arange(N), probe)
+        sizes = [1, 7, 64, 3, 130, 31, 256, 2]
+        for i, sz in enumerate(sizes):
+            idx = others[k:k + sz]
+            k += sz
+            reqs.append(srv.submit(*_cols(d, idx)))
+            if i == 3:
+                mixed = srv.submit(*_cols(d, probe))
+        for r in reqs + [mixed]:
+            assert r.wait(timeout=120)
+        info = srv.info()
+        assert info["finished"] == len(sizes) + 2
+    assert np.array_equal(alone.u_final, mixed.u_final)
+    assert np.array_equal(alone.retcode, mixed.retcode) and np.array_equal(alone.stats, mixed.stats)
+    # identical to the batch engine on the same trajectories
+    ea = odemi.EnsembleMI355X(precision=prec)
+    b = odemi.solve_arrays(fun, alg, *_cols(d, probe), ea, reltol=d["reltol"], abstol=d["abstol"])
+    assert np.array_equal(alone.u_final, b["u_final"])
+    assert np.array_equal(alone.stats, b["stats"]) and np.array_equal(alone.retcode, b["retcode"])
+    # and, for FP64, to the CPU restatement
+    if prec == "fp64":
+        for c, j in enumerate(probe[::9]):
+            r = O.solve_builtin(pid, d["u0"][:, j], d["p"][:, j], (d["t0"][j], d["tf"][j]), alg=alg.code,
+                                reltol=d["reltol"], abstol=d["abstol"], save_start=False)
+            col = int(np.where(probe == j)[0][0])
+            assert np.array_equal(alone.u_final[:, col], r["u_final"])
+            assert int(alone.retcode[col]) == r["retcode"]
+            assert alone.stats[1, col] == r["naccept"] and alone.stats[2, col] == r["nreject"]
+
+
+def test_admission_under_slot_budget_and_threads():
+    """More offered trajectories than slots: requests queue, are admitted as
+    slots free up, and every result matches the batch engine.  Submissions come
+    from several host threads at once."""
+    d = P.lorenz_large(N=1200)
+    n, N = d["u0"].shape
+    print("[case] serving admission budget 300 slots, 4 submit threads", flush=True)
+    ref = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"])
+    chunks = np.array_split(np.arange(N), 24)
+    out = [None] * len(chunks)
+    with odemi.EnsembleServer(P.LORENZ, n, 3, odemi.Tsit5(), reltol=d["reltol"], abstol=d["abstol"],
+                              max_slots=300, tick_attempts=8) as srv:
+        assert srv.info()["slots"] == 300
+
+        def client(t):
+            for c in range(t, len(chunks), 4):
+                out[c] = srv.submit(*_cols(d, chunks[c]))
+
+        th = [threading.Thread(target=client, args=(t,)) for t in range(4)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        for r in out:
+            assert r.wait(timeout=120)
+        assert srv.info()["finished"] == len(chunks)
+        for r in out:
+            ts, ta, td = r.times()
+            assert ts <= ta <= td
+    with pytest.raises(RuntimeError):
+        out[0].times()                                   # the server is closed
+    for c, idx in enumerate(chunks):
+        assert np.array_equal(out[c].u_final, ref["u_final"][:, idx])
+        assert np.array_equal(out[c].stats, ref["stats"][:, idx])
+
+
+def test_hbm_budget_sizes_pool_and_rejects_oversized_request():
+    print("[case] serving hbm budget", flush=True)
+    with odemi.EnsembleServer(P.LORENZ, 3, 3, odemi.Tsit5(), hbm_budget_bytes=1 << 20) as srv:
+        slots = srv.info()["slots"]
+        assert 0 < slots < 10000
+        d = P.lorenz_large(N=slots + 1)
+        with pytest.raises(RuntimeError):
+            srv.submit(*_cols(d, np.arange(slots + 1)))
+        r = srv.submit(*_cols(d, np.arange(slots)))
+        assert r.wait(timeout=120) and (r.retcode == 1).all()
