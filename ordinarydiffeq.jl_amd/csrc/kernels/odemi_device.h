@@ -80,6 +80,7 @@ struct Params {
     i64 slot0, nslots;        // init kernel: slot range to initialise
     const int* slot_req;      // [N] request table index of each slot
     u64* req_done;            // [R] finished trajectories per request
+    float* wg_jbuf;           // workgroup Gauss-Jordan kernel: [grid][nmax][nmax] FP32 Jacobian scratch
 };
 
 // ---------------------------------------------------------------------------

@@ -1,5 +1,5 @@
 // odemi_wg.h -- Rodas5P with one workgroup per trajectory, for systems too
-// large for a lane (n up to 256) and for batches of uneven dimension (nvec).
+// large for a lane (n up to 128) and for batches of uneven dimension (nvec).
 //
 // Same step-control semantics as the lane kernels (see odemi_device.h) and the
 // same arithmetic order as the CPU restatement: every element of the LU and of
@@ -9,17 +9,25 @@
 //
 // LDS layout (dynamic, sized by the host from nmax):
 //   W    F[nmax][ld], ld = nmax + 1       J, then W = J - I/(dt*gamma), then its LU
+//                                         (FP64 variant only)
 //   vec  double[NV][nmax]                 u, uprev, fsal, du, dT, ks[8] (+ dense[3]);
 //   pd   double[32]                       parameters (FP64)
 //   fsc  float[nmax + 32]                 FP32 copies of uprev and p (mixed variant)
-//   piv  int[nmax], perm int[nmax]   pivots and the equivalent row gather
+//   piv  int[nmax], perm int[nmax]        pivots and the equivalent row gather
+//   colbuf, rowbuf float[2][NP], zbuf float[NP]   Gauss-Jordan broadcasts (NP = nmax
+//                                         rounded up to 16)
 //   red  double[32] + int[32]             reductions / broadcasts
-// FP64 variant: W is FP64 (132 KB at n = 128, one workgroup per CU).
-// Mixed variant: W is FP32 and every stage solve gets `refine_sweeps` FP64
-// iterative-refinement sweeps whose residual b - (J x - x/(dt*gamma)) uses the
-// traced Jacobian-vector product in FP64, so J is never stored in FP64 and the
-// refined solution is that of the FP64 system (about 80 KB at n = 128, two
-// workgroups per CU).
+// FP64 variant: W is FP64 in LDS (132 KB at n = 128, one workgroup per CU),
+// factored by LU with partial pivoting.
+// Mixed variant (n <= 128): the FP32 Jacobian is evaluated into a per-workgroup
+// global scratch block, W = J - I/(dt*gamma) is held in registers as a 16 x 16
+// grid of R x R tiles (R = ceil(nmax/16)) and inverted in place by swap-free
+// Gauss-Jordan with partial pivoting, so each stage solve is one parallel
+// matrix-vector product instead of two serial substitutions.  Every stage solve
+// gets `refine_sweeps` FP64 iterative-refinement sweeps whose residual
+// b - (J x - x/(dt*gamma)) uses the traced Jacobian-vector product in FP64, so
+// the refined solution is that of the FP64 system.  LDS use is a few tens of KB,
+// so several trajectories share a CU.
 #pragma once
 #ifdef ODEMI_WITH_WG
 
@@ -40,19 +48,23 @@ struct WGPtrs {
     float* fsc;
     float* pf;
     int* piv;
-    int* perm;   // row gather: after the LU, (P b)[i] = b[perm[i]]
+    int* perm;   // row gather: after the LU, (P b)[i] = b[perm[i]]; Gauss-Jordan: pivot row of step k
+    float *colbuf, *rowbuf, *zbuf;
     double* red;
     int* ired;
     int ld;
 };
 
+__host__ __device__ inline int wg_np(int nmax) { return 16 * ((nmax + 15) / 16); }
+
+// w_in_lds: W lives in LDS (LU variants); the Gauss-Jordan variant keeps it in registers.
 template <typename F>
-__device__ __forceinline__ WGPtrs wg_layout(unsigned char* base, int nmax, bool saves) {
+__device__ __forceinline__ WGPtrs wg_layout(unsigned char* base, int nmax, bool saves, bool w_in_lds) {
     WGPtrs L;
     L.ld = nmax + 1;
     size_t off = 0;
     L.W = base;
-    off += ((size_t)nmax * L.ld * sizeof(F) + 15) & ~(size_t)15;
+    if (w_in_lds) off += ((size_t)nmax * L.ld * sizeof(F) + 15) & ~(size_t)15;
     double* v = (double*)(base + off);
     L.u = v; L.uprev = v + nmax; L.fsal = v + 2 * nmax; L.du = v + 3 * nmax; L.dT = v + 4 * nmax;
     L.ks = v + 5 * nmax; L.lt = L.ks; L.dense = v + 13 * nmax;
@@ -66,6 +78,11 @@ __device__ __forceinline__ WGPtrs wg_layout(unsigned char* base, int nmax, bool 
     off += (size_t)nmax * 4;
     L.perm = (int*)(base + off);
     off += (size_t)nmax * 4;
+    const int np16 = wg_np(nmax);
+    L.colbuf = (float*)(base + off);
+    L.rowbuf = L.colbuf + 2 * np16;
+    L.zbuf = L.rowbuf + 2 * np16;
+    off += (size_t)5 * np16 * 4;
     off = (off + 15) & ~(size_t)15;
     L.red = (double*)(base + off);
     L.ired = (int*)(L.red + 32);
@@ -271,6 +288,178 @@ __device__ void wg_wsolve(const WGPtrs& S, int n, double* x, double* rb, double*
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-tiled Gauss-Jordan (mixed variant).  Thread (ti, tj) = (tid / 16,
+// tid % 16) of the 256-thread workgroup holds rows ti + 16a and columns
+// tj + 16b (a, b < R) of the matrix, so the 16 threads sharing a row block are
+// one DPP row of a wave.
+// ---------------------------------------------------------------------------
+#if ODEMI_N <= 128
+#define ODEMI_GJ 1
+#define ODEMI_GJ_R ((ODEMI_N + 15) / 16)
+#define ODEMI_GJ_NP (16 * ODEMI_GJ_R)
+
+// Sum over the 16 lanes of each DPP row; the total lands in the row's lane 15.
+// Fixed addition order (inclusive Hillis-Steele scan), so results are
+// deterministic.  All 64 lanes must be active.
+__device__ __forceinline__ float dpp_row_sum16(float x) {
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xf, 0xf, false));  // row_shr:1
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xf, 0xf, false));  // row_shr:2
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xf, 0xf, false));  // row_shr:4
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xf, 0xf, false));  // row_shr:8
+    return x;
+}
+// Maximum over the wave, returned to every lane.  All 64 lanes must be active.
+__device__ __forceinline__ unsigned dpp_wave_max_u32(unsigned x) {
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x111, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x112, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xf, 0xf, false));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+struct GJTile {
+    float T[ODEMI_GJ_R][ODEMI_GJ_R];
+};
+
+// T <- W = J - sigma I for the n x n leading block of J (FP32, row stride ldj),
+// identity outside it.  The diagonal shift is formed in FP64 and rounded once.
+__device__ __forceinline__ void gj_load(GJTile& G, const float* J, int ldj, int n, double sigma) {
+    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+#pragma unroll
+    for (int a = 0; a < ODEMI_GJ_R; a++)
+#pragma unroll
+        for (int b = 0; b < ODEMI_GJ_R; b++) {
+            const int i = ti + 16 * a, j = tj + 16 * b;
+            float v = (i == j) ? 1.0f : 0.0f;
+            if (i < n && j < n) {
+                v = J[i * ldj + j];
+                if (i == j) v = (float)fma(-1.0, sigma, (double)v);
+            }
+            G.T[a][b] = v;
+        }
+}
+
+// In-place Gauss-Jordan with partial pivoting and no row interchanges: step k
+// pivots on the largest |T[i][k]| among rows not yet used (ties to the smaller
+// row) and exchanges the roles of row p_k and column k.  On return
+// W^{-1} b = z[pk], z = T * b[pk] (pk[k] = p_k).  Two barriers per step; the
+// pivot search is repeated by every wave so the pivot needs no broadcast.
+__device__ void gj_invert(GJTile& G, int n, int* pk, float* colbuf, float* rowbuf) {
+    constexpr int R = ODEMI_GJ_R, NP = ODEMI_GJ_NP, NQ = (NP + 63) / 64;
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15, lane = tid & 63;
+    unsigned used = 0;   // bit q: row lane + 64 q has been a pivot row
+#pragma unroll
+    for (int kb = 0; kb < R; kb++) {
+        const int kend = min(16, n - 16 * kb);
+        for (int kt = 0; kt < kend; kt++) {
+            const int k = 16 * kb + kt;
+            float* cb = colbuf + (k & 1) * NP;
+            float* rb = rowbuf + (k & 1) * NP;
+            const bool colk = (tj == kt);
+            if (colk) {
+#pragma unroll
+                for (int a = 0; a < R; a++) cb[ti + 16 * a] = G.T[a][kb];
+            }
+            __syncthreads();
+            unsigned key = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int i = lane + 64 * q;
+                if (i < n && !((used >> q) & 1u)) {
+                    // a NaN entry competes as zero, so p is always an unused row < n
+                    const float v = fabsf(cb[i]);
+                    const unsigned vb = (v != v) ? 0u : (__float_as_uint(v) & ~127u);
+                    key = max(key, vb | (unsigned)(127 - i));
+                }
+            }
+            key = dpp_wave_max_u32(key);
+            const int p = 127 - (int)(key & 127u);
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (p == lane + 64 * q) used |= 1u << q;
+            if (tid == 0) pk[k] = p;
+            const int pti = p & 15, pa = p >> 4;
+            if (ti == pti) {
+#pragma unroll
+                for (int a = 0; a < R; a++)
+                    if (a == pa) {
+#pragma unroll
+                        for (int b = 0; b < R; b++) rb[tj + 16 * b] = G.T[a][b];
+                    }
+            }
+            __syncthreads();
+            const float d = 1.0f / cb[p];
+            float c[R], rs[R];
+#pragma unroll
+            for (int a = 0; a < R; a++) c[a] = cb[ti + 16 * a];
+#pragma unroll
+            for (int b = 0; b < R; b++) rs[b] = rb[tj + 16 * b] * d;
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+#pragma unroll
+                for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-c[a], rs[b], G.T[a][b]);
+                G.T[a][kb] = colk ? -c[a] * d : G.T[a][kb];
+            }
+            if (ti == pti) {
+#pragma unroll
+                for (int a = 0; a < R; a++)
+                    if (a == pa) {
+#pragma unroll
+                        for (int b = 0; b < R; b++) G.T[a][b] = (b == kb && colk) ? d : rs[b];
+                    }
+            }
+        }
+    }
+}
+
+// x <- W^{-1} x (x FP64 in LDS, accumulated in FP32).  All threads must call.
+__device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, double* x, float* zbuf) {
+    constexpr int R = ODEMI_GJ_R;
+    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    float bp[R];
+#pragma unroll
+    for (int b = 0; b < R; b++) {
+        const int j = tj + 16 * b;
+        bp[b] = (j < n) ? (float)x[pk[j]] : 0.0f;
+    }
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        float z = 0.0f;
+#pragma unroll
+        for (int b = 0; b < R; b++) z = fmaf(G.T[a][b], bp[b], z);
+        z = dpp_row_sum16(z);
+        if (tj == 15) zbuf[ti + 16 * a] = z;
+    }
+    __syncthreads();
+    for (int k = tid; k < n; k += blockDim.x) x[k] = (double)zbuf[pk[k]];
+    __syncthreads();
+}
+
+// Stage solve W x = b for the Gauss-Jordan variant (b in x on entry), with the
+// same FP64 refinement sweeps as wg_wsolve.
+__device__ void wg_wsolve_gj(const GJTile& G, const WGPtrs& S, int n, double* x, double* rb, double* r, double* d,
+                             const double* uprev, const double* p, double t, double sigma, int sweeps) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    for (int i = tid; i < n; i += NT) rb[i] = x[i];
+    __syncthreads();
+    gj_solve(G, n, S.perm, x, S.zbuf);
+    for (int sw = 0; sw < sweeps; sw++) {
+        odemi_jvp_wg<double>(r, x, uprev, p, t, n, tid, NT);
+        __syncthreads();
+        for (int i = tid; i < n; i += NT) d[i] = fma(sigma, x[i], rb[i] - r[i]);
+        __syncthreads();
+        gj_solve(G, n, S.perm, d, S.zbuf);
+        for (int i = tid; i < n; i += NT) x[i] = x[i] + d[i];
+        __syncthreads();
+    }
+}
+#else
+#define ODEMI_GJ 0
+#endif
+
 __device__ __forceinline__ void wg_save_point(Ctl& c, double& tlast, const Params& P, i64 j, double t,
                                               const double* v, int n) {
     const int s = c.nsaved;
@@ -370,11 +559,12 @@ __device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, con
     return res;
 }
 
-template <typename F, bool SAVES>
+template <typename F, bool SAVES, bool GJ>
 __device__ void rodas_wg_impl(const Params& P) {
+    static_assert(!GJ || (sizeof(F) == 4 && ODEMI_GJ), "Gauss-Jordan variant: FP32, n <= 128");
     extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
     const int tid = threadIdx.x, NT = blockDim.x;
-    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES);
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES, !GJ);
     const CtlConst kc = ctl_const(true);
     double* u = S.u;
     double* uprev = S.uprev;
@@ -386,6 +576,10 @@ __device__ void rodas_wg_impl(const Params& P) {
     double* red = S.red;
     const int nmax = P.nmax, ld = S.ld;
     F* W = (F*)S.W;
+#if ODEMI_GJ
+    GJTile G;
+    float* Jg = GJ ? P.wg_jbuf + (i64)blockIdx.x * nmax * nmax : nullptr;
+#endif
     for (;;) {
         if (tid == 0) ((i64*)red)[30] = (i64)atomicAdd(&P.counters[0], 1ULL);
         __syncthreads();
@@ -401,6 +595,12 @@ __device__ void rodas_wg_impl(const Params& P) {
         Ctl c;
         ctl_init(c, P, j);
         double tlast = 0.0;
+#if ODEMI_GJ
+        // the traced Jacobian writes the same entries on every call for a given n,
+        // so the rest of the block is cleared once per trajectory
+        if constexpr (GJ)
+            for (int e = tid; e < n * nmax; e += NT) Jg[e] = 0.0f;
+#endif
         __syncthreads();
         if (P.save_start) wg_save_point(c, tlast, P, j, c.t, u, n);
         if (P.dt_user != 0.0) {
@@ -442,11 +642,16 @@ __device__ void rodas_wg_impl(const Params& P) {
                 odemi_f_wg<double>(fsal, uprev, pp, t, n, tid, NT);
                 c.nf += 1;
                 const bool new_jac = (c.iter <= 1) ? true : !(c.iter > 1 && c.EEst > 1.0);
-                for (int e = tid; e < n * ld; e += NT) W[e] = (F)0;
+                if constexpr (!GJ)
+                    for (int e = tid; e < n * ld; e += NT) W[e] = (F)0;
                 if (sizeof(F) != sizeof(double))
                     for (int i = tid; i < n; i += NT) S.fsc[i] = (float)uprev[i];
                 __syncthreads();
-                if constexpr (sizeof(F) == sizeof(double)) {
+                if constexpr (GJ) {
+#if ODEMI_GJ
+                    odemi_jac_wg<float>(Jg, nmax, S.fsc, pf, (float)t, n, tid, NT);
+#endif
+                } else if constexpr (sizeof(F) == sizeof(double)) {
                     odemi_jac_wg<double>((double*)W, ld, uprev, pp, t, n, tid, NT);
                 } else {
                     odemi_jac_wg<float>((float*)W, ld, S.fsc, pf, (float)t, n, tid, NT);
@@ -456,8 +661,9 @@ __device__ void rodas_wg_impl(const Params& P) {
                     for (int i = tid; i < n; i += NT) dT[i] = 0.0;
                 }
                 __syncthreads();
-                for (int i = tid; i < n; i += NT)
-                    W[i * ld + i] = (F)fma(-1.0, sigma, (double)W[i * ld + i]);
+                if constexpr (!GJ)
+                    for (int i = tid; i < n; i += NT)
+                        W[i * ld + i] = (F)fma(-1.0, sigma, (double)W[i * ld + i]);
                 if (new_jac) {
                     odemi_tgrad_wg<double>(dT, uprev, pp, t, n, tid, NT);
                     if (!ODEMI_HAS_TGRAD) c.nf += 1;
@@ -466,9 +672,17 @@ __device__ void rodas_wg_impl(const Params& P) {
                 __syncthreads();
                 const double dtd0 = dt * kRd[0];
                 for (int i = tid; i < n; i += NT) ks[i] = fsal[i] + dtd0 * dT[i];
-                wg_lu<F>(W, ld, n, S.piv, S.perm, red, S.ired);
                 // rb = fsal, r = du, d = u are free during the stage-1 solve
-                wg_wsolve<F>(S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                if constexpr (GJ) {
+#if ODEMI_GJ
+                    gj_load(G, Jg, nmax, n, sigma);
+                    gj_invert(G, n, S.perm, S.colbuf, S.rowbuf);
+                    wg_wsolve_gj(G, S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+#endif
+                } else {
+                    wg_lu<F>(W, ld, n, S.piv, S.perm, red, S.ired);
+                    wg_wsolve<F>(S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                }
                 for (int i = tid; i < n; i += NT) ks[i] = -ks[i];
                 c.nsolve += 1;
                 __syncthreads();
@@ -490,7 +704,13 @@ __device__ void rodas_wg_impl(const Params& P) {
                         x[i] = acc;
                     }
                     __syncthreads();
-                    wg_wsolve<F>(S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                    if constexpr (GJ) {
+#if ODEMI_GJ
+                        wg_wsolve_gj(G, S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+#endif
+                    } else {
+                        wg_wsolve<F>(S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                    }
                     for (int i = tid; i < n; i += NT) x[i] = -x[i];
                     c.nsolve += 1;
                     __syncthreads();

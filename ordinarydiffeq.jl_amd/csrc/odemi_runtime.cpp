@@ -65,6 +65,7 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t slot0, nslots;
     const int* slot_req;
     unsigned long long* req_done;
+    float* wg_jbuf;
 };
 
 // Longest-predicted-cost-first work order: key_j = (tf_j - t0_j) * n_j^3 as FP32.
@@ -245,11 +246,14 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
              " { odemi::rodas_plain_step<ODEMI_N>(P); }\n";
     }
     if (flags & ODEMI_COMPILE_WG) {
-        const char* v[4][3] = {{"odemi_rodas_wg_f64", "double", "false"}, {"odemi_rodas_wg_f64_sv", "double", "true"},
-                               {"odemi_rodas_wg_mixed", "float", "false"}, {"odemi_rodas_wg_mixed_sv", "float", "true"}};
+        // the mixed variant uses the register-tiled Gauss-Jordan solver when n <= 128
+        const char* v[4][4] = {{"odemi_rodas_wg_f64", "double", "false", "false"},
+                               {"odemi_rodas_wg_f64_sv", "double", "true", "false"},
+                               {"odemi_rodas_wg_mixed", "float", "false", "(ODEMI_GJ != 0)"},
+                               {"odemi_rodas_wg_mixed_sv", "float", "true", "(ODEMI_GJ != 0)"}};
         for (auto& k : v)
             s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) " << k[0]
-              << "(odemi::Params P) { odemi::rodas_wg_impl<" << k[1] << ", " << k[2] << ">(P); }\n";
+              << "(odemi::Params P) { odemi::rodas_wg_impl<" << k[1] << ", " << k[2] << ", " << k[3] << ">(P); }\n";
     }
     return s.str();
 }
@@ -468,6 +472,33 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     const size_t N_ = (size_t)N;
     const size_t ms_ = (size_t)std::max(a->max_saves, 0);
     const size_t vrows = rodas ? (size_t)(3 + nmax) : 4;
+    // ---- launch geometry of the persistent kernels ----
+    const int block = a->block > 0 ? a->block : 256;
+    const int threads = use_wg ? 256 : block;   // ODEMI_WG_THREADS for the workgroup kernels
+    const bool use_gj = use_wg && mixed && nmax <= 128;   // register-tiled Gauss-Jordan variant
+    size_t shmem = 0;
+    int64_t grid = 0;
+    if (!plain) {
+        if (use_wg) {
+            // must match odemi::wg_layout in kernels/odemi_wg.h
+            const size_t ld = (size_t)nmax + 1, fb = mixed ? 4 : 8;
+            const size_t np16 = 16 * (((size_t)nmax + 15) / 16);
+            const bool saves = a->nsaveat > 0 || a->save_everystep;
+            shmem = use_gj ? 0 : (((size_t)nmax * ld * fb + 15) & ~(size_t)15);
+            shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
+            shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 8;  // fsc + pf, piv + perm
+            shmem += 5 * np16 * 4;                                 // colbuf, rowbuf, zbuf
+            shmem = (shmem + 15) & ~(size_t)15;
+            shmem += 32 * 8 + 32 * 4;
+            if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
+        }
+        int nb = 0;
+        HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, threads, shmem));
+        if (nb < 1) nb = 1;
+        grid = (int64_t)nb * std::max(ctx->cu_count, 1);
+        const int64_t per_block = use_wg ? 1 : threads;
+        grid = std::min<int64_t>(grid, (N + per_block - 1) / per_block);
+    }
     // ---- workspace layout ----
     size_t need = 0;
     auto add = [&](size_t b) { need += ((b + 255) & ~size_t(255)); };
@@ -480,6 +511,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         add(N_ * 16); add(sort_tmp + 1024);
     }
     if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(12 * N_ * 4); }
+    if (use_gj) add((size_t)grid * nmax * nmax * 4);
     if (!dev) {
         add(N_ * nmax * 8); add(N_ * std::max(np, 1) * 8); add(N_ * 8); add(N_ * 8);
         add(std::max(a->nsaveat, 1) * 8);
@@ -548,9 +580,9 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         P.st_int = (int*)ar.take(12 * N_ * 4);
         HIPCHK(ctx, hipMemsetAsync(P.st_int, 0, 12 * N_ * 4, s));
     }
+    if (use_gj) P.wg_jbuf = (float*)ar.take((size_t)grid * nmax * nmax * 4);
     if (P.nsaved) HIPCHK(ctx, hipMemsetAsync(P.nsaved, 0, N_ * 4, s));
 
-    const int block = a->block > 0 ? a->block : 256;
     int64_t launches = 0;
     HIPCHK(ctx, hipEventRecord(ctx->ev0, s));
     if (plain) {
@@ -570,25 +602,6 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
             if (launches > (int64_t)a->maxiters * 4 + 1024) { ctx->err = "plain engine did not terminate"; return -4; }
         }
     } else {
-        const int threads = use_wg ? 256 : block;   // ODEMI_WG_THREADS for the workgroup kernels
-        size_t shmem = 0;
-        if (use_wg) {
-            // must match odemi::wg_layout in kernels/odemi_wg.h
-            const size_t ld = (size_t)nmax + 1, fb = mixed ? 4 : 8;
-            const bool saves = a->nsaveat > 0 || a->save_everystep;
-            shmem = ((size_t)nmax * ld * fb + 15) & ~(size_t)15;
-            shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
-            shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 8;  // fsc + pf, piv + perm
-            shmem = (shmem + 15) & ~(size_t)15;
-            shmem += 32 * 8 + 32 * 4;
-            if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
-        }
-        int nb = 0;
-        HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, threads, shmem));
-        if (nb < 1) nb = 1;
-        int64_t grid = (int64_t)nb * std::max(ctx->cu_count, 1);
-        const int64_t per_block = use_wg ? 1 : threads;
-        grid = std::min<int64_t>(grid, (N + per_block - 1) / per_block);
         if (launch(ctx, fn_main, (int)grid, threads, P, s, shmem)) return -1;
         launches++;
     }
