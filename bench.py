@@ -138,13 +138,99 @@ def bench_rodas(args, workload):
     print(json.dumps(res))
 
 
+def bench_serving(args):
+    """Open-loop Poisson arrivals of ensemble requests of ragged size, served by
+    continuous batching (one EnsembleServer per instance type, each with its own
+    stream and worker thread).  Latency = completion - scheduled arrival."""
+    import threading
+    import odemi
+    from odemi import problems as P
+    rate, dur = args.rate, args.duration
+    rng = np.random.Generator(np.random.PCG64(7))
+    pools = {
+        "lorenz_small": (P.lorenz_small(N=65536), P.LORENZ, odemi.Tsit5(), "fp64"),
+        "lorenz_large": (P.lorenz_large(N=262144), P.LORENZ, odemi.Tsit5(), "mixed"),
+        "robertson": (P.robertson_ensemble(N=262144), P.ROBERTSON, odemi.Rodas5P(), "mixed"),
+    }
+    kinds = list(pools)
+    servers = {}
+    for k, (d, fun, alg, prec) in pools.items():
+        servers[k] = odemi.EnsembleServer(fun, d["u0"].shape[0], d["p"].shape[0], alg, precision=prec,
+                                          reltol=d["reltol"], abstol=d["abstol"], tick_attempts=args.tick_attempts,
+                                          hbm_budget_bytes=args.hbm_budget_gb * (1 << 30) // len(pools))
+    # schedule
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 16))
+    t_arr = t_arr[t_arr < dur]
+    nreq = len(t_arr)
+    kind = rng.integers(0, len(kinds), nreq)
+    size = rng.integers(1, args.max_request + 1, nreq)
+    off = [int(rng.integers(0, pools[kinds[k]][0]["u0"].shape[1] - s + 1)) for k, s in zip(kind, size)]
+    # warm the kernels (hipRTC compile happened at server creation)
+    for k in kinds:
+        d = pools[k][0]
+        servers[k].submit(d["u0"][:, :8], d["p"][:, :8], d["t0"][:8], d["tf"][:8]).wait(timeout=120)
+    print(f"[bench] serving {nreq} requests over {dur:.0f} s at {rate:.0f} req/s", file=sys.stderr, flush=True)
+    reqs = [None] * nreq
+    lag = np.zeros(nreq)
+    t0 = time.monotonic() + 0.05
+    for q in range(nreq):
+        ts = t0 + t_arr[q]
+        while True:
+            now = time.monotonic()
+            if now >= ts:
+                break
+            if ts - now > 0.002:
+                time.sleep(ts - now - 0.001)
+        k = kinds[kind[q]]
+        d = pools[k][0]
+        a, b = off[q], off[q] + int(size[q])
+        reqs[q] = servers[k].submit(d["u0"][:, a:b], d["p"][:, a:b], d["t0"][a:b], d["tf"][a:b], stats=False)
+        lag[q] = time.monotonic() - ts
+        if q % 2000 == 0:
+            print(f"[bench] submitted {q}/{nreq}", file=sys.stderr, flush=True)
+    for r in reqs:
+        r.wait(timeout=600)
+    lat = np.array([r.times()[2] for r in reqs]) - (t0 + t_arr)
+    t_end = max(r.times()[2] for r in reqs)
+    ok = np.mean([float((r.retcode == 1).mean()) for r in reqs])
+    info = {k: s.info() for k, s in servers.items()}
+    for s_ in servers.values():
+        s_.close()
+    keep = t_arr >= 0.1 * dur                   # drop the first 10% of the window (warm-up)
+    traj = int(size.sum())
+    res = {
+        "metric": "serving: trajectories/s and p50/p99 request latency, open-loop Poisson arrivals",
+        "value": traj / (t_end - t0), "unit": "trajectories/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": (t_end - t0) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (Lorenz reltol 1e-8), f32 stages/LU + f64 state (Lorenz 1e-6, Robertson)",
+        "data": "synthetic (seeded PCG64)",
+        "config": {"workload": f"open-loop Poisson arrivals at {rate:.0f} req/s for {dur:.0f} s; each request "
+                               f"1..{args.max_request} trajectories (uniform) of one of: Lorenz (0,1) reltol 1e-8 "
+                               "Tsit5; Lorenz t_end~U[1,10] reltol 1e-6 Tsit5; Robertson (0,1e5) Rodas5P",
+                   "requests": nreq, "trajectories": traj,
+                   "parallelism": "one MI355X, one server (stream + worker thread) per instance type"},
+        "latency_ms": {"p50": float(np.quantile(lat[keep], 0.5) * 1e3), "p99": float(np.quantile(lat[keep], 0.99) * 1e3),
+                       "max": float(lat[keep].max() * 1e3), "mean": float(lat[keep].mean() * 1e3)},
+        "submit_lag_ms": {"p99": float(np.quantile(lag, 0.99) * 1e3), "max": float(lag.max() * 1e3)},
+        "success_fraction": float(ok),
+        "servers": info,
+        "roofline": None,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=0)
-    ap.add_argument("--workload", default="lorenz", choices=["lorenz", "robertson", "chain"])
+    ap.add_argument("--workload", default="lorenz", choices=["lorenz", "robertson", "chain", "serving"])
+    ap.add_argument("--rate", type=float, default=2000.0, help="serving: requests/s")
+    ap.add_argument("--duration", type=float, default=60.0, help="serving: arrival window, s")
+    ap.add_argument("--max-request", type=int, default=4096, help="serving: trajectories per request, max")
+    ap.add_argument("--tick-attempts", type=int, default=32, help="serving: step attempts per tick")
+    ap.add_argument("--hbm-budget-gb", type=int, default=8, help="serving: slot-pool budget, GiB")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1048576)
@@ -153,6 +239,8 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "serving":
+        return bench_serving(args)
     if args.workload != "lorenz":
         return bench_rodas(args, args.workload)
     args.N = args.N or 1048576

@@ -14,8 +14,8 @@
 //   pd   double[32]                       parameters (FP64)
 //   fsc  float[nmax + 32]                 FP32 copies of uprev and p (mixed variant)
 //   piv  int[nmax], perm int[nmax]        pivots and the equivalent row gather
-//   colbuf, rowbuf float[2][NP], zbuf float[NP]   Gauss-Jordan broadcasts (NP = nmax
-//                                         rounded up to 16)
+//   gjbuf float[6 NP + 2 R NP]            Gauss-Jordan broadcasts and mat-vec
+//                                         result (mixed variant; NP = 16 R, R = ceil(nmax/16))
 //   red  double[32] + int[32]             reductions / broadcasts
 // FP64 variant: W is FP64 in LDS (132 KB at n = 128, one workgroup per CU),
 // factored by LU with partial pivoting.
@@ -49,13 +49,19 @@ struct WGPtrs {
     float* pf;
     int* piv;
     int* perm;   // row gather: after the LU, (P b)[i] = b[perm[i]]; Gauss-Jordan: pivot row of step k
-    float *colbuf, *rowbuf, *zbuf;
+    float* gjbuf;   // Gauss-Jordan broadcasts; zbuf = gjbuf + 5 NP + 2 R NP
     double* red;
     int* ired;
     int ld;
 };
 
-__host__ __device__ inline int wg_np(int nmax) { return 16 * ((nmax + 15) / 16); }
+// Gauss-Jordan broadcast buffers (floats): column k and k+1 [2][NP], row staging
+// [2][R][NP], pivot-row scales [NP] and the mat-vec result zbuf [NP];
+// NP = 16 R, R = ceil(nmax / 16).
+__host__ __device__ inline int wg_gj_floats(int nmax, bool gj) {
+    const int r = (nmax + 15) / 16, np16 = 16 * r;
+    return gj ? 4 * np16 + 2 * r * np16 + 2 * np16 : 0;
+}
 
 // w_in_lds: W lives in LDS (LU variants); the Gauss-Jordan variant keeps it in registers.
 template <typename F>
@@ -78,11 +84,9 @@ __device__ __forceinline__ WGPtrs wg_layout(unsigned char* base, int nmax, bool 
     off += (size_t)nmax * 4;
     L.perm = (int*)(base + off);
     off += (size_t)nmax * 4;
-    const int np16 = wg_np(nmax);
-    L.colbuf = (float*)(base + off);
-    L.rowbuf = L.colbuf + 2 * np16;
-    L.zbuf = L.rowbuf + 2 * np16;
-    off += (size_t)5 * np16 * 4;
+    off = (off + 15) & ~(size_t)15;
+    L.gjbuf = (float*)(base + off);
+    off += (size_t)wg_gj_floats(nmax, !w_in_lds) * 4;
     off = (off + 15) & ~(size_t)15;
     L.red = (double*)(base + off);
     L.ired = (int*)(L.red + 32);
@@ -343,75 +347,165 @@ __device__ __forceinline__ void gj_load(GJTile& G, const float* J, int ldj, int 
 }
 
 // In-place Gauss-Jordan with partial pivoting and no row interchanges: step k
-// pivots on the largest |T[i][k]| among rows not yet used (ties to the smaller
+// pivots on the largest |W[i][k]| among rows not yet used (ties to the smaller
 // row) and exchanges the roles of row p_k and column k.  On return
-// W^{-1} b = z[pk], z = T * b[pk] (pk[k] = p_k).  Two barriers per step; the
-// pivot search is repeated by every wave so the pivot needs no broadcast.
-__device__ void gj_invert(GJTile& G, int n, int* pk, float* colbuf, float* rowbuf) {
+// W^{-1} b = z[pk], z = T * b[pk] (pk[k] = p_k).
+//
+// The tile holds raw rows R with W = diag(sc) R.  A pivot row is not rescaled
+// in the tile: its scale sc_p becomes d = 1/W[p][k] and its raw entries stay,
+// so the only per-element work of a step is the rank-1 update of the other
+// rows, and no thread has to write a register chosen by a runtime index.
+// Rows not yet used keep sc = 1, so the pivot search and the pivot row read
+// raw values.  The scales (one LDS word per row) are folded into the tile at
+// the end.
+//
+// One barrier per step.  Before the barrier that opens step k the owners
+// publish column k, column k+1 and (all R row slots of the thread row holding)
+// row p_k, as step k-1 left them.  After its update every wave forms column
+// k+1 as step k leaves it (the same fma as the owners' update, so
+// bit-identical) and searches it for p_{k+1}, so the owners of row p_{k+1}
+// can stage it before the next barrier.  Buffers alternate by step parity.
+//   cbuf, nbuf [2][16][R]: entry (ti, a) is row ti + 16 a
+//   stg [2][R][16][R]:     entry (a, tj, b) is T[a][b] of thread (p_k % 16, tj)
+__device__ __forceinline__ unsigned gj_key(float v, int i) {
+    // larger |v| wins, ties to the smaller row; a NaN entry competes as zero so
+    // the pivot is always an unused row < n (n <= 128)
+    const float a = fabsf(v);
+    return ((a != a) ? 0u : (__float_as_uint(a) & ~127u)) | (unsigned)(127 - i);
+}
+__device__ __forceinline__ int gj_cidx(int i) { return (i & 15) * ODEMI_GJ_R + (i >> 4); }
+
+__device__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
     constexpr int R = ODEMI_GJ_R, NP = ODEMI_GJ_NP, NQ = (NP + 63) / 64;
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15, lane = tid & 63;
+    float* cbuf = gjbuf;
+    float* nbuf = gjbuf + 2 * NP;
+    float* stg = gjbuf + 4 * NP;
+    float* scl = gjbuf + 4 * NP + 2 * R * NP;   // [NP] scale of each pivot row
     unsigned used = 0;   // bit q: row lane + 64 q has been a pivot row
+    // ---- prologue: columns 0 and 1, pivot p_0, row p_0 ----
+    if (tj == 0) {
+#pragma unroll
+        for (int a = 0; a < R; a++) cbuf[ti * R + a] = G.T[a][0];
+    }
+    if (tj == 1 && n > 1) {
+#pragma unroll
+        for (int a = 0; a < R; a++) nbuf[ti * R + a] = G.T[a][0];
+    }
+    __syncthreads();
+    int p;
+    {
+        unsigned key = 0;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int i = lane + 64 * q;
+            if (i < n) key = max(key, gj_key(cbuf[gj_cidx(i)], i));
+        }
+        key = dpp_wave_max_u32(key);
+        p = 127 - (int)(key & 127u);
+    }
+    if (tid == 0) pk[0] = p;
+    if (ti == (p & 15)) {
+#pragma unroll
+        for (int a = 0; a < R; a++)
+#pragma unroll
+            for (int b = 0; b < R; b++) stg[(a * 16 + tj) * R + b] = G.T[a][b];
+    }
+    __syncthreads();
+    // ---- steps ----
 #pragma unroll
     for (int kb = 0; kb < R; kb++) {
         const int kend = min(16, n - 16 * kb);
         for (int kt = 0; kt < kend; kt++) {
-            const int k = 16 * kb + kt;
-            float* cb = colbuf + (k & 1) * NP;
-            float* rb = rowbuf + (k & 1) * NP;
-            const bool colk = (tj == kt);
-            if (colk) {
+            const int k = 16 * kb + kt, k1 = k + 1, k2 = k + 2;
+            const int par = k & 1;
 #pragma unroll
-                for (int a = 0; a < R; a++) cb[ti + 16 * a] = G.T[a][kb];
+            for (int q = 0; q < NQ; q++)
+                if (p == lane + 64 * q) used |= 1u << q;
+            const float* cb = cbuf + par * NP;
+            const float* nb = nbuf + par * NP;
+            const float* st = stg + par * R * NP;
+            const int pa = p >> 4;
+            const bool own = (ti == (p & 15));
+            const bool colk = (tj == kt);
+            // all reads of the step first (the search for p_{k+1} only needs
+            // these, so its latency overlaps the tile update).  At the last step
+            // k1 = n <= NP: the reads stay inside the buffers and are unused.
+            const float piv = cb[gj_cidx(p)];
+            float c[R], r[R], cq[NQ], nq[NQ];
+#pragma unroll
+            for (int a = 0; a < R; a++) c[a] = cb[ti * R + a];
+#pragma unroll
+            for (int b = 0; b < R; b++) r[b] = st[(pa * 16 + tj) * R + b];
+            const float r1 = st[(pa * 16 + (k1 & 15)) * R + (k1 >> 4)];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int ci = gj_cidx(min(lane + 64 * q, NP - 1));
+                cq[q] = cb[ci];
+                nq[q] = nb[ci];
             }
-            __syncthreads();
+            const float d = __builtin_amdgcn_rcpf(piv);
+            // pivot of step k+1 from column k+1 as step k leaves it (same fma as
+            // the owners' update)
+            const float rs1 = r1 * d;
             unsigned key = 0;
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
                 const int i = lane + 64 * q;
-                if (i < n && !((used >> q) & 1u)) {
-                    // a NaN entry competes as zero, so p is always an unused row < n
-                    const float v = fabsf(cb[i]);
-                    const unsigned vb = (v != v) ? 0u : (__float_as_uint(v) & ~127u);
-                    key = max(key, vb | (unsigned)(127 - i));
-                }
+                if (i < n && !((used >> q) & 1u)) key = max(key, gj_key(fmaf(-cq[q], rs1, nq[q]), i));
             }
-            key = dpp_wave_max_u32(key);
-            const int p = 127 - (int)(key & 127u);
-#pragma unroll
-            for (int q = 0; q < NQ; q++)
-                if (p == lane + 64 * q) used |= 1u << q;
-            if (tid == 0) pk[k] = p;
-            const int pti = p & 15, pa = p >> 4;
-            if (ti == pti) {
-#pragma unroll
-                for (int a = 0; a < R; a++)
-                    if (a == pa) {
-#pragma unroll
-                        for (int b = 0; b < R; b++) rb[tj + 16 * b] = G.T[a][b];
-                    }
-            }
-            __syncthreads();
-            const float d = 1.0f / cb[p];
-            float c[R], rs[R];
-#pragma unroll
-            for (int a = 0; a < R; a++) c[a] = cb[ti + 16 * a];
-#pragma unroll
-            for (int b = 0; b < R; b++) rs[b] = rb[tj + 16 * b] * d;
+            // rank-1 update of every row but p_k; column k becomes -c d (1 on row p_k)
 #pragma unroll
             for (int a = 0; a < R; a++) {
+                const bool isp = own && (a == pa);
+                const float ct = isp ? 0.0f : c[a];
 #pragma unroll
-                for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-c[a], rs[b], G.T[a][b]);
-                G.T[a][kb] = colk ? -c[a] * d : G.T[a][kb];
+                for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-ct, r[b] * d, G.T[a][b]);
+                G.T[a][kb] = colk ? (isp ? 1.0f : -c[a] * d) : G.T[a][kb];
             }
-            if (ti == pti) {
+            if (tid == 0) scl[p] = d;
+            key = dpp_wave_max_u32(key);
+            if (k1 >= n) break;
+            p = 127 - (int)(key & 127u);
+            if (tid == 0) pk[k1] = p;
+            // publish column k+1, column k+2 and row p_{k+1} for the next step
+            float* cbn = cbuf + (NP - par * NP);
+            float* nbn = nbuf + (NP - par * NP);
+            float* stn = stg + (R * NP - par * R * NP);
+            if (tj == (k1 & 15)) {
+                if (kt < 15) {
+#pragma unroll
+                    for (int a = 0; a < R; a++) cbn[ti * R + a] = G.T[a][kb];
+                } else {
+#pragma unroll
+                    for (int a = 0; a < R; a++) cbn[ti * R + a] = G.T[a][kb + 1 < R ? kb + 1 : kb];
+                }
+            }
+            if (k2 < n && tj == (k2 & 15)) {
+                if (kt < 14) {
+#pragma unroll
+                    for (int a = 0; a < R; a++) nbn[ti * R + a] = G.T[a][kb];
+                } else {
+#pragma unroll
+                    for (int a = 0; a < R; a++) nbn[ti * R + a] = G.T[a][kb + 1 < R ? kb + 1 : kb];
+                }
+            }
+            if (ti == (p & 15)) {
 #pragma unroll
                 for (int a = 0; a < R; a++)
-                    if (a == pa) {
 #pragma unroll
-                        for (int b = 0; b < R; b++) G.T[a][b] = (b == kb && colk) ? d : rs[b];
-                    }
+                    for (int b = 0; b < R; b++) stn[(a * 16 + tj) * R + b] = G.T[a][b];
             }
+            __syncthreads();
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        const int i = ti + 16 * a;
+        const float sa = (i < n) ? scl[i] : 1.0f;
+#pragma unroll
+        for (int b = 0; b < R; b++) G.T[a][b] *= sa;
     }
 }
 
@@ -445,13 +539,13 @@ __device__ void wg_wsolve_gj(const GJTile& G, const WGPtrs& S, int n, double* x,
     const int tid = threadIdx.x, NT = blockDim.x;
     for (int i = tid; i < n; i += NT) rb[i] = x[i];
     __syncthreads();
-    gj_solve(G, n, S.perm, x, S.zbuf);
+    gj_solve(G, n, S.perm, x, S.gjbuf + 5 * ODEMI_GJ_NP + 2 * ODEMI_GJ_R * ODEMI_GJ_NP);
     for (int sw = 0; sw < sweeps; sw++) {
         odemi_jvp_wg<double>(r, x, uprev, p, t, n, tid, NT);
         __syncthreads();
         for (int i = tid; i < n; i += NT) d[i] = fma(sigma, x[i], rb[i] - r[i]);
         __syncthreads();
-        gj_solve(G, n, S.perm, d, S.zbuf);
+        gj_solve(G, n, S.perm, d, S.gjbuf + 5 * ODEMI_GJ_NP + 2 * ODEMI_GJ_R * ODEMI_GJ_NP);
         for (int i = tid; i < n; i += NT) x[i] = x[i] + d[i];
         __syncthreads();
     }
@@ -473,16 +567,6 @@ __device__ __forceinline__ void wg_save_point(Ctl& c, double& tlast, const Param
 
 // Index-order fma-chain norms, evaluated by one thread (bit-identical to the
 // sequential restatement).
-__device__ __forceinline__ double wg_residual_norm(const double* ut, const double* u0, const double* u1, int n,
-                                                   double abstol, double reltol) {
-    double s = 0.0;
-    for (int i = 0; i < n; i++) {
-        double den = fma(max_fast(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
-        double r = ut[i] / den;
-        s = fma(r, r, s);
-    }
-    return sqrt(s / (double)(n > 1 ? n : 1));
-}
 __device__ __forceinline__ double wg_norm_div(const double* a, const double* sk, int n) {
     double s = 0.0;
     for (int i = 0; i < n; i++) { double x = a[i] / sk[i] * 1.0; s = fma(x, x, s); }
@@ -676,7 +760,7 @@ __device__ void rodas_wg_impl(const Params& P) {
                 if constexpr (GJ) {
 #if ODEMI_GJ
                     gj_load(G, Jg, nmax, n, sigma);
-                    gj_invert(G, n, S.perm, S.colbuf, S.rowbuf);
+                    gj_invert(G, n, S.perm, S.gjbuf);
                     wg_wsolve_gj(G, S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
 #endif
                 } else {
@@ -734,7 +818,16 @@ __device__ void rodas_wg_impl(const Params& P) {
                         }
                 }
                 __syncthreads();
-                if (tid == 0) red[24] = wg_residual_norm(du, uprev, u, n, P.abstol, P.reltol);
+                // error norm: the scaled residuals in parallel, then one thread's
+                // index-order fma chain (the same value as the sequential norm)
+                for (int i = tid; i < n; i += NT)
+                    fsal[i] = du[i] / fma(max_fast(fabs(uprev[i]), fabs(u[i])), P.reltol, P.abstol);
+                __syncthreads();
+                if (tid == 0) {
+                    double sq = 0.0;
+                    for (int i = 0; i < n; i++) sq = fma(fsal[i], fsal[i], sq);
+                    red[24] = sqrt(sq / (double)(n > 1 ? n : 1));
+                }
                 __syncthreads();
                 c.EEst = red[24];
             }

@@ -251,9 +251,16 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
                                {"odemi_rodas_wg_f64_sv", "double", "true", "false"},
                                {"odemi_rodas_wg_mixed", "float", "false", "(ODEMI_GJ != 0)"},
                                {"odemi_rodas_wg_mixed_sv", "float", "true", "(ODEMI_GJ != 0)"}};
-        for (auto& k : v)
-            s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) " << k[0]
-              << "(odemi::Params P) { odemi::rodas_wg_impl<" << k[1] << ", " << k[2] << ", " << k[3] << ">(P); }\n";
+        // register cap for the mixed variant: minimum waves per SIMD (more
+        // workgroups per CU against a few spills; 3 measured best at n = 128)
+        const char* ww = getenv("ODEMI_WG_MIN_WAVES");
+        const int wgw = (ww && *ww) ? atoi(ww) : 3;
+        for (auto& k : v) {
+            s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) ";
+            if (wgw > 0 && k[1][0] == 'f') s << "__attribute__((amdgpu_waves_per_eu(" << wgw << "))) ";
+            s << k[0] << "(odemi::Params P) { odemi::rodas_wg_impl<" << k[1] << ", " << k[2] << ", " << k[3]
+              << ">(P); }\n";
+        }
     }
     return s.str();
 }
@@ -487,7 +494,9 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
             shmem = use_gj ? 0 : (((size_t)nmax * ld * fb + 15) & ~(size_t)15);
             shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
             shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 8;  // fsc + pf, piv + perm
-            shmem += 5 * np16 * 4;                                 // colbuf, rowbuf, zbuf
+            shmem = (shmem + 15) & ~(size_t)15;
+            const size_t r16 = np16 / 16;
+            if (use_gj) shmem += (4 * np16 + 2 * r16 * np16 + 2 * np16) * 4;  // Gauss-Jordan buffers + zbuf
             shmem = (shmem + 15) & ~(size_t)15;
             shmem += 32 * 8 + 32 * 4;
             if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }

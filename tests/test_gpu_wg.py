@@ -55,6 +55,23 @@ def test_chain_large_n_fp64_and_mixed():
     assert np.quantile(err.max(axis=0), 0.99) <= 10.0
 
 
+def test_chain_uneven_mixed_gauss_jordan_within_tolerance():
+    """Mixed variant (register-tiled Gauss-Jordan, FP32, FP64 refinement) on an
+    uneven batch n = 2..48 against the FP64 restatement."""
+    d = _chain(192, m_lo=1, m_hi=24, seed=12)
+    print("[case] chain uneven n 2..48 wg mixed", flush=True)
+    ref = _oracle(d)
+    mx = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(precision="mixed"), reltol=d["reltol"], abstol=d["abstol"],
+                            nvec=d["nvec"])
+    assert (mx["retcode"] == 1).all()
+    err = np.abs(mx["u_final"] - ref["u_final"]) / (d["abstol"] + d["reltol"] * np.abs(ref["u_final"]))
+    same = (mx["stats"][1] == ref["naccept"]).mean()
+    print(f"mixed scaled err p99 {np.quantile(err.max(axis=0), 0.99):.3e} max {err.max():.3e} same steps {same:.3f}")
+    assert np.quantile(err.max(axis=0), 0.99) <= 10.0
+    assert same >= 0.9
+
+
 def test_wg_kernel_robertson_matches_lane_kernel():
     """n = 3 through the workgroup kernel (nvec given) equals the lane kernel bit for bit."""
     d = P.robertson_ensemble(N=128)
