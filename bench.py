@@ -41,6 +41,17 @@ def rodas_flops_per_attempt(F_f, F_J, n, saves=False):
     return 8 * F_f + F_J + (2 * n ** 3 / 3 - n ** 2 / 2 - n / 6) + 16 * n ** 2 + 175 * n + 60 + (48 * n if saves else 0)
 
 
+def _traffic(key):
+    """HBM bytes per launch of a kernel on its bench workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json (scaled by a
+    copy-kernel calibration there)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)[key]["bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _timed(run, steps, warmup, torch):
     out = None
     for _ in range(warmup):
@@ -86,6 +97,9 @@ def bench_rodas(args, workload):
     mixed = odemi.EnsembleMI355X(precision="mixed")
     fp64 = odemi.EnsembleMI355X(precision="fp64")
     run = lambda ea: odemi.solve_arrays(fun, odemi.Rodas5P(), u0, p, t0, tf, ea, **kw)
+    # hipRTC compile (cached on disk) stays outside every timed region
+    odemi.compile_problem(fun, d["u0"].shape[0], d["p"].shape[0], algs=("rodas5p",),
+                          ns=None if nvec is None else sorted(set(int(x) for x in nvec)))
     out, ms, kms = _timed(lambda: run(mixed), args.steps, args.warmup, torch)
     st = out["stats"].cpu().numpy()
     att = (st[1] + st[2]).astype(np.float64)
@@ -260,6 +274,8 @@ def main():
     def run(ea):
         return odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), u0, p, t0, tf, ea, **kw)
 
+    odemi.compile_problem(P.LORENZ, 3, 3, algs=("tsit5",))   # hipRTC compile outside the timed region
+
     print(f"[bench] warmup x{args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         out = run(batched)
@@ -293,7 +309,8 @@ def main():
         "step_attempts": attempts, "success_fraction": ok,
         "roofline": {"bound": "valu-fp32", "achieved": flops / (kernel_ms / 1e3) / 1e12,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": flops / (kernel_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": flops / (kernel_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS,
+                     "traffic": _traffic("odemi_tsit5_persist_mixed@lorenz_1m") if args.N == 1048576 else None,
                      "kernel": "odemi_tsit5_persist_mixed", "kernel_ms": kernel_ms,
                      "flops_per_attempt": tsit5_flops_per_attempt(F_f, 3)},
     }
