@@ -155,8 +155,18 @@ def bench_rodas(args, workload):
 def bench_serving(args):
     """Open-loop Poisson arrivals of ensemble requests of ragged size, served by
     continuous batching (one EnsembleServer per instance type, each with its own
-    stream and worker thread).  Latency = completion - scheduled arrival."""
-    import threading
+    stream and worker thread).  Latency = completion - scheduled arrival.
+
+    Two merged arrival streams: `rate` req/s over the lane-sized instance types
+    (Lorenz at reltol 1e-8 and 1e-6, Robertson; 1..max_request trajectories) and
+    `chain_rate` req/s of uneven oscillator chains n = 8..128 (1..chain_max
+    trajectories, workgroup kernels), whose per-trajectory cost is ~10^4 times
+    a Lorenz trajectory's."""
+    # one hardware queue per server stream (4 servers + the context stream), so a
+    # long workgroup tick on the chain server's stream never sits in front of a
+    # lane server's tick in a shared in-order queue
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     import odemi
     from odemi import problems as P
     rate, dur = args.rate, args.duration
@@ -166,24 +176,44 @@ def bench_serving(args):
         "lorenz_large": (P.lorenz_large(N=262144), P.LORENZ, odemi.Tsit5(), "mixed"),
         "robertson": (P.robertson_ensemble(N=262144), P.ROBERTSON, odemi.Rodas5P(), "mixed"),
     }
+    if args.chain_rate > 0:
+        pools["chain"] = (P.chain_ensemble(N=16384), P.CHAIN, odemi.Rodas5P(), "mixed")
     kinds = list(pools)
     servers = {}
+    budget = args.hbm_budget_gb * (1 << 30) // len(pools)
     for k, (d, fun, alg, prec) in pools.items():
+        ns = sorted(set(int(x) for x in d["nvec"])) if "nvec" in d else None
         servers[k] = odemi.EnsembleServer(fun, d["u0"].shape[0], d["p"].shape[0], alg, precision=prec,
-                                          reltol=d["reltol"], abstol=d["abstol"], tick_attempts=args.tick_attempts,
-                                          hbm_budget_bytes=args.hbm_budget_gb * (1 << 30) // len(pools))
-    # schedule
-    t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 16))
-    t_arr = t_arr[t_arr < dur]
+                                          reltol=d["reltol"], abstol=d["abstol"], ns=ns,
+                                          tick_attempts=args.tick_attempts,
+                                          hbm_budget_bytes=budget)
+    # schedule: two Poisson streams merged by arrival time
+    t_a = np.cumsum(rng.exponential(1.0 / rate, int(rate * dur * 1.5) + 16))
+    t_a = t_a[t_a < dur]
+    kind_a = rng.integers(0, 3, len(t_a))
+    size_a = rng.integers(1, args.max_request + 1, len(t_a))
+    if args.chain_rate > 0:
+        t_c = np.cumsum(rng.exponential(1.0 / args.chain_rate, int(args.chain_rate * dur * 1.5) + 16))
+        t_c = t_c[t_c < dur]
+    else:
+        t_c = np.zeros(0)
+    kind_c = np.full(len(t_c), 3)
+    size_c = rng.integers(1, args.chain_max + 1, len(t_c))
+    t_arr = np.concatenate([t_a, t_c])
+    order = np.argsort(t_arr, kind="stable")
+    t_arr, kind, size = t_arr[order], np.concatenate([kind_a, kind_c])[order], np.concatenate([size_a, size_c])[order]
     nreq = len(t_arr)
-    kind = rng.integers(0, len(kinds), nreq)
-    size = rng.integers(1, args.max_request + 1, nreq)
     off = [int(rng.integers(0, pools[kinds[k]][0]["u0"].shape[1] - s + 1)) for k, s in zip(kind, size)]
-    # warm the kernels (hipRTC compile happened at server creation)
-    for k in kinds:
+
+    def submit(k, a, b):
         d = pools[k][0]
-        servers[k].submit(d["u0"][:, :8], d["p"][:, :8], d["t0"][:8], d["tf"][:8]).wait(timeout=120)
-    print(f"[bench] serving {nreq} requests over {dur:.0f} s at {rate:.0f} req/s", file=sys.stderr, flush=True)
+        nv = d["nvec"][a:b] if "nvec" in d else None
+        return servers[k].submit(d["u0"][:, a:b], d["p"][:, a:b], d["t0"][a:b], d["tf"][a:b], stats=False, nvec=nv)
+
+    for k in kinds:   # warm the kernels (hipRTC compile happened at server creation)
+        submit(k, 0, 2).wait(timeout=300)
+    print(f"[bench] serving {nreq} requests over {dur:.0f} s: {rate:.0f} req/s lane types + "
+          f"{args.chain_rate:g} req/s chains", file=sys.stderr, flush=True)
     reqs = [None] * nreq
     lag = np.zeros(nreq)
     t0 = time.monotonic() + 0.05
@@ -195,38 +225,45 @@ def bench_serving(args):
                 break
             if ts - now > 0.002:
                 time.sleep(ts - now - 0.001)
-        k = kinds[kind[q]]
-        d = pools[k][0]
-        a, b = off[q], off[q] + int(size[q])
-        reqs[q] = servers[k].submit(d["u0"][:, a:b], d["p"][:, a:b], d["t0"][a:b], d["tf"][a:b], stats=False)
+        reqs[q] = submit(kinds[kind[q]], off[q], off[q] + int(size[q]))
         lag[q] = time.monotonic() - ts
         if q % 2000 == 0:
             print(f"[bench] submitted {q}/{nreq}", file=sys.stderr, flush=True)
-    for r in reqs:
-        r.wait(timeout=600)
-    lat = np.array([r.times()[2] for r in reqs]) - (t0 + t_arr)
-    t_end = max(r.times()[2] for r in reqs)
-    ok = np.mean([float((r.retcode == 1).mean()) for r in reqs])
-    info = {k: s.info() for k, s in servers.items()}
+    for q, r in enumerate(reqs):
+        if not r.wait(timeout=900):
+            raise RuntimeError(f"request {q} ({kinds[kind[q]]}) did not finish")
+    done = np.array([r.times()[2] for r in reqs])
+    lat = done - (t0 + t_arr)
+    t_end = float(done.max())
+    ok = float(np.mean([float((r.retcode == 1).mean()) for r in reqs]))
+    info = {k: s_.info() for k, s_ in servers.items()}
     for s_ in servers.values():
         s_.close()
     keep = t_arr >= 0.1 * dur                   # drop the first 10% of the window (warm-up)
+
+    def q(x):
+        return {"p50": float(np.quantile(x, 0.5) * 1e3), "p99": float(np.quantile(x, 0.99) * 1e3),
+                "max": float(x.max() * 1e3), "n": int(len(x))} if len(x) else None
+
+    lane = keep & (kind < 3)
     traj = int(size.sum())
     res = {
         "metric": "serving: trajectories/s and p50/p99 request latency, open-loop Poisson arrivals",
         "value": traj / (t_end - t0), "unit": "trajectories/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": (t_end - t0) * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (Lorenz reltol 1e-8), f32 stages/LU + f64 state (Lorenz 1e-6, Robertson)",
+        "dtype": "f64 (Lorenz reltol 1e-8), f32 stages/LU/Gauss-Jordan + f64 state (Lorenz 1e-6, Robertson, chains)",
         "data": "synthetic (seeded PCG64)",
-        "config": {"workload": f"open-loop Poisson arrivals at {rate:.0f} req/s for {dur:.0f} s; each request "
-                               f"1..{args.max_request} trajectories (uniform) of one of: Lorenz (0,1) reltol 1e-8 "
-                               "Tsit5; Lorenz t_end~U[1,10] reltol 1e-6 Tsit5; Robertson (0,1e5) Rodas5P",
+        "config": {"workload": f"open-loop Poisson arrivals for {dur:.0f} s: {rate:.0f} req/s of 1..{args.max_request} "
+                               "trajectories (uniform) of one of Lorenz (0,1) reltol 1e-8 Tsit5 / Lorenz "
+                               "t_end~U[1,10] reltol 1e-6 Tsit5 / Robertson (0,1e5) Rodas5P, plus "
+                               f"{args.chain_rate:g} req/s of 1..{args.chain_max} oscillator chains n=8..128 "
+                               "tspan (0,10) Rodas5P",
                    "requests": nreq, "trajectories": traj,
                    "parallelism": "one MI355X, one server (stream + worker thread) per instance type"},
-        "latency_ms": {"p50": float(np.quantile(lat[keep], 0.5) * 1e3), "p99": float(np.quantile(lat[keep], 0.99) * 1e3),
-                       "max": float(lat[keep].max() * 1e3), "mean": float(lat[keep].mean() * 1e3)},
+        "latency_ms": q(lat[lane]),
+        "latency_ms_by_type": {k: q(lat[keep & (kind == i)]) for i, k in enumerate(kinds)},
         "submit_lag_ms": {"p99": float(np.quantile(lag, 0.99) * 1e3), "max": float(lag.max() * 1e3)},
-        "success_fraction": float(ok),
+        "success_fraction": ok,
         "servers": info,
         "roofline": None,
     }
@@ -244,6 +281,8 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0, help="serving: arrival window, s")
     ap.add_argument("--max-request", type=int, default=4096, help="serving: trajectories per request, max")
     ap.add_argument("--tick-attempts", type=int, default=32, help="serving: step attempts per tick")
+    ap.add_argument("--chain-rate", type=float, default=20.0, help="serving: chain requests/s (0: none)")
+    ap.add_argument("--chain-max", type=int, default=16, help="serving: chains per chain request, max")
     ap.add_argument("--hbm-budget-gb", type=int, default=8, help="serving: slot-pool budget, GiB")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

@@ -151,7 +151,8 @@ void odemi_default_args(odemi_solve_args* args, int alg);
  * ------------------------------------------------------------------------- */
 typedef struct odemi_server odemi_server;
 typedef struct {
-    int alg;                  /* ODEMI_TSIT5 / ODEMI_RODAS5P (lane kernels) */
+    int alg;                  /* ODEMI_TSIT5 / ODEMI_RODAS5P (lane kernels, or the workgroup
+                                 kernels when the problem was compiled with ODEMI_COMPILE_WG) */
     int precision;            /* ODEMI_FP64 / ODEMI_MIXED */
     double reltol, abstol, dtmax, dtmin;
     int64_t maxiters;
@@ -164,11 +165,13 @@ typedef struct {
 void odemi_server_default_opts(odemi_server_opts* opts, int alg);
 odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, const odemi_server_opts* opts);
 void odemi_server_destroy(odemi_server* server);
-/* Submit N trajectories (host SoA inputs, copied before return).  Outputs are
- * written at completion: u_final [n][N], retcode [N], stats [6][N] (stats may be
- * NULL).  Returns a request id (> 0) or a negative error. */
-int64_t odemi_submit(odemi_server* server, int64_t N, const double* u0, const double* p, const double* t0,
-                     const double* tf, double* u_final, int32_t* retcode, int32_t* stats);
+/* Submit N trajectories (host SoA inputs, copied before return; u0 is [n][N]
+ * with n the compiled dimension).  nvec [N] gives each trajectory's own
+ * dimension for a workgroup (uneven-n Rodas5P) server, or NULL for n.  Outputs
+ * are written at completion: u_final [n][N], retcode [N], stats [6][N] (stats
+ * may be NULL).  Returns a request id (> 0) or a negative error. */
+int64_t odemi_submit(odemi_server* server, int64_t N, const int32_t* nvec, const double* u0, const double* p,
+                     const double* t0, const double* tf, double* u_final, int32_t* retcode, int32_t* stats);
 /* 1 done, 0 pending or running, < 0 error */
 int odemi_poll(odemi_server* server, int64_t request);
 /* Block until done (timeout_s <= 0: no timeout).  1 done, 0 timed out, < 0 error. */

@@ -81,6 +81,9 @@ struct Params {
     const int* slot_req;      // [N] request table index of each slot
     u64* req_done;            // [R] finished trajectories per request
     float* wg_jbuf;           // workgroup Gauss-Jordan kernel: [grid][nmax][nmax] FP32 Jacobian scratch
+    // start-up values computed ahead of the persistent lane kernels (or null):
+    double* pre_dt;           // [N] initial dt (initdt or the user's dt)
+    double* pre_k1;           // [nmax][N] Tsit5 FSAL stage f(u0), exact in FP64
 };
 
 // ---------------------------------------------------------------------------
@@ -490,16 +493,20 @@ struct Tsit5Lane {
     K p[ODEMI_NP];
 };
 
+// Loads u0 and p of trajectory j into the lane (FP64 copies in u0d / pd).
 template <typename S, typename K, int N>
-__device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+__device__ __forceinline__ void tsit5_load(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, double (&u0d)[N],
+                                           double (&pd)[ODEMI_NP]) {
     ctl_init(L.c, P, j);
-    double u0d[N], pd[ODEMI_NP];
 #pragma unroll
     for (int i = 0; i < N; i++) { u0d[i] = P.u0[(i64)i * P.N + j]; L.u[i] = (S)u0d[i]; L.uprev[i] = L.u[i]; }
 #pragma unroll
     for (int k = 0; k < ODEMI_NP; k++) { pd[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; L.p[k] = (K)pd[k]; }
-    if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
-    // initialize! : FSAL pre-start
+}
+// initialize!: FSAL pre-start k1 = f(u0), then the initial dt.
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_first_stage_dt(Tsit5Lane<S, K, N>& L, const Params& P, const double (&u0d)[N],
+                                                     const double (&pd)[ODEMI_NP]) {
     K uk[N];
 #pragma unroll
     for (int i = 0; i < N; i++) uk[i] = (K)L.uprev[i];
@@ -511,7 +518,38 @@ __device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& 
         L.c.dt = initdt_lane<N>(L.c, u0d, pd, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
         L.c.nf += 2;
     }
+}
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+    double u0d[N], pd[ODEMI_NP];
+    tsit5_load(L, P, j, u0d, pd);
+    if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
+    if (P.pre_dt) {
+        // computed by tsit5_prep with the same arithmetic
+#pragma unroll
+        for (int i = 0; i < N; i++) L.k1[i] = (K)P.pre_k1[(i64)i * P.N + j];
+        L.c.dt = P.pre_dt[j];
+        L.c.nf += (P.dt_user != 0.0) ? 1 : 3;
+    } else {
+        tsit5_first_stage_dt(L, P, u0d, pd);
+    }
     L.c.dtpropose = L.c.dt;
+}
+// One thread per trajectory, ahead of the persistent kernel: the start-up work
+// (one RHS in the stage precision, initdt's two FP64 RHS and norms) runs fully
+// parallel instead of stalling the other lanes of a wave whenever one lane
+// picks up a new trajectory.
+template <typename S, typename K, int N>
+__device__ void tsit5_prep(const Params& P) {
+    const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    Tsit5Lane<S, K, N> L;
+    double u0d[N], pd[ODEMI_NP];
+    tsit5_load(L, P, j, u0d, pd);
+    tsit5_first_stage_dt(L, P, u0d, pd);
+#pragma unroll
+    for (int i = 0; i < N; i++) P.pre_k1[(i64)i * P.N + j] = (double)L.k1[i];
+    P.pre_dt[j] = L.c.dt;
 }
 
 template <typename S, typename K, int N>
@@ -900,13 +938,31 @@ __device__ __forceinline__ void rodas_init(RodasLane<F, N>& L, const Params& P, 
 #pragma unroll
         for (int b = 0; b < N; b++) L.J[a][b] = 0.0;
     if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
-    if (P.dt_user != 0.0) {  // user dt (a NaN dt is reported as DtNaN)
+    if (P.pre_dt) {  // computed by rodas_prep with the same arithmetic
+        L.c.dt = P.pre_dt[j];
+        if (P.dt_user == 0.0) L.c.nf += 2;
+    } else if (P.dt_user != 0.0) {  // user dt (a NaN dt is reported as DtNaN)
         L.c.dt = P.dt_user;
     } else {
         L.c.dt = initdt_lane<N>(L.c, L.u, L.p, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
         L.c.nf += 2;
     }
     L.c.dtpropose = L.c.dt;
+}
+
+// initdt for every trajectory ahead of the persistent kernel (see tsit5_prep).
+template <int N>
+__device__ void rodas_prep(const Params& P) {
+    const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    Ctl c;
+    ctl_init(c, P, j);
+    double u[N], p[ODEMI_NP];
+#pragma unroll
+    for (int i = 0; i < N; i++) u[i] = P.u0[(i64)i * P.N + j];
+#pragma unroll
+    for (int k = 0; k < ODEMI_NP; k++) p[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0;
+    P.pre_dt[j] = (P.dt_user != 0.0) ? P.dt_user : initdt_lane<N>(c, u, p, P.reltol, P.abstol, c.dtmax, P.dtmin);
 }
 
 // W x = b with W held in FP64; returns x in b.

@@ -375,9 +375,14 @@ __device__ __forceinline__ unsigned gj_key(float v, int i) {
 }
 __device__ __forceinline__ int gj_cidx(int i) { return (i & 15) * ODEMI_GJ_R + (i >> 4); }
 
-__device__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
+__device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
     constexpr int R = ODEMI_GJ_R, NP = ODEMI_GJ_NP, NQ = (NP + 63) / 64;
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15, lane = tid & 63;
+    // row slots this trajectory occupies (uniform): rows beyond n are identity
+    // padding that no step changes (their column entries are 0), so their
+    // slots are skipped; padding columns stay in the update (the pivot row is 0
+    // there) so every staged row slot is whole
+    const int rn = (n + 15) >> 4;
     float* cbuf = gjbuf;
     float* nbuf = gjbuf + 2 * NP;
     float* stg = gjbuf + 4 * NP;
@@ -409,7 +414,8 @@ __device__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
 #pragma unroll
         for (int a = 0; a < R; a++)
 #pragma unroll
-            for (int b = 0; b < R; b++) stg[(a * 16 + tj) * R + b] = G.T[a][b];
+            for (int b = 0; b < R; b++)
+                if (a < rn) stg[(a * 16 + tj) * R + b] = G.T[a][b];
     }
     __syncthreads();
     // ---- steps ----
@@ -457,11 +463,13 @@ __device__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
             // rank-1 update of every row but p_k; column k becomes -c d (1 on row p_k)
 #pragma unroll
             for (int a = 0; a < R; a++) {
-                const bool isp = own && (a == pa);
-                const float ct = isp ? 0.0f : c[a];
+                if (a < rn) {   // uniform
+                    const bool isp = own && (a == pa);
+                    const float ct = isp ? 0.0f : c[a];
 #pragma unroll
-                for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-ct, r[b] * d, G.T[a][b]);
-                G.T[a][kb] = colk ? (isp ? 1.0f : -c[a] * d) : G.T[a][kb];
+                    for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-ct, r[b] * d, G.T[a][b]);
+                    G.T[a][kb] = colk ? (isp ? 1.0f : -c[a] * d) : G.T[a][kb];
+                }
             }
             if (tid == 0) scl[p] = d;
             key = dpp_wave_max_u32(key);
@@ -494,7 +502,8 @@ __device__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
 #pragma unroll
                 for (int a = 0; a < R; a++)
 #pragma unroll
-                    for (int b = 0; b < R; b++) stn[(a * 16 + tj) * R + b] = G.T[a][b];
+                    for (int b = 0; b < R; b++)
+                        if (a < rn) stn[(a * 16 + tj) * R + b] = G.T[a][b];
             }
             __syncthreads();
         }
@@ -513,6 +522,7 @@ __device__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
 __device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, double* x, float* zbuf) {
     constexpr int R = ODEMI_GJ_R;
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+    const int rn = (n + 15) >> 4;   // tile slots this trajectory occupies (uniform)
     float bp[R];
 #pragma unroll
     for (int b = 0; b < R; b++) {
@@ -521,11 +531,13 @@ __device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, 
     }
 #pragma unroll
     for (int a = 0; a < R; a++) {
-        float z = 0.0f;
+        if (a < rn) {   // uniform
+            float z = 0.0f;
 #pragma unroll
-        for (int b = 0; b < R; b++) z = fmaf(G.T[a][b], bp[b], z);
-        z = dpp_row_sum16(z);
-        if (tj == 15) zbuf[ti + 16 * a] = z;
+            for (int b = 0; b < R; b++) z = fmaf(G.T[a][b], bp[b], z);
+            z = dpp_row_sum16(z);
+            if (tj == 15) zbuf[ti + 16 * a] = z;
+        }
     }
     __syncthreads();
     for (int k = tid; k < n; k += blockDim.x) x[k] = (double)zbuf[pk[k]];
@@ -534,7 +546,7 @@ __device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, 
 
 // Stage solve W x = b for the Gauss-Jordan variant (b in x on entry), with the
 // same FP64 refinement sweeps as wg_wsolve.
-__device__ void wg_wsolve_gj(const GJTile& G, const WGPtrs& S, int n, double* x, double* rb, double* r, double* d,
+__device__ __forceinline__ void wg_wsolve_gj(const GJTile& G, const WGPtrs& S, int n, double* x, double* rb, double* r, double* d,
                              const double* uprev, const double* p, double t, double sigma, int sweeps) {
     const int tid = threadIdx.x, NT = blockDim.x;
     for (int i = tid; i < n; i += NT) rb[i] = x[i];
@@ -643,12 +655,50 @@ __device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, con
     return res;
 }
 
-template <typename F, bool SAVES, bool GJ>
-__device__ void rodas_wg_impl(const Params& P) {
-    static_assert(!GJ || (sizeof(F) == 4 && ODEMI_GJ), "Gauss-Jordan variant: FP32, n <= 128");
-    extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
+// ---------------------------------------------------------------------------
+// Workgroup Rodas5P, in three phases so the batch kernel and the serving
+// kernels share every line of the step: wg_begin (u0, parameters, controller,
+// initial dt), wg_run (step attempts), wg_finish (postamble and write-out).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wg_load_params(const WGPtrs& S, const Params& P, i64 j) {
+    for (int k = threadIdx.x; k < ODEMI_NP; k += blockDim.x) {
+        const double v = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0;
+        S.pd[k] = v;
+        S.pf[k] = (float)v;
+    }
+}
+
+template <bool GJ>
+__device__ __forceinline__ void wg_begin(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double& tlast,
+                                         float* Jg) {
     const int tid = threadIdx.x, NT = blockDim.x;
-    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES, !GJ);
+    wg_load_params(S, P, j);
+    for (int i = tid; i < n; i += NT) { double v = P.u0[(i64)i * P.N + j]; S.u[i] = v; S.uprev[i] = v; S.dT[i] = 0.0; }
+    ctl_init(c, P, j);
+    tlast = 0.0;
+#if ODEMI_GJ
+    // the traced Jacobian writes the same entries on every call for a given n,
+    // so the rest of the block is cleared once per trajectory
+    if constexpr (GJ)
+        for (int e = tid; e < n * P.nmax; e += NT) Jg[e] = 0.0f;
+#endif
+    __syncthreads();
+    if (P.save_start) wg_save_point(c, tlast, P, j, c.t, S.u, n);
+    if (P.dt_user != 0.0) {
+        c.dt = P.dt_user;
+    } else {
+        c.dt = wg_initdt(S, c, S.u, S.pd, n, P.reltol, P.abstol, P.dtmin);
+        c.nf += 2;
+    }
+    c.dtpropose = c.dt;
+}
+
+// Step attempts until the trajectory ends (returns its ReturnCode) or `limit`
+// attempts have been made (returns 0; the state is in c and S.u/uprev/dT).
+template <typename F, bool SAVES, bool GJ>
+__device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double& tlast,
+                                      float* Jg, int limit) {
+    const int tid = threadIdx.x, NT = blockDim.x;
     const CtlConst kc = ctl_const(true);
     double* u = S.u;
     double* uprev = S.uprev;
@@ -658,241 +708,340 @@ __device__ void rodas_wg_impl(const Params& P) {
     double* ks = S.ks;
     double* dense = S.dense;
     double* red = S.red;
+    const double* pp = S.pd;
+    const float* pf = S.pf;
     const int nmax = P.nmax, ld = S.ld;
     F* W = (F*)S.W;
+    (void)Jg; (void)W; (void)ld; (void)pf; (void)dense;
 #if ODEMI_GJ
     GJTile G;
-    float* Jg = GJ ? P.wg_jbuf + (i64)blockIdx.x * nmax * nmax : nullptr;
+#endif
+    int rc = RC_SUCCESS;
+    for (int att = 0; c.t < c.tf; att++) {
+        if (att == limit) return 0;   // uniform: every thread holds the same controller state
+        // ---- loopheader! ----
+        if (c.iter > 0) {
+            if (c.accept_step) {
+                c.success_iter++;
+                for (int i = tid; i < n; i += NT) uprev[i] = u[i];
+                c.dt = c.dtpropose;
+                modify_dt_for_tstops(c);
+            } else {
+                reject_controller(c, kc);
+            }
+        }
+        c.iter++;
+        fix_dt_at_bounds(c, P.dtmin);
+        modify_dt_for_tstops(c);
+        int bad = 0;
+        for (int i = tid; i < n; i += NT) bad |= !isfinite(u[i]);
+        const bool finite = !__syncthreads_or(bad);
+        rc = check_error(c, P, finite);
+        if (rc != RC_SUCCESS) break;
+        if (c.next_step_tstop && fabs(c.dt) < jl_eps(fabs(c.t))) {
+            c.accept_step = 1;
+        } else {
+            // ---- perform_step! (rosenbrock_perform_step.jl:729-847) ----
+            const double t = c.t, dt = c.dt;
+            const double idt = 1.0 / dt;
+            const double dtgamma = dt * RodasTab::gamma;
+            const double sigma = 1.0 / dtgamma;
+            odemi_f_wg<double>(fsal, uprev, pp, t, n, tid, NT);
+            c.nf += 1;
+            const bool new_jac = (c.iter <= 1) ? true : !(c.iter > 1 && c.EEst > 1.0);
+            if constexpr (!GJ)
+                for (int e = tid; e < n * ld; e += NT) W[e] = (F)0;
+            if (sizeof(F) != sizeof(double))
+                for (int i = tid; i < n; i += NT) S.fsc[i] = (float)uprev[i];
+            __syncthreads();
+            if constexpr (GJ) {
+#if ODEMI_GJ
+                odemi_jac_wg<float>(Jg, nmax, S.fsc, pf, (float)t, n, tid, NT);
+#endif
+            } else if constexpr (sizeof(F) == sizeof(double)) {
+                odemi_jac_wg<double>((double*)W, ld, uprev, pp, t, n, tid, NT);
+            } else {
+                odemi_jac_wg<float>((float*)W, ld, S.fsc, pf, (float)t, n, tid, NT);
+            }
+            if (new_jac) {
+                c.njacs += 1;
+                for (int i = tid; i < n; i += NT) dT[i] = 0.0;
+            }
+            __syncthreads();
+            if constexpr (!GJ)
+                for (int i = tid; i < n; i += NT)
+                    W[i * ld + i] = (F)fma(-1.0, sigma, (double)W[i * ld + i]);
+            if (new_jac) {
+                odemi_tgrad_wg<double>(dT, uprev, pp, t, n, tid, NT);
+                if (!ODEMI_HAS_TGRAD) c.nf += 1;
+            }
+            c.nw += 1;
+            __syncthreads();
+            const double dtd0 = dt * kRd[0];
+            for (int i = tid; i < n; i += NT) ks[i] = fsal[i] + dtd0 * dT[i];
+            // rb = fsal, r = du, d = u are free during the stage-1 solve
+            if constexpr (GJ) {
+#if ODEMI_GJ
+                gj_load(G, Jg, nmax, n, sigma);
+                gj_invert(G, n, S.perm, S.gjbuf);
+                wg_wsolve_gj(G, S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+#endif
+            } else {
+                wg_lu<F>(W, ld, n, S.piv, S.perm, red, S.ired);
+                wg_wsolve<F>(S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+            }
+            for (int i = tid; i < n; i += NT) ks[i] = -ks[i];
+            c.nsolve += 1;
+            __syncthreads();
+            for (int s = 1; s < 8; s++) {
+                for (int i = tid; i < n; i += NT) {
+                    double acc = uprev[i];
+                    for (int q = 0; q < s; q++) acc = fma(kRA[s][q], ks[q * nmax + i], acc);
+                    u[i] = acc;
+                }
+                __syncthreads();
+                odemi_f_wg<double>(du, u, pp, fma(kRc[s], dt, t), n, tid, NT);
+                c.nf += 1;
+                __syncthreads();
+                const double dtds = dt * kRd[s];
+                double* x = ks + s * nmax;
+                for (int i = tid; i < n; i += NT) {
+                    double acc = fma(dtds, dT[i], du[i]);
+                    for (int q = 0; q < s; q++) acc = fma(kRC[s][q] * idt, ks[q * nmax + i], acc);
+                    x[i] = acc;
+                }
+                __syncthreads();
+                if constexpr (GJ) {
+#if ODEMI_GJ
+                    wg_wsolve_gj(G, S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+#endif
+                } else {
+                    wg_wsolve<F>(S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                }
+                for (int i = tid; i < n; i += NT) x[i] = -x[i];
+                c.nsolve += 1;
+                __syncthreads();
+            }
+            for (int i = tid; i < n; i += NT) {
+                double acc = uprev[i];
+                for (int q = 0; q < 7; q++) acc = fma(kRA[7][q], ks[q * nmax + i], acc);
+                acc = fma(1.0, ks[7 * nmax + i], acc);
+                u[i] = acc;
+                double e = 0.0;
+                for (int q = 0; q < 7; q++) e = fma(0.0, ks[q * nmax + i], e);
+                e = fma(1.0, ks[7 * nmax + i], e);
+                du[i] = e;
+            }
+            if (SAVES && P.nsaveat > 0) {
+                for (int r = 0; r < 3; r++)
+                    for (int i = tid; i < n; i += NT) {
+                        double acc = 0.0;
+                        for (int q = 0; q < 8; q++) acc = fma(kRH[r][q], ks[q * nmax + i], acc);
+                        dense[r * nmax + i] = acc;
+                    }
+            }
+            __syncthreads();
+            // error norm: the scaled residuals in parallel, then one thread's
+            // index-order fma chain (the same value as the sequential norm)
+            for (int i = tid; i < n; i += NT)
+                fsal[i] = du[i] / fma(max_fast(fabs(uprev[i]), fabs(u[i])), P.reltol, P.abstol);
+            __syncthreads();
+            if (tid == 0) {
+                double sq = 0.0;
+                for (int i = 0; i < n; i++) sq = fma(fsal[i], fsal[i], sq);
+                red[24] = sqrt(sq / (double)(n > 1 ? n : 1));
+            }
+            __syncthreads();
+            c.EEst = red[24];
+        }
+        // ---- loopfooter! ----
+        const double ttmp = c.t + c.dt;
+        const double q = stepsize_controller(c, kc);
+        c.accept_step = (c.EEst <= 1.0);
+        if (c.accept_step) {
+            c.naccept++;
+            c.tprev = c.t;
+            if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
+            else c.t = ttmp;
+            accept_controller(c, kc, q, P.dtmin);
+            if (SAVES) {
+                while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
+                    const double curt = P.saveat[c.spos++];
+                    if (curt != c.t) {
+                        const double th = (curt - c.tprev) / c.dt, th1 = 1.0 - th;
+                        const int sidx = c.nsaved;
+                        if (sidx < P.max_saves) {
+                            if (tid == 0) P.t_out[(i64)sidx * P.N + j] = curt;
+                            for (int i = tid; i < n; i += NT) {
+                                double in1 = fma(th, dense[2 * nmax + i], dense[nmax + i]);
+                                double in2 = fma(th, in1, dense[i]);
+                                double in3 = fma(th1, in2, u[i]);
+                                P.u_out[((i64)sidx * P.nmax + i) * P.N + j] = fma(th, in3, th1 * uprev[i]);
+                            }
+                        }
+                        c.nsaved = sidx + 1;
+                        tlast = curt;
+                    } else {
+                        if (curt == c.tf && !P.save_end) continue;
+                        wg_save_point(c, tlast, P, j, c.t, u, n);
+                    }
+                }
+                if (P.save_everystep) {
+                    if (c.nsaved == 0 || ((c.t != tlast || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
+                        wg_save_point(c, tlast, P, j, c.t, u, n);
+                }
+            }
+        } else {
+            c.nreject++;
+        }
+    }
+    return rc;
+}
+
+template <bool SAVES>
+__device__ __forceinline__ void wg_finish(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double tlast,
+                                          int rc) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int nmax = P.nmax;
+    const double* u = S.u;
+    // ---- postamble! / result write-out ----
+    if (rc == RC_SUCCESS && P.save_end) {
+        bool doit = (c.nsaved == 0);
+        if (!doit)
+            doit = (tlast != c.t) && (P.save_end_user == 1 || c.tf_in_saveat || c.t == c.tf || P.nsaveat == 0);
+        if (doit) wg_save_point(c, tlast, P, j, c.t, u, n);
+    }
+    if (P.u_final)
+        for (int i = tid; i < nmax; i += NT) P.u_final[(i64)i * P.N + j] = (i < n) ? u[i] : 0.0;
+    if (tid == 0) {
+        if (P.t_final) P.t_final[j] = c.t;
+        if (P.nsaved) P.nsaved[j] = c.nsaved;
+        P.retcode[j] = rc;
+        if (P.stats) {
+            P.stats[ST_NF * P.N + j] = c.nf;
+            P.stats[ST_NACCEPT * P.N + j] = c.naccept;
+            P.stats[ST_NREJECT * P.N + j] = c.nreject;
+            P.stats[ST_NJACS * P.N + j] = c.njacs;
+            P.stats[ST_NW * P.N + j] = c.nw;
+            P.stats[ST_NSOLVE * P.N + j] = c.nsolve;
+        }
+    }
+}
+
+template <typename F, bool SAVES, bool GJ>
+__device__ __forceinline__ void rodas_wg_impl(const Params& P) {
+    static_assert(!GJ || (sizeof(F) == 4 && ODEMI_GJ), "Gauss-Jordan variant: FP32, n <= 128");
+    extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
+    const int tid = threadIdx.x;
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES, !GJ);
+    float* Jg = nullptr;
+#if ODEMI_GJ
+    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
 #endif
     for (;;) {
-        if (tid == 0) ((i64*)red)[30] = (i64)atomicAdd(&P.counters[0], 1ULL);
+        if (tid == 0) ((i64*)S.red)[30] = (i64)atomicAdd(&P.counters[0], 1ULL);
         __syncthreads();
-        const i64 slot = ((i64*)red)[30];
+        const i64 slot = ((i64*)S.red)[30];
         __syncthreads();
         if (slot >= P.N) break;
         const i64 j = P.order ? (i64)P.order[slot] : slot;
-        const int n = P.nvec ? P.nvec[j] : nmax;
-        double* pp = S.pd;
-        float* pf = S.pf;
-        for (int k = tid; k < ODEMI_NP; k += NT) { pp[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0; pf[k] = (float)pp[k]; }
-        for (int i = tid; i < n; i += NT) { double v = P.u0[(i64)i * P.N + j]; u[i] = v; uprev[i] = v; dT[i] = 0.0; }
+        const int n = P.nvec ? P.nvec[j] : P.nmax;
         Ctl c;
-        ctl_init(c, P, j);
-        double tlast = 0.0;
+        double tlast;
+        wg_begin<GJ>(S, P, j, n, c, tlast, Jg);
+        const int rc = wg_run<F, SAVES, GJ>(S, P, j, n, c, tlast, Jg, 0x7fffffff);
+        wg_finish<SAVES>(S, P, j, n, c, tlast, rc);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Serving (continuous batching) with the workgroup kernel: the slot pool keeps
+// the controller in the plain-engine layout and the rows u, uprev, dT in
+// st_vec; W, the stages and the Jacobian are rebuilt by every attempt, so
+// that is the whole state.  Slot state st_int row 7: 0 free, 1 active.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wg_store_state(const WGPtrs& S, const Params& P, i64 j, int n, const Ctl& c,
+                                               int active) {
+    const i64 M = P.N;
+    if (threadIdx.x == 0) {
+        plain_store_ctl<1>(c, P, j, active);
+        P.st_int[9 * M + j] = c.njacs; P.st_int[10 * M + j] = c.nw; P.st_int[11 * M + j] = c.nsolve;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        P.st_vec[((i64)0 * P.nmax + i) * M + j] = S.u[i];
+        P.st_vec[((i64)1 * P.nmax + i) * M + j] = S.uprev[i];
+        P.st_vec[((i64)2 * P.nmax + i) * M + j] = S.dT[i];
+    }
+}
+__device__ __forceinline__ void wg_load_state(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c) {
+    const i64 M = P.N;
+    plain_load_ctl<1>(c, P, j);
+    c.njacs = P.st_int[9 * M + j]; c.nw = P.st_int[10 * M + j]; c.nsolve = P.st_int[11 * M + j];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        S.u[i] = P.st_vec[((i64)0 * P.nmax + i) * M + j];
+        S.uprev[i] = P.st_vec[((i64)1 * P.nmax + i) * M + j];
+        S.dT[i] = P.st_vec[((i64)2 * P.nmax + i) * M + j];
+    }
+}
+
+// Admit slots [slot0, slot0 + nslots): one workgroup per slot.
+template <typename F, bool GJ>
+__device__ __forceinline__ void rodas_wg_serve_init(const Params& P) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, false, !GJ);
+    float* Jg = nullptr;
 #if ODEMI_GJ
-        // the traced Jacobian writes the same entries on every call for a given n,
-        // so the rest of the block is cleared once per trajectory
+    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
+#endif
+    for (i64 q = blockIdx.x; q < P.nslots; q += gridDim.x) {
+        const i64 j = P.slot0 + q;
+        const int n = P.nvec[j];
+        Ctl c;
+        double tlast;
+        wg_begin<GJ>(S, P, j, n, c, tlast, Jg);
+        wg_store_state(S, P, j, n, c, 1);
+        __syncthreads();
+    }
+}
+
+// Advance every active slot in [0, nslots) by up to tick_attempts attempts;
+// slots are handed out through counters[0] (cleared by the host per tick).
+template <typename F, bool GJ>
+__device__ __forceinline__ void rodas_wg_serve_tick(const Params& P) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
+    const int tid = threadIdx.x;
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, false, !GJ);
+    float* Jg = nullptr;
+#if ODEMI_GJ
+    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
+#endif
+    for (;;) {
+        if (tid == 0) ((i64*)S.red)[30] = (i64)atomicAdd(&P.counters[0], 1ULL);
+        __syncthreads();
+        const i64 j = ((i64*)S.red)[30];
+        __syncthreads();
+        if (j >= P.nslots) break;
+        if (P.st_int[7 * P.N + j] != 1) continue;
+        const int n = P.nvec[j];
+        wg_load_params(S, P, j);
+        Ctl c;
+        wg_load_state(S, P, j, n, c);
+#if ODEMI_GJ
         if constexpr (GJ)
-            for (int e = tid; e < n * nmax; e += NT) Jg[e] = 0.0f;
+            for (int e = tid; e < n * P.nmax; e += blockDim.x) Jg[e] = 0.0f;
 #endif
         __syncthreads();
-        if (P.save_start) wg_save_point(c, tlast, P, j, c.t, u, n);
-        if (P.dt_user != 0.0) {
-            c.dt = P.dt_user;
+        double tlast = c.t;
+        const int rc = wg_run<F, false, GJ>(S, P, j, n, c, tlast, Jg, P.tick_attempts);
+        if (rc) {
+            wg_finish<false>(S, P, j, n, c, tlast, rc);
+            if (tid == 0) {
+                P.st_int[7 * P.N + j] = 0;
+                atomicAdd(&P.req_done[P.slot_req[j]], 1ULL);
+            }
         } else {
-            c.dt = wg_initdt(S, c, u, pp, n, P.reltol, P.abstol, P.dtmin);
-            c.nf += 2;
-        }
-        c.dtpropose = c.dt;
-        int rc = RC_SUCCESS;
-        while (c.t < c.tf) {
-            // ---- loopheader! ----
-            if (c.iter > 0) {
-                if (c.accept_step) {
-                    c.success_iter++;
-                    for (int i = tid; i < n; i += NT) uprev[i] = u[i];
-                    c.dt = c.dtpropose;
-                    modify_dt_for_tstops(c);
-                } else {
-                    reject_controller(c, kc);
-                }
-            }
-            c.iter++;
-            fix_dt_at_bounds(c, P.dtmin);
-            modify_dt_for_tstops(c);
-            int bad = 0;
-            for (int i = tid; i < n; i += NT) bad |= !isfinite(u[i]);
-            const bool finite = !__syncthreads_or(bad);
-            rc = check_error(c, P, finite);
-            if (rc != RC_SUCCESS) break;
-            if (c.next_step_tstop && fabs(c.dt) < jl_eps(fabs(c.t))) {
-                c.accept_step = 1;
-            } else {
-                // ---- perform_step! (rosenbrock_perform_step.jl:729-847) ----
-                const double t = c.t, dt = c.dt;
-                const double idt = 1.0 / dt;
-                const double dtgamma = dt * RodasTab::gamma;
-                const double sigma = 1.0 / dtgamma;
-                odemi_f_wg<double>(fsal, uprev, pp, t, n, tid, NT);
-                c.nf += 1;
-                const bool new_jac = (c.iter <= 1) ? true : !(c.iter > 1 && c.EEst > 1.0);
-                if constexpr (!GJ)
-                    for (int e = tid; e < n * ld; e += NT) W[e] = (F)0;
-                if (sizeof(F) != sizeof(double))
-                    for (int i = tid; i < n; i += NT) S.fsc[i] = (float)uprev[i];
-                __syncthreads();
-                if constexpr (GJ) {
-#if ODEMI_GJ
-                    odemi_jac_wg<float>(Jg, nmax, S.fsc, pf, (float)t, n, tid, NT);
-#endif
-                } else if constexpr (sizeof(F) == sizeof(double)) {
-                    odemi_jac_wg<double>((double*)W, ld, uprev, pp, t, n, tid, NT);
-                } else {
-                    odemi_jac_wg<float>((float*)W, ld, S.fsc, pf, (float)t, n, tid, NT);
-                }
-                if (new_jac) {
-                    c.njacs += 1;
-                    for (int i = tid; i < n; i += NT) dT[i] = 0.0;
-                }
-                __syncthreads();
-                if constexpr (!GJ)
-                    for (int i = tid; i < n; i += NT)
-                        W[i * ld + i] = (F)fma(-1.0, sigma, (double)W[i * ld + i]);
-                if (new_jac) {
-                    odemi_tgrad_wg<double>(dT, uprev, pp, t, n, tid, NT);
-                    if (!ODEMI_HAS_TGRAD) c.nf += 1;
-                }
-                c.nw += 1;
-                __syncthreads();
-                const double dtd0 = dt * kRd[0];
-                for (int i = tid; i < n; i += NT) ks[i] = fsal[i] + dtd0 * dT[i];
-                // rb = fsal, r = du, d = u are free during the stage-1 solve
-                if constexpr (GJ) {
-#if ODEMI_GJ
-                    gj_load(G, Jg, nmax, n, sigma);
-                    gj_invert(G, n, S.perm, S.gjbuf);
-                    wg_wsolve_gj(G, S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
-#endif
-                } else {
-                    wg_lu<F>(W, ld, n, S.piv, S.perm, red, S.ired);
-                    wg_wsolve<F>(S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
-                }
-                for (int i = tid; i < n; i += NT) ks[i] = -ks[i];
-                c.nsolve += 1;
-                __syncthreads();
-                for (int s = 1; s < 8; s++) {
-                    for (int i = tid; i < n; i += NT) {
-                        double acc = uprev[i];
-                        for (int q = 0; q < s; q++) acc = fma(kRA[s][q], ks[q * nmax + i], acc);
-                        u[i] = acc;
-                    }
-                    __syncthreads();
-                    odemi_f_wg<double>(du, u, pp, fma(kRc[s], dt, t), n, tid, NT);
-                    c.nf += 1;
-                    __syncthreads();
-                    const double dtds = dt * kRd[s];
-                    double* x = ks + s * nmax;
-                    for (int i = tid; i < n; i += NT) {
-                        double acc = fma(dtds, dT[i], du[i]);
-                        for (int q = 0; q < s; q++) acc = fma(kRC[s][q] * idt, ks[q * nmax + i], acc);
-                        x[i] = acc;
-                    }
-                    __syncthreads();
-                    if constexpr (GJ) {
-#if ODEMI_GJ
-                        wg_wsolve_gj(G, S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
-#endif
-                    } else {
-                        wg_wsolve<F>(S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
-                    }
-                    for (int i = tid; i < n; i += NT) x[i] = -x[i];
-                    c.nsolve += 1;
-                    __syncthreads();
-                }
-                for (int i = tid; i < n; i += NT) {
-                    double acc = uprev[i];
-                    for (int q = 0; q < 7; q++) acc = fma(kRA[7][q], ks[q * nmax + i], acc);
-                    acc = fma(1.0, ks[7 * nmax + i], acc);
-                    u[i] = acc;
-                    double e = 0.0;
-                    for (int q = 0; q < 7; q++) e = fma(0.0, ks[q * nmax + i], e);
-                    e = fma(1.0, ks[7 * nmax + i], e);
-                    du[i] = e;
-                }
-                if (SAVES && P.nsaveat > 0) {
-                    for (int r = 0; r < 3; r++)
-                        for (int i = tid; i < n; i += NT) {
-                            double acc = 0.0;
-                            for (int q = 0; q < 8; q++) acc = fma(kRH[r][q], ks[q * nmax + i], acc);
-                            dense[r * nmax + i] = acc;
-                        }
-                }
-                __syncthreads();
-                // error norm: the scaled residuals in parallel, then one thread's
-                // index-order fma chain (the same value as the sequential norm)
-                for (int i = tid; i < n; i += NT)
-                    fsal[i] = du[i] / fma(max_fast(fabs(uprev[i]), fabs(u[i])), P.reltol, P.abstol);
-                __syncthreads();
-                if (tid == 0) {
-                    double sq = 0.0;
-                    for (int i = 0; i < n; i++) sq = fma(fsal[i], fsal[i], sq);
-                    red[24] = sqrt(sq / (double)(n > 1 ? n : 1));
-                }
-                __syncthreads();
-                c.EEst = red[24];
-            }
-            // ---- loopfooter! ----
-            const double ttmp = c.t + c.dt;
-            const double q = stepsize_controller(c, kc);
-            c.accept_step = (c.EEst <= 1.0);
-            if (c.accept_step) {
-                c.naccept++;
-                c.tprev = c.t;
-                if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
-                else c.t = ttmp;
-                accept_controller(c, kc, q, P.dtmin);
-                if (SAVES) {
-                    while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
-                        const double curt = P.saveat[c.spos++];
-                        if (curt != c.t) {
-                            const double th = (curt - c.tprev) / c.dt, th1 = 1.0 - th;
-                            const int sidx = c.nsaved;
-                            if (sidx < P.max_saves) {
-                                if (tid == 0) P.t_out[(i64)sidx * P.N + j] = curt;
-                                for (int i = tid; i < n; i += NT) {
-                                    double in1 = fma(th, dense[2 * nmax + i], dense[nmax + i]);
-                                    double in2 = fma(th, in1, dense[i]);
-                                    double in3 = fma(th1, in2, u[i]);
-                                    P.u_out[((i64)sidx * P.nmax + i) * P.N + j] = fma(th, in3, th1 * uprev[i]);
-                                }
-                            }
-                            c.nsaved = sidx + 1;
-                            tlast = curt;
-                        } else {
-                            if (curt == c.tf && !P.save_end) continue;
-                            wg_save_point(c, tlast, P, j, c.t, u, n);
-                        }
-                    }
-                    if (P.save_everystep) {
-                        if (c.nsaved == 0 || ((c.t != tlast || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
-                            wg_save_point(c, tlast, P, j, c.t, u, n);
-                    }
-                }
-            } else {
-                c.nreject++;
-            }
-        }
-        // ---- postamble! / result write-out ----
-        if (rc == RC_SUCCESS && P.save_end) {
-            bool doit = (c.nsaved == 0);
-            if (!doit)
-                doit = (tlast != c.t) && (P.save_end_user == 1 || c.tf_in_saveat || c.t == c.tf || P.nsaveat == 0);
-            if (doit) wg_save_point(c, tlast, P, j, c.t, u, n);
-        }
-        if (P.u_final)
-            for (int i = tid; i < nmax; i += NT) P.u_final[(i64)i * P.N + j] = (i < n) ? u[i] : 0.0;
-        if (tid == 0) {
-            if (P.t_final) P.t_final[j] = c.t;
-            if (P.nsaved) P.nsaved[j] = c.nsaved;
-            P.retcode[j] = rc;
-            if (P.stats) {
-                P.stats[ST_NF * P.N + j] = c.nf;
-                P.stats[ST_NACCEPT * P.N + j] = c.naccept;
-                P.stats[ST_NREJECT * P.N + j] = c.nreject;
-                P.stats[ST_NJACS * P.N + j] = c.njacs;
-                P.stats[ST_NW * P.N + j] = c.nw;
-                P.stats[ST_NSOLVE * P.N + j] = c.nsolve;
-            }
+            wg_store_state(S, P, j, n, c, 1);
         }
         __syncthreads();
     }

@@ -66,6 +66,8 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     const int* slot_req;
     unsigned long long* req_done;
     float* wg_jbuf;
+    double* pre_dt;
+    double* pre_k1;
 };
 
 // Longest-predicted-cost-first work order: key_j = (tf_j - t0_j) * n_j^3 as FP32.
@@ -129,6 +131,8 @@ struct odemi_rhs {
     hipFunction_t rodas_wg_f64 = nullptr, rodas_wg_mixed = nullptr;
     hipFunction_t rodas_wg_f64_sv = nullptr, rodas_wg_mixed_sv = nullptr;
     hipFunction_t serve_init[2][2] = {}, serve_tick[2][2] = {};  // [alg][fp64, mixed]
+    hipFunction_t tsit5_prep[2] = {}, rodas_prep = nullptr;      // start-up kernels [fp64, mixed]
+    hipFunction_t wg_serve_init[2] = {}, wg_serve_tick[2] = {};  // workgroup serving [fp64, mixed]
 };
 
 #define HIPCHK(ctx, call)                                                                        \
@@ -216,6 +220,10 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
               << ", ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_init(odemi::Params P)"
              " { odemi::tsit5_plain_init<ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_prep_f64(odemi::Params P)"
+             " { odemi::tsit5_prep<double, double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_prep_mixed(odemi::Params P)"
+             " { odemi::tsit5_prep<double, float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_init_f64(odemi::Params P)"
              " { odemi::tsit5_serve_init<double, double, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_init_mixed(odemi::Params P)"
@@ -234,6 +242,8 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
              " { odemi::rodas_persistent<float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_plain_init(odemi::Params P)"
              " { odemi::rodas_plain_init<ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_prep(odemi::Params P)"
+             " { odemi::rodas_prep<ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_serve_init_f64(odemi::Params P)"
              " { odemi::rodas_serve_init<double, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_serve_init_mixed(odemi::Params P)"
@@ -260,6 +270,15 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
             if (wgw > 0 && k[1][0] == 'f') s << "__attribute__((amdgpu_waves_per_eu(" << wgw << "))) ";
             s << k[0] << "(odemi::Params P) { odemi::rodas_wg_impl<" << k[1] << ", " << k[2] << ", " << k[3]
               << ">(P); }\n";
+        }
+        const char* sv[4][4] = {{"odemi_rodas_wg_serve_init_f64", "rodas_wg_serve_init", "double", "false"},
+                                {"odemi_rodas_wg_serve_tick_f64", "rodas_wg_serve_tick", "double", "false"},
+                                {"odemi_rodas_wg_serve_init_mixed", "rodas_wg_serve_init", "float", "(ODEMI_GJ != 0)"},
+                                {"odemi_rodas_wg_serve_tick_mixed", "rodas_wg_serve_tick", "float", "(ODEMI_GJ != 0)"}};
+        for (auto& k : sv) {
+            s << "extern \"C\" __global__ void __launch_bounds__(ODEMI_WG_THREADS) ";
+            if (wgw > 0 && k[2][0] == 'f') s << "__attribute__((amdgpu_waves_per_eu(" << wgw << "))) ";
+            s << k[0] << "(odemi::Params P) { odemi::" << k[1] << "<" << k[2] << ", " << k[3] << ">(P); }\n";
         }
     }
     return s.str();
@@ -374,12 +393,15 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         get("odemi_tsit5_serve_tick_f64", &r->serve_tick[ODEMI_TSIT5][0]);
         get("odemi_tsit5_serve_tick_mixed", &r->serve_tick[ODEMI_TSIT5][1]);
         get("odemi_tsit5_plain_init", &r->tsit5_plain_init);
+        get("odemi_tsit5_prep_f64", &r->tsit5_prep[0]);
+        get("odemi_tsit5_prep_mixed", &r->tsit5_prep[1]);
         get("odemi_tsit5_plain_step", &r->tsit5_plain_step);
     }
     if (flags & ODEMI_COMPILE_RODAS5P) {
         get("odemi_rodas_persist_f64", &r->rodas_persist_f64);
         get("odemi_rodas_persist_mixed", &r->rodas_persist_mixed);
         get("odemi_rodas_plain_init", &r->rodas_plain_init);
+        get("odemi_rodas_prep", &r->rodas_prep);
         get("odemi_rodas_plain_step", &r->rodas_plain_step);
         get("odemi_rodas_serve_init_f64", &r->serve_init[ODEMI_RODAS5P][0]);
         get("odemi_rodas_serve_init_mixed", &r->serve_init[ODEMI_RODAS5P][1]);
@@ -391,6 +413,10 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         get("odemi_rodas_wg_mixed", &r->rodas_wg_mixed);
         get("odemi_rodas_wg_f64_sv", &r->rodas_wg_f64_sv);
         get("odemi_rodas_wg_mixed_sv", &r->rodas_wg_mixed_sv);
+        get("odemi_rodas_wg_serve_init_f64", &r->wg_serve_init[0]);
+        get("odemi_rodas_wg_serve_init_mixed", &r->wg_serve_init[1]);
+        get("odemi_rodas_wg_serve_tick_f64", &r->wg_serve_tick[0]);
+        get("odemi_rodas_wg_serve_tick_mixed", &r->wg_serve_tick[1]);
     }
     return r;
 }
@@ -427,6 +453,20 @@ int ensure_ws(odemi_ctx* c, size_t bytes) {
     return 0;
 }
 
+// Dynamic LDS of the workgroup kernels; must match odemi::wg_layout in kernels/odemi_wg.h.
+size_t wg_shmem_bytes(int nmax, bool mixed, bool saves, bool gj) {
+    const size_t ld = (size_t)nmax + 1, fb = mixed ? 4 : 8;
+    const size_t np16 = 16 * (((size_t)nmax + 15) / 16), r16 = np16 / 16;
+    size_t shmem = gj ? 0 : (((size_t)nmax * ld * fb + 15) & ~(size_t)15);
+    shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
+    shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 8;          // fsc + pf, piv + perm
+    shmem = (shmem + 15) & ~(size_t)15;
+    if (gj) shmem += (4 * np16 + 2 * r16 * np16 + 2 * np16) * 4;   // Gauss-Jordan buffers + zbuf
+    shmem = (shmem + 15) & ~(size_t)15;
+    shmem += 32 * 8 + 32 * 4;
+    return shmem;
+}
+
 int launch(odemi_ctx* c, hipFunction_t fn, int grid, int block, Params& P, hipStream_t s, size_t shmem = 0) {
     void* args[] = {&P};
     HIPCHK(c, hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)shmem, s, args, nullptr));
@@ -457,18 +497,20 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     const bool plain = (a->engine == ODEMI_ENGINE_PLAIN);
     const bool mixed = (a->precision == ODEMI_MIXED);
     const bool use_wg = !plain && rodas && (rhs->flags & ODEMI_COMPILE_WG);
-    hipFunction_t fn_main = nullptr, fn_init = nullptr;
+    hipFunction_t fn_main = nullptr, fn_init = nullptr, fn_prep = nullptr;
     if (!rodas) {
         if (plain) { fn_init = rhs->tsit5_plain_init; fn_main = rhs->tsit5_plain_step; }
         else if (a->nsaveat > 0 || a->save_everystep)
             fn_main = mixed ? rhs->tsit5_persist_mixed_sv : rhs->tsit5_persist_f64_sv;
         else fn_main = mixed ? rhs->tsit5_persist_mixed : rhs->tsit5_persist_f64;
+        if (!plain) fn_prep = rhs->tsit5_prep[mixed ? 1 : 0];
     } else if (use_wg) {
         if (a->nsaveat > 0 || a->save_everystep) fn_main = mixed ? rhs->rodas_wg_mixed_sv : rhs->rodas_wg_f64_sv;
         else fn_main = mixed ? rhs->rodas_wg_mixed : rhs->rodas_wg_f64;
     } else {
         if (plain) { fn_init = rhs->rodas_plain_init; fn_main = rhs->rodas_plain_step; }
         else fn_main = mixed ? rhs->rodas_persist_mixed : rhs->rodas_persist_f64;
+        if (!plain) fn_prep = rhs->rodas_prep;
     }
     if (plain && mixed) { ctx->err = "the plain engine is FP64 only"; return -3; }
     if (!fn_main || (plain && !fn_init)) { ctx->err = "requested kernel was not compiled for this problem"; return -3; }
@@ -487,18 +529,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     int64_t grid = 0;
     if (!plain) {
         if (use_wg) {
-            // must match odemi::wg_layout in kernels/odemi_wg.h
-            const size_t ld = (size_t)nmax + 1, fb = mixed ? 4 : 8;
-            const size_t np16 = 16 * (((size_t)nmax + 15) / 16);
-            const bool saves = a->nsaveat > 0 || a->save_everystep;
-            shmem = use_gj ? 0 : (((size_t)nmax * ld * fb + 15) & ~(size_t)15);
-            shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
-            shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 8;  // fsc + pf, piv + perm
-            shmem = (shmem + 15) & ~(size_t)15;
-            const size_t r16 = np16 / 16;
-            if (use_gj) shmem += (4 * np16 + 2 * r16 * np16 + 2 * np16) * 4;  // Gauss-Jordan buffers + zbuf
-            shmem = (shmem + 15) & ~(size_t)15;
-            shmem += 32 * 8 + 32 * 4;
+            shmem = wg_shmem_bytes(nmax, mixed, a->nsaveat > 0 || a->save_everystep, use_gj);
             if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
         }
         int nb = 0;
@@ -521,6 +552,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     }
     if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(12 * N_ * 4); }
     if (use_gj) add((size_t)grid * nmax * nmax * 4);
+    if (fn_prep) { add(N_ * 8); if (!rodas) add(N_ * nmax * 8); }
     if (!dev) {
         add(N_ * nmax * 8); add(N_ * std::max(np, 1) * 8); add(N_ * 8); add(N_ * 8);
         add(std::max(a->nsaveat, 1) * 8);
@@ -590,6 +622,10 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         HIPCHK(ctx, hipMemsetAsync(P.st_int, 0, 12 * N_ * 4, s));
     }
     if (use_gj) P.wg_jbuf = (float*)ar.take((size_t)grid * nmax * nmax * 4);
+    if (fn_prep) {
+        P.pre_dt = (double*)ar.take(N_ * 8);
+        if (!rodas) P.pre_k1 = (double*)ar.take(N_ * nmax * 8);
+    }
     if (P.nsaved) HIPCHK(ctx, hipMemsetAsync(P.nsaved, 0, N_ * 4, s));
 
     int64_t launches = 0;
@@ -611,6 +647,10 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
             if (launches > (int64_t)a->maxiters * 4 + 1024) { ctx->err = "plain engine did not terminate"; return -4; }
         }
     } else {
+        if (fn_prep) {
+            if (launch(ctx, fn_prep, (int)((N + 255) / 256), 256, P, s)) return -1;
+            launches++;
+        }
         if (launch(ctx, fn_main, (int)grid, threads, P, s, shmem)) return -1;
         launches++;
     }
@@ -645,6 +685,7 @@ double now_s() {
 struct Request {
     int64_t id = 0, N = 0;
     std::vector<double> u0, p, t0, tf;   // host copies of the inputs (SoA, N columns)
+    std::vector<int> nvec;               // per-trajectory n (workgroup servers)
     std::vector<int> sreq;               // slot -> request-table index for the admitted range
     double* u_final = nullptr;           // caller's output buffers
     int32_t* retcode = nullptr;
@@ -694,6 +735,9 @@ struct odemi_server {
     int64_t S = 0;                       // slots
     int R = 4096;                        // request table entries
     int rows = 4;
+    bool wg = false, gj = false;         // workgroup kernels (uneven n), Gauss-Jordan variant
+    size_t shmem = 0;
+    int wg_grid = 0;                     // resident workgroups of the tick kernel
     hipStream_t stream = nullptr;
     void* mem = nullptr;
     Params P;
@@ -728,7 +772,9 @@ extern "C" void odemi_server_default_opts(odemi_server_opts* o, int alg) {
 
 extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, const odemi_server_opts* opts) {
     if (!ctx || !rhs || !opts) return nullptr;
-    if (opts->alg < 0 || opts->alg > 1 || !rhs->serve_tick[opts->alg][opts->precision == ODEMI_MIXED ? 1 : 0]) {
+    const int pi = opts->precision == ODEMI_MIXED ? 1 : 0;
+    const bool wg = (rhs->flags & ODEMI_COMPILE_WG) && opts->alg == ODEMI_RODAS5P;
+    if (opts->alg < 0 || opts->alg > 1 || !(wg ? rhs->wg_serve_tick[pi] : rhs->serve_tick[opts->alg][pi])) {
         ctx->err = "the compiled problem has no serving kernels for this algorithm";
         return nullptr;
     }
@@ -738,12 +784,27 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     S->rhs = rhs;
     S->o = *opts;
     const int n = rhs->n, np = std::max(rhs->np, 1);
-    S->rows = (opts->alg == ODEMI_RODAS5P) ? 3 + n : 4;
+    S->wg = wg;
+    S->gj = wg && pi == 1 && n <= 128;
+    S->rows = wg ? 3 : (opts->alg == ODEMI_RODAS5P) ? 3 + n : 4;
+    if (wg) {
+        // resident workgroups of the tick kernel (the Gauss-Jordan variant needs a
+        // Jacobian block per workgroup)
+        S->shmem = wg_shmem_bytes(n, pi == 1, false, S->gj);
+        if (S->shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; delete S; return nullptr; }
+        int nb = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs->wg_serve_tick[pi], 256, S->shmem) != hipSuccess)
+            nb = 1;
+        S->wg_grid = std::max(nb, 1) * std::max(ctx->cu_count, 1);
+    }
     // bytes per slot: inputs, state, outputs
     const size_t per_slot = (size_t)(n + np + 2) * 8 + 16 * 8 + (size_t)S->rows * n * 8 + 12 * 4 +
-                            (size_t)n * 8 + 8 + 4 + 4 + 6 * 4 + 4;
+                            (size_t)n * 8 + 8 + 4 + 4 + 6 * 4 + 4 + (wg ? 4 : 0);
+    const size_t fixed = S->gj ? (size_t)S->wg_grid * n * n * 4 : 0;
     int64_t slots = opts->max_slots > 0 ? opts->max_slots : (1 << 20);
-    if (opts->hbm_budget_bytes > 0) slots = std::min<int64_t>(slots, (int64_t)(opts->hbm_budget_bytes / per_slot));
+    if (opts->hbm_budget_bytes > 0)
+        slots = std::min<int64_t>(slots, opts->hbm_budget_bytes > fixed
+                                             ? (int64_t)((opts->hbm_budget_bytes - fixed) / per_slot) : 0);
     if (slots < 1) { ctx->err = "HBM budget too small for one slot"; delete S; return nullptr; }
     S->S = slots;
     const size_t Sz = (size_t)slots;
@@ -753,6 +814,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     size_t o_sc = al(16 * Sz * 8), o_vec = al((size_t)S->rows * n * Sz * 8), o_int = al(12 * Sz * 4);
     size_t o_uf = al(Sz * n * 8), o_tfin = al(Sz * 8), o_rc = al(Sz * 4), o_st = al(6 * Sz * 4), o_ns = al(Sz * 4);
     size_t o_sreq = al(Sz * 4), o_done = al((size_t)S->R * 8), o_cnt = al(64);
+    size_t o_nvec = wg ? al(Sz * 4) : 0, o_jb = S->gj ? al((size_t)S->wg_grid * n * n * 4) : 0;
     if (hipMalloc(&S->mem, bytes) != hipSuccess) { ctx->err = "serving pool allocation failed"; delete S; return nullptr; }
     char* b = (char*)S->mem;
     (void)hipMemset(b + o_int, 0, 12 * Sz * 4);
@@ -770,6 +832,8 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     P.stats = (int*)(b + o_st); P.nsaved = (int*)(b + o_ns);
     P.slot_req = (int*)(b + o_sreq); P.req_done = (unsigned long long*)(b + o_done);
     P.counters = (unsigned long long*)(b + o_cnt);
+    if (wg) P.nvec = (const int*)(b + o_nvec);
+    if (S->gj) P.wg_jbuf = (float*)(b + o_jb);
     P.refine_sweeps = opts->refine_sweeps;
     P.tick_attempts = opts->tick_attempts > 0 ? opts->tick_attempts : 64;
     S->done_h.assign(S->R, 0);
@@ -798,8 +862,9 @@ extern "C" void odemi_server_destroy(odemi_server* S) {
     delete S;
 }
 
-extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const double* u0, const double* p, const double* t0,
-                                const double* tf, double* u_final, int32_t* retcode, int32_t* stats) {
+extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const int32_t* nvec, const double* u0, const double* p,
+                                const double* t0, const double* tf, double* u_final, int32_t* retcode,
+                                int32_t* stats) {
     if (!S || N <= 0 || !u0 || !t0 || !tf || !retcode) return -1;
     if (S->rhs->np > 0 && !p) { S->err = "the problem has parameters but p is NULL"; return -1; }
     if (N > S->S) { S->err = "request larger than the slot pool"; return -2; }
@@ -810,6 +875,18 @@ extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const double* u0, co
     if (np > 0 && p) r->p.assign(p, p + (size_t)np * N); else r->p.assign((size_t)std::max(np, 1) * N, 0.0);
     r->t0.assign(t0, t0 + N);
     r->tf.assign(tf, tf + N);
+    if (S->wg) {
+        r->nvec.assign(N, n);
+        if (nvec) {
+            for (int64_t k = 0; k < N; k++) {
+                if (nvec[k] < 1 || nvec[k] > n) { S->err = "nvec entries must lie in [1, n]"; delete r; return -3; }
+                r->nvec[k] = nvec[k];
+            }
+        }
+    } else if (nvec) {
+        for (int64_t k = 0; k < N; k++)
+            if (nvec[k] != n) { S->err = "uneven nvec needs a workgroup (Rodas5P) server"; delete r; return -3; }
+    }
     r->u_final = u_final; r->retcode = retcode; r->stats = stats;
     r->t_submit = now_s();
     {
@@ -892,7 +969,8 @@ static void server_loop(odemi_server* S) {
     (void)hipSetDevice(S->ctx->device);
     const int n = S->rhs->n, np = std::max(S->rhs->np, 1);
     const int pi = S->o.precision == ODEMI_MIXED ? 1 : 0;
-    hipFunction_t f_init = S->rhs->serve_init[S->o.alg][pi], f_tick = S->rhs->serve_tick[S->o.alg][pi];
+    hipFunction_t f_init = S->wg ? S->rhs->wg_serve_init[pi] : S->rhs->serve_init[S->o.alg][pi];
+    hipFunction_t f_tick = S->wg ? S->rhs->wg_serve_tick[pi] : S->rhs->serve_tick[S->o.alg][pi];
     const size_t Sz = (size_t)S->S;
     for (;;) {
         std::vector<Request*> admit;
@@ -929,14 +1007,28 @@ static void server_loop(odemi_server* S) {
             SCHK(hipMemcpyAsync((int*)S->P.slot_req + s0, r->sreq.data(), N * 4, hipMemcpyHostToDevice, S->stream));
             SCHK(hipMemsetAsync(S->P.req_done + r->rt, 0, 8, S->stream));
             SCHK(hipMemsetAsync((int*)S->P.nsaved + s0, 0, N * 4, S->stream));
+            if (S->wg) SCHK(hipMemcpyAsync((int*)S->P.nvec + s0, r->nvec.data(), N * 4, hipMemcpyHostToDevice, S->stream));
             Params Pi = S->P;
             Pi.slot0 = (int64_t)s0;
             Pi.nslots = (int64_t)N;
             void* args[] = {&Pi};
-            SCHK(hipModuleLaunchKernel(f_init, (unsigned)((N + 255) / 256), 1, 1, 256, 1, 1, 0, S->stream, args, nullptr));
+            if (S->wg)   // one workgroup per admitted slot
+                SCHK(hipModuleLaunchKernel(f_init, (unsigned)std::min<size_t>(N, (size_t)S->wg_grid), 1, 1, 256, 1, 1,
+                                           (unsigned)S->shmem, S->stream, args, nullptr));
+            else
+                SCHK(hipModuleLaunchKernel(f_init, (unsigned)((N + 255) / 256), 1, 1, 256, 1, 1, 0, S->stream, args, nullptr));
             // the host input vectors live in the Request until it finishes, so no wait here
         }
-        if (high > 0) {
+        if (high > 0 && S->wg) {
+            // workgroups take slots of [0, high) from counters[0]
+            SCHK(hipMemsetAsync(S->P.counters, 0, 8, S->stream));
+            Params Pt = S->P;
+            Pt.slot0 = 0;
+            Pt.nslots = high;
+            void* args[] = {&Pt};
+            SCHK(hipModuleLaunchKernel(f_tick, (unsigned)std::min<int64_t>(high, S->wg_grid), 1, 1, 256, 1, 1,
+                                       (unsigned)S->shmem, S->stream, args, nullptr));
+        } else if (high > 0) {
             Params Pt = S->P;
             void* args[] = {&Pt};
             SCHK(hipModuleLaunchKernel(f_tick, (unsigned)((high + 255) / 256), 1, 1, 256, 1, 1, 0, S->stream, args, nullptr));
@@ -980,6 +1072,7 @@ static void server_loop(odemi_server* S) {
                 std::vector<double>().swap(r->t0);
                 std::vector<double>().swap(r->tf);
                 std::vector<int>().swap(r->sreq);
+                std::vector<int>().swap(r->nvec);
                 S->finished++;
             }
             S->cv_done.notify_all();

@@ -87,7 +87,7 @@ def load():
     L.odemi_server_create.argtypes = [c_vp, c_vp, P(ServerOpts)]
     L.odemi_server_create.restype = c_vp
     L.odemi_server_destroy.argtypes = [c_vp]
-    L.odemi_submit.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
+    L.odemi_submit.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
     L.odemi_submit.restype = c_i64
     L.odemi_poll.argtypes = [c_vp, c_i64]
     L.odemi_poll.restype = c_i

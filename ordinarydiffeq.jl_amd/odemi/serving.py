@@ -57,15 +57,21 @@ class Request:
 
 
 class EnsembleServer:
-    """Serve one problem type (ODEFunction + state size + algorithm)."""
+    """Serve one problem type (ODEFunction + state size + algorithm).
+
+    `ns` (Rodas5P): the state sizes requests may use (uneven n, workgroup
+    kernels, n = max(ns)); each request then passes its per-trajectory `nvec`."""
 
     def __init__(self, fun, n, np_, alg, *, precision="fp64", reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0,
                  maxiters=1_000_000, max_slots=1 << 20, hbm_budget_bytes=0, tick_attempts=64, refine_sweeps=2,
-                 ctx=None):
+                 ns=None, ctx=None):
         fun = fun if isinstance(fun, ODEFunction) else ODEFunction(fun)
         self.ctx = ctx or get_context()
         algname = "tsit5" if alg.code == _lib.TSIT5 else "rodas5p"
-        self._rhs, _, _ = compile_problem(fun, n, np_, algs=(algname,), ctx=self.ctx)
+        self.ns = sorted(set(int(x) for x in ns)) if ns is not None else None
+        if self.ns is not None and (self.ns[0] < 1 or self.ns[-1] > n):
+            raise ValueError("ns entries must lie in [1, n]")
+        self._rhs, _, _ = compile_problem(fun, n, np_, algs=(algname,), ctx=self.ctx, ns=self.ns)
         self.n, self.np = n, np_
         self._L = self.ctx.L
         o = _lib.ServerOpts()
@@ -92,9 +98,10 @@ class EnsembleServer:
     def error(self):
         return self._L.odemi_server_last_error(self._handle()).decode(errors="replace")
 
-    def submit(self, u0, p, t0, tf, stats=True):
-        """u0 (n, N), p (np, N) or None, t0/tf (N,) or scalars.  Inputs are
-        copied before return; returns a Request."""
+    def submit(self, u0, p, t0, tf, stats=True, nvec=None):
+        """u0 (n, N), p (np, N) or None, t0/tf (N,) or scalars, nvec (N,) per-
+        trajectory sizes (servers created with `ns`).  Inputs are copied before
+        return; returns a Request."""
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         if u0.ndim == 1:
             u0 = u0[:, None]
@@ -106,11 +113,21 @@ class EnsembleServer:
         pp = None
         if self.np > 0:
             pp = np.ascontiguousarray(p, dtype=np.float64).reshape(self.np, N)
+        nv = None
+        if nvec is not None:
+            nv = np.ascontiguousarray(nvec, dtype=np.int32).reshape(N)
+            if self.ns is None and not (nv == self.n).all():
+                raise ValueError("uneven nvec needs a server created with ns")
+            if self.ns is not None and not np.isin(nv, self.ns).all():
+                raise ValueError("nvec entries must be among the server's ns")
+        elif self.ns is not None and self.n not in self.ns:
+            raise ValueError("pass nvec: n itself is not among the server's ns")
         uf = np.empty((n, N), dtype=np.float64)
         rc = np.zeros(N, dtype=np.int32)
         st = np.zeros((6, N), dtype=np.int32) if stats else None
         ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
-        rid = self._L.odemi_submit(self._handle(), N, ptr(u0), ptr(pp), ptr(t0), ptr(tf), ptr(uf), ptr(rc), ptr(st))
+        rid = self._L.odemi_submit(self._handle(), N, ptr(nv), ptr(u0), ptr(pp), ptr(t0), ptr(tf), ptr(uf), ptr(rc),
+                                   ptr(st))
         if rid <= 0:
             raise RuntimeError(f"odemi_submit failed ({rid}): {self.error()}")
         self._live[rid] = (uf, rc, st)

@@ -115,3 +115,45 @@ def test_hbm_budget_sizes_pool_and_rejects_oversized_request():
             srv.submit(*_cols(d, np.arange(slots + 1)))
         r = srv.submit(*_cols(d, np.arange(slots)))
         assert r.wait(timeout=120) and (r.retcode == 1).all()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_chain_serving_uneven_n_alone_equals_in_mix(prec):
+    """Uneven-n Rodas5P requests through the workgroup tick kernel: a request's
+    result is the same alone, in a mix, and in the one-launch batch engine
+    (and, for FP64, the CPU restatement), across many short ticks."""
+    N = 96
+    d = P.chain_ensemble(N=N, seed=21, m_lo=2, m_hi=16)
+    d["tf"] = np.full(N, 0.5)
+    nmax = d["u0"].shape[0]
+    ns = sorted(set(int(x) for x in d["nvec"]))
+    print(f"[case] chain serving {prec} n {ns[0]}..{ns[-1]}", flush=True)
+    cols = lambda idx: (d["u0"][:, idx], d["p"][:, idx], d["t0"][idx], d["tf"][idx])  # noqa: E731
+    probe = np.arange(10, 26)
+    kw = dict(precision=prec, reltol=d["reltol"], abstol=d["abstol"], tick_attempts=8, ns=ns)
+    with odemi.EnsembleServer(P.CHAIN, nmax, 3, odemi.Rodas5P(), **kw) as srv:
+        alone = srv.submit(*cols(probe), nvec=d["nvec"][probe])
+        assert alone.wait(timeout=150)
+        others = np.setdiff1d(np.arange(N), probe)
+        reqs, k = [], 0
+        for i, sz in enumerate([1, 5, 20, 3, 30]):
+            idx = others[k:k + sz]
+            k += sz
+            reqs.append(srv.submit(*cols(idx), nvec=d["nvec"][idx]))
+            if i == 2:
+                mixed = srv.submit(*cols(probe), nvec=d["nvec"][probe])
+        for r in reqs + [mixed]:
+            assert r.wait(timeout=150)
+    assert (alone.retcode == 1).all()
+    assert np.array_equal(alone.u_final, mixed.u_final) and np.array_equal(alone.stats, mixed.stats)
+    b = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                           odemi.EnsembleMI355X(precision=prec), reltol=d["reltol"], abstol=d["abstol"], nvec=d["nvec"])
+    assert np.array_equal(alone.u_final, b["u_final"][:, probe])
+    assert np.array_equal(alone.stats, b["stats"][:, probe])
+    if prec == "fp64":
+        for col, j in enumerate(probe[::5]):
+            n = int(d["nvec"][j])
+            r = O.solve_builtin(O.PROB_CHAIN, d["u0"][:n, j], d["p"][:, j], (0.0, 0.5), alg=O.RODAS5P,
+                                reltol=d["reltol"], abstol=d["abstol"], save_start=False)
+            c = int(np.where(probe == j)[0][0])
+            assert np.array_equal(alone.u_final[:n, c], r["u_final"])
