@@ -378,11 +378,6 @@ __device__ __forceinline__ int gj_cidx(int i) { return (i & 15) * ODEMI_GJ_R + (
 __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
     constexpr int R = ODEMI_GJ_R, NP = ODEMI_GJ_NP, NQ = (NP + 63) / 64;
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15, lane = tid & 63;
-    // row slots this trajectory occupies (uniform): rows beyond n are identity
-    // padding that no step changes (their column entries are 0), so their
-    // slots are skipped; padding columns stay in the update (the pivot row is 0
-    // there) so every staged row slot is whole
-    const int rn = (n + 15) >> 4;
     float* cbuf = gjbuf;
     float* nbuf = gjbuf + 2 * NP;
     float* stg = gjbuf + 4 * NP;
@@ -414,8 +409,7 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
 #pragma unroll
         for (int a = 0; a < R; a++)
 #pragma unroll
-            for (int b = 0; b < R; b++)
-                if (a < rn) stg[(a * 16 + tj) * R + b] = G.T[a][b];
+            for (int b = 0; b < R; b++) stg[(a * 16 + tj) * R + b] = G.T[a][b];
     }
     __syncthreads();
     // ---- steps ----
@@ -463,13 +457,11 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
             // rank-1 update of every row but p_k; column k becomes -c d (1 on row p_k)
 #pragma unroll
             for (int a = 0; a < R; a++) {
-                if (a < rn) {   // uniform
-                    const bool isp = own && (a == pa);
-                    const float ct = isp ? 0.0f : c[a];
+                const bool isp = own && (a == pa);
+                const float ct = isp ? 0.0f : c[a];
 #pragma unroll
-                    for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-ct, r[b] * d, G.T[a][b]);
-                    G.T[a][kb] = colk ? (isp ? 1.0f : -c[a] * d) : G.T[a][kb];
-                }
+                for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-ct, r[b] * d, G.T[a][b]);
+                G.T[a][kb] = colk ? (isp ? 1.0f : -c[a] * d) : G.T[a][kb];
             }
             if (tid == 0) scl[p] = d;
             key = dpp_wave_max_u32(key);
@@ -502,8 +494,7 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
 #pragma unroll
                 for (int a = 0; a < R; a++)
 #pragma unroll
-                    for (int b = 0; b < R; b++)
-                        if (a < rn) stn[(a * 16 + tj) * R + b] = G.T[a][b];
+                    for (int b = 0; b < R; b++) stn[(a * 16 + tj) * R + b] = G.T[a][b];
             }
             __syncthreads();
         }
@@ -522,7 +513,6 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
 __device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, double* x, float* zbuf) {
     constexpr int R = ODEMI_GJ_R;
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-    const int rn = (n + 15) >> 4;   // tile slots this trajectory occupies (uniform)
     float bp[R];
 #pragma unroll
     for (int b = 0; b < R; b++) {
@@ -531,13 +521,11 @@ __device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, 
     }
 #pragma unroll
     for (int a = 0; a < R; a++) {
-        if (a < rn) {   // uniform
-            float z = 0.0f;
+        float z = 0.0f;
 #pragma unroll
-            for (int b = 0; b < R; b++) z = fmaf(G.T[a][b], bp[b], z);
-            z = dpp_row_sum16(z);
-            if (tj == 15) zbuf[ti + 16 * a] = z;
-        }
+        for (int b = 0; b < R; b++) z = fmaf(G.T[a][b], bp[b], z);
+        z = dpp_row_sum16(z);
+        if (tj == 15) zbuf[ti + 16 * a] = z;
     }
     __syncthreads();
     for (int k = tid; k < n; k += blockDim.x) x[k] = (double)zbuf[pk[k]];
