@@ -52,6 +52,36 @@ def _traffic(key):
         return None
 
 
+def _accuracy(prob_id, d, outs, sample, nvec=None):
+    """Norm-wise relative final-state error |u - u_ref| / |u_ref| of each engine on
+    the first `sample` trajectories of the batch, against an FP64 CPU reference
+    at reltol 1e-12: Vern9 (the C restatement in oracle/) for non-stiff problems,
+    scipy's Radau (rtol 1e-11) for stiff Robertson, where an explicit method is
+    impractical."""
+    import oracle as O
+    S = min(sample, d["u0"].shape[1])
+    if prob_id == O.PROB_ROBERTSON:
+        from scipy.integrate import solve_ivp
+        ref = np.zeros((3, S))
+        for j in range(S):
+            k = d["p"][:, j]
+            r = solve_ivp(lambda t, u: [-k[0] * u[0] + k[2] * u[1] * u[2],
+                                        k[0] * u[0] - k[1] * u[1] ** 2 - k[2] * u[1] * u[2], k[1] * u[1] ** 2],
+                          (d["t0"][j], d["tf"][j]), d["u0"][:, j], method="Radau", rtol=1e-11, atol=1e-18)
+            ref[:, j] = r.y[:, -1]
+        kind = "scipy Radau rtol 1e-11 atol 1e-18"
+    else:
+        r = O.vern9_ensemble(prob_id, d["u0"][:, :S], d["p"][:, :S], d["t0"][:S], d["tf"][:S],
+                             nvec=None if nvec is None else nvec[:S], reltol=1e-12, abstol=1e-12)
+        ref = r["u_final"]
+        kind = "Vern9 (C restatement) reltol = abstol = 1e-12"
+    res = {"reference": kind, "sample": f"first {S} trajectories"}
+    for name, uf in outs.items():
+        e = np.linalg.norm(uf[:, :S] - ref, axis=0) / np.maximum(np.linalg.norm(ref, axis=0), 1e-300)
+        res[name] = {"median": float(np.median(e)), "p99": float(np.quantile(e, 0.99)), "max": float(e.max())}
+    return res
+
+
 def _timed(run, steps, warmup, torch):
     out = None
     for _ in range(warmup):
@@ -134,6 +164,8 @@ def bench_rodas(args, workload):
     err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
     res["mixed_vs_fp64_scaled_err"] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
                                        "max": float(err.max())}
+    res["error_vs_reference"] = _accuracy(pid, d, {"mixed": um, "fp64": u64}, 8 if workload == "robertson" else 64,
+                                          nvec=nvec)
     if workload == "robertson" and not args.no_plain:
         plain = odemi.EnsembleMI355X(engine="plain")
         op, msp, _ = _timed(lambda: run(plain), 1, 0, torch)
@@ -365,6 +397,8 @@ def main():
     um = out["u_final"].cpu().numpy()
     relm = (np.abs(um - uref) / (1e-6 + 1e-6 * np.abs(uref))).max(axis=0)
     res["mixed_vs_fp64_scaled_err"] = {"median": float(np.median(relm)), "p99": float(np.quantile(relm, 0.99))}
+    import oracle as O
+    res["error_vs_reference"] = _accuracy(O.PROB_LORENZ, d, {"mixed": um, "fp64": uref}, 256)
     if not args.no_plain:
         print("[bench] plain fp64 engine", file=sys.stderr, flush=True)
         t_a = time.perf_counter()

@@ -1062,3 +1062,199 @@ double oo_ensemble_builtin(int prob, int64_t N, int nmax, const int* nvec,
     pthread_mutex_destroy(&E.mu);
     return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Vern9: the high-accuracy FP64 reference (reltol 1e-12) used to state the   */
+/* engines' final-state error.  Tableau: Verner's 9(8) "most efficient" pair  */
+/* as held by lib/OrdinaryDiffEqVerner/src/verner_tableaus.jl:3608-3725 (the  */
+/* Float64 literals); stage structure verner_rk_perform_step.jl:1274-1402     */
+/* (16 RHS per step, u from stages 1,8..15, error from stages 1,8..16).  Step  */
+/* control: a PI controller of order 9 (controllers.jl:244-293 shape) with    */
+/* the textbook HNW initial step; the controller only decides how finely the  */
+/* reference is resolved, not its value at convergence.                       */
+/* ------------------------------------------------------------------------ */
+
+static const double V9C[16] = {0.0, 0.03462, 0.09702435063878045, 0.14553652595817068, 0.561,
+                               0.22900791159048503, 0.544992088409515, 0.645, 0.48375, 0.06757, 0.25,
+                               0.6590650618730999, 0.8206, 0.9012, 1.0, 1.0};
+/* A[s][j]: coefficient of k_{j+1} in stage s+1 (zero-based) */
+static double V9A[16][16];
+static double V9B[16], V9BT[16];
+static int v9_ready = 0;
+
+static void v9_init(void) {
+    if (v9_ready) return;
+    memset(V9A, 0, sizeof(V9A));
+    memset(V9B, 0, sizeof(V9B));
+    memset(V9BT, 0, sizeof(V9BT));
+#define A_(s, j, v) V9A[(s) - 1][(j) - 1] = (v)
+    A_(2, 1, 0.03462);
+    A_(3, 1, -0.03893354388572875); A_(3, 2, 0.13595789452450918);
+    A_(4, 1, 0.03638413148954267); A_(4, 3, 0.10915239446862801);
+    A_(5, 1, 2.0257639143939694); A_(5, 3, -7.638023836496291); A_(5, 4, 6.173259922102322);
+    A_(6, 1, 0.05112275589406061); A_(6, 4, 0.17708237945550218); A_(6, 5, 0.0008027762409222536);
+    A_(7, 1, 0.13160063579752163); A_(7, 4, -0.2957276252669636); A_(7, 5, 0.08781378035642955);
+    A_(7, 6, 0.6213052975225274);
+    A_(8, 1, 0.07166666666666667); A_(8, 6, 0.33055335789153195); A_(8, 7, 0.2427799754418014);
+    A_(9, 1, 0.071806640625); A_(9, 6, 0.3294380283228177); A_(9, 7, 0.1165190029271823);
+    A_(9, 8, -0.034013671875);
+    A_(10, 1, 0.04836757646340646); A_(10, 6, 0.03928989925676164); A_(10, 7, 0.10547409458903446);
+    A_(10, 8, -0.021438652846483126); A_(10, 9, -0.10412291746271944);
+    A_(11, 1, -0.026645614872014785); A_(11, 6, 0.03333333333333333); A_(11, 7, -0.1631072244872467);
+    A_(11, 8, 0.03396081684127761); A_(11, 9, 0.1572319413814626); A_(11, 10, 0.21522674780318796);
+    A_(12, 1, 0.03689009248708622); A_(12, 6, -0.1465181576725543); A_(12, 7, 0.2242577768172024);
+    A_(12, 8, 0.02294405717066073); A_(12, 9, -0.0035850052905728597); A_(12, 10, 0.08669223316444385);
+    A_(12, 11, 0.43838406519683376);
+    A_(13, 1, -0.4866012215113341); A_(13, 6, -6.304602650282853); A_(13, 7, -0.2812456182894729);
+    A_(13, 8, -2.679019236219849); A_(13, 9, 0.5188156639241577); A_(13, 10, 1.3653531876033418);
+    A_(13, 11, 5.8850910885039465); A_(13, 12, 2.8028087862720628);
+    A_(14, 1, 0.4185367457753472); A_(14, 6, 6.724547581906459); A_(14, 7, -0.42544428016461133);
+    A_(14, 8, 3.3432791530012653); A_(14, 9, 0.6170816631175374); A_(14, 10, -0.9299661239399329);
+    A_(14, 11, -6.099948804751011); A_(14, 12, -3.002206187889399); A_(14, 13, 0.2553202529443446);
+    A_(15, 1, -0.7793740861228848); A_(15, 6, -13.937342538107776); A_(15, 7, 1.2520488533793563);
+    A_(15, 8, -14.691500408016868); A_(15, 9, -0.494705058533141); A_(15, 10, 2.2429749091462368);
+    A_(15, 11, 13.367893803828643); A_(15, 12, 14.396650486650687); A_(15, 13, -0.79758133317768);
+    A_(15, 14, 0.4409353709534278);
+    A_(16, 1, 2.0580513374668867); A_(16, 6, 22.357937727968032); A_(16, 7, 0.9094981099755646);
+    A_(16, 8, 35.89110098240264); A_(16, 9, -3.442515027624454); A_(16, 10, -4.865481358036369);
+    A_(16, 11, -18.909803813543427); A_(16, 12, -34.26354448030452); A_(16, 13, 1.2647565216956427);
+#undef A_
+    V9B[0] = 0.014611976858423152; V9B[7] = -0.3915211862331339; V9B[8] = 0.23109325002895065;
+    V9B[9] = 0.12747667699928525; V9B[10] = 0.2246434176204158; V9B[11] = 0.5684352689748513;
+    V9B[12] = 0.058258715572158275; V9B[13] = 0.13643174034822156; V9B[14] = 0.030570139830827976;
+    V9BT[0] = -0.005357988290444578; V9BT[7] = -2.583020491182464; V9BT[8] = 0.14252253154686625;
+    V9BT[9] = 0.013420653512688676; V9BT[10] = -0.02867296291409493; V9BT[11] = 2.624999655215792;
+    V9BT[12] = -0.2825509643291537; V9BT[13] = 0.13643174034822156; V9BT[14] = 0.030570139830827976;
+    V9BT[15] = -0.04834231373823958;
+    v9_ready = 1;
+}
+
+static double v9_rms(const double* x, const double* sc, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; i++) { double r = x[i] / sc[i]; s += r * r; }
+    return sqrt(s / (n > 0 ? n : 1));
+}
+
+/* One Vern9 solve of a built-in problem from t0 to tf.  Returns 0 on success,
+ * 1 when maxsteps was exceeded, 2 when dt underflowed; *nsteps = accepted steps. */
+int oo_vern9_builtin(int prob, int n, const double* u0, const double* p, double t0, double tf,
+                     double reltol, double abstol, int64_t maxsteps, double* u_final, int64_t* nsteps) {
+    oo_rhs_fn f; oo_jac_fn jfn;
+    if (builtin_fns(prob, &f, &jfn) < 0 || n <= 0 || n > 256) return -1;
+    v9_init();
+    builtin_user bu = {n};
+    double u[256], uprev[256], tmp[256], ut[256], sc[256], k[16][256];
+    for (int i = 0; i < n; i++) uprev[i] = u0[i];
+    double t = t0;
+    /* HNW initial step, order 9 */
+    f(k[0], uprev, p, t, &bu);
+    for (int i = 0; i < n; i++) sc[i] = abstol + reltol * fabs(uprev[i]);
+    double d0 = v9_rms(uprev, sc, n), d1 = v9_rms(k[0], sc, n);
+    double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+    if (h0 > tf - t0) h0 = tf - t0;
+    for (int i = 0; i < n; i++) tmp[i] = uprev[i] + h0 * k[0][i];
+    f(k[1], tmp, p, t + h0, &bu);
+    for (int i = 0; i < n; i++) ut[i] = k[1][i] - k[0][i];
+    double d2 = v9_rms(ut, sc, n) / h0;
+    double mx = d1 > d2 ? d1 : d2;
+    double h1 = mx <= 1e-15 ? fmax(1e-6, h0 * 1e-3) : pow(0.01 / mx, 1.0 / 10.0);
+    double dt = fmin(100.0 * h0, h1);
+    if (dt > tf - t0) dt = tf - t0;
+    const double beta1 = 7.0 / 90.0, beta2 = 2.0 / 45.0, qmin = 0.2, qmax = 10.0, gamma = 0.9;
+    double qold = 1e-4;
+    int64_t acc = 0, it = 0;
+    while (t < tf) {
+        if (++it > maxsteps) { for (int i = 0; i < n; i++) u_final[i] = uprev[i]; *nsteps = acc; return 1; }
+        if (t + dt > tf) dt = tf - t;
+        if (dt < 1e-14 * fmax(fabs(t), 1.0)) { for (int i = 0; i < n; i++) u_final[i] = uprev[i]; *nsteps = acc; return 2; }
+        f(k[0], uprev, p, t, &bu);
+        for (int s = 1; s < 16; s++) {
+            for (int i = 0; i < n; i++) {
+                double a = 0.0;
+                for (int q = 0; q < s; q++) a += V9A[s][q] * k[q][i];
+                tmp[i] = uprev[i] + dt * a;
+            }
+            f(k[s], tmp, p, t + V9C[s] * dt, &bu);
+        }
+        for (int i = 0; i < n; i++) {
+            double b = 0.0, e = 0.0;
+            for (int q = 0; q < 16; q++) { b += V9B[q] * k[q][i]; e += V9BT[q] * k[q][i]; }
+            u[i] = uprev[i] + dt * b;
+            ut[i] = dt * e;
+            sc[i] = abstol + reltol * fmax(fabs(uprev[i]), fabs(u[i]));
+        }
+        double EEst = v9_rms(ut, sc, n);
+        if (isnan(EEst)) EEst = 1e10;
+        double q11 = pow(EEst, beta1);
+        if (EEst <= 1.0) {
+            double q = q11 / pow(qold, beta2) / gamma;
+            q = fmax(1.0 / qmax, fmin(1.0 / qmin, q));
+            t += dt;
+            for (int i = 0; i < n; i++) uprev[i] = u[i];
+            qold = fmax(EEst, 1e-4);
+            acc++;
+            dt = dt / q;
+        } else {
+            double q = fmin(1.0 / qmin, q11 / gamma);
+            dt = dt / q;
+        }
+    }
+    for (int i = 0; i < n; i++) u_final[i] = uprev[i];
+    *nsteps = acc;
+    return 0;
+}
+
+typedef struct {
+    int prob; int64_t N; int nmax; const int* nvec; const double* u0; int np; const double* p;
+    const double* t0; const double* tf; double reltol, abstol; int64_t maxsteps;
+    double* u_final; int32_t* status; int64_t* nsteps; int64_t next; pthread_mutex_t mu;
+} v9ens_t;
+
+static void* v9_worker(void* arg) {
+    v9ens_t* E = (v9ens_t*)arg;
+    double u0[256], pp[16], uf[256];
+    for (;;) {
+        pthread_mutex_lock(&E->mu);
+        int64_t j = E->next++;
+        pthread_mutex_unlock(&E->mu);
+        if (j >= E->N) break;
+        int n = E->nvec ? E->nvec[j] : E->nmax;
+        for (int i = 0; i < n; i++) u0[i] = E->u0[(size_t)i * E->N + j];
+        for (int k = 0; k < E->np; k++) pp[k] = E->p[(size_t)k * E->N + j];
+        int64_t ns = 0;
+        int st = oo_vern9_builtin(E->prob, n, u0, pp, E->t0[j], E->tf[j], E->reltol, E->abstol, E->maxsteps, uf, &ns);
+        for (int i = 0; i < E->nmax; i++) E->u_final[(size_t)i * E->N + j] = (i < n) ? uf[i] : 0.0;
+        if (E->status) E->status[j] = st;
+        if (E->nsteps) E->nsteps[j] = ns;
+    }
+    return NULL;
+}
+
+double oo_vern9_ensemble(int prob, int64_t N, int nmax, const int* nvec, const double* u0, int np, const double* p,
+                         const double* t0, const double* tf, double reltol, double abstol, int64_t maxsteps,
+                         int threads, double* u_final, int32_t* status, int64_t* nsteps) {
+    v9_init();
+    v9ens_t E;
+    E.prob = prob; E.N = N; E.nmax = nmax; E.nvec = nvec; E.u0 = u0; E.np = np; E.p = p; E.t0 = t0; E.tf = tf;
+    E.reltol = reltol; E.abstol = abstol; E.maxsteps = maxsteps; E.u_final = u_final; E.status = status;
+    E.nsteps = nsteps; E.next = 0;
+    pthread_mutex_init(&E.mu, NULL);
+    if (threads < 1) threads = 1;
+    struct timespec a, b;
+    clock_gettime(CLOCK_MONOTONIC, &a);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * threads);
+    for (int i = 0; i < threads; i++) pthread_create(&th[i], NULL, v9_worker, &E);
+    for (int i = 0; i < threads; i++) pthread_join(th[i], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &b);
+    free(th);
+    pthread_mutex_destroy(&E.mu);
+    return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+}
+
+void oo_vern9_tableau(double* c /*16*/, double* a /*16x16*/, double* b /*16*/, double* btilde /*16*/) {
+    v9_init();
+    memcpy(c, V9C, sizeof(V9C));
+    memcpy(a, V9A, sizeof(V9A));
+    memcpy(b, V9B, sizeof(V9B));
+    memcpy(btilde, V9BT, sizeof(V9BT));
+}

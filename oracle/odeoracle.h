@@ -97,6 +97,16 @@ void oo_rodas5p_tableau(double* A /*8x8*/, double* C /*8x8*/, double* c /*8*/, d
                         double* H /*3x8*/, double* b /*8*/, double* btilde /*8*/, double* gamma);
 void oo_tsit5_interp_coeffs(double* r /*7x4, r[i*4 + j] = r_{i+1, j+1}*/);
 
+/* Vern9 high-accuracy reference (reltol ~1e-12) for built-in problems; the
+ * ensemble form runs `threads` workers and returns wall seconds.  status per
+ * trajectory: 0 ok, 1 maxsteps exceeded, 2 dt underflow. */
+int oo_vern9_builtin(int prob, int n, const double* u0, const double* p, double t0, double tf,
+                     double reltol, double abstol, int64_t maxsteps, double* u_final, int64_t* nsteps);
+double oo_vern9_ensemble(int prob, int64_t N, int nmax, const int* nvec, const double* u0, int np, const double* p,
+                         const double* t0, const double* tf, double reltol, double abstol, int64_t maxsteps,
+                         int threads, double* u_final, int32_t* status, int64_t* nsteps);
+void oo_vern9_tableau(double* c /*16*/, double* a /*16x16*/, double* b /*16*/, double* btilde /*16*/);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,12 @@ def lib():
         L.oo_tsit5_tableau.argtypes = [P(d)] * 4
         L.oo_rodas5p_tableau.argtypes = [P(d)] * 8
         L.oo_tsit5_interp_coeffs.argtypes = [P(d)]
+        L.oo_vern9_builtin.argtypes = [ctypes.c_int, ctypes.c_int, P(d), P(d), d, d, d, d, i64, P(d), P(i64)]
+        L.oo_vern9_builtin.restype = ctypes.c_int
+        L.oo_vern9_ensemble.argtypes = [ctypes.c_int, i64, ctypes.c_int, P(ctypes.c_int), P(d), ctypes.c_int, P(d),
+                                        P(d), P(d), d, d, i64, ctypes.c_int, P(d), P(ctypes.c_int32), P(i64)]
+        L.oo_vern9_ensemble.restype = d
+        L.oo_vern9_tableau.argtypes = [P(d)] * 4
         _lib = L
     return _lib
 
@@ -245,3 +251,42 @@ def det_log(x):
 
 def det_exp(x):
     return lib().oo_det_exp(x)
+
+
+def vern9_tableau():
+    """Vern9 (c[16], A[16][16], b[16], btilde[16]) as held by the C restatement."""
+    c, a, b, bt = np.zeros(16), np.zeros((16, 16)), np.zeros(16), np.zeros(16)
+    lib().oo_vern9_tableau(_dp(c), _dp(a), _dp(b), _dp(bt))
+    return c, a, b, bt
+
+
+def vern9_builtin(prob, u0, p, tspan, reltol=1e-12, abstol=1e-12, maxsteps=10_000_000):
+    """High-accuracy Vern9 reference solve of a built-in problem; returns (u_final, status, nsteps)."""
+    u0 = np.ascontiguousarray(u0, dtype=np.float64)
+    p = np.ascontiguousarray(p if p is not None and len(p) else [0.0], dtype=np.float64)
+    uf = np.zeros(len(u0))
+    ns = ctypes.c_int64()
+    st = lib().oo_vern9_builtin(prob, len(u0), _dp(u0), _dp(p), float(tspan[0]), float(tspan[1]), reltol, abstol,
+                                maxsteps, _dp(uf), ctypes.byref(ns))
+    return uf, st, ns.value
+
+
+def vern9_ensemble(prob, u0_soa, p_soa, t0, tf, nvec=None, reltol=1e-12, abstol=1e-12, maxsteps=10_000_000,
+                   threads=None):
+    """Vern9 reference for every trajectory of an SoA batch (threaded)."""
+    u0_soa = np.ascontiguousarray(u0_soa, dtype=np.float64)
+    p_soa = np.ascontiguousarray(p_soa, dtype=np.float64)
+    nmax, N = u0_soa.shape
+    t0 = np.ascontiguousarray(np.broadcast_to(t0, (N,)), dtype=np.float64)
+    tf = np.ascontiguousarray(np.broadcast_to(tf, (N,)), dtype=np.float64)
+    nv = None if nvec is None else np.ascontiguousarray(nvec, dtype=np.int32)
+    uf = np.zeros((nmax, N))
+    st = np.zeros(N, dtype=np.int32)
+    ns = np.zeros(N, dtype=np.int64)
+    threads = threads or min(len(os.sched_getaffinity(0)), 16)
+    secs = lib().oo_vern9_ensemble(prob, N, nmax, nv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)) if nv is not None
+                                   else None, _dp(u0_soa), p_soa.shape[0], _dp(p_soa), _dp(t0), _dp(tf), reltol,
+                                   abstol, maxsteps, threads, _dp(uf),
+                                   st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return dict(u_final=uf, status=st, nsteps=ns, seconds=secs)

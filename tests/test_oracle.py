@@ -381,6 +381,39 @@ def test_tsit5_tight_tolerance_vs_dop853():
     assert np.max(np.abs(r["u_final"] - ref.y[:, -1]) / np.abs(ref.y[:, -1])) < 1e-8
 
 
+def test_vern9_tableau_order_conditions():
+    """The Vern9 reference: all 486 order conditions through order 9 for b, the
+    embedded pair b - btilde to order 8 only, row sums = c."""
+    c, a, b, bt = O.vern9_tableau()
+    A = [[mp.mpf(float(x)) for x in row] for row in a]
+    B = [mp.mpf(float(x)) for x in b]
+    for i in range(16):
+        assert abs(mp.fsum(A[i]) - mp.mpf(float(c[i]))) < 1e-14
+    assert max(_order_residuals(A, B, 9)) < 1e-12
+    Bh = [B[i] - mp.mpf(float(bt[i])) for i in range(16)]
+    assert max(_order_residuals(A, Bh, 8)) < 1e-12
+    assert max(_order_residuals(A, Bh, 9)) > 1e-8
+
+
+def test_vern9_reference_accuracy():
+    """u' = 1.01 u to machine precision; Lorenz to t = 1 against scipy DOP853 at
+    rtol 1e-13; an oscillator chain against the converged Rodas5P restatement."""
+    from scipy.integrate import solve_ivp
+    uf, st, ns = O.vern9_builtin(O.PROB_LINEAR, [0.5], [1.01], (0.0, 1.0))
+    assert st == 0 and abs(uf[0] / (0.5 * math.exp(1.01)) - 1) < 1e-14
+    p = [10.0, 28.0, 8 / 3]
+    uf, st, _ = O.vern9_builtin(O.PROB_LORENZ, [1.0, 0.0, 0.0], p, (0.0, 1.0))
+    ref = solve_ivp(lambda t, u: [p[0] * (u[1] - u[0]), u[0] * (p[1] - u[2]) - u[1], u[0] * u[1] - p[2] * u[2]],
+                    (0, 1), [1, 0, 0], method="DOP853", rtol=1e-13, atol=1e-14)
+    assert st == 0 and np.max(np.abs(uf - ref.y[:, -1]) / np.abs(ref.y[:, -1])) < 1e-10
+    # chain, m = 4 (n = 8), moderate stiffness
+    u0 = [0.1, -0.05, 0.02, 0.0, 0, 0, 0, 0]
+    pc = [100.0, 0.3, 0.5]
+    uf, st, _ = O.vern9_builtin(O.PROB_CHAIN, u0, pc, (0.0, 2.0))
+    r = O.solve_builtin(O.PROB_CHAIN, u0, pc, (0.0, 2.0), alg=O.RODAS5P, reltol=1e-10, abstol=1e-12)
+    assert st == 0 and np.linalg.norm(uf - r["u_final"]) / np.linalg.norm(uf) < 1e-7
+
+
 def test_lu_matches_numpy():
     rng = np.random.default_rng(7)
     for n in (1, 2, 3, 5, 8, 17):
