@@ -105,26 +105,27 @@ def bench_rodas(args, workload):
     FP32 LU + FP64 refinement engine, CPU restatement on a bounded sample."""
     import torch
     import odemi
+    import oracle as O
     from odemi import problems as P
     dev = torch.device("cuda:0")
     if workload == "robertson":
         N = args.N or 262144
         d = P.robertson_ensemble(N=N)
-        fun, pid = P.ROBERTSON, 1
+        fun, pid = P.ROBERTSON, O.PROB_ROBERTSON
         nvec = None
         desc = ("Robertson ensemble, 262,144 trajectories, Rodas5P, k=(0.04,3e7,1e4)*LogU[0.5,2], u0=(1,0,0), "
                 "tspan (0,1e5), reltol 1e-6, abstol 1e-10, analytic 3x3 J")
     else:
         N = args.N or 16384
         d = P.chain_ensemble(N=N)
-        fun, pid = P.CHAIN, 3
+        fun, pid = P.CHAIN, O.PROB_CHAIN
         nvec = d["nvec"]
         desc = ("Uneven oscillator chains, 16,384 trajectories, n=2m, m~U{4..64}, k~U[1,1e4], c~U[0.01,1], "
                 "alpha~U[0,1], x0~N(0,0.1^2), tspan (0,10), Rodas5P, reltol=abstol=1e-6, one launch")
     T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()
     u0, p, t0, tf = T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"])
     kw = dict(reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=True, nvec=nvec)
-    mixed = odemi.EnsembleMI355X(precision="mixed")
+    mixed = odemi.EnsembleMI355X(precision="mixed", refine_sweeps=args.refine_sweeps)
     fp64 = odemi.EnsembleMI355X(precision="fp64")
     run = lambda ea: odemi.solve_arrays(fun, odemi.Rodas5P(), u0, p, t0, tf, ea, **kw)
     # hipRTC compile (cached on disk) stays outside every timed region
@@ -157,22 +158,24 @@ def bench_rodas(args, workload):
                      "traffic": None, "kernel_ms": kms,
                      "kernel": "odemi_rodas_persist_mixed" if nvec is None else "odemi_rodas_wg_mixed"},
     }
-    o64, ms64, kms64 = _timed(lambda: run(fp64), 1, 0, torch)
-    res["persistent_fp64"] = {"trajectories_per_s": N / (ms64 / 1e3), "ms": ms64, "kernel_ms": kms64}
-    u64 = o64["u_final"].cpu().numpy()
+    res["refine_sweeps"] = args.refine_sweeps
     um = out["u_final"].cpu().numpy()
-    err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
-    res["mixed_vs_fp64_scaled_err"] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
-                                       "max": float(err.max())}
-    res["error_vs_reference"] = _accuracy(pid, d, {"mixed": um, "fp64": u64}, 8 if workload == "robertson" else 64,
-                                          nvec=nvec)
+    engines = {"mixed": um}
+    if not args.no_fp64:
+        o64, ms64, kms64 = _timed(lambda: run(fp64), 1, 0, torch)
+        res["persistent_fp64"] = {"trajectories_per_s": N / (ms64 / 1e3), "ms": ms64, "kernel_ms": kms64}
+        u64 = o64["u_final"].cpu().numpy()
+        err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
+        res["mixed_vs_fp64_scaled_err"] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
+                                           "max": float(err.max())}
+        engines["fp64"] = u64
+    res["error_vs_reference"] = _accuracy(pid, d, engines, 8 if workload == "robertson" else 64, nvec=nvec)
     if workload == "robertson" and not args.no_plain:
         plain = odemi.EnsembleMI355X(engine="plain")
         op, msp, _ = _timed(lambda: run(plain), 1, 0, torch)
         res["plain_fp64"] = {"trajectories_per_s": N / (msp / 1e3), "ms": msp, "launches": int(op["launches"])}
         res["batched_over_plain"] = (N / (ms / 1e3)) / (N / (msp / 1e3))
     if not args.no_cpu:
-        import oracle as O
         threads = min(len(os.sched_getaffinity(0)), 16)
         S = min(args.cpu_sample if workload == "robertson" else 256, N)
         print(f"[bench] cpu baseline {S} trajectories on {threads} threads", file=sys.stderr, flush=True)
@@ -317,6 +320,8 @@ def main():
     ap.add_argument("--chain-max", type=int, default=16, help="serving: chains per chain request, max")
     ap.add_argument("--hbm-budget-gb", type=int, default=8, help="serving: slot-pool budget, GiB")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="Rodas5P workloads: skip the FP64 engine")
+    ap.add_argument("--refine-sweeps", type=int, default=1, help="mixed Rodas5P: FP64 refinement sweeps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1048576)
     args = ap.parse_args()

@@ -98,7 +98,7 @@ typedef struct {
     double* t_final;          /* [N] (may be NULL) */
     int32_t* retcode;         /* [N] */
     int32_t* stats;           /* [6][N]: nf, naccept, nreject, njacs, nw, nsolve (may be NULL) */
-    int refine_sweeps;        /* ODEMI_MIXED Rodas5P: FP64 refinement sweeps (default 2) */
+    int refine_sweeps;        /* ODEMI_MIXED Rodas5P: FP64 refinement sweeps per stage solve (default 1) */
     int block;                /* threads per workgroup (0: engine default) */
     int waves_per_traj;       /* workgroup kernels: waves cooperating on one trajectory (0: auto) */
     int sort_by_cost;         /* 1: hand out trajectories longest-span first */

@@ -195,7 +195,7 @@ extern "C" void odemi_default_args(odemi_solve_args* a, int alg) {
     a->save_end = 1;
     a->save_everystep = 0;
     a->save_end_user = -1;
-    a->refine_sweeps = 2;
+    a->refine_sweeps = 1;
 }
 
 static std::string build_source(const char* problem_src, int n, int np, int flags) {
@@ -767,7 +767,7 @@ extern "C" void odemi_server_default_opts(odemi_server_opts* o, int alg) {
     o->maxiters = 1000000;
     o->tick_attempts = 64;
     o->max_slots = 1 << 20;
-    o->refine_sweeps = 2;
+    o->refine_sweeps = 1;
 }
 
 extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, const odemi_server_opts* opts) {

@@ -33,7 +33,7 @@ const FP64, MIXED = Cint(0), Cint(1)
 const COMPILE_TSIT5, COMPILE_RODAS5P, HAS_USER_TGRAD = Cint(1), Cint(2), Cint(4)
 
 """
-    EnsembleMI355X(; device = 0, engine = :persistent, precision = :fp64, refine_sweeps = 2)
+    EnsembleMI355X(; device = 0, engine = :persistent, precision = :fp64, refine_sweeps = 1)
 
 `engine = :plain` is the FP64 one-kernel-per-step engine; `precision = :mixed`
 runs Tsit5 stages (or the Rodas5P LU) in FP32 with FP64 state, error norm and
@@ -45,7 +45,7 @@ struct EnsembleMI355X <: SciMLBase.EnsembleAlgorithm
     precision::Symbol
     refine_sweeps::Int
 end
-EnsembleMI355X(; device = 0, engine = :persistent, precision = :fp64, refine_sweeps = 2) =
+EnsembleMI355X(; device = 0, engine = :persistent, precision = :fp64, refine_sweeps = 1) =
     EnsembleMI355X(device, engine, precision, refine_sweeps)
 
 # ---------------------------------------------------------------------------

@@ -90,7 +90,7 @@ class EnsembleMI355X:
                Rodas5P: FP32 LU with FP64 iterative refinement).
     """
 
-    def __init__(self, device=0, engine="persistent", precision="fp64", refine_sweeps=2, sort_by_cost=True,
+    def __init__(self, device=0, engine="persistent", precision="fp64", refine_sweeps=1, sort_by_cost=True,
                  block=0):
         if engine not in ("persistent", "plain"):
             raise ValueError("engine must be 'persistent' or 'plain'")

@@ -63,7 +63,7 @@ class EnsembleServer:
     kernels, n = max(ns)); each request then passes its per-trajectory `nvec`."""
 
     def __init__(self, fun, n, np_, alg, *, precision="fp64", reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0,
-                 maxiters=1_000_000, max_slots=1 << 20, hbm_budget_bytes=0, tick_attempts=64, refine_sweeps=2,
+                 maxiters=1_000_000, max_slots=1 << 20, hbm_budget_bytes=0, tick_attempts=64, refine_sweeps=1,
                  ns=None, ctx=None):
         fun = fun if isinstance(fun, ODEFunction) else ODEFunction(fun)
         self.ctx = ctx or get_context()
