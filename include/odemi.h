@@ -164,6 +164,8 @@ typedef struct {
 
 void odemi_server_default_opts(odemi_server_opts* opts, int alg);
 odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, const odemi_server_opts* opts);
+/* Stops the worker; unfinished requests end as failed (-4).  No other call may
+ * be in progress on the server. */
 void odemi_server_destroy(odemi_server* server);
 /* Submit N trajectories (host SoA inputs, copied before return; u0 is [n][N]
  * with n the compiled dimension).  nvec [N] gives each trajectory's own
@@ -180,6 +182,10 @@ int odemi_wait(odemi_server* server, int64_t request, double timeout_s);
 int odemi_request_times(odemi_server* server, int64_t request, double* t_submit, double* t_admit, double* t_done);
 int odemi_server_info(odemi_server* server, int64_t* slots, int64_t* ticks, int64_t* pending, int64_t* active,
                       int64_t* finished);
+/* Forget a finished request (its id becomes unknown).  -2 while it is queued
+ * or running.  Records of finished requests are otherwise kept for
+ * odemi_request_times until the server is destroyed. */
+int odemi_request_release(odemi_server* server, int64_t request);
 const char* odemi_server_last_error(odemi_server* server);
 
 #ifdef __cplusplus

@@ -854,12 +854,31 @@ extern "C" void odemi_server_destroy(odemi_server* S) {
     }
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
+    {
+        // requests that never finished end as failed (-4); no other call may be
+        // in progress on this server from here on
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto& kv : S->all)
+            if (kv.second->state == 0 || kv.second->state == 1) kv.second->state = -4;
+        S->cv_done.notify_all();
+    }
     (void)hipSetDevice(S->ctx->device);
     (void)hipStreamSynchronize(S->stream);
     (void)hipStreamDestroy(S->stream);
     (void)hipFree(S->mem);
     for (auto& kv : S->all) delete kv.second;
     delete S;
+}
+
+extern "C" int odemi_request_release(odemi_server* S, int64_t id) {
+    if (!S) return -1;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return -1;
+    if (it->second->state == 0 || it->second->state == 1) return -2;   // still queued or running
+    delete it->second;
+    S->all.erase(it);
+    return 0;
 }
 
 extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const int32_t* nvec, const double* u0, const double* p,
@@ -891,6 +910,7 @@ extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const int32_t* nvec,
     r->t_submit = now_s();
     {
         std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) { S->err = "the server has stopped"; delete r; return -4; }
         r->id = S->next_id++;
         S->all[r->id] = r;
         S->pending.push_back(r);

@@ -53,7 +53,7 @@ EXPORTS = ["odemi_create", "odemi_destroy", "odemi_last_error", "odemi_device_in
            "odemi_compile_rhs", "odemi_rhs_destroy", "odemi_solve_batch", "odemi_default_args",
            "odemi_rtc_codeobject", "odemi_server_default_opts", "odemi_server_create", "odemi_server_destroy",
            "odemi_submit", "odemi_poll", "odemi_wait", "odemi_request_times", "odemi_server_info",
-           "odemi_server_last_error"]
+           "odemi_server_last_error", "odemi_request_release"]
 
 _lib = None
 
@@ -97,6 +97,8 @@ def load():
     L.odemi_request_times.restype = c_i
     L.odemi_server_info.argtypes = [c_vp] + [P(c_i64)] * 5
     L.odemi_server_info.restype = c_i
+    L.odemi_request_release.argtypes = [c_vp, c_i64]
+    L.odemi_request_release.restype = c_i
     L.odemi_server_last_error.argtypes = [c_vp]
     L.odemi_server_last_error.restype = ctypes.c_char_p
     _lib = L

@@ -55,6 +55,11 @@ class Request:
         s, _, d = self.times()
         return d - s
 
+    def release(self):
+        """Drop the server's record of this finished request (times() then fails)."""
+        if self.server._h and self.server._L.odemi_request_release(self.server._h, self.id) == 0:
+            self.server._live.pop(self.id, None)
+
 
 class EnsembleServer:
     """Serve one problem type (ODEFunction + state size + algorithm).

@@ -115,6 +115,10 @@ def test_hbm_budget_sizes_pool_and_rejects_oversized_request():
             srv.submit(*_cols(d, np.arange(slots + 1)))
         r = srv.submit(*_cols(d, np.arange(slots)))
         assert r.wait(timeout=120) and (r.retcode == 1).all()
+        r.release()                                        # the record is gone, the outputs stay
+        with pytest.raises(RuntimeError):
+            r.times()
+        assert (r.retcode == 1).all()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
