@@ -713,29 +713,53 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
 }
 
 // ---------------------------------------------------------------------------
-// Work queue: wave-aggregated fetch of trajectory indices
+// Work queue.  A wave takes queue slots from the global head in chunks of
+// ODEMI_QUEUE_CHUNK (>= 64) with one atomic and deals them to its idle lanes
+// as they free up.  One device-wide counter shared by every wave of all XCDs
+// is a serialisation point: with one atomic per refill (a lane finishes every
+// few step attempts of a wave) it bounded the whole lane engine; chunking cuts
+// the atomics by the chunk size.  Slots beyond the end come back >= N.
+// `q` is wave-uniform; every lane of the wave calls this together.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ i64 wave_fetch(u64* head, bool need) {
-    u64 mask = __ballot(need);
+#ifndef ODEMI_QUEUE_CHUNK
+#define ODEMI_QUEUE_CHUNK 64
+#endif
+static_assert(ODEMI_QUEUE_CHUNK >= 64, "a refill must cover a whole wave");
+struct WaveQueue {
+    u64 next = 0, end = 0;   // slots [next, end) taken from the head, not yet dealt
+};
+__device__ __forceinline__ i64 wave_fetch(u64* head, WaveQueue& q, bool need) {
+    const u64 mask = __ballot(need);
     if (mask == 0) return -1;
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    u64 base = 0;
-    if (lane == leader) base = atomicAdd(head, (u64)__popcll(mask));
-    base = __shfl(base, leader);
-    if (!need) return -1;
-    u64 below = mask & ((lane == 0) ? 0ULL : (~0ULL >> (64 - lane)));
-    return (i64)(base + (u64)__popcll(below));
+    const int lane = __lane_id();
+    const u64 cnt = (u64)__popcll(mask);
+    const u64 rank = (u64)__popcll(mask & ((lane == 0) ? 0ULL : (~0ULL >> (64 - lane))));
+    const u64 avail = q.end - q.next;
+    i64 slot = (i64)(q.next + rank);
+    if (avail < cnt) {
+        // the undealt rest first, then a fresh chunk
+        const int leader = __ffsll((long long)mask) - 1;
+        u64 base = 0;
+        if (lane == leader) base = atomicAdd(head, (u64)ODEMI_QUEUE_CHUNK);
+        base = __shfl(base, leader);
+        if (rank >= avail) slot = (i64)(base + (rank - avail));
+        q.next = base + (cnt - avail);
+        q.end = base + (u64)ODEMI_QUEUE_CHUNK;
+    } else {
+        q.next += cnt;
+    }
+    return need ? slot : -1;
 }
 
 template <bool SAVES, typename S, typename K, int N>
 __device__ void tsit5_persistent(const Params& P) {
     Tsit5Lane<S, K, N> L;
+    WaveQueue q;
     bool active = false, exhausted = false;
     i64 j = 0;
     for (;;) {
         bool need = !active && !exhausted;
-        i64 slot = wave_fetch(&P.counters[0], need);
+        i64 slot = wave_fetch(&P.counters[0], q, need);
         if (need) {
             if (slot < P.N) {
                 j = P.order ? P.order[slot] : slot;
@@ -1166,11 +1190,12 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
 template <typename F, int N>
 __device__ void rodas_persistent(const Params& P) {
     RodasLane<F, N> L;
+    WaveQueue q;
     bool active = false, exhausted = false;
     i64 j = 0;
     for (;;) {
         bool need = !active && !exhausted;
-        i64 slot = wave_fetch(&P.counters[0], need);
+        i64 slot = wave_fetch(&P.counters[0], q, need);
         if (need) {
             if (slot < P.N) {
                 j = P.order ? P.order[slot] : slot;

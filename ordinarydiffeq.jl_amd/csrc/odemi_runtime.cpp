@@ -205,6 +205,9 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     // minimum waves per SIMD for the persistent lane kernels (occupancy vs spills)
     const char* mw = getenv("ODEMI_MIN_WAVES");
     s << "#define ODEMI_MIN_WAVES " << ((mw && *mw) ? atoi(mw) : 4) << "\n";
+    // queue slots a wave takes per atomic (>= 64)
+    const char* qc = getenv("ODEMI_QUEUE_CHUNK");
+    if (qc && *qc && atoi(qc) >= 64) s << "#define ODEMI_QUEUE_CHUNK " << atoi(qc) << "\n";
     if (flags & ODEMI_COMPILE_WG) s << "#define ODEMI_WITH_WG 1\n";
     s << problem_src << "\n";
     s << kDeviceSource << "\n";
