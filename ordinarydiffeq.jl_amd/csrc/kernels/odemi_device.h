@@ -191,6 +191,24 @@ __device__ __forceinline__ double fastpower(double x, double y) {
     float prod = (float)y * lg;
     return (double)(float)det_exp((double)prod * kLN2);
 }
+// The same fastpower with the hardware reciprocal in Goldberg's rational and
+// the hardware exp2 (v_rcp_f32 / v_exp_f32, ~1 ulp): the mixed-precision lane
+// engines' controller.  FastPower evaluates exp2 under @fastmath; this form is
+// not bit-matched to the CPU restatement (those engines are checked to
+// tolerance).
+__device__ __forceinline__ float fastpower_hw(double x, double y) {
+    if (x == 0.0) return 0.0f;
+    if (isinf(x) && isinf(y)) return __builtin_huge_valf();
+    const float xf = (float)x;
+    const unsigned ux = __float_as_uint(xf);
+    const unsigned ex = (ux & 0x7F800000u) >> 23;
+    const bool hi = (ux & 0x00400000u) != 0u;
+    const float signif = __uint_as_float((ux & 0x007FFFFFu) | (hi ? 0x3f000000u : 0x3f800000u)) - 1.0f;
+    const float fexp = (float)ex - (hi ? 126.0f : 127.0f);
+    const float t3 = signif * (0.338953f * signif + 2.198599f);
+    const float lg = fexp + t3 * __builtin_amdgcn_rcpf(signif + 1.523692f);
+    return __builtin_amdgcn_exp2f((float)y * lg);
+}
 
 // ---------------------------------------------------------------------------
 // Tableaus
@@ -290,7 +308,14 @@ __device__ __forceinline__ CtlConst ctl_const(bool rosenbrock) {
     return c;
 }
 
-// integrator_utils.jl:268-324 (adaptive branch, tstops = [tf])
+// integrator_utils.jl:268-324 (adaptive branch, tstops = [tf]).
+// apply_step! (integrator_utils.jl:165-191) also calls it right after
+// dt = dtpropose; the engines skip that call.  With t < tf and dt >= 0 (or
+// NaN) it is redundant: the loopheader's fix_dt_at_bounds + modify_dt_for_tstops
+// that follow overwrite dtpropose and next_step_tstop, and min(dt, tf - t)
+// taken before dt is clamped to [max(eps(t), dtmin), dtmax] leaves the same dt
+// after the final min(dt, tf - t) (cases dt <= tf - t, floor > tf - t,
+// dtmax < tf - t, dtmax >= tf - t).  The CPU restatement keeps both calls.
 __device__ __forceinline__ void modify_dt_for_tstops(Ctl& c) {
     double distance = fabs(c.tf - c.t);
     double tol = 0.0;
@@ -306,26 +331,40 @@ __device__ __forceinline__ void fix_dt_at_bounds(Ctl& c, double dtmin) {
     c.dt = jl_min(c.dtmax, c.dt);
     c.dt = jl_max(c.dt, fabs(jl_max(jl_eps(c.t), dtmin)));
 }
-// controllers.jl:805-821
+// controllers.jl:805-821.  HW: the mixed engines' form (hardware fastpower,
+// reciprocals instead of the two divisions).
+template <bool HW = false>
 __device__ __forceinline__ double stepsize_controller(Ctl& c, const CtlConst& k) {
     double qmax = (c.success_iter == 0) ? k.qmax_first : k.qmax;
     double q;
     if (c.EEst == 0.0) {
         q = 1.0 / qmax;
     } else {
-        double q11 = fastpower(c.EEst, k.beta1);
-        q = q11 / fastpower(c.errold, k.beta2);
-        c.q11 = q11;
-        double x = q / k.gamma, lo = 1.0 / qmax, hi = 1.0 / k.qmin;
+        double x;
+        if constexpr (HW) {
+            // errold >= qoldinit = 1e-4, so the divisor is >= 1e-4^0.08 > 0
+            const float q11 = fastpower_hw(c.EEst, k.beta1);
+            c.q11 = q11;
+            x = (double)(q11 * __builtin_amdgcn_rcpf(fastpower_hw(c.errold, k.beta2) * (float)k.gamma));
+        } else {
+            double q11 = fastpower(c.EEst, k.beta1);
+            q = q11 / fastpower(c.errold, k.beta2);
+            c.q11 = q11;
+            x = q / k.gamma;
+        }
+        double lo = 1.0 / qmax, hi = 1.0 / k.qmin;
         q = (x > hi) ? hi : ((x < lo) ? lo : x);
     }
     return q;
 }
 // controllers.jl:823-836 + integrator_utils.jl:1199-1210
+template <bool HW = false>
 __device__ __forceinline__ void accept_controller(Ctl& c, const CtlConst& k, double q, double dtmin) {
     if (k.qsteady_min <= q && q <= k.qsteady_max) q = 1.0;
     c.errold = jl_max(c.EEst, k.qoldinit);
-    double dtnew = c.dt / q;
+    double dtnew;
+    if constexpr (HW) dtnew = c.dt * (double)__builtin_amdgcn_rcpf((float)q);   // q in [1/qmax, 1/qmin]
+    else dtnew = c.dt / q;
     double dp = jl_min(fabs(c.dtmax), fabs(dtnew));
     c.dtpropose = jl_max(fabs(dp), fabs(jl_max(jl_eps(c.t), dtmin)));
 }
@@ -387,6 +426,23 @@ __device__ __forceinline__ double residual_norm(const E* ut, const S* u0, const 
         s = fma(r, r, s);
     }
     return sqrt(s / (double)(N > 1 ? N : 1));
+}
+
+// The mixed engines' RMS error norm: ut = dt * e with the embedded error sum e
+// in FP32, scaled by the FP64 denominators, squared and summed in FP32
+// (hardware reciprocal and square root).  Needs abstol above FP32's range
+// floor; the host rejects abstol < 1e-30 for these engines.
+template <int N>
+__device__ __forceinline__ double residual_norm_hw(const float* e, float dtf, const double* u0, const double* u1,
+                                                   double abstol, double reltol) {
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double den = fma(max_fast(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
+        float r = (dtf * e[i]) * __builtin_amdgcn_rcpf((float)den);
+        s = fmaf(r, r, s);
+    }
+    return (double)__builtin_amdgcn_sqrtf(s * (1.0f / (float)(N > 1 ? N : 1)));
 }
 
 template <int N>
@@ -552,8 +608,89 @@ __device__ void tsit5_prep(const Params& P) {
     P.pre_dt[j] = L.c.dt;
 }
 
+// Mixed engine (K = float): stage inputs are FP32 increments on an FP32 copy of
+// uprev, the embedded error sum is FP32, and only the solution update
+// u = uprev + dt * sum(a7j kj) is formed in FP64 on the FP64 uprev.  On CDNA4
+// FP32 and FP64 VALU ops issue at one rate, so the gain is in the operations
+// dropped (per-stage conversions, the FP64 division and square root of the
+// norm, the controller's exact elementary functions).
+template <typename S, typename K, int N>
+__device__ __forceinline__ void tsit5_perform_step_mixed(Tsit5Lane<S, K, N>& L, const Params& P) {
+    typedef Tsit5Tab T;
+    const double t = L.c.t, dt = L.c.dt;
+    const float dtf = (float)dt;
+    float ub[N], e[N], tmp[N];
+    const float a = (float)(dt * T::a21);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        ub[i] = (float)L.uprev[i];
+        tmp[i] = fmaf(a, L.k1[i], ub[i]);
+        e[i] = (float)T::bt1 * L.k1[i];
+    }
+    odemi_f<float>(L.k2, tmp, L.p, (float)fma(T::c1, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        float acc = (float)T::a31 * L.k1[i];
+        acc = fmaf((float)T::a32, L.k2[i], acc);
+        tmp[i] = fmaf(dtf, acc, ub[i]);
+        e[i] = fmaf((float)T::bt2, L.k2[i], e[i]);
+    }
+    odemi_f<float>(L.k3, tmp, L.p, (float)fma(T::c2, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        float acc = (float)T::a41 * L.k1[i];
+        acc = fmaf((float)T::a42, L.k2[i], acc);
+        acc = fmaf((float)T::a43, L.k3[i], acc);
+        tmp[i] = fmaf(dtf, acc, ub[i]);
+        e[i] = fmaf((float)T::bt3, L.k3[i], e[i]);
+    }
+    odemi_f<float>(L.k4, tmp, L.p, (float)fma(T::c3, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        float acc = (float)T::a51 * L.k1[i];
+        acc = fmaf((float)T::a52, L.k2[i], acc);
+        acc = fmaf((float)T::a53, L.k3[i], acc);
+        acc = fmaf((float)T::a54, L.k4[i], acc);
+        tmp[i] = fmaf(dtf, acc, ub[i]);
+        e[i] = fmaf((float)T::bt4, L.k4[i], e[i]);
+    }
+    odemi_f<float>(L.k5, tmp, L.p, (float)fma(T::c4, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        float acc = (float)T::a61 * L.k1[i];
+        acc = fmaf((float)T::a62, L.k2[i], acc);
+        acc = fmaf((float)T::a63, L.k3[i], acc);
+        acc = fmaf((float)T::a64, L.k4[i], acc);
+        acc = fmaf((float)T::a65, L.k5[i], acc);
+        tmp[i] = fmaf(dtf, acc, ub[i]);
+        e[i] = fmaf((float)T::bt5, L.k5[i], e[i]);
+    }
+    odemi_f<float>(L.k6, tmp, L.p, (float)(t + dt));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        float acc = (float)T::a71 * L.k1[i];
+        acc = fmaf((float)T::a72, L.k2[i], acc);
+        acc = fmaf((float)T::a73, L.k3[i], acc);
+        acc = fmaf((float)T::a74, L.k4[i], acc);
+        acc = fmaf((float)T::a75, L.k5[i], acc);
+        acc = fmaf((float)T::a76, L.k6[i], acc);
+        L.u[i] = fma(dt, (double)acc, L.uprev[i]);
+        e[i] = fmaf((float)T::bt6, L.k6[i], e[i]);
+        tmp[i] = (float)L.u[i];
+    }
+    odemi_f<float>(L.k7, tmp, L.p, (float)(t + dt));
+    L.c.nf += 6;
+#pragma unroll
+    for (int i = 0; i < N; i++) e[i] = fmaf((float)T::bt7, L.k7[i], e[i]);
+    L.c.EEst = residual_norm_hw<N>(e, dtf, L.uprev, L.u, P.abstol, P.reltol);
+}
+
 template <typename S, typename K, int N>
 __device__ __forceinline__ void tsit5_perform_step(Tsit5Lane<S, K, N>& L, const Params& P) {
+    if constexpr (sizeof(K) < sizeof(S)) {
+        tsit5_perform_step_mixed(L, P);
+        return;
+    }
     typedef Tsit5Tab T;
     const double t = L.c.t, dt = L.c.dt;
     K tmp[N];
@@ -680,8 +817,7 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
             c.success_iter++;
 #pragma unroll
             for (int i = 0; i < N; i++) { L.uprev[i] = L.u[i]; L.k1[i] = L.k7[i]; }
-            c.dt = c.dtpropose;
-            modify_dt_for_tstops(c);
+            c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (see above)
         } else {
             reject_controller(c, kc);
         }
@@ -697,14 +833,15 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
         tsit5_perform_step(L, P);
     }
     double ttmp = c.t + c.dt;
-    double q = stepsize_controller(c, kc);
+    constexpr bool HW = sizeof(K) < sizeof(S);   // mixed: hardware-math controller
+    double q = stepsize_controller<HW>(c, kc);
     c.accept_step = (c.EEst <= 1.0);
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;
         if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
         else c.t = ttmp;
-        accept_controller(c, kc, q, P.dtmin);
+        accept_controller<HW>(c, kc, q, P.dtmin);
         if (SAVES && (P.nsaveat > 0 || P.save_everystep)) tsit5_savevalues(L, P, j);
     } else {
         c.nreject++;
@@ -1155,8 +1292,7 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
             c.success_iter++;
 #pragma unroll
             for (int i = 0; i < N; i++) L.uprev[i] = L.u[i];
-            c.dt = c.dtpropose;
-            modify_dt_for_tstops(c);
+            c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (see above)
         } else {
             reject_controller(c, kc);
         }

@@ -712,8 +712,7 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
             if (c.accept_step) {
                 c.success_iter++;
                 for (int i = tid; i < n; i += NT) uprev[i] = u[i];
-                c.dt = c.dtpropose;
-                modify_dt_for_tstops(c);
+                c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (odemi_device.h)
             } else {
                 reject_controller(c, kc);
             }
