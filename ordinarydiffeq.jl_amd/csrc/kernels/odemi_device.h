@@ -90,31 +90,66 @@ struct Params {
 // Julia scalar semantics
 // ---------------------------------------------------------------------------
 
-// Base.eps(::Float64)
+// Base.eps(::Float64), as selects (no divergent branches in the step loop):
+// biased exponent e = 0 -> smallest subnormal, 1..52 -> 2^(e-1) in subnormal
+// bits, 53..2046 -> exponent field e - 52, 2047 (Inf/NaN) -> NaN.
 __device__ __forceinline__ double jl_eps(double x) {
-    u64 b = (u64)__double_as_longlong(x) & 0x7fffffffffffffffULL;
-    int e = (int)(b >> 52);
-    if (e == 0x7ff) return __longlong_as_double(0x7ff8000000000000LL);
-    if (e == 0) return __longlong_as_double(1LL);
-    if (e > 52) return __longlong_as_double((i64)(e - 52) << 52);
-    return __longlong_as_double(1LL << (e - 1));
+    const u64 b = (u64)__double_as_longlong(x) & 0x7fffffffffffffffULL;
+    const int e = (int)(b >> 52);
+    const int sh = (e > 0 && e <= 52) ? e - 1 : 0;
+    u64 r = (e > 52) ? ((u64)(e - 52) << 52) : (1ULL << sh);
+    r = (e == 0x7ff) ? 0x7ff8000000000000ULL : r;
+    return __longlong_as_double((i64)r);
 }
 __device__ __forceinline__ double nextfloat_pos(double x) {  // nextfloat for x >= 0 finite
     return __longlong_as_double(__double_as_longlong(x) + 1);
 }
-// Base.min / Base.max (NaN-propagating, -0.0 < 0.0)
+// Base.min / Base.max (NaN-propagating, -0.0 < 0.0), as selects
 __device__ __forceinline__ double jl_min(double x, double y) {
-    if (isnan(x) || isnan(y)) return __longlong_as_double(0x7ff8000000000000LL);
-    if (x == y) return signbit(x) ? x : y;
-    return x < y ? x : y;
+    const bool take_x = (x < y) || (x == y && signbit(x));
+    const double r = take_x ? x : y;
+    return __builtin_isunordered(x, y) ? __longlong_as_double(0x7ff8000000000000LL) : r;
 }
 __device__ __forceinline__ double jl_max(double x, double y) {
-    if (isnan(x) || isnan(y)) return __longlong_as_double(0x7ff8000000000000LL);
-    if (x == y) return signbit(x) ? y : x;
-    return x > y ? x : y;
+    const bool take_x = (x > y) || (x == y && !signbit(x));
+    const double r = take_x ? x : y;
+    return __builtin_isunordered(x, y) ? __longlong_as_double(0x7ff8000000000000LL) : r;
 }
 template <typename T>
 __device__ __forceinline__ T max_fast(T x, T y) { return y > x ? y : x; }
+
+// FP64 reciprocal, quotient and square root from the hardware estimates
+// (v_rcp_f64, v_rsq_f64) with Newton-Raphson refinement: FP64-accurate to about
+// one ulp but not always the correctly rounded result.  The mixed-precision
+// lane engine's error norm and controller use them (FP64 range and precision
+// without the IEEE division and square-root sequences); the FP64 engines keep
+// IEEE / and sqrt for bit parity with the CPU restatement.  Positive normal
+// operands (tolerance-scaled denominators, step-size ratios).
+__device__ __forceinline__ double rcp_nr(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double div_nr(double a, double b) {
+    const double r = rcp_nr(b);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {   // x >= 0
+    const bool tiny = x < 0x1p-1000;                       // keep the estimate off subnormals
+    const double xs = tiny ? x * 0x1p+1000 : x;
+    const double y = __builtin_amdgcn_rsq(xs);
+    double g = xs * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, xs), h, g);
+    g = fma(fma(-g, g, xs), h, g);
+    g = tiny ? g * 0x1p-500 : g;
+    return (x > 0.0 && x < __builtin_huge_val()) ? g : x;   // 0, inf, NaN pass through
+}
 
 // Deterministic log / exp (fdlibm-style argument reduction + minimax
 // polynomials, ~1 ulp).  Used wherever the controller needs an elementary
@@ -318,8 +353,7 @@ __device__ __forceinline__ CtlConst ctl_const(bool rosenbrock) {
 // dtmax < tf - t, dtmax >= tf - t).  The CPU restatement keeps both calls.
 __device__ __forceinline__ void modify_dt_for_tstops(Ctl& c) {
     double distance = fabs(c.tf - c.t);
-    double tol = 0.0;
-    if (isfinite(c.tf) && isfinite(c.t)) tol = 100.0 * jl_eps(jl_max(fabs(c.t), fabs(c.tf)));
+    const double tol = (isfinite(c.tf) && isfinite(c.t)) ? 100.0 * jl_eps(jl_max(fabs(c.t), fabs(c.tf))) : 0.0;
     double original_dt = fabs(c.dt);
     c.dtpropose = original_dt;
     // the only tstop is tf, so the tstop target is always c.tf
@@ -331,28 +365,26 @@ __device__ __forceinline__ void fix_dt_at_bounds(Ctl& c, double dtmin) {
     c.dt = jl_min(c.dtmax, c.dt);
     c.dt = jl_max(c.dt, fabs(jl_max(jl_eps(c.t), dtmin)));
 }
-// controllers.jl:805-821.  HW: the mixed engines' form (hardware fastpower,
-// reciprocals instead of the two divisions).
+// controllers.jl:805-821.  HW: the mixed engine's form (FastPower's Float32
+// log2/exp2 with the hardware reciprocal and exp2, FP64 quotients by div_nr).
 template <bool HW = false>
 __device__ __forceinline__ double stepsize_controller(Ctl& c, const CtlConst& k) {
-    double qmax = (c.success_iter == 0) ? k.qmax_first : k.qmax;
+    // 1/qmax as a select of two folded constants, not a division per attempt
+    const double lo = (c.success_iter == 0) ? 1.0 / k.qmax_first : 1.0 / k.qmax, hi = 1.0 / k.qmin;
     double q;
     if (c.EEst == 0.0) {
-        q = 1.0 / qmax;
+        q = lo;
     } else {
-        double x;
+        double q11, x;
         if constexpr (HW) {
-            // errold >= qoldinit = 1e-4, so the divisor is >= 1e-4^0.08 > 0
-            const float q11 = fastpower_hw(c.EEst, k.beta1);
-            c.q11 = q11;
-            x = (double)(q11 * __builtin_amdgcn_rcpf(fastpower_hw(c.errold, k.beta2) * (float)k.gamma));
+            q11 = fastpower_hw(c.EEst, k.beta1);
+            x = div_nr(q11, (double)fastpower_hw(c.errold, k.beta2) * k.gamma);
         } else {
-            double q11 = fastpower(c.EEst, k.beta1);
+            q11 = fastpower(c.EEst, k.beta1);
             q = q11 / fastpower(c.errold, k.beta2);
-            c.q11 = q11;
             x = q / k.gamma;
         }
-        double lo = 1.0 / qmax, hi = 1.0 / k.qmin;
+        c.q11 = q11;
         q = (x > hi) ? hi : ((x < lo) ? lo : x);
     }
     return q;
@@ -363,7 +395,7 @@ __device__ __forceinline__ void accept_controller(Ctl& c, const CtlConst& k, dou
     if (k.qsteady_min <= q && q <= k.qsteady_max) q = 1.0;
     c.errold = jl_max(c.EEst, k.qoldinit);
     double dtnew;
-    if constexpr (HW) dtnew = c.dt * (double)__builtin_amdgcn_rcpf((float)q);   // q in [1/qmax, 1/qmin]
+    if constexpr (HW) dtnew = div_nr(c.dt, q);   // q in [1/qmax, 1/qmin]
     else dtnew = c.dt / q;
     double dp = jl_min(fabs(c.dtmax), fabs(dtnew));
     c.dtpropose = jl_max(fabs(dp), fabs(jl_max(jl_eps(c.t), dtmin)));
@@ -428,21 +460,21 @@ __device__ __forceinline__ double residual_norm(const E* ut, const S* u0, const 
     return sqrt(s / (double)(N > 1 ? N : 1));
 }
 
-// The mixed engines' RMS error norm: ut = dt * e with the embedded error sum e
-// in FP32, scaled by the FP64 denominators, squared and summed in FP32
-// (hardware reciprocal and square root).  Needs abstol above FP32's range
-// floor; the host rejects abstol < 1e-30 for these engines.
+// The mixed engine's FP64 error norm: the same sum as residual_norm with the
+// Newton-refined FP64 quotient and square root.
 template <int N>
-__device__ __forceinline__ double residual_norm_hw(const float* e, float dtf, const double* u0, const double* u1,
+__device__ __forceinline__ double residual_norm_nr(const double* ut, const double* u0, const double* u1,
                                                    double abstol, double reltol) {
-    float s = 0.0f;
+    double s = 0.0;
 #pragma unroll
     for (int i = 0; i < N; i++) {
         double den = fma(max_fast(fabs(u0[i]), fabs(u1[i])), reltol, abstol);
-        float r = (dtf * e[i]) * __builtin_amdgcn_rcpf((float)den);
-        s = fmaf(r, r, s);
+        double r;
+        if (__builtin_expect(!(den >= 0x1p-1000), 0)) r = ut[i] / den;   // abstol ~ 0 and u ~ 0
+        else r = div_nr(ut[i], den);
+        s = fma(r, r, s);
     }
-    return (double)__builtin_amdgcn_sqrtf(s * (1.0f / (float)(N > 1 ? N : 1)));
+    return sqrt_nr(s * (1.0 / (double)(N > 1 ? N : 1)));
 }
 
 template <int N>
@@ -609,11 +641,13 @@ __device__ void tsit5_prep(const Params& P) {
 }
 
 // Mixed engine (K = float): stage inputs are FP32 increments on an FP32 copy of
-// uprev, the embedded error sum is FP32, and only the solution update
-// u = uprev + dt * sum(a7j kj) is formed in FP64 on the FP64 uprev.  On CDNA4
-// FP32 and FP64 VALU ops issue at one rate, so the gain is in the operations
-// dropped (per-stage conversions, the FP64 division and square root of the
-// norm, the controller's exact elementary functions).
+// uprev and the embedded error combination sum(btilde_i k_i) is FP32 like the
+// other stage sums; the solution update u = uprev + dt * sum(a7j kj), t + dt,
+// the error norm and the controller stay FP64 (Newton-refined hardware
+// reciprocal and square root instead of the IEEE sequences).  On CDNA4 FP32
+// and FP64 VALU ops issue at one rate, so the gain is in the operations dropped
+// (per-stage conversions, FastPower's exp2 through the exact FP64 det_exp, the
+// IEEE division and square-root fix-up sequences).
 template <typename S, typename K, int N>
 __device__ __forceinline__ void tsit5_perform_step_mixed(Tsit5Lane<S, K, N>& L, const Params& P) {
     typedef Tsit5Tab T;
@@ -680,9 +714,10 @@ __device__ __forceinline__ void tsit5_perform_step_mixed(Tsit5Lane<S, K, N>& L, 
     }
     odemi_f<float>(L.k7, tmp, L.p, (float)(t + dt));
     L.c.nf += 6;
+    double ea[N];   // utilde = dt * e, then the FP64 norm
 #pragma unroll
-    for (int i = 0; i < N; i++) e[i] = fmaf((float)T::bt7, L.k7[i], e[i]);
-    L.c.EEst = residual_norm_hw<N>(e, dtf, L.uprev, L.u, P.abstol, P.reltol);
+    for (int i = 0; i < N; i++) ea[i] = dt * (double)fmaf((float)T::bt7, L.k7[i], e[i]);
+    L.c.EEst = residual_norm_nr<N>(ea, L.uprev, L.u, P.abstol, P.reltol);
 }
 
 template <typename S, typename K, int N>
@@ -833,7 +868,7 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
         tsit5_perform_step(L, P);
     }
     double ttmp = c.t + c.dt;
-    constexpr bool HW = sizeof(K) < sizeof(S);   // mixed: hardware-math controller
+    constexpr bool HW = sizeof(K) < sizeof(S);   // mixed: hardware FastPower, Newton-refined FP64 quotients
     double q = stepsize_controller<HW>(c, kc);
     c.accept_step = (c.EEst <= 1.0);
     if (c.accept_step) {

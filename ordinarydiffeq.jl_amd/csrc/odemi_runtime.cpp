@@ -516,9 +516,6 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         if (!plain) fn_prep = rhs->rodas_prep;
     }
     if (plain && mixed) { ctx->err = "the plain engine is FP64 only"; return -3; }
-    if (mixed && a->alg == ODEMI_TSIT5 && !(a->abstol >= 1e-30)) {
-        ctx->err = "the mixed Tsit5 engine forms its error norm in FP32: abstol must be >= 1e-30"; return -2;
-    }
     if (!fn_main || (plain && !fn_init)) { ctx->err = "requested kernel was not compiled for this problem"; return -3; }
     if (a->nvec && !use_wg) { ctx->err = "uneven batches (nvec) need the workgroup Rodas5P kernels"; return -3; }
 
@@ -782,10 +779,6 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     const bool wg = (rhs->flags & ODEMI_COMPILE_WG) && opts->alg == ODEMI_RODAS5P;
     if (opts->alg < 0 || opts->alg > 1 || !(wg ? rhs->wg_serve_tick[pi] : rhs->serve_tick[opts->alg][pi])) {
         ctx->err = "the compiled problem has no serving kernels for this algorithm";
-        return nullptr;
-    }
-    if (pi == 1 && opts->alg == ODEMI_TSIT5 && !(opts->abstol >= 1e-30)) {
-        ctx->err = "the mixed Tsit5 engine forms its error norm in FP32: abstol must be >= 1e-30";
         return nullptr;
     }
     (void)hipSetDevice(ctx->device);
