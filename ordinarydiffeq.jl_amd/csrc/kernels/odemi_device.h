@@ -1044,7 +1044,9 @@ __device__ void tsit5_plain_step(const Params& P) {
 // Dense LU on a lane-resident matrix (compile-time N, branchless row swaps).
 // Textbook right-looking LU with partial pivoting (first maximum on ties),
 // multipliers scaled by the reciprocal pivot, trailing update a -= l*u as fma,
-// triangular solves in LAPACK's column order.
+// triangular solves in LAPACK's column order.  In FP32 (the factorisation that
+// FP64 refinement corrects) the pivot reciprocal and the back substitution use
+// the hardware reciprocal instead of IEEE division.
 // ---------------------------------------------------------------------------
 template <typename T, int N>
 __device__ __forceinline__ void lu_factor_reg(T (&A)[N][N], int (&piv)[N]) {
@@ -1068,7 +1070,9 @@ __device__ __forceinline__ void lu_factor_reg(T (&A)[N][N], int (&piv)[N]) {
                 A[i][c] = s ? a : b;
             }
         }
-        T inv = (T)1 / A[k][k];
+        T inv;
+        if constexpr (sizeof(T) == sizeof(float)) inv = __builtin_amdgcn_rcpf(A[k][k]);
+        else inv = (T)1 / A[k][k];
 #pragma unroll
         for (int i = k + 1; i < N; i++) {
             T l = A[i][k] * inv;
@@ -1100,7 +1104,8 @@ __device__ __forceinline__ void lu_solve_reg(const T (&A)[N][N], const int (&piv
     // back substitution in column order (LAPACK dtrsv 'U','N','N')
 #pragma unroll
     for (int c = N - 1; c >= 0; c--) {
-        b[c] = b[c] / A[c][c];
+        if constexpr (sizeof(T) == sizeof(float)) b[c] = b[c] * __builtin_amdgcn_rcpf(A[c][c]);
+        else b[c] = b[c] / A[c][c];
 #pragma unroll
         for (int i = 0; i < c; i++) b[i] = fma(-A[i][c], b[c], b[i]);
     }
@@ -1194,9 +1199,10 @@ __device__ __forceinline__ void rodas_wsolve(const double (&W)[N][N], const F (&
 
 template <typename F, int N>
 __device__ __forceinline__ void rodas_perform_step(RodasLane<F, N>& L, const Params& P) {
+    constexpr bool HW = sizeof(F) < sizeof(double);   // mixed: Newton-refined FP64 reciprocals and norm
     Ctl& c = L.c;
     const double t = c.t, dt = c.dt;
-    const double idt = 1.0 / dt;
+    const double idt = HW ? rcp_nr(dt) : 1.0 / dt;
     const double dtgamma = dt * RodasTab::gamma;
     double fsal[N], du[N], lt[N];
     odemi_f<double>(fsal, L.uprev, L.p, t);
@@ -1208,7 +1214,7 @@ __device__ __forceinline__ void rodas_perform_step(RodasLane<F, N>& L, const Par
         c.njacs += 1;
     }
     // jacobian2W! : W = J - I/(dt*gamma)
-    const double invdtgamma = 1.0 / dtgamma;
+    const double invdtgamma = HW ? rcp_nr(dtgamma) : 1.0 / dtgamma;
     double W[N][N];
     F LU[N][N];
     int piv[N];
@@ -1273,7 +1279,8 @@ __device__ __forceinline__ void rodas_perform_step(RodasLane<F, N>& L, const Par
         acc = fma(1.0, L.ks[7][i], acc);
         du[i] = acc;
     }
-    c.EEst = residual_norm<N>(du, L.uprev, L.u, P.abstol, P.reltol);
+    if constexpr (HW) c.EEst = residual_norm_nr<N>(du, L.uprev, L.u, P.abstol, P.reltol);
+    else c.EEst = residual_norm<N>(du, L.uprev, L.u, P.abstol, P.reltol);
     if (P.nsaveat > 0) {
 #pragma unroll
         for (int r = 0; r < 3; r++)
@@ -1343,14 +1350,15 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
         rodas_perform_step(L, P);
     }
     double ttmp = c.t + c.dt;
-    double q = stepsize_controller(c, kc);
+    constexpr bool HW = sizeof(F) < sizeof(double);   // mixed: hardware FastPower, Newton-refined FP64 quotients
+    double q = stepsize_controller<HW>(c, kc);
     c.accept_step = (c.EEst <= 1.0);
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;
         if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
         else c.t = ttmp;
-        accept_controller(c, kc, q, P.dtmin);
+        accept_controller<HW>(c, kc, q, P.dtmin);
         if (P.nsaveat > 0 || P.save_everystep) rodas_savevalues(L, P, j);
     } else {
         c.nreject++;
