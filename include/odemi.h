@@ -103,7 +103,7 @@ typedef struct {
     int waves_per_traj;       /* workgroup kernels: waves cooperating on one trajectory (0: auto) */
     int sort_by_cost;         /* 1: hand out trajectories longest-span first */
     void* stream;             /* hipStream_t to run on, or NULL for the ctx stream */
-    double kernel_ms;         /* out: device time of the solve kernels (HIP events) */
+    double kernel_ms;         /* out: device time (HIP events) of the persistent kernel, or of all plain-engine launches */
     int64_t launches;         /* out: kernel launches issued */
 } odemi_solve_args;
 
