@@ -378,7 +378,7 @@ def main():
         "metric": "ensemble trajectories/s at reltol 1e-6 (Tsit5, Lorenz), batched persistent mixed-precision engine",
         "value": value, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 stages + f64 state/t/error/controller", "data": "synthetic (seeded PCG64)",
+        "dtype": "f32 stages and stage sums + f64 state/t/error norm/controller", "data": "synthetic (seeded PCG64)",
         "config": {"workload": "Lorenz ensemble, 1,048,576 trajectories, Tsit5, rho~U[0,28], sigma~U[8,12], "
                                "u0~U[-1,1]^3, t_end~U[1,10], reltol=abstol=1e-6, final state only",
                    "trajectories": args.N, "parallelism": "one MI355X, lane per trajectory, work queue"},
