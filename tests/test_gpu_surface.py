@@ -103,6 +103,29 @@ def test_return_codes():
     assert sol.retcode == odemi.ReturnCode.DtNaN
 
 
+@pytest.mark.parametrize("alg,prec", [(odemi.Tsit5(), "mixed"), (odemi.Tsit5(), "fp64"), (odemi.Rodas5P(), "fp64")],
+                         ids=["tsit5-mixed", "tsit5-fp64", "rodas5p-fp64"])
+def test_work_order_is_a_permutation(alg, prec):
+    """The longest-first work order (a counting sort over cost bins) hands out
+    every trajectory exactly once: results equal the identity order's for a
+    batch that is not a multiple of the sort's 4,096-trajectory tile, with
+    zero-length spans and a NaN end time (cost bin 0) mixed in."""
+    N = 4096 + 777
+    d = P.lorenz_large(N=N)
+    d["tf"][::97] = d["t0"][::97]
+    d["tf"][5] = float("nan")
+    print(f"[case] work order permutation {alg.code} {prec} N={N}", flush=True)
+    outs = []
+    for srt in (True, False):
+        ea = odemi.EnsembleMI355X(precision=prec, sort_by_cost=srt)
+        outs.append(odemi.solve_arrays(P.LORENZ, alg, d["u0"], d["p"], d["t0"], d["tf"], ea, reltol=1e-6,
+                                       abstol=1e-6))
+    a, b = outs
+    assert np.array_equal(a["u_final"], b["u_final"], equal_nan=True)
+    assert np.array_equal(a["stats"], b["stats"]) and np.array_equal(a["retcode"], b["retcode"])
+    assert (a["retcode"] == 1).all() and (a["stats"][1] > 0).sum() == N - len(range(0, N, 97)) - 1
+
+
 def test_stats_and_user_dt_match_oracle():
     print("[case] stats + user dt", flush=True)
     d = P.lorenz_small(N=128)
