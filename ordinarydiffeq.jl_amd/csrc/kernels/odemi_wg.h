@@ -300,8 +300,12 @@ __device__ void wg_wsolve(const WGPtrs& S, int n, double* x, double* rb, double*
 // ---------------------------------------------------------------------------
 #if ODEMI_N <= 128
 #define ODEMI_GJ 1
-#define ODEMI_GJ_R ((ODEMI_N + 15) / 16)
-#define ODEMI_GJ_NP (16 * ODEMI_GJ_R)
+#define ODEMI_GJ_R ((ODEMI_N + 15) / 16)   // tile of the largest state (nmax)
+// Tile sizes R = ceil(n/16) present in the batch (bit R-1); the problem source
+// may define it from its state sizes, else every R up to ODEMI_GJ_R.
+#ifndef ODEMI_GJ_RMASK
+#define ODEMI_GJ_RMASK 0xff
+#endif
 
 // Sum over the 16 lanes of each DPP row; the total lands in the row's lane 15.
 // Fixed addition order (inclusive Hillis-Steele scan), so results are
@@ -324,18 +328,22 @@ __device__ __forceinline__ unsigned dpp_wave_max_u32(unsigned x) {
     return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
 }
 
+// R x R tile per thread; a trajectory of n states uses R = ceil(n / 16)
+// (16 R >= n; rows and columns in [n, 16 R) are identity padding).
+template <int R>
 struct GJTile {
-    float T[ODEMI_GJ_R][ODEMI_GJ_R];
+    float T[R][R];
 };
 
 // T <- W = J - sigma I for the n x n leading block of J (FP32, row stride ldj),
 // identity outside it.  The diagonal shift is formed in FP64 and rounded once.
-__device__ __forceinline__ void gj_load(GJTile& G, const float* J, int ldj, int n, double sigma) {
+template <int R>
+__device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, int n, double sigma) {
     const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
 #pragma unroll
-    for (int a = 0; a < ODEMI_GJ_R; a++)
+    for (int a = 0; a < R; a++)
 #pragma unroll
-        for (int b = 0; b < ODEMI_GJ_R; b++) {
+        for (int b = 0; b < R; b++) {
             const int i = ti + 16 * a, j = tj + 16 * b;
             float v = (i == j) ? 1.0f : 0.0f;
             if (i < n && j < n) {
@@ -373,10 +381,12 @@ __device__ __forceinline__ unsigned gj_key(float v, int i) {
     const float a = fabsf(v);
     return ((a != a) ? 0u : (__float_as_uint(a) & ~127u)) | (unsigned)(127 - i);
 }
-__device__ __forceinline__ int gj_cidx(int i) { return (i & 15) * ODEMI_GJ_R + (i >> 4); }
+template <int R>
+__device__ __forceinline__ int gj_cidx(int i) { return (i & 15) * R + (i >> 4); }
 
-__device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbuf) {
-    constexpr int R = ODEMI_GJ_R, NP = ODEMI_GJ_NP, NQ = (NP + 63) / 64;
+template <int R>
+__device__ __forceinline__ void gj_invert(GJTile<R>& G, int n, int* pk, float* gjbuf) {
+    constexpr int NP = 16 * R, NQ = (NP + 63) / 64;
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15, lane = tid & 63;
     float* cbuf = gjbuf;
     float* nbuf = gjbuf + 2 * NP;
@@ -399,7 +409,7 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             const int i = lane + 64 * q;
-            if (i < n) key = max(key, gj_key(cbuf[gj_cidx(i)], i));
+            if (i < n) key = max(key, gj_key(cbuf[gj_cidx<R>(i)], i));
         }
         key = dpp_wave_max_u32(key);
         p = 127 - (int)(key & 127u);
@@ -431,7 +441,7 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
             // all reads of the step first (the search for p_{k+1} only needs
             // these, so its latency overlaps the tile update).  At the last step
             // k1 = n <= NP: the reads stay inside the buffers and are unused.
-            const float piv = cb[gj_cidx(p)];
+            const float piv = cb[gj_cidx<R>(p)];
             float c[R], r[R], cq[NQ], nq[NQ];
 #pragma unroll
             for (int a = 0; a < R; a++) c[a] = cb[ti * R + a];
@@ -440,7 +450,7 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
             const float r1 = st[(pa * 16 + (k1 & 15)) * R + (k1 >> 4)];
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                const int ci = gj_cidx(min(lane + 64 * q, NP - 1));
+                const int ci = gj_cidx<R>(min(lane + 64 * q, NP - 1));
                 cq[q] = cb[ci];
                 nq[q] = nb[ci];
             }
@@ -510,8 +520,8 @@ __device__ __forceinline__ void gj_invert(GJTile& G, int n, int* pk, float* gjbu
 }
 
 // x <- W^{-1} x (x FP64 in LDS, accumulated in FP32).  All threads must call.
-__device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, double* x, float* zbuf) {
-    constexpr int R = ODEMI_GJ_R;
+template <int R>
+__device__ __forceinline__ void gj_solve(const GJTile<R>& G, int n, const int* pk, double* x, float* zbuf) {
     const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
     float bp[R];
 #pragma unroll
@@ -533,19 +543,22 @@ __device__ __forceinline__ void gj_solve(const GJTile& G, int n, const int* pk, 
 }
 
 // Stage solve W x = b for the Gauss-Jordan variant (b in x on entry), with the
-// same FP64 refinement sweeps as wg_wsolve.
-__device__ __forceinline__ void wg_wsolve_gj(const GJTile& G, const WGPtrs& S, int n, double* x, double* rb, double* r, double* d,
+// same FP64 refinement sweeps as wg_wsolve.  The buffers of gj_invert<R> sit at
+// the front of gjbuf (sized for the largest R), zbuf after them.
+template <int R>
+__device__ __forceinline__ void wg_wsolve_gj(const GJTile<R>& G, const WGPtrs& S, int n, double* x, double* rb, double* r, double* d,
                              const double* uprev, const double* p, double t, double sigma, int sweeps) {
     const int tid = threadIdx.x, NT = blockDim.x;
+    float* zbuf = S.gjbuf + 5 * 16 * R + 2 * R * 16 * R;
     for (int i = tid; i < n; i += NT) rb[i] = x[i];
     __syncthreads();
-    gj_solve(G, n, S.perm, x, S.gjbuf + 5 * ODEMI_GJ_NP + 2 * ODEMI_GJ_R * ODEMI_GJ_NP);
+    gj_solve<R>(G, n, S.perm, x, zbuf);
     for (int sw = 0; sw < sweeps; sw++) {
         odemi_jvp_wg<double>(r, x, uprev, p, t, n, tid, NT);
         __syncthreads();
         for (int i = tid; i < n; i += NT) d[i] = fma(sigma, x[i], rb[i] - r[i]);
         __syncthreads();
-        gj_solve(G, n, S.perm, d, S.gjbuf + 5 * ODEMI_GJ_NP + 2 * ODEMI_GJ_R * ODEMI_GJ_NP);
+        gj_solve<R>(G, n, S.perm, d, zbuf);
         for (int i = tid; i < n; i += NT) x[i] = x[i] + d[i];
         __syncthreads();
     }
@@ -683,7 +696,8 @@ __device__ __forceinline__ void wg_begin(const WGPtrs& S, const Params& P, i64 j
 
 // Step attempts until the trajectory ends (returns its ReturnCode) or `limit`
 // attempts have been made (returns 0; the state is in c and S.u/uprev/dT).
-template <typename F, bool SAVES, bool GJ>
+// R: Gauss-Jordan tile (GJ variant only; 16 R >= n).
+template <typename F, bool SAVES, bool GJ, int R = 1>
 __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double& tlast,
                                       float* Jg, int limit) {
     const int tid = threadIdx.x, NT = blockDim.x;
@@ -702,7 +716,7 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
     F* W = (F*)S.W;
     (void)Jg; (void)W; (void)ld; (void)pf; (void)dense;
 #if ODEMI_GJ
-    GJTile G;
+    GJTile<R> G;
 #endif
     int rc = RC_SUCCESS;
     for (int att = 0; c.t < c.tf; att++) {
@@ -769,9 +783,9 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
             // rb = fsal, r = du, d = u are free during the stage-1 solve
             if constexpr (GJ) {
 #if ODEMI_GJ
-                gj_load(G, Jg, nmax, n, sigma);
-                gj_invert(G, n, S.perm, S.gjbuf);
-                wg_wsolve_gj(G, S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                gj_load<R>(G, Jg, nmax, n, sigma);
+                gj_invert<R>(G, n, S.perm, S.gjbuf);
+                wg_wsolve_gj<R>(G, S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
 #endif
             } else {
                 wg_lu<F>(W, ld, n, S.piv, S.perm, red, S.ired);
@@ -800,7 +814,7 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
                 __syncthreads();
                 if constexpr (GJ) {
 #if ODEMI_GJ
-                    wg_wsolve_gj(G, S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
+                    wg_wsolve_gj<R>(G, S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
 #endif
                 } else {
                     wg_wsolve<F>(S, n, x, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
@@ -915,6 +929,27 @@ __device__ __forceinline__ void wg_finish(const WGPtrs& S, const Params& P, i64 
     }
 }
 
+// wg_run with the Gauss-Jordan tile of this trajectory, R = ceil(n / 16), so a
+// short chain in a batch padded to nmax does not pay for the largest tile
+// (measured: n <= 64 chains 1.55x faster at R = 4 than padded to R = 8).
+// Only the tile sizes in ODEMI_GJ_RMASK are instantiated; the largest always.
+template <typename F, bool SAVES, bool GJ, int RR = 1>
+__device__ __forceinline__ int wg_run_tiled(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double& tlast,
+                                            float* Jg, int limit) {
+#if ODEMI_GJ
+    if constexpr (GJ) {
+        if constexpr (RR < ODEMI_GJ_R) {
+            if constexpr (((ODEMI_GJ_RMASK >> (RR - 1)) & 1) != 0)
+                if (((n + 15) >> 4) == RR) return wg_run<F, SAVES, GJ, RR>(S, P, j, n, c, tlast, Jg, limit);
+            return wg_run_tiled<F, SAVES, GJ, RR + 1>(S, P, j, n, c, tlast, Jg, limit);
+        } else {
+            return wg_run<F, SAVES, GJ, ODEMI_GJ_R>(S, P, j, n, c, tlast, Jg, limit);
+        }
+    }
+#endif
+    return wg_run<F, SAVES, GJ>(S, P, j, n, c, tlast, Jg, limit);
+}
+
 template <typename F, bool SAVES, bool GJ>
 __device__ __forceinline__ void rodas_wg_impl(const Params& P) {
     static_assert(!GJ || (sizeof(F) == 4 && ODEMI_GJ), "Gauss-Jordan variant: FP32, n <= 128");
@@ -936,7 +971,7 @@ __device__ __forceinline__ void rodas_wg_impl(const Params& P) {
         Ctl c;
         double tlast;
         wg_begin<GJ>(S, P, j, n, c, tlast, Jg);
-        const int rc = wg_run<F, SAVES, GJ>(S, P, j, n, c, tlast, Jg, 0x7fffffff);
+        const int rc = wg_run_tiled<F, SAVES, GJ>(S, P, j, n, c, tlast, Jg, 0x7fffffff);
         wg_finish<SAVES>(S, P, j, n, c, tlast, rc);
         __syncthreads();
     }
@@ -1020,7 +1055,7 @@ __device__ __forceinline__ void rodas_wg_serve_tick(const Params& P) {
 #endif
         __syncthreads();
         double tlast = c.t;
-        const int rc = wg_run<F, false, GJ>(S, P, j, n, c, tlast, Jg, P.tick_attempts);
+        const int rc = wg_run_tiled<F, false, GJ>(S, P, j, n, c, tlast, Jg, P.tick_attempts);
         if (rc) {
             wg_finish<false>(S, P, j, n, c, tlast, rc);
             if (tid == 0) {

@@ -607,7 +607,12 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None):
             _CUR = None
     hdr = "template <typename T>\n__device__ __noinline__ void "
     args = "const T* __restrict__ u, const T* __restrict__ p, T t, int n, int tid, int nt"
-    src = "\n".join([
+    # Gauss-Jordan tile sizes R = ceil(n/16) the kernel instantiates (bit R-1)
+    rmask = 0
+    for n in ns:
+        if n <= 128:
+            rmask |= 1 << ((n + 15) // 16 - 1)
+    src = f"#define ODEMI_GJ_RMASK 0x{rmask:x}\n" + "\n".join([
         hdr + f"odemi_f_wg(T* __restrict__ du, {args}) {{\n    (void)u; (void)p; (void)t;\n    switch (n) {{\n"
         + "\n".join(fcases) + "\n    default: break;\n    }\n}",
         hdr + f"odemi_jac_wg(T* __restrict__ J, int ld, {args}) {{\n    (void)u; (void)p; (void)t;\n    switch (n) {{\n"
