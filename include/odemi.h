@@ -100,7 +100,7 @@ typedef struct {
     int32_t* stats;           /* [6][N]: nf, naccept, nreject, njacs, nw, nsolve (may be NULL) */
     int refine_sweeps;        /* ODEMI_MIXED Rodas5P: FP64 refinement sweeps per stage solve (default 1) */
     int block;                /* threads per workgroup (0: engine default) */
-    int waves_per_traj;       /* workgroup kernels: waves cooperating on one trajectory (0: auto) */
+    int blocks_per_cu;        /* persistent kernels: resident workgroups per CU (0: library default; capped by occupancy) */
     int sort_by_cost;         /* 1: hand out trajectories longest-span first */
     void* stream;             /* hipStream_t to run on, or NULL for the ctx stream */
     double kernel_ms;         /* out: device time (HIP events) of the persistent kernel, or of all plain-engine launches */

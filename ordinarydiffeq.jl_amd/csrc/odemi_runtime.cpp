@@ -602,13 +602,15 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         }
         int nb = 0;
         HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, threads, shmem));
-        // Lane kernels: at most 4 resident workgroups per CU (16 waves).  The
-        // kernels are VALU-bound at that occupancy, and fewer lanes give each
-        // lane more trajectories, so the longest-first tail is shorter
-        // (Lorenz 1M mixed: 5 per CU 2.44 ms, 4 per CU 2.37 ms, 3 per CU
-        // 2.42 ms).  ODEMI_BLOCKS_PER_CU overrides the cap.
+        // Resident workgroups per CU: args.blocks_per_cu (the autotuner's
+        // choice), else $ODEMI_BLOCKS_PER_CU, else for the lane kernels at most
+        // 4 (16 waves): they are VALU-bound at that occupancy, and fewer lanes
+        // give each lane more trajectories, so the longest-first tail is
+        // shorter (Lorenz 1M mixed: 5 per CU 2.44 ms, 4 per CU 2.37 ms, 3 per
+        // CU 2.42 ms).  Never above what occupancy allows.
         const char* bpc = getenv("ODEMI_BLOCKS_PER_CU");
-        if (bpc && atoi(bpc) > 0) nb = std::min(nb, atoi(bpc));
+        if (a->blocks_per_cu > 0) nb = std::min(nb, a->blocks_per_cu);
+        else if (bpc && atoi(bpc) > 0) nb = std::min(nb, atoi(bpc));
         else if (!use_wg) nb = std::min(nb, 4);
         if (nb < 1) nb = 1;
         grid = (int64_t)nb * std::max(ctx->cu_count, 1);

@@ -62,7 +62,7 @@ mutable struct OdemiSolveArgs
     max_saves::Cint
     t_out::Ptr{Float64}; u_out::Ptr{Float64}; nsaved::Ptr{Int32}
     u_final::Ptr{Float64}; t_final::Ptr{Float64}; retcode::Ptr{Int32}; stats::Ptr{Int32}
-    refine_sweeps::Cint; block::Cint; waves_per_traj::Cint; sort_by_cost::Cint
+    refine_sweeps::Cint; block::Cint; blocks_per_cu::Cint; sort_by_cost::Cint
     stream::Ptr{Cvoid}
     kernel_ms::Float64; launches::Int64
     OdemiSolveArgs() = new()

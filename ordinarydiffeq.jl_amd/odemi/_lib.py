@@ -33,7 +33,7 @@ class SolveArgs(ctypes.Structure):
         ("max_saves", c_i),
         ("t_out", c_vp), ("u_out", c_vp), ("nsaved", c_vp), ("u_final", c_vp), ("t_final", c_vp),
         ("retcode", c_vp), ("stats", c_vp),
-        ("refine_sweeps", c_i), ("block", c_i), ("waves_per_traj", c_i), ("sort_by_cost", c_i),
+        ("refine_sweeps", c_i), ("block", c_i), ("blocks_per_cu", c_i), ("sort_by_cost", c_i),
         ("stream", c_vp),
         ("kernel_ms", c_d), ("launches", c_i64),
     ]

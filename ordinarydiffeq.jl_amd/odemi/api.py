@@ -91,7 +91,7 @@ class EnsembleMI355X:
     """
 
     def __init__(self, device=0, engine="persistent", precision="fp64", refine_sweeps=1, sort_by_cost=True,
-                 block=0):
+                 block=0, blocks_per_cu=0, autotune=False):
         if engine not in ("persistent", "plain"):
             raise ValueError("engine must be 'persistent' or 'plain'")
         if precision not in ("fp64", "mixed"):
@@ -100,6 +100,32 @@ class EnsembleMI355X:
             raise ValueError("the plain engine is FP64 only")
         self.device, self.engine, self.precision = device, engine, precision
         self.refine_sweeps, self.sort_by_cost, self.block = refine_sweeps, sort_by_cost, block
+        self.blocks_per_cu, self.autotune = int(blocks_per_cu), bool(autotune)
+
+
+# Autotuner choices, per context: (problem handle, alg, precision, N, nmax, save
+# mode) -> (blocks_per_cu, {candidate: kernel ms}).
+_TUNED = {}
+
+
+def _autotune(ctx, h, a, key, wg):
+    """Pick the resident workgroups per CU of the persistent kernel (the number
+    of lanes, or of concurrent trajectories for the workgroup kernels) by timing
+    each candidate on the caller's own batch; results do not depend on it.  Runs
+    once per key; the outputs of the last run are overwritten by the caller's."""
+    hit = _TUNED.get(key)
+    if hit is not None:
+        return hit[0]
+    cands = (1, 2, 3, 4) if wg else (2, 3, 4, 5, 6, 8)
+    times = {}
+    for b in cands:
+        a.blocks_per_cu = b
+        ctx.solve(h, a)           # first run of a candidate also warms the module
+        ctx.solve(h, a)
+        times[b] = a.kernel_ms
+    best = min(times, key=times.get)
+    _TUNED[key] = (best, times)
+    return best
 
 
 class EnsembleContext:
@@ -291,6 +317,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
     a.refine_sweeps = ea.refine_sweeps
     a.sort_by_cost = int(ea.sort_by_cost)
     a.block = ea.block
+    a.blocks_per_cu = ea.blocks_per_cu
     a.N, a.nmax, a.np = N, n, np_
     a.reltol, a.abstol = reltol, abstol
     a.dtmax = 0.0 if dtmax is None else float(dtmax)
@@ -347,6 +374,10 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
     a.saveat, a.nsaveat = ptr(sat), len(sa)
     a.t_out, a.u_out, a.nsaved = ptr(o["t_out"]), ptr(o["u_out"]), ptr(o["nsaved"])
     a.u_final, a.t_final, a.retcode, a.stats = ptr(o["u_final"]), ptr(o["t_final"]), ptr(o["retcode"]), ptr(o["stats"])
+    if ea.autotune and ea.engine != "plain" and not ea.blocks_per_cu:
+        key = (id(ctx), int(h), alg.code, ea.precision, N, n, int(save_everystep), len(sa), ns is not None)
+        a.blocks_per_cu = _autotune(ctx, h, a, key, wg=(alg.code == _lib.RODAS5P and (ns is not None or n > LANE_MAX_RODAS)))
+        o["tuning"] = dict(zip(("blocks_per_cu", "kernel_ms"), _TUNED[key]))
     t_s = time.perf_counter()
     ctx.solve(h, a)
     o["wall_s"] = time.perf_counter() - t_s

@@ -126,6 +126,24 @@ def test_work_order_is_a_permutation(alg, prec):
     assert (a["retcode"] == 1).all() and (a["stats"][1] > 0).sum() == N - len(range(0, N, 97)) - 1
 
 
+def test_autotune_lanes_keeps_results():
+    """The autotuner times the resident workgroups per CU on the caller's batch and
+    keeps the fastest; the launch shape does not change any result."""
+    d = P.lorenz_large(N=20000)
+    print("[case] autotune blocks_per_cu", flush=True)
+    base = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"],
+                              odemi.EnsembleMI355X(precision="mixed"), reltol=1e-6, abstol=1e-6)
+    tuned = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"],
+                               odemi.EnsembleMI355X(precision="mixed", autotune=True), reltol=1e-6, abstol=1e-6)
+    best, times = tuned["tuning"]["blocks_per_cu"], tuned["tuning"]["kernel_ms"]
+    print(f"autotune: best {best} per CU, kernel ms {times}")
+    assert best in times and times[best] == min(times.values())
+    assert np.array_equal(base["u_final"], tuned["u_final"]) and np.array_equal(base["stats"], tuned["stats"])
+    fixed = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"],
+                               odemi.EnsembleMI355X(precision="mixed", blocks_per_cu=2), reltol=1e-6, abstol=1e-6)
+    assert np.array_equal(base["u_final"], fixed["u_final"])
+
+
 def test_stats_and_user_dt_match_oracle():
     print("[case] stats + user dt", flush=True)
     d = P.lorenz_small(N=128)
