@@ -400,9 +400,12 @@ __device__ __forceinline__ void accept_controller(Ctl& c, const CtlConst& k, dou
     double dp = jl_min(fabs(c.dtmax), fabs(dtnew));
     c.dtpropose = jl_max(fabs(dp), fabs(jl_max(jl_eps(c.t), dtmin)));
 }
-// controllers.jl:838-843
+// controllers.jl:838-843 (HW: the mixed engines' Newton-refined FP64 quotients;
+// a wave runs this whenever any of its lanes rejects, i.e. nearly every attempt)
+template <bool HW = false>
 __device__ __forceinline__ void reject_controller(Ctl& c, const CtlConst& k) {
-    c.dt = c.dt / jl_min(1.0 / k.qmin, c.q11 / k.gamma);
+    if constexpr (HW) c.dt = div_nr(c.dt, jl_min(1.0 / k.qmin, div_nr(c.q11, k.gamma)));
+    else c.dt = c.dt / jl_min(1.0 / k.qmin, c.q11 / k.gamma);
 }
 
 template <typename S, int N>
@@ -854,7 +857,7 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
             for (int i = 0; i < N; i++) { L.uprev[i] = L.u[i]; L.k1[i] = L.k7[i]; }
             c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (see above)
         } else {
-            reject_controller(c, kc);
+            reject_controller<(sizeof(K) < sizeof(S))>(c, kc);
         }
     }
     c.iter++;
@@ -1336,7 +1339,7 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
             for (int i = 0; i < N; i++) L.uprev[i] = L.u[i];
             c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (see above)
         } else {
-            reject_controller(c, kc);
+            reject_controller<(sizeof(F) < sizeof(double))>(c, kc);
         }
     }
     c.iter++;
