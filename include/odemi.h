@@ -88,6 +88,9 @@ typedef struct {
     int64_t maxiters;         /* 1_000_000 (solve.jl:161) */
     const double* saveat;     /* sorted output times shared by all trajectories, or NULL */
     int nsaveat;
+    const double* tstops;     /* sorted user stopping times shared by all trajectories, or NULL
+                                 (solve.jl:440-446 `tstops`; each trajectory keeps those in (t0, tf)) */
+    int ntstops;
     int save_start, save_end, save_everystep;
     int save_end_user;        /* -1: `save_end = nothing`, 0/1: user Bool */
     int max_saves;            /* capacity of t_out/u_out per trajectory */

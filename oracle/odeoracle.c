@@ -372,7 +372,12 @@ typedef struct {
     /* saving */
     double* t_out; double* u_out; int max_out;
     double* saveat_int; int n_saveat, saveat_pos;
+    /* tstops queue (initialize_tstops, solve.jl:988-1030): user times strictly
+     * inside (t0, tf), sorted, then tf; [tst_pos, n_tst) are still queued */
+    double* tst; int n_tst, tst_pos;
 } integ_t;
+
+static inline double first_tstop(const integ_t* I) { return I->tst[I->tst_pos]; }
 
 static void save_point(integ_t* I, double t, const double* v) {
     int s = I->res->nsaved;
@@ -393,7 +398,7 @@ static inline void F(integ_t* I, double* du, const double* u, double t) {
 static double initdt(integ_t* I) {
     const int n = I->n;
     const double t = I->t;
-    const double dtmax_tdir = jl_min(fabs(I->dtmax), fabs(I->tf - t));
+    const double dtmax_tdir = jl_min(fabs(I->dtmax), fabs(first_tstop(I) - t));
     const double dtmin = nextafter(jl_max(I->dtmin, oo_jl_eps(t)), INFINITY);
     const double smalldt = jl_max(dtmin, 1e-6);
     if (n == 0) return jl_max(smalldt, dtmin);
@@ -428,15 +433,17 @@ static double initdt(integ_t* I) {
     return jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
 }
 
-/* integrator_utils.jl:268-324 (adaptive branch, tstops = [tf], tdir = +1) */
+/* integrator_utils.jl:268-324 (adaptive and dtchangeable branches, tdir = +1);
+ * the queue is never empty inside the solve loop (tf is its last entry) */
 static void modify_dt_for_tstops(integ_t* I) {
-    double distance = fabs(I->tf - I->t);
+    const double tstop = first_tstop(I);
+    double distance = fabs(tstop - I->t);
     double tol = 0.0;
-    if (isfinite(I->tf) && isfinite(I->t))
-        tol = 100.0 * oo_jl_eps(jl_max(fabs(I->t), fabs(I->tf)));
+    if (isfinite(tstop) && isfinite(I->t))
+        tol = 100.0 * oo_jl_eps(jl_max(fabs(I->t), fabs(tstop)));
     if (!I->o->adaptive) {   /* dtchangeable branch: always try dtcache */
         I->next_step_tstop = !(fabs(I->dtcache) + tol < distance);
-        if (I->next_step_tstop) I->tstop_target = I->tf;
+        if (I->next_step_tstop) I->tstop_target = tstop;
         I->dt = jl_min(fabs(I->dtcache), distance);
         return;
     }
@@ -446,9 +453,17 @@ static void modify_dt_for_tstops(integ_t* I) {
         I->next_step_tstop = 0;
     } else {
         I->next_step_tstop = 1;
-        I->tstop_target = I->tf;
+        I->tstop_target = tstop;
     }
     I->dt = jl_min(original_dt, distance);
+}
+
+/* handle_tstop! (integrator_utils.jl:1290-1314), dtchangeable: pop every queued
+ * copy of t once t reaches the first tstop.  Stepping past a tstop is an error
+ * in the reference; it cannot happen here (dt is clipped to the distance). */
+static void handle_tstop(integ_t* I) {
+    if (I->tst_pos < I->n_tst && I->t == first_tstop(I))
+        while (I->tst_pos < I->n_tst && I->t == first_tstop(I)) I->tst_pos++;
 }
 
 /* integrator_utils.jl:1243-1256 with utils.jl:90-94 */
@@ -497,7 +512,7 @@ static int check_error(integ_t* I) {
     if (isnan(I->dt)) return OO_RC_DTNAN;
     if (I->iter > I->o->maxiters) return OO_RC_MAXITERS;
     if (I->o->adaptive) {
-        if (fabs(I->dt) <= fabs(I->dtmin) && (!I->accept_step || I->t + I->dt < I->tf))
+        if (fabs(I->dt) <= fabs(I->dtmin) && (!I->accept_step || I->t + I->dt < first_tstop(I)))
             return OO_RC_DTLESSTHANMIN;
         else if (!I->accept_step && fabs(I->dt) <= fabs(oo_jl_eps(I->t)))
             return OO_RC_UNSTABLE;
@@ -782,6 +797,16 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
         qsort(I->saveat_int, I->n_saveat, sizeof(double), cmp_double);
     }
 
+    /* initialize_tstops (solve.jl:1019-1030) */
+    I->tst = (double*)malloc(sizeof(double) * (size_t)(opts->ntstops + 1));
+    I->n_tst = 0; I->tst_pos = 0;
+    for (int i = 0; i < opts->ntstops; i++) {
+        double s = opts->tstops[i];
+        if (t0 < s && s < tf) I->tst[I->n_tst++] = s;
+    }
+    qsort(I->tst, I->n_tst, sizeof(double), cmp_double);
+    I->tst[I->n_tst++] = tf;
+
     memcpy(I->u, u0, sizeof(double) * n);
     memcpy(I->uprev, u0, sizeof(double) * n);
     I->t = t0; I->tprev = t0;
@@ -804,8 +829,11 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     res->dt_init = I->dt;
 
     int rc = OO_RC_SUCCESS;
-    /* solve! (solve.jl:904-946) with tstops = [tf] */
-    while (I->t < I->tf) {
+    /* solve! (solve.jl:904-946): outer loop over the tstops queue, inner loop
+     * of step attempts while t < first_tstop */
+    while (I->tst_pos < I->n_tst && rc == OO_RC_SUCCESS) {
+      const double first = first_tstop(I);
+      while (I->t < first) {
         /* loopheader! (integrator_utils.jl:84-127) */
         if (I->iter > 0) {
             if (I->accept_step) {
@@ -824,6 +852,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
         modify_dt_for_tstops(I);
         rc = check_error(I);
         if (rc != OO_RC_SUCCESS) break;
+        const int should_exit = I->next_step_tstop;
         if (I->next_step_tstop && fabs(I->dt) < oo_jl_eps(fabs(I->t))) {
             I->accept_step = 1;   /* handle_tstop_step!: skip the tiny step */
         } else if (I->alg == OO_TSIT5) {
@@ -841,6 +870,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
             I->accept_step = 1;
             I->dtpropose = I->dt;
             savevalues(I);
+            if (should_exit) break;
             continue;
         }
         double q = stepsize_controller(I);
@@ -857,6 +887,10 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
         } else {
             res->nreject++;
         }
+        if (should_exit) break;
+      }
+      if (rc != OO_RC_SUCCESS) break;
+      handle_tstop(I);
     }
     if (rc == OO_RC_SUCCESS) {
         /* postamble! -> solution_endpoint_match_cur_integrator! (493-588) */
@@ -878,6 +912,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     res->dt_last = I->dt;
     if (u_final) memcpy(u_final, I->u, sizeof(double) * n);
     free(I->saveat_int);
+    free(I->tst);
     free(mem);
     free(piv);
     return rc;

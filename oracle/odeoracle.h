@@ -44,6 +44,8 @@ typedef struct {
     int nsaveat;
     int has_tgrad;        /* Rodas stats: no nf increment for dT when 1 */
     int adaptive;         /* 0: fixed steps of opts.dt (solve.jl `adaptive = false`) */
+    const double* tstops; /* user stopping times (any order, may be NULL); solve.jl:440-446 */
+    int ntstops;
 } oo_opts;
 
 typedef struct {

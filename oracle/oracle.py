@@ -27,7 +27,8 @@ class Opts(ctypes.Structure):
                 ("maxiters", ctypes.c_int64), ("save_start", ctypes.c_int), ("save_end", ctypes.c_int),
                 ("save_everystep", ctypes.c_int), ("save_end_user", ctypes.c_int),
                 ("saveat", ctypes.POINTER(ctypes.c_double)), ("nsaveat", ctypes.c_int),
-                ("has_tgrad", ctypes.c_int), ("adaptive", ctypes.c_int)]
+                ("has_tgrad", ctypes.c_int), ("adaptive", ctypes.c_int),
+                ("tstops", ctypes.POINTER(ctypes.c_double)), ("ntstops", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -92,7 +93,7 @@ def _dp(a):
 
 def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0, maxiters=1000000,
               save_start=True, save_end=True, save_everystep=False, save_end_user=None, saveat=None,
-              has_tgrad=False, adaptive=True):
+              has_tgrad=False, adaptive=True, tstops=None):
     o = Opts()
     lib().oo_default_opts(ctypes.byref(o), alg)
     o.reltol, o.abstol = reltol, abstol
@@ -106,6 +107,10 @@ def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0
         o.saveat, o.nsaveat = _dp(keep), len(keep)
     o.has_tgrad = int(has_tgrad)
     o.adaptive = int(adaptive)
+    if tstops is not None and len(tstops) > 0:
+        ts = np.ascontiguousarray(tstops, dtype=np.float64)
+        o.tstops, o.ntstops = _dp(ts), len(ts)
+        keep = (keep, ts)
     return o, keep
 
 

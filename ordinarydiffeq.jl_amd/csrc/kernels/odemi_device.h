@@ -60,6 +60,8 @@ struct Params {
     i64 maxiters;
     const double* saveat;     // sorted, shared by all trajectories
     int nsaveat;
+    const double* tstops;     // user stopping times, sorted, shared by all trajectories
+    int ntstops;
     int save_start, save_end, save_everystep, save_end_user;  // save_end_user: -1 nothing
     int max_saves;
     double* t_out;            // [max_saves][N]
@@ -329,6 +331,10 @@ struct Ctl {
     int iter, success_iter, accept_step, next_step_tstop;
     int nf, naccept, nreject, njacs, nw, nsolve;
     int spos, send, nsaved, tf_in_saveat;
+    // tstops queue (initialize_tstops, solve.jl:1019-1030): user times strictly
+    // inside (t0, tf) are P.tstops[tsp..tsend), then tf; tnext = the first one
+    int tsp, tsend;
+    double tnext;
 };
 
 // controller constants (alg_utils.jl:407-879; controllers.jl:744-803)
@@ -343,22 +349,26 @@ __device__ __forceinline__ CtlConst ctl_const(bool rosenbrock) {
     return c;
 }
 
-// integrator_utils.jl:268-324 (adaptive branch, tstops = [tf]).
-// apply_step! (integrator_utils.jl:165-191) also calls it right after
-// dt = dtpropose; the engines skip that call.  With t < tf and dt >= 0 (or
-// NaN) it is redundant: the loopheader's fix_dt_at_bounds + modify_dt_for_tstops
-// that follow overwrite dtpropose and next_step_tstop, and min(dt, tf - t)
-// taken before dt is clamped to [max(eps(t), dtmin), dtmax] leaves the same dt
-// after the final min(dt, tf - t) (cases dt <= tf - t, floor > tf - t,
-// dtmax < tf - t, dtmax >= tf - t).  The CPU restatement keeps both calls.
+// integrator_utils.jl:268-324 (adaptive branch).  apply_step!
+// (integrator_utils.jl:165-191) also calls it right after dt = dtpropose; of
+// that call only its clip dt = min(|dt|, distance) survives (the loop header's
+// fix_dt_at_bounds + modify_dt_for_tstops overwrite dtpropose and the flag), so
+// the engines apply just that clip there (apply_step_tstop_clip).  It matters:
+// the second call sets dtpropose from the clipped dt, and a tstop step's accept
+// restores dt = dtpropose.
 __device__ __forceinline__ void modify_dt_for_tstops(Ctl& c) {
-    double distance = fabs(c.tf - c.t);
-    const double tol = (isfinite(c.tf) && isfinite(c.t)) ? 100.0 * jl_eps(jl_max(fabs(c.t), fabs(c.tf))) : 0.0;
+    const double tstop = c.tnext;   // first queued tstop (tf last)
+    double distance = fabs(tstop - c.t);
+    const double tol = (isfinite(tstop) && isfinite(c.t)) ? 100.0 * jl_eps(jl_max(fabs(c.t), fabs(tstop))) : 0.0;
     double original_dt = fabs(c.dt);
     c.dtpropose = original_dt;
-    // the only tstop is tf, so the tstop target is always c.tf
+    // the tstop target of a flagged step is c.tnext (used at its accept)
     c.next_step_tstop = (original_dt + tol < distance) ? 0 : 1;
     c.dt = jl_min(original_dt, distance);
+}
+// The lasting effect of apply_step!'s modify_dt_for_tstops! (see above).
+__device__ __forceinline__ void apply_step_tstop_clip(Ctl& c) {
+    c.dt = jl_min(fabs(c.dt), fabs(c.tnext - c.t));
 }
 // integrator_utils.jl:1243-1256, utils.jl:90-94
 __device__ __forceinline__ void fix_dt_at_bounds(Ctl& c, double dtmin) {
@@ -420,10 +430,30 @@ __device__ __forceinline__ bool all_finite(const S* u) {
 __device__ __forceinline__ int check_error(const Ctl& c, const Params& P, bool u_finite) {
     if (isnan(c.dt)) return RC_DTNAN;
     if ((i64)c.iter > P.maxiters) return RC_MAXITERS;
-    if (fabs(c.dt) <= fabs(P.dtmin) && (!c.accept_step || c.t + c.dt < c.tf)) return RC_DTLESSTHANMIN;
+    if (fabs(c.dt) <= fabs(P.dtmin) && (!c.accept_step || c.t + c.dt < c.tnext)) return RC_DTLESSTHANMIN;
     else if (!c.accept_step && fabs(c.dt) <= fabs(jl_eps(c.t))) return RC_UNSTABLE;
     if (c.accept_step && !u_finite) return RC_UNSTABLE;
     return RC_SUCCESS;
+}
+
+// tstops cursor: the queued user tstops are P.tstops[tsp..tsend) (strictly
+// inside (t, tf)), tnext the first tstop, tf once they are used up.
+__device__ __forceinline__ void tstops_init(Ctl& c, const Params& P, double t) {
+    int lo = 0, hi = P.ntstops;
+    while (lo < hi) { int m = (lo + hi) >> 1; if (P.tstops[m] <= t) lo = m + 1; else hi = m; }
+    c.tsp = lo;
+    hi = P.ntstops;
+    while (lo < hi) { int m = (lo + hi) >> 1; if (P.tstops[m] < c.tf) lo = m + 1; else hi = m; }
+    c.tsend = lo;
+    c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
+}
+// handle_tstop! (integrator_utils.jl:1290-1314) after an accepted step: once t
+// reaches the first tstop every queued copy of it is popped (tf ends the loop).
+__device__ __forceinline__ void handle_tstop(Ctl& c, const Params& P) {
+    if (c.tsp < c.tsend && c.t == c.tnext) {
+        while (c.tsp < c.tsend && P.tstops[c.tsp] == c.t) c.tsp++;
+        c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
+    }
 }
 
 // Saveat cursor setup: saveat_internal = sorted times in (t0, tf]
@@ -493,7 +523,7 @@ template <int N>
 __device__ double initdt_lane(const Ctl& c, const double* u0, const double* p, double reltol,
                               double abstol, double dtmax_opt, double dtmin_opt) {
     const double t = c.t;
-    const double dtmax_tdir = jl_min(fabs(dtmax_opt), fabs(c.tf - t));
+    const double dtmax_tdir = jl_min(fabs(dtmax_opt), fabs(c.tnext - t));   // first tstop
     const double dtmin = nextfloat_pos(jl_max(dtmin_opt, jl_eps(t)));
     const double smalldt = jl_max(dtmin, 1e-6);
     if (N == 0) return jl_max(smalldt, dtmin);
@@ -539,6 +569,8 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     c.nf = 0; c.naccept = 0; c.nreject = 0; c.njacs = 0; c.nw = 0; c.nsolve = 0;
     c.nsaved = 0; c.spos = 0; c.send = 0; c.tf_in_saveat = 0;
     if (P.nsaveat > 0) saveat_init(c, P, t0);
+    c.tsp = 0; c.tsend = 0; c.tnext = tf;
+    if (P.ntstops > 0) tstops_init(c, P, t0);
 }
 
 // solution_endpoint_match_cur_integrator! (integrator_utils.jl:540-588) and
@@ -855,7 +887,8 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
             c.success_iter++;
 #pragma unroll
             for (int i = 0; i < N; i++) { L.uprev[i] = L.u[i]; L.k1[i] = L.k7[i]; }
-            c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (see above)
+            c.dt = c.dtpropose;
+            apply_step_tstop_clip(c);
         } else {
             reject_controller<(sizeof(K) < sizeof(S))>(c, kc);
         }
@@ -877,8 +910,9 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;
-        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
+        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tnext; }
         else c.t = ttmp;
+        handle_tstop(c, P);
         accept_controller<HW>(c, kc, q, P.dtmin);
         if (SAVES && (P.nsaveat > 0 || P.save_everystep)) tsit5_savevalues(L, P, j);
     } else {
@@ -958,6 +992,7 @@ __device__ void tsit5_persistent(const Params& P) {
 // Plain engine: one kernel launch per step attempt, state in HBM.
 // st_scalar rows: t dt dtpropose tprev (unused) tf dtmax EEst q11 errold
 // st_int rows: iter success_iter accept_step next_step_tstop nf naccept nreject done
+//               saveat cursor, njacs nw nsolve (Rodas5P), tstops cursor
 // st_vec rows: u uprev k1 k7 (Tsit5)
 template <int N>
 __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
@@ -976,6 +1011,12 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
         saveat_init(c, P, P.t0[j]);
         c.spos = q[8 * M + j];
     }
+    c.tsp = 0; c.tsend = 0; c.tnext = c.tf;
+    if (P.ntstops > 0) {
+        tstops_init(c, P, P.t0[j]);
+        c.tsp = q[12 * M + j];
+        c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
+    }
     c.njacs = 0; c.nw = 0; c.nsolve = 0;
 }
 template <int N>
@@ -988,7 +1029,7 @@ __device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i
     int* q = P.st_int;
     q[0 * M + j] = c.iter; q[1 * M + j] = c.success_iter; q[2 * M + j] = c.accept_step;
     q[3 * M + j] = c.next_step_tstop; q[4 * M + j] = c.nf; q[5 * M + j] = c.naccept;
-    q[6 * M + j] = c.nreject; q[7 * M + j] = done; q[8 * M + j] = c.spos;
+    q[6 * M + j] = c.nreject; q[7 * M + j] = done; q[8 * M + j] = c.spos; q[12 * M + j] = c.tsp;
     if (P.nsaved) P.nsaved[j] = c.nsaved;
 }
 
@@ -1337,7 +1378,8 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
             c.success_iter++;
 #pragma unroll
             for (int i = 0; i < N; i++) L.uprev[i] = L.u[i];
-            c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (see above)
+            c.dt = c.dtpropose;
+            apply_step_tstop_clip(c);
         } else {
             reject_controller<(sizeof(F) < sizeof(double))>(c, kc);
         }
@@ -1359,8 +1401,9 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;
-        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
+        if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tnext; }
         else c.t = ttmp;
+        handle_tstop(c, P);
         accept_controller<HW>(c, kc, q, P.dtmin);
         if (P.nsaveat > 0 || P.save_everystep) rodas_savevalues(L, P, j);
     } else {

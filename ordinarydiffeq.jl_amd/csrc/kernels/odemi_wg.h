@@ -598,7 +598,7 @@ __device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, con
                             double reltol, double abstol, double dtmin_opt) {
     const int tid = threadIdx.x, NT = blockDim.x;
     const double t = c.t;
-    const double dtmax_tdir = jl_min(fabs(c.dtmax), fabs(c.tf - t));
+    const double dtmax_tdir = jl_min(fabs(c.dtmax), fabs(c.tnext - t));   // first tstop
     const double dtmin = nextfloat_pos(jl_max(dtmin_opt, jl_eps(t)));
     const double smalldt = jl_max(dtmin, 1e-6);
     double* sk = S.lt;
@@ -726,7 +726,8 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
             if (c.accept_step) {
                 c.success_iter++;
                 for (int i = tid; i < n; i += NT) uprev[i] = u[i];
-                c.dt = c.dtpropose;   // apply_step!'s modify_dt_for_tstops is redundant (odemi_device.h)
+                c.dt = c.dtpropose;
+                apply_step_tstop_clip(c);
             } else {
                 reject_controller(c, kc);
             }
@@ -862,8 +863,9 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
         if (c.accept_step) {
             c.naccept++;
             c.tprev = c.t;
-            if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tf; }
+            if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tnext; }
             else c.t = ttmp;
+            handle_tstop(c, P);
             accept_controller(c, kc, q, P.dtmin);
             if (SAVES) {
                 while (c.spos < c.send && P.saveat[c.spos] <= c.t) {

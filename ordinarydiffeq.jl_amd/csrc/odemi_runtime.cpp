@@ -36,6 +36,11 @@ const char* kDeviceSource =
 #include "odemi_device_src.inc"
     ;
 
+// rows of the int32 integrator state of the plain engine and the serving pool
+// (iter, success_iter, accept, next_step_tstop, nf, naccept, nreject, slot
+// state, saveat cursor, njacs, nw, nsolve, tstops cursor)
+constexpr size_t kIntRows = 13;
+
 struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t N;
     int nmax, np;
@@ -46,6 +51,8 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t maxiters;
     const double* saveat;
     int nsaveat;
+    const double* tstops;
+    int ntstops;
     int save_start, save_end, save_everystep, save_end_user;
     int max_saves;
     double* t_out;
@@ -209,7 +216,7 @@ struct odemi_rhs {
         }                                                                                        \
     } while (0)
 
-extern "C" int odemi_abi_version(void) { return 1; }
+extern "C" int odemi_abi_version(void) { return 2; }
 
 extern "C" odemi_ctx* odemi_create(int device) {
     odemi_ctx* c = new odemi_ctx();
@@ -561,6 +568,9 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     if (a->nsaveat < 0 || (a->nsaveat > 0 && !a->saveat)) { ctx->err = "saveat buffer missing"; return -2; }
     for (int i = 1; i < a->nsaveat; i++)
         if (!a->device_ptrs && a->saveat[i] < a->saveat[i - 1]) { ctx->err = "saveat must be sorted"; return -2; }
+    if (a->ntstops < 0 || (a->ntstops > 0 && !a->tstops)) { ctx->err = "tstops buffer missing"; return -2; }
+    for (int i = 1; i < a->ntstops; i++)
+        if (!a->device_ptrs && a->tstops[i] < a->tstops[i - 1]) { ctx->err = "tstops must be sorted"; return -2; }
     const bool rodas = (a->alg == ODEMI_RODAS5P);
     const bool plain = (a->engine == ODEMI_ENGINE_PLAIN);
     const bool mixed = (a->precision == ODEMI_MIXED);
@@ -625,12 +635,13 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         if (N > 0x7fffffffLL) { ctx->err = "N too large for the work-queue order"; return -2; }
         add(N_ * 4); add(kCostBins * 4);
     }
-    if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(12 * N_ * 4); }
+    if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(kIntRows * N_ * 4); }
     if (use_gj) add((size_t)grid * nmax * nmax * 4);
     if (fn_prep) { add(N_ * 8); if (!rodas) add(N_ * nmax * 8); }
     if (!dev) {
         add(N_ * nmax * 8); add(N_ * std::max(np, 1) * 8); add(N_ * 8); add(N_ * 8);
         add(std::max(a->nsaveat, 1) * 8);
+        add(std::max(a->ntstops, 1) * 8);
         add(N_ * std::max<size_t>(ms_, 1) * 8); add(N_ * std::max<size_t>(ms_, 1) * nmax * 8);
         add(N_ * 4); add(N_ * nmax * 8); add(N_ * 8); add(N_ * 4); add(N_ * 6 * 4);
         if (a->nvec) add(N_ * 4);
@@ -643,6 +654,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     P.reltol = a->reltol; P.abstol = a->abstol; P.dtmax = a->dtmax; P.dtmin = a->dtmin; P.dt_user = a->dt;
     P.maxiters = a->maxiters;
     P.nsaveat = a->nsaveat;
+    P.ntstops = a->ntstops;
     P.save_start = a->save_start; P.save_end = a->save_end; P.save_everystep = a->save_everystep;
     P.save_end_user = a->save_end_user; P.max_saves = a->max_saves;
     P.refine_sweeps = a->refine_sweeps;
@@ -650,6 +662,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     HIPCHK(ctx, hipMemsetAsync(P.counters, 0, 64, s));
     if (dev) {
         P.u0 = a->u0; P.p = a->p; P.t0 = a->t0; P.tf = a->tf; P.saveat = a->saveat; P.nvec = a->nvec;
+        P.tstops = a->tstops;
         P.t_out = a->t_out; P.u_out = a->u_out; P.nsaved = a->nsaved; P.u_final = a->u_final;
         P.t_final = a->t_final; P.retcode = a->retcode; P.stats = a->stats;
     } else {
@@ -663,7 +676,9 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         HIPCHK(ctx, hipMemcpyAsync(t0, a->t0, N_ * 8, hipMemcpyHostToDevice, s));
         HIPCHK(ctx, hipMemcpyAsync(tf, a->tf, N_ * 8, hipMemcpyHostToDevice, s));
         if (a->nsaveat > 0) HIPCHK(ctx, hipMemcpyAsync(sa, a->saveat, a->nsaveat * 8, hipMemcpyHostToDevice, s));
-        P.u0 = u0; P.p = pp; P.t0 = t0; P.tf = tf; P.saveat = sa;
+        double* ts = (double*)ar.take(std::max(a->ntstops, 1) * 8);
+        if (a->ntstops > 0) HIPCHK(ctx, hipMemcpyAsync(ts, a->tstops, a->ntstops * 8, hipMemcpyHostToDevice, s));
+        P.u0 = u0; P.p = pp; P.t0 = t0; P.tf = tf; P.saveat = sa; P.tstops = ts;
         P.t_out = (double*)ar.take(N_ * std::max<size_t>(ms_, 1) * 8);
         P.u_out = (double*)ar.take(N_ * std::max<size_t>(ms_, 1) * nmax * 8);
         P.nsaved = (int*)ar.take(N_ * 4);
@@ -691,8 +706,8 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     if (plain) {
         P.st_scalar = (double*)ar.take(16 * N_ * 8);
         P.st_vec = (double*)ar.take(vrows * nmax * N_ * 8);
-        P.st_int = (int*)ar.take(12 * N_ * 4);
-        HIPCHK(ctx, hipMemsetAsync(P.st_int, 0, 12 * N_ * 4, s));
+        P.st_int = (int*)ar.take(kIntRows * N_ * 4);
+        HIPCHK(ctx, hipMemsetAsync(P.st_int, 0, kIntRows * N_ * 4, s));
     }
     if (use_gj) P.wg_jbuf = (float*)ar.take((size_t)grid * nmax * nmax * 4);
     if (fn_prep) {
@@ -874,7 +889,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
         S->wg_grid = std::max(nb, 1) * std::max(ctx->cu_count, 1);
     }
     // bytes per slot: inputs, state, outputs
-    const size_t per_slot = (size_t)(n + np + 2) * 8 + 16 * 8 + (size_t)S->rows * n * 8 + 12 * 4 +
+    const size_t per_slot = (size_t)(n + np + 2) * 8 + 16 * 8 + (size_t)S->rows * n * 8 + kIntRows * 4 +
                             (size_t)n * 8 + 8 + 4 + 4 + 6 * 4 + 4 + (wg ? 4 : 0);
     const size_t fixed = S->gj ? (size_t)S->wg_grid * n * n * 4 : 0;
     int64_t slots = opts->max_slots > 0 ? opts->max_slots : (1 << 20);
@@ -887,13 +902,13 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     size_t bytes = 0;
     auto al = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~size_t(255); return o; };
     size_t o_u0 = al(Sz * n * 8), o_p = al(Sz * np * 8), o_t0 = al(Sz * 8), o_tf = al(Sz * 8);
-    size_t o_sc = al(16 * Sz * 8), o_vec = al((size_t)S->rows * n * Sz * 8), o_int = al(12 * Sz * 4);
+    size_t o_sc = al(16 * Sz * 8), o_vec = al((size_t)S->rows * n * Sz * 8), o_int = al(kIntRows * Sz * 4);
     size_t o_uf = al(Sz * n * 8), o_tfin = al(Sz * 8), o_rc = al(Sz * 4), o_st = al(6 * Sz * 4), o_ns = al(Sz * 4);
     size_t o_sreq = al(Sz * 4), o_done = al((size_t)S->R * 8), o_cnt = al(64);
     size_t o_nvec = wg ? al(Sz * 4) : 0, o_jb = S->gj ? al((size_t)S->wg_grid * n * n * 4) : 0;
     if (hipMalloc(&S->mem, bytes) != hipSuccess) { ctx->err = "serving pool allocation failed"; delete S; return nullptr; }
     char* b = (char*)S->mem;
-    (void)hipMemset(b + o_int, 0, 12 * Sz * 4);
+    (void)hipMemset(b + o_int, 0, kIntRows * Sz * 4);
     (void)hipMemset(b + o_done, 0, (size_t)S->R * 8);
     (void)hipMemset(b + o_cnt, 0, 64);
     Params& P = S->P;

@@ -58,6 +58,7 @@ mutable struct OdemiSolveArgs
     reltol::Float64; abstol::Float64; dtmax::Float64; dtmin::Float64; dt::Float64
     maxiters::Int64
     saveat::Ptr{Float64}; nsaveat::Cint
+    tstops::Ptr{Float64}; ntstops::Cint
     save_start::Cint; save_end::Cint; save_everystep::Cint; save_end_user::Cint
     max_saves::Cint
     t_out::Ptr{Float64}; u_out::Ptr{Float64}; nsaved::Ptr{Int32}
@@ -242,7 +243,7 @@ end
 function SciMLBase.__solve(ep::SciMLBase.EnsembleProblem, alg::Union{Tsit5,Rodas5P}, ea::EnsembleMI355X;
                            trajectories, reltol = 1e-3, abstol = 1e-6, saveat = Float64[], save_start = nothing,
                            save_end = nothing, save_everystep = isempty(saveat), dt = 0.0, dtmax = 0.0,
-                           dtmin = 0.0, maxiters = 1_000_000, max_saves = 4096, kwargs...)
+                           dtmin = 0.0, maxiters = 1_000_000, max_saves = 4096, tstops = Float64[], kwargs...)
     probs = [ep.prob_func(deepcopy(ep.prob), i, 1) for i in 1:trajectories]
     n = length(probs[1].u0)
     np = probs[1].p isa SciMLBase.NullParameters ? 0 : length(probs[1].p)
@@ -265,6 +266,7 @@ function SciMLBase.__solve(ep::SciMLBase.EnsembleProblem, alg::Union{Tsit5,Rodas
         t0[j], tf[j] = pr.tspan
     end
     sa = collect(Float64, saveat)
+    tst = sort!(collect(Float64, tstops))
     ss = save_start === nothing ? (isempty(sa) || first(sa) == first(probs[1].tspan)) : save_start
     se = save_end === nothing ? true : save_end
     ms = save_everystep ? max_saves : length(sa) + 2
@@ -272,7 +274,7 @@ function SciMLBase.__solve(ep::SciMLBase.EnsembleProblem, alg::Union{Tsit5,Rodas
     u_final = zeros(N, n); t_final = zeros(N); retcode = zeros(Int32, N); stats = zeros(Int32, N, 6)
     a = OdemiSolveArgs()
     odemi_default_args!(a, code)
-    GC.@preserve u0 P t0 tf sa t_out u_out nsaved u_final t_final retcode stats begin
+    GC.@preserve u0 P t0 tf sa tst t_out u_out nsaved u_final t_final retcode stats begin
         a.engine = ea.engine === :plain ? ENGINE_PLAIN : ENGINE_PERSISTENT
         a.precision = ea.precision === :mixed ? MIXED : FP64
         a.refine_sweeps = ea.refine_sweeps
@@ -280,6 +282,7 @@ function SciMLBase.__solve(ep::SciMLBase.EnsembleProblem, alg::Union{Tsit5,Rodas
         a.u0 = pointer(u0); a.p = pointer(P); a.t0 = pointer(t0); a.tf = pointer(tf)
         a.reltol = reltol; a.abstol = abstol; a.dt = dt; a.dtmax = dtmax; a.dtmin = dtmin; a.maxiters = maxiters
         a.saveat = isempty(sa) ? Ptr{Float64}(C_NULL) : pointer(sa); a.nsaveat = length(sa)
+        a.tstops = isempty(tst) ? Ptr{Float64}(C_NULL) : pointer(tst); a.ntstops = length(tst)
         a.save_start = ss; a.save_end = se; a.save_everystep = save_everystep
         a.save_end_user = save_end === nothing ? -1 : Int(save_end); a.max_saves = ms
         a.t_out = pointer(t_out); a.u_out = pointer(u_out); a.nsaved = pointer(nsaved)

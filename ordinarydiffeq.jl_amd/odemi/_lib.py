@@ -29,6 +29,7 @@ class SolveArgs(ctypes.Structure):
         ("reltol", c_d), ("abstol", c_d), ("dtmax", c_d), ("dtmin", c_d), ("dt", c_d),
         ("maxiters", c_i64),
         ("saveat", c_vp), ("nsaveat", c_i),
+        ("tstops", c_vp), ("ntstops", c_i),
         ("save_start", c_i), ("save_end", c_i), ("save_everystep", c_i), ("save_end_user", c_i),
         ("max_saves", c_i),
         ("t_out", c_vp), ("u_out", c_vp), ("nsaved", c_vp), ("u_final", c_vp), ("t_final", c_vp),

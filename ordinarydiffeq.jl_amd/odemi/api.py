@@ -264,6 +264,20 @@ def saveat_times(saveat, t0, tf):
     return arr
 
 
+def _tstops_array(tstops):
+    """User tstops (solve.jl:440-446) as one sorted FP64 list shared by every
+    trajectory; a scalar is one time.  Callables (late-binding tstops of a single
+    problem) are not supported for an ensemble."""
+    if tstops is None:
+        return np.zeros(0)
+    if callable(tstops):
+        raise ValueError("tstops must be times shared by all trajectories (callables are not supported)")
+    arr = np.atleast_1d(np.asarray(tstops, dtype=np.float64))
+    if arr.ndim != 1:
+        raise ValueError("tstops must be a 1-D list of times")
+    return np.sort(arr)
+
+
 def _resolve_save_flags(saveat, save_start, save_end, save_everystep, t0, tf):
     is_num = saveat is not None and np.isscalar(saveat)
     empty = saveat is None or (not is_num and len(saveat) == 0)
@@ -284,9 +298,12 @@ def _is_torch(x):
 
 def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abstol=None, saveat=None,
                  save_start=False, save_end=True, save_everystep=False, save_end_user=-1, max_saves=None,
-                 dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None, nvec=None):
+                 dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None, nvec=None,
+                 tstops=None):
     """Bulk entry point on packed SoA arrays: u0 (nmax, N), p (np, N), t0/tf (N,);
-    nvec (N,) gives per-trajectory dimensions for uneven batches (Rodas5P).
+    nvec (N,) gives per-trajectory dimensions for uneven batches (Rodas5P);
+    tstops: stopping times shared by all trajectories (each keeps those strictly
+    inside its own (t0, tf)).
 
     Arrays may be numpy (host) or torch CUDA tensors (device-resident, zero
     copies).  Returns a dict of output arrays of the same kind."""
@@ -307,6 +324,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
     reltol = 1e-3 if reltol is None else float(reltol)
     abstol = 1e-6 if abstol is None else float(abstol)
     sa = np.zeros(0) if saveat is None else np.asarray(saveat, dtype=np.float64)
+    tsv = _tstops_array(tstops)
     if max_saves is None:
         max_saves = (1 if save_end else 0) + (1 if save_start else 0) + len(sa)
         if save_everystep:
@@ -336,6 +354,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         pc = (p.contiguous() if p is not None else torch.zeros((1, N), **dd))
         t0c, tfc = t0.contiguous(), tf.contiguous()
         sat = torch.as_tensor(sa, **dd) if len(sa) else torch.zeros(1, **dd)
+        tst = torch.as_tensor(tsv, **dd) if len(tsv) else torch.zeros(1, **dd)
         out = outputs or {}
         o = dict(
             t_out=out.get("t_out", torch.empty((max(max_saves, 1), N), **dd)),
@@ -346,7 +365,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
             retcode=out.get("retcode", torch.empty(N, dtype=torch.int32, device=devc)),
             stats=out.get("stats", torch.empty((6, N), dtype=torch.int32, device=devc)),
         )
-        keep += [u0c, pc, t0c, tfc, sat]
+        keep += [u0c, pc, t0c, tfc, sat, tst]
         if nvec is not None:
             nvc = torch.as_tensor(nvec, dtype=torch.int32, device=devc).contiguous()
             keep.append(nvc)
@@ -360,10 +379,11 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         t0c = np.ascontiguousarray(np.broadcast_to(t0, (N,)), dtype=np.float64)
         tfc = np.ascontiguousarray(np.broadcast_to(tf, (N,)), dtype=np.float64)
         sat = np.ascontiguousarray(sa if len(sa) else np.zeros(1))
+        tst = np.ascontiguousarray(tsv if len(tsv) else np.zeros(1))
         o = dict(t_out=np.empty((max(max_saves, 1), N)), u_out=np.empty((max(max_saves, 1), n, N)),
                  nsaved=np.empty(N, dtype=np.int32), u_final=np.empty((n, N)), t_final=np.empty(N),
                  retcode=np.empty(N, dtype=np.int32), stats=np.empty((6, N), dtype=np.int32))
-        keep += [u0c, pc, t0c, tfc, sat]
+        keep += [u0c, pc, t0c, tfc, sat, tst]
         if nvec is not None:
             nvc = np.ascontiguousarray(nvec, dtype=np.int32)
             keep.append(nvc)
@@ -372,10 +392,12 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         a.device_ptrs = 0
     a.u0, a.p, a.t0, a.tf = ptr(u0c), ptr(pc), ptr(t0c), ptr(tfc)
     a.saveat, a.nsaveat = ptr(sat), len(sa)
+    a.tstops, a.ntstops = ptr(tst), len(tsv)
     a.t_out, a.u_out, a.nsaved = ptr(o["t_out"]), ptr(o["u_out"]), ptr(o["nsaved"])
     a.u_final, a.t_final, a.retcode, a.stats = ptr(o["u_final"]), ptr(o["t_final"]), ptr(o["retcode"]), ptr(o["stats"])
     if ea.autotune and ea.engine != "plain" and not ea.blocks_per_cu:
-        key = (id(ctx), int(h), alg.code, ea.precision, N, n, int(save_everystep), len(sa), ns is not None)
+        key = (id(ctx), int(h), alg.code, ea.precision, N, n, int(save_everystep), len(sa), len(tsv),
+               ns is not None)
         a.blocks_per_cu = _autotune(ctx, h, a, key, wg=(alg.code == _lib.RODAS5P and (ns is not None or n > LANE_MAX_RODAS)))
         o["tuning"] = dict(zip(("blocks_per_cu", "kernel_ms"), _TUNED[key]))
     t_s = time.perf_counter()
@@ -391,7 +413,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
 
 def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=None, saveat=None,
           save_start=None, save_end=None, save_everystep=None, dt=None, dtmax=None, dtmin=0.0,
-          maxiters=1_000_000, seed=0, batch_size=None, max_saves=None, **kw):
+          maxiters=1_000_000, seed=0, batch_size=None, max_saves=None, tstops=None, **kw):
     """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...) -> EnsembleSolution.
 
     A plain ODEProblem is solved as a one-trajectory ensemble and returns its ODESolution."""
@@ -436,7 +458,7 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=No
         max_saves = 4096
     o = solve_arrays(base.f, alg, u0, pp, t0, tf, ea, reltol=reltol, abstol=abstol, saveat=sa, save_start=ss,
                      save_end=se, save_everystep=sev, save_end_user=seu, max_saves=max_saves, dt=dt,
-                     dtmax=dtmax, dtmin=dtmin, maxiters=maxiters)
+                     dtmax=dtmax, dtmin=dtmin, maxiters=maxiters, tstops=tstops)
     if sev and int(o["nsaved"].max(initial=0)) > o["max_saves"]:
         raise RuntimeError(f"save_everystep produced more than max_saves={o['max_saves']} points; "
                            "pass a larger max_saves")

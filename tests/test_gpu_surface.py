@@ -83,6 +83,95 @@ def test_save_everystep_bit_identical(alg, prob_id, fun, dgen):
         assert np.array_equal(out["u_out"][:ns, :, j], r["u"])
 
 
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()], ids=["tsit5", "rodas5p"])
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_saveat_inside_final_clipped_step(alg, engine):
+    """Saveat points inside the last step, which modify_dt_for_tstops! clips to
+    tf: the accept restores dt = dtpropose, and the dtpropose the reference's two
+    clips (apply_step! and the loop header) leave is the clipped dt, so the
+    interpolation is the step's own (R10).  Bit-identical to the CPU restatement."""
+    d = P.lorenz_small(N=64)
+    sa = [0.97, 0.99, 0.999, 0.9999, 0.99999]
+    print(f"[case] saveat in final step {alg.code} {engine}", flush=True)
+    out = odemi.solve_arrays(P.LORENZ, alg, d["u0"], d["p"], d["t0"], d["tf"], odemi.EnsembleMI355X(engine=engine),
+                             reltol=d["reltol"], abstol=d["abstol"], saveat=sa, save_start=False, save_end=True)
+    for j in range(0, 64, 3):
+        r = O.solve_builtin(O.PROB_LORENZ, d["u0"][:, j], d["p"][:, j], (0.0, 1.0), alg=alg.code,
+                            reltol=d["reltol"], abstol=d["abstol"], saveat=sa, save_start=False, save_end=True)
+        ns = int(out["nsaved"][j])
+        assert ns == r["nsaved"] == len(sa) + 1
+        assert np.array_equal(out["t_out"][:ns, j], r["t"])
+        assert np.array_equal(out["u_out"][:ns, :, j], r["u"])
+
+
+@pytest.mark.parametrize("alg,prob_id,fun,dgen,ts", [
+    (odemi.Tsit5(), O.PROB_LORENZ, P.LORENZ, lambda: P.lorenz_small(N=128), [0.13, 0.5, 0.5, 0.77, 0.0, 2.0]),
+    (odemi.Rodas5P(), O.PROB_ROBERTSON, P.ROBERTSON, lambda: P.robertson_ensemble(N=128),
+     [1e-3, 0.37, 40.0, 40.0, 5e3, 1e6]),
+], ids=["tsit5-lorenz", "rodas5p-robertson"])
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_tstops_bit_identical(alg, prob_id, fun, dgen, ts, engine):
+    """User tstops (shared list; times outside (t0, tf) and duplicates dropped):
+    every step time, state and counter equals the CPU restatement, which runs
+    the reference's nested tstops loop (solve.jl:904-927), and every tstop
+    inside the span is a step time."""
+    d = dgen()
+    N = d["u0"].shape[1]
+    print(f"[case] tstops {alg.code} {engine} N={N}", flush=True)
+    out = odemi.solve_arrays(fun, alg, d["u0"], d["p"], d["t0"], d["tf"], odemi.EnsembleMI355X(engine=engine),
+                             reltol=d["reltol"], abstol=d["abstol"], save_everystep=True, save_start=True,
+                             save_end=True, max_saves=4096, tstops=ts)
+    for j in range(0, N, 11):
+        r = O.solve_builtin(prob_id, d["u0"][:, j], d["p"][:, j], (d["t0"][j], d["tf"][j]), alg=alg.code,
+                            reltol=d["reltol"], abstol=d["abstol"], save_everystep=True, tstops=ts)
+        ns = int(out["nsaved"][j])
+        assert ns == r["nsaved"], (j, ns, r["nsaved"])
+        assert np.array_equal(out["t_out"][:ns, j], r["t"])
+        assert np.array_equal(out["u_out"][:ns, :, j], r["u"])
+        st = out["stats"][:, j]
+        assert (st[0], st[1], st[2]) == (r["nf"], r["naccept"], r["nreject"])
+        for s in ts:
+            if d["t0"][j] < s < d["tf"][j]:
+                assert s in out["t_out"][:ns, j]
+
+
+def test_tstops_workgroup_and_mixed():
+    """tstops through the workgroup kernel (uneven-n chains, FP64 bit-identical
+    to the CPU restatement) and the mixed lane engine (tstops hit exactly,
+    final states within the tolerance scale of FP64)."""
+    ts = [0.05, 0.2, 0.2, 0.61]
+    d = P.chain_ensemble(N=24, seed=5, m_lo=2, m_hi=12)
+    d["tf"] = np.full(24, 1.0)
+    print("[case] tstops workgroup chains", flush=True)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"], save_everystep=True, save_start=True,
+                             max_saves=4096, tstops=ts)
+    for j in range(0, 24, 5):
+        n = int(d["nvec"][j])
+        r = O.solve_builtin(O.PROB_CHAIN, d["u0"][:n, j], d["p"][:, j], (0.0, 1.0), alg=O.RODAS5P,
+                            reltol=d["reltol"], abstol=d["abstol"], save_everystep=True, tstops=ts)
+        ns = int(out["nsaved"][j])
+        assert ns == r["nsaved"]
+        assert np.array_equal(out["t_out"][:ns, j], r["t"])
+        assert np.array_equal(out["u_out"][:ns, :n, j], r["u"])
+        assert all(s in out["t_out"][:ns, j] for s in ts)
+    d = P.lorenz_large(N=2048)
+    print("[case] tstops mixed lane", flush=True)
+    kw = dict(reltol=1e-6, abstol=1e-6, save_everystep=True, save_start=True, max_saves=4096, tstops=ts)
+    o64 = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"], **kw)
+    om = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(precision="mixed"), **kw)
+    assert (om["retcode"] == 1).all()
+    for j in range(0, 2048, 97):
+        ns = int(om["nsaved"][j])
+        assert all(s in om["t_out"][:ns, j] for s in ts)
+    uf64 = np.stack([o64["u_out"][int(o64["nsaved"][j]) - 1, :, j] for j in range(2048)], axis=1)
+    ufm = np.stack([om["u_out"][int(om["nsaved"][j]) - 1, :, j] for j in range(2048)], axis=1)
+    err = np.abs(ufm - uf64) / (1e-6 + 1e-6 * np.abs(uf64))
+    print(f"tstops mixed scaled err median {np.median(err):.3e}")
+    assert np.median(err.max(axis=0)) < 10.0
+
+
 def _expb(du, u, p, t):
     du[0] = odemi.sym_exp(u[0])
 

@@ -304,6 +304,37 @@ def test_saveat_save_start_save_end_rules():
     assert 0.0 not in r["t"] and 0.4 not in r["t"]
 
 
+def test_tstops_reference_known_answers():
+    """test/InterfaceI/ode_tstops_tests.jl:8-62 on prob_ode_linear (u' = 1.01 u,
+    u0 = 1/2, tspan (0, 1)): tstops are hit exactly, a fixed step is split at a
+    tstop, endpoints and duplicates are dropped, and a 0:0.1:1 grid is a subset
+    of the solution times."""
+    lin = lambda du, u, p, t: du.__setitem__(0, 1.01 * u[0])
+    kw = dict(tspan=(0.0, 1.0), p=[0.0], save_everystep=True)
+    for alg in (O.TSIT5, O.RODAS5P):
+        r = O.solve(lin, [0.5], alg=alg, dt=1 / 64, tstops=[0.5], **kw)
+        assert 0.5 in list(r["t"])
+        # fixed steps of 1/3: [0, 1/3, 1/2, 1/3 + 1/2, 1] (the reference's RK4 case)
+        r = O.solve(lin, [0.5], alg=alg, dt=1 / 3, adaptive=False, tstops=[0.5], **kw)
+        assert list(r["t"]) == [0.0, 1 / 3, 0.5, 1 / 3 + 0.5, 1.0]
+        ts = [0, 1 / 5, 1 / 4, 1 / 3, 1 / 2, 3 / 4, 1, 1 / 2]
+        r = O.solve(lin, [0.5], alg=alg, tstops=ts, **kw)
+        t = list(r["t"])
+        assert all(s in t for s in ts) and len(set(t)) == len(t)
+        grid = [k / 10 for k in range(11)]
+        r = O.solve(lin, [0.5], alg=alg, tstops=grid, **kw)
+        assert set(grid) <= set(r["t"])
+        # same final state as without tstops to the tolerance, more (shorter) steps
+        r0 = O.solve(lin, [0.5], alg=alg, reltol=1e-8, abstol=1e-8, **kw)
+        r1 = O.solve(lin, [0.5], alg=alg, reltol=1e-8, abstol=1e-8, tstops=grid, **kw)
+        exact = 0.5 * math.exp(1.01)
+        assert abs(r1["u"][-1][0] - exact) < 1e-6 and abs(r0["u"][-1][0] - exact) < 1e-6
+        assert r1["naccept"] >= r0["naccept"]
+    # initdt is bounded by the first tstop, not by tf (integrator_interface.jl:644-662)
+    r = O.solve(lin, [0.5], tspan=(0.0, 1.0), p=[0.0], tstops=[1e-4], save_everystep=True)
+    assert r["t"][1] <= 1e-4
+
+
 def test_saveat_interpolant_accuracy():
     # dense output (Tsit5 free 4th-order interpolant, Rodas 3rd-order H-rows) vs exp
     sa = list(np.linspace(0.05, 0.95, 19))
