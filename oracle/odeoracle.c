@@ -362,7 +362,7 @@ typedef struct {
     /* integrator scalars */
     double t, dt, dtpropose, tprev, tstop_target, EEst, q11, errold, dtcache;
     int64_t iter, success_iter;
-    int accept_step, next_step_tstop;
+    int accept_step, next_step_tstop, isout;
     oo_result* res;
     /* vectors */
     double *u, *uprev, *tmp, *atmp;
@@ -811,7 +811,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     memcpy(I->uprev, u0, sizeof(double) * n);
     I->t = t0; I->tprev = t0;
     I->EEst = 1.0; I->q11 = 1.0; I->errold = I->qoldinit;
-    I->iter = 0; I->success_iter = 0; I->accept_step = 0; I->next_step_tstop = 0;
+    I->iter = 0; I->success_iter = 0; I->accept_step = 0; I->next_step_tstop = 0; I->isout = 0;
     I->tstop_target = 0.0;
 
     if (opts->save_start) save_point(I, t0, I->u);
@@ -843,6 +843,8 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
                 I->dt = I->dtpropose;
                 if (I->alg == OO_TSIT5) memcpy(I->k[0], I->k[6], sizeof(double) * n);
                 modify_dt_for_tstops(I);
+            } else if (I->isout) {
+                I->dt = I->dt * I->qmin;   /* handle_step_rejection! (integrator_utils.jl:135-136) */
             } else {
                 step_reject_controller(I);
             }
@@ -874,7 +876,8 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
             continue;
         }
         double q = stepsize_controller(I);
-        I->accept_step = (I->EEst <= 1.0);
+        I->isout = opts->isoutofdomain ? (opts->isoutofdomain(I->u, I->p, ttmp, opts->isout_user) != 0) : 0;
+        I->accept_step = !I->isout && (I->EEst <= 1.0);
         if (I->accept_step) {
             res->naccept++;
             I->tprev = I->t;

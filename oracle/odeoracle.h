@@ -46,6 +46,10 @@ typedef struct {
     int adaptive;         /* 0: fixed steps of opts.dt (solve.jl `adaptive = false`) */
     const double* tstops; /* user stopping times (any order, may be NULL); solve.jl:440-446 */
     int ntstops;
+    /* isoutofdomain(u, p, t) (solve.jl kwarg; integrator_utils.jl:612-614): nonzero
+     * rejects the step; the next dt is dt * qmin (integrator_utils.jl:135-136).  NULL: never. */
+    int (*isoutofdomain)(const double* u, const double* p, double t, void* user);
+    void* isout_user;
 } oo_opts;
 
 typedef struct {

@@ -17,6 +17,8 @@ PROB_USER, PROB_LORENZ, PROB_ROBERTSON, PROB_CHAIN, PROB_LINEAR, PROB_EXPBLOWUP 
 RC_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN", 4: "MaxIters",
             5: "MaxNumSub", 6: "DtLessThanMin", 7: "Unstable"}
 
+ISOUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                            ctypes.c_double, ctypes.c_void_p)
 RHS_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_void_p)
 
@@ -28,7 +30,8 @@ class Opts(ctypes.Structure):
                 ("save_everystep", ctypes.c_int), ("save_end_user", ctypes.c_int),
                 ("saveat", ctypes.POINTER(ctypes.c_double)), ("nsaveat", ctypes.c_int),
                 ("has_tgrad", ctypes.c_int), ("adaptive", ctypes.c_int),
-                ("tstops", ctypes.POINTER(ctypes.c_double)), ("ntstops", ctypes.c_int)]
+                ("tstops", ctypes.POINTER(ctypes.c_double)), ("ntstops", ctypes.c_int),
+                ("isoutofdomain", ctypes.c_void_p), ("isout_user", ctypes.c_void_p)]
 
 
 class Result(ctypes.Structure):
@@ -93,7 +96,7 @@ def _dp(a):
 
 def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0, maxiters=1000000,
               save_start=True, save_end=True, save_everystep=False, save_end_user=None, saveat=None,
-              has_tgrad=False, adaptive=True, tstops=None):
+              has_tgrad=False, adaptive=True, tstops=None, isoutofdomain=None, isout_shape=(0, 0)):
     o = Opts()
     lib().oo_default_opts(ctypes.byref(o), alg)
     o.reltol, o.abstol = reltol, abstol
@@ -111,6 +114,17 @@ def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0
         ts = np.ascontiguousarray(tstops, dtype=np.float64)
         o.tstops, o.ntstops = _dp(ts), len(ts)
         keep = (keep, ts)
+    if isoutofdomain is not None:
+        # isoutofdomain(u, p, t) -> bool on numpy copies of the state and parameters
+        n, np_ = isout_shape
+
+        def _io(u, p, t, _user, _f=isoutofdomain):
+            uu = np.array(np.ctypeslib.as_array(u, shape=(max(n, 1),))[:n])
+            pp = np.array(np.ctypeslib.as_array(p, shape=(max(np_, 1),))[:np_])
+            return 1 if _f(uu, pp, t) else 0
+        cb = ISOUT_FN(_io)
+        o.isoutofdomain = ctypes.cast(cb, ctypes.c_void_p)
+        keep = (keep, cb)
     return o, keep
 
 
@@ -148,7 +162,7 @@ def solve(f, u0, p, tspan, jac=None, tgrad=None, max_out=4096, **kw):
     cj = RHS_FN(_j) if jac is not None else RHS_FN(lambda *a: None)
     ct = ctypes.cast(RHS_FN(_tg), ctypes.c_void_p) if tgrad is not None else None
     keep_tg = ct
-    opts, keep = make_opts(has_tgrad=tgrad is not None, **kw)
+    opts, keep = make_opts(has_tgrad=tgrad is not None, isout_shape=(n, len(pp)), **kw)
     t_out = np.zeros(max_out)
     u_out = np.zeros((max_out, max(n, 1)))
     uf = np.zeros(max(n, 1))
@@ -165,7 +179,7 @@ def solve_builtin(prob, u0, p, tspan, max_out=4096, **kw):
     u0 = np.ascontiguousarray(u0, dtype=np.float64)
     n = len(u0)
     pp = np.ascontiguousarray(p, dtype=np.float64)
-    opts, keep = make_opts(**kw)
+    opts, keep = make_opts(isout_shape=(n, len(pp)), **kw)
     t_out = np.zeros(max_out)
     u_out = np.zeros((max_out, max(n, 1)))
     uf = np.zeros(max(n, 1))

@@ -329,6 +329,7 @@ __constant__ double kRH[3][8] = {
 struct Ctl {
     double t, dt, dtpropose, tprev, tf, dtmax, EEst, q11, errold;
     int iter, success_iter, accept_step, next_step_tstop;
+    int isout;   // the last attempt's isoutofdomain verdict (integrator.isout)
     int nf, naccept, nreject, njacs, nw, nsolve;
     int spos, send, nsaved, tf_in_saveat;
     // tstops queue (initialize_tstops, solve.jl:1019-1030): user times strictly
@@ -564,7 +565,7 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     c.t = t0; c.tprev = t0; c.tf = tf;
     c.dtmax = (P.dtmax > 0.0) ? P.dtmax : (tf - t0);
     c.EEst = 1.0; c.q11 = 1.0; c.errold = 1e-4;
-    c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0;
+    c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0; c.isout = 0;
     c.dt = 0.0; c.dtpropose = 0.0;
     c.nf = 0; c.naccept = 0; c.nreject = 0; c.njacs = 0; c.nw = 0; c.nsolve = 0;
     c.nsaved = 0; c.spos = 0; c.send = 0; c.tf_in_saveat = 0;
@@ -890,7 +891,8 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
             c.dt = c.dtpropose;
             apply_step_tstop_clip(c);
         } else {
-            reject_controller<(sizeof(K) < sizeof(S))>(c, kc);
+            if (c.isout) c.dt = c.dt * kc.qmin;   // handle_step_rejection! (integrator_utils.jl:135-136)
+            else reject_controller<(sizeof(K) < sizeof(S))>(c, kc);
         }
     }
     c.iter++;
@@ -906,7 +908,10 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
     double ttmp = c.t + c.dt;
     constexpr bool HW = sizeof(K) < sizeof(S);   // mixed: hardware FastPower, Newton-refined FP64 quotients
     double q = stepsize_controller<HW>(c, kc);
-    c.accept_step = (c.EEst <= 1.0);
+#ifdef ODEMI_HAS_ISOUT
+    c.isout = odemi_isout<double, K>(L.u, L.p, ttmp) ? 1 : 0;   // integrator_utils.jl:612-614
+#endif
+    c.accept_step = !c.isout && (c.EEst <= 1.0);
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;
@@ -992,7 +997,7 @@ __device__ void tsit5_persistent(const Params& P) {
 // Plain engine: one kernel launch per step attempt, state in HBM.
 // st_scalar rows: t dt dtpropose tprev (unused) tf dtmax EEst q11 errold
 // st_int rows: iter success_iter accept_step next_step_tstop nf naccept nreject done
-//               saveat cursor, njacs nw nsolve (Rodas5P), tstops cursor
+//               saveat cursor, njacs nw nsolve (Rodas5P), tstops cursor, isout
 // st_vec rows: u uprev k1 k7 (Tsit5)
 template <int N>
 __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
@@ -1017,6 +1022,7 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
         c.tsp = q[12 * M + j];
         c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
     }
+    c.isout = q[13 * M + j];
     c.njacs = 0; c.nw = 0; c.nsolve = 0;
 }
 template <int N>
@@ -1030,6 +1036,7 @@ __device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i
     q[0 * M + j] = c.iter; q[1 * M + j] = c.success_iter; q[2 * M + j] = c.accept_step;
     q[3 * M + j] = c.next_step_tstop; q[4 * M + j] = c.nf; q[5 * M + j] = c.naccept;
     q[6 * M + j] = c.nreject; q[7 * M + j] = done; q[8 * M + j] = c.spos; q[12 * M + j] = c.tsp;
+    q[13 * M + j] = c.isout;
     if (P.nsaved) P.nsaved[j] = c.nsaved;
 }
 
@@ -1381,7 +1388,8 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
             c.dt = c.dtpropose;
             apply_step_tstop_clip(c);
         } else {
-            reject_controller<(sizeof(F) < sizeof(double))>(c, kc);
+            if (c.isout) c.dt = c.dt * kc.qmin;   // handle_step_rejection! (integrator_utils.jl:135-136)
+            else reject_controller<(sizeof(F) < sizeof(double))>(c, kc);
         }
     }
     c.iter++;
@@ -1397,7 +1405,10 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
     double ttmp = c.t + c.dt;
     constexpr bool HW = sizeof(F) < sizeof(double);   // mixed: hardware FastPower, Newton-refined FP64 quotients
     double q = stepsize_controller<HW>(c, kc);
-    c.accept_step = (c.EEst <= 1.0);
+#ifdef ODEMI_HAS_ISOUT
+    c.isout = odemi_isout<double, double>(L.u, L.p, ttmp) ? 1 : 0;   // integrator_utils.jl:612-614
+#endif
+    c.accept_step = !c.isout && (c.EEst <= 1.0);
     if (c.accept_step) {
         c.naccept++;
         c.tprev = c.t;

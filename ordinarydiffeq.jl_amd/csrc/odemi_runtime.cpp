@@ -38,8 +38,8 @@ const char* kDeviceSource =
 
 // rows of the int32 integrator state of the plain engine and the serving pool
 // (iter, success_iter, accept, next_step_tstop, nf, naccept, nreject, slot
-// state, saveat cursor, njacs, nw, nsolve, tstops cursor)
-constexpr size_t kIntRows = 13;
+// state, saveat cursor, njacs, nw, nsolve, tstops cursor, isoutofdomain verdict)
+constexpr size_t kIntRows = 14;
 
 struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t N;

@@ -4,9 +4,10 @@ from .api import (DEStats, EnsembleContext, EnsembleMI355X, EnsembleProblem, Ens
                   ODEProblem, ODESolution, ReturnCode, Rodas5P, Tsit5, compile_problem, get_context, remake,
                   solve, solve_arrays)
 from .serving import EnsembleServer, Request
-from .tracer import Sym, sym_cos, sym_exp, sym_log, sym_sin, sym_sqrt, sym_tanh, trace_problem
+from .tracer import Sym, sym_all, sym_any, sym_cos, sym_exp, sym_log, sym_sin, sym_sqrt, sym_tanh, trace_problem
 
 __all__ = ["DEStats", "EnsembleContext", "EnsembleMI355X", "EnsembleProblem", "EnsembleSolution", "ODEFunction",
            "ODEProblem", "ODESolution", "ReturnCode", "Rodas5P", "Tsit5", "compile_problem", "get_context", "remake",
-           "solve", "solve_arrays", "Sym", "sym_cos", "sym_exp", "sym_log", "sym_sin", "sym_sqrt", "sym_tanh",
+           "solve", "solve_arrays", "Sym", "sym_all", "sym_any", "sym_cos", "sym_exp", "sym_log", "sym_sin",
+           "sym_sqrt", "sym_tanh",
            "trace_problem", "EnsembleServer", "Request"]

@@ -201,15 +201,21 @@ template <typename T> __device__ __forceinline__ void odemi_tgrad(T* dT, const T
 """
 
 
-def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None):
+def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, isoutofdomain=None):
     """Trace fun (ODEFunction) for state size n (or the sizes `ns` of an uneven
-    batch, n = max) and hipRTC-compile it.  Returns (rhs, traced, seconds)."""
+    batch, n = max) and hipRTC-compile it.  isoutofdomain(u, p, t): a traceable
+    predicate (comparisons, &, |, ~, odemi.sym_any / sym_all) for the lane
+    kernels.  Returns (rhs, traced, seconds)."""
     wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
+    if wg and isoutofdomain is not None:
+        raise ValueError("isoutofdomain is supported by the lane kernels (Tsit5; Rodas5P with n <= "
+                         f"{LANE_MAX_RODAS}), not by the workgroup kernels")
     if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
         raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
     if ns is not None and "tsit5" in algs:
         raise ValueError("uneven-size batches (nvec) are supported with Rodas5P")
-    key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_, tuple(ns) if ns is not None else None, wg)
+    key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_, tuple(ns) if ns is not None else None, wg,
+           id(isoutofdomain))
     tr = _TRACES.get(key)
     if tr is None:
         if wg:
@@ -217,7 +223,7 @@ def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None):
             tr.src = _LANE_STUBS + tr.src
             tr.has_user_tgrad = fun.tgrad is not None
         else:
-            tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad)
+            tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad, isout=isoutofdomain)
         _TRACES[key] = tr
     flags = 0
     if wg:
@@ -299,11 +305,12 @@ def _is_torch(x):
 def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abstol=None, saveat=None,
                  save_start=False, save_end=True, save_everystep=False, save_end_user=-1, max_saves=None,
                  dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None, nvec=None,
-                 tstops=None):
+                 tstops=None, isoutofdomain=None):
     """Bulk entry point on packed SoA arrays: u0 (nmax, N), p (np, N), t0/tf (N,);
     nvec (N,) gives per-trajectory dimensions for uneven batches (Rodas5P);
     tstops: stopping times shared by all trajectories (each keeps those strictly
-    inside its own (t0, tf)).
+    inside its own (t0, tf)); isoutofdomain(u, p, t): traced predicate that
+    rejects a step whose end state it flags (the next dt is dt * qmin).
 
     Arrays may be numpy (host) or torch CUDA tensors (device-resident, zero
     copies).  Returns a dict of output arrays of the same kind."""
@@ -320,7 +327,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         ns = sorted(set(int(x) for x in nv_host))
         if ns[-1] > n or ns[0] < 1:
             raise ValueError("nvec entries must lie in [1, u0.shape[0]]")
-    h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx, ns=ns)
+    h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx, ns=ns, isoutofdomain=isoutofdomain)
     reltol = 1e-3 if reltol is None else float(reltol)
     abstol = 1e-6 if abstol is None else float(abstol)
     sa = np.zeros(0) if saveat is None else np.asarray(saveat, dtype=np.float64)
@@ -413,7 +420,8 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
 
 def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=None, saveat=None,
           save_start=None, save_end=None, save_everystep=None, dt=None, dtmax=None, dtmin=0.0,
-          maxiters=1_000_000, seed=0, batch_size=None, max_saves=None, tstops=None, **kw):
+          maxiters=1_000_000, seed=0, batch_size=None, max_saves=None, tstops=None, isoutofdomain=None,
+          **kw):
     """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...) -> EnsembleSolution.
 
     A plain ODEProblem is solved as a one-trajectory ensemble and returns its ODESolution."""
@@ -458,7 +466,8 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=No
         max_saves = 4096
     o = solve_arrays(base.f, alg, u0, pp, t0, tf, ea, reltol=reltol, abstol=abstol, saveat=sa, save_start=ss,
                      save_end=se, save_everystep=sev, save_end_user=seu, max_saves=max_saves, dt=dt,
-                     dtmax=dtmax, dtmin=dtmin, maxiters=maxiters, tstops=tstops)
+                     dtmax=dtmax, dtmin=dtmin, maxiters=maxiters, tstops=tstops,
+                     isoutofdomain=isoutofdomain)
     if sev and int(o["nsaved"].max(initial=0)) > o["max_saves"]:
         raise RuntimeError(f"save_everystep produced more than max_saves={o['max_saves']} points; "
                            "pass a larger max_saves")

@@ -20,7 +20,7 @@ import math
 import struct
 
 __all__ = ["Sym", "trace_problem", "TracedProblem", "sym_exp", "sym_log", "sym_sin", "sym_cos",
-           "sym_sqrt", "sym_tanh", "sym_abs"]
+           "sym_sqrt", "sym_tanh", "sym_abs", "sym_any", "sym_all"]
 
 
 class _Graph:
@@ -113,10 +113,21 @@ class Sym:
 
     def __abs__(self): return self.g.make("abs", (self,))
 
-    # comparisons are not traceable (data-dependent control flow)
+    # comparisons build predicates (0/1 values) for isoutofdomain; branching on
+    # a traced value is not traceable (data-dependent control flow)
+    def __lt__(self, o): return self.g.make("lt", (self, _lift(o)))
+    def __le__(self, o): return self.g.make("le", (self, _lift(o)))
+    def __gt__(self, o): return self.g.make("gt", (self, _lift(o)))
+    def __ge__(self, o): return self.g.make("ge", (self, _lift(o)))
+    def __or__(self, o): return self.g.make("or", (self, _lift(o)))
+    def __ror__(self, o): return self.g.make("or", (_lift(o), self))
+    def __and__(self, o): return self.g.make("and", (self, _lift(o)))
+    def __rand__(self, o): return self.g.make("and", (_lift(o), self))
+    def __invert__(self): return self.g.make("not", (self,))
+
     def __bool__(self):
         raise TypeError("data-dependent control flow on a traced value is not supported; "
-                        "use ifelse-free arithmetic")
+                        "use ifelse-free arithmetic (and odemi.sym_any / sym_all for predicates)")
 
     # math methods (numpy ufunc dispatch on object operands calls these) ------
     def _fn(self, name):
@@ -161,6 +172,28 @@ def sym_cos(x): return _lift(x).cos() if isinstance(x, Sym) else math.cos(x)
 def sym_sqrt(x): return _lift(x).sqrt() if isinstance(x, Sym) else math.sqrt(x)
 def sym_tanh(x): return _lift(x).tanh() if isinstance(x, Sym) else math.tanh(x)
 def sym_abs(x): return abs(x)
+
+
+def sym_any(xs):
+    """Traceable any(): the `or` of predicates (plain bools on plain values)."""
+    xs = list(xs)
+    if not any(isinstance(x, Sym) for x in xs):
+        return any(bool(x) for x in xs)
+    r = _lift(0.0)
+    for x in xs:
+        r = r | x
+    return r
+
+
+def sym_all(xs):
+    """Traceable all(): the `and` of predicates."""
+    xs = list(xs)
+    if not any(isinstance(x, Sym) for x in xs):
+        return all(bool(x) for x in xs)
+    r = _lift(1.0)
+    for x in xs:
+        r = r & x
+    return r
 
 
 class _Recorder:
@@ -302,8 +335,12 @@ _FN = {"exp": "exp", "log": "log", "sin": "sin", "cos": "cos", "tan": "tan", "sq
        "tanh": "tanh", "atan": "atan", "abs": "fabs"}
 
 
-def _emit_body(outputs, out_name, indent="    "):
-    """outputs: list of (lvalue_index_expr, Sym).  Emits temporaries in DAG order."""
+_CMP = {"lt": "<", "le": "<=", "gt": ">", "ge": ">="}
+
+
+def _emit_body(outputs, out_name, indent="    ", pname="p[{}]"):
+    """outputs: list of (lvalue_index_expr, Sym).  Emits temporaries in DAG order.
+    Predicates (comparisons, and/or/not) are emitted as T(0) / T(1)."""
     needed = set()
     stack = [s for _, s in outputs]
     while stack:
@@ -328,7 +365,7 @@ def _emit_body(outputs, out_name, indent="    "):
             names[n.id] = f"u[{n.val}]"
             continue
         if op == "p":
-            names[n.id] = f"p[{n.val}]"
+            names[n.id] = pname.format(n.val)
             continue
         if op == "t":
             names[n.id] = "t"
@@ -341,6 +378,10 @@ def _emit_body(outputs, out_name, indent="    "):
         elif op == "neg": e = f"-{a[0]}"
         elif op == "pow": e = f"pow({a[0]}, {a[1]})"
         elif op == "sign": e = f"T(({a[0]} > T(0)) - ({a[0]} < T(0)))"
+        elif op in _CMP: e = f"T(({a[0]} {_CMP[op]} {a[1]}) ? 1 : 0)"
+        elif op == "or": e = f"T((({a[0]} != T(0)) || ({a[1]} != T(0))) ? 1 : 0)"
+        elif op == "and": e = f"T((({a[0]} != T(0)) && ({a[1]} != T(0))) ? 1 : 0)"
+        elif op == "not": e = f"T(({a[0]} == T(0)) ? 1 : 0)"
         else: e = f"{_FN[op]}({a[0]})"
         nm = f"v{n.id}"
         lines.append(f"{indent}const T {nm} = {e};")
@@ -375,8 +416,10 @@ def _count_flops(outputs):
     return cnt
 
 
-def trace_problem(f, n, np_, jac=None, tgrad=None):
-    """Trace f (and jac/tgrad when given) for state dimension n and np_ params."""
+def trace_problem(f, n, np_, jac=None, tgrad=None, isout=None):
+    """Trace f (and jac/tgrad when given) for state dimension n and np_ params;
+    isout(u, p, t) is the isoutofdomain predicate (solve kwarg), traced into
+    odemi_isout."""
     global _CUR
     g = _Graph()
     _CUR = g
@@ -430,6 +473,17 @@ def trace_problem(f, n, np_, jac=None, tgrad=None):
                 src.append(f"    dT[{i}] = T(0);")
         src.append(_emit_body(tz, "dT"))
         src.append("}")
+        if isout is not None:
+            r = isout(u, p, t)
+            r = r if isinstance(r, Sym) else _lift(1.0 if r else 0.0)
+            src.append("#define ODEMI_HAS_ISOUT 1")
+            src.append("template <typename T, typename Q>\n__device__ __forceinline__ bool odemi_isout("
+                       "const T* __restrict__ u, const Q* __restrict__ p, T t) {")
+            src.append("    (void)u; (void)p; (void)t;")
+            src.append("    T r_[1];")
+            src.append(_emit_body([(0, r)], "r_", pname="T(p[{}])"))
+            src.append("    return r_[0] != T(0);")
+            src.append("}")
         return TracedProblem(n, np_, "\n".join(src) + "\n", has_user_jac, tgrad is not None, autonomous,
                              _count_flops(fout), _count_flops(jout))
     finally:

@@ -172,6 +172,50 @@ def test_tstops_workgroup_and_mixed():
     assert np.median(err.max(axis=0)) < 10.0
 
 
+def _lin(du, u, p, t):
+    du[0] = p[0] * u[0]
+
+
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()], ids=["tsit5", "rodas5p"])
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_isoutofdomain_bit_identical(alg, engine):
+    """isoutofdomain through the lane engines, bit-identical to the CPU
+    restatement: (a) decay u' = lambda u with a loose abstol and u < 0 flagged
+    (Rodas5P leaves the domain without it), (b) Lorenz with y > 8 flagged, where
+    trajectories that would cross stall with the reference's Unstable code."""
+    N = 96
+    rng = np.random.default_rng(7)
+    lam = rng.uniform(-50.0, -10.0, N)
+    print(f"[case] isoutofdomain {alg.code} {engine}", flush=True)
+    ea = odemi.EnsembleMI355X(engine=engine)
+    out = odemi.solve_arrays(_lin, alg, np.ones((1, N)), lam[None, :], np.zeros(N), np.full(N, 5.0), ea,
+                             reltol=1e-3, abstol=0.1, save_everystep=True, save_start=True, max_saves=4096,
+                             isoutofdomain=lambda u, p, t: u[0] < 0.0)
+    for j in range(0, N, 7):
+        r = O.solve(lambda du, u, p, t: du.__setitem__(0, p[0] * u[0]), [1.0], [lam[j]], (0.0, 5.0),
+                    jac=lambda J, u, p, t: J.__setitem__((0, 0), p[0]), alg=alg.code, reltol=1e-3,
+                    abstol=0.1, save_everystep=True, isoutofdomain=lambda u, p, t: u[0] < 0.0)
+        ns = int(out["nsaved"][j])
+        assert ns == r["nsaved"] and int(out["retcode"][j]) == r["retcode"] == 1
+        assert np.array_equal(out["t_out"][:ns, j], r["t"]) and np.array_equal(out["u_out"][:ns, 0, j], r["u"][:, 0])
+        assert (out["u_out"][:ns, 0, j] >= 0).all()
+        st = out["stats"][:, j]
+        assert (st[0], st[1], st[2]) == (r["nf"], r["naccept"], r["nreject"])
+    d = P.lorenz_small(N=64)
+    out = odemi.solve_arrays(P.LORENZ, alg, d["u0"], d["p"], d["t0"], d["tf"], ea, reltol=1e-6, abstol=1e-6,
+                             isoutofdomain=lambda u, p, t: u[1] > 8.0)
+    codes = set()
+    for j in range(0, 64, 3):
+        r = O.solve_builtin(O.PROB_LORENZ, d["u0"][:, j], d["p"][:, j], (0.0, 1.0), alg=alg.code, reltol=1e-6,
+                            abstol=1e-6, save_start=False, isoutofdomain=lambda u, p, t: u[1] > 8.0)
+        assert int(out["retcode"][j]) == r["retcode"]
+        assert np.array_equal(out["u_final"][:, j], r["u_final"])
+        st = out["stats"][:, j]
+        assert (st[0], st[1], st[2]) == (r["nf"], r["naccept"], r["nreject"])
+        codes.add(r["retcode"])
+    assert codes == {1, 7}   # some trajectories cross y = 8 and stall, some never reach it
+
+
 def _expb(du, u, p, t):
     du[0] = odemi.sym_exp(u[0])
 

@@ -335,6 +335,27 @@ def test_tstops_reference_known_answers():
     assert r["t"][1] <= 1e-4
 
 
+def test_isoutofdomain_rejects_and_keeps_the_domain():
+    """isoutofdomain (integrator_utils.jl:612-614, 135-136): a flagged step is
+    rejected and the next dt is dt * qmin.  Rodas5P on u' = -30 u with a loose
+    abstol goes below zero without it; with u < 0 flagged every saved state
+    stays >= 0 at the price of rejections; a predicate that is never true
+    changes nothing; one that is always true past t = 0.5 stalls the
+    integrator there with the reference's Unstable code (dt shrinks by qmin
+    each attempt until dt <= eps(t))."""
+    lin = lambda du, u, p, t: du.__setitem__(0, p[0] * u[0])
+    kw = dict(tspan=(0.0, 10.0), p=[-30.0], save_everystep=True, reltol=1e-3, abstol=0.1, alg=O.RODAS5P)
+    r0 = O.solve(lin, [1.0], **kw)
+    assert r0["u"].min() < 0
+    r1 = O.solve(lin, [1.0], isoutofdomain=lambda u, p, t: u[0] < 0, **kw)
+    assert r1["retcode"] == 1 and r1["u"].min() >= 0 and r1["nreject"] > r0["nreject"]   # Success
+    r2 = O.solve(lin, [1.0], isoutofdomain=lambda u, p, t: False, **kw)
+    assert np.array_equal(r2["u"], r0["u"]) and r2["naccept"] == r0["naccept"] and r2["nreject"] == r0["nreject"]
+    for alg in (O.TSIT5, O.RODAS5P):
+        r3 = O.solve(lin, [1.0], tspan=(0.0, 1.0), p=[-1.0], alg=alg, isoutofdomain=lambda u, p, t: t > 0.5)
+        assert r3["retcode"] == 7 and 0.4 < r3["t_final"] <= 0.5   # Unstable
+
+
 def test_saveat_interpolant_accuracy():
     # dense output (Tsit5 free 4th-order interpolant, Rodas 3rd-order H-rows) vs exp
     sa = list(np.linspace(0.05, 0.95, 19))
