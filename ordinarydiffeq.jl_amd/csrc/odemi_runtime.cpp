@@ -283,6 +283,8 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     if (flags & ODEMI_COMPILE_WG) s << "#define ODEMI_WITH_WG 1\n";
     s << problem_src << "\n";
     s << kDeviceSource << "\n";
+    // layout check of the kernel argument struct against this file's mirror
+    s << "extern \"C\" __device__ const unsigned odemi_params_bytes = sizeof(odemi::Params);\n";
     if (flags & ODEMI_COMPILE_TSIT5) {
         // persistent kernels: final-state-only (no save path) and saving variants
         const char* v[4][3] = {{"odemi_tsit5_persist_f64", "false", "double"},
@@ -454,6 +456,20 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         ctx->err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
         delete r;
         return nullptr;
+    }
+    {
+        // the kernels take Params by value: its two declarations must agree
+        hipDeviceptr_t gp = nullptr;
+        size_t gsz = 0;
+        unsigned dev_bytes = 0;
+        if (hipModuleGetGlobal(&gp, &gsz, r->mod, "odemi_params_bytes") != hipSuccess || gsz != sizeof(unsigned) ||
+            hipMemcpyDtoH(&dev_bytes, gp, sizeof(unsigned)) != hipSuccess || dev_bytes != sizeof(Params)) {
+            ctx->err = "kernel Params layout differs from the runtime's (device " + std::to_string(dev_bytes) +
+                       " bytes, host " + std::to_string(sizeof(Params)) + ")";
+            (void)hipModuleUnload(r->mod);
+            delete r;
+            return nullptr;
+        }
     }
     auto get = [&](const char* name, hipFunction_t* fn) {
         if (hipModuleGetFunction(fn, r->mod, name) != hipSuccess) *fn = nullptr;
