@@ -21,7 +21,7 @@
  *   retcode[j]              <- `sol.retcode` (SciMLBase.ReturnCode order;
  *                              lib/DiffEqBase/src/check_error.jl:69-118)
  *   stats[k*N + j]          <- `sol.stats` (DEStats: nf, naccept, nreject, njacs,
- *                              nw, nsolve; lib/DiffEqBase/src/stats.jl:1-60)
+ *                              nw, nsolve, ncondition; lib/DiffEqBase/src/stats.jl:1-60)
  *   odemi_serve_*           <- no counterpart in the reference: a continuous-batching
  *                              request loop for many concurrent ensemble solves
  *
@@ -63,6 +63,7 @@ typedef struct odemi_rhs odemi_rhs;
 /* SciMLBase.ReturnCode values used per trajectory */
 #define ODEMI_RC_DEFAULT 0
 #define ODEMI_RC_SUCCESS 1
+#define ODEMI_RC_TERMINATED 2     /* a discrete callback's affect called terminate! */
 #define ODEMI_RC_DTNAN 3
 #define ODEMI_RC_MAXITERS 4
 #define ODEMI_RC_DTLESSTHANMIN 6
@@ -100,7 +101,7 @@ typedef struct {
     double* u_final;          /* [nmax][N] integrator state at exit (may be NULL) */
     double* t_final;          /* [N] (may be NULL) */
     int32_t* retcode;         /* [N] */
-    int32_t* stats;           /* [6][N]: nf, naccept, nreject, njacs, nw, nsolve (may be NULL) */
+    int32_t* stats;           /* [7][N]: nf, naccept, nreject, njacs, nw, nsolve, ncondition (may be NULL) */
     int refine_sweeps;        /* ODEMI_MIXED Rodas5P: FP64 refinement sweeps per stage solve (default 1) */
     int block;                /* threads per workgroup (0: engine default) */
     int blocks_per_cu;        /* persistent kernels: resident workgroups per CU (0: library default; capped by occupancy) */
@@ -123,6 +124,16 @@ int odemi_abi_version(void);
  *   template<typename T> __device__ void odemi_jac(T* J, const T* u, const T* p, T t); (row-major n x n)
  *   template<typename T> __device__ void odemi_tgrad(T* dT, const T* u, const T* p, T t);
  * and, for uneven batches, the workgroup forms odemi_f_wg / odemi_jac_wg.
+ * Optional, lane kernels only (Tsit5; Rodas5P without ODEMI_COMPILE_WG):
+ *   #define ODEMI_HAS_ISOUT 1 with
+ *   template<typename T, typename Q> __device__ bool odemi_isout(const T* u, const Q* p, T t);
+ *     <- the `isoutofdomain` solve option (integrator_utils.jl:612-614)
+ *   #define ODEMI_NDCB k with template<int K> struct OdemiDcb, K = 0..k-1, holding
+ *     static constexpr int save0, save1, term;
+ *     template<typename T, typename Q> static __device__ bool cond(const T* u, const Q* p, T t);
+ *     template<typename T, typename Q> static __device__ void affect(T* u, const Q* p, T t);
+ *     <- a CallbackSet of k DiscreteCallbacks (condition, affect! on u, save_positions,
+ *        terminate! when term; lib/DiffEqBase/src/callbacks.jl:645-683)
  * n/np are the (maximum) state and parameter counts.  Code objects are cached
  * on disk by source hash ($ODEMI_CACHE_DIR).  compile_seconds may be NULL. */
 odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src, int n, int np, int flags,
@@ -173,7 +184,7 @@ void odemi_server_destroy(odemi_server* server);
 /* Submit N trajectories (host SoA inputs, copied before return; u0 is [n][N]
  * with n the compiled dimension).  nvec [N] gives each trajectory's own
  * dimension for a workgroup (uneven-n Rodas5P) server, or NULL for n.  Outputs
- * are written at completion: u_final [n][N], retcode [N], stats [6][N] (stats
+ * are written at completion: u_final [n][N], retcode [N], stats [7][N] (stats
  * may be NULL).  Returns a request id (> 0) or a negative error. */
 int64_t odemi_submit(odemi_server* server, int64_t N, const int32_t* nvec, const double* u0, const double* p,
                      const double* t0, const double* tf, double* u_final, int32_t* retcode, int32_t* stats);

@@ -363,6 +363,7 @@ typedef struct {
     double t, dt, dtpropose, tprev, tstop_target, EEst, q11, errold, dtcache;
     int64_t iter, success_iter;
     int accept_step, next_step_tstop, isout;
+    int reeval_fsal;      /* a callback modified u: the next accept re-evaluates the FSAL stage */
     oo_result* res;
     /* vectors */
     double *u, *uprev, *tmp, *atmp;
@@ -708,9 +709,11 @@ static void rodas_interp(integ_t* I, double theta, double* out) {
     }
 }
 
-/* integrator_utils.jl:340-414 (savevalues!, save_idxs = nothing) */
-static void savevalues(integ_t* I) {
+/* integrator_utils.jl:340-414 (_savevalues!, save_idxs = nothing); returns
+ * savedexactly.  force_save: a callback's save of the current state. */
+static int savevalues(integ_t* I, int force_save) {
     const oo_opts* o = I->o;
+    int savedexactly = 0;
     while (I->saveat_pos < I->n_saveat && I->saveat_int[I->saveat_pos] <= I->t) {
         double curt = I->saveat_int[I->saveat_pos++];
         if (curt != I->t) {
@@ -720,16 +723,43 @@ static void savevalues(integ_t* I) {
             save_point(I, curt, I->tmp);
         } else {
             if (curt == I->tf && !o->save_end) continue;
+            savedexactly = 1;
             save_point(I, I->t, I->u);
         }
     }
-    if (o->save_everystep) {
-        int s = I->res->nsaved;
-        int empty = (s == 0);
-        double last = (!empty && s - 1 < I->max_out && I->t_out) ? I->t_out[s - 1] : NAN;
-        if (empty || ((I->t != last || I->dt == 0.0) && (o->save_end || I->t != I->tf)))
-            save_point(I, I->t, I->u);
+    int s = I->res->nsaved;
+    int empty = (s == 0);
+    double last = (!empty && s - 1 < I->max_out && I->t_out) ? I->t_out[s - 1] : NAN;
+    if (force_save ||
+        (o->save_everystep && (empty || ((I->t != last || I->dt == 0.0) && (o->save_end || I->t != I->tf))))) {
+        savedexactly = 1;
+        save_point(I, I->t, I->u);
     }
+    return savedexactly;
+}
+
+/* handle_callbacks! (integrator_utils.jl:1081-1132) for a CallbackSet of
+ * DiscreteCallbacks, each applied as apply_discrete_callback!
+ * (lib/DiffEqBase/src/callbacks.jl:645-683).  An affect! leaves
+ * derivative_discontinuity set, so reeval_internals_due_to_modification!
+ * (integrator_interface.jl:54-80) sets reeval_fsal.  Returns 1 when an affect!
+ * called terminate!. */
+static int handle_callbacks(integ_t* I) {
+    const oo_opts* o = I->o;
+    int saved_in_cb = 0, term = 0;
+    for (int k = 0; k < o->ndcb; k++) {
+        if (o->dcb_cond(I->u, I->p, I->t, k, o->dcb_user)) {
+            int savedexactly = savevalues(I, 0);
+            saved_in_cb = 1;
+            if (o->dcb_save[2 * k] && !savedexactly) savevalues(I, 1);
+            if (o->dcb_affect(I->u, I->p, I->t, k, o->dcb_user)) term = 1;
+            I->reeval_fsal = 1;
+            if (o->dcb_save[2 * k + 1]) { savevalues(I, 1); saved_in_cb = 1; }
+        }
+        I->res->ncondition++;
+    }
+    if (!saved_in_cb) savevalues(I, 0);
+    return term;
 }
 
 static int cmp_double(const void* a, const void* b) {
@@ -828,7 +858,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     }
     res->dt_init = I->dt;
 
-    int rc = OO_RC_SUCCESS;
+    int rc = OO_RC_SUCCESS, terminated = 0;
     /* solve! (solve.jl:904-946): outer loop over the tstops queue, inner loop
      * of step attempts while t < first_tstop */
     while (I->tst_pos < I->n_tst && rc == OO_RC_SUCCESS) {
@@ -841,7 +871,13 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
                 /* apply_step! (175-203) */
                 memcpy(I->uprev, I->u, sizeof(double) * n);
                 I->dt = I->dtpropose;
-                if (I->alg == OO_TSIT5) memcpy(I->k[0], I->k[6], sizeof(double) * n);
+                /* update_fsal! (215-239): reset_fsal! (1325-1338) after a callback
+                 * modified u, else the FSAL copy; Rodas5P is not FSAL */
+                if (I->alg == OO_TSIT5) {
+                    if (I->reeval_fsal) F(I, I->k[0], I->u, I->t);
+                    else memcpy(I->k[0], I->k[6], sizeof(double) * n);
+                }
+                I->reeval_fsal = 0;   /* loopfooter_reset! (720-725) */
                 modify_dt_for_tstops(I);
             } else if (I->isout) {
                 I->dt = I->dt * I->qmin;   /* handle_step_rejection! (integrator_utils.jl:135-136) */
@@ -871,7 +907,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
             else I->t = ttmp;
             I->accept_step = 1;
             I->dtpropose = I->dt;
-            savevalues(I);
+            if (handle_callbacks(I)) { terminated = 1; I->tst_pos = I->n_tst; break; }
             if (should_exit) break;
             continue;
         }
@@ -886,7 +922,8 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
             else I->t = ttmp;
             double dtnew = step_accept_controller(I, q);
             calc_dt_propose(I, dtnew);
-            savevalues(I);
+            /* terminate! empties the tstops queue, which ends both solve! loops */
+            if (handle_callbacks(I)) { terminated = 1; I->tst_pos = I->n_tst; break; }
         } else {
             res->nreject++;
         }
@@ -896,7 +933,8 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
       handle_tstop(I);
     }
     if (rc == OO_RC_SUCCESS) {
-        /* postamble! -> solution_endpoint_match_cur_integrator! (493-588) */
+        /* postamble! -> solution_endpoint_match_cur_integrator! (493-588), after
+         * terminate! as well */
         if (opts->save_end) {
             int s = res->nsaved;
             int doit = (s == 0);
@@ -910,6 +948,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
             if (doit) save_point(I, I->t, I->u);
         }
     }
+    if (rc == OO_RC_SUCCESS && terminated) rc = OO_RC_TERMINATED;
     res->retcode = rc;
     res->t_final = I->t;
     res->dt_last = I->dt;

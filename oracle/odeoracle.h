@@ -30,6 +30,11 @@ typedef void (*oo_rhs_fn)(double* du, const double* u, const double* p, double t
 /* J is row-major n x n: J[i*n + j] = d f_i / d u_j */
 typedef void (*oo_jac_fn)(double* J, const double* u, const double* p, double t, void* user);
 typedef void (*oo_tgrad_fn)(double* dT, const double* u, const double* p, double t, void* user);
+/* DiscreteCallback k of a CallbackSet (lib/DiffEqBase/src/callbacks.jl:645-683):
+ * condition(u, t) after an accepted step; affect! rewrites u in place and
+ * returns nonzero when it calls terminate! (integrator_interface.jl:443-446). */
+typedef int (*oo_dcb_cond_fn)(const double* u, const double* p, double t, int k, void* user);
+typedef int (*oo_dcb_affect_fn)(double* u, const double* p, double t, int k, void* user);
 
 typedef struct {
     int alg;              /* OO_TSIT5 / OO_RODAS5P */
@@ -50,6 +55,11 @@ typedef struct {
      * rejects the step; the next dt is dt * qmin (integrator_utils.jl:135-136).  NULL: never. */
     int (*isoutofdomain)(const double* u, const double* p, double t, void* user);
     void* isout_user;
+    int ndcb;                    /* discrete callbacks, applied in this order (0: none) */
+    oo_dcb_cond_fn dcb_cond;
+    oo_dcb_affect_fn dcb_affect;
+    const int* dcb_save;         /* save_positions of callback k: dcb_save[2k], dcb_save[2k+1] */
+    void* dcb_user;
 } oo_opts;
 
 typedef struct {
@@ -59,6 +69,7 @@ typedef struct {
     double t_final;       /* integrator t at exit */
     double dt_last;       /* integrator dt at exit */
     double dt_init;       /* dt after handle_dt! (automatic initial step) */
+    int64_t ncondition;   /* callback condition calls (stats.ncondition) */
 } oo_result;
 
 void oo_default_opts(oo_opts* o, int alg);

@@ -17,6 +17,8 @@ PROB_USER, PROB_LORENZ, PROB_ROBERTSON, PROB_CHAIN, PROB_LINEAR, PROB_EXPBLOWUP 
 RC_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN", 4: "MaxIters",
             5: "MaxNumSub", 6: "DtLessThanMin", 7: "Unstable"}
 
+DCB_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                          ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
 ISOUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                             ctypes.c_double, ctypes.c_void_p)
 RHS_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -31,14 +33,16 @@ class Opts(ctypes.Structure):
                 ("saveat", ctypes.POINTER(ctypes.c_double)), ("nsaveat", ctypes.c_int),
                 ("has_tgrad", ctypes.c_int), ("adaptive", ctypes.c_int),
                 ("tstops", ctypes.POINTER(ctypes.c_double)), ("ntstops", ctypes.c_int),
-                ("isoutofdomain", ctypes.c_void_p), ("isout_user", ctypes.c_void_p)]
+                ("isoutofdomain", ctypes.c_void_p), ("isout_user", ctypes.c_void_p),
+                ("ndcb", ctypes.c_int), ("dcb_cond", ctypes.c_void_p), ("dcb_affect", ctypes.c_void_p),
+                ("dcb_save", ctypes.POINTER(ctypes.c_int)), ("dcb_user", ctypes.c_void_p)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("retcode", ctypes.c_int), ("nf", ctypes.c_int64), ("naccept", ctypes.c_int64),
                 ("nreject", ctypes.c_int64), ("njacs", ctypes.c_int64), ("nw", ctypes.c_int64),
                 ("nsolve", ctypes.c_int64), ("nsaved", ctypes.c_int), ("t_final", ctypes.c_double),
-                ("dt_last", ctypes.c_double), ("dt_init", ctypes.c_double)]
+                ("dt_last", ctypes.c_double), ("dt_init", ctypes.c_double), ("ncondition", ctypes.c_int64)]
 
 
 def build():
@@ -94,9 +98,24 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+class _CallbackIntegrator:
+    """The integrator a callback sees in the oracle: u (numpy, writable or
+    reassignable), p, t, terminate()."""
+
+    def __init__(self, u, p, t):
+        self.u, self.p, self.t, self._term = u, p, t, False
+
+    def terminate(self):
+        self._term = True
+
+
 def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0, maxiters=1000000,
               save_start=True, save_end=True, save_everystep=False, save_end_user=None, saveat=None,
-              has_tgrad=False, adaptive=True, tstops=None, isoutofdomain=None, isout_shape=(0, 0)):
+              has_tgrad=False, adaptive=True, tstops=None, isoutofdomain=None, isout_shape=(0, 0),
+              callbacks=None):
+    """callbacks: objects with condition(u, t, integrator), affect(integrator)
+    and save_positions (DiscreteCallback's fields), run as plain Python on numpy
+    copies of the state."""
     o = Opts()
     lib().oo_default_opts(ctypes.byref(o), alg)
     o.reltol, o.abstol = reltol, abstol
@@ -125,6 +144,31 @@ def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0
         cb = ISOUT_FN(_io)
         o.isoutofdomain = ctypes.cast(cb, ctypes.c_void_p)
         keep = (keep, cb)
+    if callbacks:
+        n, np_ = isout_shape
+        cbl = list(callbacks)
+
+        def _cond(u, p, t, k, _user):
+            uu = np.array(np.ctypeslib.as_array(u, shape=(max(n, 1),))[:n])
+            pp = np.array(np.ctypeslib.as_array(p, shape=(max(np_, 1),))[:np_])
+            return 1 if cbl[k].condition(uu, t, _CallbackIntegrator(uu.copy(), pp, t)) else 0
+
+        def _affect(u, p, t, k, _user):
+            ua = np.ctypeslib.as_array(u, shape=(max(n, 1),))
+            pp = np.array(np.ctypeslib.as_array(p, shape=(max(np_, 1),))[:np_])
+            itg = _CallbackIntegrator(np.array(ua[:n]), pp, t)
+            cbl[k].affect(itg)
+            new = np.asarray(itg.u, dtype=np.float64).reshape(-1)
+            if len(new) != n:
+                raise ValueError("affect! must keep the state length")
+            ua[:n] = new
+            return 1 if itg._term else 0
+        cc, ca = DCB_FN(_cond), DCB_FN(_affect)
+        sv = np.ascontiguousarray([int(bool(x)) for c in cbl for x in c.save_positions], dtype=np.int32)
+        o.ndcb = len(cbl)
+        o.dcb_cond, o.dcb_affect = ctypes.cast(cc, ctypes.c_void_p), ctypes.cast(ca, ctypes.c_void_p)
+        o.dcb_save = sv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        keep = (keep, cc, ca, sv)
     return o, keep
 
 
@@ -172,7 +216,8 @@ def solve(f, u0, p, tspan, jac=None, tgrad=None, max_out=4096, **kw):
     ns = min(res.nsaved, max_out)
     return dict(t=t_out[:ns].copy(), u=u_out[:ns, :n].copy(), u_final=uf[:n].copy(), retcode=res.retcode,
                 nf=res.nf, naccept=res.naccept, nreject=res.nreject, njacs=res.njacs, nw=res.nw,
-                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved, dt_init=res.dt_init)
+                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved, dt_init=res.dt_init,
+                ncondition=res.ncondition)
 
 
 def solve_builtin(prob, u0, p, tspan, max_out=4096, **kw):

@@ -36,10 +36,10 @@ namespace odemi {
 typedef long long i64;
 typedef unsigned long long u64;
 
-enum { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_DTNAN = 3, RC_MAXITERS = 4, RC_DTLESSTHANMIN = 6,
-       RC_UNSTABLE = 7 };
+enum { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_TERMINATED = 2, RC_DTNAN = 3, RC_MAXITERS = 4,
+       RC_DTLESSTHANMIN = 6, RC_UNSTABLE = 7 };
 enum { ST_NF = 0, ST_NACCEPT = 1, ST_NREJECT = 2, ST_NJACS = 3, ST_NW = 4, ST_NSOLVE = 5,
-       ST_COUNT = 6 };
+       ST_NCONDITION = 6, ST_COUNT = 7 };
 
 // Kernel argument block (one struct, passed by value).  All arrays are SoA over
 // the N trajectories of the batch: x[i * N + j] is component i of trajectory j.
@@ -336,6 +336,9 @@ struct Ctl {
     // inside (t0, tf) are P.tstops[tsp..tsend), then tf; tnext = the first one
     int tsp, tsend;
     double tnext;
+    // discrete callbacks: a callback modified u after the last accepted step
+    // (integrator.reeval_fsal), and the condition calls so far (stats.ncondition)
+    int reeval_fsal, ncond;
 };
 
 // controller constants (alg_utils.jl:407-879; controllers.jl:744-803)
@@ -479,6 +482,50 @@ __device__ __forceinline__ void save_point(Ctl& c, const Params& P, i64 j, doubl
     c.nsaved = s + 1;
 }
 
+// handle_callbacks! (integrator_utils.jl:1081-1132) for a CallbackSet of
+// DiscreteCallbacks, after an accepted step's dt proposal.  Each callback is
+// applied as apply_discrete_callback! does (lib/DiffEqBase/src/callbacks.jl:
+// 645-683): when its condition holds at (u, t), savevalues! (saveat points,
+// save_everystep), a forced save of the pre-affect state unless savevalues!
+// saved at exactly t, affect!, then a forced save of the new state; the
+// modified state makes the next accept re-evaluate the FSAL stage
+// (reeval_internals_due_to_modification!).  Every condition call counts in
+// stats.ncondition.  `savevalues` is the method's savevalues! and returns
+// savedexactly.  The result is true when an affect! called terminate!.
+#ifdef ODEMI_NDCB
+template <int K, int N, typename S, typename Q, typename SaveFn>
+__device__ __forceinline__ void dcb_apply(Ctl& c, const Params& P, i64 j, S* u, const Q* p, SaveFn& savevalues,
+                                          bool& saved_in_cb, bool& term) {
+    if constexpr (K < ODEMI_NDCB) {
+        c.ncond++;
+        if (OdemiDcb<K>::template cond<S, Q>(u, p, (S)c.t)) {
+            const bool savedexactly = savevalues();
+            saved_in_cb = true;
+            if (OdemiDcb<K>::save0 && !savedexactly) save_point<N>(c, P, j, c.t, u);
+            OdemiDcb<K>::template affect<S, Q>(u, p, (S)c.t);
+            c.reeval_fsal = 1;
+            if (OdemiDcb<K>::term) term = true;   // terminate!: the tstops queue is emptied
+            if (OdemiDcb<K>::save1) save_point<N>(c, P, j, c.t, u);
+        }
+        dcb_apply<K + 1, N>(c, P, j, u, p, savevalues, saved_in_cb, term);
+    }
+}
+#endif
+template <int N, typename S, typename Q, typename SaveFn>
+__device__ __forceinline__ bool handle_callbacks(Ctl& c, const Params& P, i64 j, S* u, const Q* p,
+                                                 SaveFn&& savevalues) {
+#ifdef ODEMI_NDCB
+    bool saved_in_cb = false, term = false;
+    dcb_apply<0, N>(c, P, j, u, p, savevalues, saved_in_cb, term);
+    if (!saved_in_cb) savevalues();
+    return term;
+#else
+    (void)c; (void)P; (void)j; (void)u; (void)p;
+    savevalues();
+    return false;
+#endif
+}
+
 // RMS norm of residuals, FP64 accumulation (calculate_residuals.jl:9-14 +
 // common_defaults.jl:69-75)
 template <int N, typename E, typename S>
@@ -566,6 +613,7 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     c.dtmax = (P.dtmax > 0.0) ? P.dtmax : (tf - t0);
     c.EEst = 1.0; c.q11 = 1.0; c.errold = 1e-4;
     c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0; c.isout = 0;
+    c.reeval_fsal = 0; c.ncond = 0;
     c.dt = 0.0; c.dtpropose = 0.0;
     c.nf = 0; c.naccept = 0; c.nreject = 0; c.njacs = 0; c.nw = 0; c.nsolve = 0;
     c.nsaved = 0; c.spos = 0; c.send = 0; c.tf_in_saveat = 0;
@@ -578,7 +626,7 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
 // result write-out.
 template <int N, typename S>
 __device__ void finish_lane(Ctl& c, const Params& P, i64 j, const S* u, int rc) {
-    if (rc == RC_SUCCESS && P.save_end) {
+    if ((rc == RC_SUCCESS || rc == RC_TERMINATED) && P.save_end) {   // postamble! runs after terminate! too
         bool doit = (c.nsaved == 0);
         if (!doit) {
             int s = c.nsaved - 1;
@@ -601,6 +649,7 @@ __device__ void finish_lane(Ctl& c, const Params& P, i64 j, const S* u, int rc) 
         P.stats[ST_NJACS * P.N + j] = c.njacs;
         P.stats[ST_NW * P.N + j] = c.nw;
         P.stats[ST_NSOLVE * P.N + j] = c.nsolve;
+        P.stats[ST_NCONDITION * P.N + j] = c.ncond;
     }
 }
 
@@ -852,10 +901,11 @@ __device__ __forceinline__ void tsit5_interp(const Tsit5Lane<S, K, N>& L, double
     }
 }
 
-// savevalues! (integrator_utils.jl:340-414)
+// savevalues! (integrator_utils.jl:340-414); returns savedexactly.
 template <typename S, typename K, int N>
-__device__ __forceinline__ void tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+__device__ __forceinline__ bool tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
     Ctl& c = L.c;
+    bool exact = false;
     while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
         double curt = P.saveat[c.spos++];
         if (curt != c.t) {
@@ -866,14 +916,18 @@ __device__ __forceinline__ void tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Pa
         } else {
             if (curt == c.tf && !P.save_end) continue;
             save_point<N>(c, P, j, c.t, L.u);
+            exact = true;
         }
     }
     if (P.save_everystep) {
         int s = c.nsaved;
         double last = (s > 0 && s - 1 < P.max_saves) ? P.t_out[(i64)(s - 1) * P.N + j] : __longlong_as_double(0x7ff8000000000000LL);
-        if (s == 0 || ((c.t != last || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
+        if (s == 0 || ((c.t != last || c.dt == 0.0) && (P.save_end || c.t != c.tf))) {
             save_point<N>(c, P, j, c.t, L.u);
+            exact = true;
+        }
     }
+    return exact;
 }
 
 // One step attempt: loopheader! + check_error! + perform_step! + loopfooter!.
@@ -888,6 +942,16 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
             c.success_iter++;
 #pragma unroll
             for (int i = 0; i < N; i++) { L.uprev[i] = L.u[i]; L.k1[i] = L.k7[i]; }
+#ifdef ODEMI_NDCB
+            if (c.reeval_fsal) {   // update_fsal! -> reset_fsal! (integrator_utils.jl:215-239,1325-1338)
+                K uk[N];
+#pragma unroll
+                for (int i = 0; i < N; i++) uk[i] = (K)L.u[i];
+                odemi_f<K>(L.k1, uk, L.p, (K)c.t);
+                c.nf += 1;
+                c.reeval_fsal = 0;
+            }
+#endif
             c.dt = c.dtpropose;
             apply_step_tstop_clip(c);
         } else {
@@ -919,7 +983,10 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
         else c.t = ttmp;
         handle_tstop(c, P);
         accept_controller<HW>(c, kc, q, P.dtmin);
-        if (SAVES && (P.nsaveat > 0 || P.save_everystep)) tsit5_savevalues(L, P, j);
+        const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, [&]() -> bool {
+            return (SAVES && (P.nsaveat > 0 || P.save_everystep)) ? tsit5_savevalues(L, P, j) : false;
+        });
+        if (term) return RC_TERMINATED;
     } else {
         c.nreject++;
     }
@@ -997,7 +1064,8 @@ __device__ void tsit5_persistent(const Params& P) {
 // Plain engine: one kernel launch per step attempt, state in HBM.
 // st_scalar rows: t dt dtpropose tprev (unused) tf dtmax EEst q11 errold
 // st_int rows: iter success_iter accept_step next_step_tstop nf naccept nreject done
-//               saveat cursor, njacs nw nsolve (Rodas5P), tstops cursor, isout
+//               saveat cursor, njacs nw nsolve (Rodas5P), tstops cursor,
+//               isout | reeval_fsal << 1, ncondition
 // st_vec rows: u uprev k1 k7 (Tsit5)
 template <int N>
 __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
@@ -1022,7 +1090,9 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
         c.tsp = q[12 * M + j];
         c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
     }
-    c.isout = q[13 * M + j];
+    const int fl = q[13 * M + j];
+    c.isout = fl & 1; c.reeval_fsal = (fl >> 1) & 1;
+    c.ncond = q[14 * M + j];
     c.njacs = 0; c.nw = 0; c.nsolve = 0;
 }
 template <int N>
@@ -1036,7 +1106,8 @@ __device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i
     q[0 * M + j] = c.iter; q[1 * M + j] = c.success_iter; q[2 * M + j] = c.accept_step;
     q[3 * M + j] = c.next_step_tstop; q[4 * M + j] = c.nf; q[5 * M + j] = c.naccept;
     q[6 * M + j] = c.nreject; q[7 * M + j] = done; q[8 * M + j] = c.spos; q[12 * M + j] = c.tsp;
-    q[13 * M + j] = c.isout;
+    q[13 * M + j] = c.isout | (c.reeval_fsal << 1);
+    q[14 * M + j] = c.ncond;
     if (P.nsaved) P.nsaved[j] = c.nsaved;
 }
 
@@ -1346,8 +1417,9 @@ __device__ __forceinline__ void rodas_perform_step(RodasLane<F, N>& L, const Par
 }
 
 template <typename F, int N>
-__device__ __forceinline__ void rodas_savevalues(RodasLane<F, N>& L, const Params& P, i64 j) {
+__device__ __forceinline__ bool rodas_savevalues(RodasLane<F, N>& L, const Params& P, i64 j) {
     Ctl& c = L.c;
+    bool exact = false;
     while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
         double curt = P.saveat[c.spos++];
         if (curt != c.t) {
@@ -1365,14 +1437,18 @@ __device__ __forceinline__ void rodas_savevalues(RodasLane<F, N>& L, const Param
         } else {
             if (curt == c.tf && !P.save_end) continue;
             save_point<N>(c, P, j, c.t, L.u);
+            exact = true;
         }
     }
     if (P.save_everystep) {
         int s = c.nsaved;
         double last = (s > 0 && s - 1 < P.max_saves) ? P.t_out[(i64)(s - 1) * P.N + j] : __longlong_as_double(0x7ff8000000000000LL);
-        if (s == 0 || ((c.t != last || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
+        if (s == 0 || ((c.t != last || c.dt == 0.0) && (P.save_end || c.t != c.tf))) {
             save_point<N>(c, P, j, c.t, L.u);
+            exact = true;
+        }
     }
+    return exact;
 }
 
 template <typename F, int N>
@@ -1416,7 +1492,10 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
         else c.t = ttmp;
         handle_tstop(c, P);
         accept_controller<HW>(c, kc, q, P.dtmin);
-        if (P.nsaveat > 0 || P.save_everystep) rodas_savevalues(L, P, j);
+        const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, [&]() -> bool {
+            return (P.nsaveat > 0 || P.save_everystep) ? rodas_savevalues(L, P, j) : false;
+        });
+        if (term) return RC_TERMINATED;
     } else {
         c.nreject++;
     }

@@ -927,6 +927,7 @@ __device__ __forceinline__ void wg_finish(const WGPtrs& S, const Params& P, i64 
             P.stats[ST_NJACS * P.N + j] = c.njacs;
             P.stats[ST_NW * P.N + j] = c.nw;
             P.stats[ST_NSOLVE * P.N + j] = c.nsolve;
+            P.stats[ST_NCONDITION * P.N + j] = c.ncond;
         }
     }
 }

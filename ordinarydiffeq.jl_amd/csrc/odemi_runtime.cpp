@@ -39,7 +39,9 @@ const char* kDeviceSource =
 // rows of the int32 integrator state of the plain engine and the serving pool
 // (iter, success_iter, accept, next_step_tstop, nf, naccept, nreject, slot
 // state, saveat cursor, njacs, nw, nsolve, tstops cursor, isoutofdomain verdict)
-constexpr size_t kIntRows = 14;
+constexpr size_t kIntRows = 15;
+// DEStats rows written per trajectory: nf naccept nreject njacs nw nsolve ncondition
+constexpr size_t kStatRows = 7;
 
 struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t N;
@@ -195,6 +197,7 @@ struct odemi_rhs {
     odemi_ctx* ctx = nullptr;
     hipModule_t mod = nullptr;
     int n = 0, np = 0, flags = 0;
+    int features = 0;   // 1: isoutofdomain, 2: discrete callbacks (lane kernels only)
     hipFunction_t tsit5_persist_f64 = nullptr, tsit5_persist_mixed = nullptr;
     hipFunction_t tsit5_persist_f64_sv = nullptr, tsit5_persist_mixed_sv = nullptr;
     hipFunction_t tsit5_plain_init = nullptr, tsit5_plain_step = nullptr;
@@ -216,7 +219,7 @@ struct odemi_rhs {
         }                                                                                        \
     } while (0)
 
-extern "C" int odemi_abi_version(void) { return 2; }
+extern "C" int odemi_abi_version(void) { return 3; }
 
 extern "C" odemi_ctx* odemi_create(int device) {
     odemi_ctx* c = new odemi_ctx();
@@ -285,6 +288,9 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     s << kDeviceSource << "\n";
     // layout check of the kernel argument struct against this file's mirror
     s << "extern \"C\" __device__ const unsigned odemi_params_bytes = sizeof(odemi::Params);\n";
+    // per-step hooks of the problem source, which only the lane kernels run
+    s << "extern \"C\" __device__ const int odemi_features = 0\n#ifdef ODEMI_HAS_ISOUT\n | 1\n#endif\n"
+         "#ifdef ODEMI_NDCB\n | 2\n#endif\n;\n";
     if (flags & ODEMI_COMPILE_TSIT5) {
         // persistent kernels: final-state-only (no save path) and saving variants
         const char* v[4][3] = {{"odemi_tsit5_persist_f64", "false", "double"},
@@ -470,6 +476,13 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
             delete r;
             return nullptr;
         }
+        if (hipModuleGetGlobal(&gp, &gsz, r->mod, "odemi_features") != hipSuccess || gsz != sizeof(int) ||
+            hipMemcpyDtoH(&r->features, gp, sizeof(int)) != hipSuccess) {
+            ctx->err = "module lacks the odemi_features word";
+            (void)hipModuleUnload(r->mod);
+            delete r;
+            return nullptr;
+        }
     }
     auto get = [&](const char* name, hipFunction_t* fn) {
         if (hipModuleGetFunction(fn, r->mod, name) != hipSuccess) *fn = nullptr;
@@ -609,6 +622,10 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     if (plain && mixed) { ctx->err = "the plain engine is FP64 only"; return -3; }
     if (!fn_main || (plain && !fn_init)) { ctx->err = "requested kernel was not compiled for this problem"; return -3; }
     if (a->nvec && !use_wg) { ctx->err = "uneven batches (nvec) need the workgroup Rodas5P kernels"; return -3; }
+    if (use_wg && rhs->features) {
+        ctx->err = "isoutofdomain and callbacks run in the lane kernels only (compile without ODEMI_COMPILE_WG)";
+        return -3;
+    }
 
     hipStream_t s = a->stream ? (hipStream_t)a->stream : ctx->stream;
     const bool dev = a->device_ptrs != 0;
@@ -659,7 +676,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         add(std::max(a->nsaveat, 1) * 8);
         add(std::max(a->ntstops, 1) * 8);
         add(N_ * std::max<size_t>(ms_, 1) * 8); add(N_ * std::max<size_t>(ms_, 1) * nmax * 8);
-        add(N_ * 4); add(N_ * nmax * 8); add(N_ * 8); add(N_ * 4); add(N_ * 6 * 4);
+        add(N_ * 4); add(N_ * nmax * 8); add(N_ * 8); add(N_ * 4); add(N_ * kStatRows * 4);
         if (a->nvec) add(N_ * 4);
     }
     if (ensure_ws(ctx, need)) return -1;
@@ -701,7 +718,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         P.u_final = (double*)ar.take(N_ * nmax * 8);
         P.t_final = (double*)ar.take(N_ * 8);
         P.retcode = (int*)ar.take(N_ * 4);
-        P.stats = (int*)ar.take(N_ * 6 * 4);
+        P.stats = (int*)ar.take(N_ * kStatRows * 4);
         if (a->nvec) {
             int* nv = (int*)ar.take(N_ * 4);
             HIPCHK(ctx, hipMemcpyAsync(nv, a->nvec, N_ * 4, hipMemcpyHostToDevice, s));
@@ -771,7 +788,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         if (a->u_final) HIPCHK(ctx, hipMemcpyAsync(a->u_final, P.u_final, N_ * nmax * 8, hipMemcpyDeviceToHost, s));
         if (a->t_final) HIPCHK(ctx, hipMemcpyAsync(a->t_final, P.t_final, N_ * 8, hipMemcpyDeviceToHost, s));
         HIPCHK(ctx, hipMemcpyAsync(a->retcode, P.retcode, N_ * 4, hipMemcpyDeviceToHost, s));
-        if (a->stats) HIPCHK(ctx, hipMemcpyAsync(a->stats, P.stats, N_ * 6 * 4, hipMemcpyDeviceToHost, s));
+        if (a->stats) HIPCHK(ctx, hipMemcpyAsync(a->stats, P.stats, N_ * kStatRows * 4, hipMemcpyDeviceToHost, s));
     }
     HIPCHK(ctx, hipStreamSynchronize(s));
     float ms = 0.f;
@@ -885,6 +902,10 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
         ctx->err = "the compiled problem has no serving kernels for this algorithm";
         return nullptr;
     }
+    if (wg && rhs->features) {
+        ctx->err = "isoutofdomain and callbacks run in the lane kernels only (compile without ODEMI_COMPILE_WG)";
+        return nullptr;
+    }
     (void)hipSetDevice(ctx->device);
     odemi_server* S = new odemi_server();
     S->ctx = ctx;
@@ -906,7 +927,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     }
     // bytes per slot: inputs, state, outputs
     const size_t per_slot = (size_t)(n + np + 2) * 8 + 16 * 8 + (size_t)S->rows * n * 8 + kIntRows * 4 +
-                            (size_t)n * 8 + 8 + 4 + 4 + 6 * 4 + 4 + (wg ? 4 : 0);
+                            (size_t)n * 8 + 8 + 4 + 4 + kStatRows * 4 + 4 + (wg ? 4 : 0);
     const size_t fixed = S->gj ? (size_t)S->wg_grid * n * n * 4 : 0;
     int64_t slots = opts->max_slots > 0 ? opts->max_slots : (1 << 20);
     if (opts->hbm_budget_bytes > 0)
@@ -919,7 +940,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     auto al = [&](size_t b) { size_t o = bytes; bytes += (b + 255) & ~size_t(255); return o; };
     size_t o_u0 = al(Sz * n * 8), o_p = al(Sz * np * 8), o_t0 = al(Sz * 8), o_tf = al(Sz * 8);
     size_t o_sc = al(16 * Sz * 8), o_vec = al((size_t)S->rows * n * Sz * 8), o_int = al(kIntRows * Sz * 4);
-    size_t o_uf = al(Sz * n * 8), o_tfin = al(Sz * 8), o_rc = al(Sz * 4), o_st = al(6 * Sz * 4), o_ns = al(Sz * 4);
+    size_t o_uf = al(Sz * n * 8), o_tfin = al(Sz * 8), o_rc = al(Sz * 4), o_st = al(kStatRows * Sz * 4), o_ns = al(Sz * 4);
     size_t o_sreq = al(Sz * 4), o_done = al((size_t)S->R * 8), o_cnt = al(64);
     size_t o_nvec = wg ? al(Sz * 4) : 0, o_jb = S->gj ? al((size_t)S->wg_grid * n * n * 4) : 0;
     if (hipMalloc(&S->mem, bytes) != hipSuccess) { ctx->err = "serving pool allocation failed"; delete S; return nullptr; }
@@ -1183,7 +1204,7 @@ static void server_loop(odemi_server* S) {
                 SCHK(hipMemcpy2DAsync(r->u_final, N * 8, (double*)S->P.u_final + s0, Sz * 8, N * 8, n, hipMemcpyDeviceToHost, S->stream));
             SCHK(hipMemcpyAsync(r->retcode, (int*)S->P.retcode + s0, N * 4, hipMemcpyDeviceToHost, S->stream));
             if (r->stats)
-                SCHK(hipMemcpy2DAsync(r->stats, N * 4, (int*)S->P.stats + s0, Sz * 4, N * 4, 6, hipMemcpyDeviceToHost, S->stream));
+                SCHK(hipMemcpy2DAsync(r->stats, N * 4, (int*)S->P.stats + s0, Sz * 4, N * 4, kStatRows, hipMemcpyDeviceToHost, S->stream));
         }
         if (!fin.empty()) SCHK(hipStreamSynchronize(S->stream));
         if (!fin.empty()) {

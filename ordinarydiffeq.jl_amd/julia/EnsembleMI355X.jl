@@ -271,7 +271,7 @@ function SciMLBase.__solve(ep::SciMLBase.EnsembleProblem, alg::Union{Tsit5,Rodas
     se = save_end === nothing ? true : save_end
     ms = save_everystep ? max_saves : length(sa) + 2
     t_out = zeros(N, ms); u_out = zeros(N, n, ms); nsaved = zeros(Int32, N)
-    u_final = zeros(N, n); t_final = zeros(N); retcode = zeros(Int32, N); stats = zeros(Int32, N, 6)
+    u_final = zeros(N, n); t_final = zeros(N); retcode = zeros(Int32, N); stats = zeros(Int32, N, 7)
     a = OdemiSolveArgs()
     odemi_default_args!(a, code)
     GC.@preserve u0 P t0 tf sa tst t_out u_out nsaved u_final t_final retcode stats begin
@@ -292,14 +292,17 @@ function SciMLBase.__solve(ep::SciMLBase.EnsembleProblem, alg::Union{Tsit5,Rodas
         rc == 0 || error("odemi_solve_batch ($rc): " * odemi_last_error(ctx))
     end
     # Julia's column-major (N, ...) arrays are the C side's [..][N] SoA layout.
-    codes = (1 => SciMLBase.ReturnCode.Success, 3 => SciMLBase.ReturnCode.DtNaN, 4 => SciMLBase.ReturnCode.MaxIters,
+    codes = (1 => SciMLBase.ReturnCode.Success, 2 => SciMLBase.ReturnCode.Terminated, 3 => SciMLBase.ReturnCode.DtNaN, 4 => SciMLBase.ReturnCode.MaxIters,
              6 => SciMLBase.ReturnCode.DtLessThanMin, 7 => SciMLBase.ReturnCode.Unstable)
     rcmap = Dict(codes)
     sols = map(1:N) do j
         k = min(Int(nsaved[j]), ms)
         ts = t_out[j, 1:k]
         us = [u_out[j, :, s] for s in 1:k]
-        st = SciMLBase.DEStats(stats[j, 1], 0, 0, 0, stats[j, 4], stats[j, 5], 0, stats[j, 6], 0, 0,
+        # rows: nf naccept nreject njacs nw nsolve ncondition; DEStats field order
+        # (lib/DiffEqBase/src/stats.jl:30-42): nf nf2 nw nsolve njacs nnonliniter
+        # nnonlinconvfail ncondition naccept nreject maxeig
+        st = SciMLBase.DEStats(stats[j, 1], 0, stats[j, 5], stats[j, 6], stats[j, 4], 0, 0, stats[j, 7],
                                stats[j, 2], stats[j, 3], 0.0)
         SciMLBase.build_solution(probs[j], alg, ts, us; retcode = get(rcmap, Int(retcode[j]), SciMLBase.ReturnCode.Failure),
                                  stats = st)

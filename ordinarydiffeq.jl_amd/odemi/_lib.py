@@ -13,6 +13,8 @@ DEFAULT_CACHE = os.path.join(_HERE, "_cache")
 TSIT5, RODAS5P = 0, 1
 ENGINE_PLAIN, ENGINE_PERSISTENT = 0, 1
 FP64, MIXED = 0, 1
+# rows of the per-trajectory stats output: nf naccept nreject njacs nw nsolve ncondition
+NSTATS = 7
 COMPILE_TSIT5, COMPILE_RODAS5P, HAS_USER_TGRAD, COMPILE_WG = 1, 2, 4, 8
 
 P = ctypes.POINTER

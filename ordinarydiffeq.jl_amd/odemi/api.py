@@ -150,15 +150,60 @@ class EnsembleProblem:
 
 
 class DEStats:
-    __slots__ = ("nf", "naccept", "nreject", "njacs", "nw", "nsolve")
+    __slots__ = ("nf", "naccept", "nreject", "njacs", "nw", "nsolve", "ncondition")
 
-    def __init__(self, nf, naccept, nreject, njacs, nw, nsolve):
+    def __init__(self, nf, naccept, nreject, njacs, nw, nsolve, ncondition=0):
         self.nf, self.naccept, self.nreject = int(nf), int(naccept), int(nreject)
         self.njacs, self.nw, self.nsolve = int(njacs), int(nw), int(nsolve)
+        self.ncondition = int(ncondition)
 
     def __repr__(self):
         return (f"DEStats(nf={self.nf}, naccept={self.naccept}, nreject={self.nreject}, "
-                f"njacs={self.njacs}, nw={self.nw}, nsolve={self.nsolve})")
+                f"njacs={self.njacs}, nw={self.nw}, nsolve={self.nsolve}, ncondition={self.ncondition})")
+
+
+class DiscreteCallback:
+    """DiscreteCallback(condition, affect!; save_positions) (lib/DiffEqBase/src/
+    callbacks.jl:645-683).  After every accepted step, when condition(u, t,
+    integrator) holds, affect(integrator) runs; it may rewrite integrator.u (by
+    index, slice or whole) and call odemi.terminate(integrator).  Both are
+    traced like f: conditions are predicates (==, <, &, |, ~, odemi.sym_in /
+    sym_any / sym_all), the affect is branch-free.  save_positions = (save
+    before, save after) the affect."""
+
+    def __init__(self, condition, affect, save_positions=(True, True)):
+        if len(save_positions) != 2:
+            raise ValueError("save_positions is a pair of Bools")
+        self.condition, self.affect = condition, affect
+        self.save_positions = (bool(save_positions[0]), bool(save_positions[1]))
+
+
+class CallbackSet:
+    """CallbackSet(cb...) of DiscreteCallbacks, applied in order."""
+
+    def __init__(self, *callbacks):
+        self.discrete_callbacks = _discrete_list(list(callbacks))
+
+
+def _discrete_list(cb):
+    if cb is None:
+        return []
+    if isinstance(cb, DiscreteCallback):
+        return [cb]
+    if isinstance(cb, CallbackSet):
+        return list(cb.discrete_callbacks)
+    if isinstance(cb, (list, tuple)):
+        out = []
+        for c in cb:
+            out += _discrete_list(c)
+        return out
+    raise TypeError(f"unsupported callback {type(cb).__name__} (DiscreteCallback / CallbackSet only)")
+
+
+def terminate(integrator):
+    """terminate!(integrator) inside a traced affect: the trajectory stops after
+    this step with retcode Terminated (integrator_interface.jl:443-446)."""
+    integrator.terminate()
 
 
 class ODESolution:
@@ -201,21 +246,24 @@ template <typename T> __device__ __forceinline__ void odemi_tgrad(T* dT, const T
 """
 
 
-def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, isoutofdomain=None):
+def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, isoutofdomain=None,
+                    callback=None):
     """Trace fun (ODEFunction) for state size n (or the sizes `ns` of an uneven
     batch, n = max) and hipRTC-compile it.  isoutofdomain(u, p, t): a traceable
     predicate (comparisons, &, |, ~, odemi.sym_any / sym_all) for the lane
-    kernels.  Returns (rhs, traced, seconds)."""
+    kernels; callback: DiscreteCallback / CallbackSet, traced likewise.
+    Returns (rhs, traced, seconds)."""
     wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
-    if wg and isoutofdomain is not None:
-        raise ValueError("isoutofdomain is supported by the lane kernels (Tsit5; Rodas5P with n <= "
-                         f"{LANE_MAX_RODAS}), not by the workgroup kernels")
+    cbs = _discrete_list(callback)
+    if wg and (isoutofdomain is not None or cbs):
+        raise ValueError("isoutofdomain and callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
+                         f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
     if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
         raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
     if ns is not None and "tsit5" in algs:
         raise ValueError("uneven-size batches (nvec) are supported with Rodas5P")
     key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_, tuple(ns) if ns is not None else None, wg,
-           id(isoutofdomain))
+           id(isoutofdomain), tuple((id(c.condition), id(c.affect), c.save_positions) for c in cbs))
     tr = _TRACES.get(key)
     if tr is None:
         if wg:
@@ -223,7 +271,7 @@ def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, i
             tr.src = _LANE_STUBS + tr.src
             tr.has_user_tgrad = fun.tgrad is not None
         else:
-            tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad, isout=isoutofdomain)
+            tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad, isout=isoutofdomain, callbacks=cbs)
         _TRACES[key] = tr
     flags = 0
     if wg:
@@ -305,12 +353,14 @@ def _is_torch(x):
 def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abstol=None, saveat=None,
                  save_start=False, save_end=True, save_everystep=False, save_end_user=-1, max_saves=None,
                  dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None, nvec=None,
-                 tstops=None, isoutofdomain=None):
+                 tstops=None, isoutofdomain=None, callback=None):
     """Bulk entry point on packed SoA arrays: u0 (nmax, N), p (np, N), t0/tf (N,);
     nvec (N,) gives per-trajectory dimensions for uneven batches (Rodas5P);
     tstops: stopping times shared by all trajectories (each keeps those strictly
     inside its own (t0, tf)); isoutofdomain(u, p, t): traced predicate that
-    rejects a step whose end state it flags (the next dt is dt * qmin).
+    rejects a step whose end state it flags (the next dt is dt * qmin);
+    callback: DiscreteCallback / CallbackSet (forced saves of save_positions need
+    max_saves room; nsaved beyond it is reported, the extra points dropped).
 
     Arrays may be numpy (host) or torch CUDA tensors (device-resident, zero
     copies).  Returns a dict of output arrays of the same kind."""
@@ -327,7 +377,8 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         ns = sorted(set(int(x) for x in nv_host))
         if ns[-1] > n or ns[0] < 1:
             raise ValueError("nvec entries must lie in [1, u0.shape[0]]")
-    h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx, ns=ns, isoutofdomain=isoutofdomain)
+    h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx, ns=ns, isoutofdomain=isoutofdomain,
+                                   callback=callback)
     reltol = 1e-3 if reltol is None else float(reltol)
     abstol = 1e-6 if abstol is None else float(abstol)
     sa = np.zeros(0) if saveat is None else np.asarray(saveat, dtype=np.float64)
@@ -370,7 +421,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
             u_final=out.get("u_final", torch.empty((n, N), **dd)),
             t_final=out.get("t_final", torch.empty(N, **dd)),
             retcode=out.get("retcode", torch.empty(N, dtype=torch.int32, device=devc)),
-            stats=out.get("stats", torch.empty((6, N), dtype=torch.int32, device=devc)),
+            stats=out.get("stats", torch.empty((_lib.NSTATS, N), dtype=torch.int32, device=devc)),
         )
         keep += [u0c, pc, t0c, tfc, sat, tst]
         if nvec is not None:
@@ -389,7 +440,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         tst = np.ascontiguousarray(tsv if len(tsv) else np.zeros(1))
         o = dict(t_out=np.empty((max(max_saves, 1), N)), u_out=np.empty((max(max_saves, 1), n, N)),
                  nsaved=np.empty(N, dtype=np.int32), u_final=np.empty((n, N)), t_final=np.empty(N),
-                 retcode=np.empty(N, dtype=np.int32), stats=np.empty((6, N), dtype=np.int32))
+                 retcode=np.empty(N, dtype=np.int32), stats=np.empty((_lib.NSTATS, N), dtype=np.int32))
         keep += [u0c, pc, t0c, tfc, sat, tst]
         if nvec is not None:
             nvc = np.ascontiguousarray(nvec, dtype=np.int32)
@@ -421,7 +472,7 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
 def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=None, saveat=None,
           save_start=None, save_end=None, save_everystep=None, dt=None, dtmax=None, dtmin=0.0,
           maxiters=1_000_000, seed=0, batch_size=None, max_saves=None, tstops=None, isoutofdomain=None,
-          **kw):
+          callback=None, **kw):
     """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...) -> EnsembleSolution.
 
     A plain ODEProblem is solved as a one-trajectory ensemble and returns its ODESolution."""
@@ -462,15 +513,17 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=No
         sa = saveat_times(saveat, 0, 0)
     else:
         sa = np.zeros(0)
-    if max_saves is None and sev:
+    cbs = _discrete_list(callback)
+    forced = any(any(c.save_positions) for c in cbs)
+    if max_saves is None and (sev or forced):
         max_saves = 4096
     o = solve_arrays(base.f, alg, u0, pp, t0, tf, ea, reltol=reltol, abstol=abstol, saveat=sa, save_start=ss,
                      save_end=se, save_everystep=sev, save_end_user=seu, max_saves=max_saves, dt=dt,
                      dtmax=dtmax, dtmin=dtmin, maxiters=maxiters, tstops=tstops,
-                     isoutofdomain=isoutofdomain)
-    if sev and int(o["nsaved"].max(initial=0)) > o["max_saves"]:
-        raise RuntimeError(f"save_everystep produced more than max_saves={o['max_saves']} points; "
-                           "pass a larger max_saves")
+                     isoutofdomain=isoutofdomain, callback=cbs)
+    if (sev or forced) and int(o["nsaved"].max(initial=0)) > o["max_saves"]:
+        raise RuntimeError(f"the saves (save_everystep / callback save_positions) exceeded max_saves="
+                           f"{o['max_saves']} points; pass a larger max_saves")
     sols = []
     outs = []
     for j in range(N):

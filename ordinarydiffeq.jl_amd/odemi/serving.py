@@ -129,7 +129,7 @@ class EnsembleServer:
             raise ValueError("pass nvec: n itself is not among the server's ns")
         uf = np.empty((n, N), dtype=np.float64)
         rc = np.zeros(N, dtype=np.int32)
-        st = np.zeros((6, N), dtype=np.int32) if stats else None
+        st = np.zeros((_lib.NSTATS, N), dtype=np.int32) if stats else None
         ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
         rid = self._L.odemi_submit(self._handle(), N, ptr(nv), ptr(u0), ptr(pp), ptr(t0), ptr(tf), ptr(uf), ptr(rc),
                                    ptr(st))

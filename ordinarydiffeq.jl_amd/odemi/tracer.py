@@ -125,6 +125,20 @@ class Sym:
     def __rand__(self, o): return self.g.make("and", (_lift(o), self))
     def __invert__(self): return self.g.make("not", (self,))
 
+    # equality is a predicate too (callback conditions such as t == 2.0); Syms
+    # stay hashable by identity
+    def __eq__(self, o):
+        if not isinstance(o, (Sym, int, float)) and type(o).__module__ != "numpy":
+            return NotImplemented
+        return self.g.make("eq", (self, _lift(o)))
+
+    def __ne__(self, o):
+        if not isinstance(o, (Sym, int, float)) and type(o).__module__ != "numpy":
+            return NotImplemented
+        return self.g.make("ne", (self, _lift(o)))
+
+    __hash__ = object.__hash__
+
     def __bool__(self):
         raise TypeError("data-dependent control flow on a traced value is not supported; "
                         "use ifelse-free arithmetic (and odemi.sym_any / sym_all for predicates)")
@@ -183,6 +197,11 @@ def sym_any(xs):
     for x in xs:
         r = r | x
     return r
+
+
+def sym_in(x, values):
+    """Traceable `x in values` (Julia's `t in tstops`): the `or` of x == v."""
+    return sym_any([x == v for v in values])
 
 
 def sym_all(xs):
@@ -335,7 +354,7 @@ _FN = {"exp": "exp", "log": "log", "sin": "sin", "cos": "cos", "tan": "tan", "sq
        "tanh": "tanh", "atan": "atan", "abs": "fabs"}
 
 
-_CMP = {"lt": "<", "le": "<=", "gt": ">", "ge": ">="}
+_CMP = {"lt": "<", "le": "<=", "gt": ">", "ge": ">=", "eq": "==", "ne": "!="}
 
 
 def _emit_body(outputs, out_name, indent="    ", pname="p[{}]"):
@@ -416,10 +435,112 @@ def _count_flops(outputs):
     return cnt
 
 
-def trace_problem(f, n, np_, jac=None, tgrad=None, isout=None):
+class _TracedState:
+    """integrator.u while a callback is traced: reads give the current entry,
+    writes (index or slice) replace it."""
+
+    def __init__(self, vals):
+        self.vals = list(vals)
+
+    def __len__(self):
+        return len(self.vals)
+
+    def __iter__(self):
+        return iter(list(self.vals))
+
+    def __getitem__(self, k):
+        if isinstance(k, slice):
+            return list(self.vals[k])
+        return self.vals[int(k)]
+
+    def __setitem__(self, k, v):
+        if isinstance(k, slice):
+            idx = list(range(*k.indices(len(self.vals))))
+            if isinstance(v, (Sym, int, float)) or (type(v).__module__ == "numpy" and getattr(v, "ndim", 1) == 0):
+                vs = [v] * len(idx)
+            else:
+                vs = list(v)
+                if len(vs) != len(idx):
+                    raise ValueError("slice assignment of a different length")
+            for i, x in zip(idx, vs):
+                self.vals[i] = _lift(x)
+        else:
+            self.vals[int(k)] = _lift(v)
+
+
+class _TracedIntegrator:
+    """The integrator a traced DiscreteCallback sees: `u` (assignable by index,
+    slice or whole), `p` and `t` (read-only), `terminate()` (terminate!)."""
+
+    def __init__(self, u, p, t):
+        self._u = _TracedState(u)
+        self._p, self._t, self._term = tuple(p), t, False
+
+    @property
+    def u(self):
+        return self._u
+
+    @u.setter
+    def u(self, v):
+        vals = list(v)
+        if len(vals) != len(self._u):
+            raise ValueError("integrator.u must keep its length")
+        self._u = _TracedState([_lift(x) for x in vals])
+
+    @property
+    def p(self):
+        return self._p
+
+    @property
+    def t(self):
+        return self._t
+
+    def terminate(self):
+        self._term = True
+
+
+def _emit_callbacks(callbacks, n, u, p, t):
+    """A CallbackSet of DiscreteCallbacks as OdemiDcb<K> specialisations: the
+    condition traced on (u, t, integrator), affect! traced on an integrator
+    whose u it may rewrite and which it may terminate!."""
+    src = [f"#define ODEMI_NDCB {len(callbacks)}", "template <int K> struct OdemiDcb;"]
+    for k, cb in enumerate(callbacks):
+        c = cb.condition(list(u), t, _TracedIntegrator(u, p, t))
+        c = c if isinstance(c, Sym) else _lift(1.0 if c else 0.0)
+        itg = _TracedIntegrator(u, p, t)
+        cb.affect(itg)
+        outs = [(i, v) for i, v in enumerate(itg.u.vals) if v is not u[i]]
+        s0, s1 = (int(bool(x)) for x in cb.save_positions)
+        src.append(f"template <> struct OdemiDcb<{k}> {{")
+        src.append(f"    static constexpr int save0 = {s0}, save1 = {s1}, term = {int(itg._term)};")
+        src.append("    template <typename T, typename Q>")
+        src.append("    static __device__ __forceinline__ bool cond(const T* __restrict__ u, "
+                   "const Q* __restrict__ p, T t) {")
+        src.append("        (void)u; (void)p; (void)t;")
+        src.append("        T r_[1];")
+        src.append(_emit_body([(0, c)], "r_", indent="        ", pname="T(p[{}])"))
+        src.append("        return r_[0] != T(0);")
+        src.append("    }")
+        src.append("    template <typename T, typename Q>")
+        src.append("    static __device__ __forceinline__ void affect(T* __restrict__ u, "
+                   "const Q* __restrict__ p, T t) {")
+        src.append("        (void)u; (void)p; (void)t;")
+        if outs:
+            # every new value from the pre-affect state, then the writes
+            src.append(f"        T nu_[{n}];")
+            src.append(_emit_body(outs, "nu_", indent="        ", pname="T(p[{}])"))
+            for i, _ in outs:
+                src.append(f"        u[{i}] = nu_[{i}];")
+        src.append("    }")
+        src.append("};")
+    return "\n".join(src)
+
+
+def trace_problem(f, n, np_, jac=None, tgrad=None, isout=None, callbacks=None):
     """Trace f (and jac/tgrad when given) for state dimension n and np_ params;
     isout(u, p, t) is the isoutofdomain predicate (solve kwarg), traced into
-    odemi_isout."""
+    odemi_isout; callbacks: DiscreteCallbacks (condition, affect, save_positions)
+    traced into OdemiDcb<K>."""
     global _CUR
     g = _Graph()
     _CUR = g
@@ -484,6 +605,8 @@ def trace_problem(f, n, np_, jac=None, tgrad=None, isout=None):
             src.append(_emit_body([(0, r)], "r_", pname="T(p[{}])"))
             src.append("    return r_[0] != T(0);")
             src.append("}")
+        if callbacks:
+            src.append(_emit_callbacks(callbacks, n, u, p, t))
         return TracedProblem(n, np_, "\n".join(src) + "\n", has_user_jac, tgrad is not None, autonomous,
                              _count_flops(fout), _count_flops(jout))
     finally:

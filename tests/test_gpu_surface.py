@@ -216,6 +216,123 @@ def test_isoutofdomain_bit_identical(alg, engine):
     assert codes == {1, 7}   # some trajectories cross y = 8 and stall, some never reach it
 
 
+def _pk(du, u, p, t):
+    du[0] = -p[0] * u[0]
+    du[1] = p[0] * u[0] - p[1] * u[1]
+
+
+def _pk_jac(J, u, p, t):
+    J[0, 0] = -p[0]
+    J[0, 1] = 0.0
+    J[1, 0] = p[0]
+    J[1, 1] = -p[1]
+
+
+_DOSES = [2.0, 4.0, 6.0]
+
+
+def _dose_cond(u, t, itg):
+    return odemi.sym_in(t, _DOSES)
+
+
+def _dose(itg):
+    itg.u[0] = itg.u[0] + itg.p[2]
+
+
+def _cap_cond(u, t, itg):
+    return u[1] > itg.p[3]
+
+
+def _stop(itg):
+    odemi.terminate(itg)
+
+
+_PK_CBS = odemi.CallbackSet(odemi.DiscreteCallback(_dose_cond, _dose),
+                            odemi.DiscreteCallback(_cap_cond, _stop, save_positions=(False, True)))
+
+
+def _pk_batch(N, seed=11):
+    rng = np.random.default_rng(seed)
+    p = np.stack([rng.uniform(0.5, 2.0, N), rng.uniform(0.1, 0.5, N), rng.uniform(1.0, 3.0, N),
+                  rng.uniform(1.5, 4.0, N)])
+    u0 = np.stack([p[2].copy(), np.zeros(N)])
+    return u0, p
+
+
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()], ids=["tsit5", "rodas5p"])
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_discrete_callbacks_bit_identical(alg, engine):
+    """A CallbackSet of two DiscreteCallbacks (lib/DiffEqBase/src/callbacks.jl:
+    645-683) through the lane engines, bit-identical to the CPU restatement:
+    two-compartment dosing, u[0] += dose at the tstops 2, 4, 6 (saved before
+    and after), and terminate! once u[1] exceeds a per-trajectory cap (saved
+    after), with save_everystep, with saveat, and final state only (forced
+    saves still land in the outputs).  Retcodes (Success / Terminated), every
+    saved point and nf / naccept / nreject / ncondition agree."""
+    N = 96
+    u0, p = _pk_batch(N)
+    fun = odemi.ODEFunction(_pk, jac=_pk_jac)
+    ea = odemi.EnsembleMI355X(engine=engine)
+    modes = [dict(save_everystep=True, save_start=True, save_end=True),
+             dict(saveat=[1.0, 2.0, 3.0, 5.0, 7.5], save_start=False, save_end=True),
+             dict(save_start=False, save_end=True)]
+    codes = set()
+    for m in modes:
+        print(f"[case] discrete callbacks {alg.code} {engine} {sorted(m)}", flush=True)
+        out = odemi.solve_arrays(fun, alg, u0, p, np.zeros(N), np.full(N, 10.0), ea, reltol=1e-6, abstol=1e-8,
+                                 tstops=_DOSES, callback=_PK_CBS, max_saves=2048, **m)
+        for j in range(0, N, 5):
+            r = O.solve(lambda du, u, pp, t: _pk(du, u, pp, t), u0[:, j], p[:, j], (0.0, 10.0),
+                        jac=lambda J, u, pp, t: _pk_jac(J, u, pp, t), alg=alg.code, reltol=1e-6, abstol=1e-8,
+                        tstops=_DOSES, callbacks=_PK_CBS.discrete_callbacks, **m)
+            ns = int(out["nsaved"][j])
+            assert int(out["retcode"][j]) == r["retcode"] and ns == r["nsaved"], (j, m)
+            assert np.array_equal(out["t_out"][:ns, j], r["t"]), (j, m)
+            assert np.array_equal(out["u_out"][:ns, :, j], r["u"]), (j, m)
+            assert np.array_equal(out["u_final"][:, j], r["u_final"])
+            st = out["stats"][:, j]
+            assert (st[0], st[1], st[2], st[6]) == (r["nf"], r["naccept"], r["nreject"], r["ncondition"])
+            codes.add(r["retcode"])
+            if r["retcode"] == 1:   # every dose saved twice, before and after
+                assert all(list(r["t"]).count(d) == 2 for d in _DOSES)
+    assert codes == {1, 2}
+
+
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()], ids=["tsit5", "rodas5p"])
+def test_discrete_callbacks_mixed_and_surface(alg):
+    """The same callbacks on the mixed engines (within the FP32-stage error of
+    the FP64 engine, same save times) and through solve(EnsembleProblem):
+    ODESolution.retcode Terminated / Success and stats.ncondition."""
+    N = 256
+    u0, p = _pk_batch(N, seed=12)
+    fun = odemi.ODEFunction(_pk, jac=_pk_jac)
+    kw = dict(reltol=1e-6, abstol=1e-8, tstops=_DOSES, callback=_PK_CBS, save_start=False, save_end=True,
+              max_saves=64)
+    print(f"[case] discrete callbacks mixed {alg.code}", flush=True)
+    o64 = odemi.solve_arrays(fun, alg, u0, p, np.zeros(N), np.full(N, 10.0), **kw)
+    om = odemi.solve_arrays(fun, alg, u0, p, np.zeros(N), np.full(N, 10.0),
+                            odemi.EnsembleMI355X(precision="mixed"), **kw)
+    same = (om["retcode"] == o64["retcode"])
+    assert same.mean() > 0.95 and set(np.unique(o64["retcode"])) == {1, 2}
+    for j in np.nonzero(same & (o64["retcode"] == 1))[0][:40]:
+        ns = int(om["nsaved"][j])
+        assert ns == int(o64["nsaved"][j]) and np.array_equal(om["t_out"][:ns, j], o64["t_out"][:ns, j])
+    ok = same & (o64["retcode"] == 1)
+    err = np.abs(om["u_final"][:, ok] - o64["u_final"][:, ok]) / (1e-8 + 1e-6 * np.abs(o64["u_final"][:, ok]))
+    print(f"callbacks mixed scaled err median {np.median(err):.3e} max {err.max():.3e}")
+    assert np.median(err) < 10.0
+    # the EnsembleProblem surface
+    base = odemi.ODEProblem(fun, list(u0[:, 0]), (0.0, 10.0), list(p[:, 0]))
+    ens = odemi.EnsembleProblem(base, prob_func=lambda pr, ctx: odemi.remake(
+        pr, u0=list(u0[:, ctx.sim_id - 1]), p=list(p[:, ctx.sim_id - 1])))
+    sim = odemi.solve(ens, alg, odemi.EnsembleMI355X(), trajectories=32, reltol=1e-6, abstol=1e-8,
+                      tstops=_DOSES, callback=_PK_CBS, save_everystep=False)
+    for j, sol in enumerate(sim.u):
+        assert int(sol.retcode) == int(o64["retcode"][j])
+        assert sol.stats.ncondition == 2 * sol.stats.naccept
+        assert np.array_equal(sol.u[-1], o64["u_final"][:, j])
+
+
 def _expb(du, u, p, t):
     du[0] = odemi.sym_exp(u[0])
 

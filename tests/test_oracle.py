@@ -12,7 +12,8 @@ term, a wrong sign or index, a transposed operand) fails it:
     u(100) = exp(-100) (test/InterfaceI/controllers.jl:29-73), the exp(u)
     blow-up return codes (test/Integrators_I/check_error.jl:1-44), nf accounting
     (test/InterfaceIII/stats_tests.jl:1-40), empty-state dt = 1e-6
-    (test/Regression_I/ode_adaptive_tests.jl:111-118);
+    (test/Regression_I/ode_adaptive_tests.jl:111-118), DiscreteCallback save
+    and terminate! rules (test/Integrators_I/ode_event_tests.jl);
   * the textbook Hairer-Norsett-Wanner initial step recomputed in mpmath;
   * invariants: Robertson mass conservation; agreement with scipy's DOP853 /
     Radau at tight tolerance; LU against numpy; Julia eps/min semantics.
@@ -354,6 +355,93 @@ def test_isoutofdomain_rejects_and_keeps_the_domain():
     for alg in (O.TSIT5, O.RODAS5P):
         r3 = O.solve(lin, [1.0], tspan=(0.0, 1.0), p=[-1.0], alg=alg, isoutofdomain=lambda u, p, t: t > 0.5)
         assert r3["retcode"] == 7 and 0.4 < r3["t_final"] <= 0.5   # Unstable
+
+
+class _CB:
+    """DiscreteCallback's fields (condition, affect!, save_positions)."""
+
+    def __init__(self, condition, affect, save_positions=(True, True)):
+        self.condition, self.affect, self.save_positions = condition, affect, save_positions
+
+
+def test_discrete_callback_reference_known_answers():
+    """DiscreteCallback handling (integrator_utils.jl:1081-1132,
+    lib/DiffEqBase/src/callbacks.jl:645-683) against the reference's tests:
+    test/Integrators_I/ode_event_tests.jl:400-419 (u .= 1 at tstops 5 and 8:
+    each tstop twice in sol.t, also with saveat = prevfloat.(tstop)), 235-258
+    (always-true callbacks with save_positions (true, false) on free fall: one
+    adds nothing to save_everystep, two give 2 len - 1 points), 463-475 (two
+    no-save callbacks: no repeated time), test/InterfaceI/ode_tstops_tests.jl:
+    64-81 (sol.t[end] == 1.5574) and lib/DiffEqBase/test/downstream/
+    stats_tests.jl (ncondition = condition calls).  The pre-affect save is the
+    integrated state (closed form u1 = 20 - 10 exp(-t/2) at t = 5), the
+    post-affect one exactly the affect's value; a modified state costs one FSAL
+    re-evaluation at the next accept; terminate! ends with Terminated after the
+    endpoint save."""
+    def f(du, u, p, t):
+        du[0] = -0.5 * u[0] + 10
+        du[1] = -0.5 * u[1]
+    tstop = [5.0, 8.0]
+    calls = [0]
+
+    def cond(u, t, itg):
+        calls[0] += 1
+        return t in tstop
+
+    def ones(itg):
+        itg.u[:] = 1.0
+    for alg in (O.TSIT5, O.RODAS5P):
+        jac = (lambda J, u, p, t: J.__setitem__(slice(None), [[-0.5, 0.0], [0.0, -0.5]]))
+        kw = dict(alg=alg, tstops=tstop, callbacks=[_CB(cond, ones)], save_start=False, save_end=False,
+                  save_everystep=False)
+        calls[0] = 0
+        r = O.solve(f, [10.0, 10.0], [0.0], (0.0, 10.0), jac=jac, saveat=tstop, **kw)
+        assert list(r["t"]) == [5.0, 5.0, 8.0, 8.0]
+        assert abs(r["u"][0, 0] - (20 - 10 * math.exp(-2.5))) < 1e-2 and list(r["u"][1]) == [1.0, 1.0]
+        assert r["ncondition"] == calls[0] == r["naccept"] and r["retcode"] == 1
+        sa = list(np.nextafter(np.array(tstop), -np.inf))
+        r = O.solve(f, [10.0, 10.0], [0.0], (0.0, 10.0), jac=jac, saveat=sa, **kw)
+        assert [list(r["t"]).count(x) for x in tstop] == [2, 2] and len(r["t"]) == 6
+
+    def fall(du, u, p, t):
+        du[0] = u[1]
+        du[1] = -9.81
+    always, nop = (lambda u, t, itg: True), (lambda itg: None)
+    r0 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), save_everystep=True)
+    r1 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), save_everystep=True,
+                 callbacks=[_CB(always, nop, (True, False))])
+    r2 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), save_everystep=True,
+                 callbacks=[_CB(always, nop, (True, False))] * 2)
+    assert np.array_equal(r1["t"], r0["t"]) and len(r2["t"]) == 2 * len(r1["t"]) - 1
+    assert r1["nf"] == r0["nf"] + r0["naccept"] - 1 and r1["ncondition"] == r1["naccept"]
+    assert r2["ncondition"] == 2 * r2["naccept"]
+
+    def grow(du, u, p, t):
+        du[0] = 1.01 * u[0]
+    r = O.solve(grow, [1.0], [0.0], (0.0, 10.0), tstops=[2.5], save_everystep=True,
+                callbacks=[_CB(lambda u, t, itg: t == 2.5, nop, (False, False))] * 2)
+    assert 2.5 in r["t"] and not np.any(np.diff(r["t"]) == 0)
+
+    def de(du, u, p, t):
+        du[0] = p[0] * u[0]
+        du[1] = p[1] * u[1]
+
+    def bump(itg):
+        itg.u[0] += 1.0
+    r = O.solve(de, [0.0, 0.0], [0.3, 0.7], (-1.0, 3.0), tstops=[0.010823], saveat=[0.0, 0.0094777, 1.5574],
+                callbacks=[_CB(lambda u, t, itg: t == 0.010823, bump)], save_start=False, save_end=False)
+    assert list(r["t"]) == [0.0, 0.0094777, 0.010823, 0.010823, 1.5574]
+    assert list(r["u"][2]) == [0.0, 0.0] and list(r["u"][3]) == [1.0, 0.0]
+
+    def stop(itg):
+        itg.terminate()
+    for alg in (O.TSIT5, O.RODAS5P):
+        r = O.solve(f, [10.0, 10.0], [0.0], (0.0, 10.0), alg=alg, save_start=False,
+                    jac=lambda J, u, p, t: J.__setitem__(slice(None), [[-0.5, 0.0], [0.0, -0.5]]),
+                    callbacks=[_CB(lambda u, t, itg: u[0] > 15.0, stop, (False, False))])
+        assert r["retcode"] == 2 and 1.0 < r["t_final"] < 10.0   # Terminated
+        assert list(r["t"]) == [r["t_final"]] and r["u"][0, 0] > 15.0
+        assert abs(r["u"][0, 0] - 20 + 10 * math.exp(-r["t_final"] / 2)) < 1e-2
 
 
 def test_saveat_interpolant_accuracy():

@@ -1,0 +1,85 @@
+"""Host-side tracing (no GPU): the HIP source the tracer emits for f, the
+isoutofdomain predicate and DiscreteCallbacks, checked by structure and by
+hipRTC-compiling it for gfx950 (odemi_rtc_codeobject needs no device)."""
+import ctypes
+import os
+import sys
+
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "ordinarydiffeq.jl_amd"))
+import odemi  # noqa: E402
+from odemi import _lib  # noqa: E402
+from odemi.tracer import trace_problem  # noqa: E402
+
+
+def _decay(du, u, p, t):
+    du[0] = -0.5 * u[0] + 10
+    du[1] = -p[0] * u[1]
+
+
+def _dose_cond(u, t, itg):
+    return odemi.sym_in(t, [5.0, 8.0])
+
+
+def _dose(itg):
+    itg.u[1] = itg.u[1] + itg.p[1]
+
+
+def _reset_cond(u, t, itg):
+    return (u[0] >= 19.0) & ~(t < 1.0)
+
+
+def _reset_stop(itg):
+    itg.u[:] = 1.0
+    odemi.terminate(itg)
+
+
+def test_equality_builds_predicates():
+    tr = trace_problem(_decay, 2, 2, isout=lambda u, p, t: (t == 3.0) | (u[0] != p[0]))
+    assert "#define ODEMI_HAS_ISOUT 1" in tr.src
+    assert "(t == T(3.0))" in tr.src and "(u[0] != T(p[0]))" in tr.src
+    assert odemi.sym_in(2.0, [1.0, 2.0]) is True and odemi.sym_in(2.5, [1.0, 2.0]) is False
+
+
+def test_discrete_callbacks_emit_condition_affect_and_flags():
+    cbs = odemi.CallbackSet(odemi.DiscreteCallback(_dose_cond, _dose),
+                            odemi.DiscreteCallback(_reset_cond, _reset_stop, save_positions=(False, True)))
+    tr = trace_problem(_decay, 2, 2, callbacks=cbs.discrete_callbacks)
+    src = tr.src
+    assert "#define ODEMI_NDCB 2" in src
+    assert "static constexpr int save0 = 1, save1 = 1, term = 0;" in src
+    assert "static constexpr int save0 = 0, save1 = 1, term = 1;" in src
+    first = src[src.index("struct OdemiDcb<0>"):src.index("struct OdemiDcb<1>")]
+    # the dose writes u[1] only, from the pre-affect state
+    assert "u[1] = nu_[1];" in first and "u[0] = nu_[0];" not in first
+    second = src[src.index("struct OdemiDcb<1>"):]
+    assert "u[0] = nu_[0];" in second and "u[1] = nu_[1];" in second
+
+
+def test_callback_module_compiles_for_gfx950():
+    cbs = [odemi.DiscreteCallback(_dose_cond, _dose),
+           odemi.DiscreteCallback(_reset_cond, _reset_stop, save_positions=(False, True))]
+    tr = trace_problem(_decay, 2, 2, callbacks=cbs, isout=lambda u, p, t: u[1] < 0.0)
+    L = _lib.load()
+    for flags in (_lib.COMPILE_TSIT5, _lib.COMPILE_RODAS5P):
+        path = ctypes.create_string_buffer(512)
+        log = ctypes.create_string_buffer(1 << 16)
+        secs = ctypes.c_double()
+        rc = L.odemi_rtc_codeobject(tr.src.encode(), 2, 2, flags, path, 512, ctypes.byref(secs), log, 1 << 16)
+        assert rc == 0, log.value.decode()[:2000]
+
+
+def test_callbacks_rejected_by_workgroup_kernels_and_unknown_types():
+    cb = odemi.DiscreteCallback(_dose_cond, _dose)
+
+    def big(du, u, p, t):
+        for i in range(8):
+            du[i] = -u[i]
+    with pytest.raises(ValueError, match="lane kernels"):
+        odemi.compile_problem(odemi.ODEFunction(big), 8, 2, algs=("rodas5p",), callback=cb)
+    with pytest.raises(TypeError):
+        odemi.compile_problem(odemi.ODEFunction(_decay), 2, 2, algs=("tsit5",), callback=object())
+    with pytest.raises(TypeError):
+        trace_problem(_decay, 2, 2, callbacks=[odemi.DiscreteCallback(lambda u, t, i: u[0] if u[0] > 1 else 0,
+                                                                     _dose)])
