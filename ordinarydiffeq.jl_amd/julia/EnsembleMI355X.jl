@@ -244,6 +244,12 @@ function SciMLBase.__solve(ep::SciMLBase.EnsembleProblem, alg::Union{Tsit5,Rodas
                            trajectories, reltol = 1e-3, abstol = 1e-6, saveat = Float64[], save_start = nothing,
                            save_end = nothing, save_everystep = isempty(saveat), dt = 0.0, dtmax = 0.0,
                            dtmin = 0.0, maxiters = 1_000_000, max_saves = 4096, tstops = Float64[], kwargs...)
+    # this front-end traces f / jac / tgrad only; the Python front-end also
+    # traces isoutofdomain and DiscreteCallbacks
+    for k in (:callback, :isoutofdomain)
+        haskey(kwargs, k) && error("EnsembleMI355X (Julia front-end): `$k` is not supported")
+    end
+    isempty(kwargs) || error("EnsembleMI355X: unsupported keyword(s) $(collect(keys(kwargs)))")
     probs = [ep.prob_func(deepcopy(ep.prob), i, 1) for i in 1:trajectories]
     n = length(probs[1].u0)
     np = probs[1].p isa SciMLBase.NullParameters ? 0 : length(probs[1].p)
