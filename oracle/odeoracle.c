@@ -364,6 +364,9 @@ typedef struct {
     int64_t iter, success_iter;
     int accept_step, next_step_tstop, isout;
     int reeval_fsal;      /* a callback modified u: the next accept re-evaluates the FSAL stage */
+    int event_last;       /* integrator.event_last_time: 1 + the continuous callback of the last event, 0 none */
+    double last_event_error;
+    double* ctmp;         /* interpolated state for continuous-callback conditions */
     oo_result* res;
     /* vectors */
     double *u, *uprev, *tmp, *atmp;
@@ -607,6 +610,60 @@ static void tsit5_interp(integ_t* I, double theta, double dt, double* out) {
     }
 }
 
+/* _ode_addsteps! for Tsit5Cache (tsit_perform_step.jl:1-38, @muladd) with
+ * always_calc_begin: the stages k2..k7 of the step from uprev at tprev with
+ * the current dt (k1 kept), as reeval_internals_due_to_modification! runs it
+ * after change_t_via_interpolation!.  The reference charges no f calls here. */
+static void tsit5_addsteps(integ_t* I) {
+    const int n = I->n;
+    const double t = I->tprev, dt = I->dt;
+    double *k1 = I->k[0], *k2 = I->k[1], *k3 = I->k[2], *k4 = I->k[3], *k5 = I->k[4],
+           *k6 = I->k[5], *k7 = I->k[6];
+    double *tmp = I->tmp, *uprev = I->uprev;
+    for (int i = 0; i < n; i++) tmp[i] = fma(dt, TS_a21 * k1[i], uprev[i]);
+    I->f(k2, tmp, I->p, fma(TS_c1, dt, t), I->user);
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a31 * k1[i];
+        acc = fma(TS_a32, k2[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    I->f(k3, tmp, I->p, fma(TS_c2, dt, t), I->user);
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a41 * k1[i];
+        acc = fma(TS_a42, k2[i], acc);
+        acc = fma(TS_a43, k3[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    I->f(k4, tmp, I->p, fma(TS_c3, dt, t), I->user);
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a51 * k1[i];
+        acc = fma(TS_a52, k2[i], acc);
+        acc = fma(TS_a53, k3[i], acc);
+        acc = fma(TS_a54, k4[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    I->f(k5, tmp, I->p, fma(TS_c4, dt, t), I->user);
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a61 * k1[i];
+        acc = fma(TS_a62, k2[i], acc);
+        acc = fma(TS_a63, k3[i], acc);
+        acc = fma(TS_a64, k4[i], acc);
+        acc = fma(TS_a65, k5[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    I->f(k6, tmp, I->p, t + dt, I->user);
+    for (int i = 0; i < n; i++) {
+        double acc = TS_a71 * k1[i];
+        acc = fma(TS_a72, k2[i], acc);
+        acc = fma(TS_a73, k3[i], acc);
+        acc = fma(TS_a74, k4[i], acc);
+        acc = fma(TS_a75, k5[i], acc);
+        acc = fma(TS_a76, k6[i], acc);
+        tmp[i] = fma(dt, acc, uprev[i]);
+    }
+    I->f(k7, tmp, I->p, t + dt, I->user);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Rodas5P perform_step! (rosenbrock_perform_step.jl:729-847)                 */
 /* ------------------------------------------------------------------------ */
@@ -738,6 +795,117 @@ static int savevalues(integ_t* I, int force_save) {
     return savedexactly;
 }
 
+/* Continuous callbacks (Tsit5).  Julia's sign(): -1, 0, 1, NaN for NaN. */
+static double jl_sign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : (x == 0 ? 0.0 : x)); }
+
+/* get_condition (lib/DiffEqBase/src/callbacks.jl:91-146): the state at abst is
+ * u at t, uprev at tprev, else the step's dense output; every call counts in
+ * stats.ncondition. */
+static double ccb_condition(integ_t* I, int k, double abst) {
+    const double* v;
+    if (abst == I->t) {
+        v = I->u;
+    } else if (abst == I->tprev) {
+        v = I->uprev;
+    } else {
+        tsit5_interp(I, (abst - I->tprev) / I->dt, I->dt, I->ctmp);
+        v = I->ctmp;
+    }
+    I->res->ncondition++;
+    return I->o->ccb_cond(v, I->p, abst, k, I->o->ccb_user);
+}
+
+/* is_event_occurrence (callbacks.jl:525-530) */
+static int ccb_is_event(const int* fl, double prev_sign, double next_sign) {
+    return ((prev_sign < 0 && fl[OO_CCB_HAS_AFFECT]) || (prev_sign > 0 && fl[OO_CCB_HAS_AFFECT_NEG])) &&
+           prev_sign * next_sign <= 0;
+}
+
+/* find_callback_time for a ContinuousCallback (callbacks.jl:364-412) with
+ * nudge_tprev (414-426), check_event_occurrence (428-451) and find_root
+ * (480-497).  The root is bracketed down to adjacent floats (the interval
+ * solve with abstol = reltol = 0) by bisection; LeftRootFind keeps the float on
+ * the bottom side, RightRootFind the other.  The interp_points checks are at
+ * tprev + (i - 1) h, h = (t - tprev) / (interp_points - 1), and t itself last.
+ * Returns 1 when the step holds an event. */
+static int ccb_find_time(integ_t* I, int k, double* cb_t, double* bottom_sign, double* residual) {
+    const int* fl = I->o->ccb_flags + OO_CCB_NFLAGS * k;
+    const double* tl = I->o->ccb_tol + 2 * k;
+    double bottom_t = I->tprev;
+    double bc = ccb_condition(I, k, bottom_t);
+    if (I->event_last == k + 1) {
+        if (fabs(bc - I->last_event_error) <= tl[0]) bottom_t = I->tprev + I->dt * tl[1];
+        bc = ccb_condition(I, k, bottom_t);
+    }
+    const double bs = jl_sign(bc);
+    double top_t = I->t;
+    double ts = jl_sign(ccb_condition(I, k, top_t));
+    int ev = ccb_is_event(fl, bs, ts);
+    const int npts = fl[OO_CCB_INTERP_POINTS];
+    if (!ev && npts != 0) {
+        const double h = (I->t - I->tprev) / (double)(npts - 1);
+        for (int i = 2; i <= npts; i++) {
+            top_t = (i == npts) ? I->t : fma((double)(i - 1), h, I->tprev);
+            ts = jl_sign(ccb_condition(I, k, top_t));
+            ev = ccb_is_event(fl, bs, ts);
+            if (ev) break;
+        }
+    }
+    *bottom_sign = bs;
+    if (!ev) { *cb_t = I->t; *residual = 0.0; return 0; }
+    if (fl[OO_CCB_ROOTFIND] == 2 || ts == 0.0) { *cb_t = top_t; *residual = 0.0; return 1; }
+    double l = bottom_t, r = top_t;
+    for (int it = 0; it < 2200; it++) {
+        const double m = l + 0.5 * (r - l);
+        if (m == l || m == r) break;
+        const double sm = jl_sign(ccb_condition(I, k, m));
+        if (sm == 0.0) { l = m; r = m; break; }
+        if (sm == bs) l = m; else r = m;
+    }
+    *cb_t = fl[OO_CCB_ROOTFIND] == 1 ? r : l;
+    *residual = ccb_condition(I, k, *cb_t);
+    return 1;
+}
+
+/* apply_callback! for a ContinuousCallback (callbacks.jl:557-640):
+ * set_proposed_dt! (dtrelax = 1), change_t_via_interpolation!
+ * (integrator_interface.jl:5-40: u from the dense output, t, dt = t - tprev,
+ * then reeval_internals_due_to_modification! recomputes the stages), saves,
+ * affect! on an upcrossing / affect_neg! on a downcrossing.  Returns 1 for
+ * terminate!. */
+static int ccb_apply(integ_t* I, int k, double cb_t, double bs, int* saved_in_cb) {
+    const oo_opts* o = I->o;
+    const int* fl = o->ccb_flags + OO_CCB_NFLAGS * k;
+    if (o->adaptive) {
+        const double dtp = jl_max(nextafter(I->dtmin, INFINITY), I->dt);
+        I->dtpropose = dtp;
+        I->dtcache = dtp;
+    }
+    if (cb_t != I->t) {
+        tsit5_interp(I, (cb_t - I->tprev) / I->dt, I->dt, I->u);
+        I->t = cb_t;
+        I->dt = I->t - I->tprev;
+        tsit5_addsteps(I);
+        I->reeval_fsal = 1;
+    }
+    int savedexactly = savevalues(I, 0);
+    *saved_in_cb = 1;
+    if (fl[OO_CCB_SAVE0] && !savedexactly) savevalues(I, 1);
+    int modified = 1, term = 0;
+    if (bs < 0) {
+        if (fl[OO_CCB_HAS_AFFECT]) term = o->ccb_affect(I->u, I->p, I->t, k, 0, o->ccb_user) != 0;
+        else modified = 0;
+    } else if (bs > 0) {
+        if (fl[OO_CCB_HAS_AFFECT_NEG]) term = o->ccb_affect(I->u, I->p, I->t, k, 1, o->ccb_user) != 0;
+        else modified = 0;
+    }
+    if (modified) {
+        I->reeval_fsal = 1;
+        if (fl[OO_CCB_SAVE1]) { savevalues(I, 1); *saved_in_cb = 1; }
+    }
+    return term;
+}
+
 /* handle_callbacks! (integrator_utils.jl:1081-1132) for a CallbackSet of
  * DiscreteCallbacks, each applied as apply_discrete_callback!
  * (lib/DiffEqBase/src/callbacks.jl:645-683).  An affect! leaves
@@ -747,6 +915,25 @@ static int savevalues(integ_t* I, int force_save) {
 static int handle_callbacks(integ_t* I) {
     const oo_opts* o = I->o;
     int saved_in_cb = 0, term = 0;
+    if (o->nccb > 0) {
+        /* find_first_continuous_callback (callbacks.jl:157-237): the earliest
+         * event time wins, the first callback on ties */
+        int win = -1;
+        double tmin = 0.0, bs_win = 0.0, res_win = 0.0;
+        for (int k = 0; k < o->nccb; k++) {
+            double cbt, bs, rs;
+            if (ccb_find_time(I, k, &cbt, &bs, &rs) && (win < 0 || cbt < tmin)) {
+                win = k; tmin = cbt; bs_win = bs; res_win = rs;
+            }
+        }
+        if (win >= 0) {
+            I->last_event_error = res_win;
+            I->event_last = win + 1;
+            if (ccb_apply(I, win, tmin, bs_win, &saved_in_cb)) term = 1;
+        } else {
+            I->event_last = 0;
+        }
+    }
     for (int k = 0; k < o->ndcb; k++) {
         if (o->dcb_cond(I->u, I->p, I->t, k, o->dcb_user)) {
             int savedexactly = savevalues(I, 0);
@@ -813,6 +1000,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     for (int s = 0; s < 8; s++) { I->k[s] = m; m += nn; }
     I->du = m; m += nn; I->dT = m; m += nn; I->lt = m; m += nn; I->fsal = m; m += nn;
     for (int r = 0; r < 3; r++) { I->dense[r] = m; m += nn; }
+    I->ctmp = m; m += nn;
     I->J = m; m += nn * nn; I->W = m; m += nn * nn;
     I->piv = piv;
 

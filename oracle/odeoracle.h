@@ -35,6 +35,13 @@ typedef void (*oo_tgrad_fn)(double* dT, const double* u, const double* p, double
  * returns nonzero when it calls terminate! (integrator_interface.jl:443-446). */
 typedef int (*oo_dcb_cond_fn)(const double* u, const double* p, double t, int k, void* user);
 typedef int (*oo_dcb_affect_fn)(double* u, const double* p, double t, int k, void* user);
+/* ContinuousCallback k (SciMLBase; handled by lib/DiffEqBase/src/callbacks.jl:
+ * 239-640): the condition's value at (u, t); affect! (neg = 0, upcrossing) or
+ * affect_neg! (neg = 1) rewrites u and returns nonzero for terminate!. */
+typedef double (*oo_ccb_cond_fn)(const double* u, const double* p, double t, int k, void* user);
+typedef int (*oo_ccb_affect_fn)(double* u, const double* p, double t, int k, int neg, void* user);
+enum { OO_CCB_HAS_AFFECT = 0, OO_CCB_HAS_AFFECT_NEG = 1, OO_CCB_SAVE0 = 2, OO_CCB_SAVE1 = 3,
+       OO_CCB_ROOTFIND = 4 /* 0 left, 1 right, 2 none */, OO_CCB_INTERP_POINTS = 5, OO_CCB_NFLAGS = 6 };
 
 typedef struct {
     int alg;              /* OO_TSIT5 / OO_RODAS5P */
@@ -60,6 +67,12 @@ typedef struct {
     oo_dcb_affect_fn dcb_affect;
     const int* dcb_save;         /* save_positions of callback k: dcb_save[2k], dcb_save[2k+1] */
     void* dcb_user;
+    int nccb;                    /* continuous callbacks (Tsit5), checked before the discrete ones */
+    oo_ccb_cond_fn ccb_cond;
+    oo_ccb_affect_fn ccb_affect;
+    const int* ccb_flags;        /* [OO_CCB_NFLAGS * k + OO_CCB_*] */
+    const double* ccb_tol;       /* [2k]: abstol, [2k + 1]: repeat_nudge */
+    void* ccb_user;
 } oo_opts;
 
 typedef struct {

@@ -17,6 +17,10 @@ PROB_USER, PROB_LORENZ, PROB_ROBERTSON, PROB_CHAIN, PROB_LINEAR, PROB_EXPBLOWUP 
 RC_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN", 4: "MaxIters",
             5: "MaxNumSub", 6: "DtLessThanMin", 7: "Unstable"}
 
+CCB_COND_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
+CCB_AFFECT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
 DCB_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
 ISOUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -35,7 +39,10 @@ class Opts(ctypes.Structure):
                 ("tstops", ctypes.POINTER(ctypes.c_double)), ("ntstops", ctypes.c_int),
                 ("isoutofdomain", ctypes.c_void_p), ("isout_user", ctypes.c_void_p),
                 ("ndcb", ctypes.c_int), ("dcb_cond", ctypes.c_void_p), ("dcb_affect", ctypes.c_void_p),
-                ("dcb_save", ctypes.POINTER(ctypes.c_int)), ("dcb_user", ctypes.c_void_p)]
+                ("dcb_save", ctypes.POINTER(ctypes.c_int)), ("dcb_user", ctypes.c_void_p),
+                ("nccb", ctypes.c_int), ("ccb_cond", ctypes.c_void_p), ("ccb_affect", ctypes.c_void_p),
+                ("ccb_flags", ctypes.POINTER(ctypes.c_int)), ("ccb_tol", ctypes.POINTER(ctypes.c_double)),
+                ("ccb_user", ctypes.c_void_p)]
 
 
 class Result(ctypes.Structure):
@@ -112,10 +119,13 @@ class _CallbackIntegrator:
 def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0, maxiters=1000000,
               save_start=True, save_end=True, save_everystep=False, save_end_user=None, saveat=None,
               has_tgrad=False, adaptive=True, tstops=None, isoutofdomain=None, isout_shape=(0, 0),
-              callbacks=None):
+              callbacks=None, ccallbacks=None):
     """callbacks: objects with condition(u, t, integrator), affect(integrator)
     and save_positions (DiscreteCallback's fields), run as plain Python on numpy
-    copies of the state."""
+    copies of the state.  ccallbacks: objects with ContinuousCallback's fields
+    condition (a value), affect / affect_neg (callable or None), rootfind
+    ("left" / "right" / "none"), interp_points, save_positions, abstol,
+    repeat_nudge."""
     o = Opts()
     lib().oo_default_opts(ctypes.byref(o), alg)
     o.reltol, o.abstol = reltol, abstol
@@ -169,6 +179,33 @@ def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0
         o.dcb_cond, o.dcb_affect = ctypes.cast(cc, ctypes.c_void_p), ctypes.cast(ca, ctypes.c_void_p)
         o.dcb_save = sv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         keep = (keep, cc, ca, sv)
+    if ccallbacks:
+        n, np_ = isout_shape
+        ccl = list(ccallbacks)
+
+        def _ccond(u, p, t, k, _user):
+            uu = np.array(np.ctypeslib.as_array(u, shape=(max(n, 1),))[:n])
+            pp = np.array(np.ctypeslib.as_array(p, shape=(max(np_, 1),))[:np_])
+            return float(ccl[k].condition(uu, t, _CallbackIntegrator(uu.copy(), pp, t)))
+
+        def _caffect(u, p, t, k, neg, _user):
+            ua = np.ctypeslib.as_array(u, shape=(max(n, 1),))
+            pp = np.array(np.ctypeslib.as_array(p, shape=(max(np_, 1),))[:np_])
+            itg = _CallbackIntegrator(np.array(ua[:n]), pp, t)
+            (ccl[k].affect_neg if neg else ccl[k].affect)(itg)
+            ua[:n] = np.asarray(itg.u, dtype=np.float64).reshape(-1)
+            return 1 if itg._term else 0
+        rf = {"left": 0, "right": 1, "none": 2}
+        fl = np.ascontiguousarray([[int(c.affect is not None), int(c.affect_neg is not None),
+                                    int(bool(c.save_positions[0])), int(bool(c.save_positions[1])),
+                                    rf[c.rootfind], int(c.interp_points)] for c in ccl], dtype=np.int32)
+        tl = np.ascontiguousarray([[float(c.abstol), float(c.repeat_nudge)] for c in ccl], dtype=np.float64)
+        c1, c2 = CCB_COND_FN(_ccond), CCB_AFFECT_FN(_caffect)
+        o.nccb = len(ccl)
+        o.ccb_cond, o.ccb_affect = ctypes.cast(c1, ctypes.c_void_p), ctypes.cast(c2, ctypes.c_void_p)
+        o.ccb_flags = fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        o.ccb_tol = tl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        keep = (keep, c1, c2, fl, tl)
     return o, keep
 
 

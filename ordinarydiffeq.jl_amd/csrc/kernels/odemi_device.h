@@ -339,6 +339,10 @@ struct Ctl {
     // discrete callbacks: a callback modified u after the last accepted step
     // (integrator.reeval_fsal), and the condition calls so far (stats.ncondition)
     int reeval_fsal, ncond;
+    // continuous callbacks: 1 + the callback of the last event (0: none) and its
+    // condition value there (integrator.event_last_time, last_event_error)
+    int event_last;
+    double last_event_error;
 };
 
 // controller constants (alg_utils.jl:407-879; controllers.jl:744-803)
@@ -511,16 +515,20 @@ __device__ __forceinline__ void dcb_apply(Ctl& c, const Params& P, i64 j, S* u, 
     }
 }
 #endif
-template <int N, typename S, typename Q, typename SaveFn>
+// `continuous(saved_in_cb)` runs the method's continuous callbacks first
+// (find_first_continuous_callback + apply_callback!) and returns terminate!.
+template <int N, typename S, typename Q, typename SaveFn, typename ContFn>
 __device__ __forceinline__ bool handle_callbacks(Ctl& c, const Params& P, i64 j, S* u, const Q* p,
-                                                 SaveFn&& savevalues) {
+                                                 SaveFn&& savevalues, ContFn&& continuous) {
+#if defined(ODEMI_NDCB) || defined(ODEMI_NCCB)
+    bool saved_in_cb = false, term = continuous(saved_in_cb);
 #ifdef ODEMI_NDCB
-    bool saved_in_cb = false, term = false;
     dcb_apply<0, N>(c, P, j, u, p, savevalues, saved_in_cb, term);
+#endif
     if (!saved_in_cb) savevalues();
     return term;
 #else
-    (void)c; (void)P; (void)j; (void)u; (void)p;
+    (void)c; (void)P; (void)j; (void)u; (void)p; (void)continuous;
     savevalues();
     return false;
 #endif
@@ -613,7 +621,7 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     c.dtmax = (P.dtmax > 0.0) ? P.dtmax : (tf - t0);
     c.EEst = 1.0; c.q11 = 1.0; c.errold = 1e-4;
     c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0; c.isout = 0;
-    c.reeval_fsal = 0; c.ncond = 0;
+    c.reeval_fsal = 0; c.ncond = 0; c.event_last = 0; c.last_event_error = 0.0;
     c.dt = 0.0; c.dtpropose = 0.0;
     c.nf = 0; c.naccept = 0; c.nreject = 0; c.njacs = 0; c.nw = 0; c.nsolve = 0;
     c.nsaved = 0; c.spos = 0; c.send = 0; c.tf_in_saveat = 0;
@@ -930,6 +938,233 @@ __device__ __forceinline__ bool tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Pa
     return exact;
 }
 
+#ifdef ODEMI_NCCB
+// ---------------------------------------------------------------------------
+// ContinuousCallbacks on the Tsit5 lane (lib/DiffEqBase/src/callbacks.jl:
+// 157-640; OdemiCcb<K> holds the traced condition / affect! / affect_neg! and
+// the options as constants).
+// ---------------------------------------------------------------------------
+// Julia's sign(): -1, 0, 1, NaN for NaN
+__device__ __forceinline__ double jl_sign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : (x == 0 ? 0.0 : x)); }
+
+// _ode_addsteps! for Tsit5Cache (tsit_perform_step.jl:1-38, @muladd) with
+// always_calc_begin: stages k2..k7 of the step from uprev at tprev with the
+// current dt, k1 kept (reeval_internals_due_to_modification! after
+// change_t_via_interpolation!).  No stats change.  The mixed engine uses its
+// FP32 stage forms.
+template <typename S, typename K, int N>
+__device__ void tsit5_addsteps(Tsit5Lane<S, K, N>& L) {
+    typedef Tsit5Tab T;
+    const double t = L.c.tprev, dt = L.c.dt;
+    constexpr bool MIX = sizeof(K) < sizeof(S);
+    K tmp[N];
+    float ub[N];
+    const float dtf = (float)dt;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        ub[i] = (float)L.uprev[i];
+        if constexpr (MIX) tmp[i] = fmaf(dtf, (float)T::a21 * L.k1[i], ub[i]);
+        else tmp[i] = fma(dt, T::a21 * L.k1[i], L.uprev[i]);
+    }
+    odemi_f<K>(L.k2, tmp, L.p, (K)fma(T::c1, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a31 * L.k1[i];
+        acc = fma((K)T::a32, L.k2[i], acc);
+        if constexpr (MIX) tmp[i] = fmaf(dtf, acc, ub[i]);
+        else tmp[i] = fma(dt, acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k3, tmp, L.p, (K)fma(T::c2, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a41 * L.k1[i];
+        acc = fma((K)T::a42, L.k2[i], acc);
+        acc = fma((K)T::a43, L.k3[i], acc);
+        if constexpr (MIX) tmp[i] = fmaf(dtf, acc, ub[i]);
+        else tmp[i] = fma(dt, acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k4, tmp, L.p, (K)fma(T::c3, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a51 * L.k1[i];
+        acc = fma((K)T::a52, L.k2[i], acc);
+        acc = fma((K)T::a53, L.k3[i], acc);
+        acc = fma((K)T::a54, L.k4[i], acc);
+        if constexpr (MIX) tmp[i] = fmaf(dtf, acc, ub[i]);
+        else tmp[i] = fma(dt, acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k5, tmp, L.p, (K)fma(T::c4, dt, t));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a61 * L.k1[i];
+        acc = fma((K)T::a62, L.k2[i], acc);
+        acc = fma((K)T::a63, L.k3[i], acc);
+        acc = fma((K)T::a64, L.k4[i], acc);
+        acc = fma((K)T::a65, L.k5[i], acc);
+        if constexpr (MIX) tmp[i] = fmaf(dtf, acc, ub[i]);
+        else tmp[i] = fma(dt, acc, L.uprev[i]);
+    }
+    odemi_f<K>(L.k6, tmp, L.p, (K)(t + dt));
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        K acc = (K)T::a71 * L.k1[i];
+        acc = fma((K)T::a72, L.k2[i], acc);
+        acc = fma((K)T::a73, L.k3[i], acc);
+        acc = fma((K)T::a74, L.k4[i], acc);
+        acc = fma((K)T::a75, L.k5[i], acc);
+        acc = fma((K)T::a76, L.k6[i], acc);
+        tmp[i] = (K)fma(dt, (double)acc, (double)L.uprev[i]);
+    }
+    odemi_f<K>(L.k7, tmp, L.p, (K)(t + dt));
+}
+
+// get_condition (callbacks.jl:91-146): u at t, uprev at tprev, else the
+// step's dense output; every call counts in stats.ncondition.
+template <int CB, typename S, typename K, int N>
+__device__ __forceinline__ double ccb_condition(Tsit5Lane<S, K, N>& L, double abst) {
+    Ctl& c = L.c;
+    c.ncond++;
+    if (abst == c.t) return (double)OdemiCcb<CB>::template cond<double, K>(L.u, L.p, abst);
+    if (abst == c.tprev) return (double)OdemiCcb<CB>::template cond<double, K>(L.uprev, L.p, abst);
+    double v[N];
+    tsit5_interp(L, (abst - c.tprev) / c.dt, c.dt, v);
+    return (double)OdemiCcb<CB>::template cond<double, K>(v, L.p, abst);
+}
+
+// is_event_occurrence (callbacks.jl:525-530)
+template <int CB>
+__device__ __forceinline__ bool ccb_is_event(double prev_sign, double next_sign) {
+    return ((prev_sign < 0 && OdemiCcb<CB>::has_affect) || (prev_sign > 0 && OdemiCcb<CB>::has_affect_neg)) &&
+           prev_sign * next_sign <= 0;
+}
+
+// find_callback_time (callbacks.jl:364-412) with nudge_tprev (414-426),
+// check_event_occurrence (428-451; the interp_points checks at
+// tprev + (i - 1) h, h = (t - tprev) / (interp_points - 1), t last) and
+// find_root (480-497): the sign change bracketed down to adjacent floats by
+// bisection (the interval solve with abstol = reltol = 0), the bottom-side
+// float for LeftRootFind, the other for RightRootFind.  Bisection in FP64
+// halves the bracket each call, so the loop ends within ~2100 calls for any
+// finite bracket.
+template <int CB, typename S, typename K, int N>
+__device__ bool ccb_find_time(Tsit5Lane<S, K, N>& L, double& cb_t, double& bottom_sign, double& residual) {
+    typedef OdemiCcb<CB> C;
+    Ctl& c = L.c;
+    double bottom_t = c.tprev;
+    double bc = ccb_condition<CB>(L, bottom_t);
+    if (c.event_last == CB + 1) {
+        if (fabs(bc - c.last_event_error) <= C::abstol) bottom_t = c.tprev + c.dt * C::repeat_nudge;
+        bc = ccb_condition<CB>(L, bottom_t);
+    }
+    const double bs = jl_sign(bc);
+    double top_t = c.t;
+    double ts = jl_sign(ccb_condition<CB>(L, top_t));
+    bool ev = ccb_is_event<CB>(bs, ts);
+    if (C::interp_points != 0 && !ev) {
+        const double h = (c.t - c.tprev) / (double)(C::interp_points - 1);
+        for (int i = 2; i <= C::interp_points; i++) {
+            top_t = (i == C::interp_points) ? c.t : fma((double)(i - 1), h, c.tprev);
+            ts = jl_sign(ccb_condition<CB>(L, top_t));
+            ev = ccb_is_event<CB>(bs, ts);
+            if (ev) break;
+        }
+    }
+    bottom_sign = bs;
+    if (!ev) { cb_t = c.t; residual = 0.0; return false; }
+    if (C::rootfind == 2 || ts == 0.0) { cb_t = top_t; residual = 0.0; return true; }
+    double l = bottom_t, r = top_t;
+    for (int it = 0; it < 2200; it++) {
+        const double m = l + 0.5 * (r - l);
+        if (m == l || m == r) break;
+        const double sm = jl_sign(ccb_condition<CB>(L, m));
+        if (sm == 0.0) { l = m; r = m; break; }
+        if (sm == bs) l = m;
+        else r = m;
+    }
+    cb_t = (C::rootfind == 1) ? r : l;
+    residual = ccb_condition<CB>(L, cb_t);
+    return true;
+}
+
+// apply_callback! (callbacks.jl:557-640): set_proposed_dt! (dtrelax = 1),
+// change_t_via_interpolation! (integrator_interface.jl:5-40: u from the dense
+// output, t, dt = t - tprev, stages recomputed), saves, affect! on an
+// upcrossing / affect_neg! on a downcrossing (a missing one modifies nothing).
+// Returns terminate!.
+template <int CB, typename S, typename K, int N, typename SaveFn>
+__device__ bool ccb_apply(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, double cb_t, double bs,
+                          SaveFn& savevalues, bool& saved_in_cb) {
+    typedef OdemiCcb<CB> C;
+    Ctl& c = L.c;
+    c.dtpropose = jl_max(nextfloat_pos(P.dtmin), c.dt);
+    if (cb_t != c.t) {
+        double v[N];
+        tsit5_interp(L, (cb_t - c.tprev) / c.dt, c.dt, v);
+#pragma unroll
+        for (int i = 0; i < N; i++) L.u[i] = (S)v[i];
+        c.t = cb_t;
+        c.dt = c.t - c.tprev;
+        tsit5_addsteps(L);
+        c.reeval_fsal = 1;
+    }
+    const bool savedexactly = savevalues();
+    saved_in_cb = true;
+    if (C::save0 && !savedexactly) save_point<N>(c, P, j, c.t, L.u);
+    bool modified = true, term = false;
+    if (bs < 0) {
+        if constexpr (C::has_affect != 0) { C::template affect<S, K>(L.u, L.p, (S)c.t); term = C::term != 0; }
+        else modified = false;
+    } else if (bs > 0) {
+        if constexpr (C::has_affect_neg != 0) {
+            C::template affect_neg<S, K>(L.u, L.p, (S)c.t);
+            term = C::term_neg != 0;
+        } else {
+            modified = false;
+        }
+    }
+    if (modified) {
+        c.reeval_fsal = 1;
+        if (C::save1) { save_point<N>(c, P, j, c.t, L.u); saved_in_cb = true; }
+    }
+    return term;
+}
+
+template <int CB, typename S, typename K, int N>
+__device__ __forceinline__ void ccb_find_first(Tsit5Lane<S, K, N>& L, int& win, double& tmin, double& bs_win,
+                                               double& res_win) {
+    if constexpr (CB < ODEMI_NCCB) {
+        double cbt, bs, rs;
+        if (ccb_find_time<CB>(L, cbt, bs, rs) && (win < 0 || cbt < tmin)) {
+            win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+        }
+        ccb_find_first<CB + 1>(L, win, tmin, bs_win, res_win);
+    }
+}
+template <int CB, typename S, typename K, int N, typename SaveFn>
+__device__ __forceinline__ bool ccb_apply_winner(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, int win, double cbt,
+                                                 double bs, SaveFn& savevalues, bool& saved_in_cb) {
+    if constexpr (CB < ODEMI_NCCB) {
+        if (win == CB) return ccb_apply<CB>(L, P, j, cbt, bs, savevalues, saved_in_cb);
+        return ccb_apply_winner<CB + 1>(L, P, j, win, cbt, bs, savevalues, saved_in_cb);
+    } else {
+        return false;
+    }
+}
+// find_first_continuous_callback (callbacks.jl:157-237): the earliest event
+// wins, the first callback on ties; then apply_callback! for it.
+template <typename S, typename K, int N, typename SaveFn>
+__device__ bool tsit5_continuous_callbacks(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, SaveFn& savevalues,
+                                           bool& saved_in_cb) {
+    int win = -1;
+    double tmin = 0.0, bs = 0.0, rs = 0.0;
+    ccb_find_first<0>(L, win, tmin, bs, rs);
+    if (win < 0) { L.c.event_last = 0; return false; }
+    L.c.last_event_error = rs;
+    L.c.event_last = win + 1;
+    return ccb_apply_winner<0>(L, P, j, win, tmin, bs, savevalues, saved_in_cb);
+}
+#endif
+
 // One step attempt: loopheader! + check_error! + perform_step! + loopfooter!.
 // Returns 0 while running, otherwise the final retcode (lane finished).
 template <bool SAVES, typename S, typename K, int N>
@@ -942,7 +1177,7 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
             c.success_iter++;
 #pragma unroll
             for (int i = 0; i < N; i++) { L.uprev[i] = L.u[i]; L.k1[i] = L.k7[i]; }
-#ifdef ODEMI_NDCB
+#if defined(ODEMI_NDCB) || defined(ODEMI_NCCB)
             if (c.reeval_fsal) {   // update_fsal! -> reset_fsal! (integrator_utils.jl:215-239,1325-1338)
                 K uk[N];
 #pragma unroll
@@ -981,11 +1216,19 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
         c.tprev = c.t;
         if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tnext; }
         else c.t = ttmp;
-        handle_tstop(c, P);
         accept_controller<HW>(c, kc, q, P.dtmin);
-        const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, [&]() -> bool {
+        auto savefn = [&]() -> bool {
             return (SAVES && (P.nsaveat > 0 || P.save_everystep)) ? tsit5_savevalues(L, P, j) : false;
+        };
+        const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, savefn, [&](bool& saved_in_cb) -> bool {
+#ifdef ODEMI_NCCB
+            return tsit5_continuous_callbacks(L, P, j, savefn, saved_in_cb);
+#else
+            (void)saved_in_cb;
+            return false;
+#endif
         });
+        handle_tstop(c, P);   // after the callbacks: an event can move t back before the tstop
         if (term) return RC_TERMINATED;
     } else {
         c.nreject++;
@@ -1062,16 +1305,17 @@ __device__ void tsit5_persistent(const Params& P) {
 }
 
 // Plain engine: one kernel launch per step attempt, state in HBM.
-// st_scalar rows: t dt dtpropose tprev (unused) tf dtmax EEst q11 errold
+// st_scalar rows: t dt dtpropose tprev last_event_error tf dtmax EEst q11 errold
 // st_int rows: iter success_iter accept_step next_step_tstop nf naccept nreject done
 //               saveat cursor, njacs nw nsolve (Rodas5P), tstops cursor,
-//               isout | reeval_fsal << 1, ncondition
+//               isout | reeval_fsal << 1, ncondition, event_last
 // st_vec rows: u uprev k1 k7 (Tsit5)
 template <int N>
 __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
     const double* s = P.st_scalar;
     const i64 M = P.N;
     c.t = s[0 * M + j]; c.dt = s[1 * M + j]; c.dtpropose = s[2 * M + j]; c.tprev = s[3 * M + j];
+    c.last_event_error = s[4 * M + j];
     c.tf = s[5 * M + j]; c.dtmax = s[6 * M + j];
     c.EEst = s[7 * M + j]; c.q11 = s[8 * M + j]; c.errold = s[9 * M + j];
     const int* q = P.st_int;
@@ -1093,6 +1337,7 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
     const int fl = q[13 * M + j];
     c.isout = fl & 1; c.reeval_fsal = (fl >> 1) & 1;
     c.ncond = q[14 * M + j];
+    c.event_last = q[15 * M + j];
     c.njacs = 0; c.nw = 0; c.nsolve = 0;
 }
 template <int N>
@@ -1100,6 +1345,7 @@ __device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i
     double* s = P.st_scalar;
     const i64 M = P.N;
     s[0 * M + j] = c.t; s[1 * M + j] = c.dt; s[2 * M + j] = c.dtpropose; s[3 * M + j] = c.tprev;
+    s[4 * M + j] = c.last_event_error;
     s[5 * M + j] = c.tf; s[6 * M + j] = c.dtmax;
     s[7 * M + j] = c.EEst; s[8 * M + j] = c.q11; s[9 * M + j] = c.errold;
     int* q = P.st_int;
@@ -1108,6 +1354,7 @@ __device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i
     q[6 * M + j] = c.nreject; q[7 * M + j] = done; q[8 * M + j] = c.spos; q[12 * M + j] = c.tsp;
     q[13 * M + j] = c.isout | (c.reeval_fsal << 1);
     q[14 * M + j] = c.ncond;
+    q[15 * M + j] = c.event_last;
     if (P.nsaved) P.nsaved[j] = c.nsaved;
 }
 
@@ -1490,11 +1737,12 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
         c.tprev = c.t;
         if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tnext; }
         else c.t = ttmp;
-        handle_tstop(c, P);
         accept_controller<HW>(c, kc, q, P.dtmin);
+        // continuous callbacks are Tsit5-only (the runtime refuses Rodas5P modules with them)
         const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, [&]() -> bool {
             return (P.nsaveat > 0 || P.save_everystep) ? rodas_savevalues(L, P, j) : false;
-        });
+        }, [](bool&) -> bool { return false; });
+        handle_tstop(c, P);
         if (term) return RC_TERMINATED;
     } else {
         c.nreject++;

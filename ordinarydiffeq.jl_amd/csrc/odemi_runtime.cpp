@@ -39,7 +39,7 @@ const char* kDeviceSource =
 // rows of the int32 integrator state of the plain engine and the serving pool
 // (iter, success_iter, accept, next_step_tstop, nf, naccept, nreject, slot
 // state, saveat cursor, njacs, nw, nsolve, tstops cursor, isoutofdomain verdict)
-constexpr size_t kIntRows = 15;
+constexpr size_t kIntRows = 16;
 // DEStats rows written per trajectory: nf naccept nreject njacs nw nsolve ncondition
 constexpr size_t kStatRows = 7;
 
@@ -197,7 +197,7 @@ struct odemi_rhs {
     odemi_ctx* ctx = nullptr;
     hipModule_t mod = nullptr;
     int n = 0, np = 0, flags = 0;
-    int features = 0;   // 1: isoutofdomain, 2: discrete callbacks (lane kernels only)
+    int features = 0;   // 1: isoutofdomain, 2: discrete, 4: continuous callbacks (lane kernels; 4: Tsit5 only)
     hipFunction_t tsit5_persist_f64 = nullptr, tsit5_persist_mixed = nullptr;
     hipFunction_t tsit5_persist_f64_sv = nullptr, tsit5_persist_mixed_sv = nullptr;
     hipFunction_t tsit5_plain_init = nullptr, tsit5_plain_step = nullptr;
@@ -290,7 +290,7 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     s << "extern \"C\" __device__ const unsigned odemi_params_bytes = sizeof(odemi::Params);\n";
     // per-step hooks of the problem source, which only the lane kernels run
     s << "extern \"C\" __device__ const int odemi_features = 0\n#ifdef ODEMI_HAS_ISOUT\n | 1\n#endif\n"
-         "#ifdef ODEMI_NDCB\n | 2\n#endif\n;\n";
+         "#ifdef ODEMI_NDCB\n | 2\n#endif\n#ifdef ODEMI_NCCB\n | 4\n#endif\n;\n";
     if (flags & ODEMI_COMPILE_TSIT5) {
         // persistent kernels: final-state-only (no save path) and saving variants
         const char* v[4][3] = {{"odemi_tsit5_persist_f64", "false", "double"},
@@ -626,6 +626,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         ctx->err = "isoutofdomain and callbacks run in the lane kernels only (compile without ODEMI_COMPILE_WG)";
         return -3;
     }
+    if (rodas && (rhs->features & 4)) { ctx->err = "continuous callbacks are supported with Tsit5 only"; return -3; }
 
     hipStream_t s = a->stream ? (hipStream_t)a->stream : ctx->stream;
     const bool dev = a->device_ptrs != 0;

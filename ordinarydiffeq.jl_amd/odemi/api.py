@@ -178,26 +178,60 @@ class DiscreteCallback:
         self.save_positions = (bool(save_positions[0]), bool(save_positions[1]))
 
 
+_SAME = object()
+
+
+class ContinuousCallback:
+    """ContinuousCallback(condition, affect!, affect_neg! = affect!; rootfind,
+    save_positions, interp_points, abstol, repeat_nudge) (SciMLBase; handled by
+    lib/DiffEqBase/src/callbacks.jl:239-640), Tsit5 only.  condition(u, t,
+    integrator) is a traced value whose sign change within an accepted step is
+    an event: the step is cut back to the root (found to adjacent floats; the
+    float before the root for rootfind="left", after it for "right", the first
+    detected sample point for "none") and affect (upcrossing, - to +) or
+    affect_neg (downcrossing) runs there; None makes that direction no event.
+    Sign changes are looked for at the step's ends and, when those show none,
+    at interp_points equally spaced times."""
+
+    def __init__(self, condition, affect, affect_neg=_SAME, *, rootfind="left", save_positions=(True, True),
+                 interp_points=10, abstol=10 * 2.220446049250313e-16, repeat_nudge=1 / 100):
+        if rootfind not in ("left", "right", "none"):
+            raise ValueError('rootfind is "left", "right" or "none"')
+        if int(interp_points) == 1 or int(interp_points) < 0:
+            raise ValueError("interp_points is 0 or at least 2")
+        if len(save_positions) != 2:
+            raise ValueError("save_positions is a pair of Bools")
+        self.condition, self.affect = condition, affect
+        self.affect_neg = affect if affect_neg is _SAME else affect_neg
+        self.rootfind, self.interp_points = rootfind, int(interp_points)
+        self.save_positions = (bool(save_positions[0]), bool(save_positions[1]))
+        self.abstol, self.repeat_nudge = float(abstol), float(repeat_nudge)
+
+
 class CallbackSet:
-    """CallbackSet(cb...) of DiscreteCallbacks, applied in order."""
+    """CallbackSet(cb...): continuous callbacks are checked first (the earliest
+    event of the step wins), then the discrete ones in order."""
 
     def __init__(self, *callbacks):
-        self.discrete_callbacks = _discrete_list(list(callbacks))
+        cbs = _callback_list(list(callbacks))
+        self.continuous_callbacks = [c for c in cbs if isinstance(c, ContinuousCallback)]
+        self.discrete_callbacks = [c for c in cbs if isinstance(c, DiscreteCallback)]
 
 
-def _discrete_list(cb):
+def _callback_list(cb):
     if cb is None:
         return []
-    if isinstance(cb, DiscreteCallback):
+    if isinstance(cb, (DiscreteCallback, ContinuousCallback)):
         return [cb]
     if isinstance(cb, CallbackSet):
-        return list(cb.discrete_callbacks)
+        return list(cb.continuous_callbacks) + list(cb.discrete_callbacks)
     if isinstance(cb, (list, tuple)):
         out = []
         for c in cb:
-            out += _discrete_list(c)
+            out += _callback_list(c)
         return out
-    raise TypeError(f"unsupported callback {type(cb).__name__} (DiscreteCallback / CallbackSet only)")
+    raise TypeError(f"unsupported callback {type(cb).__name__} (DiscreteCallback / ContinuousCallback / "
+                    "CallbackSet)")
 
 
 def terminate(integrator):
@@ -246,24 +280,34 @@ template <typename T> __device__ __forceinline__ void odemi_tgrad(T* dT, const T
 """
 
 
+def _callback_key(c):
+    if isinstance(c, DiscreteCallback):
+        return ("d", id(c.condition), id(c.affect), c.save_positions)
+    return ("c", id(c.condition), id(c.affect), id(c.affect_neg), c.rootfind, c.interp_points,
+            c.save_positions, c.abstol, c.repeat_nudge)
+
+
 def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, isoutofdomain=None,
                     callback=None):
     """Trace fun (ODEFunction) for state size n (or the sizes `ns` of an uneven
     batch, n = max) and hipRTC-compile it.  isoutofdomain(u, p, t): a traceable
     predicate (comparisons, &, |, ~, odemi.sym_any / sym_all) for the lane
-    kernels; callback: DiscreteCallback / CallbackSet, traced likewise.
+    kernels; callback: DiscreteCallback / ContinuousCallback / CallbackSet,
+    traced likewise.
     Returns (rhs, traced, seconds)."""
     wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
-    cbs = _discrete_list(callback)
+    cbs = _callback_list(callback)
     if wg and (isoutofdomain is not None or cbs):
         raise ValueError("isoutofdomain and callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
                          f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
+    if "rodas5p" in algs and any(isinstance(c, ContinuousCallback) for c in cbs):
+        raise ValueError("ContinuousCallback is supported with Tsit5 only")
     if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
         raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
     if ns is not None and "tsit5" in algs:
         raise ValueError("uneven-size batches (nvec) are supported with Rodas5P")
     key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_, tuple(ns) if ns is not None else None, wg,
-           id(isoutofdomain), tuple((id(c.condition), id(c.affect), c.save_positions) for c in cbs))
+           id(isoutofdomain), tuple(_callback_key(c) for c in cbs))
     tr = _TRACES.get(key)
     if tr is None:
         if wg:
@@ -513,7 +557,7 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=No
         sa = saveat_times(saveat, 0, 0)
     else:
         sa = np.zeros(0)
-    cbs = _discrete_list(callback)
+    cbs = _callback_list(callback)
     forced = any(any(c.save_positions) for c in cbs)
     if max_saves is None and (sev or forced):
         max_saves = 4096

@@ -536,11 +536,67 @@ def _emit_callbacks(callbacks, n, u, p, t):
     return "\n".join(src)
 
 
+def _emit_affect(affect, n, u, p, t, indent, name):
+    """affect(integrator) traced into `name(T* u, const Q* p, T t)`; returns
+    (source lines, terminate! called)."""
+    itg = _TracedIntegrator(u, p, t)
+    affect(itg)
+    outs = [(i, v) for i, v in enumerate(itg.u.vals) if v is not u[i]]
+    src = [f"{indent}template <typename T, typename Q>",
+           f"{indent}static __device__ __forceinline__ void {name}(T* __restrict__ u, "
+           "const Q* __restrict__ p, T t) {",
+           f"{indent}    (void)u; (void)p; (void)t;"]
+    if outs:
+        src.append(f"{indent}    T nu_[{n}];")
+        src.append(_emit_body(outs, "nu_", indent=indent + "    ", pname="T(p[{}])"))
+        for i, _ in outs:
+            src.append(f"{indent}    u[{i}] = nu_[{i}];")
+    src.append(f"{indent}}}")
+    return src, itg._term
+
+
+def _emit_continuous_callbacks(callbacks, n, u, p, t):
+    """ContinuousCallbacks as OdemiCcb<K>: the condition's value traced on
+    (u, t, integrator), affect! / affect_neg! (None: that crossing is no event)
+    and the options as constants."""
+    src = [f"#define ODEMI_NCCB {len(callbacks)}", "template <int K> struct OdemiCcb;"]
+    for k, cb in enumerate(callbacks):
+        c = _lift(cb.condition(list(u), t, _TracedIntegrator(u, p, t)))
+        s0, s1 = (int(bool(x)) for x in cb.save_positions)
+        rf = {"left": 0, "right": 1, "none": 2}[cb.rootfind]
+        body = [f"template <> struct OdemiCcb<{k}> {{"]
+        aff_src, term = ([], False)
+        neg_src, term_neg = ([], False)
+        if cb.affect is not None:
+            aff_src, term = _emit_affect(cb.affect, n, u, p, t, "    ", "affect")
+        if cb.affect_neg is not None:
+            neg_src, term_neg = _emit_affect(cb.affect_neg, n, u, p, t, "    ", "affect_neg")
+        body.append(f"    static constexpr int has_affect = {int(cb.affect is not None)}, "
+                    f"has_affect_neg = {int(cb.affect_neg is not None)}, save0 = {s0}, save1 = {s1};")
+        body.append(f"    static constexpr int rootfind = {rf}, interp_points = {int(cb.interp_points)}, "
+                    f"term = {int(term)}, term_neg = {int(term_neg)};")
+        body.append(f"    static constexpr double abstol = {float(cb.abstol).hex()}, "
+                    f"repeat_nudge = {float(cb.repeat_nudge).hex()};")
+        body.append("    template <typename T, typename Q>")
+        body.append("    static __device__ __forceinline__ T cond(const T* __restrict__ u, "
+                    "const Q* __restrict__ p, T t) {")
+        body.append("        (void)u; (void)p; (void)t;")
+        body.append("        T r_[1];")
+        body.append(_emit_body([(0, c)], "r_", indent="        ", pname="T(p[{}])"))
+        body.append("        return r_[0];")
+        body.append("    }")
+        body += aff_src + neg_src
+        body.append("};")
+        src += body
+    return "\n".join(src)
+
+
 def trace_problem(f, n, np_, jac=None, tgrad=None, isout=None, callbacks=None):
     """Trace f (and jac/tgrad when given) for state dimension n and np_ params;
     isout(u, p, t) is the isoutofdomain predicate (solve kwarg), traced into
     odemi_isout; callbacks: DiscreteCallbacks (condition, affect, save_positions)
-    traced into OdemiDcb<K>."""
+    traced into OdemiDcb<K>, ContinuousCallbacks (also affect_neg, rootfind,
+    interp_points, abstol, repeat_nudge) into OdemiCcb<K>."""
     global _CUR
     g = _Graph()
     _CUR = g
@@ -605,8 +661,12 @@ def trace_problem(f, n, np_, jac=None, tgrad=None, isout=None, callbacks=None):
             src.append(_emit_body([(0, r)], "r_", pname="T(p[{}])"))
             src.append("    return r_[0] != T(0);")
             src.append("}")
-        if callbacks:
-            src.append(_emit_callbacks(callbacks, n, u, p, t))
+        dcbs = [c for c in (callbacks or []) if not hasattr(c, "affect_neg")]
+        ccbs = [c for c in (callbacks or []) if hasattr(c, "affect_neg")]
+        if dcbs:
+            src.append(_emit_callbacks(dcbs, n, u, p, t))
+        if ccbs:
+            src.append(_emit_continuous_callbacks(ccbs, n, u, p, t))
         return TracedProblem(n, np_, "\n".join(src) + "\n", has_user_jac, tgrad is not None, autonomous,
                              _count_flops(fout), _count_flops(jout))
     finally:

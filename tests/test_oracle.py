@@ -444,6 +444,64 @@ def test_discrete_callback_reference_known_answers():
         assert abs(r["u"][0, 0] - 20 + 10 * math.exp(-r["t_final"] / 2)) < 1e-2
 
 
+class _CCB:
+    """ContinuousCallback's fields with the reference's defaults."""
+
+    def __init__(self, condition, affect, affect_neg="same", rootfind="left", interp_points=10,
+                 save_positions=(True, True), abstol=10 * 2.220446049250313e-16, repeat_nudge=1 / 100):
+        self.condition, self.affect = condition, affect
+        self.affect_neg = affect if affect_neg == "same" else affect_neg
+        self.rootfind, self.interp_points, self.save_positions = rootfind, interp_points, save_positions
+        self.abstol, self.repeat_nudge = abstol, repeat_nudge
+
+
+def test_continuous_callback_reference_known_answers():
+    """ContinuousCallback handling (lib/DiffEqBase/src/callbacks.jl:157-640,
+    change_t_via_interpolation!, the Tsit5 re-step) against the reference's
+    bouncing-ball tests (test/Integrators_I/ode_event_tests.jl:100-200,
+    300-316): u = (height, velocity), u' = (v, -9.81), u0 = (50, 0); with
+    affect! = nothing, affect_neg! flipping v and interp_points = 100 the middle
+    save of save_everystep = false is an event, saveat at it or at t - eps(t)
+    gives that time exactly twice, v < 0 before and > 0 after; terminate! on the
+    ground (tspan (0, Inf)) ends Terminated at t = sqrt(100 / 9.81) with height
+    < 3e-12.  Closed forms: the bounce times are sqrt(100/9.81) (1 + 2k) to the
+    root's float; RightRootFind lands just past the root (height < 0),
+    LeftRootFind just before it."""
+    def fall(du, u, p, t):
+        du[0] = u[1]
+        du[1] = -9.81
+
+    def flip(itg):
+        itg.u[1] = -itg.u[1]
+    t1 = math.sqrt(100 / 9.81)
+    cb = _CCB(lambda u, t, itg: u[0], None, flip, interp_points=100)
+    r = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), ccallbacks=[cb], save_everystep=False)
+    assert r["retcode"] == 1 and len(r["t"]) == 6
+    t = r["t"][len(r["t"]) // 2 - 1]
+    assert abs(t - t1) < 1e-13 and abs(r["t"][3] - 3 * t1) < 1e-12
+    for sa in ([t], [t - np.spacing(t)]):
+        r2 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), ccallbacks=[cb], saveat=sa, save_start=False,
+                     save_end=False)
+        ts = list(r2["t"])
+        assert ts.count(t) == 2
+        i = ts.index(t)
+        assert r2["u"][i, 1] < 0 < r2["u"][i + 1, 1]
+
+    def stop(itg):
+        itg.terminate()
+    for rf in ("left", "right"):
+        r5 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, math.inf), ccallbacks=[_CCB(lambda u, t, itg: u[0], stop,
+                                                                                   rootfind=rf)])
+        assert r5["retcode"] == 2 and abs(r5["t"][-1] - t1) < 1e-13
+        assert abs(r5["u"][-1][0]) < 3e-12 and (r5["u"][-1][0] > 0) == (rf == "left")
+    # no event: a condition that never changes sign changes nothing but the condition count
+    r6 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 2.0), save_everystep=True)
+    r7 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 2.0), save_everystep=True,
+                 ccallbacks=[_CCB(lambda u, t, itg: u[0] + 100.0, flip)])
+    assert np.array_equal(r6["t"], r7["t"]) and np.array_equal(r6["u"], r7["u"])
+    assert r7["ncondition"] == r7["naccept"] * 11   # tprev, t and 9 interpolated points per step
+
+
 def test_saveat_interpolant_accuracy():
     # dense output (Tsit5 free 4th-order interpolant, Rodas 3rd-order H-rows) vs exp
     sa = list(np.linspace(0.05, 0.95, 19))

@@ -83,3 +83,38 @@ def test_callbacks_rejected_by_workgroup_kernels_and_unknown_types():
     with pytest.raises(TypeError):
         trace_problem(_decay, 2, 2, callbacks=[odemi.DiscreteCallback(lambda u, t, i: u[0] if u[0] > 1 else 0,
                                                                      _dose)])
+
+
+def test_continuous_callbacks_emit_and_compile():
+    def fall(du, u, p, t):
+        du[0] = u[1]
+        du[1] = -p[0]
+
+    def bounce(itg):
+        itg.u[1] = -p_e(itg) * itg.u[1]
+
+    def p_e(itg):
+        return itg.p[1]
+    cbs = odemi.CallbackSet(odemi.ContinuousCallback(lambda u, t, i: u[0], None, bounce, interp_points=0),
+                            odemi.ContinuousCallback(lambda u, t, i: t - 7.5, lambda i: odemi.terminate(i),
+                                                     rootfind="right", save_positions=(False, True)),
+                            odemi.DiscreteCallback(_dose_cond, _dose))
+    assert len(cbs.continuous_callbacks) == 2 and len(cbs.discrete_callbacks) == 1
+    tr = trace_problem(fall, 2, 2, callbacks=cbs.continuous_callbacks + cbs.discrete_callbacks)
+    src = tr.src
+    assert "#define ODEMI_NCCB 2" in src and "#define ODEMI_NDCB 1" in src
+    c0 = src[src.index("struct OdemiCcb<0>"):src.index("struct OdemiCcb<1>")]
+    assert "has_affect = 0, has_affect_neg = 1" in c0 and "interp_points = 0" in c0 and "affect_neg(" in c0
+    c1 = src[src.index("struct OdemiCcb<1>"):]
+    assert "rootfind = 1" in c1 and "term = 1, term_neg = 1" in c1 and "save0 = 0, save1 = 1" in c1
+    L = _lib.load()
+    path = ctypes.create_string_buffer(512)
+    log = ctypes.create_string_buffer(1 << 16)
+    secs = ctypes.c_double()
+    rc = L.odemi_rtc_codeobject(tr.src.encode(), 2, 2, _lib.COMPILE_TSIT5, path, 512, ctypes.byref(secs), log,
+                                1 << 16)
+    assert rc == 0, log.value.decode()[:2000]
+    with pytest.raises(ValueError, match="Tsit5 only"):
+        odemi.compile_problem(odemi.ODEFunction(fall), 2, 2, algs=("rodas5p",), callback=cbs)
+    with pytest.raises(ValueError):
+        odemi.ContinuousCallback(lambda u, t, i: u[0], bounce, interp_points=1)
