@@ -744,8 +744,9 @@ static void rodas_step(integ_t* I) {
         du[i] = acc;
     }
     I->EEst = residual_norm(du, uprev, u, n, I->abstol, I->reltol);
-    /* dense rows (calck <=> saveat non-empty) */
-    if (I->n_saveat > 0) {
+    /* dense rows (calck: saveat non-empty, or continuous callbacks, whose
+     * event search interpolates every step) */
+    if (I->n_saveat > 0 || I->o->nccb > 0) {
         for (int r = 0; r < 3; r++)
             for (int i = 0; i < n; i++) {
                 double acc = 0.0;
@@ -753,6 +754,56 @@ static void rodas_step(integ_t* I) {
                 I->dense[r][i] = acc;
             }
     }
+}
+
+/* _ode_addsteps! for RosenbrockCache (stiff_addsteps.jl:140-224) with
+ * always_calc_begin, after change_t_via_interpolation!: the stages of the step
+ * from uprev at tprev with the new dt, reusing the step's J and dT and its
+ * fsalfirst f(uprev, tprev); W refactored.  That function has no @muladd: only
+ * linsolve_tmp = fsalfirst + dtgamma dT (an explicit @muladd) is fused, the
+ * stage sums, t + c dt, the stage right-hand sides and the dense rows
+ * H ks round the product and the sum separately; dtC = C ./ dt.  The f calls
+ * and solves are not counted in the stats (they go through the cache). */
+static void rodas_addsteps(integ_t* I) {
+    const int n = I->n;
+    const double t = I->tprev, dt = I->dt;
+    double *uprev = I->uprev, *du = I->du, *dT = I->dT, *lt = I->lt, *st = I->atmp;
+    double dtC[8][7], dtd[8];
+    for (int s = 0; s < 8; s++) {
+        for (int j = 0; j < 7; j++) dtC[s][j] = RP_C[s][j] / dt;
+        dtd[s] = dt * RP_d[s];
+    }
+    const double dtgamma = dt * RP_gamma;
+    I->f(I->fsal, uprev, I->p, t, I->user);
+    for (int i = 0; i < n; i++) lt[i] = fma(dtgamma, dT[i], I->fsal[i]);
+    const double invdtgamma = 1.0 / dtgamma;
+    for (int i = 0; i < n * n; i++) I->W[i] = I->J[i];
+    for (int i = 0; i < n; i++) I->W[i * n + i] = fma(-1.0, invdtgamma, I->J[i * n + i]);
+    lu_factor(n, I->W, I->piv);
+    for (int i = 0; i < n; i++) I->tmp[i] = lt[i];
+    lu_solve(n, I->W, I->piv, I->tmp);
+    for (int i = 0; i < n; i++) I->k[0][i] = -I->tmp[i];
+    for (int s = 1; s < 8; s++) {
+        for (int i = 0; i < n; i++) {
+            double acc = uprev[i];
+            for (int j = 0; j < s; j++) acc = acc + RP_A[s][j] * I->k[j][i];
+            st[i] = acc;
+        }
+        I->f(du, st, I->p, t + RP_c[s] * dt, I->user);
+        for (int i = 0; i < n; i++) {
+            double acc = du[i] + dtd[s] * dT[i];
+            for (int j = 0; j < s; j++) acc = acc + dtC[s][j] * I->k[j][i];
+            I->tmp[i] = acc;
+        }
+        lu_solve(n, I->W, I->piv, I->tmp);
+        for (int i = 0; i < n; i++) I->k[s][i] = -I->tmp[i];
+    }
+    for (int r = 0; r < 3; r++)
+        for (int i = 0; i < n; i++) {
+            double acc = 0.0;
+            for (int j = 0; j < 8; j++) acc = acc + RP_H[r][j] * I->k[j][i];
+            I->dense[r][i] = acc;
+        }
 }
 
 /* rosenbrock_interpolants.jl:172-206, interp_order = 3 */
@@ -808,7 +859,9 @@ static double ccb_condition(integ_t* I, int k, double abst) {
     } else if (abst == I->tprev) {
         v = I->uprev;
     } else {
-        tsit5_interp(I, (abst - I->tprev) / I->dt, I->dt, I->ctmp);
+        const double th = (abst - I->tprev) / I->dt;
+        if (I->alg == OO_TSIT5) tsit5_interp(I, th, I->dt, I->ctmp);
+        else rodas_interp(I, th, I->ctmp);
         v = I->ctmp;
     }
     I->res->ncondition++;
@@ -882,10 +935,14 @@ static int ccb_apply(integ_t* I, int k, double cb_t, double bs, int* saved_in_cb
         I->dtcache = dtp;
     }
     if (cb_t != I->t) {
-        tsit5_interp(I, (cb_t - I->tprev) / I->dt, I->dt, I->u);
+        const double th = (cb_t - I->tprev) / I->dt;
+        if (I->alg == OO_TSIT5) tsit5_interp(I, th, I->dt, I->ctmp);
+        else rodas_interp(I, th, I->ctmp);
+        memcpy(I->u, I->ctmp, sizeof(double) * I->n);
         I->t = cb_t;
         I->dt = I->t - I->tprev;
-        tsit5_addsteps(I);
+        if (I->alg == OO_TSIT5) tsit5_addsteps(I);
+        else rodas_addsteps(I);
         I->reeval_fsal = 1;
     }
     int savedexactly = savevalues(I, 0);

@@ -940,9 +940,10 @@ __device__ __forceinline__ bool tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Pa
 
 #ifdef ODEMI_NCCB
 // ---------------------------------------------------------------------------
-// ContinuousCallbacks on the Tsit5 lane (lib/DiffEqBase/src/callbacks.jl:
-// 157-640; OdemiCcb<K> holds the traced condition / affect! / affect_neg! and
-// the options as constants).
+// ContinuousCallbacks on the lanes (lib/DiffEqBase/src/callbacks.jl:157-640;
+// OdemiCcb<K> holds the traced condition / affect! / affect_neg! and the
+// options as constants).  Generic over the lane type through lane_interp and
+// lane_addsteps.
 // ---------------------------------------------------------------------------
 // Julia's sign(): -1, 0, 1, NaN for NaN
 __device__ __forceinline__ double jl_sign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : (x == 0 ? 0.0 : x)); }
@@ -1018,17 +1019,29 @@ __device__ void tsit5_addsteps(Tsit5Lane<S, K, N>& L) {
     odemi_f<K>(L.k7, tmp, L.p, (K)(t + dt));
 }
 
+// The method's dense output at theta of the current step and its event
+// re-step; the Rodas5P overloads follow its lane.
+template <typename S, typename K, int N>
+__device__ __forceinline__ void lane_interp(const Tsit5Lane<S, K, N>& L, double th, double* v) {
+    tsit5_interp(L, th, L.c.dt, v);
+}
+template <typename S, typename K, int N>
+__device__ __forceinline__ void lane_addsteps(Tsit5Lane<S, K, N>& L, const Params& P) {
+    (void)P;
+    tsit5_addsteps(L);
+}
+
 // get_condition (callbacks.jl:91-146): u at t, uprev at tprev, else the
 // step's dense output; every call counts in stats.ncondition.
-template <int CB, typename S, typename K, int N>
-__device__ __forceinline__ double ccb_condition(Tsit5Lane<S, K, N>& L, double abst) {
+template <int CB, int N, typename Lane>
+__device__ __forceinline__ double ccb_condition(Lane& L, double abst) {
     Ctl& c = L.c;
     c.ncond++;
-    if (abst == c.t) return (double)OdemiCcb<CB>::template cond<double, K>(L.u, L.p, abst);
-    if (abst == c.tprev) return (double)OdemiCcb<CB>::template cond<double, K>(L.uprev, L.p, abst);
+    if (abst == c.t) return (double)OdemiCcb<CB>::template cond<double>(L.u, L.p, abst);
+    if (abst == c.tprev) return (double)OdemiCcb<CB>::template cond<double>(L.uprev, L.p, abst);
     double v[N];
-    tsit5_interp(L, (abst - c.tprev) / c.dt, c.dt, v);
-    return (double)OdemiCcb<CB>::template cond<double, K>(v, L.p, abst);
+    lane_interp(L, (abst - c.tprev) / c.dt, v);
+    return (double)OdemiCcb<CB>::template cond<double>(v, L.p, abst);
 }
 
 // is_event_occurrence (callbacks.jl:525-530)
@@ -1046,25 +1059,25 @@ __device__ __forceinline__ bool ccb_is_event(double prev_sign, double next_sign)
 // float for LeftRootFind, the other for RightRootFind.  Bisection in FP64
 // halves the bracket each call, so the loop ends within ~2100 calls for any
 // finite bracket.
-template <int CB, typename S, typename K, int N>
-__device__ bool ccb_find_time(Tsit5Lane<S, K, N>& L, double& cb_t, double& bottom_sign, double& residual) {
+template <int CB, int N, typename Lane>
+__device__ bool ccb_find_time(Lane& L, double& cb_t, double& bottom_sign, double& residual) {
     typedef OdemiCcb<CB> C;
     Ctl& c = L.c;
     double bottom_t = c.tprev;
-    double bc = ccb_condition<CB>(L, bottom_t);
+    double bc = ccb_condition<CB, N>(L, bottom_t);
     if (c.event_last == CB + 1) {
         if (fabs(bc - c.last_event_error) <= C::abstol) bottom_t = c.tprev + c.dt * C::repeat_nudge;
-        bc = ccb_condition<CB>(L, bottom_t);
+        bc = ccb_condition<CB, N>(L, bottom_t);
     }
     const double bs = jl_sign(bc);
     double top_t = c.t;
-    double ts = jl_sign(ccb_condition<CB>(L, top_t));
+    double ts = jl_sign(ccb_condition<CB, N>(L, top_t));
     bool ev = ccb_is_event<CB>(bs, ts);
     if (C::interp_points != 0 && !ev) {
         const double h = (c.t - c.tprev) / (double)(C::interp_points - 1);
         for (int i = 2; i <= C::interp_points; i++) {
             top_t = (i == C::interp_points) ? c.t : fma((double)(i - 1), h, c.tprev);
-            ts = jl_sign(ccb_condition<CB>(L, top_t));
+            ts = jl_sign(ccb_condition<CB, N>(L, top_t));
             ev = ccb_is_event<CB>(bs, ts);
             if (ev) break;
         }
@@ -1076,13 +1089,13 @@ __device__ bool ccb_find_time(Tsit5Lane<S, K, N>& L, double& cb_t, double& botto
     for (int it = 0; it < 2200; it++) {
         const double m = l + 0.5 * (r - l);
         if (m == l || m == r) break;
-        const double sm = jl_sign(ccb_condition<CB>(L, m));
+        const double sm = jl_sign(ccb_condition<CB, N>(L, m));
         if (sm == 0.0) { l = m; r = m; break; }
         if (sm == bs) l = m;
         else r = m;
     }
     cb_t = (C::rootfind == 1) ? r : l;
-    residual = ccb_condition<CB>(L, cb_t);
+    residual = ccb_condition<CB, N>(L, cb_t);
     return true;
 }
 
@@ -1091,20 +1104,20 @@ __device__ bool ccb_find_time(Tsit5Lane<S, K, N>& L, double& cb_t, double& botto
 // output, t, dt = t - tprev, stages recomputed), saves, affect! on an
 // upcrossing / affect_neg! on a downcrossing (a missing one modifies nothing).
 // Returns terminate!.
-template <int CB, typename S, typename K, int N, typename SaveFn>
-__device__ bool ccb_apply(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, double cb_t, double bs,
-                          SaveFn& savevalues, bool& saved_in_cb) {
+template <int CB, int N, typename Lane, typename SaveFn>
+__device__ bool ccb_apply(Lane& L, const Params& P, i64 j, double cb_t, double bs, SaveFn& savevalues,
+                          bool& saved_in_cb) {
     typedef OdemiCcb<CB> C;
     Ctl& c = L.c;
     c.dtpropose = jl_max(nextfloat_pos(P.dtmin), c.dt);
     if (cb_t != c.t) {
         double v[N];
-        tsit5_interp(L, (cb_t - c.tprev) / c.dt, c.dt, v);
+        lane_interp(L, (cb_t - c.tprev) / c.dt, v);
 #pragma unroll
-        for (int i = 0; i < N; i++) L.u[i] = (S)v[i];
+        for (int i = 0; i < N; i++) L.u[i] = v[i];
         c.t = cb_t;
         c.dt = c.t - c.tprev;
-        tsit5_addsteps(L);
+        lane_addsteps(L, P);
         c.reeval_fsal = 1;
     }
     const bool savedexactly = savevalues();
@@ -1112,11 +1125,11 @@ __device__ bool ccb_apply(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, double 
     if (C::save0 && !savedexactly) save_point<N>(c, P, j, c.t, L.u);
     bool modified = true, term = false;
     if (bs < 0) {
-        if constexpr (C::has_affect != 0) { C::template affect<S, K>(L.u, L.p, (S)c.t); term = C::term != 0; }
+        if constexpr (C::has_affect != 0) { C::template affect<double>(L.u, L.p, c.t); term = C::term != 0; }
         else modified = false;
     } else if (bs > 0) {
         if constexpr (C::has_affect_neg != 0) {
-            C::template affect_neg<S, K>(L.u, L.p, (S)c.t);
+            C::template affect_neg<double>(L.u, L.p, c.t);
             term = C::term_neg != 0;
         } else {
             modified = false;
@@ -1129,39 +1142,37 @@ __device__ bool ccb_apply(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, double 
     return term;
 }
 
-template <int CB, typename S, typename K, int N>
-__device__ __forceinline__ void ccb_find_first(Tsit5Lane<S, K, N>& L, int& win, double& tmin, double& bs_win,
-                                               double& res_win) {
+template <int CB, int N, typename Lane>
+__device__ __forceinline__ void ccb_find_first(Lane& L, int& win, double& tmin, double& bs_win, double& res_win) {
     if constexpr (CB < ODEMI_NCCB) {
         double cbt, bs, rs;
-        if (ccb_find_time<CB>(L, cbt, bs, rs) && (win < 0 || cbt < tmin)) {
+        if (ccb_find_time<CB, N>(L, cbt, bs, rs) && (win < 0 || cbt < tmin)) {
             win = CB; tmin = cbt; bs_win = bs; res_win = rs;
         }
-        ccb_find_first<CB + 1>(L, win, tmin, bs_win, res_win);
+        ccb_find_first<CB + 1, N>(L, win, tmin, bs_win, res_win);
     }
 }
-template <int CB, typename S, typename K, int N, typename SaveFn>
-__device__ __forceinline__ bool ccb_apply_winner(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, int win, double cbt,
-                                                 double bs, SaveFn& savevalues, bool& saved_in_cb) {
+template <int CB, int N, typename Lane, typename SaveFn>
+__device__ __forceinline__ bool ccb_apply_winner(Lane& L, const Params& P, i64 j, int win, double cbt, double bs,
+                                                 SaveFn& savevalues, bool& saved_in_cb) {
     if constexpr (CB < ODEMI_NCCB) {
-        if (win == CB) return ccb_apply<CB>(L, P, j, cbt, bs, savevalues, saved_in_cb);
-        return ccb_apply_winner<CB + 1>(L, P, j, win, cbt, bs, savevalues, saved_in_cb);
+        if (win == CB) return ccb_apply<CB, N>(L, P, j, cbt, bs, savevalues, saved_in_cb);
+        return ccb_apply_winner<CB + 1, N>(L, P, j, win, cbt, bs, savevalues, saved_in_cb);
     } else {
         return false;
     }
 }
 // find_first_continuous_callback (callbacks.jl:157-237): the earliest event
 // wins, the first callback on ties; then apply_callback! for it.
-template <typename S, typename K, int N, typename SaveFn>
-__device__ bool tsit5_continuous_callbacks(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, SaveFn& savevalues,
-                                           bool& saved_in_cb) {
+template <int N, typename Lane, typename SaveFn>
+__device__ bool lane_continuous_callbacks(Lane& L, const Params& P, i64 j, SaveFn& savevalues, bool& saved_in_cb) {
     int win = -1;
     double tmin = 0.0, bs = 0.0, rs = 0.0;
-    ccb_find_first<0>(L, win, tmin, bs, rs);
+    ccb_find_first<0, N>(L, win, tmin, bs, rs);
     if (win < 0) { L.c.event_last = 0; return false; }
     L.c.last_event_error = rs;
     L.c.event_last = win + 1;
-    return ccb_apply_winner<0>(L, P, j, win, tmin, bs, savevalues, saved_in_cb);
+    return ccb_apply_winner<0, N>(L, P, j, win, tmin, bs, savevalues, saved_in_cb);
 }
 #endif
 
@@ -1222,7 +1233,7 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
         };
         const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, savefn, [&](bool& saved_in_cb) -> bool {
 #ifdef ODEMI_NCCB
-            return tsit5_continuous_callbacks(L, P, j, savefn, saved_in_cb);
+            return lane_continuous_callbacks<N>(L, P, j, savefn, saved_in_cb);
 #else
             (void)saved_in_cb;
             return false;
@@ -1650,7 +1661,12 @@ __device__ __forceinline__ void rodas_perform_step(RodasLane<F, N>& L, const Par
     }
     if constexpr (HW) c.EEst = residual_norm_nr<N>(du, L.uprev, L.u, P.abstol, P.reltol);
     else c.EEst = residual_norm<N>(du, L.uprev, L.u, P.abstol, P.reltol);
-    if (P.nsaveat > 0) {
+#ifdef ODEMI_NCCB
+    constexpr bool dense_always = true;   // event search interpolates every step
+#else
+    constexpr bool dense_always = false;
+#endif
+    if (dense_always || P.nsaveat > 0) {
 #pragma unroll
         for (int r = 0; r < 3; r++)
 #pragma unroll
@@ -1663,6 +1679,19 @@ __device__ __forceinline__ void rodas_perform_step(RodasLane<F, N>& L, const Par
     }
 }
 
+// rosenbrock_interpolants.jl:172-206 (interp_order 3)
+template <typename F, int N>
+__device__ __forceinline__ void rodas_interp(const RodasLane<F, N>& L, double th, double* v) {
+    const double th1 = 1.0 - th;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double in1 = fma(th, L.dense[2][i], L.dense[1][i]);
+        double in2 = fma(th, in1, L.dense[0][i]);
+        double in3 = fma(th1, in2, L.u[i]);
+        v[i] = fma(th, in3, th1 * L.uprev[i]);
+    }
+}
+
 template <typename F, int N>
 __device__ __forceinline__ bool rodas_savevalues(RodasLane<F, N>& L, const Params& P, i64 j) {
     Ctl& c = L.c;
@@ -1670,16 +1699,8 @@ __device__ __forceinline__ bool rodas_savevalues(RodasLane<F, N>& L, const Param
     while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
         double curt = P.saveat[c.spos++];
         if (curt != c.t) {
-            double th = (curt - c.tprev) / c.dt;
-            double th1 = 1.0 - th;
             double v[N];
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                double in1 = fma(th, L.dense[2][i], L.dense[1][i]);
-                double in2 = fma(th, in1, L.dense[0][i]);
-                double in3 = fma(th1, in2, L.u[i]);
-                v[i] = fma(th, in3, th1 * L.uprev[i]);
-            }
+            rodas_interp(L, (curt - c.tprev) / c.dt, v);
             save_point<N>(c, P, j, curt, v);
         } else {
             if (curt == c.tf && !P.save_end) continue;
@@ -1697,6 +1718,79 @@ __device__ __forceinline__ bool rodas_savevalues(RodasLane<F, N>& L, const Param
     }
     return exact;
 }
+
+#ifdef ODEMI_NCCB
+// _ode_addsteps! for RosenbrockCache (stiff_addsteps.jl:140-224) with
+// always_calc_begin, after change_t_via_interpolation!: the stages of the step
+// from uprev at tprev with the new dt, reusing the step's J and dT and its
+// fsalfirst f(uprev, tprev); W refactored; no @muladd in that function, so
+// only linsolve_tmp = fsalfirst + dtgamma dT is fused; dtC = C ./ dt; the
+// dense rows from the new stages.  No stats change.
+template <typename F, int N>
+__device__ void rodas_addsteps(RodasLane<F, N>& L, const Params& P) {
+    constexpr bool HW = sizeof(F) < sizeof(double);
+    const double t = L.c.tprev, dt = L.c.dt;
+    const double dtgamma = dt * RodasTab::gamma;
+    double fsal[N], du[N], lt[N], st[N];
+    odemi_f<double>(fsal, L.uprev, L.p, t);
+#pragma unroll
+    for (int i = 0; i < N; i++) lt[i] = fma(dtgamma, L.dT[i], fsal[i]);
+    const double invdtgamma = HW ? rcp_nr(dtgamma) : 1.0 / dtgamma;
+    double W[N][N];
+    F LU[N][N];
+    int piv[N];
+#pragma unroll
+    for (int a = 0; a < N; a++)
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            W[a][b] = (a == b) ? fma(-1.0, invdtgamma, L.J[a][b]) : L.J[a][b];
+            LU[a][b] = (F)W[a][b];
+        }
+    lu_factor_reg<F, N>(LU, piv);
+    rodas_wsolve<F, N>(W, LU, piv, lt, P.refine_sweeps);
+#pragma unroll
+    for (int i = 0; i < N; i++) L.ks[0][i] = -lt[i];
+#pragma unroll
+    for (int s = 1; s < 8; s++) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double acc = L.uprev[i];
+#pragma unroll
+            for (int q = 0; q < s; q++) acc = acc + kRA[s][q] * L.ks[q][i];
+            st[i] = acc;
+        }
+        odemi_f<double>(du, st, L.p, t + kRc[s] * dt);
+        const double dtds = dt * kRd[s];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double acc = du[i] + dtds * L.dT[i];
+#pragma unroll
+            for (int q = 0; q < s; q++) acc = acc + (kRC[s][q] / dt) * L.ks[q][i];
+            lt[i] = acc;
+        }
+        rodas_wsolve<F, N>(W, LU, piv, lt, P.refine_sweeps);
+#pragma unroll
+        for (int i = 0; i < N; i++) L.ks[s][i] = -lt[i];
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) acc = acc + kRH[r][q] * L.ks[q][i];
+            L.dense[r][i] = acc;
+        }
+}
+template <typename F, int N>
+__device__ __forceinline__ void lane_interp(const RodasLane<F, N>& L, double th, double* v) {
+    rodas_interp(L, th, v);
+}
+template <typename F, int N>
+__device__ __forceinline__ void lane_addsteps(RodasLane<F, N>& L, const Params& P) {
+    rodas_addsteps(L, P);
+}
+#endif
 
 template <typename F, int N>
 __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P, i64 j) {
@@ -1738,10 +1832,17 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
         if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tnext; }
         else c.t = ttmp;
         accept_controller<HW>(c, kc, q, P.dtmin);
-        // continuous callbacks are Tsit5-only (the runtime refuses Rodas5P modules with them)
-        const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, [&]() -> bool {
+        auto savefn = [&]() -> bool {
             return (P.nsaveat > 0 || P.save_everystep) ? rodas_savevalues(L, P, j) : false;
-        }, [](bool&) -> bool { return false; });
+        };
+        const bool term = handle_callbacks<N>(c, P, j, L.u, L.p, savefn, [&](bool& saved_in_cb) -> bool {
+#ifdef ODEMI_NCCB
+            return lane_continuous_callbacks<N>(L, P, j, savefn, saved_in_cb);
+#else
+            (void)saved_in_cb;
+            return false;
+#endif
+        });
         handle_tstop(c, P);
         if (term) return RC_TERMINATED;
     } else {

@@ -197,7 +197,7 @@ struct odemi_rhs {
     odemi_ctx* ctx = nullptr;
     hipModule_t mod = nullptr;
     int n = 0, np = 0, flags = 0;
-    int features = 0;   // 1: isoutofdomain, 2: discrete, 4: continuous callbacks (lane kernels; 4: Tsit5 only)
+    int features = 0;   // 1: isoutofdomain, 2: discrete, 4: continuous callbacks (lane kernels only)
     hipFunction_t tsit5_persist_f64 = nullptr, tsit5_persist_mixed = nullptr;
     hipFunction_t tsit5_persist_f64_sv = nullptr, tsit5_persist_mixed_sv = nullptr;
     hipFunction_t tsit5_plain_init = nullptr, tsit5_plain_step = nullptr;
@@ -626,7 +626,6 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         ctx->err = "isoutofdomain and callbacks run in the lane kernels only (compile without ODEMI_COMPILE_WG)";
         return -3;
     }
-    if (rodas && (rhs->features & 4)) { ctx->err = "continuous callbacks are supported with Tsit5 only"; return -3; }
 
     hipStream_t s = a->stream ? (hipStream_t)a->stream : ctx->stream;
     const bool dev = a->device_ptrs != 0;

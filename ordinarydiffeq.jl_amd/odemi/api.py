@@ -184,7 +184,7 @@ _SAME = object()
 class ContinuousCallback:
     """ContinuousCallback(condition, affect!, affect_neg! = affect!; rootfind,
     save_positions, interp_points, abstol, repeat_nudge) (SciMLBase; handled by
-    lib/DiffEqBase/src/callbacks.jl:239-640), Tsit5 only.  condition(u, t,
+    lib/DiffEqBase/src/callbacks.jl:239-640), lane kernels.  condition(u, t,
     integrator) is a traced value whose sign change within an accepted step is
     an event: the step is cut back to the root (found to adjacent floats; the
     float before the root for rootfind="left", after it for "right", the first
@@ -300,8 +300,6 @@ def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, i
     if wg and (isoutofdomain is not None or cbs):
         raise ValueError("isoutofdomain and callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
                          f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
-    if "rodas5p" in algs and any(isinstance(c, ContinuousCallback) for c in cbs):
-        raise ValueError("ContinuousCallback is supported with Tsit5 only")
     if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
         raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
     if ns is not None and "tsit5" in algs:

@@ -338,6 +338,13 @@ def _ball(du, u, p, t):
     du[1] = -9.81
 
 
+def _ball_jac(J, u, p, t):
+    J[0, 0] = 0.0
+    J[0, 1] = 1.0
+    J[1, 0] = 0.0
+    J[1, 1] = 0.0
+
+
 def _bounce(itg):
     itg.u[1] = -itg.p[0] * itg.u[1]
 
@@ -361,11 +368,12 @@ class _PyCCB:
         self.__dict__.update(cb.__dict__)
 
 
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()], ids=["tsit5", "rodas5p"])
 @pytest.mark.parametrize("engine", ["persistent", "plain"])
-def test_continuous_callbacks_bit_identical(engine):
-    """Two ContinuousCallbacks through the Tsit5 lane engines, bit-identical to
-    the CPU restatement (event search at tprev / t / 9 interpolated points,
-    root to adjacent floats, change_t_via_interpolation! with the stage
+def test_continuous_callbacks_bit_identical(alg, engine):
+    """Two ContinuousCallbacks through the lane engines, bit-identical to the
+    CPU restatement (event search at tprev / t / 9 interpolated points, root to
+    adjacent floats, change_t_via_interpolation! with the method's stage
     re-step, affect_neg! only, then terminate! at a per-trajectory time with
     RightRootFind): bouncing balls with restitution p[0], stopped at p[1] when
     that comes before tf = 10; save_everystep, saveat and final state only.
@@ -373,18 +381,19 @@ def test_continuous_callbacks_bit_identical(engine):
     N = 96
     u0, p = _ball_batch(N)
     ea = odemi.EnsembleMI355X(engine=engine)
+    fun = odemi.ODEFunction(_ball, jac=_ball_jac)
     modes = [dict(save_everystep=True, save_start=True, save_end=True),
              dict(saveat=list(np.linspace(0.5, 10.0, 20)), save_start=False, save_end=True),
              dict(save_start=False, save_end=True)]
     codes = set()
     ccbs = [_PyCCB(c) for c in _BALL_CBS.continuous_callbacks]
     for m in modes:
-        print(f"[case] continuous callbacks {engine} {sorted(m)}", flush=True)
-        out = odemi.solve_arrays(_ball, odemi.Tsit5(), u0, p, np.zeros(N), np.full(N, 10.0), ea, reltol=1e-6,
+        print(f"[case] continuous callbacks {alg.code} {engine} {sorted(m)}", flush=True)
+        out = odemi.solve_arrays(fun, alg, u0, p, np.zeros(N), np.full(N, 10.0), ea, reltol=1e-6,
                                  abstol=1e-6, callback=_BALL_CBS, max_saves=2048, **m)
         for j in range(0, N, 5):
-            r = O.solve(_ball, u0[:, j], p[:, j], (0.0, 10.0), alg=O.TSIT5, reltol=1e-6, abstol=1e-6,
-                        ccallbacks=ccbs, **m)
+            r = O.solve(_ball, u0[:, j], p[:, j], (0.0, 10.0), jac=_ball_jac, alg=alg.code, reltol=1e-6,
+                        abstol=1e-6, ccallbacks=ccbs, **m)
             ns = int(out["nsaved"][j])
             assert int(out["retcode"][j]) == r["retcode"] and ns == r["nsaved"], (j, m)
             assert np.array_equal(out["t_out"][:ns, j], r["t"]), (j, m)
@@ -398,26 +407,28 @@ def test_continuous_callbacks_bit_identical(engine):
     assert codes == {1, 2}
 
 
-def test_continuous_callbacks_mixed_and_surface():
-    """The same events on the mixed Tsit5 engine (FP32 stages): bounce times
-    within the integration error of the FP64 engine's, same retcodes; and
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()], ids=["tsit5", "rodas5p"])
+def test_continuous_callbacks_mixed_and_surface(alg):
+    """The same events on the mixed engines (FP32 stages / FP32 LU): bounce
+    times within the integration error of the FP64 engine's, same retcodes; and
     through solve(EnsembleProblem): every bounce saved twice with the
-    velocity's sign flipped; Rodas5P refuses continuous callbacks."""
+    velocity's sign flipped."""
     N = 256
     u0, p = _ball_batch(N, seed=22)
+    fun = odemi.ODEFunction(_ball, jac=_ball_jac)
     kw = dict(reltol=1e-6, abstol=1e-6, callback=_BALL_CBS, save_start=False, save_end=True, max_saves=256)
-    print("[case] continuous callbacks mixed", flush=True)
-    o64 = odemi.solve_arrays(_ball, odemi.Tsit5(), u0, p, np.zeros(N), np.full(N, 10.0), **kw)
-    om = odemi.solve_arrays(_ball, odemi.Tsit5(), u0, p, np.zeros(N), np.full(N, 10.0),
+    print(f"[case] continuous callbacks mixed {alg.code}", flush=True)
+    o64 = odemi.solve_arrays(fun, alg, u0, p, np.zeros(N), np.full(N, 10.0), **kw)
+    om = odemi.solve_arrays(fun, alg, u0, p, np.zeros(N), np.full(N, 10.0),
                             odemi.EnsembleMI355X(precision="mixed"), **kw)
     assert np.array_equal(om["retcode"], o64["retcode"]) and np.array_equal(om["nsaved"], o64["nsaved"])
     for j in range(N):
         ns = int(o64["nsaved"][j])
         assert np.allclose(om["t_out"][:ns, j], o64["t_out"][:ns, j], rtol=1e-5, atol=1e-5)
-    base = odemi.ODEProblem(_ball, list(u0[:, 0]), (0.0, 10.0), list(p[:, 0]))
+    base = odemi.ODEProblem(fun, list(u0[:, 0]), (0.0, 10.0), list(p[:, 0]))
     ens = odemi.EnsembleProblem(base, prob_func=lambda pr, ctx: odemi.remake(
         pr, u0=list(u0[:, ctx.sim_id - 1]), p=list(p[:, ctx.sim_id - 1])))
-    sim = odemi.solve(ens, odemi.Tsit5(), odemi.EnsembleMI355X(), trajectories=16, reltol=1e-6, abstol=1e-6,
+    sim = odemi.solve(ens, alg, odemi.EnsembleMI355X(), trajectories=16, reltol=1e-6, abstol=1e-6,
                       callback=_BALL_CBS, save_everystep=False)
     for j, sol in enumerate(sim.u):
         assert int(sol.retcode) == int(o64["retcode"][j])
@@ -426,8 +437,6 @@ def test_continuous_callbacks_mixed_and_surface():
         for t in bounces:
             i = ts.index(t)
             assert sol.u[i][1] < 0 < sol.u[i + 1][1] and abs(sol.u[i][0]) < 1e-9
-    with pytest.raises(ValueError, match="Tsit5 only"):
-        odemi.solve_arrays(_ball, odemi.Rodas5P(), u0, p, np.zeros(N), np.full(N, 10.0), **kw)
 
 
 def _expb(du, u, p, t):

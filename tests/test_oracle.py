@@ -494,6 +494,13 @@ def test_continuous_callback_reference_known_answers():
                                                                                    rootfind=rf)])
         assert r5["retcode"] == 2 and abs(r5["t"][-1] - t1) < 1e-13
         assert abs(r5["u"][-1][0]) < 3e-12 and (r5["u"][-1][0] > 0) == (rf == "left")
+    # Rodas5P: its dense output and event re-step (stiff_addsteps.jl:140-224) find the same bounces
+    jac = lambda J, u, p, t: J.__setitem__(slice(None), [[0.0, 1.0], [0.0, 0.0]])
+    r8 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), jac=jac, alg=O.RODAS5P, ccallbacks=[cb],
+                 save_everystep=False)
+    assert r8["retcode"] == 1 and len(r8["t"]) == 6
+    assert abs(r8["t"][1] - t1) < 1e-12 and abs(r8["t"][3] - 3 * t1) < 1e-12 and r8["t"][1] == r8["t"][2]
+    assert r8["u"][1][1] < 0 < r8["u"][2][1]
     # no event: a condition that never changes sign changes nothing but the condition count
     r6 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 2.0), save_everystep=True)
     r7 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 2.0), save_everystep=True,

@@ -114,7 +114,8 @@ def test_continuous_callbacks_emit_and_compile():
     rc = L.odemi_rtc_codeobject(tr.src.encode(), 2, 2, _lib.COMPILE_TSIT5, path, 512, ctypes.byref(secs), log,
                                 1 << 16)
     assert rc == 0, log.value.decode()[:2000]
-    with pytest.raises(ValueError, match="Tsit5 only"):
-        odemi.compile_problem(odemi.ODEFunction(fall), 2, 2, algs=("rodas5p",), callback=cbs)
+    rc = L.odemi_rtc_codeobject(tr.src.encode(), 2, 2, _lib.COMPILE_RODAS5P, path, 512, ctypes.byref(secs), log,
+                                1 << 16)
+    assert rc == 0, log.value.decode()[:2000]
     with pytest.raises(ValueError):
         odemi.ContinuousCallback(lambda u, t, i: u[0], bounce, interp_points=1)
