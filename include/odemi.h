@@ -134,7 +134,7 @@ int odemi_abi_version(void);
  *     template<typename T, typename Q> static __device__ void affect(T* u, const Q* p, T t);
  *     <- a CallbackSet of k DiscreteCallbacks (condition, affect! on u, save_positions,
  *        terminate! when term; lib/DiffEqBase/src/callbacks.jl:645-683)
- *   #define ODEMI_NCCB k with template<int K> struct OdemiCcb (Tsit5 only) holding
+ *   #define ODEMI_NCCB k with template<int K> struct OdemiCcb holding
  *     static constexpr int has_affect, has_affect_neg, save0, save1, rootfind (0 left,
  *       1 right, 2 none), interp_points, term, term_neg;
  *     static constexpr double abstol, repeat_nudge;
