@@ -283,6 +283,9 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     // queue slots a wave takes per atomic (>= 64)
     const char* qc = getenv("ODEMI_QUEUE_CHUNK");
     if (qc && *qc && atoi(qc) >= 64) s << "#define ODEMI_QUEUE_CHUNK " << atoi(qc) << "\n";
+    // idle lanes a wave gathers before refilling them (1..64)
+    const char* rm = getenv("ODEMI_REFILL_MIN");
+    if (rm && *rm && atoi(rm) >= 1 && atoi(rm) <= 64) s << "#define ODEMI_REFILL_MIN " << atoi(rm) << "\n";
     if (flags & ODEMI_COMPILE_WG) s << "#define ODEMI_WITH_WG 1\n";
     s << problem_src << "\n";
     s << kDeviceSource << "\n";
