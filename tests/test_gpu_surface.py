@@ -554,3 +554,58 @@ def test_mixed_precision_saveat_within_tolerance():
     ns = int(o64["nsaved"].min())
     err = np.abs(om["u_out"][:ns] - o64["u_out"][:ns]) / (1e-6 + 1e-6 * np.abs(o64["u_out"][:ns]))
     assert np.quantile(err, 0.99) < 20.0
+
+
+def _ring16(du, u, p, t):
+    for i in range(16):
+        du[i] = -(0.1 * (i + 1)) * u[i] + 0.3 * u[(i + 1) % 16] + p[0] * t
+
+
+def _decay_nop(du, u, p, t):
+    du[0] = -2.0 * u[0]
+    du[1] = u[0] - u[1]
+
+
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_edge_cases_match_oracle(engine):
+    """Edge cases of the ensemble surface against the CPU restatement, FP64
+    bit for bit: the largest lane state (Tsit5, n = 16, non-autonomous);
+    spans of zero length (tf == t0), negative and shifted t0 in one batch; a
+    trajectory whose u0 holds a NaN among finite ones (only it fails, with the
+    restatement's code); an empty batch; a problem without parameters."""
+    ea = odemi.EnsembleMI355X(engine=engine)
+    rng = np.random.default_rng(31)
+    N = 40
+    u0 = rng.uniform(-1.0, 1.0, (16, N))
+    p = rng.uniform(-0.5, 0.5, (1, N))
+    t0 = rng.choice([0.0, -1.5, 2.0], N)
+    tf = t0 + rng.choice([0.0, 0.7, 3.0], N)
+    u0[3, 7] = np.nan
+    print(f"[case] edge cases n=16 {engine}", flush=True)
+    out = odemi.solve_arrays(_ring16, odemi.Tsit5(), u0, p, t0, tf, ea, reltol=1e-7, abstol=1e-9,
+                             save_everystep=True, save_start=True, max_saves=4096)
+    seen = set()
+    for j in range(N):
+        r = O.solve(_ring16, u0[:, j], p[:, j], (t0[j], tf[j]), alg=O.TSIT5, reltol=1e-7, abstol=1e-9,
+                    save_everystep=True)
+        ns = int(out["nsaved"][j])
+        assert int(out["retcode"][j]) == r["retcode"] and ns == r["nsaved"], j
+        assert np.array_equal(out["t_out"][:ns, j], r["t"], equal_nan=True), j
+        assert np.array_equal(out["u_out"][:ns, :, j], r["u"], equal_nan=True), j
+        st = out["stats"][:, j]
+        assert (st[0], st[1], st[2]) == (r["nf"], r["naccept"], r["nreject"]), j
+        seen.add((tf[j] == t0[j], r["retcode"]))
+    assert (True, 1) in seen and (False, 1) in seen and int(out["retcode"][7]) != 1
+    assert all(int(out["retcode"][j]) == 1 for j in range(N) if j != 7)
+    # an empty batch
+    e = odemi.solve_arrays(_ring16, odemi.Tsit5(), np.zeros((16, 0)), np.zeros((1, 0)), np.zeros(0), np.zeros(0),
+                           ea)
+    assert e["retcode"].shape == (0,) and e["u_final"].shape == (16, 0)
+    # no parameters: ODEProblem without p
+    for alg in (odemi.Tsit5(), odemi.Rodas5P()):
+        sol = odemi.solve(odemi.ODEProblem(_decay_nop, [1.0, 0.0], (0.0, 2.0)), alg, ea, reltol=1e-6, abstol=1e-8)
+        r = O.solve(_decay_nop, [1.0, 0.0], [], (0.0, 2.0), alg=alg.code, reltol=1e-6, abstol=1e-8,
+                    save_everystep=True, jac=lambda J, u, pp, t: J.__setitem__(slice(None), [[-2.0, 0.0],
+                                                                                          [1.0, -1.0]]))
+        assert int(sol.retcode) == r["retcode"] == 1
+        assert np.array_equal(np.array(sol.t), r["t"]) and np.array_equal(np.array(sol.u), r["u"])
