@@ -365,6 +365,7 @@ typedef struct {
     int accept_step, next_step_tstop, isout;
     int reeval_fsal;      /* a callback modified u: the next accept re-evaluates the FSAL stage */
     int event_last;       /* integrator.event_last_time: 1 + the continuous callback of the last event, 0 none */
+    unsigned sim_mask;    /* components of that event when it was a vector callback's (prev_simultaneous_events) */
     double last_event_error;
     double* ctmp;         /* interpolated state for continuous-callback conditions */
     oo_result* res;
@@ -920,13 +921,123 @@ static int ccb_find_time(integ_t* I, int k, double* cb_t, double* bottom_sign, d
     return 1;
 }
 
+/* get_condition for a VectorContinuousCallback: all len values at abst. */
+static void vcb_condition(integ_t* I, int k, double abst, double* out) {
+    const double* v;
+    if (abst == I->t) {
+        v = I->u;
+    } else if (abst == I->tprev) {
+        v = I->uprev;
+    } else {
+        const double th = (abst - I->tprev) / I->dt;
+        if (I->alg == OO_TSIT5) tsit5_interp(I, th, I->dt, I->ctmp);
+        else rodas_interp(I, th, I->ctmp);
+        v = I->ctmp;
+    }
+    I->res->ncondition++;
+    I->o->vcb_cond(out, v, I->p, abst, k, I->o->ccb_user);
+}
+
+/* findall_events! (callbacks.jl:499-521) on the values at top_t: event i when
+ * the bottom sign is nonzero and bottom_sign * value <= 0.  Returns the mask. */
+static unsigned vcb_events(int len, const double* bsign, const double* val) {
+    unsigned m = 0;
+    for (int i = 0; i < len; i++)
+        if (bsign[i] != 0 && bsign[i] * val[i] <= 0) m |= 1u << i;
+    return m;
+}
+
+/* find_callback_time for a VectorContinuousCallback (callbacks.jl:239-362):
+ * after this callback's own last event the bottom signs of the components
+ * that fired are re-read at the nudged tprev (nudge_tprev on the value
+ * closest to zero among them); the interp_points checks run unless every
+ * component has an event at t; each event component is rooted to adjacent
+ * floats (from the nudged tprev when it fired last time), the earliest root
+ * wins, and components rooted at that same time are simultaneous, with
+ * direction -sign(bottom).  ev receives the directions. */
+static int vcb_find_time(integ_t* I, int k, double* cb_t, double* ev, double* residual) {
+    const int* fl = I->o->ccb_flags + OO_CCB_NFLAGS * k;
+    const double* tl = I->o->ccb_tol + 2 * k;
+    const int len = fl[OO_CCB_VLEN];
+    double bc[OO_VCB_MAXLEN], bs[OO_VCB_MAXLEN], top[OO_VCB_MAXLEN], ts[OO_VCB_MAXLEN];
+    const double bottom_t = I->tprev;
+    vcb_condition(I, k, bottom_t, bc);
+    for (int i = 0; i < len; i++) { bs[i] = jl_sign(bc[i]); ev[i] = 0.0; }
+    double nudged_t = bottom_t;
+    const unsigned prev = (I->event_last == k + 1) ? I->sim_mask : 0u;
+    if (I->event_last == k + 1) {
+        double minval = 0.0, minabs = INFINITY;
+        for (int i = 0; i < len; i++)
+            if ((prev >> i) & 1u) {
+                if (fabs(bc[i]) < minabs) { minabs = fabs(bc[i]); minval = bc[i]; }
+            }
+        if (fabs(minval - I->last_event_error) <= tl[0]) nudged_t = I->tprev + I->dt * tl[1];
+        double tmp[OO_VCB_MAXLEN];
+        vcb_condition(I, k, nudged_t, tmp);
+        for (int i = 0; i < len; i++)
+            if ((prev >> i) & 1u) bs[i] = jl_sign(tmp[i]);
+    }
+    double top_t = I->t;
+    vcb_condition(I, k, top_t, top);
+    unsigned mask = vcb_events(len, bs, top);
+    const int npts = fl[OO_CCB_INTERP_POINTS];
+    const unsigned all = (len >= 32) ? 0xffffffffu : ((1u << len) - 1u);
+    if (npts != 0 && mask != all) {
+        const double h = (I->t - I->tprev) / (double)(npts - 1);
+        for (int i = 2; i <= npts; i++) {
+            top_t = (i == npts) ? I->t : fma((double)(i - 1), h, I->tprev);
+            vcb_condition(I, k, top_t, top);
+            mask = vcb_events(len, bs, top);
+            if (mask) break;
+        }
+    }
+    for (int i = 0; i < len; i++) ts[i] = jl_sign(top[i]);
+    if (!mask) { *cb_t = I->t; *residual = 0.0; return 0; }
+    if (fl[OO_CCB_ROOTFIND] == 2) {
+        for (int i = 0; i < len; i++)
+            if ((mask >> i) & 1u) ev[i] = -bs[i];
+        *cb_t = top_t; *residual = 0.0;
+        return 1;
+    }
+    double cbt = nextafter(top_t, INFINITY), res = 0.0;
+    double tmp[OO_VCB_MAXLEN];
+    for (int idx = 0; idx < len; idx++) {
+        if (!((mask >> idx) & 1u)) continue;
+        double cbi;
+        if (ts[idx] == 0.0) {
+            cbi = top_t;
+        } else {
+            double l = ((prev >> idx) & 1u) ? nudged_t : bottom_t, r = top_t;
+            for (int it = 0; it < 2200; it++) {
+                const double m = l + 0.5 * (r - l);
+                if (m == l || m == r) break;
+                vcb_condition(I, k, m, tmp);
+                const double sm = jl_sign(tmp[idx]);
+                if (sm == 0.0) { l = m; r = m; break; }
+                if (sm == bs[idx]) l = m; else r = m;
+            }
+            cbi = fl[OO_CCB_ROOTFIND] == 1 ? r : l;
+        }
+        if (cbi < cbt)
+            for (int i = 0; i < len; i++) ev[i] = 0.0;
+        if (cbi <= cbt) {
+            cbt = cbi;
+            vcb_condition(I, k, cbi, tmp);
+            res = tmp[idx];
+            ev[idx] = -bs[idx];
+        }
+    }
+    *cb_t = cbt; *residual = res;
+    return 1;
+}
+
 /* apply_callback! for a ContinuousCallback (callbacks.jl:557-640):
  * set_proposed_dt! (dtrelax = 1), change_t_via_interpolation!
  * (integrator_interface.jl:5-40: u from the dense output, t, dt = t - tprev,
  * then reeval_internals_due_to_modification! recomputes the stages), saves,
  * affect! on an upcrossing / affect_neg! on a downcrossing.  Returns 1 for
  * terminate!. */
-static int ccb_apply(integ_t* I, int k, double cb_t, double bs, int* saved_in_cb) {
+static int ccb_apply(integ_t* I, int k, double cb_t, double bs, const double* ev, int* saved_in_cb) {
     const oo_opts* o = I->o;
     const int* fl = o->ccb_flags + OO_CCB_NFLAGS * k;
     if (o->adaptive) {
@@ -949,7 +1060,11 @@ static int ccb_apply(integ_t* I, int k, double cb_t, double bs, int* saved_in_cb
     *saved_in_cb = 1;
     if (fl[OO_CCB_SAVE0] && !savedexactly) savevalues(I, 1);
     int modified = 1, term = 0;
-    if (bs < 0) {
+    if (fl[OO_CCB_VLEN] > 0) {
+        /* VectorContinuousCallback: affect!(integrator, simultaneous_events) */
+        if (fl[OO_CCB_HAS_AFFECT]) term = o->vcb_affect(I->u, I->p, I->t, k, ev, o->ccb_user) != 0;
+        else modified = 0;
+    } else if (bs < 0) {
         if (fl[OO_CCB_HAS_AFFECT]) term = o->ccb_affect(I->u, I->p, I->t, k, 0, o->ccb_user) != 0;
         else modified = 0;
     } else if (bs > 0) {
@@ -977,16 +1092,24 @@ static int handle_callbacks(integ_t* I) {
          * event time wins, the first callback on ties */
         int win = -1;
         double tmin = 0.0, bs_win = 0.0, res_win = 0.0;
+        double ev_win[OO_VCB_MAXLEN] = {0}, ev[OO_VCB_MAXLEN];
         for (int k = 0; k < o->nccb; k++) {
-            double cbt, bs, rs;
-            if (ccb_find_time(I, k, &cbt, &bs, &rs) && (win < 0 || cbt < tmin)) {
+            double cbt, bs = 0.0, rs;
+            const int vlen = o->ccb_flags[OO_CCB_NFLAGS * k + OO_CCB_VLEN];
+            const int occ = vlen > 0 ? vcb_find_time(I, k, &cbt, ev, &rs) : ccb_find_time(I, k, &cbt, &bs, &rs);
+            if (occ && (win < 0 || cbt < tmin)) {
                 win = k; tmin = cbt; bs_win = bs; res_win = rs;
+                for (int i = 0; i < OO_VCB_MAXLEN; i++) ev_win[i] = i < vlen ? ev[i] : 0.0;
             }
         }
         if (win >= 0) {
             I->last_event_error = res_win;
             I->event_last = win + 1;
-            if (ccb_apply(I, win, tmin, bs_win, &saved_in_cb)) term = 1;
+            I->sim_mask = 0;
+            const int vlen = o->ccb_flags[OO_CCB_NFLAGS * win + OO_CCB_VLEN];
+            for (int i = 0; i < vlen; i++)
+                if (ev_win[i] != 0.0) I->sim_mask |= 1u << i;
+            if (ccb_apply(I, win, tmin, bs_win, ev_win, &saved_in_cb)) term = 1;
         } else {
             I->event_last = 0;
         }

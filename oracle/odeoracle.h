@@ -40,8 +40,15 @@ typedef int (*oo_dcb_affect_fn)(double* u, const double* p, double t, int k, voi
  * affect_neg! (neg = 1) rewrites u and returns nonzero for terminate!. */
 typedef double (*oo_ccb_cond_fn)(const double* u, const double* p, double t, int k, void* user);
 typedef int (*oo_ccb_affect_fn)(double* u, const double* p, double t, int k, int neg, void* user);
+/* VectorContinuousCallback k: the len condition values into out; affect!
+ * with the event directions ev[i] in {-1, 0, 1} (0: no event). */
+typedef void (*oo_vcb_cond_fn)(double* out, const double* u, const double* p, double t, int k, void* user);
+typedef int (*oo_vcb_affect_fn)(double* u, const double* p, double t, int k, const double* ev, void* user);
 enum { OO_CCB_HAS_AFFECT = 0, OO_CCB_HAS_AFFECT_NEG = 1, OO_CCB_SAVE0 = 2, OO_CCB_SAVE1 = 3,
-       OO_CCB_ROOTFIND = 4 /* 0 left, 1 right, 2 none */, OO_CCB_INTERP_POINTS = 5, OO_CCB_NFLAGS = 6 };
+       OO_CCB_ROOTFIND = 4 /* 0 left, 1 right, 2 none */, OO_CCB_INTERP_POINTS = 5,
+       OO_CCB_VLEN = 6 /* 0: ContinuousCallback, > 0: VectorContinuousCallback of that length */,
+       OO_CCB_NFLAGS = 7 };
+#define OO_VCB_MAXLEN 32
 
 typedef struct {
     int alg;              /* OO_TSIT5 / OO_RODAS5P */
@@ -73,6 +80,8 @@ typedef struct {
     const int* ccb_flags;        /* [OO_CCB_NFLAGS * k + OO_CCB_*] */
     const double* ccb_tol;       /* [2k]: abstol, [2k + 1]: repeat_nudge */
     void* ccb_user;
+    oo_vcb_cond_fn vcb_cond;     /* the vector callbacks' conditions and affects */
+    oo_vcb_affect_fn vcb_affect;
 } oo_opts;
 
 typedef struct {

@@ -21,6 +21,10 @@ CCB_COND_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
 CCB_AFFECT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                  ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+VCB_COND_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_double), ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
+VCB_AFFECT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                 ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 DCB_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                           ctypes.c_double, ctypes.c_int, ctypes.c_void_p)
 ISOUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -42,7 +46,7 @@ class Opts(ctypes.Structure):
                 ("dcb_save", ctypes.POINTER(ctypes.c_int)), ("dcb_user", ctypes.c_void_p),
                 ("nccb", ctypes.c_int), ("ccb_cond", ctypes.c_void_p), ("ccb_affect", ctypes.c_void_p),
                 ("ccb_flags", ctypes.POINTER(ctypes.c_int)), ("ccb_tol", ctypes.POINTER(ctypes.c_double)),
-                ("ccb_user", ctypes.c_void_p)]
+                ("ccb_user", ctypes.c_void_p), ("vcb_cond", ctypes.c_void_p), ("vcb_affect", ctypes.c_void_p)]
 
 
 class Result(ctypes.Structure):
@@ -112,8 +116,9 @@ class _CallbackIntegrator:
     def __init__(self, u, p, t):
         self.u, self.p, self.t, self._term = u, p, t, False
 
-    def terminate(self):
-        self._term = True
+    def terminate(self, when=None):
+        if when is None or bool(when):
+            self._term = True
 
 
 def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0, maxiters=1000000,
@@ -125,7 +130,8 @@ def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0
     copies of the state.  ccallbacks: objects with ContinuousCallback's fields
     condition (a value), affect / affect_neg (callable or None), rootfind
     ("left" / "right" / "none"), interp_points, save_positions, abstol,
-    repeat_nudge."""
+    repeat_nudge; a VectorContinuousCallback also has len, its condition
+    fills out[0:len] and its affect takes (integrator, events)."""
     o = Opts()
     lib().oo_default_opts(ctypes.byref(o), alg)
     o.reltol, o.abstol = reltol, abstol
@@ -195,17 +201,37 @@ def make_opts(alg=TSIT5, reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0, dt=0.0
             (ccl[k].affect_neg if neg else ccl[k].affect)(itg)
             ua[:n] = np.asarray(itg.u, dtype=np.float64).reshape(-1)
             return 1 if itg._term else 0
+
+        def _vcond(out, u, p, t, k, _user):
+            uu = np.array(np.ctypeslib.as_array(u, shape=(max(n, 1),))[:n])
+            pp = np.array(np.ctypeslib.as_array(p, shape=(max(np_, 1),))[:np_])
+            o_ = np.zeros(ccl[k].len)
+            ccl[k].condition(o_, uu, t, _CallbackIntegrator(uu.copy(), pp, t))
+            for i in range(ccl[k].len):
+                out[i] = float(o_[i])
+
+        def _vaffect(u, p, t, k, ev, _user):
+            ua = np.ctypeslib.as_array(u, shape=(max(n, 1),))
+            pp = np.array(np.ctypeslib.as_array(p, shape=(max(np_, 1),))[:np_])
+            evs = np.array(np.ctypeslib.as_array(ev, shape=(ccl[k].len,)))
+            itg = _CallbackIntegrator(np.array(ua[:n]), pp, t)
+            ccl[k].affect(itg, evs)
+            ua[:n] = np.asarray(itg.u, dtype=np.float64).reshape(-1)
+            return 1 if itg._term else 0
         rf = {"left": 0, "right": 1, "none": 2}
-        fl = np.ascontiguousarray([[int(c.affect is not None), int(c.affect_neg is not None),
+        fl = np.ascontiguousarray([[int(c.affect is not None), int(getattr(c, "affect_neg", None) is not None),
                                     int(bool(c.save_positions[0])), int(bool(c.save_positions[1])),
-                                    rf[c.rootfind], int(c.interp_points)] for c in ccl], dtype=np.int32)
+                                    rf[c.rootfind], int(c.interp_points), int(getattr(c, "len", 0))]
+                                   for c in ccl], dtype=np.int32)
         tl = np.ascontiguousarray([[float(c.abstol), float(c.repeat_nudge)] for c in ccl], dtype=np.float64)
         c1, c2 = CCB_COND_FN(_ccond), CCB_AFFECT_FN(_caffect)
+        c3, c4 = VCB_COND_FN(_vcond), VCB_AFFECT_FN(_vaffect)
         o.nccb = len(ccl)
         o.ccb_cond, o.ccb_affect = ctypes.cast(c1, ctypes.c_void_p), ctypes.cast(c2, ctypes.c_void_p)
+        o.vcb_cond, o.vcb_affect = ctypes.cast(c3, ctypes.c_void_p), ctypes.cast(c4, ctypes.c_void_p)
         o.ccb_flags = fl.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
         o.ccb_tol = tl.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
-        keep = (keep, c1, c2, fl, tl)
+        keep = (keep, c1, c2, c3, c4, fl, tl)
     return o, keep
 
 

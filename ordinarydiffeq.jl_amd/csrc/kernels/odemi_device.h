@@ -344,6 +344,7 @@ struct Ctl {
     // continuous callbacks: 1 + the callback of the last event (0: none) and its
     // condition value there (integrator.event_last_time, last_event_error)
     int event_last;
+    unsigned sim_mask;   // its components when a vector callback's (prev_simultaneous_events)
     double last_event_error;
     // mixed engines: fastpower(errold, beta2), kept from the accept that set
     // errold (NaN: not known yet), and log2(EEst) of the last controller call
@@ -637,7 +638,7 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     c.dtmax = (P.dtmax > 0.0) ? P.dtmax : (tf - t0);
     c.EEst = 1.0; c.q11 = 1.0; c.errold = 1e-4;
     c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0; c.isout = 0;
-    c.reeval_fsal = 0; c.ncond = 0; c.event_last = 0; c.last_event_error = 0.0;
+    c.reeval_fsal = 0; c.ncond = 0; c.event_last = 0; c.sim_mask = 0u; c.last_event_error = 0.0;
     c.erroldp = __builtin_nanf(""); c.lgEEst = 0.0f;
     c.dt = 0.0; c.dtpropose = 0.0;
     c.nf = 0; c.naccept = 0; c.nreject = 0; c.njacs = 0; c.nw = 0; c.nsolve = 0;
@@ -1122,8 +1123,8 @@ __device__ bool ccb_find_time(Lane& L, double& cb_t, double& bottom_sign, double
 // upcrossing / affect_neg! on a downcrossing (a missing one modifies nothing).
 // Returns terminate!.
 template <int CB, int N, typename Lane, typename SaveFn>
-__device__ bool ccb_apply(Lane& L, const Params& P, i64 j, double cb_t, double bs, SaveFn& savevalues,
-                          bool& saved_in_cb) {
+__device__ bool ccb_apply(Lane& L, const Params& P, i64 j, double cb_t, double bs, const double* ev,
+                          SaveFn& savevalues, bool& saved_in_cb) {
     typedef OdemiCcb<CB> C;
     Ctl& c = L.c;
     c.dtpropose = jl_max(nextfloat_pos(P.dtmin), c.dt);
@@ -1141,7 +1142,11 @@ __device__ bool ccb_apply(Lane& L, const Params& P, i64 j, double cb_t, double b
     saved_in_cb = true;
     if (C::save0 && !savedexactly) save_point<N>(c, P, j, c.t, L.u);
     bool modified = true, term = false;
-    if (bs < 0) {
+    if constexpr (C::vlen > 0) {
+        // VectorContinuousCallback: affect!(integrator, simultaneous_events)
+        if constexpr (C::has_affect != 0) term = C::template vaffect<double>(L.u, L.p, c.t, ev);
+        else modified = false;
+    } else if (bs < 0) {
         if constexpr (C::has_affect != 0) { C::template affect<double>(L.u, L.p, c.t); term = C::term != 0; }
         else modified = false;
     } else if (bs > 0) {
@@ -1159,22 +1164,147 @@ __device__ bool ccb_apply(Lane& L, const Params& P, i64 j, double cb_t, double b
     return term;
 }
 
+#ifndef ODEMI_VCB_MAXLEN
+#define ODEMI_VCB_MAXLEN 1
+#endif
+// get_condition for a VectorContinuousCallback: all vlen values at abst
 template <int CB, int N, typename Lane>
-__device__ __forceinline__ void ccb_find_first(Lane& L, int& win, double& tmin, double& bs_win, double& res_win) {
-    if constexpr (CB < ODEMI_NCCB) {
-        double cbt, bs, rs;
-        if (ccb_find_time<CB, N>(L, cbt, bs, rs) && (win < 0 || cbt < tmin)) {
-            win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+__device__ __forceinline__ void vcb_condition(Lane& L, double abst, double* out) {
+    Ctl& c = L.c;
+    c.ncond++;
+    if (abst == c.t) { OdemiCcb<CB>::template vcond<double>(out, L.u, L.p, abst); return; }
+    if (abst == c.tprev) { OdemiCcb<CB>::template vcond<double>(out, L.uprev, L.p, abst); return; }
+    double v[N];
+    lane_interp(L, (abst - c.tprev) / c.dt, v);
+    OdemiCcb<CB>::template vcond<double>(out, v, L.p, abst);
+}
+// findall_events! (callbacks.jl:499-521): event i when the bottom sign is
+// nonzero and bottom sign * value <= 0
+template <int V>
+__device__ __forceinline__ unsigned vcb_events(const double* bs, const double* val) {
+    unsigned m = 0u;
+#pragma unroll
+    for (int i = 0; i < V; i++)
+        if (bs[i] != 0 && bs[i] * val[i] <= 0) m |= 1u << i;
+    return m;
+}
+// find_callback_time for a VectorContinuousCallback (callbacks.jl:239-362):
+// after this callback's own last event the bottom signs of the components
+// that fired are re-read at the nudged tprev (nudge_tprev on the value
+// closest to zero among them); the interp_points checks run unless every
+// component has an event at t; each event component is rooted to adjacent
+// floats (from the nudged tprev when it fired last time), the earliest root
+// wins, and components rooted at that same time are simultaneous, with
+// direction -sign(bottom), written to ev.
+template <int CB, int N, typename Lane>
+__device__ bool vcb_find_time(Lane& L, double& cb_t, double* ev, double& residual) {
+    typedef OdemiCcb<CB> C;
+    constexpr int V = C::vlen;
+    Ctl& c = L.c;
+    double bc[V], bs[V], top[V], tmp[V];
+    const double bottom_t = c.tprev;
+    vcb_condition<CB, N>(L, bottom_t, bc);
+#pragma unroll
+    for (int i = 0; i < V; i++) { bs[i] = jl_sign(bc[i]); ev[i] = 0.0; }
+    double nudged_t = bottom_t;
+    const unsigned prev = (c.event_last == CB + 1) ? c.sim_mask : 0u;
+    if (c.event_last == CB + 1) {
+        double minval = 0.0, minabs = __builtin_inf();
+#pragma unroll
+        for (int i = 0; i < V; i++)
+            if (((prev >> i) & 1u) && fabs(bc[i]) < minabs) { minabs = fabs(bc[i]); minval = bc[i]; }
+        if (fabs(minval - c.last_event_error) <= C::abstol) nudged_t = c.tprev + c.dt * C::repeat_nudge;
+        vcb_condition<CB, N>(L, nudged_t, tmp);
+#pragma unroll
+        for (int i = 0; i < V; i++)
+            if ((prev >> i) & 1u) bs[i] = jl_sign(tmp[i]);
+    }
+    double top_t = c.t;
+    vcb_condition<CB, N>(L, top_t, top);
+    unsigned mask = vcb_events<V>(bs, top);
+    constexpr unsigned all = (V >= 32) ? 0xffffffffu : ((1u << V) - 1u);
+    if (C::interp_points != 0 && mask != all) {
+        const double h = (c.t - c.tprev) / (double)(C::interp_points - 1);
+        for (int i = 2; i <= C::interp_points; i++) {
+            top_t = (i == C::interp_points) ? c.t : fma((double)(i - 1), h, c.tprev);
+            vcb_condition<CB, N>(L, top_t, top);
+            mask = vcb_events<V>(bs, top);
+            if (mask) break;
         }
-        ccb_find_first<CB + 1, N>(L, win, tmin, bs_win, res_win);
+    }
+    if (!mask) { cb_t = c.t; residual = 0.0; return false; }
+    if (C::rootfind == 2) {
+#pragma unroll
+        for (int i = 0; i < V; i++)
+            if ((mask >> i) & 1u) ev[i] = -bs[i];
+        cb_t = top_t; residual = 0.0;
+        return true;
+    }
+    double cbt = __longlong_as_double(__double_as_longlong(top_t) + (top_t >= 0.0 ? 1 : -1)), res = 0.0;
+    if (top_t == 0.0) cbt = __longlong_as_double(1);   // nextfloat(0.0)
+#pragma unroll
+    for (int idx = 0; idx < V; idx++) {
+        if (!((mask >> idx) & 1u)) continue;
+        double cbi;
+        if (jl_sign(top[idx]) == 0.0) {
+            cbi = top_t;
+        } else {
+            double l = ((prev >> idx) & 1u) ? nudged_t : bottom_t, r = top_t;
+            for (int it = 0; it < 2200; it++) {
+                const double m = l + 0.5 * (r - l);
+                if (m == l || m == r) break;
+                vcb_condition<CB, N>(L, m, tmp);
+                const double sm = jl_sign(tmp[idx]);
+                if (sm == 0.0) { l = m; r = m; break; }
+                if (sm == bs[idx]) l = m;
+                else r = m;
+            }
+            cbi = (C::rootfind == 1) ? r : l;
+        }
+        if (cbi < cbt) {
+#pragma unroll
+            for (int i = 0; i < V; i++) ev[i] = 0.0;
+        }
+        if (cbi <= cbt) {
+            cbt = cbi;
+            vcb_condition<CB, N>(L, cbi, tmp);
+            res = tmp[idx];
+            ev[idx] = -bs[idx];
+        }
+    }
+    cb_t = cbt; residual = res;
+    return true;
+}
+
+template <int CB, int N, typename Lane>
+__device__ __forceinline__ void ccb_find_first(Lane& L, int& win, double& tmin, double& bs_win, double& res_win,
+                                               double* ev_win) {
+    if constexpr (CB < ODEMI_NCCB) {
+        typedef OdemiCcb<CB> C;
+        double cbt, bs = 0.0, rs;
+        if constexpr (C::vlen > 0) {
+            double ev[C::vlen];
+            if (vcb_find_time<CB, N>(L, cbt, ev, rs) && (win < 0 || cbt < tmin)) {
+                win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+#pragma unroll
+                for (int i = 0; i < C::vlen; i++) ev_win[i] = ev[i];
+            }
+        } else {
+            if (ccb_find_time<CB, N>(L, cbt, bs, rs) && (win < 0 || cbt < tmin)) {
+                win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+#pragma unroll
+                for (int i = 0; i < ODEMI_VCB_MAXLEN; i++) ev_win[i] = 0.0;
+            }
+        }
+        ccb_find_first<CB + 1, N>(L, win, tmin, bs_win, res_win, ev_win);
     }
 }
 template <int CB, int N, typename Lane, typename SaveFn>
 __device__ __forceinline__ bool ccb_apply_winner(Lane& L, const Params& P, i64 j, int win, double cbt, double bs,
-                                                 SaveFn& savevalues, bool& saved_in_cb) {
+                                                 const double* ev, SaveFn& savevalues, bool& saved_in_cb) {
     if constexpr (CB < ODEMI_NCCB) {
-        if (win == CB) return ccb_apply<CB, N>(L, P, j, cbt, bs, savevalues, saved_in_cb);
-        return ccb_apply_winner<CB + 1, N>(L, P, j, win, cbt, bs, savevalues, saved_in_cb);
+        if (win == CB) return ccb_apply<CB, N>(L, P, j, cbt, bs, ev, savevalues, saved_in_cb);
+        return ccb_apply_winner<CB + 1, N>(L, P, j, win, cbt, bs, ev, savevalues, saved_in_cb);
     } else {
         return false;
     }
@@ -1185,11 +1315,19 @@ template <int N, typename Lane, typename SaveFn>
 __device__ bool lane_continuous_callbacks(Lane& L, const Params& P, i64 j, SaveFn& savevalues, bool& saved_in_cb) {
     int win = -1;
     double tmin = 0.0, bs = 0.0, rs = 0.0;
-    ccb_find_first<0, N>(L, win, tmin, bs, rs);
+    double ev[ODEMI_VCB_MAXLEN];
+#pragma unroll
+    for (int i = 0; i < ODEMI_VCB_MAXLEN; i++) ev[i] = 0.0;
+    ccb_find_first<0, N>(L, win, tmin, bs, rs, ev);
     if (win < 0) { L.c.event_last = 0; return false; }
     L.c.last_event_error = rs;
     L.c.event_last = win + 1;
-    return ccb_apply_winner<0, N>(L, P, j, win, tmin, bs, savevalues, saved_in_cb);
+    unsigned m = 0u;
+#pragma unroll
+    for (int i = 0; i < ODEMI_VCB_MAXLEN; i++)
+        if (ev[i] != 0.0) m |= 1u << i;
+    L.c.sim_mask = m;
+    return ccb_apply_winner<0, N>(L, P, j, win, tmin, bs, ev, savevalues, saved_in_cb);
 }
 #endif
 
@@ -1349,7 +1487,7 @@ __device__ void tsit5_persistent(const Params& P) {
 // st_scalar rows: t dt dtpropose tprev last_event_error tf dtmax EEst q11 errold
 // st_int rows: iter success_iter accept_step next_step_tstop nf naccept nreject done
 //               saveat cursor, njacs nw nsolve (Rodas5P), tstops cursor,
-//               isout | reeval_fsal << 1, ncondition, event_last
+//               isout | reeval_fsal << 1, ncondition, event_last | sim_mask << 8
 // st_vec rows: u uprev k1 k7 (Tsit5)
 template <int N>
 __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
@@ -1378,7 +1516,8 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
     const int fl = q[13 * M + j];
     c.isout = fl & 1; c.reeval_fsal = (fl >> 1) & 1;
     c.ncond = q[14 * M + j];
-    c.event_last = q[15 * M + j];
+    c.event_last = q[15 * M + j] & 0xff;
+    c.sim_mask = (unsigned)q[15 * M + j] >> 8;
     c.njacs = 0; c.nw = 0; c.nsolve = 0;
     c.erroldp = __builtin_nanf(""); c.lgEEst = 0.0f;
 }
@@ -1396,7 +1535,7 @@ __device__ __forceinline__ void plain_store_ctl(const Ctl& c, const Params& P, i
     q[6 * M + j] = c.nreject; q[7 * M + j] = done; q[8 * M + j] = c.spos; q[12 * M + j] = c.tsp;
     q[13 * M + j] = c.isout | (c.reeval_fsal << 1);
     q[14 * M + j] = c.ncond;
-    q[15 * M + j] = c.event_last;
+    q[15 * M + j] = (int)((unsigned)c.event_last | (c.sim_mask << 8));
     if (P.nsaved) P.nsaved[j] = c.nsaved;
 }
 

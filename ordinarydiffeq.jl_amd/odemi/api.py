@@ -12,6 +12,7 @@ run in libodemi.so on the GPU; there is no CPU fallback.
 """
 from __future__ import annotations
 
+import builtins
 import enum
 import math
 import time
@@ -179,6 +180,7 @@ class DiscreteCallback:
 
 
 _SAME = object()
+builtins_len = builtins.len
 
 
 class ContinuousCallback:
@@ -208,20 +210,49 @@ class ContinuousCallback:
         self.abstol, self.repeat_nudge = float(abstol), float(repeat_nudge)
 
 
+class VectorContinuousCallback:
+    """VectorContinuousCallback(condition, affect!, len; rootfind,
+    save_positions, interp_points, abstol, repeat_nudge) (SciMLBase; handled by
+    lib/DiffEqBase/src/callbacks.jl:239-362,499-640), lane kernels.
+    condition(out, u, t, integrator) fills out[0:len] with traced values; a
+    sign change of any of them within an accepted step is an event: the step
+    is cut back to the earliest root and affect(integrator, events) runs once,
+    events[i] the direction of component i (-1 down, +1 up, 0 none; several
+    when simultaneous).  The affect is traced like the others, choosing by
+    event with odemi.sym_ifelse, and may call odemi.terminate(integrator,
+    when=predicate).  len <= 16."""
+
+    def __init__(self, condition, affect, len, *, rootfind="left", save_positions=(True, True),
+                 interp_points=10, abstol=10 * 2.220446049250313e-16, repeat_nudge=1 / 100):
+        if not 1 <= int(len) <= 16:
+            raise ValueError("len is 1..16")
+        if rootfind not in ("left", "right", "none"):
+            raise ValueError('rootfind is "left", "right" or "none"')
+        if int(interp_points) == 1 or int(interp_points) < 0:
+            raise ValueError("interp_points is 0 or at least 2")
+        if builtins_len(save_positions) != 2:
+            raise ValueError("save_positions is a pair of Bools")
+        self.condition, self.affect, self.affect_neg, self.len = condition, affect, None, int(len)
+        self.rootfind, self.interp_points = rootfind, int(interp_points)
+        self.save_positions = (bool(save_positions[0]), bool(save_positions[1]))
+        self.abstol, self.repeat_nudge = float(abstol), float(repeat_nudge)
+
+
 class CallbackSet:
     """CallbackSet(cb...): continuous callbacks are checked first (the earliest
-    event of the step wins), then the discrete ones in order."""
+    event of the step wins, in the order given on ties), then the discrete ones
+    in order."""
 
     def __init__(self, *callbacks):
         cbs = _callback_list(list(callbacks))
-        self.continuous_callbacks = [c for c in cbs if isinstance(c, ContinuousCallback)]
+        self.continuous_callbacks = [c for c in cbs if isinstance(c, (ContinuousCallback, VectorContinuousCallback))]
         self.discrete_callbacks = [c for c in cbs if isinstance(c, DiscreteCallback)]
 
 
 def _callback_list(cb):
     if cb is None:
         return []
-    if isinstance(cb, (DiscreteCallback, ContinuousCallback)):
+    if isinstance(cb, (DiscreteCallback, ContinuousCallback, VectorContinuousCallback)):
         return [cb]
     if isinstance(cb, CallbackSet):
         return list(cb.continuous_callbacks) + list(cb.discrete_callbacks)
@@ -234,10 +265,12 @@ def _callback_list(cb):
                     "CallbackSet)")
 
 
-def terminate(integrator):
+def terminate(integrator, when=None):
     """terminate!(integrator) inside a traced affect: the trajectory stops after
-    this step with retcode Terminated (integrator_interface.jl:443-446)."""
-    integrator.terminate()
+    this step with retcode Terminated (integrator_interface.jl:443-446).
+    `when`, a traced predicate, makes it conditional (VectorContinuousCallback
+    affects)."""
+    integrator.terminate(when)
 
 
 class ODESolution:
@@ -283,6 +316,9 @@ template <typename T> __device__ __forceinline__ void odemi_tgrad(T* dT, const T
 def _callback_key(c):
     if isinstance(c, DiscreteCallback):
         return ("d", id(c.condition), id(c.affect), c.save_positions)
+    if isinstance(c, VectorContinuousCallback):
+        return ("v", id(c.condition), id(c.affect), c.len, c.rootfind, c.interp_points, c.save_positions,
+                c.abstol, c.repeat_nudge)
     return ("c", id(c.condition), id(c.affect), id(c.affect_neg), c.rootfind, c.interp_points,
             c.save_positions, c.abstol, c.repeat_nudge)
 

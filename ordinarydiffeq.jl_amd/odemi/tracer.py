@@ -199,6 +199,14 @@ def sym_any(xs):
     return r
 
 
+def sym_ifelse(c, a, b):
+    """Traceable `c ? a : b` (c a predicate): the way a traced affect chooses by
+    an event direction or a state value."""
+    if not isinstance(c, Sym):
+        return a if c else b
+    return _g().make("select", (c, _lift(a), _lift(b)))
+
+
 def sym_in(x, values):
     """Traceable `x in values` (Julia's `t in tstops`): the `or` of x == v."""
     return sym_any([x == v for v in values])
@@ -389,6 +397,9 @@ def _emit_body(outputs, out_name, indent="    ", pname="p[{}]"):
         if op == "t":
             names[n.id] = "t"
             continue
+        if op == "ev":
+            names[n.id] = f"ev[{n.val}]"
+            continue
         a = [names[x.id] for x in n.args]
         if op == "add": e = f"{a[0]} + {a[1]}"
         elif op == "sub": e = f"{a[0]} - {a[1]}"
@@ -401,6 +412,7 @@ def _emit_body(outputs, out_name, indent="    ", pname="p[{}]"):
         elif op == "or": e = f"T((({a[0]} != T(0)) || ({a[1]} != T(0))) ? 1 : 0)"
         elif op == "and": e = f"T((({a[0]} != T(0)) && ({a[1]} != T(0))) ? 1 : 0)"
         elif op == "not": e = f"T(({a[0]} == T(0)) ? 1 : 0)"
+        elif op == "select": e = f"({a[0]} != T(0)) ? {a[1]} : {a[2]}"
         else: e = f"{_FN[op]}({a[0]})"
         nm = f"v{n.id}"
         lines.append(f"{indent}const T {nm} = {e};")
@@ -495,8 +507,14 @@ class _TracedIntegrator:
     def t(self):
         return self._t
 
-    def terminate(self):
-        self._term = True
+    def terminate(self, when=None):
+        """terminate!; `when` (a traced predicate) makes it conditional."""
+        if when is None:
+            self._term = True
+        elif isinstance(when, Sym):
+            self._term = when if self._term is False else (_lift(1.0 if self._term is True else self._term) | when)
+        else:
+            self._term = self._term or bool(when)
 
 
 def _emit_callbacks(callbacks, n, u, p, t):
@@ -512,6 +530,8 @@ def _emit_callbacks(callbacks, n, u, p, t):
         outs = [(i, v) for i, v in enumerate(itg.u.vals) if v is not u[i]]
         s0, s1 = (int(bool(x)) for x in cb.save_positions)
         src.append(f"template <> struct OdemiDcb<{k}> {{")
+        if isinstance(itg._term, Sym):
+            raise TypeError("a conditional terminate is supported in VectorContinuousCallback affects only")
         src.append(f"    static constexpr int save0 = {s0}, save1 = {s1}, term = {int(itg._term)};")
         src.append("    template <typename T, typename Q>")
         src.append("    static __device__ __forceinline__ bool cond(const T* __restrict__ u, "
@@ -555,12 +575,68 @@ def _emit_affect(affect, n, u, p, t, indent, name):
     return src, itg._term
 
 
+def _emit_vector_callback(k, cb, n, u, p, t):
+    """A VectorContinuousCallback as OdemiCcb<k>: condition(out, u, t,
+    integrator) traced into vcond, affect(integrator, events) into vaffect with
+    the event directions as inputs ev[i] (-1, 0, 1), returning terminate!."""
+    out = _Recorder((cb.len,))
+    cb.condition(out, list(u), t, _TracedIntegrator(u, p, t))
+    couts = [(i, out.vals.get(i, _lift(0.0))) for i in range(cb.len)]
+    s0, s1 = (int(bool(x)) for x in cb.save_positions)
+    rf = {"left": 0, "right": 1, "none": 2}[cb.rootfind]
+    body = [f"template <> struct OdemiCcb<{k}> {{",
+            f"    static constexpr int vlen = {cb.len}, has_affect = {int(cb.affect is not None)}, "
+            f"has_affect_neg = 0, save0 = {s0}, save1 = {s1};",
+            f"    static constexpr int rootfind = {rf}, interp_points = {int(cb.interp_points)}, term = 0, "
+            "term_neg = 0;",
+            f"    static constexpr double abstol = {float(cb.abstol).hex()}, "
+            f"repeat_nudge = {float(cb.repeat_nudge).hex()};",
+            "    template <typename T, typename Q>",
+            "    static __device__ __forceinline__ void vcond(T* __restrict__ out, const T* __restrict__ u, "
+            "const Q* __restrict__ p, T t) {",
+            "        (void)u; (void)p; (void)t;",
+            _emit_body(couts, "out", indent="        ", pname="T(p[{}])"),
+            "    }",
+            "    template <typename T, typename Q>",
+            "    static __device__ __forceinline__ bool vaffect(T* __restrict__ u, const Q* __restrict__ p, T t, "
+            "const double* __restrict__ ev) {",
+            "        (void)u; (void)p; (void)t; (void)ev;"]
+    if cb.affect is not None:
+        g = _g()
+        evs = [g.make("ev", (), i) for i in range(cb.len)]
+        itg = _TracedIntegrator(u, p, t)
+        cb.affect(itg, evs)
+        outs = [(i, v) for i, v in enumerate(itg.u.vals) if v is not u[i]]
+        term = itg._term
+        tsym = term if isinstance(term, Sym) else None
+        allouts = outs + ([(n, tsym)] if tsym is not None else [])
+        if allouts:
+            body.append(f"        T nu_[{n + 1}];")
+            body.append(_emit_body(allouts, "nu_", indent="        ", pname="T(p[{}])"))
+            for i, _ in outs:
+                body.append(f"        u[{i}] = nu_[{i}];")
+        if tsym is not None:
+            body.append(f"        return nu_[{n}] != T(0);")
+        else:
+            body.append(f"        return {'true' if term else 'false'};")
+    else:
+        body.append("        return false;")
+    body += ["    }", "};"]
+    return body
+
+
 def _emit_continuous_callbacks(callbacks, n, u, p, t):
     """ContinuousCallbacks as OdemiCcb<K>: the condition's value traced on
     (u, t, integrator), affect! / affect_neg! (None: that crossing is no event)
-    and the options as constants."""
-    src = [f"#define ODEMI_NCCB {len(callbacks)}", "template <int K> struct OdemiCcb;"]
+    and the options as constants; VectorContinuousCallbacks (with `len`) take
+    the vector form."""
+    vmax = max([getattr(c, "len", 0) for c in callbacks] + [1])
+    src = [f"#define ODEMI_NCCB {len(callbacks)}", f"#define ODEMI_VCB_MAXLEN {vmax}",
+           "template <int K> struct OdemiCcb;"]
     for k, cb in enumerate(callbacks):
+        if getattr(cb, "len", 0):
+            src += _emit_vector_callback(k, cb, n, u, p, t)
+            continue
         c = _lift(cb.condition(list(u), t, _TracedIntegrator(u, p, t)))
         s0, s1 = (int(bool(x)) for x in cb.save_positions)
         rf = {"left": 0, "right": 1, "none": 2}[cb.rootfind]
@@ -573,7 +649,9 @@ def _emit_continuous_callbacks(callbacks, n, u, p, t):
             neg_src, term_neg = _emit_affect(cb.affect_neg, n, u, p, t, "    ", "affect_neg")
         body.append(f"    static constexpr int has_affect = {int(cb.affect is not None)}, "
                     f"has_affect_neg = {int(cb.affect_neg is not None)}, save0 = {s0}, save1 = {s1};")
-        body.append(f"    static constexpr int rootfind = {rf}, interp_points = {int(cb.interp_points)}, "
+        if isinstance(term, Sym) or isinstance(term_neg, Sym):
+            raise TypeError("a conditional terminate is supported in VectorContinuousCallback affects only")
+        body.append(f"    static constexpr int vlen = 0, rootfind = {rf}, interp_points = {int(cb.interp_points)}, "
                     f"term = {int(term)}, term_neg = {int(term_neg)};")
         body.append(f"    static constexpr double abstol = {float(cb.abstol).hex()}, "
                     f"repeat_nudge = {float(cb.repeat_nudge).hex()};")

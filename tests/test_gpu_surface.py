@@ -609,3 +609,66 @@ def test_edge_cases_match_oracle(engine):
                                                                                           [1.0, -1.0]]))
         assert int(sol.retcode) == r["retcode"] == 1
         assert np.array_equal(np.array(sol.t), r["t"]) and np.array_equal(np.array(sol.u), r["u"])
+
+
+def _box(du, u, p, t):
+    du[0] = u[1]
+    du[1] = -9.81
+    du[2] = u[3]
+    du[3] = 0.0
+
+
+def _box_jac(J, u, p, t):
+    for i in range(4):
+        for k in range(4):
+            J[i, k] = 1.0 if (i, k) in ((0, 1), (2, 3)) else 0.0
+
+
+def _box_cond(out, u, t, itg):
+    out[0] = u[0]
+    out[1] = (u[2] - 10.0) * u[2]
+
+
+def _box_affect(itg, ev):
+    itg.u[1] = odemi.sym_ifelse(ev[0] == -1, -itg.p[0] * itg.u[1], itg.u[1])
+    itg.u[3] = odemi.sym_ifelse(ev[1] != 0, -itg.u[3], itg.u[3])
+    odemi.terminate(itg, when=itg.t > itg.p[1])
+
+
+_BOX_CBS = odemi.CallbackSet(odemi.VectorContinuousCallback(_box_cond, _box_affect, 2),
+                             odemi.ContinuousCallback(lambda u, t, itg: u[2] - 5.0, None, None))
+
+
+@pytest.mark.parametrize("alg", [odemi.Tsit5(), odemi.Rodas5P()], ids=["tsit5", "rodas5p"])
+@pytest.mark.parametrize("engine", ["persistent", "plain"])
+def test_vector_continuous_callbacks_bit_identical(alg, engine):
+    """A VectorContinuousCallback (floor and walls of a box, restitution p[0],
+    a conditional terminate! once t > p[1]) in a CallbackSet with a scalar
+    ContinuousCallback that is never an event (affect! = affect_neg! =
+    nothing) through the lane engines, bit-identical to the CPU restatement:
+    save_everystep and final state only; times, states, retcodes, nf / naccept
+    / nreject / ncondition."""
+    N = 64
+    rng = np.random.default_rng(41)
+    u0 = np.stack([rng.uniform(10.0, 50.0, N), rng.uniform(-3.0, 3.0, N), rng.uniform(1.0, 9.0, N),
+                   rng.uniform(-6.0, 6.0, N)])
+    p = np.stack([rng.uniform(0.8, 0.95, N), rng.uniform(3.0, 14.0, N)])
+    fun = odemi.ODEFunction(_box, jac=_box_jac)
+    ea = odemi.EnsembleMI355X(engine=engine)
+    codes = set()
+    for m in (dict(save_everystep=True, save_start=True, save_end=True), dict(save_start=False, save_end=True)):
+        print(f"[case] vector continuous callbacks {alg.code} {engine} {sorted(m)}", flush=True)
+        out = odemi.solve_arrays(fun, alg, u0, p, np.zeros(N), np.full(N, 10.0), ea, reltol=1e-6, abstol=1e-6,
+                                 callback=_BOX_CBS, max_saves=4096, **m)
+        for j in range(0, N, 3):
+            r = O.solve(_box, u0[:, j], p[:, j], (0.0, 10.0), jac=_box_jac, alg=alg.code, reltol=1e-6,
+                        abstol=1e-6, ccallbacks=_BOX_CBS.continuous_callbacks, **m)
+            ns = int(out["nsaved"][j])
+            assert int(out["retcode"][j]) == r["retcode"] and ns == r["nsaved"], (j, m)
+            assert np.array_equal(out["t_out"][:ns, j], r["t"]), (j, m)
+            assert np.array_equal(out["u_out"][:ns, :, j], r["u"]), (j, m)
+            st = out["stats"][:, j]
+            assert (st[0], st[1], st[2], st[6]) == (r["nf"], r["naccept"], r["nreject"], r["ncondition"])
+            codes.add(r["retcode"])
+            assert r["u"][:, 2].min() >= -1e-9 and r["u"][:, 2].max() <= 10.0 + 1e-9
+    assert codes == {1, 2}

@@ -509,6 +509,84 @@ def test_continuous_callback_reference_known_answers():
     assert r7["ncondition"] == r7["naccept"] * 11   # tprev, t and 9 interpolated points per step
 
 
+class _VCB:
+    """VectorContinuousCallback's fields with the reference's defaults."""
+
+    def __init__(self, condition, affect, len_, rootfind="left", interp_points=10, save_positions=(True, True),
+                 abstol=10 * 2.220446049250313e-16, repeat_nudge=1 / 100):
+        self.condition, self.affect, self.affect_neg, self.len = condition, affect, None, len_
+        self.rootfind, self.interp_points, self.save_positions = rootfind, interp_points, save_positions
+        self.abstol, self.repeat_nudge = abstol, repeat_nudge
+
+
+def test_vector_continuous_callback_reference_known_answers():
+    """VectorContinuousCallback (callbacks.jl:239-362, 499-521, 557-640)
+    against the reference's vector bouncing-ball tests
+    (test/Integrators_I/ode_event_tests.jl:121-140, 168-190, 262-316): the same
+    bounce times and saves as the scalar form, saveat at a bounce (or
+    t - eps(t)) gives it twice, terminate! on the ground at sqrt(100/9.81);
+    a ball in a box (floor and walls as two components) stays inside, each
+    bounce's affect sees only its own component's direction; two components
+    crossing at the same time are simultaneous."""
+    def fall(du, u, p, t):
+        du[0] = u[1]
+        du[1] = -9.81
+
+    def vcond(out, u, t, itg):
+        out[0] = u[0]
+
+    def vflip(itg, ev):
+        if ev[0] == -1:
+            itg.u[1] = -itg.u[1]
+    t1 = math.sqrt(100 / 9.81)
+    cb = _VCB(vcond, vflip, 1, interp_points=100)
+    r = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), ccallbacks=[cb], save_everystep=False)
+    rs = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), save_everystep=False,
+                 ccallbacks=[_CCB(lambda u, t, itg: u[0], None, lambda itg: itg.u.__setitem__(1, -itg.u[1]),
+                                  interp_points=100)])
+    assert np.array_equal(r["t"], rs["t"]) and np.array_equal(r["u"], rs["u"]) and len(r["t"]) == 6
+    t = r["t"][len(r["t"]) // 2 - 1]
+    for sa in ([t], [t - np.spacing(t)]):
+        r2 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, 15.0), ccallbacks=[cb], saveat=sa, save_start=False,
+                     save_end=False)
+        assert list(r2["t"]).count(t) == 2
+    r5 = O.solve(fall, [50.0, 0.0], [0.0], (0.0, math.inf),
+                 ccallbacks=[_VCB(vcond, lambda itg, ev: itg.terminate(when=ev[0] != 0), 1)])
+    assert r5["retcode"] == 2 and abs(r5["t"][-1] - t1) < 1e-13 and abs(r5["u"][-1][0]) < 3e-12
+
+    def box(du, u, p, t):
+        du[0] = u[1]
+        du[1] = -9.81
+        du[2] = u[3]
+        du[3] = 0.0
+    seen = []
+
+    def bcond(out, u, t, itg):
+        out[0] = u[0]
+        out[1] = (u[2] - 10.0) * u[2]
+
+    def baff(itg, ev):
+        seen.append(tuple(ev))
+        if ev[0] == -1:
+            itg.u[1] = -0.9 * itg.u[1]
+        if ev[1] != 0:
+            itg.u[3] = -itg.u[3]
+    r6 = O.solve(box, [50.0, 0.0, 5.0, 4.0], [0.0], (0.0, 15.0), ccallbacks=[_VCB(bcond, baff, 2)],
+                 save_everystep=True)
+    assert r6["retcode"] == 1 and r6["u"][:, 2].min() >= -1e-12 and r6["u"][:, 2].max() <= 10 + 1e-12
+    assert r6["u"][:, 0].min() >= -1e-12
+    # the floor is a downcrossing of u[0]; (x - 10) x < 0 inside, so either wall is an upcrossing
+    assert set(seen) == {(-1.0, 0.0), (0.0, 1.0)}
+    # walls hit at x = 10 and 0 alternately every 2.5 s from x = 5 at 4 m/s: 1.25, 3.75, 6.25, ...
+    wall_t = [tt for tt in r6["t"] if min(abs(tt - (1.25 + 2.5 * m)) for m in range(6)) < 1e-9]
+    assert len(set(wall_t)) == 6
+    seen.clear()
+    r7 = O.solve(box, [50.0, 0.0, 5.0, 0.0], [0.0], (0.0, 4.0), save_everystep=False,
+                 ccallbacks=[_VCB(lambda out, u, t, itg: (out.__setitem__(0, u[0]),
+                                                          out.__setitem__(1, u[0] - 0.0 * t)), baff, 2)])
+    assert (-1.0, -1.0) in seen   # two identical components: one simultaneous event
+
+
 def test_saveat_interpolant_accuracy():
     # dense output (Tsit5 free 4th-order interpolant, Rodas 3rd-order H-rows) vs exp
     sa = list(np.linspace(0.05, 0.95, 19))

@@ -119,3 +119,34 @@ def test_continuous_callbacks_emit_and_compile():
     assert rc == 0, log.value.decode()[:2000]
     with pytest.raises(ValueError):
         odemi.ContinuousCallback(lambda u, t, i: u[0], bounce, interp_points=1)
+
+
+def test_vector_continuous_callback_emits_and_compiles():
+    def box(du, u, p, t):
+        du[0] = u[1]
+        du[1] = -9.81
+        du[2] = u[3]
+        du[3] = 0.0
+
+    def cond(out, u, t, itg):
+        out[0] = u[0]
+        out[1] = (u[2] - 10.0) * u[2]
+
+    def aff(itg, ev):
+        itg.u[1] = odemi.sym_ifelse(ev[0] == -1, -itg.u[1], itg.u[1])
+        odemi.terminate(itg, when=ev[1] != 0)
+    tr = trace_problem(box, 4, 1, callbacks=[odemi.VectorContinuousCallback(cond, aff, 2)])
+    src = tr.src
+    assert "#define ODEMI_VCB_MAXLEN 2" in src and "vlen = 2" in src
+    assert "ev[0] == T(-1.0)" in src and "return nu_[4] != T(0);" in src and "u[1] = nu_[1];" in src
+    L = _lib.load()
+    path = ctypes.create_string_buffer(512)
+    log = ctypes.create_string_buffer(1 << 16)
+    secs = ctypes.c_double()
+    rc = L.odemi_rtc_codeobject(src.encode(), 4, 1, _lib.COMPILE_TSIT5, path, 512, ctypes.byref(secs), log, 1 << 16)
+    assert rc == 0, log.value.decode()[:2000]
+    with pytest.raises(TypeError, match="conditional terminate"):
+        trace_problem(box, 4, 1, callbacks=[odemi.DiscreteCallback(
+            lambda u, t, i: t == 1.0, lambda i: odemi.terminate(i, when=i.u[0] > 1.0))])
+    with pytest.raises(ValueError):
+        odemi.VectorContinuousCallback(cond, aff, 17)
