@@ -135,11 +135,13 @@ int odemi_abi_version(void);
  *     <- a CallbackSet of k DiscreteCallbacks (condition, affect! on u, save_positions,
  *        terminate! when term; lib/DiffEqBase/src/callbacks.jl:645-683)
  *   #define ODEMI_NCCB k with template<int K> struct OdemiCcb holding
- *     static constexpr int has_affect, has_affect_neg, save0, save1, rootfind (0 left,
+ *     static constexpr int vlen, has_affect, has_affect_neg, save0, save1, rootfind (0 left,
  *       1 right, 2 none), interp_points, term, term_neg;
  *     static constexpr double abstol, repeat_nudge;
  *     template<typename T, typename Q> static __device__ T cond(const T* u, const Q* p, T t);
- *     affect / affect_neg as above
+ *     affect / affect_neg as above; with vlen > 0 (VectorContinuousCallback, ODEMI_VCB_MAXLEN
+ *     the largest) instead vcond(T* out, const T* u, const Q* p, T t) and
+ *     bool vaffect(T* u, const Q* p, T t, const double* ev) returning terminate!
  *     <- ContinuousCallbacks (event root finding; lib/DiffEqBase/src/callbacks.jl:157-640)
  * n/np are the (maximum) state and parameter counts.  Code objects are cached
  * on disk by source hash ($ODEMI_CACHE_DIR).  compile_seconds may be NULL. */
