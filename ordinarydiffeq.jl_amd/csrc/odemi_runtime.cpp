@@ -280,6 +280,11 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     // minimum waves per SIMD for the persistent lane kernels (occupancy vs spills)
     const char* mw = getenv("ODEMI_MIN_WAVES");
     s << "#define ODEMI_MIN_WAVES " << ((mw && *mw) ? atoi(mw) : 4) << "\n";
+    // the mixed lane Rodas5P persistent kernel: 3 waves per SIMD (168 VGPRs, a
+    // few spills) measured 20 % faster than 2 (198 VGPRs) and than 4 (128 VGPRs,
+    // heavy spills); the FP64 kernel gains nothing from a cap
+    const char* rw = getenv("ODEMI_RODAS_MIN_WAVES");
+    s << "#define ODEMI_RODAS_MIN_WAVES " << ((rw && *rw) ? atoi(rw) : 3) << "\n";
     // queue slots a wave takes per atomic (>= 64)
     const char* qc = getenv("ODEMI_QUEUE_CHUNK");
     if (qc && *qc && atoi(qc) >= 64) s << "#define ODEMI_QUEUE_CHUNK " << atoi(qc) << "\n";
@@ -324,7 +329,7 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     if (flags & ODEMI_COMPILE_RODAS5P) {
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_persist_f64(odemi::Params P)"
              " { odemi::rodas_persistent<double, ODEMI_N>(P); }\n";
-        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_persist_mixed(odemi::Params P)"
+        s << "extern \"C\" __global__ void __launch_bounds__(256, ODEMI_RODAS_MIN_WAVES) odemi_rodas_persist_mixed(odemi::Params P)"
              " { odemi::rodas_persistent<float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rodas_plain_init(odemi::Params P)"
              " { odemi::rodas_plain_init<ODEMI_N>(P); }\n";
