@@ -18,6 +18,20 @@
  *   lib/OrdinaryDiffEqCore/src/integrators/controllers.jl:244-293,744-843; alg_utils.jl:407-879
  *   lib/DiffEqBase/src/calculate_residuals.jl:9-14, common_defaults.jl:69-75,190-225,
  *   lib/DiffEqBase/src/check_error.jl:69-118, utils.jl:90-94
+ *   lib/OrdinaryDiffEqCore/src/integrators/integrator_utils.jl:215-239,720-725,1081-1145,1325-1338
+ *       and integrator_interface.jl:5-80,443-446                (callbacks, reeval_fsal, terminate!)
+ *   lib/DiffEqBase/src/callbacks.jl:91-146,157-640,645-719     (Discrete / Continuous / Vector callbacks)
+ *   lib/OrdinaryDiffEqTsit5/src/tsit_perform_step.jl:1-38, lib/OrdinaryDiffEqRosenbrock/src/
+ *       stiff_addsteps.jl:140-224                               (the event re-step)
+ *
+ * Parity unpinned (no reference test or fixture fixes them; the reference
+ * cannot run here): FastPower's Float32 log2/exp2 rounding; LinearSolve's LU
+ * operation order; MuladdMacro's nesting beyond its documented example; the
+ * continuous-callback root finder (the reference's ModAB interval solve from
+ * BracketingNonlinearSolve is restated as bisection to the same adjacent-float
+ * bracket, so the event float agrees but the number of condition calls,
+ * stats.ncondition, does not) and the interp_points sample times
+ * (tprev + (i - 1) h here, Julia's range within an ulp).
  *
  * Floating-point conventions of the restatement (shared with the GPU engine by
  * specification, not by code):
