@@ -38,7 +38,8 @@ const char* kDeviceSource =
 
 // rows of the int32 integrator state of the plain engine and the serving pool
 // (iter, success_iter, accept, next_step_tstop, nf, naccept, nreject, slot
-// state, saveat cursor, njacs, nw, nsolve, tstops cursor, isoutofdomain verdict)
+// state, saveat cursor, njacs, nw, nsolve, tstops cursor, isout | reeval_fsal
+// << 1, ncondition, last continuous event | its vector components << 8)
 constexpr size_t kIntRows = 16;
 // DEStats rows written per trajectory: nf naccept nreject njacs nw nsolve ncondition
 constexpr size_t kStatRows = 7;
