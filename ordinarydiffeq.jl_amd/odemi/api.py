@@ -547,13 +547,46 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
     return o
 
 
+def _concat_outputs(parts):
+    """Per-batch output dicts joined along the trajectory axis (the last one);
+    save buffers are padded with NaN to the largest batch capacity."""
+    if len(parts) == 1:
+        return parts[0]
+    out = dict(parts[-1])
+    ms = max(q["t_out"].shape[0] for q in parts)
+
+    def pad(a):
+        if a.shape[0] == ms:
+            return a
+        fill = np.full((ms - a.shape[0],) + a.shape[1:], np.nan)
+        return np.concatenate([a, fill], axis=0)
+    out["t_out"] = np.concatenate([pad(q["t_out"]) for q in parts], axis=-1)
+    out["u_out"] = np.concatenate([pad(q["u_out"]) for q in parts], axis=-1)
+    out["max_saves"] = max(q["max_saves"] for q in parts)
+    for k in ("nsaved", "u_final", "t_final", "retcode", "stats"):
+        out[k] = np.concatenate([q[k] for q in parts], axis=-1)
+    out["wall_s"] = sum(q["wall_s"] for q in parts)
+    out["kernel_ms"] = sum(q["kernel_ms"] for q in parts)
+    out["launches"] = sum(q["launches"] for q in parts)
+    return out
+
+
 def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=None, saveat=None,
           save_start=None, save_end=None, save_everystep=None, dt=None, dtmax=None, dtmin=0.0,
           maxiters=1_000_000, seed=0, batch_size=None, max_saves=None, tstops=None, isoutofdomain=None,
           callback=None, **kw):
     """solve(EnsembleProblem, alg, EnsembleMI355X(); trajectories, ...) -> EnsembleSolution.
 
-    A plain ODEProblem is solved as a one-trajectory ensemble and returns its ODESolution."""
+    SciMLBase's ensemble semantics (NEWS.md:209-225 for the prob_func /
+    output_func contexts): trajectories run in batches of `batch_size` (one
+    launch each, default all); after a batch, output_func(sol, ctx) runs per
+    trajectory and a trajectory whose output_func asks for a rerun is redrawn
+    with prob_func at ctx.repeat + 1 and solved again; then reduction(u,
+    batch_outputs, I) -> (u, converged) folds the batch and a converged
+    reduction ends the solve.  ctx.rng is seeded from (seed, sim_id, repeat).
+    EnsembleSolution.arrays holds the packed outputs of the batches' first
+    launches.  A plain ODEProblem is solved as a one-trajectory ensemble and
+    returns its ODESolution."""
     if kw:
         raise TypeError(f"unsupported keyword(s): {sorted(kw)}")
     single = isinstance(prob, ODEProblem)
@@ -564,60 +597,88 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=No
     t_start = time.perf_counter()
     base = ens.prob
     N = int(trajectories)
-    probs = []
-    for i in range(N):
-        ctx_i = EnsembleContext(i + 1, 1, np.random.default_rng([int(seed), i + 1]))
-        pi = ens.prob_func(remake(base) if ens.safetycopy else base, ctx_i)
-        probs.append(pi)
+    bs = N if not batch_size else max(1, int(batch_size))
     n = len(base.u0)
     np_ = len(base.p)
-    for pi in probs:
+    # save rules are resolved against the template tspan (shared by the common case)
+    ss, se, sev, seu = _resolve_save_flags(saveat, save_start, save_end, save_everystep,
+                                           base.tspan[0], base.tspan[1])
+    cbs = _callback_list(callback)
+    forced = any(any(c.save_positions) for c in cbs)
+    if max_saves is None and (sev or forced):
+        max_saves = 4096
+
+    def draw(i, repeat):
+        rng = np.random.default_rng([int(seed), i + 1] if repeat == 1 else [int(seed), i + 1, repeat])
+        pi = ens.prob_func(remake(base) if ens.safetycopy else base, EnsembleContext(i + 1, repeat, rng))
         if len(pi.u0) != n or len(pi.p) != np_:
             raise ValueError("all trajectories must share the state and parameter dimensions")
         if pi.f.f is not base.f.f:
             raise ValueError("prob_func must not change the right-hand side function")
         if not (pi.tspan[1] > pi.tspan[0]) and pi.tspan[1] != pi.tspan[0]:
             raise ValueError("only forward integration (tspan[2] >= tspan[1]) is supported")
-    u0 = np.stack([pi.u0 for pi in probs], axis=1) if N else np.zeros((n, 0))
-    pp = np.stack([pi.p for pi in probs], axis=1) if np_ else None
-    t0 = np.array([pi.tspan[0] for pi in probs])
-    tf = np.array([pi.tspan[1] for pi in probs])
-    # save rules are resolved against the template tspan (shared by the common case)
-    ss, se, sev, seu = _resolve_save_flags(saveat, save_start, save_end, save_everystep,
-                                           base.tspan[0], base.tspan[1])
-    if saveat is not None and np.isscalar(saveat):
-        sa = saveat_times(saveat, float(t0.min()), float(tf.max()))
-    elif saveat is not None and len(saveat):
-        sa = saveat_times(saveat, 0, 0)
-    else:
-        sa = np.zeros(0)
-    cbs = _callback_list(callback)
-    forced = any(any(c.save_positions) for c in cbs)
-    if max_saves is None and (sev or forced):
-        max_saves = 4096
-    o = solve_arrays(base.f, alg, u0, pp, t0, tf, ea, reltol=reltol, abstol=abstol, saveat=sa, save_start=ss,
-                     save_end=se, save_everystep=sev, save_end_user=seu, max_saves=max_saves, dt=dt,
-                     dtmax=dtmax, dtmin=dtmin, maxiters=maxiters, tstops=tstops,
-                     isoutofdomain=isoutofdomain, callback=cbs)
-    if (sev or forced) and int(o["nsaved"].max(initial=0)) > o["max_saves"]:
-        raise RuntimeError(f"the saves (save_everystep / callback save_positions) exceeded max_saves="
-                           f"{o['max_saves']} points; pass a larger max_saves")
-    sols = []
-    outs = []
-    for j in range(N):
-        ns = min(int(o["nsaved"][j]), o["max_saves"])
-        st = o["stats"][:, j]
-        sol = ODESolution(o["t_out"][:ns, j].copy(), [o["u_out"][k, :, j].copy() for k in range(ns)],
-                          ReturnCode(int(o["retcode"][j])), DEStats(*st), probs[j], alg)
-        res, _rerun = ens.output_func(sol, EnsembleContext(j + 1, 1, None))
-        outs.append(res)
-    if ens.reduction is not None:
-        u_acc = ens.u_init
-        u_acc, _conv = ens.reduction(u_acc, outs, list(range(1, N + 1)))
-        sols = u_acc
-    else:
-        sols = outs
+        return pi
+
+    def run(probs):
+        Nb = len(probs)
+        u0 = np.stack([pi.u0 for pi in probs], axis=1) if Nb else np.zeros((n, 0))
+        pp = np.stack([pi.p for pi in probs], axis=1) if np_ else None
+        t0 = np.array([pi.tspan[0] for pi in probs])
+        tf = np.array([pi.tspan[1] for pi in probs])
+        if saveat is not None and np.isscalar(saveat):
+            sa = saveat_times(saveat, float(t0.min(initial=base.tspan[0])), float(tf.max(initial=base.tspan[1])))
+        elif saveat is not None and len(saveat):
+            sa = saveat_times(saveat, 0, 0)
+        else:
+            sa = np.zeros(0)
+        o = solve_arrays(base.f, alg, u0, pp, t0, tf, ea, reltol=reltol, abstol=abstol, saveat=sa,
+                         save_start=ss, save_end=se, save_everystep=sev, save_end_user=seu, max_saves=max_saves,
+                         dt=dt, dtmax=dtmax, dtmin=dtmin, maxiters=maxiters, tstops=tstops,
+                         isoutofdomain=isoutofdomain, callback=cbs)
+        if (sev or forced) and int(o["nsaved"].max(initial=0)) > o["max_saves"]:
+            raise RuntimeError(f"the saves (save_everystep / callback save_positions) exceeded max_saves="
+                               f"{o['max_saves']} points; pass a larger max_saves")
+        sols = []
+        for j in range(Nb):
+            ns = min(int(o["nsaved"][j]), o["max_saves"])
+            st = o["stats"][:, j]
+            sols.append(ODESolution(o["t_out"][:ns, j].copy(), [o["u_out"][k, :, j].copy() for k in range(ns)],
+                                    ReturnCode(int(o["retcode"][j])), DEStats(*st), probs[j], alg))
+        return sols, o
+
+    u_acc = ens.u_init
+    converged = False
+    collected, parts = [], []
+    for b0 in range(0, N, bs):
+        idx = list(range(b0, min(N, b0 + bs)))
+        sols, o = run([draw(i, 1) for i in idx])
+        parts.append(o)
+        outs = [None] * len(idx)
+        pending = list(range(len(idx)))
+        repeat = 1
+        while pending:
+            redo = []
+            for q in pending:
+                res, rerun = ens.output_func(sols[q], EnsembleContext(idx[q] + 1, repeat, None))
+                outs[q] = res
+                if rerun:
+                    redo.append(q)
+            if not redo:
+                break
+            repeat += 1
+            resols, _ = run([draw(idx[q], repeat) for q in redo])
+            for q, sq in zip(redo, resols):
+                sols[q] = sq
+            pending = redo
+        if ens.reduction is not None:
+            u_acc, converged = ens.reduction(u_acc, outs, [i + 1 for i in idx])
+            if converged:
+                break
+        else:
+            collected += outs
+    o = _concat_outputs(parts) if parts else run([])[1]
+    result = u_acc if ens.reduction is not None else collected
     elapsed = time.perf_counter() - t_start
     if single:
-        return sols[0]
-    return EnsembleSolution(sols, elapsed, o)
+        return result[0]
+    return EnsembleSolution(result, elapsed, o, converged=bool(converged) if ens.reduction is not None else True)

@@ -672,3 +672,34 @@ def test_vector_continuous_callbacks_bit_identical(alg, engine):
             codes.add(r["retcode"])
             assert r["u"][:, 2].min() >= -1e-9 and r["u"][:, 2].max() <= 10.0 + 1e-9
     assert codes == {1, 2}
+
+
+def test_ensemble_batches_rerun_and_converged_reduction():
+    """SciMLBase's ensemble loop: batch_size batches, output_func's rerun flag
+    (the trajectory is redrawn by prob_func with ctx.repeat + 1 and solved
+    again), reduction(u, batch, I) -> (u, converged) stopping at convergence;
+    outputs equal to one unbatched solve."""
+    print("[case] ensemble batches", flush=True)
+    prob = odemi.ODEProblem(_lin, [1.0], (0.0, 1.0), [-1.0])
+    pf = lambda pr, ctx: odemi.remake(pr, p=[-0.1 * ctx.sim_id - (ctx.repeat - 1)])
+    full = odemi.solve(odemi.EnsembleProblem(prob, prob_func=pf), odemi.Tsit5(), trajectories=10,
+                       reltol=1e-8, abstol=1e-10)
+    ends = [s.u[-1][0] for s in full.u]
+    seen = []
+
+    def red(u, batch, I):
+        seen.append(list(I))
+        u = u + [s.u[-1][0] for s in batch]
+        return u, len(u) >= 6
+    sim = odemi.solve(odemi.EnsembleProblem(prob, prob_func=pf, reduction=red, u_init=[]), odemi.Tsit5(),
+                      trajectories=10, batch_size=3, reltol=1e-8, abstol=1e-10)
+    assert seen == [[1, 2, 3], [4, 5, 6]] and sim.converged and sim.u == ends[:6]
+    calls = {}
+
+    def of(sol, ctx):   # rerun trajectory 2 twice: its third draw has p = -0.2 - 2
+        calls[ctx.sim_id] = calls.get(ctx.sim_id, 0) + 1
+        return (ctx.repeat, sol.u[-1][0]), ctx.sim_id == 2 and ctx.repeat < 3
+    sim = odemi.solve(odemi.EnsembleProblem(prob, prob_func=pf, output_func=of), odemi.Tsit5(), trajectories=4,
+                      batch_size=2, reltol=1e-8, abstol=1e-10)
+    assert [r for r, _ in sim.u] == [1, 3, 1, 1] and calls == {1: 1, 2: 3, 3: 1, 4: 1}
+    assert abs(sim.u[1][1] - math.exp(-2.2)) < 1e-7 and sim.u[0][1] == ends[0]
