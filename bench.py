@@ -155,7 +155,9 @@ def bench_rodas(args, workload):
         "roofline": {"bound": "valu-fp64" if nvec is None else "lds/valu-fp32", "achieved": flops / (kms / 1e3) / 1e12,
                      "peak": PEAK_FP64_TFLOPS if nvec is None else PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": flops / (kms / 1e3) / 1e12 / (PEAK_FP64_TFLOPS if nvec is None else PEAK_FP32_TFLOPS),
-                     "traffic": None, "kernel_ms": kms,
+                     "traffic": (_traffic("odemi_rodas_persist_mixed@robertson_262k") if nvec is None and N == 262144
+                                 else _traffic("odemi_rodas_wg_mixed@chain_2048") if nvec is not None and N == 2048
+                                 else None), "kernel_ms": kms,
                      "kernel": "odemi_rodas_persist_mixed" if nvec is None else "odemi_rodas_wg_mixed"},
     }
     res["refine_sweeps"] = args.refine_sweeps
