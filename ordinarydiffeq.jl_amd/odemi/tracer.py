@@ -630,6 +630,9 @@ def _emit_continuous_callbacks(callbacks, n, u, p, t):
     (u, t, integrator), affect! / affect_neg! (None: that crossing is no event)
     and the options as constants; VectorContinuousCallbacks (with `len`) take
     the vector form."""
+    if len(callbacks) > 255:
+        raise ValueError("at most 255 continuous callbacks (the integrator state keeps the last one's index "
+                         "in 8 bits)")
     vmax = max([getattr(c, "len", 0) for c in callbacks] + [1])
     src = [f"#define ODEMI_NCCB {len(callbacks)}", f"#define ODEMI_VCB_MAXLEN {vmax}",
            "template <int K> struct OdemiCcb;"]
