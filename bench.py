@@ -157,6 +157,7 @@ def bench_rodas(args, workload):
                      "frac": flops / (kms / 1e3) / 1e12 / (PEAK_FP64_TFLOPS if nvec is None else PEAK_FP32_TFLOPS),
                      "traffic": (_traffic("odemi_rodas_persist_mixed@robertson_262k") if nvec is None and N == 262144
                                  else _traffic("odemi_rodas_wg_mixed@chain_2048") if nvec is not None and N == 2048
+                                 else _traffic("odemi_rodas_wg_mixed@chain_16k") if nvec is not None and N == 16384
                                  else None), "kernel_ms": kms,
                      "kernel": "odemi_rodas_persist_mixed" if nvec is None else "odemi_rodas_wg_mixed"},
     }
