@@ -1433,6 +1433,27 @@ static void expb_jac(double* J, const double* u, const double* p, double t, void
     for (int i = 0; i < n; i++) J[i * n + i] = exp(u[i]);
 }
 
+static void ball_f(double* du, const double* u, const double* p, double t, void* user) {
+    (void)p; (void)t; (void)user;
+    du[0] = u[1];
+    du[1] = -9.81;
+}
+static void ball_jac(double* J, const double* u, const double* p, double t, void* user) {
+    (void)u; (void)p; (void)t; (void)user;
+    J[0] = 0.0; J[1] = 1.0; J[2] = 0.0; J[3] = 0.0;
+}
+static double ball_cond(const double* u, const double* p, double t, int k, void* user) {
+    (void)p; (void)t; (void)k; (void)user;
+    return u[0];
+}
+static int ball_affect(double* u, const double* p, double t, int k, int neg, void* user) {
+    (void)t; (void)k; (void)user;
+    if (neg) u[1] = (-p[0]) * u[1];
+    return 0;
+}
+static const int ball_cb_flags[OO_CCB_NFLAGS] = {0, 1, 0, 0, 0, 10, 0};
+static const double ball_cb_tol[2] = {10.0 * 2.220446049250313e-16, 0.01};
+
 static int builtin_fns(int prob, oo_rhs_fn* f, oo_jac_fn* j) {
     switch (prob) {
         case OO_PROB_LORENZ: *f = lorenz_f; *j = lorenz_jac; return 3;
@@ -1440,6 +1461,7 @@ static int builtin_fns(int prob, oo_rhs_fn* f, oo_jac_fn* j) {
         case OO_PROB_CHAIN: *f = chain_f; *j = chain_jac; return 3;
         case OO_PROB_LINEAR: *f = linear_f; *j = linear_jac; return 1;
         case OO_PROB_EXPBLOWUP: *f = expb_f; *j = expb_jac; return 0;
+        case OO_PROB_BALL: *f = ball_f; *j = ball_jac; return 2;
         default: return -1;
     }
 }
@@ -1452,6 +1474,10 @@ int oo_solve_builtin(int prob, int n, const double* u0, const double* p, double 
     builtin_user bu = {n};
     oo_opts o2 = *opts;
     o2.has_tgrad = 0;
+    if (prob == OO_PROB_BALL && o2.nccb == 0) {
+        o2.nccb = 1; o2.ccb_cond = ball_cond; o2.ccb_affect = ball_affect;
+        o2.ccb_flags = ball_cb_flags; o2.ccb_tol = ball_cb_tol; o2.ccb_user = NULL;
+    }
     return oo_solve(n, f, j, NULL, &bu, u0, p, t0, tf, &o2, t_out, u_out, max_out, u_final, res);
 }
 

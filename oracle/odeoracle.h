@@ -24,7 +24,11 @@ enum {
 
 /* Built-in right-hand sides (the benchmark problem families). */
 enum { OO_PROB_USER = 0, OO_PROB_LORENZ = 1, OO_PROB_ROBERTSON = 2, OO_PROB_CHAIN = 3,
-       OO_PROB_LINEAR = 4, OO_PROB_EXPBLOWUP = 5 };
+       OO_PROB_LINEAR = 4, OO_PROB_EXPBLOWUP = 5,
+       /* bouncing ball u = (height, velocity), u' = (v, -9.81), p = (restitution);
+        * built in with its floor event: a ContinuousCallback on u[0] with
+        * affect_neg! v <- -p[0] v, no saves (unless opts already carry callbacks) */
+       OO_PROB_BALL = 6 };
 
 typedef void (*oo_rhs_fn)(double* du, const double* u, const double* p, double t, void* user);
 /* J is row-major n x n: J[i*n + j] = d f_i / d u_j */

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "libodeoracle.so")
 
 TSIT5, RODAS5P = 0, 1
-PROB_USER, PROB_LORENZ, PROB_ROBERTSON, PROB_CHAIN, PROB_LINEAR, PROB_EXPBLOWUP = range(6)
+PROB_USER, PROB_LORENZ, PROB_ROBERTSON, PROB_CHAIN, PROB_LINEAR, PROB_EXPBLOWUP, PROB_BALL = range(7)
 RC_NAMES = {0: "Default", 1: "Success", 2: "Terminated", 3: "DtNaN", 4: "MaxIters",
             5: "MaxNumSub", 6: "DtLessThanMin", 7: "Unstable"}
 
@@ -297,7 +297,8 @@ def solve_builtin(prob, u0, p, tspan, max_out=4096, **kw):
     ns = min(res.nsaved, max_out)
     return dict(t=t_out[:ns].copy(), u=u_out[:ns, :n].copy(), u_final=uf[:n].copy(), retcode=res.retcode,
                 nf=res.nf, naccept=res.naccept, nreject=res.nreject, njacs=res.njacs, nw=res.nw,
-                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved, dt_init=res.dt_init)
+                nsolve=res.nsolve, t_final=res.t_final, nsaved=res.nsaved, dt_init=res.dt_init,
+                ncondition=res.ncondition)
 
 
 def ensemble_builtin(prob, u0_soa, p_soa, t0, tf, nvec=None, threads=None, chunk=64, **kw):

@@ -6,7 +6,7 @@ workload, so the GPU engine and the CPU oracle see identical arrays.
 """
 import numpy as np
 
-from .api import ODEFunction
+from .api import ContinuousCallback, ODEFunction
 
 
 def lorenz(du, u, p, t):
@@ -69,7 +69,26 @@ def chain_jac(J, u, p, t):
         J[r, m + i] = -c
 
 
+def ball(du, u, p, t):
+    du[0] = u[1]
+    du[1] = -9.81
+
+
+def ball_jac(J, u, p, t):
+    J[0, 0] = 0.0
+    J[0, 1] = 1.0
+    J[1, 0] = 0.0
+    J[1, 1] = 0.0
+
+
+def _bounce(integrator):
+    integrator.u[1] = -integrator.p[0] * integrator.u[1]
+
+
 LORENZ = ODEFunction(lorenz, jac=lorenz_jac)
+BALL = ODEFunction(ball, jac=ball_jac)
+# the floor: a downcrossing of the height flips the velocity with restitution p[0]
+BALL_BOUNCE = ContinuousCallback(lambda u, t, integrator: u[0], None, _bounce, save_positions=(False, False))
 ROBERTSON = ODEFunction(robertson, jac=robertson_jac)
 CHAIN = ODEFunction(chain, jac=chain_jac)
 
@@ -119,3 +138,12 @@ def chain_ensemble(N=16384, seed=4, m_lo=4, m_hi=64):
     p = np.stack([k, c, al])
     return dict(u0=u0, p=p, t0=np.zeros(N), tf=np.full(N, 10.0), nvec=(2 * m).astype(np.int32),
                 reltol=1e-6, abstol=1e-6)
+
+
+def ball_ensemble(N=262144, seed=5):
+    """Bouncing balls: h0~U[10,50], v0~U[-5,5], restitution~U[0.8,0.95], tspan (0,10);
+    every trajectory bounces 1-5 times (the accumulation of bounces lies beyond t = 10)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u0 = np.stack([rng.uniform(10.0, 50.0, N), rng.uniform(-5.0, 5.0, N)])
+    p = rng.uniform(0.8, 0.95, (1, N))
+    return dict(u0=u0, p=p, t0=np.zeros(N), tf=np.full(N, 10.0), reltol=1e-6, abstol=1e-6)
