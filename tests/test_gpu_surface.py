@@ -703,3 +703,18 @@ def test_ensemble_batches_rerun_and_converged_reduction():
                       batch_size=2, reltol=1e-8, abstol=1e-10)
     assert [r for r, _ in sim.u] == [1, 3, 1, 1] and calls == {1: 1, 2: 3, 3: 1, 4: 1}
     assert abs(sim.u[1][1] - math.exp(-2.2)) < 1e-7 and sim.u[0][1] == ends[0]
+
+
+def test_events_workload_matches_builtin_restatement():
+    """The events benchmark's problem (bouncing balls with the floor
+    ContinuousCallback) on the FP64 persistent engine equals the restatement's
+    built-in C floor event bit for bit (final state, retcode, stats)."""
+    print("[case] events workload vs built-in restatement", flush=True)
+    d = P.ball_ensemble(N=512)
+    out = odemi.solve_arrays(P.BALL, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"],
+                             odemi.EnsembleMI355X(precision="fp64"), reltol=d["reltol"], abstol=d["abstol"],
+                             save_start=False, callback=P.BALL_BOUNCE)
+    ref = O.ensemble_builtin(O.PROB_BALL, d["u0"], d["p"], d["t0"], d["tf"], alg=O.TSIT5, reltol=d["reltol"],
+                             abstol=d["abstol"], save_start=False)
+    assert np.array_equal(out["u_final"], ref["u_final"]) and np.array_equal(out["retcode"], ref["retcode"])
+    assert np.array_equal(out["stats"][1], ref["naccept"]) and np.array_equal(out["stats"][0], ref["nf"])

@@ -733,3 +733,29 @@ def test_det_log_exp_accuracy():
     ee = max(abs(O.det_exp(float(y)) - math.exp(y)) / math.exp(y) for y in ys)
     assert el < 3e-16 and ee < 3e-16
     assert O.det_exp(0.0) == 1.0 and O.det_log(1.0) == 0.0
+
+
+def test_builtin_ball_event_matches_traced_form():
+    """The restatement's built-in bouncing ball (C floor event, the CPU
+    baseline of the events workload) equals the same ContinuousCallback run
+    through the Python callback path, bit for bit, and bounces first at the
+    closed-form time (h0 + v0 t - 9.81 t^2 / 2 = 0)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "ordinarydiffeq.jl_amd"))
+    from odemi import problems as P
+    d = P.ball_ensemble(N=32)
+    for j in range(0, 32, 5):
+        h0, v0 = d["u0"][0, j], d["u0"][1, j]
+        a = O.solve_builtin(O.PROB_BALL, d["u0"][:, j], d["p"][:, j], (0.0, 10.0), alg=O.TSIT5, reltol=1e-6,
+                            abstol=1e-6, save_start=False)
+        b = O.solve(P.ball, d["u0"][:, j], d["p"][:, j], (0.0, 10.0), alg=O.TSIT5, reltol=1e-6, abstol=1e-6,
+                    save_start=False, ccallbacks=[P.BALL_BOUNCE])
+        assert np.array_equal(a["u_final"], b["u_final"]) and a["ncondition"] == b["ncondition"]
+        assert a["naccept"] == b["naccept"] and a["nf"] == b["nf"]
+        tb = (v0 + math.sqrt(v0 * v0 + 2 * 9.81 * h0)) / 9.81
+        c = O.solve_builtin(O.PROB_BALL, d["u0"][:, j], d["p"][:, j], (0.0, tb + 0.5), alg=O.TSIT5, reltol=1e-6,
+                            abstol=1e-6, save_start=False, save_everystep=True)
+        vb = v0 - 9.81 * tb
+        # just after the bounce the velocity is -restitution * impact velocity
+        assert abs(c["u"][-1][1] - (-d["p"][0, j] * vb - 9.81 * (tb + 0.5 - tb))) < 1e-6
