@@ -318,13 +318,14 @@ __device__ __forceinline__ float dpp_row_sum16(float x) {
     return x;
 }
 // Maximum over the wave, returned to every lane.  All 64 lanes must be active.
+// Lanes without a source read 0, the identity of the unsigned maximum.
 __device__ __forceinline__ unsigned dpp_wave_max_u32(unsigned x) {
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x111, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x112, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x114, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x118, 0xf, 0xf, false));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x142, 0xa, 0xf, false));  // row_bcast:15
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true));
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, true));  // row_bcast:15
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, true));  // row_bcast:31
     return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
 }
 
@@ -366,6 +367,15 @@ __device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, i
 // Rows not yet used keep sc = 1, so the pivot search and the pivot row read
 // raw values.  The scales (one LDS word per row) are folded into the tile at
 // the end.
+//
+// The step is one fma per element with no selects: column k is published with
+// 0 in the pivot row's slot (so the pivot row's update is a no-op), the
+// threads holding column k update it from zero with d in place of r[k] d (so
+// it becomes -c d), and the 1 that belongs at
+// (p_k, k) is added at the end, before the scales.  That entry is only ever
+// updated afterwards, never read as a pivot column or pivot row entry, so
+// adding it last is exact algebra.  The pivot itself is read from the staged
+// pivot row.
 //
 // One barrier per step.  Before the barrier that opens step k the owners
 // publish column k, column k+1 and (all R row slots of the thread row holding)
@@ -422,6 +432,9 @@ __device__ __forceinline__ void gj_invert(GJTile<R>& G, int n, int* pk, float* g
             for (int b = 0; b < R; b++) stg[(a * 16 + tj) * R + b] = G.T[a][b];
     }
     __syncthreads();
+    // every wave has searched column 0: its pivot slot can go to zero
+    if (tj == 0 && ti == (p & 15)) cbuf[ti * R + (p >> 4)] = 0.0f;
+    __syncthreads();
     // ---- steps ----
 #pragma unroll
     for (int kb = 0; kb < R; kb++) {
@@ -436,12 +449,11 @@ __device__ __forceinline__ void gj_invert(GJTile<R>& G, int n, int* pk, float* g
             const float* nb = nbuf + par * NP;
             const float* st = stg + par * R * NP;
             const int pa = p >> 4;
-            const bool own = (ti == (p & 15));
             const bool colk = (tj == kt);
             // all reads of the step first (the search for p_{k+1} only needs
             // these, so its latency overlaps the tile update).  At the last step
             // k1 = n <= NP: the reads stay inside the buffers and are unused.
-            const float piv = cb[gj_cidx<R>(p)];
+            const float piv = st[(pa * 16 + kt) * R + kb];
             float c[R], r[R], cq[NQ], nq[NQ];
 #pragma unroll
             for (int a = 0; a < R; a++) c[a] = cb[ti * R + a];
@@ -464,15 +476,18 @@ __device__ __forceinline__ void gj_invert(GJTile<R>& G, int n, int* pk, float* g
                 const int i = lane + 64 * q;
                 if (i < n && !((used >> q) & 1u)) key = max(key, gj_key(fmaf(-cq[q], rs1, nq[q]), i));
             }
-            // rank-1 update of every row but p_k; column k becomes -c d (1 on row p_k)
+            // rank-1 update (c is 0 on row p_k); column k becomes 0 - c d
+            float rd[R];
 #pragma unroll
-            for (int a = 0; a < R; a++) {
-                const bool isp = own && (a == pa);
-                const float ct = isp ? 0.0f : c[a];
+            for (int b = 0; b < R; b++) rd[b] = r[b] * d;
+            rd[kb] = colk ? d : rd[kb];
 #pragma unroll
-                for (int b = 0; b < R; b++) G.T[a][b] = fmaf(-ct, r[b] * d, G.T[a][b]);
-                G.T[a][kb] = colk ? (isp ? 1.0f : -c[a] * d) : G.T[a][kb];
-            }
+            for (int a = 0; a < R; a++)
+#pragma unroll
+                for (int b = 0; b < R; b++) {
+                    const float t = (b == kb && colk) ? 0.0f : G.T[a][b];
+                    G.T[a][b] = fmaf(-c[a], rd[b], t);
+                }
             if (tid == 0) scl[p] = d;
             key = dpp_wave_max_u32(key);
             if (k1 >= n) break;
@@ -506,16 +521,28 @@ __device__ __forceinline__ void gj_invert(GJTile<R>& G, int n, int* pk, float* g
 #pragma unroll
                     for (int b = 0; b < R; b++) stn[(a * 16 + tj) * R + b] = G.T[a][b];
             }
+            // the pivot slot of column k+1 goes to zero (its publisher wrote it
+            // above; nobody reads this parity's buffers before the barrier)
+            if (tj == (k1 & 15) && ti == (p & 15)) cbn[ti * R + (p >> 4)] = 0.0f;
             __syncthreads();
         }
     }
     __syncthreads();
+    float pjv[R];
+#pragma unroll
+    for (int b = 0; b < R; b++) {
+        const int j = tj + 16 * b;
+        pjv[b] = (j < n) ? (float)pk[j] : -1.0f;
+    }
 #pragma unroll
     for (int a = 0; a < R; a++) {
         const int i = ti + 16 * a;
         const float sa = (i < n) ? scl[i] : 1.0f;
 #pragma unroll
-        for (int b = 0; b < R; b++) G.T[a][b] *= sa;
+        for (int b = 0; b < R; b++) {
+            const float v = (pjv[b] == (float)i) ? G.T[a][b] + 1.0f : G.T[a][b];
+            G.T[a][b] = v * sa;
+        }
     }
 }
 
