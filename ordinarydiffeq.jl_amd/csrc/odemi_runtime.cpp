@@ -278,9 +278,10 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     std::ostringstream s;
     s << "#define ODEMI_N " << n << "\n#define ODEMI_NP " << (np > 0 ? np : 1) << "\n";
     s << "#define ODEMI_HAS_TGRAD " << ((flags & ODEMI_HAS_USER_TGRAD) ? 1 : 0) << "\n";
-    // minimum waves per SIMD for the persistent lane kernels (occupancy vs spills)
+    // minimum waves per SIMD for the persistent Tsit5 lane kernels (occupancy vs
+    // spills): $ODEMI_MIN_WAVES, else chosen after the problem source below
     const char* mw = getenv("ODEMI_MIN_WAVES");
-    s << "#define ODEMI_MIN_WAVES " << ((mw && *mw) ? atoi(mw) : 4) << "\n";
+    if (mw && *mw) s << "#define ODEMI_MIN_WAVES " << atoi(mw) << "\n";
     // the mixed lane Rodas5P persistent kernel: 3 waves per SIMD (168 VGPRs, a
     // few spills) measured 20 % faster than 2 (198 VGPRs) and than 4 (128 VGPRs,
     // heavy spills); the FP64 kernel gains nothing from a cap
@@ -294,6 +295,11 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     if (rm && *rm && atoi(rm) >= 1 && atoi(rm) <= 64) s << "#define ODEMI_REFILL_MIN " << atoi(rm) << "\n";
     if (flags & ODEMI_COMPILE_WG) s << "#define ODEMI_WITH_WG 1\n";
     s << problem_src << "\n";
+    // 4 waves per SIMD (128 VGPRs); with continuous callbacks the event search
+    // spills at that cap, and 3 measured 8 % faster (1M bouncing balls, FP64
+    // engine: 1.91 -> 1.76 ms)
+    s << "#ifndef ODEMI_MIN_WAVES\n#ifdef ODEMI_NCCB\n#define ODEMI_MIN_WAVES 3\n#else\n"
+         "#define ODEMI_MIN_WAVES 4\n#endif\n#endif\n";
     s << kDeviceSource << "\n";
     // layout check of the kernel argument struct against this file's mirror
     s << "extern \"C\" __device__ const unsigned odemi_params_bytes = sizeof(odemi::Params);\n";
