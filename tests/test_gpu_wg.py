@@ -72,6 +72,23 @@ def test_chain_uneven_mixed_gauss_jordan_within_tolerance():
     assert same >= 0.8   # the mixed arithmetic may move a step boundary; the error bound is the criterion
 
 
+def test_chain_mid_n_mixed_gauss_jordan_tiles_4_to_6():
+    """The middle Gauss-Jordan tiles (n = 50..94: R = 4, 5, 6) in one uneven
+    mixed batch, against the FP64 restatement."""
+    d = _chain(96, m_lo=25, m_hi=47, seed=21)
+    r = (d["nvec"] + 15) // 16
+    assert set(np.unique(r)) == {4, 5, 6}
+    print("[case] chain uneven n 50..94 wg mixed", flush=True)
+    ref = _oracle(d)
+    mx = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(precision="mixed"), reltol=d["reltol"], abstol=d["abstol"],
+                            nvec=d["nvec"])
+    assert (mx["retcode"] == 1).all()
+    err = np.abs(mx["u_final"] - ref["u_final"]) / (d["abstol"] + d["reltol"] * np.abs(ref["u_final"]))
+    print(f"mixed scaled err p99 {np.quantile(err.max(axis=0), 0.99):.3e} max {err.max():.3e}")
+    assert np.quantile(err.max(axis=0), 0.99) <= 10.0
+
+
 def test_wg_kernel_robertson_matches_lane_kernel():
     """n = 3 through the workgroup kernel (nvec given) equals the lane kernel bit for bit."""
     d = P.robertson_ensemble(N=128)
