@@ -235,7 +235,8 @@ def bench_events(args):
         "step_attempts": attempts, "condition_calls": conds,
         "success_fraction": float((out["retcode"].cpu().numpy() == 1).mean()),
         "roofline": {"bound": "valu-fp64", "achieved": flops / (kms / 1e3) / 1e12, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": flops / (kms / 1e3) / 1e12 / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": flops / (kms / 1e3) / 1e12 / PEAK_FP64_TFLOPS,
+                     "traffic": _traffic("odemi_tsit5_persist_f64@balls_1m") if N == 1048576 else None,
                      "kernel": "odemi_tsit5_persist_f64", "kernel_ms": kms},
     }
     om, msm, kmm = _timed(lambda: run(odemi.EnsembleMI355X(precision="mixed")), max(1, args.steps), 1, torch)
