@@ -371,11 +371,10 @@ __device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, i
 // The step is one fma per element with no selects: column k is published with
 // 0 in the pivot row's slot (so the pivot row's update is a no-op), the
 // threads holding column k update it from zero with d in place of r[k] d (so
-// it becomes -c d), and the 1 that belongs at
-// (p_k, k) is added at the end, before the scales.  That entry is only ever
-// updated afterwards, never read as a pivot column or pivot row entry, so
-// adding it last is exact algebra.  The pivot itself is read from the staged
-// pivot row.
+// it becomes -c d), and the 1 that belongs at (p_k, k) is added at the end,
+// before the scales.  That entry is only ever updated afterwards, never read
+// as a pivot column or pivot row entry, so adding it last is exact algebra.
+// The pivot itself is read from the staged pivot row.
 //
 // One barrier per step.  Before the barrier that opens step k the owners
 // publish column k, column k+1 and (all R row slots of the thread row holding)
