@@ -128,6 +128,9 @@ int odemi_abi_version(void);
  *   #define ODEMI_HAS_ISOUT 1 with
  *   template<typename T, typename Q> __device__ bool odemi_isout(const T* u, const Q* p, T t);
  *     <- the `isoutofdomain` solve option (integrator_utils.jl:612-614)
+ * Optional, workgroup kernels (ODEMI_COMPILE_WG): #define ODEMI_HAS_ISOUT_WG 1 with
+ *   template<typename T> __device__ bool odemi_isout_wg(const T* u, const T* p, T t, int n);
+ *     <- the same option, one case per state size n of the batch
  *   #define ODEMI_NDCB k with template<int K> struct OdemiDcb, K = 0..k-1, holding
  *     static constexpr int save0, save1, term;
  *     template<typename T, typename Q> static __device__ bool cond(const T* u, const Q* p, T t);

@@ -755,6 +755,10 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
                 c.dt = c.dtpropose;
                 apply_step_tstop_clip(c);
             } else {
+#ifdef ODEMI_HAS_ISOUT_WG
+                if (c.isout) c.dt = c.dt * kc.qmin;   // handle_step_rejection! (integrator_utils.jl:135-136)
+                else
+#endif
                 reject_controller(c, kc);
             }
         }
@@ -885,7 +889,13 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
         // ---- loopfooter! ----
         const double ttmp = c.t + c.dt;
         const double q = stepsize_controller(c, kc);
-        c.accept_step = (c.EEst <= 1.0);
+#ifdef ODEMI_HAS_ISOUT_WG
+        // integrator_utils.jl:612-614, one thread on the new state in LDS
+        if (tid == 0) red[25] = odemi_isout_wg<double>(u, pp, ttmp, n) ? 1.0 : 0.0;
+        __syncthreads();
+        c.isout = (red[25] != 0.0) ? 1 : 0;
+#endif
+        c.accept_step = !c.isout && (c.EEst <= 1.0);
         if (c.accept_step) {
             c.naccept++;
             c.tprev = c.t;

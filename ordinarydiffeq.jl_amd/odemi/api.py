@@ -333,8 +333,8 @@ def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, i
     Returns (rhs, traced, seconds)."""
     wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
     cbs = _callback_list(callback)
-    if wg and (isoutofdomain is not None or cbs):
-        raise ValueError("isoutofdomain and callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
+    if wg and cbs:
+        raise ValueError("callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
                          f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
     if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
         raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
@@ -345,7 +345,8 @@ def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, i
     tr = _TRACES.get(key)
     if tr is None:
         if wg:
-            tr = trace_problem_wg(fun.f, list(ns) if ns is not None else [n], np_, jac=fun.jac, tgrad=fun.tgrad)
+            tr = trace_problem_wg(fun.f, list(ns) if ns is not None else [n], np_, jac=fun.jac, tgrad=fun.tgrad,
+                                  isout=isoutofdomain)
             tr.src = _LANE_STUBS + tr.src
             tr.has_user_tgrad = fun.tgrad is not None
         else:

@@ -864,11 +864,13 @@ class TracedProblemWG:
         self.flops_f, self.flops_j, self.nnz = flops_f, flops_j, nnz
 
 
-def trace_problem_wg(f, ns, np_, jac=None, tgrad=None):
-    """Workgroup forms for every state size in `ns` (distinct, sorted)."""
+def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None):
+    """Workgroup forms for every state size in `ns` (distinct, sorted).
+    isout(u, p, t): the isoutofdomain predicate, traced per size into
+    odemi_isout_wg (one thread evaluates it on the state in LDS)."""
     global _CUR
     ns = sorted(set(int(n) for n in ns))
-    fcases, jcases, tcases, vcases = [], [], [], []
+    fcases, jcases, tcases, vcases, icases = [], [], [], [], []
     flops_f, flops_j, nnz = {}, {}, {}
     for n in ns:
         g = _Graph()
@@ -921,6 +923,12 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None):
             jcases.append(f"    case {n}: {{\n{_emit_runs(jr, lambda d: f'J[o * ld + o + ({d})]')}\n        break;\n    }}")
             tcases.append(f"    case {n}: {{\n{_emit_runs(tr, lambda d: 'dT[o]')}\n        break;\n    }}")
             vcases.append(f"    case {n}: {{\n{_emit_runs(vr, lambda d: 'Jv[o]')}\n        break;\n    }}")
+            if isout is not None:
+                r = isout(u, p, t)
+                r = r if isinstance(r, Sym) else _lift(1.0 if r else 0.0)
+                icases.append(f"    case {n}: {{\n        T r_[1];\n"
+                              + _emit_body([(0, r)], "r_", indent="        ", pname="T(p[{}])")
+                              + "\n        return r_[0] != T(0);\n    }")
         finally:
             _CUR = None
     hdr = "template <typename T>\n__device__ __noinline__ void "
@@ -941,4 +949,8 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None):
         + ") {\n    (void)u; (void)v; (void)p; (void)t;\n    switch (n) {\n"
         + "\n".join(vcases) + "\n    default: break;\n    }\n}",
     ]) + "\n"
+    if isout is not None:
+        src += ("#define ODEMI_HAS_ISOUT_WG 1\ntemplate <typename T>\n__device__ __noinline__ bool odemi_isout_wg("
+                "const T* __restrict__ u, const T* __restrict__ p, T t, int n) {\n    (void)u; (void)p; (void)t;\n"
+                "    switch (n) {\n" + "\n".join(icases) + "\n    default: return false;\n    }\n}\n")
     return TracedProblemWG(ns, np_, src, flops_f, flops_j, nnz)

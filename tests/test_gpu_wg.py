@@ -89,6 +89,35 @@ def test_chain_mid_n_mixed_gauss_jordan_tiles_4_to_6():
     assert np.quantile(err.max(axis=0), 0.99) <= 10.0
 
 
+@pytest.mark.parametrize("prec,width", [("fp64", 0.01), ("mixed", 1e-4)])
+def test_chain_isoutofdomain_workgroup(prec, width):
+    """isoutofdomain in the workgroup kernels: a step whose end time falls in
+    (0.3, 0.3 + width) is rejected with dt *= qmin (handle_step_rejection!).
+    The 0.01 window also traps most trajectories below 0.3 until dt underflows
+    (the restatement's return codes and counts, bit for bit, FP64); the 1e-4
+    window only adds rejections (mixed, within tolerance)."""
+    d = _chain(128)
+    hi = 0.3 + width
+    isout = lambda u, p, t: (t > 0.3) & (t < hi)
+    print(f"[case] chain isoutofdomain wg {prec} width {width}", flush=True)
+    # the predicate reads t only: a (1, 1) view stays inside every trajectory's state
+    ref = _oracle(d, isoutofdomain=lambda u, p, t: 0.3 < t < hi, isout_shape=(1, 1))
+    base = _oracle(d)
+    assert (ref["nreject"] > base["nreject"]).any()   # the predicate does reject steps
+    ea = odemi.EnsembleMI355X(precision=prec)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], ea, reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"], isoutofdomain=isout)
+    assert (out["retcode"] == ref["retcode"]).all()
+    if prec == "fp64":
+        assert (out["stats"][1] == ref["naccept"]).all() and (out["stats"][2] == ref["nreject"]).all()
+        assert np.array_equal(out["u_final"], ref["u_final"], equal_nan=True)
+    else:
+        assert (ref["retcode"] == 1).all()
+        err = np.abs(out["u_final"] - ref["u_final"]) / (d["abstol"] + d["reltol"] * np.abs(ref["u_final"]))
+        print(f"mixed scaled err p99 {np.quantile(err.max(axis=0), 0.99):.3e}")
+        assert np.quantile(err.max(axis=0), 0.99) <= 10.0
+
+
 def test_wg_kernel_robertson_matches_lane_kernel():
     """n = 3 through the workgroup kernel (nvec given) equals the lane kernel bit for bit."""
     d = P.robertson_ensemble(N=128)
