@@ -3,9 +3,13 @@
 Workload (N=1): 1,048,576 Lorenz trajectories, Tsit5, rho~U[0,28], sigma~U[8,12],
 beta=8/3, u0~U[-1,1]^3, t_end~U[1,10], reltol=abstol=1e-6, final state only.
 A "step" is one full ensemble solve of that batch with inputs resident in HBM.
-`value` is the batched persistent mixed-precision engine; the plain FP64
-one-kernel-per-step engine is timed beside it (`plain_fp64`), and the CPU
-restatement runs on a bounded sample on the host cores (`cpu_baseline`).
+`value` is the batched persistent FP64 engine (the reference's arithmetic,
+bit-identical to the CPU restatement); the persistent mixed-precision engine
+(`persistent_mixed`) and the plain FP64 one-kernel-per-attempt engine
+(`plain_fp64`) are timed beside it, the same FP64 solve from host buffers
+(`pcie_inclusive_fp64`), the CPU restatement on the host cores
+(`cpu_baseline`), a Robertson Rodas5P leg (`robertson`) and a short open-loop
+serving window with p50/p99 latency (`serving`).
 """
 import argparse
 import json
@@ -83,111 +87,161 @@ def _accuracy(prob_id, d, outs, sample, nvec=None):
 
 
 def _timed(run, steps, warmup, torch):
+    """W untimed warm-up solves, then exactly K solves timed as one region
+    bracketed by device synchronisation; returns (last output, ms per solve,
+    median kernel ms of the K solves)."""
     out = None
     for _ in range(warmup):
         out = run()
     torch.cuda.synchronize()
-    times, kms = [], []
+    kms = []
+    t_a = time.perf_counter()
     for s in range(steps):
-        torch.cuda.synchronize()
-        t_a = time.perf_counter()
         out = run()
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t_a)
         kms.append(out["kernel_ms"])
-        print(f"[bench] step {s} {times[-1]*1e3:.2f} ms (kernel {kms[-1]:.2f} ms)", file=sys.stderr, flush=True)
-    return out, float(np.median(times)) * 1e3, float(np.median(kms))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t_a) * 1e3 / max(steps, 1)
+    print(f"[bench] {steps} solves: {ms:.3f} ms per solve (kernel median {float(np.median(kms)):.3f} ms)",
+          file=sys.stderr, flush=True)
+    return out, ms, float(np.median(kms))
 
 
-def bench_rodas(args, workload):
-    """Robertson (262,144 trajectories, lane kernel) or the uneven oscillator-chain
-    batch (16,384 trajectories, n = 8..128, workgroup kernel): FP64 LU engine vs
-    FP32 LU + FP64 refinement engine, CPU restatement on a bounded sample."""
-    import torch
-    import odemi
+# host cores of one GPU's share of the box (the CPU baseline's thread pool)
+CPU_SHARE = 16
+
+
+def _host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def _cpu_baseline(prob_id, d, alg, sample, sample_one, nvec=None, chunk=64):
+    """The CPU restatement (oracle/, FP64, one trajectory per task with dynamic
+    chunking like EnsembleThreads) on the host: `sample` trajectories on the
+    GPU's CPU share (at most 16 threads), and `sample_one` on one thread."""
     import oracle as O
+    N = d["u0"].shape[1]
+    threads = max(1, min(len(os.sched_getaffinity(0)), CPU_SHARE))
+    kw = dict(alg=alg, reltol=d["reltol"], abstol=d["abstol"], save_start=False)
+    S = min(sample, N)
+    print(f"[bench] cpu baseline {S} trajectories on {threads} threads", file=sys.stderr, flush=True)
+    r = O.ensemble_builtin(prob_id, d["u0"][:, :S], d["p"][:, :S], d["t0"][:S], d["tf"][:S],
+                           nvec=None if nvec is None else nvec[:S], threads=threads, chunk=chunk, **kw)
+    S1 = min(sample_one, N)
+    print(f"[bench] cpu baseline {S1} trajectories on 1 thread", file=sys.stderr, flush=True)
+    r1 = O.ensemble_builtin(prob_id, d["u0"][:, :S1], d["p"][:, :S1], d["t0"][:S1], d["tf"][:S1],
+                            nvec=None if nvec is None else nvec[:S1], threads=1, chunk=chunk, **kw)
+    return {"value": S / r["seconds"], "unit": "trajectories/s", "cores": threads, "kind": "port",
+            "sample": f"first {S} trajectories of the same batch, {r['seconds']:.2f} s wall on {threads} threads",
+            "one_thread": {"value": S1 / r1["seconds"], "unit": "trajectories/s",
+                           "sample": f"first {S1} trajectories, {r1['seconds']:.2f} s wall"},
+            "host": _host_info()}
+
+
+def _rodas_data(workload, N):
     from odemi import problems as P
-    dev = torch.device("cuda:0")
+    import oracle as O
     if workload == "robertson":
-        N = args.N or 262144
+        N = N or 262144
         d = P.robertson_ensemble(N=N)
-        fun, pid = P.ROBERTSON, O.PROB_ROBERTSON
-        nvec = None
-        desc = ("Robertson ensemble, 262,144 trajectories, Rodas5P, k=(0.04,3e7,1e4)*LogU[0.5,2], u0=(1,0,0), "
-                "tspan (0,1e5), reltol 1e-6, abstol 1e-10, analytic 3x3 J")
-    else:
-        N = args.N or 16384
-        d = P.chain_ensemble(N=N)
-        fun, pid = P.CHAIN, O.PROB_CHAIN
-        nvec = d["nvec"]
-        desc = ("Uneven oscillator chains, 16,384 trajectories, n=2m, m~U{4..64}, k~U[1,1e4], c~U[0.01,1], "
-                "alpha~U[0,1], x0~N(0,0.1^2), tspan (0,10), Rodas5P, reltol=abstol=1e-6, one launch")
-    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()
-    u0, p, t0, tf = T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"])
-    kw = dict(reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=True, nvec=nvec)
-    mixed = odemi.EnsembleMI355X(precision="mixed", refine_sweeps=args.refine_sweeps)
-    fp64 = odemi.EnsembleMI355X(precision="fp64")
-    run = lambda ea: odemi.solve_arrays(fun, odemi.Rodas5P(), u0, p, t0, tf, ea, **kw)
-    # hipRTC compile (cached on disk) stays outside every timed region
-    odemi.compile_problem(fun, d["u0"].shape[0], d["p"].shape[0], algs=("rodas5p",),
-                          ns=None if nvec is None else sorted(set(int(x) for x in nvec)))
-    out, ms, kms = _timed(lambda: run(mixed), args.steps, args.warmup, torch)
+        return N, d, P.ROBERTSON, O.PROB_ROBERTSON, None, (
+            f"Robertson ensemble, {N:,} trajectories, Rodas5P, k=(0.04,3e7,1e4)*LogU[0.5,2], u0=(1,0,0), "
+            "tspan (0,1e5), reltol 1e-6, abstol 1e-10, analytic 3x3 J")
+    N = N or 16384
+    d = P.chain_ensemble(N=N)
+    return N, d, P.CHAIN, O.PROB_CHAIN, d["nvec"], (
+        f"Uneven oscillator chains, {N:,} trajectories, n=2m, m~U{{4..64}}, k~U[1,1e4], c~U[0.01,1], "
+        "alpha~U[0,1], x0~N(0,0.1^2), tspan (0,10), Rodas5P, reltol=abstol=1e-6, one launch")
+
+
+def _rodas_roofline(out, nvec, kms, kernel, peak, bound, traffic=None):
     st = out["stats"].cpu().numpy()
     att = (st[1] + st[2]).astype(np.float64)
     tr = out["traced"]
     if nvec is None:
         fl = att * rodas_flops_per_attempt(tr.flops_f, tr.flops_j, 3)
-        nrow = 3
     else:
         fl = np.array([att[j] * rodas_flops_per_attempt(tr.flops_f[int(nvec[j])], tr.flops_j[int(nvec[j])],
-                                                         int(nvec[j])) for j in range(N)])
-        nrow = int(np.max(nvec))
-    flops = float(fl.sum())
+                                                         int(nvec[j])) for j in range(len(att))])
+    ach = float(fl.sum()) / (kms / 1e3) / 1e12
+    return int(att.sum()), {"bound": bound, "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                            "traffic": traffic, "kernel": kernel, "kernel_ms": kms}
+
+
+def rodas_legs(args, workload, steps, warmup, cpu=True, plain=True):
+    """Rodas5P ensemble (Robertson: lane kernels; chains: uneven n in one
+    workgroup-kernel launch): the FP64 engine (the reference's arithmetic,
+    bit-identical to the restatement) as `value`, the FP32 LU / Gauss-Jordan +
+    FP64 refinement engine and the plain FP64 one-launch-per-attempt engine
+    beside it, accuracy against a reltol-1e-12 reference, and the CPU
+    restatement on a bounded sample."""
+    import torch
+    import odemi
+    import oracle as O
+    dev = torch.device("cuda:0")
+    N, d, fun, pid, nvec, desc = _rodas_data(workload, args.N if workload == args.workload else 0)
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    u0, p, t0, tf = T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"])
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=True, nvec=nvec)
+    mixed = odemi.EnsembleMI355X(precision="mixed", refine_sweeps=args.refine_sweeps)
+    fp64 = odemi.EnsembleMI355X(precision="fp64")
+    run = lambda ea: odemi.solve_arrays(fun, odemi.Rodas5P(), u0, p, t0, tf, ea, **kw)  # noqa: E731
+    # hipRTC compile (cached on disk) stays outside every timed region
+    odemi.compile_problem(fun, d["u0"].shape[0], d["p"].shape[0], algs=("rodas5p",),
+                          ns=None if nvec is None else sorted(set(int(x) for x in nvec)))
+    lane = nvec is None
+    o64, ms64, kms64 = _timed(lambda: run(fp64), steps, warmup, torch)
+    att64, roof64 = _rodas_roofline(o64, nvec, kms64, "odemi_rodas_persist_f64" if lane else "odemi_rodas_wg_f64",
+                                    PEAK_FP64_TFLOPS, "valu-fp64",
+                                    _traffic("odemi_rodas_persist_f64@robertson_262k") if lane and N == 262144
+                                    else None)
     res = {
-        "metric": "ensemble trajectories/s at reltol 1e-6 (Rodas5P), FP32 LU + FP64 refinement engine",
-        "value": N / (ms / 1e3), "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 LU + f64 refinement, f64 state/RHS/controller", "data": "synthetic (seeded PCG64)",
+        "metric": "ensemble trajectories/s at reltol 1e-6 (Rodas5P), batched persistent FP64 engine",
+        "value": N / (ms64 / 1e3), "unit": "trajectories/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded PCG64)",
         "config": {"workload": desc, "trajectories": N,
-                   "parallelism": "one MI355X, " + ("lane per trajectory" if nvec is None else
-                                                    "workgroup per trajectory") + ", work queue"},
-        "step_attempts": int(att.sum()), "success_fraction": float((out["retcode"].cpu().numpy() == 1).mean()),
-        "roofline": {"bound": "valu-fp64" if nvec is None else "lds/valu-fp32", "achieved": flops / (kms / 1e3) / 1e12,
-                     "peak": PEAK_FP64_TFLOPS if nvec is None else PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": flops / (kms / 1e3) / 1e12 / (PEAK_FP64_TFLOPS if nvec is None else PEAK_FP32_TFLOPS),
-                     "traffic": (_traffic("odemi_rodas_persist_mixed@robertson_262k") if nvec is None and N == 262144
-                                 else _traffic("odemi_rodas_wg_mixed@chain_2048") if nvec is not None and N == 2048
-                                 else _traffic("odemi_rodas_wg_mixed@chain_16k") if nvec is not None and N == 16384
-                                 else None), "kernel_ms": kms,
-                     "kernel": "odemi_rodas_persist_mixed" if nvec is None else "odemi_rodas_wg_mixed"},
+                   "parallelism": "one MI355X, " + ("lane per trajectory" if lane else "workgroup per trajectory")
+                                  + ", work queue"},
+        "step_attempts": att64, "success_fraction": float((o64["retcode"].cpu().numpy() == 1).mean()),
+        "roofline": roof64,
     }
-    res["refine_sweeps"] = args.refine_sweeps
-    um = out["u_final"].cpu().numpy()
-    engines = {"mixed": um}
-    if not args.no_fp64:
-        o64, ms64, kms64 = _timed(lambda: run(fp64), 1, 0, torch)
-        res["persistent_fp64"] = {"trajectories_per_s": N / (ms64 / 1e3), "ms": ms64, "kernel_ms": kms64}
-        u64 = o64["u_final"].cpu().numpy()
-        err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
-        res["mixed_vs_fp64_scaled_err"] = {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
-                                           "max": float(err.max())}
-        engines["fp64"] = u64
-    res["error_vs_reference"] = _accuracy(pid, d, engines, 8 if workload == "robertson" else 64, nvec=nvec)
-    if workload == "robertson" and not args.no_plain:
-        plain = odemi.EnsembleMI355X(engine="plain")
-        op, msp, _ = _timed(lambda: run(plain), 1, 0, torch)
+    om, msm, kmsm = _timed(lambda: run(mixed), max(1, steps), 1 if warmup else 0, torch)
+    attm, roofm = _rodas_roofline(om, nvec, kmsm, "odemi_rodas_persist_mixed" if lane else "odemi_rodas_wg_mixed",
+                                  PEAK_FP64_TFLOPS if lane else PEAK_FP32_TFLOPS,
+                                  "valu-fp64" if lane else "lds/valu-fp32",
+                                  (_traffic("odemi_rodas_persist_mixed@robertson_262k") if lane and N == 262144
+                                   else _traffic("odemi_rodas_wg_mixed@chain_16k") if not lane and N == 16384
+                                   else None))
+    u64, um = o64["u_final"].cpu().numpy(), om["u_final"].cpu().numpy()
+    err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
+    res["persistent_mixed"] = {
+        "trajectories_per_s": N / (msm / 1e3), "ms": msm, "kernel_ms": kmsm, "step_attempts": attm,
+        "dtype": "f32 LU (lane) / f32 Gauss-Jordan inverse (workgroup) + f64 refinement, f64 state/RHS/controller",
+        "refine_sweeps": args.refine_sweeps, "roofline": roofm,
+        "scaled_err_vs_fp64": {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)),
+                               "max": float(err.max())}}
+    res["error_vs_reference"] = _accuracy(pid, d, {"fp64": u64, "mixed": um}, 8 if lane else 64, nvec=nvec)
+    if plain and lane:
+        op, msp, _ = _timed(lambda: run(odemi.EnsembleMI355X(engine="plain")), 1, 0, torch)
         res["plain_fp64"] = {"trajectories_per_s": N / (msp / 1e3), "ms": msp, "launches": int(op["launches"])}
-        res["batched_over_plain"] = (N / (ms / 1e3)) / (N / (msp / 1e3))
-    if not args.no_cpu:
-        threads = min(len(os.sched_getaffinity(0)), 16)
-        S = min(args.cpu_sample if workload == "robertson" else 256, N)
-        print(f"[bench] cpu baseline {S} trajectories on {threads} threads", file=sys.stderr, flush=True)
-        r = O.ensemble_builtin(pid, d["u0"][:, :S], d["p"][:, :S], d["t0"][:S], d["tf"][:S],
-                               nvec=None if nvec is None else nvec[:S], alg=O.RODAS5P, reltol=d["reltol"],
-                               abstol=d["abstol"], save_start=False, threads=threads, chunk=1 if nvec is not None else 64)
-        res["cpu_baseline"] = {"value": S / r["seconds"], "unit": "trajectories/s", "cores": threads, "kind": "port",
-                               "sample": f"first {S} trajectories of the same batch, {r['seconds']:.2f} s wall"}
-    print(json.dumps(res))
+        res["batched_over_plain"] = {"fp64": msp / ms64, "mixed": msp / msm}
+    if cpu and not args.no_cpu:
+        res["cpu_baseline"] = _cpu_baseline(pid, d, O.RODAS5P, 262144 if lane else 256, 16384 if lane else 32,
+                                            nvec=nvec, chunk=64 if lane else 1)
+    return res
+
+
+def bench_rodas(args, workload):
+    print(json.dumps(rodas_legs(args, workload, args.steps, args.warmup, plain=not args.no_plain)))
 
 
 def bench_events(args):
@@ -268,7 +322,7 @@ def bench_events(args):
     print(json.dumps(res))
 
 
-def bench_serving(args):
+def serving_leg(args, dur):
     """Open-loop Poisson arrivals of ensemble requests of ragged size, served by
     continuous batching (one EnsembleServer per instance type, each with its own
     stream and worker thread).  Latency = completion - scheduled arrival.
@@ -278,14 +332,9 @@ def bench_serving(args):
     `chain_rate` req/s of uneven oscillator chains n = 8..128 (1..chain_max
     trajectories, workgroup kernels), whose per-trajectory cost is ~10^4 times
     a Lorenz trajectory's."""
-    # one hardware queue per server stream (4 servers + the context stream), so a
-    # long workgroup tick on the chain server's stream never sits in front of a
-    # lane server's tick in a shared in-order queue
-    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
-        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     import odemi
     from odemi import problems as P
-    rate, dur = args.rate, args.duration
+    rate = args.rate
     rng = np.random.Generator(np.random.PCG64(7))
     pools = {
         "lorenz_small": (P.lorenz_small(N=65536), P.LORENZ, odemi.Tsit5(), "fp64"),
@@ -383,7 +432,94 @@ def bench_serving(args):
         "servers": info,
         "roofline": None,
     }
-    print(json.dumps(res))
+    return res
+
+
+def bench_serving(args):
+    print(json.dumps(serving_leg(args, args.duration)))
+
+
+def bench_lorenz(args):
+    """1,048,576 Lorenz trajectories (Tsit5, reltol = abstol = 1e-6, uneven
+    t_end): `value` is the batched persistent FP64 engine (the reference's
+    arithmetic; bit-identical to the CPU restatement), the mixed-precision
+    persistent engine and the plain FP64 one-launch-per-attempt engine are
+    timed beside it on the same device-resident batch."""
+    import torch
+    import odemi
+    import oracle as O
+    from odemi import problems as P
+    N = args.N or 1048576
+    dev = torch.device("cuda:0")
+    d = P.lorenz_large(N=N)
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    u0, p, t0, tf = T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"])
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=True)
+    fp64 = odemi.EnsembleMI355X(engine="persistent", precision="fp64")
+    mixed = odemi.EnsembleMI355X(engine="persistent", precision="mixed")
+    plain = odemi.EnsembleMI355X(engine="plain", precision="fp64")
+
+    def run(ea, *arrs):
+        return odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), *(arrs or (u0, p, t0, tf)), ea, **kw)
+
+    odemi.compile_problem(P.LORENZ, 3, 3, algs=("tsit5",))   # hipRTC compile outside the timed region
+
+    def roof(out, kms, kernel, peak, bound, traffic):
+        st = out["stats"].cpu().numpy()
+        att = int(st[1].sum() + st[2].sum())
+        F_f = out["traced"].flops_f
+        fl = att * tsit5_flops_per_attempt(F_f, 3) + N * tsit5_flops_init(F_f, 3)
+        ach = fl / (kms / 1e3) / 1e12
+        return att, {"bound": bound, "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                     "traffic": traffic, "kernel": kernel, "kernel_ms": kms,
+                     "flops_per_attempt": tsit5_flops_per_attempt(F_f, 3)}
+
+    o64, ms64, kms64 = _timed(lambda: run(fp64), args.steps, args.warmup, torch)
+    att64, roof64 = roof(o64, kms64, "odemi_tsit5_persist_f64", PEAK_FP64_TFLOPS, "valu-fp64",
+                         _traffic("odemi_tsit5_persist_f64@lorenz_1m") if N == 1048576 else None)
+    res = {
+        "metric": "ensemble trajectories/s at reltol 1e-6 (Tsit5, Lorenz), batched persistent FP64 engine "
+                  "vs plain FP64 GPU engine",
+        "value": N / (ms64 / 1e3), "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded PCG64)",
+        "config": {"workload": "Lorenz ensemble, 1,048,576 trajectories, Tsit5, rho~U[0,28], sigma~U[8,12], "
+                               "u0~U[-1,1]^3, t_end~U[1,10], reltol=abstol=1e-6, final state only",
+                   "trajectories": N, "parallelism": "one MI355X, lane per trajectory, work queue"},
+        "step_attempts": att64, "success_fraction": float((o64["retcode"].cpu().numpy() == 1).mean()),
+        "roofline": roof64,
+    }
+    om, msm, kmsm = _timed(lambda: run(mixed), args.steps, 1, torch)
+    attm, roofm = roof(om, kmsm, "odemi_tsit5_persist_mixed", PEAK_FP32_TFLOPS, "valu-fp32",
+                       _traffic("odemi_tsit5_persist_mixed@lorenz_1m") if N == 1048576 else None)
+    u64, um = o64["u_final"].cpu().numpy(), om["u_final"].cpu().numpy()
+    err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
+    res["persistent_mixed"] = {
+        "trajectories_per_s": N / (msm / 1e3), "ms": msm, "kernel_ms": kmsm, "step_attempts": attm,
+        "dtype": "f32 stages and stage sums + f64 state/t/error norm/controller", "roofline": roofm,
+        "scaled_err_vs_fp64": {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99))}}
+    # FP64 engine on the first 65,536 trajectories against the CPU restatement, bit for bit
+    S = min(65536, N)
+    ref = O.ensemble_builtin(O.PROB_LORENZ, d["u0"][:, :S], d["p"][:, :S], d["t0"][:S], d["tf"][:S], alg=O.TSIT5,
+                             reltol=d["reltol"], abstol=d["abstol"], save_start=False,
+                             threads=max(1, min(len(os.sched_getaffinity(0)), CPU_SHARE)))
+    st64 = o64["stats"].cpu().numpy()
+    res["fp64_vs_restatement"] = {
+        "sample": S, "final_states_identical": float(np.mean(np.all(u64[:, :S] == ref["u_final"], axis=0))),
+        "step_counts_identical": float(np.mean((st64[1, :S] == ref["naccept"]) & (st64[2, :S] == ref["nreject"])))}
+    res["error_vs_reference"] = _accuracy(O.PROB_LORENZ, d, {"fp64": u64, "mixed": um}, 1024)
+    # the same FP64 solve from host arrays: H2D of the inputs through D2H of the outputs
+    _, mspc, _ = _timed(lambda: run(fp64, d["u0"], d["p"], d["t0"], d["tf"]), 2, 1, torch)
+    res["pcie_inclusive_fp64"] = {"trajectories_per_s": N / (mspc / 1e3), "ms": mspc,
+                                  "note": "host (pageable) inputs and outputs, H2D + solve + D2H"}
+    if not args.no_plain:
+        print("[bench] plain fp64 engine", file=sys.stderr, flush=True)
+        op, msp, _ = _timed(lambda: run(plain), 1, 0, torch)
+        res["plain_fp64"] = {"trajectories_per_s": N / (msp / 1e3), "ms": msp, "launches": int(op["launches"])}
+        res["batched_over_plain"] = {"fp64": msp / ms64, "mixed": msp / msm}
+    if not args.no_cpu:
+        res["cpu_baseline"] = _cpu_baseline(O.PROB_LORENZ, d, O.TSIT5, min(args.cpu_sample, N), 131072)
+    return res
 
 
 def main():
@@ -395,13 +531,15 @@ def main():
     ap.add_argument("--workload", default="lorenz", choices=["lorenz", "robertson", "chain", "serving", "events"])
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: requests/s")
     ap.add_argument("--duration", type=float, default=60.0, help="serving: arrival window, s")
+    ap.add_argument("--serve-window", type=float, default=5.0,
+                    help="default line: arrival window of its serving leg, s (0: no serving leg)")
     ap.add_argument("--max-request", type=int, default=4096, help="serving: trajectories per request, max")
     ap.add_argument("--tick-attempts", type=int, default=32, help="serving: step attempts per tick")
     ap.add_argument("--chain-rate", type=float, default=20.0, help="serving: chain requests/s (0: none)")
     ap.add_argument("--chain-max", type=int, default=16, help="serving: chains per chain request, max")
     ap.add_argument("--hbm-budget-gb", type=int, default=8, help="serving: slot-pool budget, GiB")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true", help="Rodas5P workloads: skip the FP64 engine")
+    ap.add_argument("--no-robertson", action="store_true", help="default line: skip the Robertson leg")
     ap.add_argument("--refine-sweeps", type=int, default=1, help="mixed Rodas5P: FP64 refinement sweeps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1048576)
@@ -410,103 +548,29 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    # one hardware queue per server stream (4 servers + the context stream), so
+    # a long workgroup tick on the chain server's stream never sits in front of
+    # a lane server's tick in a shared in-order queue; set before HIP starts
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     if args.workload == "serving":
         return bench_serving(args)
     if args.workload == "events":
         return bench_events(args)
     if args.workload != "lorenz":
         return bench_rodas(args, args.workload)
-    args.N = args.N or 1048576
-    import torch
-    import odemi
-    from odemi import problems as P
-
-    dev = torch.device("cuda:0")
-    d = P.lorenz_large(N=args.N)
-    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()
-    u0, p, t0, tf = T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"])
-    kw = dict(reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=True)
-    batched = odemi.EnsembleMI355X(engine="persistent", precision="mixed")
-    fp64p = odemi.EnsembleMI355X(engine="persistent", precision="fp64")
-    plain = odemi.EnsembleMI355X(engine="plain", precision="fp64")
-
-    def run(ea):
-        return odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), u0, p, t0, tf, ea, **kw)
-
-    odemi.compile_problem(P.LORENZ, 3, 3, algs=("tsit5",))   # hipRTC compile outside the timed region
-
-    print(f"[bench] warmup x{args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        out = run(batched)
-    torch.cuda.synchronize()
-    times, kms = [], []
-    for s in range(args.steps):
-        torch.cuda.synchronize()
-        t_a = time.perf_counter()
-        out = run(batched)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t_a)
-        kms.append(out["kernel_ms"])
-        print(f"[bench] step {s} {times[-1]*1e3:.2f} ms (kernel {kms[-1]:.2f} ms)", file=sys.stderr, flush=True)
-    stats = out["stats"].cpu().numpy()
-    attempts = int(stats[1].sum() + stats[2].sum())
-    tr = out["traced"]
-    F_f = tr.flops_f
-    flops = attempts * tsit5_flops_per_attempt(F_f, 3) + args.N * tsit5_flops_init(F_f, 3)
-    ms_step = float(np.median(times)) * 1e3
-    kernel_ms = float(np.median(kms))
-    value = args.N / (ms_step / 1e3)
-    ok = float((out["retcode"].cpu().numpy() == 1).mean())
-    res = {
-        "metric": "ensemble trajectories/s at reltol 1e-6 (Tsit5, Lorenz), batched persistent mixed-precision engine",
-        "value": value, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 stages and stage sums + f64 state/t/error norm/controller", "data": "synthetic (seeded PCG64)",
-        "config": {"workload": "Lorenz ensemble, 1,048,576 trajectories, Tsit5, rho~U[0,28], sigma~U[8,12], "
-                               "u0~U[-1,1]^3, t_end~U[1,10], reltol=abstol=1e-6, final state only",
-                   "trajectories": args.N, "parallelism": "one MI355X, lane per trajectory, work queue"},
-        "step_attempts": attempts, "success_fraction": ok,
-        "roofline": {"bound": "valu-fp32", "achieved": flops / (kernel_ms / 1e3) / 1e12,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": flops / (kernel_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS,
-                     "traffic": _traffic("odemi_tsit5_persist_mixed@lorenz_1m") if args.N == 1048576 else None,
-                     "kernel": "odemi_tsit5_persist_mixed", "kernel_ms": kernel_ms,
-                     "flops_per_attempt": tsit5_flops_per_attempt(F_f, 3)},
-    }
-    # FP64 persistent engine (same batch)
-    o64 = run(fp64p)
-    torch.cuda.synchronize()
-    t_a = time.perf_counter()
-    o64 = run(fp64p)
-    torch.cuda.synchronize()
-    t64 = time.perf_counter() - t_a
-    res["persistent_fp64"] = {"trajectories_per_s": args.N / t64, "ms": t64 * 1e3, "kernel_ms": o64["kernel_ms"]}
-    uref = o64["u_final"].cpu().numpy()
-    um = out["u_final"].cpu().numpy()
-    relm = (np.abs(um - uref) / (1e-6 + 1e-6 * np.abs(uref))).max(axis=0)
-    res["mixed_vs_fp64_scaled_err"] = {"median": float(np.median(relm)), "p99": float(np.quantile(relm, 0.99))}
-    import oracle as O
-    res["error_vs_reference"] = _accuracy(O.PROB_LORENZ, d, {"mixed": um, "fp64": uref}, 256)
-    if not args.no_plain:
-        print("[bench] plain fp64 engine", file=sys.stderr, flush=True)
-        t_a = time.perf_counter()
-        op = run(plain)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - t_a
-        res["plain_fp64"] = {"trajectories_per_s": args.N / tp, "ms": tp * 1e3, "launches": int(op["launches"])}
-        res["batched_over_plain"] = (args.N / (ms_step / 1e3)) / (args.N / tp)
-    if not args.no_cpu:
-        import oracle as O
-        ncpu = len(os.sched_getaffinity(0))
-        threads = min(ncpu, 16)
-        S = min(args.cpu_sample, args.N)
-        print(f"[bench] cpu baseline {S} trajectories on {threads} threads", file=sys.stderr, flush=True)
-        r = O.ensemble_builtin(O.PROB_LORENZ, d["u0"][:, :S], d["p"][:, :S], d["t0"][:S], d["tf"][:S],
-                               alg=O.TSIT5, reltol=d["reltol"], abstol=d["abstol"], save_start=False,
-                               threads=threads)
-        res["cpu_baseline"] = {"value": S / r["seconds"], "unit": "trajectories/s", "cores": threads,
-                               "kind": "port", "sample": f"first {S} trajectories of the same batch, "
-                                                         f"{r['seconds']:.2f} s wall"}
+    res = bench_lorenz(args)
+    if not args.no_robertson:
+        print("[bench] robertson leg", file=sys.stderr, flush=True)
+        rob = rodas_legs(args, "robertson", max(1, min(args.steps, 5)), 1, plain=not args.no_plain)
+        for k in ("metric", "unit", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+            rob.pop(k, None)
+        res["robertson"] = rob
+    if args.serve_window > 0:
+        print("[bench] serving leg", file=sys.stderr, flush=True)
+        sv = serving_leg(args, args.serve_window)
+        res["serving"] = {k: sv[k] for k in ("value", "unit", "config", "latency_ms", "latency_ms_by_type",
+                                             "submit_lag_ms", "success_fraction", "dtype")}
     print(json.dumps(res))
 
 

@@ -158,6 +158,11 @@ void odemi_rhs_destroy(odemi_rhs* rhs);
 int odemi_rtc_codeobject(const char* problem_src, int n, int np, int flags, char* path_out, int path_cap,
                          double* seconds, char* log_out, int log_cap);
 
+/* The hipRTC version that compiles (and keys the cache of) code objects.
+ * Returns 0, or -1 when hipRTC cannot report it.  No counterpart in the
+ * reference (diagnostics: which compiler built the code that ran). */
+int odemi_rtc_version(int* major, int* minor);
+
 /* Solve N independent IVPs (one EnsembleProblem solve).  Synchronous. */
 int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* args);
 

@@ -415,16 +415,27 @@ static bool rtc_compile(const std::string& src, std::vector<char>& code, std::st
     return true;
 }
 
+extern "C" int odemi_rtc_version(int* major, int* minor) {
+    int a = 0, b = 0;
+    if (hiprtcVersion(&a, &b) != HIPRTC_SUCCESS) return -1;
+    if (major) *major = a;
+    if (minor) *minor = b;
+    return 0;
+}
+
 // Compile (or fetch from the disk cache) the code object for a problem.
 // Exposed so that a build step can pre-populate the cache without a GPU.
 extern "C" int odemi_rtc_codeobject(const char* problem_src, int n, int np, int flags, char* path_out,
                                     int path_cap, double* seconds, char* log_out, int log_cap) {
     std::string src = build_source(problem_src, n, np, flags);
+    // the key covers everything that determines the code: the full source
+    // (device headers included), the options (target arch included) and the
+    // hipRTC major and minor version
     std::string key = src;
     for (const char* o : kRtcOpts) key += o;
-    int ver = 0;
-    hiprtcVersion(&ver, &ver);
-    key += std::to_string(ver);
+    int vmaj = 0, vmin = 0;
+    if (hiprtcVersion(&vmaj, &vmin) != HIPRTC_SUCCESS) vmaj = vmin = -1;
+    key += "|hiprtc " + std::to_string(vmaj) + "." + std::to_string(vmin);
     char hex[32];
     snprintf(hex, sizeof(hex), "%016llx", (unsigned long long)fnv1a(key));
     std::string dir = cache_dir();
@@ -457,10 +468,10 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
     if (n < 1) { ctx->err = "n must be >= 1"; return nullptr; }
     if (!(flags & (ODEMI_COMPILE_TSIT5 | ODEMI_COMPILE_RODAS5P | ODEMI_COMPILE_WG))) flags |= ODEMI_COMPILE_TSIT5;
     char path[4096];
-    static char log[65536];
+    std::vector<char> log(65536, 0);   // per call: contexts may compile on different threads
     double secs = 0;
-    if (odemi_rtc_codeobject(problem_src, n, np, flags, path, sizeof(path), &secs, log, sizeof(log)) != 0) {
-        ctx->err = std::string("hipRTC compile failed:\n") + log;
+    if (odemi_rtc_codeobject(problem_src, n, np, flags, path, sizeof(path), &secs, log.data(), (int)log.size()) != 0) {
+        ctx->err = std::string("hipRTC compile failed:\n") + log.data();
         return nullptr;
     }
     if (compile_seconds) *compile_seconds = secs;

@@ -56,7 +56,7 @@ EXPORTS = ["odemi_create", "odemi_destroy", "odemi_last_error", "odemi_device_in
            "odemi_compile_rhs", "odemi_rhs_destroy", "odemi_solve_batch", "odemi_default_args",
            "odemi_rtc_codeobject", "odemi_server_default_opts", "odemi_server_create", "odemi_server_destroy",
            "odemi_submit", "odemi_poll", "odemi_wait", "odemi_request_times", "odemi_server_info",
-           "odemi_server_last_error", "odemi_request_release"]
+           "odemi_server_last_error", "odemi_request_release", "odemi_rtc_version"]
 
 _lib = None
 
@@ -104,6 +104,8 @@ def load():
     L.odemi_request_release.restype = c_i
     L.odemi_server_last_error.argtypes = [c_vp]
     L.odemi_server_last_error.restype = ctypes.c_char_p
+    L.odemi_rtc_version.argtypes = [P(c_i), P(c_i)]
+    L.odemi_rtc_version.restype = c_i
     _lib = L
     return L
 
@@ -118,6 +120,14 @@ def precompile(src, n, np_, flags):
     if rc != 0:
         raise RuntimeError("hipRTC compile failed:\n" + log.value.decode(errors="replace"))
     return path.value.decode(), secs.value
+
+
+def rtc_version():
+    """(major, minor) of the hipRTC that compiles the problems' code objects."""
+    a, b = c_i(), c_i()
+    if load().odemi_rtc_version(ctypes.byref(a), ctypes.byref(b)) != 0:
+        raise RuntimeError("hipRTC did not report its version")
+    return a.value, b.value
 
 
 class Context:
@@ -148,6 +158,10 @@ class Context:
         key = (src, n, np_, flags)
         r = self._rhs.get(key)
         if r is None:
+            if os.environ.get("ODEMI_LOG_CODEOBJECTS"):
+                # name the cached code object this module loads (the file name is its key's hash)
+                path, _ = precompile(src, n, np_, flags)
+                print(f"[odemi] code object {os.path.basename(path)} (n={n}, flags={flags})", flush=True)
             secs = c_d()
             h = self.L.odemi_compile_rhs(self.h, src.encode(), n, np_, flags, ctypes.byref(secs))
             if not h:

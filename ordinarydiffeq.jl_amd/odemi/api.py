@@ -314,12 +314,16 @@ template <typename T> __device__ __forceinline__ void odemi_tgrad(T* dT, const T
 
 
 def _callback_key(c):
+    """Trace-cache key of a callback.  It holds the traced callables themselves
+    (hashed and compared by identity), never their id(): the cache keeps them
+    alive, so a later callable cannot reuse a freed one's address and be given
+    its trace."""
     if isinstance(c, DiscreteCallback):
-        return ("d", id(c.condition), id(c.affect), c.save_positions)
+        return ("d", c.condition, c.affect, c.save_positions)
     if isinstance(c, VectorContinuousCallback):
-        return ("v", id(c.condition), id(c.affect), c.len, c.rootfind, c.interp_points, c.save_positions,
+        return ("v", c.condition, c.affect, c.len, c.rootfind, c.interp_points, c.save_positions,
                 c.abstol, c.repeat_nudge)
-    return ("c", id(c.condition), id(c.affect), id(c.affect_neg), c.rootfind, c.interp_points,
+    return ("c", c.condition, c.affect, c.affect_neg, c.rootfind, c.interp_points,
             c.save_positions, c.abstol, c.repeat_nudge)
 
 
@@ -331,27 +335,7 @@ def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, i
     kernels; callback: DiscreteCallback / ContinuousCallback / CallbackSet,
     traced likewise.
     Returns (rhs, traced, seconds)."""
-    wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
-    cbs = _callback_list(callback)
-    if wg and cbs:
-        raise ValueError("callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
-                         f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
-    if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
-        raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
-    if ns is not None and "tsit5" in algs:
-        raise ValueError("uneven-size batches (nvec) are supported with Rodas5P")
-    key = (id(fun.f), id(fun.jac), id(fun.tgrad), n, np_, tuple(ns) if ns is not None else None, wg,
-           id(isoutofdomain), tuple(_callback_key(c) for c in cbs))
-    tr = _TRACES.get(key)
-    if tr is None:
-        if wg:
-            tr = trace_problem_wg(fun.f, list(ns) if ns is not None else [n], np_, jac=fun.jac, tgrad=fun.tgrad,
-                                  isout=isoutofdomain)
-            tr.src = _LANE_STUBS + tr.src
-            tr.has_user_tgrad = fun.tgrad is not None
-        else:
-            tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad, isout=isoutofdomain, callbacks=cbs)
-        _TRACES[key] = tr
+    tr, wg = trace_for(fun, n, np_, algs=algs, ns=ns, isoutofdomain=isoutofdomain, callback=callback)
     flags = 0
     if wg:
         flags |= _lib.COMPILE_WG
@@ -365,6 +349,34 @@ def compile_problem(fun, n, np_, algs=("tsit5", "rodas5p"), ctx=None, ns=None, i
     ctx = ctx or get_context()
     h, secs = ctx.compile(tr.src, n, max(np_, 1), flags)
     return h, tr, secs
+
+
+def trace_for(fun, n, np_, algs=("tsit5", "rodas5p"), ns=None, isoutofdomain=None, callback=None):
+    """The traced HIP source of a problem (cached), and whether it targets the
+    workgroup kernels.  No device is touched."""
+    wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
+    cbs = _callback_list(callback)
+    if wg and cbs:
+        raise ValueError("callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
+                         f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
+    if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
+        raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
+    if ns is not None and "tsit5" in algs:
+        raise ValueError("uneven-size batches (nvec) are supported with Rodas5P")
+    # the key holds the callables (identity hash), so they stay alive while cached
+    key = (fun.f, fun.jac, fun.tgrad, n, np_, tuple(ns) if ns is not None else None, wg,
+           isoutofdomain, tuple(_callback_key(c) for c in cbs))
+    tr = _TRACES.get(key)
+    if tr is None:
+        if wg:
+            tr = trace_problem_wg(fun.f, list(ns) if ns is not None else [n], np_, jac=fun.jac, tgrad=fun.tgrad,
+                                  isout=isoutofdomain)
+            tr.src = _LANE_STUBS + tr.src
+            tr.has_user_tgrad = fun.tgrad is not None
+        else:
+            tr = trace_problem(fun.f, n, np_, jac=fun.jac, tgrad=fun.tgrad, isout=isoutofdomain, callbacks=cbs)
+        _TRACES[key] = tr
+    return tr, wg
 
 
 def _decimal(x):

@@ -161,3 +161,72 @@ def test_chain_serving_uneven_n_alone_equals_in_mix(prec):
                                 reltol=d["reltol"], abstol=d["abstol"], save_start=False)
             c = int(np.where(probe == j)[0][0])
             assert np.array_equal(alone.u_final[:n, c], r["u_final"])
+
+
+def test_poisson_mixed_types_under_budget_alone_equals_batch():
+    """Open-loop Poisson arrivals (seeded) of requests of mixed type — Lorenz
+    Tsit5, Robertson Rodas5P (lane servers) and uneven chains n = 4..32
+    (workgroup server) — at 2,000 req/s for 1 s, each server's pool smaller than
+    the load it is offered.  Requests are admitted as slots free up and run
+    overlapped on the device; every request's result equals the batch engine's
+    (that is, its result alone) on the same trajectories, bit for bit."""
+    import time
+    rng = np.random.Generator(np.random.PCG64(11))
+    pools = {
+        "lorenz": (P.lorenz_large(N=65536), P.LORENZ, odemi.Tsit5(), 64),
+        "robertson": (P.robertson_ensemble(N=65536), P.ROBERTSON, odemi.Rodas5P(), 64),
+        "chain": (P.chain_ensemble(N=2048, seed=12, m_lo=2, m_hi=16), P.CHAIN, odemi.Rodas5P(), 4),
+    }
+    pools["chain"][0]["tf"] = np.full(2048, 0.5)
+    rate, dur = 2000.0, 1.0
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * dur * 2)))
+    t_arr = t_arr[t_arr < dur]
+    kinds = list(pools)
+    kind = rng.choice(3, len(t_arr), p=[0.45, 0.45, 0.10])
+    size = np.array([int(rng.integers(1, pools[kinds[k]][3] + 1)) for k in kind])
+    print(f"[case] serving poisson {len(t_arr)} requests mixed types", flush=True)
+    servers, cursor, taken = {}, {k: 0 for k in kinds}, {k: [] for k in kinds}
+    for k, (d, fun, alg, _) in pools.items():
+        ns = sorted(set(int(x) for x in d["nvec"])) if "nvec" in d else None
+        servers[k] = odemi.EnsembleServer(fun, d["u0"].shape[0], d["p"].shape[0], alg, reltol=d["reltol"],
+                                          abstol=d["abstol"], ns=ns, tick_attempts=16,
+                                          max_slots=96 if k == "chain" else 2048)
+    reqs = []
+    try:
+        t0 = time.monotonic() + 0.02
+        for q in range(len(t_arr)):
+            while time.monotonic() < t0 + t_arr[q]:
+                time.sleep(0.0002)
+            k = kinds[kind[q]]
+            d = pools[k][0]
+            a, b = cursor[k], cursor[k] + int(size[q])
+            cursor[k] = b
+            idx = np.arange(a, b)
+            taken[k].append(idx)
+            nv = d["nvec"][idx] if "nvec" in d else None
+            reqs.append((k, idx, servers[k].submit(*_cols(d, idx), nvec=nv)))
+        for k, idx, r in reqs:
+            assert r.wait(timeout=300)
+        late = sum(1 for _, _, r in reqs if r.times()[1] > r.times()[0])
+        offered = {k: cursor[k] for k in kinds}
+        print(f"offered trajectories {offered}; admitted after arrival: {late} of {len(reqs)}")
+        for k in kinds:
+            assert servers[k].info()["finished"] == len(taken[k])
+    finally:
+        for s in servers.values():
+            s.close()
+    for k in kinds:
+        d, fun, alg, _ = pools[k]
+        m = cursor[k]
+        if m == 0:
+            continue
+        cols = np.arange(m)
+        kw = dict(reltol=d["reltol"], abstol=d["abstol"])
+        if "nvec" in d:
+            kw["nvec"] = d["nvec"][cols]
+        b = odemi.solve_arrays(fun, alg, *_cols(d, cols), **kw)
+        for kk, idx, r in reqs:
+            if kk != k:
+                continue
+            assert np.array_equal(r.u_final, b["u_final"][:, idx]), k
+            assert np.array_equal(r.stats, b["stats"][:, idx]) and np.array_equal(r.retcode, b["retcode"][idx])

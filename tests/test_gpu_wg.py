@@ -118,6 +118,25 @@ def test_chain_isoutofdomain_workgroup(prec, width):
         assert np.quantile(err.max(axis=0), 0.99) <= 10.0
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_wg_repeat_solves_bit_identical(prec):
+    """The same uneven chain batch solved twice in one process (and once more
+    with a predicate that never fires, which compiles a second module) gives
+    bit-identical results: nothing in the workgroup kernels depends on state
+    left by an earlier launch."""
+    d = _chain(96, m_lo=4, m_hi=40, seed=5)
+    print(f"[case] chain wg repeat {prec}", flush=True)
+    ea = odemi.EnsembleMI355X(precision=prec)
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], nvec=d["nvec"])
+    a = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], ea, **kw)
+    b = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], ea, **kw)
+    c = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], ea,
+                           isoutofdomain=lambda u, p, t: t > 1e9, **kw)
+    for o in (b, c):
+        assert np.array_equal(a["u_final"], o["u_final"])
+        assert np.array_equal(a["stats"], o["stats"]) and np.array_equal(a["retcode"], o["retcode"])
+
+
 def test_wg_kernel_robertson_matches_lane_kernel():
     """n = 3 through the workgroup kernel (nvec given) equals the lane kernel bit for bit."""
     d = P.robertson_ensemble(N=128)
@@ -173,3 +192,55 @@ def test_mixed_saveat_within_tolerance_lane_and_workgroup():
     err = np.abs(om["u_out"][:ns] - o64["u_out"][:ns]) / (c["abstol"] + c["reltol"] * np.abs(o64["u_out"][:ns]))
     print(f"chain saveat mixed scaled err p99 {np.quantile(err, 0.99):.3e}")
     assert np.quantile(err, 0.99) < 10.0
+
+
+@pytest.fixture(scope="module")
+def chain_cfg4_sample():
+    """128 trajectories of the 16,384-chain workload as stated: m ~ U{4..64}
+    (n = 8..128), tspan (0, 10), reltol = abstol = 1e-6; the FP64 restatement
+    and the Vern9 (reltol 1e-12) solutions of each."""
+    d = P.chain_ensemble(N=128, seed=40)
+    assert d["nvec"].min() <= 24 and d["nvec"].max() >= 112 and (d["tf"] == 10.0).all()
+    ref = _oracle(d)
+    v9 = O.vern9_ensemble(O.PROB_CHAIN, d["u0"], d["p"], d["t0"], d["tf"], nvec=d["nvec"])
+    assert (v9["status"] == 0).all()
+    return d, ref, v9["u_final"]
+
+
+def test_chain_full_span_uneven_one_launch_fp64_bit_identical(chain_cfg4_sample):
+    """FP64 workgroup engine, n = 8..128 in one launch over tspan (0, 10):
+    final states and step counts bit-identical to the restatement."""
+    d, ref, _ = chain_cfg4_sample
+    print(f"[case] chain cfg4 sample n {d['nvec'].min()}..{d['nvec'].max()} tspan (0,10) wg fp64", flush=True)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"])
+    assert out["launches"] == 1
+    assert (out["retcode"] == 1).all() and (ref["retcode"] == 1).all()
+    same = (out["stats"][1] == ref["naccept"]) & (out["stats"][2] == ref["nreject"])
+    exact = np.all(out["u_final"] == ref["u_final"], axis=0)
+    print(f"steps equal {same.mean():.4f} exact {exact.mean():.4f} mean attempts "
+          f"{(ref['naccept'] + ref['nreject']).mean():.0f}")
+    assert same.mean() >= 0.99 and exact.mean() >= 0.99
+    assert (out["stats"][0] == ref["nf"]).all()
+
+
+def test_chain_full_span_uneven_one_launch_mixed_accuracy(chain_cfg4_sample):
+    """Mixed (FP32 Gauss-Jordan + FP64 refinement) engine on the same sample:
+    p99 of the tolerance-scaled error against the FP64 restatement at most 10,
+    and its norm-wise error against Vern9 no larger than 1.5x the FP64
+    restatement's (median and p99)."""
+    d, ref, v9 = chain_cfg4_sample
+    print("[case] chain cfg4 sample tspan (0,10) wg mixed", flush=True)
+    mx = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(precision="mixed"), reltol=d["reltol"], abstol=d["abstol"],
+                            nvec=d["nvec"])
+    assert mx["launches"] == 1 and (mx["retcode"] == 1).all()
+    err = np.abs(mx["u_final"] - ref["u_final"]) / (d["abstol"] + d["reltol"] * np.abs(ref["u_final"]))
+    nrm = np.maximum(np.linalg.norm(v9, axis=0), 1e-300)
+    e64 = np.linalg.norm(ref["u_final"] - v9, axis=0) / nrm
+    emx = np.linalg.norm(mx["u_final"] - v9, axis=0) / nrm
+    print(f"mixed scaled err vs fp64 p99 {np.quantile(err.max(axis=0), 0.99):.3e}; vs Vern9: fp64 median "
+          f"{np.median(e64):.3e} p99 {np.quantile(e64, 0.99):.3e}, mixed median {np.median(emx):.3e} "
+          f"p99 {np.quantile(emx, 0.99):.3e}")
+    assert np.quantile(err.max(axis=0), 0.99) <= 10.0
+    assert np.median(emx) <= 1.5 * np.median(e64) and np.quantile(emx, 0.99) <= 1.5 * np.quantile(e64, 0.99)

@@ -150,3 +150,83 @@ def test_vector_continuous_callback_emits_and_compiles():
             lambda u, t, i: t == 1.0, lambda i: odemi.terminate(i, when=i.u[0] > 1.0))])
     with pytest.raises(ValueError):
         odemi.VectorContinuousCallback(cond, aff, 17)
+
+
+def _window(hi):
+    return lambda u, p, t: (t > 0.3) & (t < hi)
+
+
+def test_trace_cache_never_reuses_a_freed_callables_trace():
+    """Predicates made one after another (each dropped by the caller before the
+    next exists, so CPython may hand the next one the same address) each get
+    their own trace: the cache keys on the callable, not on its id()."""
+    from odemi import problems as P
+    from odemi.api import trace_for
+    his = [0.31, 0.3001, 0.30001, 0.3000001]
+    srcs = []
+    for hi in his:
+        tr, wg = trace_for(P.CHAIN, 8, 3, algs=("rodas5p",), ns=[8], isoutofdomain=_window(hi))
+        assert wg
+        srcs.append(tr.src)
+    for hi, src in zip(his, srcs):
+        assert f"T({float(hi).hex()})" in src, hi
+    assert len(set(srcs)) == len(his)
+    # the same object is still a cache hit
+    f = _window(0.5)
+    a, _ = trace_for(P.CHAIN, 8, 3, algs=("rodas5p",), ns=[8], isoutofdomain=f)
+    b, _ = trace_for(P.CHAIN, 8, 3, algs=("rodas5p",), ns=[8], isoutofdomain=f)
+    assert a is b
+
+
+def test_callback_trace_cache_keys_on_the_callables():
+    def make(level):
+        return odemi.DiscreteCallback(lambda u, t, itg: u[0] >= level, _reset_stop)
+    srcs = [trace_for_cb(make(lv)) for lv in (19.0, 18.5, 18.25)]
+    assert len(set(srcs)) == 3
+
+
+def trace_for_cb(cb):
+    from odemi.api import trace_for
+    tr, _ = trace_for(odemi.ODEFunction(_decay), 2, 2, algs=("tsit5",), callback=cb)
+    return tr.src
+
+
+def test_codeobjects_compile_concurrently_from_two_threads(tmp_path):
+    """odemi_rtc_codeobject from two host threads at once (distinct problems):
+    both succeed and produce distinct cache files (the compile log buffer is
+    per call)."""
+    import threading
+    env_old = os.environ.get("ODEMI_CACHE_DIR")
+    L = _lib.load()
+    srcs = [trace_problem(_decay, 2, 2).src,
+            trace_problem(_decay, 2, 2, isout=lambda u, p, t: u[0] < 0.0).src]
+    res = [None, None]
+
+    def work(i):
+        path = ctypes.create_string_buffer(4096)
+        log = ctypes.create_string_buffer(65536)
+        secs = ctypes.c_double()
+        rc = L.odemi_rtc_codeobject(srcs[i].encode(), 2, 2, _lib.COMPILE_TSIT5, path, 4096, ctypes.byref(secs),
+                                    log, 65536)
+        res[i] = (rc, path.value.decode(), log.value.decode(errors="replace"))
+
+    os.environ["ODEMI_CACHE_DIR"] = str(tmp_path)
+    try:
+        th = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    finally:
+        if env_old is None:
+            os.environ.pop("ODEMI_CACHE_DIR", None)
+        else:
+            os.environ["ODEMI_CACHE_DIR"] = env_old
+    assert res[0][0] == 0 and res[1][0] == 0, res
+    assert res[0][1] != res[1][1]
+    assert all(os.path.getsize(r[1]) > 0 for r in res)
+
+
+def test_rtc_version_reported():
+    major, minor = _lib.rtc_version()
+    assert major >= 1 and minor >= 0
