@@ -90,6 +90,13 @@ static inline double jl_max(double x, double y) {
 static inline double max_fast(double x, double y) { return y > x ? y : x; }
 
 /* ------------------------------------------------------------------------ */
+/* det_log and det_exp follow fdlibm's e_log.c and e_exp.c (their argument
+ * reduction, polynomial constants and reconstruction), which carry this notice:
+ *   Copyright (C) 1993 (e_log.c), 2004 (e_exp.c) by Sun Microsystems, Inc.
+ *   All rights reserved.
+ *   Developed at SunSoft, a Sun Microsystems, Inc. business.
+ *   Permission to use, copy, modify, and distribute this software is freely
+ *   granted, provided that this notice is preserved. */
 /* Deterministic log / exp (fdlibm-style reduction + minimax polynomials).   */
 /* The GPU engine evaluates the same algorithm, so initdt's 10^x / log10 and */
 /* fastpower's exp2 agree bit for bit between the two.  Accuracy ~1 ulp.     */

@@ -153,6 +153,13 @@ __device__ __forceinline__ double sqrt_nr(double x) {   // x >= 0
     return (x > 0.0 && x < __builtin_huge_val()) ? g : x;   // 0, inf, NaN pass through
 }
 
+// det_log and det_exp follow fdlibm's e_log.c and e_exp.c (their argument
+// reduction, polynomial constants and reconstruction), which carry this notice:
+//   Copyright (C) 1993 (e_log.c), 2004 (e_exp.c) by Sun Microsystems, Inc.
+//   All rights reserved.
+//   Developed at SunSoft, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this software is freely
+//   granted, provided that this notice is preserved.
 // Deterministic log / exp (fdlibm-style argument reduction + minimax
 // polynomials, ~1 ulp).  Used wherever the controller needs an elementary
 // function, so the CPU restatement (same algorithm) agrees bit for bit.
