@@ -234,6 +234,23 @@ def rodas_legs(args, workload, steps, warmup, cpu=True, plain=True):
         op, msp, _ = _timed(lambda: run(odemi.EnsembleMI355X(engine="plain")), 1, 0, torch)
         res["plain_fp64"] = {"trajectories_per_s": N / (msp / 1e3), "ms": msp, "launches": int(op["launches"])}
         res["batched_over_plain"] = {"fp64": msp / ms64, "mixed": msp / msm}
+    elif plain:
+        # uneven chains: the plain FP64 engine (one launch per step attempt, the
+        # workgroup kernels with their state in HBM) against both persistent
+        # engines on the same leading sample of the batch
+        S = min(N, args.plain_sample)
+        kws = dict(kw, nvec=nvec[:S])
+        sub = [x[..., :S].contiguous() for x in (u0, p, t0, tf)]
+        runs = lambda ea: odemi.solve_arrays(fun, odemi.Rodas5P(), *sub, ea, **kws)  # noqa: E731
+        print(f"[bench] plain fp64 workgroup engine on {S} trajectories", file=sys.stderr, flush=True)
+        op, msp, _ = _timed(lambda: runs(odemi.EnsembleMI355X(engine="plain")), 1, 0, torch)
+        _, mss64, _ = _timed(lambda: runs(fp64), 1, 0, torch)
+        _, mssm, _ = _timed(lambda: runs(mixed), 1, 0, torch)
+        same = bool(np.array_equal(op["u_final"].cpu().numpy(), u64[:, :S]))
+        res["plain_fp64"] = {"sample": S, "trajectories_per_s": S / (msp / 1e3), "ms": msp,
+                             "launches": int(op["launches"]), "identical_to_persistent_fp64": same}
+        res["batched_over_plain"] = {"sample": S, "fp64": msp / mss64, "mixed": msp / mssm,
+                                     "persistent_fp64_ms": mss64, "persistent_mixed_ms": mssm}
     if cpu and not args.no_cpu:
         res["cpu_baseline"] = _cpu_baseline(pid, d, O.RODAS5P, 262144 if lane else 256, 16384 if lane else 32,
                                             nvec=nvec, chunk=64 if lane else 1)
@@ -539,6 +556,8 @@ def main():
     ap.add_argument("--chain-max", type=int, default=16, help="serving: chains per chain request, max")
     ap.add_argument("--hbm-budget-gb", type=int, default=8, help="serving: slot-pool budget, GiB")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-sample", type=int, default=2048,
+                    help="chains: trajectories of the plain-vs-batched comparison")
     ap.add_argument("--no-robertson", action="store_true", help="default line: skip the Robertson leg")
     ap.add_argument("--refine-sweeps", type=int, default=1, help="mixed Rodas5P: FP64 refinement sweeps")
     ap.add_argument("--no-cpu", action="store_true")
@@ -553,6 +572,9 @@ def main():
     # a lane server's tick in a shared in-order queue; set before HIP starts
     if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
         os.environ["GPU_MAX_HW_QUEUES"] = "8"
+    # the engine's library, bound to the HIP runtime torch brings
+    from odemi import _lib
+    _lib.load()
     if args.workload == "serving":
         return bench_serving(args)
     if args.workload == "events":

@@ -71,7 +71,8 @@ typedef struct odemi_rhs odemi_rhs;
 
 typedef struct {
     int alg;                  /* ODEMI_TSIT5 / ODEMI_RODAS5P */
-    int engine;               /* ODEMI_ENGINE_* */
+    int engine;               /* ODEMI_ENGINE_* (plain on a workgroup problem: one launch per
+                                 attempt through the workgroup kernels, start/end saves only) */
     int precision;            /* ODEMI_FP64 / ODEMI_MIXED */
     int device_ptrs;          /* 1: every array below is a device pointer */
     int64_t N;                /* trajectories */
@@ -81,9 +82,9 @@ typedef struct {
     const double* u0;         /* [nmax][N] */
     const double* p;          /* [np][N] */
     const double* t0;         /* [N] */
-    const double* tf;         /* [N] */
+    const double* tf;         /* [N]; tf < t0 integrates backward (tdir = -1) */
     double reltol, abstol;    /* solve.jl defaults: 1e-3, 1e-6 */
-    double dtmax;             /* <= 0: tf - t0 (solve.jl:153) */
+    double dtmax;             /* <= 0: tf - t0 (solve.jl:153); > 0 takes each trajectory's tdir sign */
     double dtmin;             /* 0 (solve.jl:152) */
     double dt;                /* != 0: initial dt; 0: automatic (initdt.jl) */
     int64_t maxiters;         /* 1_000_000 (solve.jl:161) */
@@ -146,6 +147,14 @@ int odemi_abi_version(void);
  *     the largest) instead vcond(T* out, const T* u, const Q* p, T t) and
  *     bool vaffect(T* u, const Q* p, T t, const double* ev) returning terminate!
  *     <- ContinuousCallbacks (event root finding; lib/DiffEqBase/src/callbacks.jl:157-640)
+ * Optional, workgroup kernels: #define ODEMI_NDCB_WG k with template<int K> struct OdemiDcbWg
+ *     holding save0, save1, term and
+ *     template<typename T> static __device__ bool cond(const T* u, const T* p, T t, int n);
+ *     template<typename T> static __device__ void affect(T* u, const T* p, T t, int n);
+ *   #define ODEMI_NCCB_WG k with OdemiCcbWg<K> holding OdemiCcb's scalar constants, a
+ *     T cond(u, p, t, n), affect and affect_neg (u, p, t, n)
+ *     <- the same DiscreteCallbacks and (scalar) ContinuousCallbacks, one case per
+ *        state size n of an uneven batch, run by one thread on the state in LDS
  * n/np are the (maximum) state and parameter counts.  Code objects are cached
  * on disk by source hash ($ODEMI_CACHE_DIR).  compile_seconds may be NULL. */
 odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src, int n, int np, int flags,
@@ -178,7 +187,8 @@ void odemi_default_args(odemi_solve_args* args, int alg);
  * the device between ticks, advances all active trajectories together, and
  * copies a request's results to the caller's buffers when its last trajectory
  * finishes.  A trajectory's result does not depend on what else is in flight.
- * Requests share the server's options (tolerances, final state only).
+ * A request may carry its own tolerances (odemi_submit_opts); the other
+ * options are the server's, and requests return the final state only.
  * ------------------------------------------------------------------------- */
 typedef struct odemi_server odemi_server;
 typedef struct {
@@ -205,6 +215,17 @@ void odemi_server_destroy(odemi_server* server);
  * may be NULL).  Returns a request id (> 0) or a negative error. */
 int64_t odemi_submit(odemi_server* server, int64_t N, const int32_t* nvec, const double* u0, const double* p,
                      const double* t0, const double* tf, double* u_final, int32_t* retcode, int32_t* stats);
+/* Per-request options (odemi_submit_opts); a value <= 0 takes the server's.
+ * No counterpart in the reference beyond the per-solve keywords it mirrors:
+ * reltol / abstol of `solve(prob, alg; reltol, abstol)`
+ * (lib/OrdinaryDiffEqCore/src/solve.jl:128-203). */
+typedef struct {
+    double reltol, abstol;
+} odemi_request_opts;
+/* odemi_submit with the request's own tolerances (ro may be NULL). */
+int64_t odemi_submit_opts(odemi_server* server, int64_t N, const int32_t* nvec, const double* u0, const double* p,
+                          const double* t0, const double* tf, const odemi_request_opts* ro, double* u_final,
+                          int32_t* retcode, int32_t* stats);
 /* 1 done, 0 pending or running, < 0 error */
 int odemi_poll(odemi_server* server, int64_t request);
 /* Block until done (timeout_s <= 0: no timeout).  1 done, 0 timed out, < 0 error. */

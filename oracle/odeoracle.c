@@ -378,6 +378,7 @@ typedef struct {
     const double* p;
     const oo_opts* o;
     double reltol, abstol, dtmax, dtmin, t0, tf;
+    double tdir;          /* integration direction sign(tf - t0) (+1 when tf == t0) */
     /* controller constants (alg_utils.jl:407-879, controllers.jl:744-803) */
     double qmin, qmax, qmax_first, gamma, beta1, beta2, qoldinit, qsteady_min, qsteady_max;
     /* integrator scalars */
@@ -399,7 +400,9 @@ typedef struct {
     double* t_out; double* u_out; int max_out;
     double* saveat_int; int n_saveat, saveat_pos;
     /* tstops queue (initialize_tstops, solve.jl:988-1030): user times strictly
-     * inside (t0, tf), sorted, then tf; [tst_pos, n_tst) are still queued */
+     * inside (t0, tf), then tf, stored multiplied by tdir and sorted ascending,
+     * so the next one in the integration direction is first; [tst_pos, n_tst)
+     * are still queued.  saveat_int likewise (initialize_saveat). */
     double* tst; int n_tst, tst_pos;
 } integ_t;
 
@@ -423,11 +426,13 @@ static inline void F(integ_t* I, double* du, const double* u, double t) {
  * integrator_interface.jl:644-662 with dtmax = min(opts.dtmax, tstop - t). */
 static double initdt(integ_t* I) {
     const int n = I->n;
-    const double t = I->t;
-    const double dtmax_tdir = jl_min(fabs(I->dtmax), fabs(first_tstop(I) - t));
+    const double t = I->t, tdir = I->tdir;
+    /* _determine_initdt passes dtmax = tdir min(|dtmax|, |first_tstop - tdir t|),
+     * which _ode_initdt_iip multiplies by tdir again */
+    const double dtmax_tdir = jl_min(fabs(I->dtmax), fabs(first_tstop(I) - tdir * t));
     const double dtmin = nextafter(jl_max(I->dtmin, oo_jl_eps(t)), INFINITY);
     const double smalldt = jl_max(dtmin, 1e-6);
-    if (n == 0) return jl_max(smalldt, dtmin);
+    if (n == 0) return tdir * jl_max(smalldt, dtmin);
     double* sk = I->atmp;      /* scratch */
     double* f0 = I->du;
     double* f1 = I->dT;
@@ -439,15 +444,16 @@ static double initdt(integ_t* I) {
     double d0 = ode_norm(tmp, n);
     for (int i = 0; i < n; i++) tmp[i] = f0[i] / sk[i] * 1.0;
     double d1 = ode_norm(tmp, n);
-    if (isnan(d1)) return dtmin;
+    if (isnan(d1)) return tdir * dtmin;
     double dt0 = ((d0 < 1e-5) | (d1 < 1e-5)) ? smalldt : (d0 / d1) / 100.0;
     dt0 = jl_min(dt0, dtmax_tdir);
-    if (dt0 < 10.0 * DBL_EPSILON) return jl_max(smalldt, dtmin);
-    for (int i = 0; i < n; i++) u1[i] = fma(dt0, f0[i], I->u[i]);
-    I->f(f1, u1, I->p, t + dt0, I->user);
+    if (dt0 < 10.0 * DBL_EPSILON) return tdir * jl_max(smalldt, dtmin);
+    const double dt0_tdir = tdir * dt0;
+    for (int i = 0; i < n; i++) u1[i] = fma(dt0_tdir, f0[i], I->u[i]);
+    I->f(f1, u1, I->p, t + dt0_tdir, I->user);
     int same = 1;
     for (int i = 0; i < n; i++) if (!(f0[i] == f1[i])) { same = 0; break; }
-    if (same) return jl_max(dtmin, 100.0 * dt0);
+    if (same) return tdir * jl_max(dtmin, 100.0 * dt0);
     for (int i = 0; i < n; i++) tmp[i] = (f1[i] - f0[i]) / sk[i] * 1.0;
     double d2 = ode_norm(tmp, n) / dt0 * 1.0;
     double mx = jl_max(d1, d2);
@@ -456,47 +462,51 @@ static double initdt(integ_t* I) {
         dt1 = jl_max(1e-6, dt0 * 0.001);
     else
         dt1 = oo_det_exp(-(2.0 + oo_det_log(mx) / D_LN10) / 5.0 * D_LN10);
-    return jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
+    return tdir * jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
 }
 
-/* integrator_utils.jl:268-324 (adaptive and dtchangeable branches, tdir = +1);
- * the queue is never empty inside the solve loop (tf is its last entry) */
+/* integrator_utils.jl:268-324 (adaptive and dtchangeable branches); the queue
+ * is never empty inside the solve loop (tf is its last entry) */
 static void modify_dt_for_tstops(integ_t* I) {
-    const double tstop = first_tstop(I);
-    double distance = fabs(tstop - I->t);
+    const double tdir = I->tdir;
+    const double tdir_t = tdir * I->t;
+    const double tdir_tstop = first_tstop(I);
+    double distance = fabs(tdir_tstop - tdir_t);
     double tol = 0.0;
-    if (isfinite(tstop) && isfinite(I->t))
-        tol = 100.0 * oo_jl_eps(jl_max(fabs(I->t), fabs(tstop)));
+    if (isfinite(tdir_tstop) && isfinite(I->t))
+        tol = 100.0 * oo_jl_eps(jl_max(fabs(I->t), fabs(tdir_tstop)));
     if (!I->o->adaptive) {   /* dtchangeable branch: always try dtcache */
         I->next_step_tstop = !(fabs(I->dtcache) + tol < distance);
-        if (I->next_step_tstop) I->tstop_target = tstop;
-        I->dt = jl_min(fabs(I->dtcache), distance);
+        if (I->next_step_tstop) I->tstop_target = tdir * tdir_tstop;
+        I->dt = tdir * jl_min(fabs(I->dtcache), distance);
         return;
     }
     double original_dt = fabs(I->dt);
-    I->dtpropose = original_dt;
+    I->dtpropose = tdir * original_dt;
     if (original_dt + tol < distance) {
         I->next_step_tstop = 0;
     } else {
         I->next_step_tstop = 1;
-        I->tstop_target = tstop;
+        I->tstop_target = tdir * tdir_tstop;
     }
-    I->dt = jl_min(original_dt, distance);
+    I->dt = tdir * jl_min(original_dt, distance);
 }
 
 /* handle_tstop! (integrator_utils.jl:1290-1314), dtchangeable: pop every queued
  * copy of t once t reaches the first tstop.  Stepping past a tstop is an error
  * in the reference; it cannot happen here (dt is clipped to the distance). */
 static void handle_tstop(integ_t* I) {
-    if (I->tst_pos < I->n_tst && I->t == first_tstop(I))
-        while (I->tst_pos < I->n_tst && I->t == first_tstop(I)) I->tst_pos++;
+    const double tdir_t = I->tdir * I->t;
+    if (I->tst_pos < I->n_tst && tdir_t == first_tstop(I))
+        while (I->tst_pos < I->n_tst && tdir_t == first_tstop(I)) I->tst_pos++;
 }
 
-/* integrator_utils.jl:1243-1256 with utils.jl:90-94 */
+/* integrator_utils.jl:1243-1256 with utils.jl:90-94 (dtmax carries tdir's
+ * sign; backward, dt = min(dt, |dtmin|) leaves a negative dt as it is) */
 static void fix_dt_at_bounds(integ_t* I) {
-    I->dt = jl_min(I->dtmax, I->dt);
+    I->dt = (I->tdir > 0) ? jl_min(I->dtmax, I->dt) : jl_max(I->dtmax, I->dt);
     double dtmin_t = fabs(jl_max(oo_jl_eps(I->t), I->dtmin));
-    I->dt = jl_max(I->dt, dtmin_t);
+    I->dt = (I->tdir > 0) ? jl_max(I->dt, dtmin_t) : jl_min(I->dt, dtmin_t);
 }
 
 /* controllers.jl:805-821 */
@@ -529,8 +539,8 @@ static void step_reject_controller(integ_t* I) {
 
 /* integrator_utils.jl:1199-1210 */
 static void calc_dt_propose(integ_t* I, double dtnew) {
-    double dp = jl_min(fabs(I->dtmax), fabs(dtnew));
-    I->dtpropose = jl_max(fabs(dp), fabs(jl_max(oo_jl_eps(I->t), I->dtmin)));
+    double dp = I->tdir * jl_min(fabs(I->dtmax), fabs(dtnew));
+    I->dtpropose = I->tdir * jl_max(fabs(dp), fabs(jl_max(oo_jl_eps(I->t), I->dtmin)));
 }
 
 /* check_error.jl:69-118 (adaptive, force_dtmin = false, default unstable_check) */
@@ -538,7 +548,7 @@ static int check_error(integ_t* I) {
     if (isnan(I->dt)) return OO_RC_DTNAN;
     if (I->iter > I->o->maxiters) return OO_RC_MAXITERS;
     if (I->o->adaptive) {
-        if (fabs(I->dt) <= fabs(I->dtmin) && (!I->accept_step || I->t + I->dt < first_tstop(I)))
+        if (fabs(I->dt) <= fabs(I->dtmin) && (!I->accept_step || I->t + I->dt < I->tdir * first_tstop(I)))
             return OO_RC_DTLESSTHANMIN;
         else if (!I->accept_step && fabs(I->dt) <= fabs(oo_jl_eps(I->t)))
             return OO_RC_UNSTABLE;
@@ -844,8 +854,8 @@ static void rodas_interp(integ_t* I, double theta, double* out) {
 static int savevalues(integ_t* I, int force_save) {
     const oo_opts* o = I->o;
     int savedexactly = 0;
-    while (I->saveat_pos < I->n_saveat && I->saveat_int[I->saveat_pos] <= I->t) {
-        double curt = I->saveat_int[I->saveat_pos++];
+    while (I->saveat_pos < I->n_saveat && I->saveat_int[I->saveat_pos] <= I->tdir * I->t) {
+        double curt = I->tdir * I->saveat_int[I->saveat_pos++];
         if (curt != I->t) {
             double theta = (curt - I->tprev) / I->dt;
             if (I->alg == OO_TSIT5) tsit5_interp(I, theta, I->dt, I->tmp);
@@ -1020,7 +1030,7 @@ static int vcb_find_time(integ_t* I, int k, double* cb_t, double* ev, double* re
         *cb_t = top_t; *residual = 0.0;
         return 1;
     }
-    double cbt = nextafter(top_t, INFINITY), res = 0.0;
+    double cbt = nextafter(top_t, I->tdir > 0 ? INFINITY : -INFINITY), res = 0.0;   /* rightfloat */
     double tmp[OO_VCB_MAXLEN];
     for (int idx = 0; idx < len; idx++) {
         if (!((mask >> idx) & 1u)) continue;
@@ -1039,9 +1049,9 @@ static int vcb_find_time(integ_t* I, int k, double* cb_t, double* ev, double* re
             }
             cbi = fl[OO_CCB_ROOTFIND] == 1 ? r : l;
         }
-        if (cbi < cbt)
+        if (I->tdir * cbi < I->tdir * cbt)
             for (int i = 0; i < len; i++) ev[i] = 0.0;
-        if (cbi <= cbt) {
+        if (I->tdir * cbi <= I->tdir * cbt) {
             cbt = cbi;
             vcb_condition(I, k, cbi, tmp);
             res = tmp[idx];
@@ -1062,7 +1072,7 @@ static int ccb_apply(integ_t* I, int k, double cb_t, double bs, const double* ev
     const oo_opts* o = I->o;
     const int* fl = o->ccb_flags + OO_CCB_NFLAGS * k;
     if (o->adaptive) {
-        const double dtp = jl_max(nextafter(I->dtmin, INFINITY), I->dt);
+        const double dtp = I->tdir * jl_max(nextafter(I->dtmin, INFINITY), I->tdir * I->dt);
         I->dtpropose = dtp;
         I->dtcache = dtp;
     }
@@ -1118,7 +1128,7 @@ static int handle_callbacks(integ_t* I) {
             double cbt, bs = 0.0, rs;
             const int vlen = o->ccb_flags[OO_CCB_NFLAGS * k + OO_CCB_VLEN];
             const int occ = vlen > 0 ? vcb_find_time(I, k, &cbt, ev, &rs) : ccb_find_time(I, k, &cbt, &bs, &rs);
-            if (occ && (win < 0 || cbt < tmin)) {
+            if (occ && (win < 0 || I->tdir * cbt < I->tdir * tmin)) {
                 win = k; tmin = cbt; bs_win = bs; res_win = rs;
                 for (int i = 0; i < OO_VCB_MAXLEN; i++) ev_win[i] = i < vlen ? ev[i] : 0.0;
             }
@@ -1183,7 +1193,9 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     I->p = p; I->o = opts; I->res = res;
     I->reltol = opts->reltol; I->abstol = opts->abstol;
     I->t0 = t0; I->tf = tf;
-    I->dtmax = (isnan(opts->dtmax) || opts->dtmax <= 0.0) ? (tf - t0) : opts->dtmax;
+    I->tdir = (tf < t0) ? -1.0 : 1.0;
+    /* dtmax defaults to tf - t0; a positive user dtmax takes tdir's sign (solve.jl:401) */
+    I->dtmax = (isnan(opts->dtmax) || opts->dtmax <= 0.0) ? (tf - t0) : I->tdir * opts->dtmax;
     I->dtmin = opts->dtmin;
     I->t_out = t_out; I->u_out = u_out; I->max_out = max_out;
     /* controller defaults: qmin 1//5, qmax 10, qmax_first_step 10000, gamma 9//10,
@@ -1205,26 +1217,28 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     I->J = m; m += nn * nn; I->W = m; m += nn * nn;
     I->piv = piv;
 
-    /* saveat_internal (solve.jl:1103-1123): times in (t0, tf], sorted */
+    /* initialize_saveat (solve.jl:1103-1123): tdir s for the times with
+     * tdir t0 < tdir s <= tdir tf, sorted */
+    const double tdir = I->tdir;
     I->saveat_int = NULL; I->n_saveat = 0;
     if (opts->nsaveat > 0) {
         I->saveat_int = (double*)malloc(sizeof(double) * opts->nsaveat);
         for (int i = 0; i < opts->nsaveat; i++) {
-            double s = opts->saveat[i];
-            if (t0 < s && s <= tf) I->saveat_int[I->n_saveat++] = s;
+            double s = tdir * opts->saveat[i];
+            if (tdir * t0 < s && s <= tdir * tf) I->saveat_int[I->n_saveat++] = s;
         }
         qsort(I->saveat_int, I->n_saveat, sizeof(double), cmp_double);
     }
 
-    /* initialize_tstops (solve.jl:1019-1030) */
+    /* initialize_tstops (solve.jl:1019-1030): tdir s strictly inside, then tdir tf */
     I->tst = (double*)malloc(sizeof(double) * (size_t)(opts->ntstops + 1));
     I->n_tst = 0; I->tst_pos = 0;
     for (int i = 0; i < opts->ntstops; i++) {
-        double s = opts->tstops[i];
-        if (t0 < s && s < tf) I->tst[I->n_tst++] = s;
+        double s = tdir * opts->tstops[i];
+        if (tdir * t0 < s && s < tdir * tf) I->tst[I->n_tst++] = s;
     }
     qsort(I->tst, I->n_tst, sizeof(double), cmp_double);
-    I->tst[I->n_tst++] = tf;
+    I->tst[I->n_tst++] = tdir * tf;
 
     memcpy(I->u, u0, sizeof(double) * n);
     memcpy(I->uprev, u0, sizeof(double) * n);
@@ -1239,7 +1253,9 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
     /* handle_dt! (solve.jl:968-985) */
     I->dtcache = opts->dt;
     if (opts->dt != 0.0) {   /* user dt (NaN included: check_error reports DtNaN) */
-        I->dt = opts->dt; I->dtpropose = opts->dt;
+        /* a positive dt of a backward solve takes tdir's sign (solve.jl:968-985) */
+        const double dt = (opts->adaptive && opts->dt > 0.0 && tdir < 0) ? tdir * opts->dt : opts->dt;
+        I->dt = dt; I->dtpropose = dt;
     } else {
         I->dt = initdt(I);
         I->dtpropose = I->dt;
@@ -1252,7 +1268,7 @@ int oo_solve(int n, oo_rhs_fn f, oo_jac_fn jac, oo_tgrad_fn tgrad, void* user,
      * of step attempts while t < first_tstop */
     while (I->tst_pos < I->n_tst && rc == OO_RC_SUCCESS) {
       const double first = first_tstop(I);
-      while (I->t < first) {
+      while (tdir * I->t < first) {
         /* loopheader! (integrator_utils.jl:84-127) */
         if (I->iter > 0) {
             if (I->accept_step) {

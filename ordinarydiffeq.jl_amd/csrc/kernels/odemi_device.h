@@ -337,12 +337,15 @@ __constant__ double kRH[3][8] = {
 // ---------------------------------------------------------------------------
 struct Ctl {
     double t, dt, dtpropose, tprev, tf, dtmax, EEst, q11, errold;
+    double tdir;   // integration direction sign(tf - t0) (+1 when tf == t0); dt and dtmax carry it
     int iter, success_iter, accept_step, next_step_tstop;
     int isout;   // the last attempt's isoutofdomain verdict (integrator.isout)
     int nf, naccept, nreject, njacs, nw, nsolve;
     int spos, send, nsaved, tf_in_saveat;
-    // tstops queue (initialize_tstops, solve.jl:1019-1030): user times strictly
-    // inside (t0, tf) are P.tstops[tsp..tsend), then tf; tnext = the first one
+    // tstops queue (initialize_tstops, solve.jl:1019-1030): the user times
+    // strictly inside (t0, tf) are P.tstops[tsp], P.tstops[tsp + dir], ... up to
+    // (not including) tsend, dir = tdir (the shared list is sorted ascending),
+    // then tf; tnext = the first one.  saveat likewise with spos / send.
     int tsp, tsend;
     double tnext;
     // discrete callbacks: a callback modified u after the last accepted step
@@ -379,22 +382,30 @@ __device__ __forceinline__ CtlConst ctl_const(bool rosenbrock) {
 // restores dt = dtpropose.
 __device__ __forceinline__ void modify_dt_for_tstops(Ctl& c) {
     const double tstop = c.tnext;   // first queued tstop (tf last)
+    // |tdir tstop - tdir t| = |tstop - t|
     double distance = fabs(tstop - c.t);
     const double tol = (isfinite(tstop) && isfinite(c.t)) ? 100.0 * jl_eps(jl_max(fabs(c.t), fabs(tstop))) : 0.0;
     double original_dt = fabs(c.dt);
-    c.dtpropose = original_dt;
+    c.dtpropose = c.tdir * original_dt;
     // the tstop target of a flagged step is c.tnext (used at its accept)
     c.next_step_tstop = (original_dt + tol < distance) ? 0 : 1;
-    c.dt = jl_min(original_dt, distance);
+    c.dt = c.tdir * jl_min(original_dt, distance);
 }
 // The lasting effect of apply_step!'s modify_dt_for_tstops! (see above).
 __device__ __forceinline__ void apply_step_tstop_clip(Ctl& c) {
-    c.dt = jl_min(fabs(c.dt), fabs(c.tnext - c.t));
+    c.dt = c.tdir * jl_min(fabs(c.dt), fabs(c.tnext - c.t));
 }
-// integrator_utils.jl:1243-1256, utils.jl:90-94
+// integrator_utils.jl:1243-1256, utils.jl:90-94 (backward, dt = min(dt, |dtmin|)
+// leaves a negative dt as it is, as in the reference)
 __device__ __forceinline__ void fix_dt_at_bounds(Ctl& c, double dtmin) {
-    c.dt = jl_min(c.dtmax, c.dt);
-    c.dt = jl_max(c.dt, fabs(jl_max(jl_eps(c.t), dtmin)));
+    const double dmin = fabs(jl_max(jl_eps(c.t), dtmin));
+    if (c.tdir > 0) {
+        c.dt = jl_min(c.dtmax, c.dt);
+        c.dt = jl_max(c.dt, dmin);
+    } else {
+        c.dt = jl_max(c.dtmax, c.dt);
+        c.dt = jl_min(c.dt, dmin);
+    }
 }
 // controllers.jl:805-821.  HW: the mixed engine's form (FastPower's Float32
 // log2/exp2 with the hardware reciprocal and exp2, FP64 quotients by div_nr).
@@ -439,8 +450,8 @@ __device__ __forceinline__ void accept_controller(Ctl& c, const CtlConst& k, dou
     } else {
         dtnew = c.dt / q;
     }
-    double dp = jl_min(fabs(c.dtmax), fabs(dtnew));
-    c.dtpropose = jl_max(fabs(dp), fabs(jl_max(jl_eps(c.t), dtmin)));
+    double dp = c.tdir * jl_min(fabs(c.dtmax), fabs(dtnew));
+    c.dtpropose = c.tdir * jl_max(fabs(dp), fabs(jl_max(jl_eps(c.t), dtmin)));
 }
 // controllers.jl:838-843 (HW: the mixed engines' Newton-refined FP64 quotients;
 // a wave runs this whenever any of its lanes rejects, i.e. nearly every attempt)
@@ -462,41 +473,63 @@ __device__ __forceinline__ bool all_finite(const S* u) {
 __device__ __forceinline__ int check_error(const Ctl& c, const Params& P, bool u_finite) {
     if (isnan(c.dt)) return RC_DTNAN;
     if ((i64)c.iter > P.maxiters) return RC_MAXITERS;
+    // t + dt < tdir first(tstops), the physical time of the first tstop, in
+    // both directions (check_error.jl:91-99)
     if (fabs(c.dt) <= fabs(P.dtmin) && (!c.accept_step || c.t + c.dt < c.tnext)) return RC_DTLESSTHANMIN;
     else if (!c.accept_step && fabs(c.dt) <= fabs(jl_eps(c.t))) return RC_UNSTABLE;
     if (c.accept_step && !u_finite) return RC_UNSTABLE;
     return RC_SUCCESS;
 }
 
-// tstops cursor: the queued user tstops are P.tstops[tsp..tsend) (strictly
-// inside (t, tf)), tnext the first tstop, tf once they are used up.
+// Index of the first entry of the sorted list a[0..n) that is > x (strict) or
+// >= x (!strict).
+__device__ __forceinline__ int sorted_bound(const double* a, int n, double x, bool strict) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (strict ? (a[m] <= x) : (a[m] < x)) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+// tstops cursor over the shared ascending list: the queued user tstops are the
+// entries strictly inside (t, tf) in the integration direction (tsp stepping
+// by tdir up to tsend), tnext the first of them, tf once they are used up.
 __device__ __forceinline__ void tstops_init(Ctl& c, const Params& P, double t) {
-    int lo = 0, hi = P.ntstops;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (P.tstops[m] <= t) lo = m + 1; else hi = m; }
-    c.tsp = lo;
-    hi = P.ntstops;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (P.tstops[m] < c.tf) lo = m + 1; else hi = m; }
-    c.tsend = lo;
-    c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
+    if (c.tdir > 0) {
+        c.tsp = sorted_bound(P.tstops, P.ntstops, t, true);
+        c.tsend = sorted_bound(P.tstops, P.ntstops, c.tf, false);
+    } else {
+        c.tsp = sorted_bound(P.tstops, P.ntstops, t, false) - 1;
+        c.tsend = sorted_bound(P.tstops, P.ntstops, c.tf, true) - 1;
+    }
+    if (c.tdir > 0 ? (c.tsend < c.tsp) : (c.tsend > c.tsp)) c.tsend = c.tsp;
+    c.tnext = (c.tsp != c.tsend) ? P.tstops[c.tsp] : c.tf;
 }
 // handle_tstop! (integrator_utils.jl:1290-1314) after an accepted step: once t
 // reaches the first tstop every queued copy of it is popped (tf ends the loop).
 __device__ __forceinline__ void handle_tstop(Ctl& c, const Params& P) {
-    if (c.tsp < c.tsend && c.t == c.tnext) {
-        while (c.tsp < c.tsend && P.tstops[c.tsp] == c.t) c.tsp++;
-        c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
+    if (c.tsp != c.tsend && c.t == c.tnext) {
+        const int dir = (c.tdir > 0) ? 1 : -1;
+        while (c.tsp != c.tsend && P.tstops[c.tsp] == c.t) c.tsp += dir;
+        c.tnext = (c.tsp != c.tsend) ? P.tstops[c.tsp] : c.tf;
     }
 }
 
-// Saveat cursor setup: saveat_internal = sorted times in (t0, tf]
+// Saveat cursor setup (initialize_saveat, solve.jl:1103-1123): the entries of
+// the shared ascending list with tdir t0 < tdir s <= tdir tf, in the
+// integration direction (spos stepping by tdir up to send).
 __device__ __forceinline__ void saveat_init(Ctl& c, const Params& P, double t0) {
-    int lo = 0, hi = P.nsaveat;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (P.saveat[m] <= t0) lo = m + 1; else hi = m; }
-    c.spos = lo;
-    lo = c.spos; hi = P.nsaveat;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (P.saveat[m] <= c.tf) lo = m + 1; else hi = m; }
-    c.send = lo;
-    c.tf_in_saveat = (c.send > 0 && P.saveat[c.send - 1] == c.tf) ? 1 : 0;
+    if (c.tdir > 0) {
+        c.spos = sorted_bound(P.saveat, P.nsaveat, t0, true);
+        c.send = sorted_bound(P.saveat, P.nsaveat, c.tf, true);
+    } else {
+        c.spos = sorted_bound(P.saveat, P.nsaveat, t0, false) - 1;
+        c.send = sorted_bound(P.saveat, P.nsaveat, c.tf, false) - 1;
+    }
+    if (c.tdir > 0 ? (c.send < c.spos) : (c.send > c.spos)) c.send = c.spos;
+    const int k = sorted_bound(P.saveat, P.nsaveat, c.tf, false);
+    c.tf_in_saveat = (k < P.nsaveat && P.saveat[k] == c.tf) ? 1 : 0;
 }
 
 template <int N, typename V>
@@ -602,11 +635,11 @@ __device__ __forceinline__ double ode_norm(const double* x) {
 template <int N>
 __device__ double initdt_lane(const Ctl& c, const double* u0, const double* p, double reltol,
                               double abstol, double dtmax_opt, double dtmin_opt) {
-    const double t = c.t;
+    const double t = c.t, tdir = c.tdir;
     const double dtmax_tdir = jl_min(fabs(dtmax_opt), fabs(c.tnext - t));   // first tstop
     const double dtmin = nextfloat_pos(jl_max(dtmin_opt, jl_eps(t)));
     const double smalldt = jl_max(dtmin, 1e-6);
-    if (N == 0) return jl_max(smalldt, dtmin);
+    if (N == 0) return tdir * jl_max(smalldt, dtmin);
     double sk[N], f0[N], f1[N], tmp[N];
 #pragma unroll
     for (int i = 0; i < N; i++) sk[i] = fma(fabs(u0[i]), reltol, abstol);
@@ -617,17 +650,18 @@ __device__ double initdt_lane(const Ctl& c, const double* u0, const double* p, d
 #pragma unroll
     for (int i = 0; i < N; i++) tmp[i] = f0[i] / sk[i] * 1.0;
     double d1 = ode_norm<N>(tmp);
-    if (isnan(d1)) return dtmin;
+    if (isnan(d1)) return tdir * dtmin;
     double dt0 = ((d0 < 1e-5) | (d1 < 1e-5)) ? smalldt : (d0 / d1) / 100.0;
     dt0 = jl_min(dt0, dtmax_tdir);
-    if (dt0 < 10.0 * 2.220446049250313e-16) return jl_max(smalldt, dtmin);
+    if (dt0 < 10.0 * 2.220446049250313e-16) return tdir * jl_max(smalldt, dtmin);
+    const double dt0_tdir = tdir * dt0;
 #pragma unroll
-    for (int i = 0; i < N; i++) tmp[i] = fma(dt0, f0[i], u0[i]);
-    odemi_f<double>(f1, tmp, p, t + dt0);
+    for (int i = 0; i < N; i++) tmp[i] = fma(dt0_tdir, f0[i], u0[i]);
+    odemi_f<double>(f1, tmp, p, t + dt0_tdir);
     bool same = true;
 #pragma unroll
     for (int i = 0; i < N; i++) same = same && (f0[i] == f1[i]);
-    if (same) return jl_max(dtmin, 100.0 * dt0);
+    if (same) return tdir * jl_max(dtmin, 100.0 * dt0);
 #pragma unroll
     for (int i = 0; i < N; i++) tmp[i] = (f1[i] - f0[i]) / sk[i] * 1.0;
     double d2 = ode_norm<N>(tmp) / dt0 * 1.0;
@@ -635,14 +669,22 @@ __device__ double initdt_lane(const Ctl& c, const double* u0, const double* p, d
     double dt1;
     if (mx < 1e-15) dt1 = jl_max(1e-6, dt0 * 0.001);
     else dt1 = det_exp(-(2.0 + det_log(mx) / kLN10) / 5.0 * kLN10);
-    return jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
+    return tdir * jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
+}
+
+// handle_dt! (solve.jl:968-985): a positive user dt of a backward solve takes
+// tdir's sign.
+__device__ __forceinline__ double user_dt(const Ctl& c, const Params& P) {
+    return (P.dt_user > 0.0 && c.tdir < 0) ? c.tdir * P.dt_user : P.dt_user;
 }
 
 // Common per-trajectory initialisation of the controller state.
 __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     double t0 = P.t0[j], tf = P.tf[j];
     c.t = t0; c.tprev = t0; c.tf = tf;
-    c.dtmax = (P.dtmax > 0.0) ? P.dtmax : (tf - t0);
+    c.tdir = (tf < t0) ? -1.0 : 1.0;
+    // dtmax defaults to tf - t0; a positive one takes tdir's sign (solve.jl:401)
+    c.dtmax = (P.dtmax > 0.0) ? c.tdir * P.dtmax : (tf - t0);
     c.EEst = 1.0; c.q11 = 1.0; c.errold = 1e-4;
     c.iter = 0; c.success_iter = 0; c.accept_step = 0; c.next_step_tstop = 0; c.isout = 0;
     c.reeval_fsal = 0; c.ncond = 0; c.event_last = 0; c.sim_mask = 0u; c.last_event_error = 0.0;
@@ -719,7 +761,7 @@ __device__ __forceinline__ void tsit5_first_stage_dt(Tsit5Lane<S, K, N>& L, cons
     odemi_f<K>(L.k1, uk, L.p, (K)L.c.t);
     L.c.nf += 1;
     if (P.dt_user != 0.0) {  // user dt (a NaN dt is reported as DtNaN)
-        L.c.dt = P.dt_user;
+        L.c.dt = user_dt(L.c, P);
     } else {
         L.c.dt = initdt_lane<N>(L.c, u0d, pd, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
         L.c.nf += 2;
@@ -939,8 +981,9 @@ template <typename S, typename K, int N>
 __device__ __forceinline__ bool tsit5_savevalues(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
     Ctl& c = L.c;
     bool exact = false;
-    while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
-        double curt = P.saveat[c.spos++];
+    while (c.spos != c.send && c.tdir * P.saveat[c.spos] <= c.tdir * c.t) {
+        double curt = P.saveat[c.spos];
+        c.spos += (c.tdir > 0) ? 1 : -1;
         if (curt != c.t) {
             double th = (curt - c.tprev) / c.dt;
             double v[N];
@@ -1134,7 +1177,7 @@ __device__ bool ccb_apply(Lane& L, const Params& P, i64 j, double cb_t, double b
                           SaveFn& savevalues, bool& saved_in_cb) {
     typedef OdemiCcb<CB> C;
     Ctl& c = L.c;
-    c.dtpropose = jl_max(nextfloat_pos(P.dtmin), c.dt);
+    c.dtpropose = c.tdir * jl_max(nextfloat_pos(P.dtmin), c.tdir * c.dt);
     if (cb_t != c.t) {
         double v[N];
         lane_interp(L, (cb_t - c.tprev) / c.dt, v);
@@ -1247,8 +1290,10 @@ __device__ bool vcb_find_time(Lane& L, double& cb_t, double* ev, double& residua
         cb_t = top_t; residual = 0.0;
         return true;
     }
-    double cbt = __longlong_as_double(__double_as_longlong(top_t) + (top_t >= 0.0 ? 1 : -1)), res = 0.0;
-    if (top_t == 0.0) cbt = __longlong_as_double(1);   // nextfloat(0.0)
+    // rightfloat(top_t, tdir): nextfloat forward, prevfloat backward
+    double cbt = (L.c.tdir > 0) ? __longlong_as_double(__double_as_longlong(top_t) + (top_t >= 0.0 ? 1 : -1))
+                                : __longlong_as_double(__double_as_longlong(top_t) + (top_t > 0.0 ? -1 : 1)), res = 0.0;
+    if (top_t == 0.0) cbt = (L.c.tdir > 0) ? __longlong_as_double(1) : -__longlong_as_double(1);
 #pragma unroll
     for (int idx = 0; idx < V; idx++) {
         if (!((mask >> idx) & 1u)) continue;
@@ -1268,11 +1313,11 @@ __device__ bool vcb_find_time(Lane& L, double& cb_t, double* ev, double& residua
             }
             cbi = (C::rootfind == 1) ? r : l;
         }
-        if (cbi < cbt) {
+        if (L.c.tdir * cbi < L.c.tdir * cbt) {
 #pragma unroll
             for (int i = 0; i < V; i++) ev[i] = 0.0;
         }
-        if (cbi <= cbt) {
+        if (L.c.tdir * cbi <= L.c.tdir * cbt) {
             cbt = cbi;
             vcb_condition<CB, N>(L, cbi, tmp);
             res = tmp[idx];
@@ -1291,13 +1336,13 @@ __device__ __forceinline__ void ccb_find_first(Lane& L, int& win, double& tmin, 
         double cbt, bs = 0.0, rs;
         if constexpr (C::vlen > 0) {
             double ev[C::vlen];
-            if (vcb_find_time<CB, N>(L, cbt, ev, rs) && (win < 0 || cbt < tmin)) {
+            if (vcb_find_time<CB, N>(L, cbt, ev, rs) && (win < 0 || L.c.tdir * cbt < L.c.tdir * tmin)) {
                 win = CB; tmin = cbt; bs_win = bs; res_win = rs;
 #pragma unroll
                 for (int i = 0; i < C::vlen; i++) ev_win[i] = ev[i];
             }
         } else {
-            if (ccb_find_time<CB, N>(L, cbt, bs, rs) && (win < 0 || cbt < tmin)) {
+            if (ccb_find_time<CB, N>(L, cbt, bs, rs) && (win < 0 || L.c.tdir * cbt < L.c.tdir * tmin)) {
                 win = CB; tmin = cbt; bs_win = bs; res_win = rs;
 #pragma unroll
                 for (int i = 0; i < ODEMI_VCB_MAXLEN; i++) ev_win[i] = 0.0;
@@ -1344,7 +1389,7 @@ template <bool SAVES, typename S, typename K, int N>
 __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
     const CtlConst kc = ctl_const(false);
     Ctl& c = L.c;
-    if (!(c.t < c.tf)) return RC_SUCCESS;
+    if (!(c.tdir * c.t < c.tdir * c.tf)) return RC_SUCCESS;
     if (c.iter > 0) {
         if (c.accept_step) {
             c.success_iter++;
@@ -1406,7 +1451,7 @@ __device__ __forceinline__ int tsit5_attempt(Tsit5Lane<S, K, N>& L, const Params
     } else {
         c.nreject++;
     }
-    return (c.t < c.tf) ? 0 : RC_SUCCESS;
+    return (c.tdir * c.t < c.tdir * c.tf) ? 0 : RC_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------
@@ -1503,6 +1548,7 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
     c.t = s[0 * M + j]; c.dt = s[1 * M + j]; c.dtpropose = s[2 * M + j]; c.tprev = s[3 * M + j];
     c.last_event_error = s[4 * M + j];
     c.tf = s[5 * M + j]; c.dtmax = s[6 * M + j];
+    c.tdir = (c.tf < P.t0[j]) ? -1.0 : 1.0;
     c.EEst = s[7 * M + j]; c.q11 = s[8 * M + j]; c.errold = s[9 * M + j];
     const int* q = P.st_int;
     c.iter = q[0 * M + j]; c.success_iter = q[1 * M + j]; c.accept_step = q[2 * M + j];
@@ -1518,7 +1564,7 @@ __device__ __forceinline__ void plain_load_ctl(Ctl& c, const Params& P, i64 j) {
     if (P.ntstops > 0) {
         tstops_init(c, P, P.t0[j]);
         c.tsp = q[12 * M + j];
-        c.tnext = (c.tsp < c.tsend) ? P.tstops[c.tsp] : c.tf;
+        c.tnext = (c.tsp != c.tsend) ? P.tstops[c.tsp] : c.tf;
     }
     const int fl = q[13 * M + j];
     c.isout = fl & 1; c.reeval_fsal = (fl >> 1) & 1;
@@ -1700,7 +1746,7 @@ __device__ __forceinline__ void rodas_init(RodasLane<F, N>& L, const Params& P, 
         L.c.dt = P.pre_dt[j];
         if (P.dt_user == 0.0) L.c.nf += 2;
     } else if (P.dt_user != 0.0) {  // user dt (a NaN dt is reported as DtNaN)
-        L.c.dt = P.dt_user;
+        L.c.dt = user_dt(L.c, P);
     } else {
         L.c.dt = initdt_lane<N>(L.c, L.u, L.p, P.reltol, P.abstol, L.c.dtmax, P.dtmin);
         L.c.nf += 2;
@@ -1720,7 +1766,7 @@ __device__ void rodas_prep(const Params& P) {
     for (int i = 0; i < N; i++) u[i] = P.u0[(i64)i * P.N + j];
 #pragma unroll
     for (int k = 0; k < ODEMI_NP; k++) p[k] = (k < P.np) ? P.p[(i64)k * P.N + j] : 0.0;
-    P.pre_dt[j] = (P.dt_user != 0.0) ? P.dt_user : initdt_lane<N>(c, u, p, P.reltol, P.abstol, c.dtmax, P.dtmin);
+    P.pre_dt[j] = (P.dt_user != 0.0) ? user_dt(c, P) : initdt_lane<N>(c, u, p, P.reltol, P.abstol, c.dtmax, P.dtmin);
 }
 
 // W x = b with W held in FP64; returns x in b.
@@ -1873,8 +1919,9 @@ template <typename F, int N>
 __device__ __forceinline__ bool rodas_savevalues(RodasLane<F, N>& L, const Params& P, i64 j) {
     Ctl& c = L.c;
     bool exact = false;
-    while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
-        double curt = P.saveat[c.spos++];
+    while (c.spos != c.send && c.tdir * P.saveat[c.spos] <= c.tdir * c.t) {
+        double curt = P.saveat[c.spos];
+        c.spos += (c.tdir > 0) ? 1 : -1;
         if (curt != c.t) {
             double v[N];
             rodas_interp(L, (curt - c.tprev) / c.dt, v);
@@ -1973,7 +2020,7 @@ template <typename F, int N>
 __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P, i64 j) {
     const CtlConst kc = ctl_const(true);
     Ctl& c = L.c;
-    if (!(c.t < c.tf)) return RC_SUCCESS;
+    if (!(c.tdir * c.t < c.tdir * c.tf)) return RC_SUCCESS;
     if (c.iter > 0) {
         if (c.accept_step) {
             c.success_iter++;
@@ -2025,7 +2072,7 @@ __device__ __forceinline__ int rodas_attempt(RodasLane<F, N>& L, const Params& P
     } else {
         c.nreject++;
     }
-    return (c.t < c.tf) ? 0 : RC_SUCCESS;
+    return (c.tdir * c.t < c.tdir * c.tf) ? 0 : RC_SUCCESS;
 }
 
 template <typename F, int N>
@@ -2134,21 +2181,32 @@ __device__ __forceinline__ void tsit5_store_state(const Tsit5Lane<S, K, N>& L, c
         P.st_vec[((i64)3 * P.nmax + i) * M + j] = (double)L.k7[i];
     }
 }
+// A serving slot's own tolerances: its request's reltol / abstol, written by
+// the host into rows 10 / 11 of the pool's scalar state at admission.
+__device__ __forceinline__ Params slot_params(const Params& P, i64 j) {
+    Params Q = P;
+    Q.reltol = P.st_scalar[10 * P.N + j];
+    Q.abstol = P.st_scalar[11 * P.N + j];
+    return Q;
+}
+
 template <typename S, typename K, int N>
-__device__ void tsit5_serve_init(const Params& P) {
+__device__ void tsit5_serve_init(const Params& P0) {
     i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= P.nslots) return;
-    const i64 j = P.slot0 + q;
+    if (q >= P0.nslots) return;
+    const i64 j = P0.slot0 + q;
+    const Params P = slot_params(P0, j);
     Tsit5Lane<S, K, N> L;
     tsit5_init(L, P, j);
     plain_store_ctl<N>(L.c, P, j, 1);
     tsit5_store_state(L, P, j);
 }
 template <typename S, typename K, int N>
-__device__ void tsit5_serve_tick(const Params& P) {
+__device__ void tsit5_serve_tick(const Params& P0) {
     const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= P.N) return;
-    if (P.st_int[7 * P.N + j] != 1) return;
+    if (j >= P0.N) return;
+    if (P0.st_int[7 * P0.N + j] != 1) return;
+    const Params P = slot_params(P0, j);
     Tsit5Lane<S, K, N> L;
     plain_load_ctl<N>(L.c, P, j);
     const i64 M = P.N;
@@ -2175,10 +2233,11 @@ __device__ void tsit5_serve_tick(const Params& P) {
 }
 
 template <typename F, int N>
-__device__ void rodas_serve_init(const Params& P) {
+__device__ void rodas_serve_init(const Params& P0) {
     i64 q = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= P.nslots) return;
-    const i64 j = P.slot0 + q;
+    if (q >= P0.nslots) return;
+    const i64 j = P0.slot0 + q;
+    const Params P = slot_params(P0, j);
     RodasLane<F, N> L;
     rodas_init(L, P, j);
     plain_store_ctl<N>(L.c, P, j, 1);
@@ -2187,10 +2246,11 @@ __device__ void rodas_serve_init(const Params& P) {
     rodas_plain_store_vec(L, P, j);
 }
 template <typename F, int N>
-__device__ void rodas_serve_tick(const Params& P) {
+__device__ void rodas_serve_tick(const Params& P0) {
     const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= P.N) return;
-    if (P.st_int[7 * P.N + j] != 1) return;
+    if (j >= P0.N) return;
+    if (P0.st_int[7 * P0.N + j] != 1) return;
+    const Params P = slot_params(P0, j);
     RodasLane<F, N> L;
     plain_load_ctl<N>(L.c, P, j);
     const i64 M = P.N;

@@ -623,7 +623,7 @@ __device__ __forceinline__ double wg_norm_diff_div(const double* a, const double
 __device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, const double* p, int n,
                             double reltol, double abstol, double dtmin_opt) {
     const int tid = threadIdx.x, NT = blockDim.x;
-    const double t = c.t;
+    const double t = c.t, tdir = c.tdir;
     const double dtmax_tdir = jl_min(fabs(c.dtmax), fabs(c.tnext - t));   // first tstop
     const double dtmin = nextfloat_pos(jl_max(dtmin_opt, jl_eps(t)));
     const double smalldt = jl_max(dtmin, 1e-6);
@@ -640,11 +640,11 @@ __device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, con
         double d1 = wg_norm_div(f0, sk, n);
         double res = 0.0, need = 0.0, dt0 = 0.0;
         if (isnan(d1)) {
-            res = dtmin;
+            res = tdir * dtmin;
         } else {
             dt0 = ((d0 < 1e-5) | (d1 < 1e-5)) ? smalldt : (d0 / d1) / 100.0;
             dt0 = jl_min(dt0, dtmax_tdir);
-            if (dt0 < 10.0 * 2.220446049250313e-16) res = jl_max(smalldt, dtmin);
+            if (dt0 < 10.0 * 2.220446049250313e-16) res = tdir * jl_max(smalldt, dtmin);
             else need = 1.0;
         }
         red[20] = res; red[21] = need; red[22] = dt0; red[23] = d1;
@@ -655,16 +655,17 @@ __device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, con
     const double dt0 = red[22];
     __syncthreads();
     if (!need) return res;
-    for (int i = tid; i < n; i += NT) u1[i] = fma(dt0, f0[i], u0[i]);
+    const double dt0_tdir = tdir * dt0;
+    for (int i = tid; i < n; i += NT) u1[i] = fma(dt0_tdir, f0[i], u0[i]);
     __syncthreads();
-    odemi_f_wg<double>(f1, u1, p, t + dt0, n, tid, NT);
+    odemi_f_wg<double>(f1, u1, p, t + dt0_tdir, n, tid, NT);
     __syncthreads();
     if (tid == 0) {
         bool same = true;
         for (int i = 0; i < n; i++) same = same && (f0[i] == f1[i]);
         double r;
         if (same) {
-            r = jl_max(dtmin, 100.0 * dt0);
+            r = tdir * jl_max(dtmin, 100.0 * dt0);
         } else {
             double d1 = red[23];
             double d2 = wg_norm_diff_div(f1, f0, sk, n) / dt0 * 1.0;
@@ -672,7 +673,7 @@ __device__ double wg_initdt(const WGPtrs& S, const Ctl& c, const double* u0, con
             double dt1;
             if (mx < 1e-15) dt1 = jl_max(1e-6, dt0 * 0.001);
             else dt1 = det_exp(-(2.0 + det_log(mx) / kLN10) / 5.0 * kLN10);
-            r = jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
+            r = tdir * jl_max(dtmin, jl_min(jl_min(100.0 * dt0, dt1), dtmax_tdir));
         }
         red[20] = r;
     }
@@ -712,13 +713,332 @@ __device__ __forceinline__ void wg_begin(const WGPtrs& S, const Params& P, i64 j
     __syncthreads();
     if (P.save_start) wg_save_point(c, tlast, P, j, c.t, S.u, n);
     if (P.dt_user != 0.0) {
-        c.dt = P.dt_user;
+        c.dt = user_dt(c, P);
     } else {
         c.dt = wg_initdt(S, c, S.u, S.pd, n, P.reltol, P.abstol, P.dtmin);
         c.nf += 2;
     }
     c.dtpropose = c.dt;
 }
+
+#if defined(ODEMI_NDCB_WG) || defined(ODEMI_NCCB_WG)
+#define ODEMI_WG_CALLBACKS 1
+#endif
+#ifdef ODEMI_NCCB_WG
+#define ODEMI_WG_EVENTS 1
+#else
+#define ODEMI_WG_EVENTS 0
+#endif
+
+#ifdef ODEMI_WG_CALLBACKS
+// ---------------------------------------------------------------------------
+// Callbacks in the workgroup kernels (handle_callbacks!, integrator_utils.jl:
+// 1081-1132; lib/DiffEqBase/src/callbacks.jl:157-683), the same sequence as
+// the lane engines: continuous callbacks first (the earliest event wins, the
+// first callback on ties), then the discrete ones in order, then savevalues!
+// unless a callback already ran it.  Conditions and affects are traced per
+// state size (OdemiCcbWg<K>, OdemiDcbWg<K>) and run by thread 0 on the state in
+// LDS; every thread holds the same controller state, so the branches on their
+// results are uniform.
+// ---------------------------------------------------------------------------
+// Julia's sign(): -1, 0, 1, NaN for NaN
+__device__ __forceinline__ double wg_sign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : (x == 0 ? 0.0 : x)); }
+
+// The Rosenbrock dense output of the accepted step at theta into v (all
+// threads; rosenbrock_interpolants.jl:172-206, the form of the saveat path).
+__device__ __forceinline__ void wg_interp(const WGPtrs& S, int n, int nmax, double th, double* v) {
+    const double th1 = 1.0 - th;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double in1 = fma(th, S.dense[2 * nmax + i], S.dense[nmax + i]);
+        const double in2 = fma(th, in1, S.dense[i]);
+        const double in3 = fma(th1, in2, S.u[i]);
+        v[i] = fma(th, in3, th1 * S.uprev[i]);
+    }
+    __syncthreads();
+}
+
+#ifdef ODEMI_NCCB_WG
+// get_condition (callbacks.jl:91-146): u at t, uprev at tprev, else the dense
+// output (into S.du, free after the step); every call counts in ncondition.
+template <int CB>
+__device__ double wg_ccb_condition(const WGPtrs& S, Ctl& c, int n, int nmax, double abst) {
+    c.ncond++;
+    const double* x = S.u;
+    if (abst != c.t) {
+        if (abst == c.tprev) {
+            x = S.uprev;
+        } else {
+            wg_interp(S, n, nmax, (abst - c.tprev) / c.dt, S.du);
+            x = S.du;
+        }
+    }
+    if (threadIdx.x == 0) S.red[27] = (double)OdemiCcbWg<CB>::template cond<double>(x, S.pd, abst, n);
+    __syncthreads();
+    const double r = S.red[27];
+    __syncthreads();
+    return r;
+}
+
+template <int CB>
+__device__ __forceinline__ bool wg_ccb_is_event(double prev_sign, double next_sign) {
+    return ((prev_sign < 0 && OdemiCcbWg<CB>::has_affect) || (prev_sign > 0 && OdemiCcbWg<CB>::has_affect_neg)) &&
+           prev_sign * next_sign <= 0;
+}
+
+// find_callback_time (callbacks.jl:364-412) with nudge_tprev (414-426) and the
+// interp_points checks; the root bracketed down to adjacent floats by
+// bisection (as the lane engines and the restatement do).
+template <int CB>
+__device__ bool wg_ccb_find_time(const WGPtrs& S, Ctl& c, int n, int nmax, double& cb_t, double& bottom_sign,
+                                 double& residual) {
+    typedef OdemiCcbWg<CB> C;
+    double bottom_t = c.tprev;
+    double bc = wg_ccb_condition<CB>(S, c, n, nmax, bottom_t);
+    if (c.event_last == CB + 1) {
+        if (fabs(bc - c.last_event_error) <= C::abstol) bottom_t = c.tprev + c.dt * C::repeat_nudge;
+        bc = wg_ccb_condition<CB>(S, c, n, nmax, bottom_t);
+    }
+    const double bs = wg_sign(bc);
+    double top_t = c.t;
+    double ts = wg_sign(wg_ccb_condition<CB>(S, c, n, nmax, top_t));
+    bool ev = wg_ccb_is_event<CB>(bs, ts);
+    if (C::interp_points != 0 && !ev) {
+        const double h = (c.t - c.tprev) / (double)(C::interp_points - 1);
+        for (int i = 2; i <= C::interp_points; i++) {
+            top_t = (i == C::interp_points) ? c.t : fma((double)(i - 1), h, c.tprev);
+            ts = wg_sign(wg_ccb_condition<CB>(S, c, n, nmax, top_t));
+            ev = wg_ccb_is_event<CB>(bs, ts);
+            if (ev) break;
+        }
+    }
+    bottom_sign = bs;
+    if (!ev) { cb_t = c.t; residual = 0.0; return false; }
+    if (C::rootfind == 2 || ts == 0.0) { cb_t = top_t; residual = 0.0; return true; }
+    double l = bottom_t, r = top_t;
+    for (int it = 0; it < 2200; it++) {
+        const double m = l + 0.5 * (r - l);
+        if (m == l || m == r) break;
+        const double sm = wg_sign(wg_ccb_condition<CB>(S, c, n, nmax, m));
+        if (sm == 0.0) { l = m; r = m; break; }
+        if (sm == bs) l = m;
+        else r = m;
+    }
+    cb_t = (C::rootfind == 1) ? r : l;
+    residual = wg_ccb_condition<CB>(S, c, n, nmax, cb_t);
+    return true;
+}
+// _ode_addsteps! for RosenbrockCache (stiff_addsteps.jl:140-224) after
+// change_t_via_interpolation!, in the lane engines' form: the stages of the
+// step from uprev at tprev with the new dt, the step's dT and fsalfirst
+// f(uprev, tprev) (re-evaluated), W = J - I/(dt gamma) from J at (uprev,
+// tprev) (the step's Jacobian: FP64 re-evaluated into LDS, or the FP32 block
+// of this attempt for Gauss-Jordan) refactored; no @muladd there, so products
+// and sums round separately except linsolve_tmp = fsalfirst + dtgamma dT;
+// dtC = C ./ dt; the dense rows from the new stages.  The dense rows serve as
+// solve scratch until they are rebuilt.  No stats change.
+template <typename F, bool GJ, int R, typename GT>
+__device__ void wg_addsteps(const WGPtrs& S, const Params& P, const Ctl& c, int n, GT& G, float* Jg) {
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int nmax = P.nmax, ld = S.ld;
+    const double t = c.tprev, dt = c.dt;
+    const double dtgamma = dt * RodasTab::gamma;
+    const double sigma = 1.0 / dtgamma;
+    double* ks = S.ks;
+    double* sc0 = S.dense;
+    double* sc1 = S.dense + nmax;
+    double* sc2 = S.dense + 2 * nmax;
+    (void)G; (void)Jg; (void)ld;
+    odemi_f_wg<double>(S.fsal, S.uprev, S.pd, t, n, tid, NT);
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) ks[i] = fma(dtgamma, S.dT[i], S.fsal[i]);
+    if constexpr (GJ) {
+#if ODEMI_GJ
+        gj_load<R>(G, Jg, nmax, n, sigma);
+        gj_invert<R>(G, n, S.perm, S.gjbuf);
+#endif
+    } else {
+        F* W = (F*)S.W;
+        for (int e = tid; e < n * ld; e += NT) W[e] = (F)0;
+        if (sizeof(F) != sizeof(double))
+            for (int i = tid; i < n; i += NT) S.fsc[i] = (float)S.uprev[i];
+        __syncthreads();
+        if constexpr (sizeof(F) == sizeof(double)) odemi_jac_wg<double>((double*)W, ld, S.uprev, S.pd, t, n, tid, NT);
+        else odemi_jac_wg<float>((float*)W, ld, S.fsc, S.pf, (float)t, n, tid, NT);
+        __syncthreads();
+        for (int i = tid; i < n; i += NT) W[i * ld + i] = (F)fma(-1.0, sigma, (double)W[i * ld + i]);
+        __syncthreads();
+        wg_lu<F>(W, ld, n, S.piv, S.perm, S.red, S.ired);
+    }
+    __syncthreads();
+    auto solve = [&](double* x) {
+        if constexpr (GJ) {
+#if ODEMI_GJ
+            wg_wsolve_gj<R>(G, S, n, x, sc0, sc1, sc2, S.uprev, S.pd, t, sigma, P.refine_sweeps);
+#endif
+        } else {
+            wg_wsolve<F>(S, n, x, sc0, sc1, sc2, S.uprev, S.pd, t, sigma, P.refine_sweeps);
+        }
+        for (int i = tid; i < n; i += NT) x[i] = -x[i];
+        __syncthreads();
+    };
+    solve(ks);
+    for (int s = 1; s < 8; s++) {
+        for (int i = tid; i < n; i += NT) {
+            double acc = S.uprev[i];
+            for (int q = 0; q < s; q++) acc = acc + kRA[s][q] * ks[q * nmax + i];
+            S.fsal[i] = acc;
+        }
+        __syncthreads();
+        odemi_f_wg<double>(S.du, S.fsal, S.pd, t + kRc[s] * dt, n, tid, NT);
+        __syncthreads();
+        const double dtds = dt * kRd[s];
+        double* x = ks + s * nmax;
+        for (int i = tid; i < n; i += NT) {
+            double acc = S.du[i] + dtds * S.dT[i];
+            for (int q = 0; q < s; q++) acc = acc + (kRC[s][q] / dt) * ks[q * nmax + i];
+            x[i] = acc;
+        }
+        __syncthreads();
+        solve(x);
+    }
+    for (int r = 0; r < 3; r++)
+        for (int i = tid; i < n; i += NT) {
+            double acc = 0.0;
+            for (int q = 0; q < 8; q++) acc = acc + kRH[r][q] * ks[q * nmax + i];
+            S.dense[r * nmax + i] = acc;
+        }
+    __syncthreads();
+}
+
+// apply_callback! (callbacks.jl:557-640): set_proposed_dt!, the state moved to
+// the event time through the dense output with the stages recomputed, saves,
+// affect! on an upcrossing / affect_neg! on a downcrossing.  Returns terminate!.
+template <int CB, typename F, bool GJ, int R, typename GT, typename SaveFn>
+__device__ bool wg_ccb_apply(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double& tlast, double cb_t,
+                             double bs, GT& G, float* Jg, SaveFn& savevalues, bool& saved_in_cb) {
+    typedef OdemiCcbWg<CB> C;
+    const int tid = threadIdx.x, nmax = P.nmax;
+    c.dtpropose = c.tdir * jl_max(nextfloat_pos(P.dtmin), c.tdir * c.dt);
+    if (cb_t != c.t) {
+        wg_interp(S, n, nmax, (cb_t - c.tprev) / c.dt, S.du);
+        for (int i = tid; i < n; i += blockDim.x) S.u[i] = S.du[i];
+        __syncthreads();
+        c.t = cb_t;
+        c.dt = c.t - c.tprev;
+        wg_addsteps<F, GJ, R>(S, P, c, n, G, Jg);
+        c.reeval_fsal = 1;
+    }
+    const bool savedexactly = savevalues();
+    saved_in_cb = true;
+    if (C::save0 && !savedexactly) wg_save_point(c, tlast, P, j, c.t, S.u, n);
+    __syncthreads();
+    bool modified = true, term = false;
+    if (bs < 0) {
+        if constexpr (C::has_affect != 0) {
+            if (tid == 0) C::template affect<double>(S.u, S.pd, c.t, n);
+            term = C::term != 0;
+        } else {
+            modified = false;
+        }
+    } else if (bs > 0) {
+        if constexpr (C::has_affect_neg != 0) {
+            if (tid == 0) C::template affect_neg<double>(S.u, S.pd, c.t, n);
+            term = C::term_neg != 0;
+        } else {
+            modified = false;
+        }
+    }
+    __syncthreads();
+    if (modified) {
+        c.reeval_fsal = 1;
+        if (C::save1) { wg_save_point(c, tlast, P, j, c.t, S.u, n); saved_in_cb = true; }
+    }
+    return term;
+}
+
+template <int CB>
+__device__ __forceinline__ void wg_ccb_find_first(const WGPtrs& S, Ctl& c, int n, int nmax, int& win, double& tmin,
+                                                  double& bs_win, double& res_win) {
+    if constexpr (CB < ODEMI_NCCB_WG) {
+        double cbt, bs = 0.0, rs;
+        if (wg_ccb_find_time<CB>(S, c, n, nmax, cbt, bs, rs) && (win < 0 || c.tdir * cbt < c.tdir * tmin)) {
+            win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+        }
+        wg_ccb_find_first<CB + 1>(S, c, n, nmax, win, tmin, bs_win, res_win);
+    }
+}
+template <int CB, typename F, bool GJ, int R, typename GT, typename SaveFn>
+__device__ __forceinline__ bool wg_ccb_apply_winner(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c,
+                                                    double& tlast, int win, double cbt, double bs, GT& G, float* Jg,
+                                                    SaveFn& savevalues, bool& saved_in_cb) {
+    if constexpr (CB < ODEMI_NCCB_WG) {
+        if (win == CB) return wg_ccb_apply<CB, F, GJ, R>(S, P, j, n, c, tlast, cbt, bs, G, Jg, savevalues, saved_in_cb);
+        return wg_ccb_apply_winner<CB + 1, F, GJ, R>(S, P, j, n, c, tlast, win, cbt, bs, G, Jg, savevalues,
+                                                     saved_in_cb);
+    } else {
+        return false;
+    }
+}
+#endif  // ODEMI_NCCB_WG
+
+#ifdef ODEMI_NDCB_WG
+// apply_discrete_callback! (callbacks.jl:645-683) for each DiscreteCallback in
+// order, as the lane engines do.
+template <int K, typename SaveFn>
+__device__ __forceinline__ void wg_dcb_apply(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double& tlast,
+                                             SaveFn& savevalues, bool& saved_in_cb, bool& term) {
+    if constexpr (K < ODEMI_NDCB_WG) {
+        typedef OdemiDcbWg<K> D;
+        c.ncond++;
+        if (threadIdx.x == 0) S.red[26] = D::template cond<double>(S.u, S.pd, c.t, n) ? 1.0 : 0.0;
+        __syncthreads();
+        const bool fire = S.red[26] != 0.0;
+        __syncthreads();
+        if (fire) {
+            const bool savedexactly = savevalues();
+            saved_in_cb = true;
+            if (D::save0 && !savedexactly) wg_save_point(c, tlast, P, j, c.t, S.u, n);
+            __syncthreads();
+            if (threadIdx.x == 0) D::template affect<double>(S.u, S.pd, c.t, n);
+            __syncthreads();
+            c.reeval_fsal = 1;
+            if (D::term) term = true;
+            if (D::save1) wg_save_point(c, tlast, P, j, c.t, S.u, n);
+        }
+        wg_dcb_apply<K + 1>(S, P, j, n, c, tlast, savevalues, saved_in_cb, term);
+    }
+}
+#endif
+
+// handle_callbacks!: returns terminate!.
+template <typename F, bool GJ, int R, typename GT, typename SaveFn>
+__device__ __forceinline__ bool wg_handle_callbacks(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c,
+                                                    double& tlast, GT& G, float* Jg, SaveFn& savevalues) {
+    bool saved_in_cb = false, term = false;
+    (void)G; (void)Jg;
+#ifdef ODEMI_NCCB_WG
+    {
+        int win = -1;
+        double tmin = 0.0, bs = 0.0, rs = 0.0;
+        wg_ccb_find_first<0>(S, c, n, P.nmax, win, tmin, bs, rs);
+        if (win < 0) {
+            c.event_last = 0;
+        } else {
+            c.last_event_error = rs;
+            c.event_last = win + 1;
+            c.sim_mask = 0u;
+            term = wg_ccb_apply_winner<0, F, GJ, R>(S, P, j, n, c, tlast, win, tmin, bs, G, Jg, savevalues,
+                                                    saved_in_cb);
+        }
+    }
+#endif
+#ifdef ODEMI_NDCB_WG
+    wg_dcb_apply<0>(S, P, j, n, c, tlast, savevalues, saved_in_cb, term);
+#endif
+    if (!saved_in_cb) savevalues();
+    return term;
+}
+#endif  // ODEMI_WG_CALLBACKS
 
 // Step attempts until the trajectory ends (returns its ReturnCode) or `limit`
 // attempts have been made (returns 0; the state is in c and S.u/uprev/dT).
@@ -745,7 +1065,7 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
     GJTile<R> G;
 #endif
     int rc = RC_SUCCESS;
-    for (int att = 0; c.t < c.tf; att++) {
+    for (int att = 0; c.tdir * c.t < c.tdir * c.tf; att++) {
         if (att == limit) return 0;   // uniform: every thread holds the same controller state
         // ---- loopheader! ----
         if (c.iter > 0) {
@@ -864,7 +1184,7 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
                 e = fma(1.0, ks[7 * nmax + i], e);
                 du[i] = e;
             }
-            if (SAVES && P.nsaveat > 0) {
+            if ((SAVES && P.nsaveat > 0) || ODEMI_WG_EVENTS) {   // the event search interpolates every step
                 for (int r = 0; r < 3; r++)
                     for (int i = tid; i < n; i += NT) {
                         double acc = 0.0;
@@ -901,11 +1221,14 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
             c.tprev = c.t;
             if (c.next_step_tstop) { c.dt = c.dtpropose; c.next_step_tstop = 0; c.t = c.tnext; }
             else c.t = ttmp;
-            handle_tstop(c, P);
-            accept_controller(c, kc, q, P.dtmin);
-            if (SAVES) {
-                while (c.spos < c.send && P.saveat[c.spos] <= c.t) {
-                    const double curt = P.saveat[c.spos++];
+            // savevalues!: saveat points up to t, save_everystep; true when it
+            // saved at exactly t
+            auto savevalues = [&]() -> bool {
+                bool exact = false;
+                if (!SAVES) return exact;
+                while (c.spos != c.send && c.tdir * P.saveat[c.spos] <= c.tdir * c.t) {
+                    const double curt = P.saveat[c.spos];
+                    c.spos += (c.tdir > 0) ? 1 : -1;
                     if (curt != c.t) {
                         const double th = (curt - c.tprev) / c.dt, th1 = 1.0 - th;
                         const int sidx = c.nsaved;
@@ -923,13 +1246,34 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
                     } else {
                         if (curt == c.tf && !P.save_end) continue;
                         wg_save_point(c, tlast, P, j, c.t, u, n);
+                        exact = true;
                     }
                 }
                 if (P.save_everystep) {
-                    if (c.nsaved == 0 || ((c.t != tlast || c.dt == 0.0) && (P.save_end || c.t != c.tf)))
+                    if (c.nsaved == 0 || ((c.t != tlast || c.dt == 0.0) && (P.save_end || c.t != c.tf))) {
                         wg_save_point(c, tlast, P, j, c.t, u, n);
+                        exact = true;
+                    }
                 }
-            }
+                return exact;
+            };
+#ifdef ODEMI_WG_CALLBACKS
+            // the lane engines' order: controller, callbacks (which run
+            // savevalues!), then the tstop pop (an event can move t back)
+            accept_controller(c, kc, q, P.dtmin);
+#if ODEMI_GJ
+            const bool term = wg_handle_callbacks<F, GJ, R>(S, P, j, n, c, tlast, G, Jg, savevalues);
+#else
+            int G0 = 0;
+            const bool term = wg_handle_callbacks<F, GJ, R>(S, P, j, n, c, tlast, G0, Jg, savevalues);
+#endif
+            handle_tstop(c, P);
+            if (term) { rc = RC_TERMINATED; break; }
+#else
+            handle_tstop(c, P);
+            accept_controller(c, kc, q, P.dtmin);
+            savevalues();
+#endif
         } else {
             c.nreject++;
         }
@@ -944,7 +1288,7 @@ __device__ __forceinline__ void wg_finish(const WGPtrs& S, const Params& P, i64 
     const int nmax = P.nmax;
     const double* u = S.u;
     // ---- postamble! / result write-out ----
-    if (rc == RC_SUCCESS && P.save_end) {
+    if ((rc == RC_SUCCESS || rc == RC_TERMINATED) && P.save_end) {   // postamble! runs after terminate! too
         bool doit = (c.nsaved == 0);
         if (!doit)
             doit = (tlast != c.t) && (P.save_end_user == 1 || c.tf_in_saveat || c.t == c.tf || P.nsaveat == 0);
@@ -994,7 +1338,7 @@ __device__ __forceinline__ void rodas_wg_impl(const Params& P) {
     static_assert(!GJ || (sizeof(F) == 4 && ODEMI_GJ), "Gauss-Jordan variant: FP32, n <= 128");
     extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
     const int tid = threadIdx.x;
-    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES, !GJ);
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES || ODEMI_WG_EVENTS, !GJ);
     float* Jg = nullptr;
 #if ODEMI_GJ
     if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
@@ -1050,7 +1394,7 @@ __device__ __forceinline__ void wg_load_state(const WGPtrs& S, const Params& P, 
 template <typename F, bool GJ>
 __device__ __forceinline__ void rodas_wg_serve_init(const Params& P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
-    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, false, !GJ);
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, ODEMI_WG_EVENTS != 0, !GJ);
     float* Jg = nullptr;
 #if ODEMI_GJ
     if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
@@ -1058,10 +1402,11 @@ __device__ __forceinline__ void rodas_wg_serve_init(const Params& P) {
     for (i64 q = blockIdx.x; q < P.nslots; q += gridDim.x) {
         const i64 j = P.slot0 + q;
         const int n = P.nvec[j];
+        const Params Q = slot_params(P, j);
         Ctl c;
         double tlast;
-        wg_begin<GJ>(S, P, j, n, c, tlast, Jg);
-        wg_store_state(S, P, j, n, c, 1);
+        wg_begin<GJ>(S, Q, j, n, c, tlast, Jg);
+        wg_store_state(S, Q, j, n, c, 1);
         __syncthreads();
     }
 }
@@ -1072,7 +1417,7 @@ template <typename F, bool GJ>
 __device__ __forceinline__ void rodas_wg_serve_tick(const Params& P) {
     extern __shared__ __attribute__((aligned(16))) unsigned char odemi_lds[];
     const int tid = threadIdx.x;
-    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, false, !GJ);
+    const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, ODEMI_WG_EVENTS != 0, !GJ);
     float* Jg = nullptr;
 #if ODEMI_GJ
     if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
@@ -1085,24 +1430,25 @@ __device__ __forceinline__ void rodas_wg_serve_tick(const Params& P) {
         if (j >= P.nslots) break;
         if (P.st_int[7 * P.N + j] != 1) continue;
         const int n = P.nvec[j];
-        wg_load_params(S, P, j);
+        const Params Q = slot_params(P, j);
+        wg_load_params(S, Q, j);
         Ctl c;
-        wg_load_state(S, P, j, n, c);
+        wg_load_state(S, Q, j, n, c);
 #if ODEMI_GJ
         if constexpr (GJ)
             for (int e = tid; e < n * P.nmax; e += blockDim.x) Jg[e] = 0.0f;
 #endif
         __syncthreads();
         double tlast = c.t;
-        const int rc = wg_run_tiled<F, false, GJ>(S, P, j, n, c, tlast, Jg, P.tick_attempts);
+        const int rc = wg_run_tiled<F, false, GJ>(S, Q, j, n, c, tlast, Jg, P.tick_attempts);
         if (rc) {
-            wg_finish<false>(S, P, j, n, c, tlast, rc);
+            wg_finish<false>(S, Q, j, n, c, tlast, rc);
             if (tid == 0) {
                 P.st_int[7 * P.N + j] = 0;
                 atomicAdd(&P.req_done[P.slot_req[j]], 1ULL);
             }
         } else {
-            wg_store_state(S, P, j, n, c, 1);
+            wg_store_state(S, Q, j, n, c, 1);
         }
         __syncthreads();
     }

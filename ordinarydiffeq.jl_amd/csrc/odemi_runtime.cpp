@@ -26,6 +26,8 @@
 #include <numeric>
 #include <sstream>
 #include <string>
+#include <climits>
+#include <dlfcn.h>
 #include <sys/stat.h>
 #include <unistd.h>
 #include <vector>
@@ -43,6 +45,10 @@ const char* kDeviceSource =
 constexpr size_t kIntRows = 16;
 // DEStats rows written per trajectory: nf naccept nreject njacs nw nsolve ncondition
 constexpr size_t kStatRows = 7;
+// odemi_features bits: the lane kernels' per-step hooks (isoutofdomain,
+// discrete, continuous callbacks) and the workgroup kernels' callbacks
+constexpr int kLaneHooks = 1 | 2 | 4;
+constexpr int kWgCallbacks = 8 | 16;
 
 struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t N;
@@ -156,6 +162,15 @@ __global__ void __launch_bounds__(256) cost_scatter_kernel(const double* t0, con
     }
 }
 
+// Word fill on the device, used instead of hipMemsetAsync where the stream is
+// captured into a graph.
+__global__ void __launch_bounds__(256) fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
+}
+__global__ void __launch_bounds__(256) fill_f64_kernel(double* p, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
+}
+
 uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL) {
     for (unsigned char c : s) { h ^= c; h *= 1099511628211ULL; }
     return h;
@@ -192,13 +207,14 @@ struct odemi_ctx {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_order = nullptr;   // ordering against the legacy default stream (no timing)
 };
 
 struct odemi_rhs {
     odemi_ctx* ctx = nullptr;
     hipModule_t mod = nullptr;
     int n = 0, np = 0, flags = 0;
-    int features = 0;   // 1: isoutofdomain, 2: discrete, 4: continuous callbacks (lane kernels only)
+    int features = 0;   // lane hooks 1 isoutofdomain, 2 discrete, 4 continuous callbacks; workgroup callbacks 8, 16
     hipFunction_t tsit5_persist_f64 = nullptr, tsit5_persist_mixed = nullptr;
     hipFunction_t tsit5_persist_f64_sv = nullptr, tsit5_persist_mixed_sv = nullptr;
     hipFunction_t tsit5_plain_init = nullptr, tsit5_plain_step = nullptr;
@@ -231,6 +247,7 @@ extern "C" odemi_ctx* odemi_create(int device) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) c->err = "stream create failed";
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
+    (void)hipEventCreateWithFlags(&c->ev_order, hipEventDisableTiming);
     return c;
 }
 
@@ -240,6 +257,7 @@ extern "C" void odemi_destroy(odemi_ctx* c) {
     if (c->ws) (void)hipFree(c->ws);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_order) (void)hipEventDestroy(c->ev_order);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -304,8 +322,10 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     // layout check of the kernel argument struct against this file's mirror
     s << "extern \"C\" __device__ const unsigned odemi_params_bytes = sizeof(odemi::Params);\n";
     // per-step hooks of the problem source, which only the lane kernels run
+    // (and the workgroup kernels' callbacks: 8 discrete, 16 continuous)
     s << "extern \"C\" __device__ const int odemi_features = 0\n#ifdef ODEMI_HAS_ISOUT\n | 1\n#endif\n"
-         "#ifdef ODEMI_NDCB\n | 2\n#endif\n#ifdef ODEMI_NCCB\n | 4\n#endif\n;\n";
+         "#ifdef ODEMI_NDCB\n | 2\n#endif\n#ifdef ODEMI_NCCB\n | 4\n#endif\n"
+         "#ifdef ODEMI_NDCB_WG\n | 8\n#endif\n#ifdef ODEMI_NCCB_WG\n | 16\n#endif\n;\n";
     if (flags & ODEMI_COMPILE_TSIT5) {
         // persistent kernels: final-state-only (no save path) and saving variants
         const char* v[4][3] = {{"odemi_tsit5_persist_f64", "false", "double"},
@@ -415,6 +435,15 @@ static bool rtc_compile(const std::string& src, std::vector<char>& code, std::st
     return true;
 }
 
+// The file of the hipRTC library this process compiles with (two ROCm builds
+// report the same hiprtcVersion, so the cache key also names the library).
+static std::string rtc_library_path() {
+    Dl_info info;
+    if (dladdr((void*)&hiprtcVersion, &info) == 0 || !info.dli_fname) return "unknown";
+    char buf[PATH_MAX];
+    return realpath(info.dli_fname, buf) ? std::string(buf) : std::string(info.dli_fname);
+}
+
 extern "C" int odemi_rtc_version(int* major, int* minor) {
     int a = 0, b = 0;
     if (hiprtcVersion(&a, &b) != HIPRTC_SUCCESS) return -1;
@@ -435,7 +464,7 @@ extern "C" int odemi_rtc_codeobject(const char* problem_src, int n, int np, int 
     for (const char* o : kRtcOpts) key += o;
     int vmaj = 0, vmin = 0;
     if (hiprtcVersion(&vmaj, &vmin) != HIPRTC_SUCCESS) vmaj = vmin = -1;
-    key += "|hiprtc " + std::to_string(vmaj) + "." + std::to_string(vmin);
+    key += "|hiprtc " + std::to_string(vmaj) + "." + std::to_string(vmin) + " " + rtc_library_path();
     char hex[32];
     snprintf(hex, sizeof(hex), "%016llx", (unsigned long long)fnv1a(key));
     std::string dir = cache_dir();
@@ -603,9 +632,22 @@ int launch(odemi_ctx* c, hipFunction_t fn, int grid, int block, Params& P, hipSt
     return 0;
 }
 
-}  // namespace
+// Zero `words` 32-bit words: hipMemsetAsync, or (where the stream is captured
+// into a graph) the fill kernel.
+int clear_words(odemi_ctx* c, void* p, size_t words, hipStream_t s, bool kernel) {
+    if (words == 0) return 0;
+    if (!kernel) {
+        HIPCHK(c, hipMemsetAsync(p, 0, words * 4, s));
+        return 0;
+    }
+    const int64_t g = std::min<int64_t>(((int64_t)words + 255) / 256, 1024);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)g), dim3(256), 0, s, (unsigned*)p, (int64_t)words, 0u);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
 
-extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
+// One ensemble solve.
+int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
     if (!ctx || !rhs || !a) return -1;
     std::lock_guard<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
@@ -630,6 +672,9 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     const bool plain = (a->engine == ODEMI_ENGINE_PLAIN);
     const bool mixed = (a->precision == ODEMI_MIXED);
     const bool use_wg = !plain && rodas && (rhs->flags & ODEMI_COMPILE_WG);
+    // plain engine of a workgroup problem: the serving pool's workgroup kernels
+    // with one step attempt per launch, state in HBM between launches (FP64 LU)
+    const bool plain_wg = plain && rodas && (rhs->flags & ODEMI_COMPILE_WG);
     hipFunction_t fn_main = nullptr, fn_init = nullptr, fn_prep = nullptr;
     if (!rodas) {
         if (plain) { fn_init = rhs->tsit5_plain_init; fn_main = rhs->tsit5_plain_step; }
@@ -637,6 +682,12 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
             fn_main = mixed ? rhs->tsit5_persist_mixed_sv : rhs->tsit5_persist_f64_sv;
         else fn_main = mixed ? rhs->tsit5_persist_mixed : rhs->tsit5_persist_f64;
         if (!plain) fn_prep = rhs->tsit5_prep[mixed ? 1 : 0];
+    } else if (plain_wg) {
+        fn_init = rhs->wg_serve_init[0]; fn_main = rhs->wg_serve_tick[0];
+        if (a->nsaveat > 0 || a->save_everystep) {
+            ctx->err = "the plain engine of uneven-size batches saves the start and end states only (no saveat, save_everystep)";
+            return -3;
+        }
     } else if (use_wg) {
         if (a->nsaveat > 0 || a->save_everystep) fn_main = mixed ? rhs->rodas_wg_mixed_sv : rhs->rodas_wg_f64_sv;
         else fn_main = mixed ? rhs->rodas_wg_mixed : rhs->rodas_wg_f64;
@@ -647,17 +698,30 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     }
     if (plain && mixed) { ctx->err = "the plain engine is FP64 only"; return -3; }
     if (!fn_main || (plain && !fn_init)) { ctx->err = "requested kernel was not compiled for this problem"; return -3; }
-    if (a->nvec && !use_wg) { ctx->err = "uneven batches (nvec) need the workgroup Rodas5P kernels"; return -3; }
-    if (use_wg && rhs->features) {
-        ctx->err = "isoutofdomain and callbacks run in the lane kernels only (compile without ODEMI_COMPILE_WG)";
+    if (a->nvec && !use_wg && !plain_wg) { ctx->err = "uneven batches (nvec) need the workgroup Rodas5P kernels"; return -3; }
+    if ((use_wg || plain_wg) && (rhs->features & kLaneHooks)) {
+        ctx->err = "the lane kernels' isoutofdomain and callback forms need a lane problem (compile without ODEMI_COMPILE_WG)";
+        return -3;
+    }
+    if (plain_wg && (rhs->features & kWgCallbacks)) {
+        ctx->err = "callbacks in uneven-size batches run in the persistent engine only";
         return -3;
     }
 
     hipStream_t s = a->stream ? (hipStream_t)a->stream : ctx->stream;
     const bool dev = a->device_ptrs != 0;
+    // Device arrays with no stream given: they may come from work on the legacy
+    // default stream, which the ctx stream (non-blocking) does not wait for.
+    const bool order_null = dev && !a->stream;
+    if (order_null) {
+        HIPCHK(ctx, hipEventRecord(ctx->ev_order, nullptr));
+        HIPCHK(ctx, hipStreamWaitEvent(s, ctx->ev_order, 0));
+    }
     const size_t N_ = (size_t)N;
     const size_t ms_ = (size_t)std::max(a->max_saves, 0);
-    const size_t vrows = rodas ? (size_t)(3 + nmax) : 4;
+    // plain-engine vector rows: Tsit5 u, uprev, k1, k7; Rodas5P lane u, uprev, dT, J;
+    // workgroup u, uprev, dT
+    const size_t vrows = plain_wg ? 3 : rodas ? (size_t)(3 + nmax) : 4;
     // ---- launch geometry of the persistent kernels ----
     const int block = a->block > 0 ? a->block : 256;
     const int threads = use_wg ? 256 : block;   // ODEMI_WG_THREADS for the workgroup kernels
@@ -666,7 +730,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     int64_t grid = 0;
     if (!plain) {
         if (use_wg) {
-            shmem = wg_shmem_bytes(nmax, mixed, a->nsaveat > 0 || a->save_everystep, use_gj);
+            shmem = wg_shmem_bytes(nmax, mixed, a->nsaveat > 0 || a->save_everystep || (rhs->features & 16), use_gj);
             if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
         }
         int nb = 0;
@@ -685,6 +749,12 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         grid = (int64_t)nb * std::max(ctx->cu_count, 1);
         const int64_t per_block = use_wg ? 1 : threads;
         grid = std::min<int64_t>(grid, (N + per_block - 1) / per_block);
+    } else if (plain_wg) {
+        shmem = wg_shmem_bytes(nmax, false, false, false);
+        if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
+        int nb = 0;
+        HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, 256, shmem));
+        grid = std::min<int64_t>((int64_t)std::max(nb, 1) * std::max(ctx->cu_count, 1), N);
     }
     // ---- workspace layout ----
     size_t need = 0;
@@ -695,6 +765,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         add(N_ * 4); add(kCostBins * 4);
     }
     if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(kIntRows * N_ * 4); }
+    if (plain_wg) { add(N_ * 4); if (!a->nvec) add(N_ * 4); }   // slot -> request (all 0); nvec = nmax
     if (use_gj) add((size_t)grid * nmax * nmax * 4);
     if (fn_prep) { add(N_ * 8); if (!rodas) add(N_ * nmax * 8); }
     if (!dev) {
@@ -718,7 +789,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     P.save_end_user = a->save_end_user; P.max_saves = a->max_saves;
     P.refine_sweeps = a->refine_sweeps;
     P.counters = (unsigned long long*)ar.take(64);
-    HIPCHK(ctx, hipMemsetAsync(P.counters, 0, 64, s));
+    if (clear_words(ctx, P.counters, 16, s, false)) return -1;
     if (dev) {
         P.u0 = a->u0; P.p = a->p; P.t0 = a->t0; P.tf = a->tf; P.saveat = a->saveat; P.nvec = a->nvec;
         P.tstops = a->tstops;
@@ -755,7 +826,7 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         int* ord = (int*)ar.take(N_ * 4);
         unsigned* bins = (unsigned*)ar.take(kCostBins * 4);
         const int g = (int)((N + kCostTile - 1) / kCostTile);
-        HIPCHK(ctx, hipMemsetAsync(bins, 0, kCostBins * 4, s));
+        if (clear_words(ctx, bins, kCostBins, s, false)) return -1;
         hipLaunchKernelGGL(cost_hist_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, bins);
         hipLaunchKernelGGL(cost_scan_kernel, dim3(1), dim3(256), 0, s, bins);
         hipLaunchKernelGGL(cost_scatter_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, bins, ord);
@@ -768,33 +839,105 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
         P.st_int = (int*)ar.take(kIntRows * N_ * 4);
         HIPCHK(ctx, hipMemsetAsync(P.st_int, 0, kIntRows * N_ * 4, s));
     }
+    if (plain_wg) {
+        // every slot reports to request-table entry 0, whose done counter is counters[1]
+        int* sreq = (int*)ar.take(N_ * 4);
+        if (clear_words(ctx, sreq, N_, s, false)) return -1;
+        P.slot_req = sreq;
+        P.req_done = P.counters + 1;
+        if (!P.nvec) {
+            int* nv = (int*)ar.take(N_ * 4);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 1024)), dim3(256), 0, s,
+                               (unsigned*)nv, (int64_t)N, (unsigned)nmax);
+            HIPCHK(ctx, hipGetLastError());
+            P.nvec = nv;
+        }
+        P.slot0 = 0;
+        P.nslots = N;
+        P.tick_attempts = 1;
+        // the workgroup tick kernels take each slot's tolerances from scalar rows 10 and 11
+        const unsigned g = (unsigned)std::min<int64_t>((N + 255) / 256, 1024);
+        hipLaunchKernelGGL(fill_f64_kernel, dim3(g), dim3(256), 0, s, P.st_scalar + 10 * N_, (int64_t)N, P.reltol);
+        hipLaunchKernelGGL(fill_f64_kernel, dim3(g), dim3(256), 0, s, P.st_scalar + 11 * N_, (int64_t)N, P.abstol);
+        HIPCHK(ctx, hipGetLastError());
+    }
     if (use_gj) P.wg_jbuf = (float*)ar.take((size_t)grid * nmax * nmax * 4);
     if (fn_prep) {
         P.pre_dt = (double*)ar.take(N_ * 8);
         if (!rodas) P.pre_k1 = (double*)ar.take(N_ * nmax * 8);
     }
-    if (P.nsaved) HIPCHK(ctx, hipMemsetAsync(P.nsaved, 0, N_ * 4, s));
+    if (P.nsaved && clear_words(ctx, P.nsaved, N_, s, false)) return -1;
 
     int64_t launches = 0;
     // kernel_ms: the plain engine's launches, or the persistent kernel alone
     // (the prep kernel runs before the first event)
     if (plain) HIPCHK(ctx, hipEventRecord(ctx->ev0, s));
     if (plain) {
-        const int grid = (int)((N + block - 1) / block);
-        if (launch(ctx, fn_init, grid, block, P, s)) return -1;
-        launches++;
-        unsigned long long live = 1;
-        const int batch = 32;
-        while (live) {
-            for (int k = 0; k < batch - 1; k++)
-                if (launch(ctx, fn_main, grid, block, P, s)) return -1;
-            HIPCHK(ctx, hipMemsetAsync(P.counters + 1, 0, 8, s));
-            if (launch(ctx, fn_main, grid, block, P, s)) return -1;
-            launches += batch;
-            HIPCHK(ctx, hipMemcpyAsync(&live, P.counters + 1, 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(ctx, hipStreamSynchronize(s));
-            if (launches > (int64_t)a->maxiters * 4 + 1024) { ctx->err = "plain engine did not terminate"; return -4; }
+        // The outer loop: batches of `kBatch` step launches, the host reading
+        // the live (lane engine) or finished (workgroup engine) count after
+        // each.  A batch is captured once per solve into a hipGraph and
+        // replayed ($ODEMI_PLAIN_GRAPH=0: launched one by one).  Counters are
+        // cleared by the fill kernel, so the graph holds kernel nodes only.
+        const int lgrid = (int)((N + block - 1) / block);
+        if (plain_wg) {
+            // init: one workgroup per trajectory (grid-strided)
+            if (launch(ctx, fn_init, (int)grid, 256, P, s, shmem)) return -1;
+        } else {
+            if (launch(ctx, fn_init, lgrid, block, P, s)) return -1;
         }
+        launches++;
+        constexpr int kBatch = 32;
+        // plain_wg: each launch takes one attempt of every live trajectory,
+        // handed out by counters[0]; lane engine: counters[1] counts the lanes
+        // still live after the batch's last launch
+        auto enqueue_batch = [&]() -> int {
+            for (int k = 0; k < kBatch; k++) {
+                if (plain_wg) {
+                    if (clear_words(ctx, P.counters, 2, s, true)) return -1;
+                    if (launch(ctx, fn_main, (int)grid, 256, P, s, shmem)) return -1;
+                } else {
+                    if (k == kBatch - 1 && clear_words(ctx, P.counters + 1, 2, s, true)) return -1;
+                    if (launch(ctx, fn_main, lgrid, block, P, s)) return -1;
+                }
+            }
+            return 0;
+        };
+        const char* pg = getenv("ODEMI_PLAIN_GRAPH");
+        const bool use_graph = !(pg && atoi(pg) == 0);
+        hipGraphExec_t exec = nullptr;
+        if (use_graph) {
+            hipGraph_t g = nullptr;
+            HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc = enqueue_batch();
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (rc || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                if (!rc) ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+                return -1;
+            }
+            const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) { ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei); return -1; }
+        }
+        unsigned long long cnt = 0;
+        int rc = 0;
+        for (;;) {
+            if (use_graph) {
+                const hipError_t e = hipGraphLaunch(exec, s);
+                if (e != hipSuccess) { ctx->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e); rc = -1; break; }
+            } else if (enqueue_batch()) {
+                rc = -1;
+                break;
+            }
+            launches += kBatch;
+            hipError_t e = hipMemcpyAsync(&cnt, P.counters + 1, 8, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) { ctx->err = std::string("plain engine count: ") + hipGetErrorString(e); rc = -1; break; }
+            if (plain_wg ? (cnt >= (unsigned long long)N) : (cnt == 0)) break;
+            if (launches > (int64_t)a->maxiters * 4 + 1024) { ctx->err = "plain engine did not terminate"; rc = -4; break; }
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (rc) return rc;
     } else {
         if (fn_prep) {
             if (launch(ctx, fn_prep, (int)((N + 255) / 256), 256, P, s)) return -1;
@@ -824,6 +967,12 @@ extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_arg
     return 0;
 }
 
+}  // namespace
+
+extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
+    return solve_impl(ctx, rhs, a);
+}
+
 // ===========================================================================
 // Serving: continuous batching over a device-resident slot pool
 // ===========================================================================
@@ -835,6 +984,7 @@ double now_s() {
 struct Request {
     int64_t id = 0, N = 0;
     std::vector<double> u0, p, t0, tf;   // host copies of the inputs (SoA, N columns)
+    std::vector<double> tol;             // reltol [N] then abstol [N] (the request's own or the server's)
     std::vector<int> nvec;               // per-trajectory n (workgroup servers)
     std::vector<int> sreq;               // slot -> request-table index for the admitted range
     double* u_final = nullptr;           // caller's output buffers
@@ -929,7 +1079,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
         return nullptr;
     }
     if (wg && rhs->features) {
-        ctx->err = "isoutofdomain and callbacks run in the lane kernels only (compile without ODEMI_COMPILE_WG)";
+        ctx->err = "workgroup servers run problems without isoutofdomain or callbacks";
         return nullptr;
     }
     (void)hipSetDevice(ctx->device);
@@ -1038,6 +1188,13 @@ extern "C" int odemi_request_release(odemi_server* S, int64_t id) {
 extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const int32_t* nvec, const double* u0, const double* p,
                                 const double* t0, const double* tf, double* u_final, int32_t* retcode,
                                 int32_t* stats) {
+    return odemi_submit_opts(S, N, nvec, u0, p, t0, tf, nullptr, u_final, retcode, stats);
+}
+
+extern "C" int64_t odemi_submit_opts(odemi_server* S, int64_t N, const int32_t* nvec, const double* u0,
+                                     const double* p, const double* t0, const double* tf,
+                                     const odemi_request_opts* ro, double* u_final, int32_t* retcode,
+                                     int32_t* stats) {
     if (!S || N <= 0 || !u0 || !t0 || !tf || !retcode) return -1;
     if (S->rhs->np > 0 && !p) { S->err = "the problem has parameters but p is NULL"; return -1; }
     if (N > S->S) { S->err = "request larger than the slot pool"; return -2; }
@@ -1048,6 +1205,10 @@ extern "C" int64_t odemi_submit(odemi_server* S, int64_t N, const int32_t* nvec,
     if (np > 0 && p) r->p.assign(p, p + (size_t)np * N); else r->p.assign((size_t)std::max(np, 1) * N, 0.0);
     r->t0.assign(t0, t0 + N);
     r->tf.assign(tf, tf + N);
+    const double rt = (ro && ro->reltol > 0.0) ? ro->reltol : S->o.reltol;
+    const double at = (ro && ro->abstol > 0.0) ? ro->abstol : S->o.abstol;
+    r->tol.assign((size_t)2 * N, rt);
+    std::fill(r->tol.begin() + N, r->tol.end(), at);
     if (S->wg) {
         r->nvec.assign(N, n);
         if (nvec) {
@@ -1177,6 +1338,10 @@ static void server_loop(odemi_server* S) {
             SCHK(hipMemcpy2DAsync((double*)S->P.p + s0, Sz * 8, r->p.data(), N * 8, N * 8, np, hipMemcpyHostToDevice, S->stream));
             SCHK(hipMemcpyAsync((double*)S->P.t0 + s0, r->t0.data(), N * 8, hipMemcpyHostToDevice, S->stream));
             SCHK(hipMemcpyAsync((double*)S->P.tf + s0, r->tf.data(), N * 8, hipMemcpyHostToDevice, S->stream));
+            // the slots' tolerances: scalar state rows 10 (reltol) and 11 (abstol)
+            SCHK(hipMemcpyAsync(S->P.st_scalar + 10 * Sz + s0, r->tol.data(), N * 8, hipMemcpyHostToDevice, S->stream));
+            SCHK(hipMemcpyAsync(S->P.st_scalar + 11 * Sz + s0, r->tol.data() + N, N * 8, hipMemcpyHostToDevice,
+                                S->stream));
             r->sreq.assign(N, r->rt);
             SCHK(hipMemcpyAsync((int*)S->P.slot_req + s0, r->sreq.data(), N * 4, hipMemcpyHostToDevice, S->stream));
             SCHK(hipMemsetAsync(S->P.req_done + r->rt, 0, 8, S->stream));

@@ -52,11 +52,16 @@ class ServerOpts(ctypes.Structure):
     ]
 
 
+class RequestOpts(ctypes.Structure):
+    """Mirror of odemi_request_opts (include/odemi.h)."""
+    _fields_ = [("reltol", c_d), ("abstol", c_d)]
+
+
 EXPORTS = ["odemi_create", "odemi_destroy", "odemi_last_error", "odemi_device_info", "odemi_abi_version",
            "odemi_compile_rhs", "odemi_rhs_destroy", "odemi_solve_batch", "odemi_default_args",
            "odemi_rtc_codeobject", "odemi_server_default_opts", "odemi_server_create", "odemi_server_destroy",
            "odemi_submit", "odemi_poll", "odemi_wait", "odemi_request_times", "odemi_server_info",
-           "odemi_server_last_error", "odemi_request_release", "odemi_rtc_version"]
+           "odemi_server_last_error", "odemi_request_release", "odemi_rtc_version", "odemi_submit_opts"]
 
 _lib = None
 
@@ -69,6 +74,15 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"odemi native library missing: {LIB_PATH} (run __graft_entry__.build())")
     os.environ.setdefault("ODEMI_CACHE_DIR", DEFAULT_CACHE)
+    # One HIP runtime per process.  PyTorch ships its own ROCm build; loaded
+    # after libodemi.so it brings a second HIP runtime into the process, and of
+    # two runtimes only the one initialised first finds the GPU.  So torch, when
+    # installed, is loaded first and libodemi.so binds to its runtime (the same
+    # soname); without torch the system ROCm runtime is used.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     L.odemi_create.argtypes = [c_i]
     L.odemi_create.restype = c_vp
@@ -92,6 +106,8 @@ def load():
     L.odemi_server_destroy.argtypes = [c_vp]
     L.odemi_submit.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
     L.odemi_submit.restype = c_i64
+    L.odemi_submit_opts.argtypes = [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, P(RequestOpts), c_vp, c_vp, c_vp]
+    L.odemi_submit_opts.restype = c_i64
     L.odemi_poll.argtypes = [c_vp, c_i64]
     L.odemi_poll.restype = c_i
     L.odemi_wait.argtypes = [c_vp, c_i64, c_d]

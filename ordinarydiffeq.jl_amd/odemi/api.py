@@ -356,8 +356,8 @@ def trace_for(fun, n, np_, algs=("tsit5", "rodas5p"), ns=None, isoutofdomain=Non
     workgroup kernels.  No device is touched."""
     wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
     cbs = _callback_list(callback)
-    if wg and cbs:
-        raise ValueError("callbacks are supported by the lane kernels (Tsit5; Rodas5P with "
+    if wg and any(getattr(c, "len", 0) for c in cbs):
+        raise ValueError("VectorContinuousCallback is supported by the lane kernels (Tsit5; Rodas5P with "
                          f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
     if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
         raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
@@ -370,7 +370,7 @@ def trace_for(fun, n, np_, algs=("tsit5", "rodas5p"), ns=None, isoutofdomain=Non
     if tr is None:
         if wg:
             tr = trace_problem_wg(fun.f, list(ns) if ns is not None else [n], np_, jac=fun.jac, tgrad=fun.tgrad,
-                                  isout=isoutofdomain)
+                                  isout=isoutofdomain, callbacks=cbs)
             tr.src = _LANE_STUBS + tr.src
             tr.has_user_tgrad = fun.tgrad is not None
         else:
@@ -384,27 +384,33 @@ def _decimal(x):
 
 
 def saveat_times(saveat, t0, tf):
-    """Saveat list for (t0, tf]: a scalar interval gives (t0+d):d:tf (decimal-exact
-    like Julia's float ranges), a collection is filtered to t0 < s <= tf."""
+    """Saveat list, sorted ascending, for the span from t0 to tf: a scalar
+    interval d gives (t0 + tdir d):(tdir d):tf (decimal-exact like Julia's float
+    ranges; initialize_saveat, solve.jl:1103-1123, tdir = sign(tf - t0)); a
+    collection is sorted (each trajectory keeps the points between its own t0,
+    exclusive, and tf, inclusive)."""
     if saveat is None:
         return np.zeros(0)
     if np.isscalar(saveat):
         d = abs(float(saveat))
         if d == 0:
             raise ValueError("saveat interval must be nonzero")
-        start = _decimal(t0) + _decimal(d)
-        step = _decimal(d)
+        tdir = -1 if tf < t0 else 1
+        step = tdir * _decimal(d)
+        start = _decimal(t0) + step
+        end = _decimal(tf)
+        slack = abs(end) * Fraction(1, 10 ** 12)
         out = []
         k = 0
         while True:
             v = start + k * step
-            if v > _decimal(tf) * (1 + Fraction(1, 10 ** 12)):
+            if tdir * (v - end) > slack:
                 break
             fv = float(v)
-            if fv <= tf:
+            if tdir * fv <= tdir * tf:
                 out.append(fv)
             k += 1
-        return np.array(out, dtype=np.float64)
+        return np.sort(np.array(out, dtype=np.float64))
     arr = np.sort(np.asarray(list(saveat), dtype=np.float64))
     return arr
 
@@ -628,8 +634,6 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=No
             raise ValueError("all trajectories must share the state and parameter dimensions")
         if pi.f.f is not base.f.f:
             raise ValueError("prob_func must not change the right-hand side function")
-        if not (pi.tspan[1] > pi.tspan[0]) and pi.tspan[1] != pi.tspan[0]:
-            raise ValueError("only forward integration (tspan[2] >= tspan[1]) is supported")
         return pi
 
     def run(probs):
@@ -639,7 +643,13 @@ def solve(prob, alg, ensemblealg=None, *, trajectories=1, reltol=None, abstol=No
         t0 = np.array([pi.tspan[0] for pi in probs])
         tf = np.array([pi.tspan[1] for pi in probs])
         if saveat is not None and np.isscalar(saveat):
-            sa = saveat_times(saveat, float(t0.min(initial=base.tspan[0])), float(tf.max(initial=base.tspan[1])))
+            back = tf < t0
+            if back.any() and not back.all():
+                raise ValueError("a saveat interval needs every trajectory to integrate in the same direction")
+            if back.any():
+                sa = saveat_times(saveat, float(t0.max()), float(tf.min()))
+            else:
+                sa = saveat_times(saveat, float(t0.min(initial=base.tspan[0])), float(tf.max(initial=base.tspan[1])))
         elif saveat is not None and len(saveat):
             sa = saveat_times(saveat, 0, 0)
         else:

@@ -103,9 +103,10 @@ class EnsembleServer:
     def error(self):
         return self._L.odemi_server_last_error(self._handle()).decode(errors="replace")
 
-    def submit(self, u0, p, t0, tf, stats=True, nvec=None):
+    def submit(self, u0, p, t0, tf, stats=True, nvec=None, reltol=None, abstol=None):
         """u0 (n, N), p (np, N) or None, t0/tf (N,) or scalars, nvec (N,) per-
-        trajectory sizes (servers created with `ns`).  Inputs are copied before
+        trajectory sizes (servers created with `ns`); reltol / abstol: this
+        request's tolerances (None: the server's).  Inputs are copied before
         return; returns a Request."""
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         if u0.ndim == 1:
@@ -131,8 +132,11 @@ class EnsembleServer:
         rc = np.zeros(N, dtype=np.int32)
         st = np.zeros((_lib.NSTATS, N), dtype=np.int32) if stats else None
         ptr = lambda a: a.ctypes.data if a is not None else None  # noqa: E731
-        rid = self._L.odemi_submit(self._handle(), N, ptr(nv), ptr(u0), ptr(pp), ptr(t0), ptr(tf), ptr(uf), ptr(rc),
-                                   ptr(st))
+        ro = _lib.RequestOpts(0.0 if reltol is None else float(reltol), 0.0 if abstol is None else float(abstol))
+        if (reltol is not None and not ro.reltol > 0) or (abstol is not None and not ro.abstol > 0):
+            raise ValueError("request tolerances must be positive")
+        rid = self._L.odemi_submit_opts(self._handle(), N, ptr(nv), ptr(u0), ptr(pp), ptr(t0), ptr(tf),
+                                        ctypes.byref(ro), ptr(uf), ptr(rc), ptr(st))
         if rid <= 0:
             raise RuntimeError(f"odemi_submit failed ({rid}): {self.error()}")
         self._live[rid] = (uf, rc, st)

@@ -864,12 +864,99 @@ class TracedProblemWG:
         self.flops_f, self.flops_j, self.nnz = flops_f, flops_j, nnz
 
 
-def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None):
+def _wg_affect_case(affect, n, u, p, t):
+    """affect(integrator) traced for state size n: the body of one `case n` of a
+    workgroup affect (new values from the pre-affect state, then the writes) and
+    whether it calls terminate!."""
+    itg = _TracedIntegrator(u, p, t)
+    affect(itg)
+    if isinstance(itg._term, Sym):
+        raise TypeError("a conditional terminate is supported in VectorContinuousCallback affects only")
+    outs = [(i, v) for i, v in enumerate(itg.u.vals) if v is not u[i]]
+    body = ""
+    if outs:
+        body = (f"        T nu_[{n}];\n" + _emit_body(outs, "nu_", indent="        ", pname="p[{}]") + "\n"
+                + "\n".join(f"        u[{i}] = nu_[{i}];" for i, _ in outs) + "\n")
+    return body, bool(itg._term)
+
+
+def _wg_value_case(value):
+    v = value if isinstance(value, Sym) else _lift(float(value))
+    return "        T r_[1];\n" + _emit_body([(0, v)], "r_", indent="        ", pname="p[{}]") + "\n"
+
+
+def _emit_callbacks_wg(callbacks, cases):
+    """DiscreteCallbacks (OdemiDcbWg<K>) and ContinuousCallbacks (OdemiCcbWg<K>)
+    for the workgroup kernels: the condition and affects traced once per state
+    size (`cases[k]` = [(n, condition body, affect body, affect_neg body, term,
+    term_neg)]), dispatched by a switch on n; one thread runs them on the state
+    in LDS."""
+    dcbs = [(k, cb) for k, cb in enumerate(callbacks) if not hasattr(cb, "affect_neg")]
+    ccbs = [(k, cb) for k, cb in enumerate(callbacks) if hasattr(cb, "affect_neg")]
+    hdr = "    template <typename T>\n    static __device__ __noinline__ "
+    src = []
+
+    def fn(kind, name, args, per_n, default):
+        body = [hdr + f"{kind} {name}({args}, int n) {{", "        (void)u; (void)p; (void)t;", "        switch (n) {"]
+        for n, code in per_n:
+            body.append(f"        case {n}: {{\n{code}        }}")
+        body += [f"        default: {default}", "        }", "    }"]
+        return body
+
+    def terms(k, idx):
+        vals = {c[idx] for c in cases[k]}
+        if len(vals) != 1:
+            raise ValueError("an affect must call terminate! for every state size or for none")
+        return int(vals.pop())
+
+    if dcbs:
+        src += [f"#define ODEMI_NDCB_WG {len(dcbs)}", "template <int K> struct OdemiDcbWg;"]
+        for q, (k, cb) in enumerate(dcbs):
+            s0, s1 = (int(bool(x)) for x in cb.save_positions)
+            src += [f"template <> struct OdemiDcbWg<{q}> {{",
+                    f"    static constexpr int save0 = {s0}, save1 = {s1}, term = {terms(k, 4)};"]
+            src += fn("bool", "cond", "const T* __restrict__ u, const T* __restrict__ p, T t",
+                      [(n, c + "        return r_[0] != T(0);\n") for n, c, *_ in cases[k]], "return false;")
+            src += fn("void", "affect", "T* __restrict__ u, const T* __restrict__ p, T t",
+                      [(n, a + "        break;\n") for n, _, a, *_ in cases[k]], "break;")
+            src.append("};")
+    if ccbs:
+        if len(ccbs) > 255:
+            raise ValueError("at most 255 continuous callbacks")
+        src += [f"#define ODEMI_NCCB_WG {len(ccbs)}", "template <int K> struct OdemiCcbWg;"]
+        for q, (k, cb) in enumerate(ccbs):
+            s0, s1 = (int(bool(x)) for x in cb.save_positions)
+            rf = {"left": 0, "right": 1, "none": 2}[cb.rootfind]
+            src += [f"template <> struct OdemiCcbWg<{q}> {{",
+                    f"    static constexpr int has_affect = {int(cb.affect is not None)}, "
+                    f"has_affect_neg = {int(cb.affect_neg is not None)}, save0 = {s0}, save1 = {s1};",
+                    f"    static constexpr int rootfind = {rf}, interp_points = {int(cb.interp_points)}, "
+                    f"term = {terms(k, 4)}, term_neg = {terms(k, 5)};",
+                    f"    static constexpr double abstol = {float(cb.abstol).hex()}, "
+                    f"repeat_nudge = {float(cb.repeat_nudge).hex()};"]
+            src += fn("T", "cond", "const T* __restrict__ u, const T* __restrict__ p, T t",
+                      [(n, c + "        return r_[0];\n") for n, c, *_ in cases[k]], "return T(0);")
+            src += fn("void", "affect", "T* __restrict__ u, const T* __restrict__ p, T t",
+                      [(n, a + "        break;\n") for n, _, a, *_ in cases[k]], "break;")
+            src += fn("void", "affect_neg", "T* __restrict__ u, const T* __restrict__ p, T t",
+                      [(n, a + "        break;\n") for n, _, _, a, *_ in cases[k]], "break;")
+            src.append("};")
+    return "\n".join(src) + "\n"
+
+
+def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None, callbacks=None):
     """Workgroup forms for every state size in `ns` (distinct, sorted).
     isout(u, p, t): the isoutofdomain predicate, traced per size into
-    odemi_isout_wg (one thread evaluates it on the state in LDS)."""
+    odemi_isout_wg (one thread evaluates it on the state in LDS).  callbacks:
+    DiscreteCallbacks and (scalar) ContinuousCallbacks, traced per size into
+    OdemiDcbWg<K> / OdemiCcbWg<K>."""
     global _CUR
     ns = sorted(set(int(n) for n in ns))
+    callbacks = list(callbacks or [])
+    for cb in callbacks:
+        if getattr(cb, "len", 0):
+            raise ValueError("VectorContinuousCallback runs in the lane kernels only, not in the workgroup kernels")
+    cbcases = [[] for _ in callbacks]
     fcases, jcases, tcases, vcases, icases = [], [], [], [], []
     flops_f, flops_j, nnz = {}, {}, {}
     for n in ns:
@@ -929,6 +1016,15 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None):
                 icases.append(f"    case {n}: {{\n        T r_[1];\n"
                               + _emit_body([(0, r)], "r_", indent="        ", pname="T(p[{}])")
                               + "\n        return r_[0] != T(0);\n    }")
+            for k, cb in enumerate(callbacks):
+                cond = cb.condition(list(u), t, _TracedIntegrator(u, p, t))
+                if not hasattr(cb, "affect_neg") and not isinstance(cond, Sym):
+                    cond = 1.0 if cond else 0.0
+                aff, term = _wg_affect_case(cb.affect, n, u, p, t) if cb.affect is not None else ("", False)
+                neg, term_neg = ("", False)
+                if getattr(cb, "affect_neg", None) is not None:
+                    neg, term_neg = _wg_affect_case(cb.affect_neg, n, u, p, t)
+                cbcases[k].append((n, _wg_value_case(cond), aff, neg, term, term_neg))
         finally:
             _CUR = None
     hdr = "template <typename T>\n__device__ __noinline__ void "
@@ -953,4 +1049,6 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None):
         src += ("#define ODEMI_HAS_ISOUT_WG 1\ntemplate <typename T>\n__device__ __noinline__ bool odemi_isout_wg("
                 "const T* __restrict__ u, const T* __restrict__ p, T t, int n) {\n    (void)u; (void)p; (void)t;\n"
                 "    switch (n) {\n" + "\n".join(icases) + "\n    default: return false;\n    }\n}\n")
+    if callbacks:
+        src += _emit_callbacks_wg(callbacks, cbcases)
     return TracedProblemWG(ns, np_, src, flops_f, flops_j, nnz)

@@ -230,3 +230,44 @@ def test_poisson_mixed_types_under_budget_alone_equals_batch():
                 continue
             assert np.array_equal(r.u_final, b["u_final"][:, idx]), k
             assert np.array_equal(r.stats, b["stats"][:, idx]) and np.array_equal(r.retcode, b["retcode"][idx])
+
+
+@pytest.mark.parametrize("kind", ["lorenz-tsit5", "robertson-rodas", "chain-wg"])
+def test_per_request_tolerances_equal_batch_engine(kind):
+    """Requests with their own reltol / abstol (odemi_submit_opts) in one pool,
+    interleaved: each result equals the batch engine's solve at that request's
+    tolerances, bit for bit (the per-slot tolerances live in the pool's state,
+    so neighbours with other tolerances change nothing)."""
+    if kind == "lorenz-tsit5":
+        d, fun, alg, ns = P.lorenz_large(N=600), P.LORENZ, odemi.Tsit5(), None
+        tols = [(1e-4, 1e-4), (None, None), (1e-8, 1e-9)]
+    elif kind == "robertson-rodas":
+        d, fun, alg, ns = P.robertson_ensemble(N=600), P.ROBERTSON, odemi.Rodas5P(), None
+        tols = [(1e-4, 1e-8), (None, None), (1e-8, 1e-12)]
+    else:
+        d = P.chain_ensemble(N=48, seed=61, m_lo=4, m_hi=12)
+        d["tf"] = np.full(48, 0.5)
+        fun, alg, ns = P.CHAIN, odemi.Rodas5P(), sorted(set(int(x) for x in d["nvec"]))
+        tols = [(1e-4, 1e-4), (None, None), (1e-8, 1e-8)]
+    n, N = d["u0"].shape
+    print(f"[case] serving per-request tolerances {kind}", flush=True)
+    chunks = np.array_split(np.arange(N), 6)
+    with odemi.EnsembleServer(fun, n, d["p"].shape[0], alg, reltol=d["reltol"], abstol=d["abstol"], ns=ns,
+                              tick_attempts=16) as srv:
+        reqs = []
+        for q, idx in enumerate(chunks):
+            rt, at = tols[q % 3]
+            nv = d["nvec"][idx] if ns is not None else None
+            reqs.append((idx, rt, at, srv.submit(*_cols(d, idx), nvec=nv, reltol=rt, abstol=at)))
+        for *_, r in reqs:
+            assert r.wait(timeout=300)
+    for idx, rt, at, r in reqs:
+        kw = dict(reltol=d["reltol"] if rt is None else rt, abstol=d["abstol"] if at is None else at)
+        if ns is not None:
+            kw["nvec"] = d["nvec"][idx]
+        b = odemi.solve_arrays(fun, alg, *_cols(d, idx), **kw)
+        assert (b["retcode"] == 1).all()
+        assert np.array_equal(r.u_final, b["u_final"]) and np.array_equal(r.stats, b["stats"]), (rt, at)
+    # the tolerances did change the work
+    steps = [int(r.stats[1].sum()) for *_, r in reqs[:3]]
+    assert steps[0] < steps[1] < steps[2]

@@ -35,6 +35,38 @@ def test_chain_uneven_fp64_bit_identical():
     assert (out["stats"][0] == ref["nf"]).all()
 
 
+def test_chain_uneven_plain_engine_bit_identical():
+    """The plain FP64 engine of uneven batches (one launch per step attempt, the
+    integrator state in HBM between launches): final states, step counts and nf
+    identical to the restatement and to the persistent FP64 workgroup engine,
+    n = 8..64 in one batch, with save_start; a batch without nvec runs every
+    trajectory at nmax."""
+    d = _chain(96, m_lo=4, m_hi=32, seed=21)
+    print("[case] chain uneven n 8..64 plain wg fp64", flush=True)
+    ref = _oracle(d)
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], nvec=d["nvec"], save_start=True)
+    pl = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(engine="plain"), **kw)
+    ps = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], **kw)
+    assert (pl["retcode"] == 1).all() and pl["launches"] > 100
+    assert np.array_equal(pl["u_final"], ps["u_final"]) and np.array_equal(pl["stats"], ps["stats"])
+    assert np.array_equal(pl["nsaved"], ps["nsaved"]) and (pl["nsaved"] == 2).all()
+    assert np.array_equal(pl["u_out"][:2], ps["u_out"][:2]) and np.array_equal(pl["t_out"][:2], ps["t_out"][:2])
+    same = (pl["stats"][1] == ref["naccept"]) & (pl["stats"][2] == ref["nreject"])
+    exact = np.all(pl["u_final"] == ref["u_final"], axis=0)
+    print(f"steps equal {same.mean():.4f} exact {exact.mean():.4f} launches {pl['launches']}")
+    assert same.mean() >= 0.99 and exact.mean() >= 0.99
+    assert (pl["stats"][0] == ref["nf"]).all()
+    # no nvec: every trajectory at the compiled n
+    e = _chain(32, m_lo=6, m_hi=6, seed=22)
+    print("[case] chain n 12 plain wg fp64 without nvec", flush=True)
+    a = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), e["u0"], e["p"], e["t0"], e["tf"],
+                           odemi.EnsembleMI355X(engine="plain"), reltol=e["reltol"], abstol=e["abstol"])
+    b = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), e["u0"], e["p"], e["t0"], e["tf"], reltol=e["reltol"],
+                           abstol=e["abstol"], nvec=e["nvec"])
+    assert np.array_equal(a["u_final"], b["u_final"]) and np.array_equal(a["stats"], b["stats"])
+
+
 def test_chain_large_n_fp64_and_mixed():
     d = _chain(64, m_lo=48, m_hi=64, seed=9)
     print("[case] chain n 96..128 wg fp64 + mixed", flush=True)
@@ -244,3 +276,99 @@ def test_chain_full_span_uneven_one_launch_mixed_accuracy(chain_cfg4_sample):
           f"p99 {np.quantile(emx, 0.99):.3e}")
     assert np.quantile(err.max(axis=0), 0.99) <= 10.0
     assert np.median(emx) <= 1.5 * np.median(e64) and np.quantile(emx, 0.99) <= 1.5 * np.quantile(e64, 0.99)
+
+
+def _wall(itg):
+    m = len(itg.u) // 2   # the first mass's velocity, reversed and halved
+    itg.u[m] = -0.5 * itg.u[m]
+
+
+def _kick(itg):
+    m = len(itg.u) // 2
+    itg.u[m] = itg.u[m] + itg.p[1]
+
+
+class _PyCB:
+    """The same callback for the CPU restatement (plain Python on numpy)."""
+
+    def __init__(self, cb):
+        self.__dict__.update(cb.__dict__)
+
+
+def _chain_events(N, seed):
+    d = _chain(N, m_lo=4, m_hi=12, seed=seed)
+    d["p"][0] = np.random.Generator(np.random.PCG64(seed + 100)).uniform(1.0, 100.0, N)   # a few events each
+    return d
+
+
+_WG_CBS = odemi.CallbackSet(
+    odemi.ContinuousCallback(lambda u, t, itg: u[0], None, affect_neg=_wall),
+    odemi.DiscreteCallback(lambda u, t, itg: odemi.sym_in(t, [0.25, 0.5]), _kick))
+
+
+@pytest.mark.parametrize("saves", ["saveat", "final"])
+def test_wg_callbacks_bit_identical(saves):
+    """Callbacks in the workgroup kernels (uneven chains n = 8..24, FP64 LU):
+    a ContinuousCallback on the first mass's position (downcrossings only: its
+    velocity reversed and halved; event search on the Rosenbrock dense output,
+    root to adjacent floats, the step's stages recomputed for the event time)
+    and a DiscreteCallback kicking that velocity at tstops 0.25 and 0.5, with
+    forced saves around each affect.  Bit-identical to the CPU restatement:
+    save times and states, final states, retcodes, nf / naccept / nreject /
+    ncondition."""
+    d = _chain_events(48, 31)
+    m = dict(saveat=[0.1, 0.6], save_start=False, save_end=True) if saves == "saveat" else \
+        dict(save_start=False, save_end=True)
+    print(f"[case] chain wg callbacks fp64 {saves}", flush=True)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"], tstops=[0.25, 0.5], callback=_WG_CBS,
+                             max_saves=512, **m)
+    ccb = [_PyCB(c) for c in _WG_CBS.continuous_callbacks]
+    dcb = [_PyCB(c) for c in _WG_CBS.discrete_callbacks]
+    events = 0
+    for j in range(d["u0"].shape[1]):
+        n = int(d["nvec"][j])
+        r = O.solve_builtin(O.PROB_CHAIN, d["u0"][:n, j], d["p"][:, j], (0.0, float(d["tf"][j])), alg=O.RODAS5P,
+                            reltol=d["reltol"], abstol=d["abstol"], tstops=[0.25, 0.5], callbacks=dcb,
+                            ccallbacks=ccb, **m)
+        ns = int(out["nsaved"][j])
+        assert int(out["retcode"][j]) == r["retcode"] == 1 and ns == r["nsaved"], j
+        assert np.array_equal(out["t_out"][:ns, j], r["t"]), j
+        assert np.array_equal(out["u_out"][:ns, :n, j], r["u"]), j
+        assert np.array_equal(out["u_final"][:n, j], r["u_final"]) and out["t_final"][j] == r["t_final"], j
+        st = out["stats"][:, j]
+        assert (st[0], st[1], st[2], st[6]) == (r["nf"], r["naccept"], r["nreject"], r["ncondition"]), j
+        events += ns
+    print(f"saved points {events} over {d['u0'].shape[1]} chains")
+    assert events > 5 * d["u0"].shape[1]   # two kicks (4 forced saves) plus wall events
+
+
+def test_wg_callbacks_terminate_and_mixed():
+    """terminate! from a ContinuousCallback's affect (the first upcrossing of
+    the first mass's position past 0.02) ends each chain at the event time with
+    Terminated, the end state saved; FP64 bit-identical to the restatement, the
+    mixed (Gauss-Jordan) engine with the same retcodes and event times within
+    1e-6 of FP64's."""
+    d = _chain_events(40, 32)
+    d["u0"][0] = -0.05   # every chain starts below the threshold
+    cb = odemi.ContinuousCallback(lambda u, t, itg: u[0] - 0.02, lambda itg: odemi.terminate(itg), None,
+                                  rootfind="right")
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], nvec=d["nvec"], callback=cb, save_start=False, save_end=True,
+              max_saves=8)
+    print("[case] chain wg callback terminate fp64 + mixed", flush=True)
+    o64 = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], **kw)
+    omx = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                             odemi.EnsembleMI355X(precision="mixed"), **kw)
+    codes = set()
+    for j in range(d["u0"].shape[1]):
+        n = int(d["nvec"][j])
+        r = O.solve_builtin(O.PROB_CHAIN, d["u0"][:n, j], d["p"][:, j], (0.0, float(d["tf"][j])), alg=O.RODAS5P,
+                            reltol=d["reltol"], abstol=d["abstol"], ccallbacks=[_PyCB(cb)], save_start=False,
+                            save_end=True)
+        assert int(o64["retcode"][j]) == r["retcode"] and o64["t_final"][j] == r["t_final"], j
+        assert np.array_equal(o64["u_final"][:n, j], r["u_final"]), j
+        assert int(o64["nsaved"][j]) == r["nsaved"] and np.array_equal(o64["t_out"][:r["nsaved"], j], r["t"]), j
+        assert int(omx["retcode"][j]) == r["retcode"], j
+        assert abs(omx["t_final"][j] - r["t_final"]) <= 1e-6, (j, omx["t_final"][j], r["t_final"])
+        codes.add(r["retcode"])
+    assert 2 in codes

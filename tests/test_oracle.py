@@ -759,3 +759,96 @@ def test_builtin_ball_event_matches_traced_form():
         vb = v0 - 9.81 * tb
         # just after the bounce the velocity is -restitution * impact velocity
         assert abs(c["u"][-1][1] - (-d["p"][0, j] * vb - 9.81 * (tb + 0.5 - tb))) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Backward integration (tf < t0): tdir = sign(tf - t0) threads through
+# initdt, the dt bounds, the tstop and saveat queues (stored as tdir * t) and
+# the loop condition (solve.jl:268-273,401,904-946,968-985,1019-1123;
+# integrator_utils.jl:268-324,340-414,1199-1256,1290-1314; initdt.jl:17-298).
+# ---------------------------------------------------------------------------
+def _lorenz_py(du, u, p, t):
+    du[0] = p[0] * (u[1] - u[0])
+    du[1] = u[0] * (p[1] - u[2]) - u[1]
+    du[2] = u[0] * u[1] - p[2] * u[2]
+
+
+def _lorenz_py_jac(J, u, p, t):
+    J[0, 0], J[0, 1], J[0, 2] = -p[0], p[0], 0.0
+    J[1, 0], J[1, 1], J[1, 2] = p[1] - u[2], -1.0, -u[0]
+    J[2, 0], J[2, 1], J[2, 2] = u[1], u[0], -p[2]
+
+
+def _forced(du, u, p, t):
+    du[0] = -p[0] * u[0] + math.sin(3.0 * t)
+    du[1] = u[0] - p[1] * u[1] * t
+
+
+def _forced_jac(J, u, p, t):
+    J[0, 0], J[0, 1] = -p[0], 0.0
+    J[1, 0], J[1, 1] = 1.0, -p[1] * t
+
+
+def _forced_tgrad(dT, u, p, t):
+    dT[0] = 3.0 * math.cos(3.0 * t)
+    dT[1] = -p[1] * u[1]
+
+
+def _reversed(f):
+    """v' = -f(v, -s): the time-reversed problem, v(s) = u(-s)."""
+    def g(du, u, p, s):
+        f(du, u, p, -s)
+        du[:] = -du
+    return g
+
+
+def _reversed_mat(jf):
+    def g(J, u, p, s):
+        jf(J, u, p, -s)
+        J[:] = -J
+    return g
+
+
+def _reversed_tgrad(tg):
+    def g(dT, u, p, s):
+        tg(dT, u, p, -s)   # d/ds (-f(v, -s)) = +f_t(v, -s)
+    return g
+
+
+@pytest.mark.parametrize("alg", [O.TSIT5, O.RODAS5P], ids=["tsit5", "rodas5p"])
+@pytest.mark.parametrize("prob", ["lorenz", "forced"])
+def test_backward_solve_is_the_forward_solve_of_the_reversed_problem(alg, prob):
+    """Integrating u' = f(u, t) from T down to 0 is, step for step, integrating
+    v' = -f(v, -s) from -T up to 0: IEEE arithmetic is symmetric under
+    negation, so the two solves agree bit for bit (states, save times negated,
+    step and function counts) — pinning every tdir factor to the forward path.
+    With saveat, tstops and a user dtmax."""
+    if prob == "lorenz":   # backward in time the Lorenz flow expands volumes by e^(13.7 t): a short span
+        f, jac, tg, u0, p, T = _lorenz_py, _lorenz_py_jac, None, [1.0, -0.5, 20.0], [10.0, 28.0, 8 / 3], 0.5
+        sa, ts, dtmax = [0.05, 0.2, 0.45, T + 1.0], [0.4, 0.15, -1.0], 0.03
+    else:
+        f, jac, tg, u0, p, T = _forced, _forced_jac, _forced_tgrad, [1.0, 0.3], [2.0, 0.7], 4.0
+        sa, ts, dtmax = [0.3, 1.1, 2.9, T + 1.0], [2.5, 0.75, -1.0], 0.5
+    kw = dict(alg=alg, reltol=1e-7, abstol=1e-9, saveat=sa, tstops=ts, dtmax=dtmax, save_start=True)
+    back = O.solve(f, u0, p, (T, 0.0), jac=jac, tgrad=tg, **kw)
+    kw_f = dict(kw, saveat=[-s for s in sa], tstops=[-s for s in ts])
+    fwd = O.solve(_reversed(f), u0, p, (-T, 0.0), jac=_reversed_mat(jac),
+                  tgrad=None if tg is None else _reversed_tgrad(tg), **kw_f)
+    assert back["retcode"] == fwd["retcode"] == 1
+    assert np.array_equal(back["u_final"], fwd["u_final"]) and back["t_final"] == -fwd["t_final"] == 0.0
+    assert np.array_equal(back["t"], -fwd["t"]) and np.array_equal(back["u"], fwd["u"])
+    assert [back[k] for k in ("nf", "naccept", "nreject", "njacs")] == [fwd[k] for k in ("nf", "naccept", "nreject",
+                                                                                            "njacs")]
+    # saveat inside [0, T) in descending order, tstops hit exactly, the end point last
+    assert list(back["t"][:4]) == [T] + sorted(sa[:3], reverse=True) and back["t"][-1] == 0.0
+    assert back["dt_init"] < 0
+
+
+@pytest.mark.parametrize("alg", [O.TSIT5, O.RODAS5P], ids=["tsit5", "rodas5p"])
+def test_backward_exponential_closed_form(alg):
+    """u' = lam u from t = 2 back to 0: u(0) = u(2) exp(-2 lam)."""
+    lam = -0.8
+    r = O.solve(lambda du, u, p, t: du.__setitem__(0, p[0] * u[0]), [math.exp(2 * lam)], [lam], (2.0, 0.0),
+                jac=lambda J, u, p, t: J.__setitem__((0, 0), p[0]), alg=alg, reltol=1e-10, abstol=1e-12)
+    assert r["retcode"] == 1 and r["t_final"] == 0.0
+    assert abs(r["u_final"][0] - 1.0) < 1e-8

@@ -70,8 +70,10 @@ def test_callback_module_compiles_for_gfx950():
         assert rc == 0, log.value.decode()[:2000]
 
 
-def test_callbacks_rejected_by_workgroup_kernels_and_unknown_types():
-    cb = odemi.DiscreteCallback(_dose_cond, _dose)
+def test_vector_callbacks_rejected_by_workgroup_kernels_and_unknown_types():
+    def vc(out, u, t, itg):
+        out[0] = u[0]
+    cb = odemi.VectorContinuousCallback(vc, lambda itg, ev: None, 1)
 
     def big(du, u, p, t):
         for i in range(8):
@@ -230,3 +232,38 @@ def test_codeobjects_compile_concurrently_from_two_threads(tmp_path):
 def test_rtc_version_reported():
     major, minor = _lib.rtc_version()
     assert major >= 1 and minor >= 0
+
+
+def _kick(itg):
+    m = len(itg.u) // 2   # the first mass's velocity, for every chain length
+    itg.u[m] = itg.u[m] + itg.p[1]
+
+
+def test_workgroup_callbacks_emit_per_size_and_compile():
+    """Discrete and continuous callbacks of an uneven batch are traced once per
+    state size into OdemiDcbWg / OdemiCcbWg (a switch on n), with the affect's
+    index following n, and compile with the workgroup kernels; a
+    VectorContinuousCallback is refused there."""
+    from odemi import problems as P
+    from odemi.api import trace_for
+    cbs = odemi.CallbackSet(
+        odemi.ContinuousCallback(lambda u, t, itg: u[0], None, affect_neg=lambda itg: odemi.terminate(itg)),
+        odemi.DiscreteCallback(lambda u, t, itg: odemi.sym_in(t, [0.25]), _kick))
+    tr, wg = trace_for(P.CHAIN, 12, 3, algs=("rodas5p",), ns=[8, 12], callback=cbs)
+    assert wg
+    src = tr.src
+    assert "#define ODEMI_NCCB_WG 1" in src and "#define ODEMI_NDCB_WG 1" in src
+    assert "term = 0, term_neg = 1" in src
+    assert "u[4] = nu_[4];" in src and "u[6] = nu_[6];" in src   # m = n / 2 for n = 8 and 12
+    L = _lib.load()
+    path = ctypes.create_string_buffer(512)
+    log = ctypes.create_string_buffer(1 << 16)
+    secs = ctypes.c_double()
+    rc = L.odemi_rtc_codeobject(src.encode(), 12, 3, _lib.COMPILE_WG, path, 512, ctypes.byref(secs), log, 1 << 16)
+    assert rc == 0, log.value.decode()[:3000]
+
+    def vc(out, u, t, itg):
+        out[0] = u[0]
+    with pytest.raises(ValueError, match="VectorContinuousCallback"):
+        trace_for(P.CHAIN, 12, 3, algs=("rodas5p",), ns=[8, 12],
+                  callback=odemi.VectorContinuousCallback(vc, lambda itg, ev: None, 1))
