@@ -175,6 +175,23 @@ int odemi_rtc_version(int* major, int* minor);
 /* Solve N independent IVPs (one EnsembleProblem solve).  Synchronous. */
 int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* args);
 
+/* odemi_solve_batch without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as odemi_solve_batch, with these limits:
+ *   - args.device_ptrs must be 1 (the caller passes inputs and outputs on the device);
+ *   - persistent engine only (ODEMI_ENGINE_PERSISTENT);
+ *   - the solve's scratch (work-queue counters and order, start-up stage buffers,
+ *     the mixed workgroup kernel's Jacobian blocks) must fit in 64 MiB, the
+ *     library's fixed per-GPU scratch array; larger solves return -2 with a message;
+ *   - asynchronous: returns once its launches are enqueued on args.stream (NULL:
+ *     the ctx stream); outputs are valid after that stream synchronises;
+ *     kernel_ms is not measured (0).
+ * Every call on one GPU shares that GPU's scratch array (it keeps the last
+ * call's contents; the call clears what must start at zero): calls that overlap
+ * on one GPU, from any ctx or stream, race.  It calls no hipMalloc / hipFree. */
+int odemi_solve_batch_noalloc(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* args);
+
 /* Fill args with the reference's solve defaults for `alg`. */
 void odemi_default_args(odemi_solve_args* args, int alg);
 

@@ -162,13 +162,43 @@ __global__ void __launch_bounds__(256) cost_scatter_kernel(const double* t0, con
     }
 }
 
-// Word fill on the device, used instead of hipMemsetAsync where the stream is
-// captured into a graph.
+// Word fill on the device, used instead of hipMemsetAsync where a caller may
+// capture the stream into a graph (the allocation-free entry point).
 __global__ void __launch_bounds__(256) fill_u32_kernel(unsigned* p, int64_t n, unsigned v) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
 }
 __global__ void __launch_bounds__(256) fill_f64_kernel(double* p, int64_t n, double v) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
+}
+
+// Device scratch of odemi_solve_batch_noalloc: one array per GPU, held until
+// the process ends.  Host code never passes `scratch` itself to a kernel (on
+// the host it names a stand-in, not device memory); it uses the address that
+// hipGetSymbolAddress returned for the device, kept in g_scratch.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Look up this device's scratch address once (the device must be current).
+int scratch_setup(int device, std::string& err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) { err = "device index out of range for the scratch table"; return -1; }
+    if (g_scratch[device]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return -1;
+    }
+    g_scratch[device] = ptr;
+    return 0;
+}
+
+void* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
 }
 
 uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL) {
@@ -208,6 +238,7 @@ struct odemi_ctx {
     size_t ws_bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_order = nullptr;   // ordering against the legacy default stream (no timing)
+    std::string scratch_err;   // why the allocation-free entry point is unavailable, if it is
 };
 
 struct odemi_rhs {
@@ -248,6 +279,8 @@ extern "C" odemi_ctx* odemi_create(int device) {
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
     (void)hipEventCreateWithFlags(&c->ev_order, hipEventDisableTiming);
+    // the allocation-free entry point's scratch address on this device
+    (void)scratch_setup(device, c->scratch_err);
     return c;
 }
 
@@ -632,8 +665,8 @@ int launch(odemi_ctx* c, hipFunction_t fn, int grid, int block, Params& P, hipSt
     return 0;
 }
 
-// Zero `words` 32-bit words: hipMemsetAsync, or (where the stream is captured
-// into a graph) the fill kernel.
+// Zero `words` 32-bit words: hipMemsetAsync, or (allocation-free path, whose
+// stream a caller may capture) the fill kernel.
 int clear_words(odemi_ctx* c, void* p, size_t words, hipStream_t s, bool kernel) {
     if (words == 0) return 0;
     if (!kernel) {
@@ -646,8 +679,11 @@ int clear_words(odemi_ctx* c, void* p, size_t words, hipStream_t s, bool kernel)
     return 0;
 }
 
-// One ensemble solve.
-int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
+// One ensemble solve.  noalloc: device pointers and the persistent engines
+// only, every scratch buffer carved from this device's `scratch` array,
+// nothing allocated, asynchronous on the stream (no synchronisation, no
+// kernel_ms).
+int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc) {
     if (!ctx || !rhs || !a) return -1;
     std::lock_guard<std::mutex> lock(ctx->mu);
     (void)hipSetDevice(ctx->device);
@@ -656,6 +692,17 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
     const int np = a->np;
     a->kernel_ms = 0.0;
     a->launches = 0;
+    if (noalloc) {
+        if (!a->device_ptrs) { ctx->err = "the allocation-free entry point takes device pointers only"; return -2; }
+        if (a->engine == ODEMI_ENGINE_PLAIN) {
+            ctx->err = "the allocation-free entry point runs the persistent engines only";
+            return -2;
+        }
+        if (!scratch_ptr(ctx->device)) {
+            ctx->err = "no scratch on this device: " + ctx->scratch_err;
+            return -1;
+        }
+    }
     if (N <= 0) return 0;
     if (nmax != rhs->n) { ctx->err = "args.nmax does not match the compiled problem dimension"; return -2; }
     if (np > std::max(rhs->np, 1)) { ctx->err = "args.np exceeds the compiled parameter count"; return -2; }
@@ -776,8 +823,18 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
         add(N_ * 4); add(N_ * nmax * 8); add(N_ * 8); add(N_ * 4); add(N_ * kStatRows * 4);
         if (a->nvec) add(N_ * 4);
     }
-    if (ensure_ws(ctx, need)) return -1;
-    Arena ar{(char*)ctx->ws, ctx->ws_bytes};
+    Arena ar{nullptr, 0};
+    if (noalloc) {
+        if (need > kScratchBytes) {
+            ctx->err = "this solve needs " + std::to_string(need) + " bytes of scratch; the allocation-free entry point has " +
+                       std::to_string(kScratchBytes);
+            return -2;
+        }
+        ar = Arena{(char*)scratch_ptr(ctx->device), kScratchBytes};
+    } else {
+        if (ensure_ws(ctx, need)) return -1;
+        ar = Arena{(char*)ctx->ws, ctx->ws_bytes};
+    }
     Params P;
     memset(&P, 0, sizeof(P));
     P.N = N; P.nmax = nmax; P.np = np;
@@ -789,7 +846,7 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
     P.save_end_user = a->save_end_user; P.max_saves = a->max_saves;
     P.refine_sweeps = a->refine_sweeps;
     P.counters = (unsigned long long*)ar.take(64);
-    if (clear_words(ctx, P.counters, 16, s, false)) return -1;
+    if (clear_words(ctx, P.counters, 16, s, noalloc)) return -1;
     if (dev) {
         P.u0 = a->u0; P.p = a->p; P.t0 = a->t0; P.tf = a->tf; P.saveat = a->saveat; P.nvec = a->nvec;
         P.tstops = a->tstops;
@@ -826,7 +883,7 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
         int* ord = (int*)ar.take(N_ * 4);
         unsigned* bins = (unsigned*)ar.take(kCostBins * 4);
         const int g = (int)((N + kCostTile - 1) / kCostTile);
-        if (clear_words(ctx, bins, kCostBins, s, false)) return -1;
+        if (clear_words(ctx, bins, kCostBins, s, noalloc)) return -1;
         hipLaunchKernelGGL(cost_hist_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, bins);
         hipLaunchKernelGGL(cost_scan_kernel, dim3(1), dim3(256), 0, s, bins);
         hipLaunchKernelGGL(cost_scatter_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, bins, ord);
@@ -866,7 +923,7 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
         P.pre_dt = (double*)ar.take(N_ * 8);
         if (!rodas) P.pre_k1 = (double*)ar.take(N_ * nmax * 8);
     }
-    if (P.nsaved && clear_words(ctx, P.nsaved, N_, s, false)) return -1;
+    if (P.nsaved && clear_words(ctx, P.nsaved, N_, s, noalloc)) return -1;
 
     int64_t launches = 0;
     // kernel_ms: the plain engine's launches, or the persistent kernel alone
@@ -943,9 +1000,17 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
             if (launch(ctx, fn_prep, (int)((N + 255) / 256), 256, P, s)) return -1;
             launches++;
         }
-        HIPCHK(ctx, hipEventRecord(ctx->ev0, s));
+        if (!noalloc) HIPCHK(ctx, hipEventRecord(ctx->ev0, s));
         if (launch(ctx, fn_main, (int)grid, threads, P, s, shmem)) return -1;
         launches++;
+    }
+    if (noalloc) {
+        if (order_null) {
+            HIPCHK(ctx, hipEventRecord(ctx->ev_order, s));
+            HIPCHK(ctx, hipStreamWaitEvent(nullptr, ctx->ev_order, 0));
+        }
+        a->launches = launches;
+        return 0;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, s));
     if (!dev) {
@@ -970,7 +1035,11 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
 }  // namespace
 
 extern "C" int odemi_solve_batch(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
-    return solve_impl(ctx, rhs, a);
+    return solve_impl(ctx, rhs, a, false);
+}
+
+extern "C" int odemi_solve_batch_noalloc(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a) {
+    return solve_impl(ctx, rhs, a, true);
 }
 
 // ===========================================================================

@@ -61,7 +61,7 @@ EXPORTS = ["odemi_create", "odemi_destroy", "odemi_last_error", "odemi_device_in
            "odemi_compile_rhs", "odemi_rhs_destroy", "odemi_solve_batch", "odemi_default_args",
            "odemi_rtc_codeobject", "odemi_server_default_opts", "odemi_server_create", "odemi_server_destroy",
            "odemi_submit", "odemi_poll", "odemi_wait", "odemi_request_times", "odemi_server_info",
-           "odemi_server_last_error", "odemi_request_release", "odemi_rtc_version", "odemi_submit_opts"]
+           "odemi_server_last_error", "odemi_request_release", "odemi_rtc_version", "odemi_submit_opts", "odemi_solve_batch_noalloc"]
 
 _lib = None
 
@@ -96,6 +96,8 @@ def load():
     L.odemi_rhs_destroy.argtypes = [c_vp]
     L.odemi_solve_batch.argtypes = [c_vp, c_vp, P(SolveArgs)]
     L.odemi_solve_batch.restype = c_i
+    L.odemi_solve_batch_noalloc.argtypes = [c_vp, c_vp, P(SolveArgs)]
+    L.odemi_solve_batch_noalloc.restype = c_i
     L.odemi_default_args.argtypes = [P(SolveArgs), c_i]
     L.odemi_rtc_codeobject.argtypes = [ctypes.c_char_p, c_i, c_i, c_i, ctypes.c_char_p, c_i, P(c_d),
                                        ctypes.c_char_p, c_i]
@@ -186,10 +188,13 @@ class Context:
             self._rhs[key] = r
         return r
 
-    def solve(self, rhs_handle, args):
-        rc = self.L.odemi_solve_batch(self.h, rhs_handle, ctypes.byref(args))
+    def solve(self, rhs_handle, args, noalloc=False):
+        """odemi_solve_batch, or with noalloc odemi_solve_batch_noalloc (device
+        pointers, persistent engine, asynchronous on args.stream)."""
+        fn = self.L.odemi_solve_batch_noalloc if noalloc else self.L.odemi_solve_batch
+        rc = fn(self.h, rhs_handle, ctypes.byref(args))
         if rc != 0:
-            raise RuntimeError(f"odemi_solve_batch failed ({rc}): {self.error()}")
+            raise RuntimeError(f"{fn.__name__} failed ({rc}): {self.error()}")
         return args
 
     def default_args(self, alg):

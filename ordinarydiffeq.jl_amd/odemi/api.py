@@ -89,19 +89,26 @@ class EnsembleMI355X:
                "plain" (FP64, one launch per step attempt, state in HBM).
     precision: "fp64", or "mixed" (Tsit5: FP32 stages, FP64 state/t/error/controller;
                Rodas5P: FP32 LU with FP64 iterative refinement).
+    allocate:  False runs odemi_solve_batch_noalloc: no device allocation by the
+               library (scratch from its fixed 64 MiB per-GPU array), device-resident
+               (torch) arrays and the persistent engine only; the solve is enqueued
+               on torch's current stream.
     """
 
     def __init__(self, device=0, engine="persistent", precision="fp64", refine_sweeps=1, sort_by_cost=True,
-                 block=0, blocks_per_cu=0, autotune=False):
+                 block=0, blocks_per_cu=0, autotune=False, allocate=True):
         if engine not in ("persistent", "plain"):
             raise ValueError("engine must be 'persistent' or 'plain'")
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if engine == "plain" and precision != "fp64":
             raise ValueError("the plain engine is FP64 only")
+        if not allocate and (engine != "persistent" or autotune):
+            raise ValueError("allocate=False runs the persistent engine without the autotuner")
         self.device, self.engine, self.precision = device, engine, precision
         self.refine_sweeps, self.sort_by_cost, self.block = refine_sweeps, sort_by_cost, block
         self.blocks_per_cu, self.autotune = int(blocks_per_cu), bool(autotune)
+        self.allocate = bool(allocate)
 
 
 # Autotuner choices, per context: (problem handle, alg, precision, N, nmax, save
@@ -556,7 +563,12 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         a.blocks_per_cu = _autotune(ctx, h, a, key, wg=(alg.code == _lib.RODAS5P and (ns is not None or n > LANE_MAX_RODAS)))
         o["tuning"] = dict(zip(("blocks_per_cu", "kernel_ms"), _TUNED[key]))
     t_s = time.perf_counter()
-    ctx.solve(h, a)
+    if not ea.allocate:
+        if not dev:
+            raise ValueError("allocate=False needs device-resident (torch CUDA) arrays")
+        ctx.solve(h, a, noalloc=True)
+    else:
+        ctx.solve(h, a)
     o["wall_s"] = time.perf_counter() - t_s
     o["kernel_ms"] = a.kernel_ms
     o["launches"] = a.launches
