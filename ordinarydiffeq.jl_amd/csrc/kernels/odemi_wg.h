@@ -14,16 +14,18 @@
 //   pd   double[32]                       parameters (FP64)
 //   fsc  float[nmax + 32]                 FP32 copies of uprev and p (mixed variant)
 //   piv  int[nmax], perm int[nmax]        pivots and the equivalent row gather
-//   gjbuf float[6 NP + 2 R NP]            Gauss-Jordan broadcasts and mat-vec
-//                                         result (mixed variant; NP = 16 R, R = ceil(nmax/16))
+//   gjbuf float[51 NP + 16]               Gauss-Jordan panels, pivot rows, scales and
+//                                         mat-vec result (mixed variant; NP = 16 R,
+//                                         R = ceil(nmax/16); layout in odemi_gj.h)
 //   red  double[32] + int[32]             reductions / broadcasts
 // FP64 variant: W is FP64 in LDS (132 KB at n = 128, one workgroup per CU),
 // factored by LU with partial pivoting.
 // Mixed variant (n <= 128): the FP32 Jacobian is evaluated into a per-workgroup
-// global scratch block, W = J - I/(dt*gamma) is held in registers as a 16 x 16
-// grid of R x R tiles (R = ceil(nmax/16)) and inverted in place by swap-free
-// Gauss-Jordan with partial pivoting, so each stage solve is one parallel
-// matrix-vector product instead of two serial substitutions.  Every stage solve
+// global scratch block, W = J - I/(dt*gamma) is held in the workgroup's
+// registers (R = ceil(n/16) per trajectory) and inverted in place by swap-free
+// Gauss-Jordan with partial pivoting (odemi_gj.h: panels of 16 columns on one
+// wave, rank-16 updates on the matrix cores), so each stage solve is one
+// parallel matrix-vector product instead of two serial substitutions.  Every stage solve
 // gets `refine_sweeps` FP64 iterative-refinement sweeps whose residual
 // b - (J x - x/(dt*gamma)) uses the traced Jacobian-vector product in FP64, so
 // the refined solution is that of the FP64 system.  LDS use is a few tens of KB,
@@ -35,6 +37,8 @@
 #define ODEMI_WG_THREADS 256
 #endif
 #define ODEMI_WG_ROWS ((ODEMI_N + 63) / 64)
+
+#include "odemi_gj.h"
 
 namespace odemi {
 
@@ -49,18 +53,20 @@ struct WGPtrs {
     float* pf;
     int* piv;
     int* perm;   // row gather: after the LU, (P b)[i] = b[perm[i]]; Gauss-Jordan: pivot row of step k
-    float* gjbuf;   // Gauss-Jordan broadcasts; zbuf = gjbuf + 5 NP + 2 R NP
+    float* gjbuf;   // Gauss-Jordan buffers; zbuf = gjbuf + gj_zbuf_off<R>()
     double* red;
     int* ired;
     int ld;
 };
 
-// Gauss-Jordan broadcast buffers (floats): column k and k+1 [2][NP], row staging
-// [2][R][NP], pivot-row scales [NP] and the mat-vec result zbuf [NP];
-// NP = 16 R, R = ceil(nmax / 16).
+// Gauss-Jordan LDS buffer (floats), NP = 16 R, R = ceil(nmax / 16): 51 NP +
+// 16 for the blocked form (two panels [NP][16], pivot rows [16][NP], scales,
+// pivot steps, mat-vec result, the current pivot row), which covers the
+// element-wise form's 6 NP + 2 R NP (column k and k+1 [2][NP], row staging
+// [2][R][NP], scales, result).
 __host__ __device__ inline int wg_gj_floats(int nmax, bool gj) {
     const int r = (nmax + 15) / 16, np16 = 16 * r;
-    return gj ? 4 * np16 + 2 * r * np16 + 2 * np16 : 0;
+    return gj ? 51 * np16 + 16 : 0;
 }
 
 // w_in_lds: W lives in LDS (LU variants); the Gauss-Jordan variant keeps it in registers.
@@ -292,290 +298,15 @@ __device__ void wg_wsolve(const WGPtrs& S, int n, double* x, double* rb, double*
     }
 }
 
-// ---------------------------------------------------------------------------
-// Register-tiled Gauss-Jordan (mixed variant).  Thread (ti, tj) = (tid / 16,
-// tid % 16) of the 256-thread workgroup holds rows ti + 16a and columns
-// tj + 16b (a, b < R) of the matrix, so the 16 threads sharing a row block are
-// one DPP row of a wave.
-// ---------------------------------------------------------------------------
-#if ODEMI_N <= 128
-#define ODEMI_GJ 1
-#define ODEMI_GJ_R ((ODEMI_N + 15) / 16)   // tile of the largest state (nmax)
-// Tile sizes R = ceil(n/16) present in the batch (bit R-1); the problem source
-// may define it from its state sizes, else every R up to ODEMI_GJ_R.
-#ifndef ODEMI_GJ_RMASK
-#define ODEMI_GJ_RMASK 0xff
-#endif
-
-// Sum over the 16 lanes of each DPP row; the total lands in the row's lane 15.
-// Fixed addition order (inclusive Hillis-Steele scan), so results are
-// deterministic.  All 64 lanes must be active.
-__device__ __forceinline__ float dpp_row_sum16(float x) {
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xf, 0xf, false));  // row_shr:1
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xf, 0xf, false));  // row_shr:2
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xf, 0xf, false));  // row_shr:4
-    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xf, 0xf, false));  // row_shr:8
-    return x;
-}
-// Maximum over the wave, returned to every lane.  All 64 lanes must be active.
-// Lanes without a source read 0, the identity of the unsigned maximum.
-__device__ __forceinline__ unsigned dpp_wave_max_u32(unsigned x) {
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true));
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, true));  // row_bcast:15
-    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, true));  // row_bcast:31
-    return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
-}
-
-// R x R tile per thread; a trajectory of n states uses R = ceil(n / 16)
-// (16 R >= n; rows and columns in [n, 16 R) are identity padding).
-template <int R>
-struct GJTile {
-    float T[R][R];
-};
-
-// T <- W = J - sigma I for the n x n leading block of J (FP32, row stride ldj),
-// identity outside it.  The diagonal shift is formed in FP64 and rounded once.
-template <int R>
-__device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, int n, double sigma) {
-    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
-#pragma unroll
-    for (int a = 0; a < R; a++)
-#pragma unroll
-        for (int b = 0; b < R; b++) {
-            const int i = ti + 16 * a, j = tj + 16 * b;
-            float v = (i == j) ? 1.0f : 0.0f;
-            if (i < n && j < n) {
-                v = J[i * ldj + j];
-                if (i == j) v = (float)fma(-1.0, sigma, (double)v);
-            }
-            G.T[a][b] = v;
-        }
-}
-
-// In-place Gauss-Jordan with partial pivoting and no row interchanges: step k
-// pivots on the largest |W[i][k]| among rows not yet used (ties to the smaller
-// row) and exchanges the roles of row p_k and column k.  On return
-// W^{-1} b = z[pk], z = T * b[pk] (pk[k] = p_k).
-//
-// The tile holds raw rows R with W = diag(sc) R.  A pivot row is not rescaled
-// in the tile: its scale sc_p becomes d = 1/W[p][k] and its raw entries stay,
-// so the only per-element work of a step is the rank-1 update of the other
-// rows, and no thread has to write a register chosen by a runtime index.
-// Rows not yet used keep sc = 1, so the pivot search and the pivot row read
-// raw values.  The scales (one LDS word per row) are folded into the tile at
-// the end.
-//
-// The step is one fma per element with no selects: column k is published with
-// 0 in the pivot row's slot (so the pivot row's update is a no-op), the
-// threads holding column k update it from zero with d in place of r[k] d (so
-// it becomes -c d), and the 1 that belongs at (p_k, k) is added at the end,
-// before the scales.  That entry is only ever updated afterwards, never read
-// as a pivot column or pivot row entry, so adding it last is exact algebra.
-// The pivot itself is read from the staged pivot row.
-//
-// One barrier per step.  Before the barrier that opens step k the owners
-// publish column k, column k+1 and (all R row slots of the thread row holding)
-// row p_k, as step k-1 left them.  After its update every wave forms column
-// k+1 as step k leaves it (the same fma as the owners' update, so
-// bit-identical) and searches it for p_{k+1}, so the owners of row p_{k+1}
-// can stage it before the next barrier.  Buffers alternate by step parity.
-//   cbuf, nbuf [2][16][R]: entry (ti, a) is row ti + 16 a
-//   stg [2][R][16][R]:     entry (a, tj, b) is T[a][b] of thread (p_k % 16, tj)
-__device__ __forceinline__ unsigned gj_key(float v, int i) {
-    // larger |v| wins, ties to the smaller row; a NaN entry competes as zero so
-    // the pivot is always an unused row < n (n <= 128)
-    const float a = fabsf(v);
-    return ((a != a) ? 0u : (__float_as_uint(a) & ~127u)) | (unsigned)(127 - i);
-}
-template <int R>
-__device__ __forceinline__ int gj_cidx(int i) { return (i & 15) * R + (i >> 4); }
-
-template <int R>
-__device__ __forceinline__ void gj_invert(GJTile<R>& G, int n, int* pk, float* gjbuf) {
-    constexpr int NP = 16 * R, NQ = (NP + 63) / 64;
-    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15, lane = tid & 63;
-    float* cbuf = gjbuf;
-    float* nbuf = gjbuf + 2 * NP;
-    float* stg = gjbuf + 4 * NP;
-    float* scl = gjbuf + 4 * NP + 2 * R * NP;   // [NP] scale of each pivot row
-    unsigned used = 0;   // bit q: row lane + 64 q has been a pivot row
-    // ---- prologue: columns 0 and 1, pivot p_0, row p_0 ----
-    if (tj == 0) {
-#pragma unroll
-        for (int a = 0; a < R; a++) cbuf[ti * R + a] = G.T[a][0];
-    }
-    if (tj == 1 && n > 1) {
-#pragma unroll
-        for (int a = 0; a < R; a++) nbuf[ti * R + a] = G.T[a][0];
-    }
-    __syncthreads();
-    int p;
-    {
-        unsigned key = 0;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            const int i = lane + 64 * q;
-            if (i < n) key = max(key, gj_key(cbuf[gj_cidx<R>(i)], i));
-        }
-        key = dpp_wave_max_u32(key);
-        p = 127 - (int)(key & 127u);
-    }
-    if (tid == 0) pk[0] = p;
-    if (ti == (p & 15)) {
-#pragma unroll
-        for (int a = 0; a < R; a++)
-#pragma unroll
-            for (int b = 0; b < R; b++) stg[(a * 16 + tj) * R + b] = G.T[a][b];
-    }
-    __syncthreads();
-    // every wave has searched column 0: its pivot slot can go to zero
-    if (tj == 0 && ti == (p & 15)) cbuf[ti * R + (p >> 4)] = 0.0f;
-    __syncthreads();
-    // ---- steps ----
-#pragma unroll
-    for (int kb = 0; kb < R; kb++) {
-        const int kend = min(16, n - 16 * kb);
-        for (int kt = 0; kt < kend; kt++) {
-            const int k = 16 * kb + kt, k1 = k + 1, k2 = k + 2;
-            const int par = k & 1;
-#pragma unroll
-            for (int q = 0; q < NQ; q++)
-                if (p == lane + 64 * q) used |= 1u << q;
-            const float* cb = cbuf + par * NP;
-            const float* nb = nbuf + par * NP;
-            const float* st = stg + par * R * NP;
-            const int pa = p >> 4;
-            const bool colk = (tj == kt);
-            // all reads of the step first (the search for p_{k+1} only needs
-            // these, so its latency overlaps the tile update).  At the last step
-            // k1 = n <= NP: the reads stay inside the buffers and are unused.
-            const float piv = st[(pa * 16 + kt) * R + kb];
-            float c[R], r[R], cq[NQ], nq[NQ];
-#pragma unroll
-            for (int a = 0; a < R; a++) c[a] = cb[ti * R + a];
-#pragma unroll
-            for (int b = 0; b < R; b++) r[b] = st[(pa * 16 + tj) * R + b];
-            const float r1 = st[(pa * 16 + (k1 & 15)) * R + (k1 >> 4)];
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const int ci = gj_cidx<R>(min(lane + 64 * q, NP - 1));
-                cq[q] = cb[ci];
-                nq[q] = nb[ci];
-            }
-            const float d = __builtin_amdgcn_rcpf(piv);
-            // pivot of step k+1 from column k+1 as step k leaves it (same fma as
-            // the owners' update)
-            const float rs1 = r1 * d;
-            unsigned key = 0;
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const int i = lane + 64 * q;
-                if (i < n && !((used >> q) & 1u)) key = max(key, gj_key(fmaf(-cq[q], rs1, nq[q]), i));
-            }
-            // rank-1 update (c is 0 on row p_k); column k becomes 0 - c d
-            float rd[R];
-#pragma unroll
-            for (int b = 0; b < R; b++) rd[b] = r[b] * d;
-            rd[kb] = colk ? d : rd[kb];
-#pragma unroll
-            for (int a = 0; a < R; a++)
-#pragma unroll
-                for (int b = 0; b < R; b++) {
-                    const float t = (b == kb && colk) ? 0.0f : G.T[a][b];
-                    G.T[a][b] = fmaf(-c[a], rd[b], t);
-                }
-            if (tid == 0) scl[p] = d;
-            key = dpp_wave_max_u32(key);
-            if (k1 >= n) break;
-            p = 127 - (int)(key & 127u);
-            if (tid == 0) pk[k1] = p;
-            // publish column k+1, column k+2 and row p_{k+1} for the next step
-            float* cbn = cbuf + (NP - par * NP);
-            float* nbn = nbuf + (NP - par * NP);
-            float* stn = stg + (R * NP - par * R * NP);
-            if (tj == (k1 & 15)) {
-                if (kt < 15) {
-#pragma unroll
-                    for (int a = 0; a < R; a++) cbn[ti * R + a] = G.T[a][kb];
-                } else {
-#pragma unroll
-                    for (int a = 0; a < R; a++) cbn[ti * R + a] = G.T[a][kb + 1 < R ? kb + 1 : kb];
-                }
-            }
-            if (k2 < n && tj == (k2 & 15)) {
-                if (kt < 14) {
-#pragma unroll
-                    for (int a = 0; a < R; a++) nbn[ti * R + a] = G.T[a][kb];
-                } else {
-#pragma unroll
-                    for (int a = 0; a < R; a++) nbn[ti * R + a] = G.T[a][kb + 1 < R ? kb + 1 : kb];
-                }
-            }
-            if (ti == (p & 15)) {
-#pragma unroll
-                for (int a = 0; a < R; a++)
-#pragma unroll
-                    for (int b = 0; b < R; b++) stn[(a * 16 + tj) * R + b] = G.T[a][b];
-            }
-            // the pivot slot of column k+1 goes to zero (its publisher wrote it
-            // above; nobody reads this parity's buffers before the barrier)
-            if (tj == (k1 & 15) && ti == (p & 15)) cbn[ti * R + (p >> 4)] = 0.0f;
-            __syncthreads();
-        }
-    }
-    __syncthreads();
-    float pjv[R];
-#pragma unroll
-    for (int b = 0; b < R; b++) {
-        const int j = tj + 16 * b;
-        pjv[b] = (j < n) ? (float)pk[j] : -1.0f;
-    }
-#pragma unroll
-    for (int a = 0; a < R; a++) {
-        const int i = ti + 16 * a;
-        const float sa = (i < n) ? scl[i] : 1.0f;
-#pragma unroll
-        for (int b = 0; b < R; b++) {
-            const float v = (pjv[b] == (float)i) ? G.T[a][b] + 1.0f : G.T[a][b];
-            G.T[a][b] = v * sa;
-        }
-    }
-}
-
-// x <- W^{-1} x (x FP64 in LDS, accumulated in FP32).  All threads must call.
-template <int R>
-__device__ __forceinline__ void gj_solve(const GJTile<R>& G, int n, const int* pk, double* x, float* zbuf) {
-    const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-    float bp[R];
-#pragma unroll
-    for (int b = 0; b < R; b++) {
-        const int j = tj + 16 * b;
-        bp[b] = (j < n) ? (float)x[pk[j]] : 0.0f;
-    }
-#pragma unroll
-    for (int a = 0; a < R; a++) {
-        float z = 0.0f;
-#pragma unroll
-        for (int b = 0; b < R; b++) z = fmaf(G.T[a][b], bp[b], z);
-        z = dpp_row_sum16(z);
-        if (tj == 15) zbuf[ti + 16 * a] = z;
-    }
-    __syncthreads();
-    for (int k = tid; k < n; k += blockDim.x) x[k] = (double)zbuf[pk[k]];
-    __syncthreads();
-}
-
+#if ODEMI_GJ
 // Stage solve W x = b for the Gauss-Jordan variant (b in x on entry), with the
-// same FP64 refinement sweeps as wg_wsolve.  The buffers of gj_invert<R> sit at
-// the front of gjbuf (sized for the largest R), zbuf after them.
+// same FP64 refinement sweeps as wg_wsolve.  gjbuf holds gj_invert<R>'s
+// buffers (sized for the largest R), zbuf at gj_zbuf_off<R>().
 template <int R>
 __device__ __forceinline__ void wg_wsolve_gj(const GJTile<R>& G, const WGPtrs& S, int n, double* x, double* rb, double* r, double* d,
                              const double* uprev, const double* p, double t, double sigma, int sweeps) {
     const int tid = threadIdx.x, NT = blockDim.x;
-    float* zbuf = S.gjbuf + 5 * 16 * R + 2 * R * 16 * R;
+    float* zbuf = S.gjbuf + gj_zbuf_off<R>();
     for (int i = tid; i < n; i += NT) rb[i] = x[i];
     __syncthreads();
     gj_solve<R>(G, n, S.perm, x, zbuf);
@@ -589,9 +320,7 @@ __device__ __forceinline__ void wg_wsolve_gj(const GJTile<R>& G, const WGPtrs& S
         __syncthreads();
     }
 }
-#else
-#define ODEMI_GJ 0
-#endif
+#endif  // ODEMI_GJ
 
 __device__ __forceinline__ void wg_save_point(Ctl& c, double& tlast, const Params& P, i64 j, double t,
                                               const double* v, int n) {

@@ -345,6 +345,10 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
     const char* rm = getenv("ODEMI_REFILL_MIN");
     if (rm && *rm && atoi(rm) >= 1 && atoi(rm) <= 64) s << "#define ODEMI_REFILL_MIN " << atoi(rm) << "\n";
     if (flags & ODEMI_COMPILE_WG) s << "#define ODEMI_WITH_WG 1\n";
+    // Gauss-Jordan form of the mixed workgroup kernels: 1 blocked with MFMA
+    // rank-16 updates (default), 0 element-wise rank-1 updates
+    const char* gm = getenv("ODEMI_GJ_MFMA");
+    if (gm && *gm) s << "#define ODEMI_GJ_MFMA " << (atoi(gm) ? 1 : 0) << "\n";
     s << problem_src << "\n";
     // 4 waves per SIMD (128 VGPRs); with continuous callbacks the event search
     // spills at that cap, and 3 measured 8 % faster (1M bouncing balls, FP64
@@ -648,12 +652,12 @@ int ensure_ws(odemi_ctx* c, size_t bytes) {
 // Dynamic LDS of the workgroup kernels; must match odemi::wg_layout in kernels/odemi_wg.h.
 size_t wg_shmem_bytes(int nmax, bool mixed, bool saves, bool gj) {
     const size_t ld = (size_t)nmax + 1, fb = mixed ? 4 : 8;
-    const size_t np16 = 16 * (((size_t)nmax + 15) / 16), r16 = np16 / 16;
+    const size_t np16 = 16 * (((size_t)nmax + 15) / 16);
     size_t shmem = gj ? 0 : (((size_t)nmax * ld * fb + 15) & ~(size_t)15);
     shmem += (size_t)(13 + (saves ? 3 : 0)) * nmax * 8 + 32 * 8;  // vectors + parameters (<= 32)
     shmem += (size_t)(nmax + 32) * 4 + (size_t)nmax * 8;          // fsc + pf, piv + perm
     shmem = (shmem + 15) & ~(size_t)15;
-    if (gj) shmem += (4 * np16 + 2 * r16 * np16 + 2 * np16) * 4;   // Gauss-Jordan buffers + zbuf
+    if (gj) shmem += (51 * np16 + 16) * 4;                        // Gauss-Jordan buffers + zbuf
     shmem = (shmem + 15) & ~(size_t)15;
     shmem += 32 * 8 + 32 * 4;
     return shmem;
