@@ -198,6 +198,18 @@ def rodas_legs(args, workload, steps, warmup, cpu=True, plain=True):
     odemi.compile_problem(fun, d["u0"].shape[0], d["p"].shape[0], algs=("rodas5p",),
                           ns=None if nvec is None else sorted(set(int(x) for x in nvec)))
     lane = nvec is None
+    if getattr(args, "no_fp64", False):
+        # profiling aid: the mixed engine alone (no FP64 leg, no comparisons)
+        om, msm, kmsm = _timed(lambda: run(mixed), max(1, steps), warmup, torch)
+        attm, roofm = _rodas_roofline(om, nvec, kmsm, "odemi_rodas_persist_mixed" if lane else "odemi_rodas_wg_mixed",
+                                      PEAK_FP64_TFLOPS if lane else PEAK_FP32_TFLOPS,
+                                      "valu-fp64" if lane else "lds/valu-fp32", None)
+        return {"metric": "ensemble trajectories/s at reltol 1e-6 (Rodas5P), persistent mixed engine only",
+                "value": N / (msm / 1e3), "unit": "trajectories/s", "n_gpus": 1, "steps": max(1, steps),
+                "warmup": warmup, "ms_per_step": msm, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32 solves + f64 state (mixed)", "data": "synthetic (seeded PCG64)",
+                "config": {"workload": desc, "trajectories": N}, "step_attempts": attm,
+                "success_fraction": float((om["retcode"].cpu().numpy() == 1).mean()), "roofline": roofm}
     o64, ms64, kms64 = _timed(lambda: run(fp64), steps, warmup, torch)
     att64, roof64 = _rodas_roofline(o64, nvec, kms64, "odemi_rodas_persist_f64" if lane else "odemi_rodas_wg_f64",
                                     PEAK_FP64_TFLOPS, "valu-fp64",
@@ -556,6 +568,8 @@ def main():
     ap.add_argument("--chain-max", type=int, default=16, help="serving: chains per chain request, max")
     ap.add_argument("--hbm-budget-gb", type=int, default=8, help="serving: slot-pool budget, GiB")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="robertson/chain: time the mixed engine alone (a profiling aid; no comparisons)")
     ap.add_argument("--plain-sample", type=int, default=2048,
                     help="chains: trajectories of the plain-vs-batched comparison")
     ap.add_argument("--no-robertson", action="store_true", help="default line: skip the Robertson leg")

@@ -10,7 +10,7 @@ __device__ void gj_check_r(const float* J, int n, double* x) {
     int* pk = (int*)sh;
     float* gjbuf = sh + 128;
     odemi::GJTile<R> G;
-    odemi::gj_load<R>(G, J, n, n, 1.0);
+    odemi::gj_load<R>(G, J, n, 1.0);
     odemi::gj_invert<R>(G, n, pk, gjbuf);
     odemi::gj_solve<R>(G, n, pk, x, gjbuf + odemi::gj_zbuf_off<R>());
 }

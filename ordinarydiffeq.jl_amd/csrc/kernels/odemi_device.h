@@ -82,7 +82,7 @@ struct Params {
     i64 slot0, nslots;        // init kernel: slot range to initialise
     const int* slot_req;      // [N] request table index of each slot
     u64* req_done;            // [R] finished trajectories per request
-    float* wg_jbuf;           // workgroup Gauss-Jordan kernel: [grid][nmax][nmax] FP32 Jacobian scratch
+    float* wg_jbuf;           // workgroup Gauss-Jordan kernel: [grid][LD][LD] FP32 Jacobian scratch, LD = 16 ceil(nmax/16)
     // start-up values computed ahead of the persistent lane kernels (or null):
     double* pre_dt;           // [N] initial dt (initdt or the user's dt)
     double* pre_k1;           // [nmax][N] Tsit5 FSAL stage f(u0), exact in FP64

@@ -24,6 +24,9 @@
 #ifndef ODEMI_GJ_MFMA
 #define ODEMI_GJ_MFMA 1
 #endif
+// Row stride of each workgroup's FP32 Jacobian block, which holds LD x LD
+// floats (a compile-time stride: immediate load offsets).
+#define ODEMI_GJ_LD (16 * ODEMI_GJ_R)
 
 namespace odemi {
 
@@ -82,10 +85,13 @@ struct GJTile {
 template <int R>
 __device__ __forceinline__ constexpr int gj_zbuf_off() { return 50 * 16 * R; }
 
-// T <- W = J - sigma I for the n x n leading block of J (FP32, row stride ldj),
-// identity outside it.  The diagonal shift is formed in FP64 and rounded once.
+// T <- W = J - sigma I for the n x n leading block of J (FP32, row stride
+// ODEMI_GJ_LD), identity outside it.  The diagonal shift is formed in FP64 and
+// rounded once.  (Loading the whole tile unconditionally and selecting the
+// padding afterwards measured 4 % slower on uneven chains: short chains then
+// load their full padded tile.)
 template <int R>
-__device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, int n, double sigma) {
+__device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int n, double sigma) {
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, g = l >> 4, c = l & 15;
 #pragma unroll
     for (int m = 0; m < GJTile<R>::MB; m++) {
@@ -97,7 +103,7 @@ __device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, i
                 const int i = 16 * I + 4 * g + r, j = 16 * b + c;
                 float v = (i == j) ? 1.0f : 0.0f;
                 if (i < n && j < n) {
-                    v = J[i * ldj + j];
+                    v = J[i * ODEMI_GJ_LD + j];
                     if (i == j) v = (float)fma(-1.0, sigma, (double)v);
                 }
                 G.D[m][b][r] = v;
@@ -247,23 +253,20 @@ __device__ __forceinline__ void gj_invert(GJTile<R>& G, int n, int* pk, float* g
         // ---- rank-16 update of every column block (4 MFMAs per block, the
         // blocks interleaved); then the panel's block takes the multipliers
         // and the next panel is published ----
-        float bv[R][4];
-#pragma unroll
-        for (int J = 0; J < R; J++)
-#pragma unroll
-            for (int kc = 0; kc < 4; kc++) bv[J][kc] = wp[(4 * kc + g) * NP + 16 * J + c];
 #pragma unroll
         for (int m = 0; m < MB; m++) {
             const int I = w + 4 * m;
             if (I < R) {
-                float a[4];
 #pragma unroll
-                for (int kc = 0; kc < 4; kc++) a[kc] = pan[(16 * I + c) * 16 + 4 * kc + g];
+                for (int kc = 0; kc < 4; kc++) {
+                    const float a = pan[(16 * I + c) * 16 + 4 * kc + g];
+                    float bv[R];
 #pragma unroll
-                for (int kc = 0; kc < 4; kc++)
+                    for (int J = 0; J < R; J++) bv[J] = wp[(4 * kc + g) * NP + 16 * J + c];
 #pragma unroll
                     for (int J = 0; J < R; J++)
-                        G.D[m][J] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kc], bv[J][kc], G.D[m][J], 0, 0, 0);
+                        G.D[m][J] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[J], G.D[m][J], 0, 0, 0);
+                }
 #pragma unroll
                 for (int J = 0; J < R; J++)
                     if (J == b) {
@@ -344,10 +347,10 @@ struct GJTile {
     float T[R][R];
 };
 
-// T <- W = J - sigma I for the n x n leading block of J (FP32, row stride ldj),
-// identity outside it.  The diagonal shift is formed in FP64 and rounded once.
+// T <- W = J - sigma I for the n x n leading block of J (FP32, row stride
+// ODEMI_GJ_LD), identity outside it.
 template <int R>
-__device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, int n, double sigma) {
+__device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int n, double sigma) {
     const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
 #pragma unroll
     for (int a = 0; a < R; a++)
@@ -356,7 +359,7 @@ __device__ __forceinline__ void gj_load(GJTile<R>& G, const float* J, int ldj, i
             const int i = ti + 16 * a, j = tj + 16 * b;
             float v = (i == j) ? 1.0f : 0.0f;
             if (i < n && j < n) {
-                v = J[i * ldj + j];
+                v = J[i * ODEMI_GJ_LD + j];
                 if (i == j) v = (float)fma(-1.0, sigma, (double)v);
             }
             G.T[a][b] = v;

@@ -437,7 +437,7 @@ __device__ __forceinline__ void wg_begin(const WGPtrs& S, const Params& P, i64 j
     // the traced Jacobian writes the same entries on every call for a given n,
     // so the rest of the block is cleared once per trajectory
     if constexpr (GJ)
-        for (int e = tid; e < n * P.nmax; e += NT) Jg[e] = 0.0f;
+        for (int e = tid; e < n * ODEMI_GJ_LD; e += NT) Jg[e] = 0.0f;
 #endif
     __syncthreads();
     if (P.save_start) wg_save_point(c, tlast, P, j, c.t, S.u, n);
@@ -582,7 +582,7 @@ __device__ void wg_addsteps(const WGPtrs& S, const Params& P, const Ctl& c, int 
     for (int i = tid; i < n; i += NT) ks[i] = fma(dtgamma, S.dT[i], S.fsal[i]);
     if constexpr (GJ) {
 #if ODEMI_GJ
-        gj_load<R>(G, Jg, nmax, n, sigma);
+        gj_load<R>(G, Jg, n, sigma);
         gj_invert<R>(G, n, S.perm, S.gjbuf);
 #endif
     } else {
@@ -837,7 +837,7 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
             __syncthreads();
             if constexpr (GJ) {
 #if ODEMI_GJ
-                odemi_jac_wg<float>(Jg, nmax, S.fsc, pf, (float)t, n, tid, NT);
+                odemi_jac_wg<float>(Jg, ODEMI_GJ_LD, S.fsc, pf, (float)t, n, tid, NT);
 #endif
             } else if constexpr (sizeof(F) == sizeof(double)) {
                 odemi_jac_wg<double>((double*)W, ld, uprev, pp, t, n, tid, NT);
@@ -863,7 +863,7 @@ __device__ __forceinline__ int wg_run(const WGPtrs& S, const Params& P, i64 j, i
             // rb = fsal, r = du, d = u are free during the stage-1 solve
             if constexpr (GJ) {
 #if ODEMI_GJ
-                gj_load<R>(G, Jg, nmax, n, sigma);
+                gj_load<R>(G, Jg, n, sigma);
                 gj_invert<R>(G, n, S.perm, S.gjbuf);
                 wg_wsolve_gj<R>(G, S, n, ks, fsal, du, u, uprev, pp, t, sigma, P.refine_sweeps);
 #endif
@@ -1070,7 +1070,7 @@ __device__ __forceinline__ void rodas_wg_impl(const Params& P) {
     const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, SAVES || ODEMI_WG_EVENTS, !GJ);
     float* Jg = nullptr;
 #if ODEMI_GJ
-    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
+    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * ODEMI_GJ_LD * ODEMI_GJ_LD;
 #endif
     for (;;) {
         if (tid == 0) ((i64*)S.red)[30] = (i64)atomicAdd(&P.counters[0], 1ULL);
@@ -1126,7 +1126,7 @@ __device__ __forceinline__ void rodas_wg_serve_init(const Params& P) {
     const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, ODEMI_WG_EVENTS != 0, !GJ);
     float* Jg = nullptr;
 #if ODEMI_GJ
-    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
+    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * ODEMI_GJ_LD * ODEMI_GJ_LD;
 #endif
     for (i64 q = blockIdx.x; q < P.nslots; q += gridDim.x) {
         const i64 j = P.slot0 + q;
@@ -1149,7 +1149,7 @@ __device__ __forceinline__ void rodas_wg_serve_tick(const Params& P) {
     const WGPtrs S = wg_layout<F>(odemi_lds, P.nmax, ODEMI_WG_EVENTS != 0, !GJ);
     float* Jg = nullptr;
 #if ODEMI_GJ
-    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * P.nmax * P.nmax;
+    if constexpr (GJ) Jg = P.wg_jbuf + (i64)blockIdx.x * ODEMI_GJ_LD * ODEMI_GJ_LD;
 #endif
     for (;;) {
         if (tid == 0) ((i64*)S.red)[30] = (i64)atomicAdd(&P.counters[0], 1ULL);
@@ -1165,7 +1165,7 @@ __device__ __forceinline__ void rodas_wg_serve_tick(const Params& P) {
         wg_load_state(S, Q, j, n, c);
 #if ODEMI_GJ
         if constexpr (GJ)
-            for (int e = tid; e < n * P.nmax; e += blockDim.x) Jg[e] = 0.0f;
+            for (int e = tid; e < n * ODEMI_GJ_LD; e += blockDim.x) Jg[e] = 0.0f;
 #endif
         __syncthreads();
         double tlast = c.t;

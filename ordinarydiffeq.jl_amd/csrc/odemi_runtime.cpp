@@ -649,6 +649,10 @@ int ensure_ws(odemi_ctx* c, size_t bytes) {
     return 0;
 }
 
+// Row stride (and side) of each workgroup's FP32 Jacobian block in the mixed
+// workgroup kernels: 16 ceil(nmax / 16), as ODEMI_GJ_LD in kernels/odemi_gj.h.
+size_t gj_ld(int nmax) { return 16 * (((size_t)nmax + 15) / 16); }
+
 // Dynamic LDS of the workgroup kernels; must match odemi::wg_layout in kernels/odemi_wg.h.
 size_t wg_shmem_bytes(int nmax, bool mixed, bool saves, bool gj) {
     const size_t ld = (size_t)nmax + 1, fb = mixed ? 4 : 8;
@@ -817,7 +821,7 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
     }
     if (plain) { add(16 * N_ * 8); add(vrows * nmax * N_ * 8); add(kIntRows * N_ * 4); }
     if (plain_wg) { add(N_ * 4); if (!a->nvec) add(N_ * 4); }   // slot -> request (all 0); nvec = nmax
-    if (use_gj) add((size_t)grid * nmax * nmax * 4);
+    if (use_gj) add((size_t)grid * gj_ld(nmax) * gj_ld(nmax) * 4);
     if (fn_prep) { add(N_ * 8); if (!rodas) add(N_ * nmax * 8); }
     if (!dev) {
         add(N_ * nmax * 8); add(N_ * std::max(np, 1) * 8); add(N_ * 8); add(N_ * 8);
@@ -922,7 +926,7 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
         hipLaunchKernelGGL(fill_f64_kernel, dim3(g), dim3(256), 0, s, P.st_scalar + 11 * N_, (int64_t)N, P.abstol);
         HIPCHK(ctx, hipGetLastError());
     }
-    if (use_gj) P.wg_jbuf = (float*)ar.take((size_t)grid * nmax * nmax * 4);
+    if (use_gj) P.wg_jbuf = (float*)ar.take((size_t)grid * gj_ld(nmax) * gj_ld(nmax) * 4);
     if (fn_prep) {
         P.pre_dt = (double*)ar.take(N_ * 8);
         if (!rodas) P.pre_k1 = (double*)ar.take(N_ * nmax * 8);
@@ -1177,7 +1181,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     // bytes per slot: inputs, state, outputs
     const size_t per_slot = (size_t)(n + np + 2) * 8 + 16 * 8 + (size_t)S->rows * n * 8 + kIntRows * 4 +
                             (size_t)n * 8 + 8 + 4 + 4 + kStatRows * 4 + 4 + (wg ? 4 : 0);
-    const size_t fixed = S->gj ? (size_t)S->wg_grid * n * n * 4 : 0;
+    const size_t fixed = S->gj ? (size_t)S->wg_grid * gj_ld(n) * gj_ld(n) * 4 : 0;
     int64_t slots = opts->max_slots > 0 ? opts->max_slots : (1 << 20);
     if (opts->hbm_budget_bytes > 0)
         slots = std::min<int64_t>(slots, opts->hbm_budget_bytes > fixed
@@ -1191,7 +1195,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     size_t o_sc = al(16 * Sz * 8), o_vec = al((size_t)S->rows * n * Sz * 8), o_int = al(kIntRows * Sz * 4);
     size_t o_uf = al(Sz * n * 8), o_tfin = al(Sz * 8), o_rc = al(Sz * 4), o_st = al(kStatRows * Sz * 4), o_ns = al(Sz * 4);
     size_t o_sreq = al(Sz * 4), o_done = al((size_t)S->R * 8), o_cnt = al(64);
-    size_t o_nvec = wg ? al(Sz * 4) : 0, o_jb = S->gj ? al((size_t)S->wg_grid * n * n * 4) : 0;
+    size_t o_nvec = wg ? al(Sz * 4) : 0, o_jb = S->gj ? al((size_t)S->wg_grid * gj_ld(n) * gj_ld(n) * 4) : 0;
     if (hipMalloc(&S->mem, bytes) != hipSuccess) { ctx->err = "serving pool allocation failed"; delete S; return nullptr; }
     char* b = (char*)S->mem;
     (void)hipMemset(b + o_int, 0, kIntRows * Sz * 4);

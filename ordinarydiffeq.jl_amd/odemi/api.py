@@ -16,6 +16,7 @@ import builtins
 import enum
 import math
 import time
+import warnings
 from fractions import Fraction
 
 import numpy as np
@@ -109,6 +110,27 @@ class EnsembleMI355X:
         self.refine_sweeps, self.sort_by_cost, self.block = refine_sweeps, sort_by_cost, block
         self.blocks_per_cu, self.autotune = int(blocks_per_cu), bool(autotune)
         self.allocate = bool(allocate)
+
+
+class MixedPrecisionWarning(UserWarning):
+    """A mixed-precision solve that is likely slower than the FP64 engine."""
+
+
+def _mixed_precision_advice(ea, algname, callback):
+    """The reason a mixed-precision solve is likely slower than FP64, or None.
+
+    Tsit5 with continuous callbacks: near an event the FP32 stage rounding puts
+    a floor under the error estimate, so the controller keeps rejecting and
+    shrinking steps (1,048,576 bouncing balls: 4.1x the step attempts and 3.3x
+    the time of the FP64 engine, whose results are also bit-identical to the
+    reference restatement)."""
+    if ea.precision != "mixed" or algname != "tsit5":
+        return None
+    if any(isinstance(c, (ContinuousCallback, VectorContinuousCallback)) for c in _callback_list(callback)):
+        return ("precision='mixed' with continuous callbacks under Tsit5: FP32 stage rounding floors the error "
+                "estimate near events, costing several times the step attempts of precision='fp64' "
+                "(4.1x on 1M bouncing balls); precision='fp64' is faster here and bit-identical to the reference")
+    return None
 
 
 # Autotuner choices, per context: (problem handle, alg, precision, N, nmax, save
@@ -483,6 +505,9 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
             raise ValueError("nvec entries must lie in [1, u0.shape[0]]")
     h, tr, csecs = compile_problem(fun, n, np_, algs=(algname,), ctx=ctx, ns=ns, isoutofdomain=isoutofdomain,
                                    callback=callback)
+    advice = _mixed_precision_advice(ea, algname, callback)
+    if advice:
+        warnings.warn(advice, MixedPrecisionWarning, stacklevel=2)
     reltol = 1e-3 if reltol is None else float(reltol)
     abstol = 1e-6 if abstol is None else float(abstol)
     sa = np.zeros(0) if saveat is None else np.asarray(saveat, dtype=np.float64)

@@ -267,3 +267,24 @@ def test_workgroup_callbacks_emit_per_size_and_compile():
     with pytest.raises(ValueError, match="VectorContinuousCallback"):
         trace_for(P.CHAIN, 12, 3, algs=("rodas5p",), ns=[8, 12],
                   callback=odemi.VectorContinuousCallback(vc, lambda itg, ev: None, 1))
+
+
+def test_mixed_precision_advice_for_tsit5_continuous_callbacks():
+    """Host check: the mixed engine is flagged for Tsit5 with a continuous
+    callback (FP32 stage rounding floors the error estimate near events), and
+    not for FP64, for Rodas5P, or for discrete callbacks alone."""
+    from odemi import api
+    cond = lambda u, t, integ: u[0]  # noqa: E731
+
+    def affect(integ):
+        integ.u[1] = -integ.u[1]
+
+    cc = api.ContinuousCallback(cond, affect)
+    dc = api.DiscreteCallback(lambda u, t, integ: t > 1.0, affect)
+    mixed, fp64 = api.EnsembleMI355X(precision="mixed"), api.EnsembleMI355X(precision="fp64")
+    assert "precision='fp64'" in api._mixed_precision_advice(mixed, "tsit5", cc)
+    assert api._mixed_precision_advice(mixed, "tsit5", api.CallbackSet(dc, cc)) is not None
+    assert api._mixed_precision_advice(fp64, "tsit5", cc) is None
+    assert api._mixed_precision_advice(mixed, "rodas5p", cc) is None
+    assert api._mixed_precision_advice(mixed, "tsit5", dc) is None
+    assert api._mixed_precision_advice(mixed, "tsit5", None) is None
