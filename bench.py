@@ -231,7 +231,7 @@ def rodas_legs(args, workload, steps, warmup, cpu=True, plain=True):
                                   PEAK_FP64_TFLOPS if lane else PEAK_FP32_TFLOPS,
                                   "valu-fp64" if lane else "lds/valu-fp32",
                                   (_traffic("odemi_rodas_persist_mixed@robertson_262k") if lane and N == 262144
-                                   else _traffic("odemi_rodas_wg_mixed@chain_16k") if not lane and N == 16384
+                                   else _traffic("odemi_rodas_wg_mixed@chain_2048") if not lane and N == 2048
                                    else None))
     u64, um = o64["u_final"].cpu().numpy(), om["u_final"].cpu().numpy()
     err = (np.abs(um - u64) / (d["abstol"] + d["reltol"] * np.abs(u64))).max(axis=0)
