@@ -117,7 +117,8 @@ void odemi_destroy(odemi_ctx* ctx);
 const char* odemi_last_error(const odemi_ctx* ctx);
 /* compute units, free and total HBM bytes of the ctx device */
 int odemi_device_info(odemi_ctx* ctx, int* cu_count, size_t* free_bytes, size_t* total_bytes);
-/* Version / ABI tag. */
+/* Version / ABI tag: 4 (odemi_server_opts with max_saves / max_tstops,
+ * odemi_request_opts with save and stopping times). */
 int odemi_abi_version(void);
 
 /* Compile a traced problem with hipRTC for gfx950.  `problem_src` defines
@@ -204,8 +205,9 @@ void odemi_default_args(odemi_solve_args* args, int alg);
  * the device between ticks, advances all active trajectories together, and
  * copies a request's results to the caller's buffers when its last trajectory
  * finishes.  A trajectory's result does not depend on what else is in flight.
- * A request may carry its own tolerances (odemi_submit_opts); the other
- * options are the server's, and requests return the final state only.
+ * A request may carry its own tolerances, save times and stopping times
+ * (odemi_submit_opts; save times and stopping times on lane servers, within
+ * the server's max_saves / max_tstops); the other options are the server's.
  * ------------------------------------------------------------------------- */
 typedef struct odemi_server odemi_server;
 typedef struct {
@@ -218,6 +220,9 @@ typedef struct {
     size_t hbm_budget_bytes;  /* > 0: cap the pool so it fits this many bytes */
     int tick_attempts;        /* step attempts per trajectory per tick */
     int refine_sweeps;
+    int max_saves;            /* saved points per trajectory a request may ask for
+                                 (save times + start + end; 0: final state only) */
+    int max_tstops;           /* stopping times per request (0: none) */
 } odemi_server_opts;
 
 void odemi_server_default_opts(odemi_server_opts* opts, int alg);
@@ -232,14 +237,34 @@ void odemi_server_destroy(odemi_server* server);
  * may be NULL).  Returns a request id (> 0) or a negative error. */
 int64_t odemi_submit(odemi_server* server, int64_t N, const int32_t* nvec, const double* u0, const double* p,
                      const double* t0, const double* tf, double* u_final, int32_t* retcode, int32_t* stats);
-/* Per-request options (odemi_submit_opts); a value <= 0 takes the server's.
- * No counterpart in the reference beyond the per-solve keywords it mirrors:
- * reltol / abstol of `solve(prob, alg; reltol, abstol)`
- * (lib/OrdinaryDiffEqCore/src/solve.jl:128-203). */
+/* Per-request options (odemi_submit_opts).  No counterpart in the reference
+ * beyond the per-solve keywords it mirrors: reltol, abstol, saveat,
+ * save_start, save_end and tstops of `solve(prob, alg; kw...)`
+ * (lib/OrdinaryDiffEqCore/src/solve.jl:128-203), shared by the request's
+ * trajectories.
+ *   reltol, abstol: <= 0 takes the server's;
+ *   saveat [nsaveat]: save times, sorted ascending; save_start, save_end and
+ *     save_end_user as in odemi_solve_args (the request's points follow the
+ *     same rules as a batch solve with those arguments); with t_out non-NULL
+ *     the saved points are written to t_out [max_saves][N], u_out
+ *     [max_saves][n][N] and nsaved [N] (points past max_saves are counted in
+ *     nsaved, not written), max_saves being the server's, which nsaveat +
+ *     save_start + save_end must not exceed;
+ *   tstops [ntstops]: stopping times, sorted ascending, at most the server's
+ *     max_tstops.
+ * Save times and stopping times need a lane (Tsit5 / Rodas5P, fixed n) server. */
 typedef struct {
     double reltol, abstol;
+    const double* saveat;
+    int64_t nsaveat;
+    int save_start, save_end, save_end_user;
+    double* t_out;
+    double* u_out;
+    int32_t* nsaved;
+    const double* tstops;
+    int64_t ntstops;
 } odemi_request_opts;
-/* odemi_submit with the request's own tolerances (ro may be NULL). */
+/* odemi_submit with the request's own options (ro may be NULL). */
 int64_t odemi_submit_opts(odemi_server* server, int64_t N, const int32_t* nvec, const double* u0, const double* p,
                           const double* t0, const double* tf, const odemi_request_opts* ro, double* u_final,
                           int32_t* retcode, int32_t* stats);

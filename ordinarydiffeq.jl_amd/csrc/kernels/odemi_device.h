@@ -40,6 +40,7 @@ enum { RC_DEFAULT = 0, RC_SUCCESS = 1, RC_TERMINATED = 2, RC_DTNAN = 3, RC_MAXIT
        RC_DTLESSTHANMIN = 6, RC_UNSTABLE = 7 };
 enum { ST_NF = 0, ST_NACCEPT = 1, ST_NREJECT = 2, ST_NJACS = 3, ST_NW = 4, ST_NSOLVE = 5,
        ST_NCONDITION = 6, ST_COUNT = 7 };
+constexpr int kReqOpts = 8;   // ints per serving request-table entry (Params::req_opts)
 
 // Kernel argument block (one struct, passed by value).  All arrays are SoA over
 // the N trajectories of the batch: x[i * N + j] is component i of trajectory j.
@@ -82,6 +83,10 @@ struct Params {
     i64 slot0, nslots;        // init kernel: slot range to initialise
     const int* slot_req;      // [N] request table index of each slot
     u64* req_done;            // [R] finished trajectories per request
+    const double* req_saveat; // [R][sa_cap] each request's save times (lane servers) or null
+    const double* req_tstops; // [R][ts_cap] each request's stopping times
+    const int* req_opts;      // [R][kReqOpts] nsaveat, ntstops, save_start, save_end, save_end_user
+    int sa_cap, ts_cap;
     float* wg_jbuf;           // workgroup Gauss-Jordan kernel: [grid][LD][LD] FP32 Jacobian scratch, LD = 16 ceil(nmax/16)
     // start-up values computed ahead of the persistent lane kernels (or null):
     double* pre_dt;           // [N] initial dt (initdt or the user's dt)
@@ -2181,12 +2186,25 @@ __device__ __forceinline__ void tsit5_store_state(const Tsit5Lane<S, K, N>& L, c
         P.st_vec[((i64)3 * P.nmax + i) * M + j] = (double)L.k7[i];
     }
 }
-// A serving slot's own tolerances: its request's reltol / abstol, written by
-// the host into rows 10 / 11 of the pool's scalar state at admission.
+// A serving slot's own options: its request's reltol / abstol, written by the
+// host into rows 10 / 11 of the pool's scalar state at admission, and (lane
+// servers with saves or stopping times) its request's save times, stopping
+// times and save flags from the request table.
 __device__ __forceinline__ Params slot_params(const Params& P, i64 j) {
     Params Q = P;
     Q.reltol = P.st_scalar[10 * P.N + j];
     Q.abstol = P.st_scalar[11 * P.N + j];
+    if (P.req_opts) {
+        const int r = P.slot_req[j];
+        const int* o = P.req_opts + kReqOpts * (i64)r;
+        Q.nsaveat = o[0];
+        Q.saveat = P.req_saveat + (i64)r * P.sa_cap;
+        Q.ntstops = o[1];
+        Q.tstops = P.req_tstops + (i64)r * P.ts_cap;
+        Q.save_start = o[2];
+        Q.save_end = o[3];
+        Q.save_end_user = o[4];
+    }
     return Q;
 }
 
@@ -2201,7 +2219,7 @@ __device__ void tsit5_serve_init(const Params& P0) {
     plain_store_ctl<N>(L.c, P, j, 1);
     tsit5_store_state(L, P, j);
 }
-template <typename S, typename K, int N>
+template <bool SAVES, typename S, typename K, int N>
 __device__ void tsit5_serve_tick(const Params& P0) {
     const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= P0.N) return;
@@ -2220,7 +2238,7 @@ __device__ void tsit5_serve_tick(const Params& P0) {
 #pragma unroll
     for (int k = 0; k < ODEMI_NP; k++) L.p[k] = (K)((k < P.np) ? P.p[(i64)k * P.N + j] : 0.0);
     for (int a = 0; a < P.tick_attempts; a++) {
-        int rc = tsit5_attempt<false>(L, P, j);
+        int rc = tsit5_attempt<SAVES>(L, P, j);
         if (rc) {
             finish_lane<N>(L.c, P, j, L.u, rc);
             plain_store_ctl<N>(L.c, P, j, 0);

@@ -80,10 +80,17 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t slot0, nslots;
     const int* slot_req;
     unsigned long long* req_done;
+    const double* req_saveat;
+    const double* req_tstops;
+    const int* req_opts;
+    int sa_cap, ts_cap;
     float* wg_jbuf;
     double* pre_dt;
     double* pre_k1;
 };
+// ints per serving request-table entry (kReqOpts): nsaveat, ntstops,
+// save_start, save_end, save_end_user
+constexpr int kReqOpts = 8;
 
 // Longest-predicted-cost-first work order, key_j = (tf_j - t0_j) * n_j^3, by a
 // counting sort over the top 11 bits of the FP32 key (exponent and three
@@ -254,6 +261,7 @@ struct odemi_rhs {
     hipFunction_t rodas_wg_f64 = nullptr, rodas_wg_mixed = nullptr;
     hipFunction_t rodas_wg_f64_sv = nullptr, rodas_wg_mixed_sv = nullptr;
     hipFunction_t serve_init[2][2] = {}, serve_tick[2][2] = {};  // [alg][fp64, mixed]
+    hipFunction_t serve_tick_sv[2][2] = {};                      // the same, requests with save times
     hipFunction_t tsit5_prep[2] = {}, rodas_prep = nullptr;      // start-up kernels [fp64, mixed]
     hipFunction_t wg_serve_init[2] = {}, wg_serve_tick[2] = {};  // workgroup serving [fp64, mixed]
 };
@@ -267,7 +275,7 @@ struct odemi_rhs {
         }                                                                                        \
     } while (0)
 
-extern "C" int odemi_abi_version(void) { return 3; }
+extern "C" int odemi_abi_version(void) { return 4; }
 
 extern "C" odemi_ctx* odemi_create(int device) {
     odemi_ctx* c = new odemi_ctx();
@@ -384,9 +392,14 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_init_mixed(odemi::Params P)"
              " { odemi::tsit5_serve_init<double, float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_tick_f64(odemi::Params P)"
-             " { odemi::tsit5_serve_tick<double, double, ODEMI_N>(P); }\n";
+             " { odemi::tsit5_serve_tick<false, double, double, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_tick_mixed(odemi::Params P)"
-             " { odemi::tsit5_serve_tick<double, float, ODEMI_N>(P); }\n";
+             " { odemi::tsit5_serve_tick<false, double, float, ODEMI_N>(P); }\n";
+        // servers whose requests may carry save times
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_tick_f64_sv(odemi::Params P)"
+             " { odemi::tsit5_serve_tick<true, double, double, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_tick_mixed_sv(odemi::Params P)"
+             " { odemi::tsit5_serve_tick<true, double, float, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_plain_step(odemi::Params P)"
              " { odemi::tsit5_plain_step<ODEMI_N>(P); }\n";
     }
@@ -588,6 +601,8 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         get("odemi_tsit5_serve_init_mixed", &r->serve_init[ODEMI_TSIT5][1]);
         get("odemi_tsit5_serve_tick_f64", &r->serve_tick[ODEMI_TSIT5][0]);
         get("odemi_tsit5_serve_tick_mixed", &r->serve_tick[ODEMI_TSIT5][1]);
+        get("odemi_tsit5_serve_tick_f64_sv", &r->serve_tick_sv[ODEMI_TSIT5][0]);
+        get("odemi_tsit5_serve_tick_mixed_sv", &r->serve_tick_sv[ODEMI_TSIT5][1]);
         get("odemi_tsit5_plain_init", &r->tsit5_plain_init);
         get("odemi_tsit5_prep_f64", &r->tsit5_prep[0]);
         get("odemi_tsit5_prep_mixed", &r->tsit5_prep[1]);
@@ -603,6 +618,9 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         get("odemi_rodas_serve_init_mixed", &r->serve_init[ODEMI_RODAS5P][1]);
         get("odemi_rodas_serve_tick_f64", &r->serve_tick[ODEMI_RODAS5P][0]);
         get("odemi_rodas_serve_tick_mixed", &r->serve_tick[ODEMI_RODAS5P][1]);
+        // the Rosenbrock lane step saves on a runtime test: one kernel for both
+        r->serve_tick_sv[ODEMI_RODAS5P][0] = r->serve_tick[ODEMI_RODAS5P][0];
+        r->serve_tick_sv[ODEMI_RODAS5P][1] = r->serve_tick[ODEMI_RODAS5P][1];
     }
     if (flags & ODEMI_COMPILE_WG) {
         get("odemi_rodas_wg_f64", &r->rodas_wg_f64);
@@ -1067,6 +1085,11 @@ struct Request {
     double* u_final = nullptr;           // caller's output buffers
     int32_t* retcode = nullptr;
     int32_t* stats = nullptr;
+    std::vector<double> sa, ts;          // the request's save times and stopping times (sorted)
+    int ropt[kReqOpts] = {};      // its request-table entry: nsaveat, ntstops, save flags
+    double* t_out = nullptr;             // caller's buffers for the saved points (or null)
+    double* u_out = nullptr;
+    int32_t* nsaved = nullptr;
     int64_t slot0 = -1;
     int rt = -1;                         // request-table index on the device
     double t_submit = 0, t_admit = 0, t_done = 0;
@@ -1159,6 +1182,15 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
         ctx->err = "workgroup servers run problems without isoutofdomain or callbacks";
         return nullptr;
     }
+    if (opts->max_saves < 0 || opts->max_tstops < 0) { ctx->err = "max_saves and max_tstops must be >= 0"; return nullptr; }
+    if (wg && (opts->max_saves > 0 || opts->max_tstops > 0)) {
+        ctx->err = "save times and stopping times need a lane (fixed-n) server";
+        return nullptr;
+    }
+    if (opts->max_saves > 0 && opts->alg == ODEMI_TSIT5 && !rhs->serve_tick_sv[ODEMI_TSIT5][pi]) {
+        ctx->err = "the compiled problem has no saving serve kernels";
+        return nullptr;
+    }
     (void)hipSetDevice(ctx->device);
     odemi_server* S = new odemi_server();
     S->ctx = ctx;
@@ -1179,8 +1211,10 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
         S->wg_grid = std::max(nb, 1) * std::max(ctx->cu_count, 1);
     }
     // bytes per slot: inputs, state, outputs
+    const int msv = opts->max_saves, mts = opts->max_tstops;
     const size_t per_slot = (size_t)(n + np + 2) * 8 + 16 * 8 + (size_t)S->rows * n * 8 + kIntRows * 4 +
-                            (size_t)n * 8 + 8 + 4 + 4 + kStatRows * 4 + 4 + (wg ? 4 : 0);
+                            (size_t)n * 8 + 8 + 4 + 4 + kStatRows * 4 + 4 + (wg ? 4 : 0) +
+                            (size_t)msv * (n + 1) * 8;   // saved points
     const size_t fixed = S->gj ? (size_t)S->wg_grid * gj_ld(n) * gj_ld(n) * 4 : 0;
     int64_t slots = opts->max_slots > 0 ? opts->max_slots : (1 << 20);
     if (opts->hbm_budget_bytes > 0)
@@ -1196,6 +1230,12 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     size_t o_uf = al(Sz * n * 8), o_tfin = al(Sz * 8), o_rc = al(Sz * 4), o_st = al(kStatRows * Sz * 4), o_ns = al(Sz * 4);
     size_t o_sreq = al(Sz * 4), o_done = al((size_t)S->R * 8), o_cnt = al(64);
     size_t o_nvec = wg ? al(Sz * 4) : 0, o_jb = S->gj ? al((size_t)S->wg_grid * gj_ld(n) * gj_ld(n) * 4) : 0;
+    // saved points of every slot, and the request table's save / stopping times and flags
+    const bool reqtab = msv > 0 || mts > 0;
+    size_t o_tout = msv > 0 ? al((size_t)msv * Sz * 8) : 0, o_uout = msv > 0 ? al((size_t)msv * n * Sz * 8) : 0;
+    size_t o_rsa = reqtab ? al((size_t)S->R * std::max(msv, 1) * 8) : 0;
+    size_t o_rts = reqtab ? al((size_t)S->R * std::max(mts, 1) * 8) : 0;
+    size_t o_ropt = reqtab ? al((size_t)S->R * kReqOpts * 4) : 0;
     if (hipMalloc(&S->mem, bytes) != hipSuccess) { ctx->err = "serving pool allocation failed"; delete S; return nullptr; }
     char* b = (char*)S->mem;
     (void)hipMemset(b + o_int, 0, kIntRows * Sz * 4);
@@ -1215,6 +1255,19 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     P.counters = (unsigned long long*)(b + o_cnt);
     if (wg) P.nvec = (const int*)(b + o_nvec);
     if (S->gj) P.wg_jbuf = (float*)(b + o_jb);
+    if (msv > 0) {
+        P.max_saves = msv;
+        P.t_out = (double*)(b + o_tout);
+        P.u_out = (double*)(b + o_uout);
+    }
+    if (reqtab) {
+        (void)hipMemset(b + o_ropt, 0, (size_t)S->R * kReqOpts * 4);
+        P.req_saveat = (const double*)(b + o_rsa);
+        P.req_tstops = (const double*)(b + o_rts);
+        P.req_opts = (const int*)(b + o_ropt);
+        P.sa_cap = std::max(msv, 1);
+        P.ts_cap = std::max(mts, 1);
+    }
     P.refine_sweeps = opts->refine_sweeps;
     P.tick_attempts = opts->tick_attempts > 0 ? opts->tick_attempts : 64;
     S->done_h.assign(S->R, 0);
@@ -1299,6 +1352,45 @@ extern "C" int64_t odemi_submit_opts(odemi_server* S, int64_t N, const int32_t* 
             if (nvec[k] != n) { S->err = "uneven nvec needs a workgroup (Rodas5P) server"; delete r; return -3; }
     }
     r->u_final = u_final; r->retcode = retcode; r->stats = stats;
+    if (ro) {
+        const bool saves = ro->t_out != nullptr;
+        if (ro->nsaveat < 0 || ro->ntstops < 0 || (ro->nsaveat > 0 && !ro->saveat) || (ro->ntstops > 0 && !ro->tstops)) {
+            S->err = "saveat / tstops: negative count or NULL array"; delete r; return -3;
+        }
+        if ((ro->nsaveat > 0 || ro->ntstops > 0 || saves) && S->wg) {
+            S->err = "save times and stopping times need a lane (fixed-n) server"; delete r; return -3;
+        }
+        if (saves) {
+            if (!ro->u_out) { S->err = "t_out without u_out"; delete r; return -3; }
+            const int64_t need = ro->nsaveat + (ro->save_start ? 1 : 0) + (ro->save_end ? 1 : 0);
+            if (need > S->o.max_saves) {
+                S->err = "the request asks for " + std::to_string(need) + " saved points; the server keeps " +
+                         std::to_string(S->o.max_saves);
+                delete r; return -3;
+            }
+            r->sa.assign(ro->saveat, ro->saveat + ro->nsaveat);
+            std::sort(r->sa.begin(), r->sa.end());
+            r->ropt[0] = (int)ro->nsaveat;
+            r->ropt[2] = ro->save_start ? 1 : 0;
+            r->ropt[3] = ro->save_end ? 1 : 0;
+            r->t_out = ro->t_out; r->u_out = ro->u_out; r->nsaved = ro->nsaved;
+        } else if (ro->nsaveat > 0) {
+            S->err = "saveat without t_out / u_out"; delete r; return -3;
+        }
+        r->ropt[4] = ro->save_end_user;
+        if (ro->ntstops > 0) {
+            if (ro->ntstops > S->o.max_tstops) {
+                S->err = "the request has " + std::to_string(ro->ntstops) + " stopping times; the server keeps " +
+                         std::to_string(S->o.max_tstops);
+                delete r; return -3;
+            }
+            r->ts.assign(ro->tstops, ro->tstops + ro->ntstops);
+            std::sort(r->ts.begin(), r->ts.end());
+            r->ropt[1] = (int)ro->ntstops;
+        }
+    } else {
+        r->ropt[4] = -1;
+    }
     r->t_submit = now_s();
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -1382,7 +1474,9 @@ static void server_loop(odemi_server* S) {
     const int n = S->rhs->n, np = std::max(S->rhs->np, 1);
     const int pi = S->o.precision == ODEMI_MIXED ? 1 : 0;
     hipFunction_t f_init = S->wg ? S->rhs->wg_serve_init[pi] : S->rhs->serve_init[S->o.alg][pi];
-    hipFunction_t f_tick = S->wg ? S->rhs->wg_serve_tick[pi] : S->rhs->serve_tick[S->o.alg][pi];
+    hipFunction_t f_tick = S->wg ? S->rhs->wg_serve_tick[pi]
+                                 : (S->o.max_saves > 0 ? S->rhs->serve_tick_sv[S->o.alg][pi]
+                                                       : S->rhs->serve_tick[S->o.alg][pi]);
     const size_t Sz = (size_t)S->S;
     for (;;) {
         std::vector<Request*> admit;
@@ -1421,6 +1515,16 @@ static void server_loop(odemi_server* S) {
                                 S->stream));
             r->sreq.assign(N, r->rt);
             SCHK(hipMemcpyAsync((int*)S->P.slot_req + s0, r->sreq.data(), N * 4, hipMemcpyHostToDevice, S->stream));
+            if (S->P.req_opts) {   // the request's save / stopping times and flags
+                if (!r->sa.empty())
+                    SCHK(hipMemcpyAsync((double*)S->P.req_saveat + (size_t)r->rt * S->P.sa_cap, r->sa.data(),
+                                        r->sa.size() * 8, hipMemcpyHostToDevice, S->stream));
+                if (!r->ts.empty())
+                    SCHK(hipMemcpyAsync((double*)S->P.req_tstops + (size_t)r->rt * S->P.ts_cap, r->ts.data(),
+                                        r->ts.size() * 8, hipMemcpyHostToDevice, S->stream));
+                SCHK(hipMemcpyAsync((int*)S->P.req_opts + (size_t)r->rt * kReqOpts, r->ropt,
+                                    sizeof(r->ropt), hipMemcpyHostToDevice, S->stream));
+            }
             SCHK(hipMemsetAsync(S->P.req_done + r->rt, 0, 8, S->stream));
             SCHK(hipMemsetAsync((int*)S->P.nsaved + s0, 0, N * 4, S->stream));
             if (S->wg) SCHK(hipMemcpyAsync((int*)S->P.nvec + s0, r->nvec.data(), N * 4, hipMemcpyHostToDevice, S->stream));
@@ -1473,6 +1577,13 @@ static void server_loop(odemi_server* S) {
             SCHK(hipMemcpyAsync(r->retcode, (int*)S->P.retcode + s0, N * 4, hipMemcpyDeviceToHost, S->stream));
             if (r->stats)
                 SCHK(hipMemcpy2DAsync(r->stats, N * 4, (int*)S->P.stats + s0, Sz * 4, N * 4, kStatRows, hipMemcpyDeviceToHost, S->stream));
+            if (r->t_out) {
+                const size_t ms = (size_t)S->o.max_saves;
+                SCHK(hipMemcpy2DAsync(r->t_out, N * 8, (double*)S->P.t_out + s0, Sz * 8, N * 8, ms, hipMemcpyDeviceToHost, S->stream));
+                SCHK(hipMemcpy2DAsync(r->u_out, N * 8, (double*)S->P.u_out + s0, Sz * 8, N * 8, ms * n, hipMemcpyDeviceToHost, S->stream));
+                if (r->nsaved)
+                    SCHK(hipMemcpyAsync(r->nsaved, (int*)S->P.nsaved + s0, N * 4, hipMemcpyDeviceToHost, S->stream));
+            }
         }
         if (!fin.empty()) SCHK(hipStreamSynchronize(S->stream));
         if (!fin.empty()) {

@@ -19,6 +19,7 @@ COMPILE_TSIT5, COMPILE_RODAS5P, HAS_USER_TGRAD, COMPILE_WG = 1, 2, 4, 8
 
 P = ctypes.POINTER
 c_d, c_i, c_i64, c_vp = ctypes.c_double, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+ABI_VERSION = 4   # odemi_abi_version() of the library these structures mirror
 
 
 class SolveArgs(ctypes.Structure):
@@ -48,13 +49,15 @@ class ServerOpts(ctypes.Structure):
         ("alg", c_i), ("precision", c_i),
         ("reltol", c_d), ("abstol", c_d), ("dtmax", c_d), ("dtmin", c_d),
         ("maxiters", c_i64), ("max_slots", c_i64), ("hbm_budget_bytes", ctypes.c_size_t),
-        ("tick_attempts", c_i), ("refine_sweeps", c_i),
+        ("tick_attempts", c_i), ("refine_sweeps", c_i), ("max_saves", c_i), ("max_tstops", c_i),
     ]
 
 
 class RequestOpts(ctypes.Structure):
     """Mirror of odemi_request_opts (include/odemi.h)."""
-    _fields_ = [("reltol", c_d), ("abstol", c_d)]
+    _fields_ = [("reltol", c_d), ("abstol", c_d), ("saveat", c_vp), ("nsaveat", c_i64),
+                ("save_start", c_i), ("save_end", c_i), ("save_end_user", c_i),
+                ("t_out", c_vp), ("u_out", c_vp), ("nsaved", c_vp), ("tstops", c_vp), ("ntstops", c_i64)]
 
 
 EXPORTS = ["odemi_create", "odemi_destroy", "odemi_last_error", "odemi_device_info", "odemi_abi_version",
@@ -91,6 +94,9 @@ def load():
     L.odemi_last_error.restype = ctypes.c_char_p
     L.odemi_device_info.argtypes = [c_vp, P(c_i), P(ctypes.c_size_t), P(ctypes.c_size_t)]
     L.odemi_abi_version.restype = c_i
+    if L.odemi_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI {L.odemi_abi_version()}, this binding expects {ABI_VERSION} "
+                           "(rebuild with __graft_entry__.build())")
     L.odemi_compile_rhs.argtypes = [c_vp, ctypes.c_char_p, c_i, c_i, c_i, P(c_d)]
     L.odemi_compile_rhs.restype = c_vp
     L.odemi_rhs_destroy.argtypes = [c_vp]

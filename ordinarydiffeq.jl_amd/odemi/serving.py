@@ -8,8 +8,9 @@ states as soon as its last trajectory finishes.  The pool size is the
 admission bound: it is set from an HBM budget (bytes) or a slot count.
 
 The reference has no serving layer; the per-trajectory semantics are the
-reference's `solve` with save_everystep=false, save_start=false (final state
-only), identical to odemi.solve_arrays on the same inputs.
+reference's `solve` with save_everystep=false, identical to odemi.solve_arrays
+on the same inputs: the final state, and (lane servers created with max_saves
+/ max_tstops) the request's own save times and stopping times.
 """
 import ctypes
 
@@ -22,9 +23,10 @@ from .api import ODEFunction, compile_problem, get_context
 class Request:
     """Handle of one submitted ensemble; outputs are filled at completion."""
 
-    def __init__(self, server, rid, N, u_final, retcode, stats, keep):
+    def __init__(self, server, rid, N, u_final, retcode, stats, keep, t_out=None, u_out=None, nsaved=None):
         self.server, self.id, self.N = server, rid, N
         self.u_final, self.retcode, self.stats = u_final, retcode, stats
+        self.t_out, self.u_out, self.nsaved = t_out, u_out, nsaved   # saved points (requests with saves)
         self._keep = keep
 
     def done(self):
@@ -65,11 +67,14 @@ class EnsembleServer:
     """Serve one problem type (ODEFunction + state size + algorithm).
 
     `ns` (Rodas5P): the state sizes requests may use (uneven n, workgroup
-    kernels, n = max(ns)); each request then passes its per-trajectory `nvec`."""
+    kernels, n = max(ns)); each request then passes its per-trajectory `nvec`.
+    `max_saves` / `max_tstops` (lane servers): the saved points per trajectory
+    and the stopping times per request a request may ask for (the pool keeps
+    max_saves points per slot, counted in the HBM budget)."""
 
     def __init__(self, fun, n, np_, alg, *, precision="fp64", reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0,
                  maxiters=1_000_000, max_slots=1 << 20, hbm_budget_bytes=0, tick_attempts=64, refine_sweeps=1,
-                 ns=None, ctx=None):
+                 ns=None, ctx=None, max_saves=0, max_tstops=0):
         fun = fun if isinstance(fun, ODEFunction) else ODEFunction(fun)
         self.ctx = ctx or get_context()
         algname = "tsit5" if alg.code == _lib.TSIT5 else "rodas5p"
@@ -90,6 +95,9 @@ class EnsembleServer:
         o.hbm_budget_bytes = int(hbm_budget_bytes)
         o.tick_attempts = int(tick_attempts)
         o.refine_sweeps = int(refine_sweeps)
+        o.max_saves = int(max_saves)
+        o.max_tstops = int(max_tstops)
+        self.max_saves = int(max_saves)
         self._h = self._L.odemi_server_create(self.ctx.h, self._rhs, ctypes.byref(o))
         if not self._h:
             raise RuntimeError("odemi_server_create failed: " + self.ctx.error())
@@ -103,10 +111,15 @@ class EnsembleServer:
     def error(self):
         return self._L.odemi_server_last_error(self._handle()).decode(errors="replace")
 
-    def submit(self, u0, p, t0, tf, stats=True, nvec=None, reltol=None, abstol=None):
+    def submit(self, u0, p, t0, tf, stats=True, nvec=None, reltol=None, abstol=None, saveat=None, tstops=None,
+               save_start=False, save_end=True, save_end_user=-1):
         """u0 (n, N), p (np, N) or None, t0/tf (N,) or scalars, nvec (N,) per-
         trajectory sizes (servers created with `ns`); reltol / abstol: this
-        request's tolerances (None: the server's).  Inputs are copied before
+        request's tolerances (None: the server's); saveat / tstops: its save
+        times and stopping times (lane servers with max_saves / max_tstops),
+        save_start / save_end / save_end_user as in solve_arrays: with saveat
+        or save_start the request's Request carries t_out (max_saves, N),
+        u_out (max_saves, n, N) and nsaved (N,).  Inputs are copied before
         return; returns a Request."""
         u0 = np.ascontiguousarray(u0, dtype=np.float64)
         if u0.ndim == 1:
@@ -135,12 +148,31 @@ class EnsembleServer:
         ro = _lib.RequestOpts(0.0 if reltol is None else float(reltol), 0.0 if abstol is None else float(abstol))
         if (reltol is not None and not ro.reltol > 0) or (abstol is not None and not ro.abstol > 0):
             raise ValueError("request tolerances must be positive")
+        sa = np.ascontiguousarray(np.sort(np.atleast_1d(np.asarray(saveat, dtype=np.float64)))) \
+            if saveat is not None else np.zeros(0)
+        ts = np.ascontiguousarray(np.sort(np.atleast_1d(np.asarray(tstops, dtype=np.float64)))) \
+            if tstops is not None else np.zeros(0)
+        tout = uout = nsv = None
+        if len(sa) or save_start:
+            if self.max_saves <= 0:
+                raise ValueError("this server keeps no saved points (create it with max_saves)")
+            tout = np.full((self.max_saves, N), np.nan)
+            uout = np.full((self.max_saves, n, N), np.nan)
+            nsv = np.zeros(N, dtype=np.int32)
+            ro.saveat, ro.nsaveat = ptr(sa) if len(sa) else None, len(sa)
+            ro.save_start, ro.save_end, ro.save_end_user = int(bool(save_start)), int(bool(save_end)), int(save_end_user)
+            ro.t_out, ro.u_out, ro.nsaved = ptr(tout), ptr(uout), ptr(nsv)
+        else:
+            ro.save_end_user = -1
+        if len(ts):
+            ro.tstops, ro.ntstops = ptr(ts), len(ts)
         rid = self._L.odemi_submit_opts(self._handle(), N, ptr(nv), ptr(u0), ptr(pp), ptr(t0), ptr(tf),
                                         ctypes.byref(ro), ptr(uf), ptr(rc), ptr(st))
         if rid <= 0:
             raise RuntimeError(f"odemi_submit failed ({rid}): {self.error()}")
-        self._live[rid] = (uf, rc, st)
-        return Request(self, rid, N, uf, rc, st, (uf, rc, st))
+        keep = (uf, rc, st, tout, uout, nsv)
+        self._live[rid] = keep
+        return Request(self, rid, N, uf, rc, st, keep, tout, uout, nsv)
 
     def info(self):
         v = [_lib.c_i64() for _ in range(5)]

@@ -271,3 +271,49 @@ def test_per_request_tolerances_equal_batch_engine(kind):
     # the tolerances did change the work
     steps = [int(r.stats[1].sum()) for *_, r in reqs[:3]]
     assert steps[0] < steps[1] < steps[2]
+
+
+SAVE_CASES = [c for c in CASES if c[0] in ("tsit5-fp64", "tsit5-mixed", "rodas-fp64")]
+
+
+@pytest.mark.parametrize("name,alg,prec,fun,pid,dgen", SAVE_CASES, ids=[c[0] for c in SAVE_CASES])
+def test_request_saveat_and_tstops_equal_batch(name, alg, prec, fun, pid, dgen):
+    """Requests carrying their own save times, stopping times and save flags
+    (lane servers with max_saves / max_tstops), in flight together with plain
+    requests: each request's saved points, final states and stats are those of
+    the batch engine run with the same keywords (bit for bit)."""
+    d = dgen()
+    n, N = d["u0"].shape
+    print(f"[case] serving saveat/tstops {name}", flush=True)
+    span = float(np.median(d["tf"] - d["t0"]))
+    specs = [   # (trajectories, saveat, tstops, save_start, save_end)
+        (np.arange(0, 48), np.linspace(0.1, 0.9, 5) * span, None, False, True),
+        (np.arange(48, 120), np.array([0.25 * span]), np.array([0.3 * span, 0.61 * span]), True, True),
+        (np.arange(120, 130), None, np.array([0.5 * span]), False, True),
+        (np.arange(130, 200), np.linspace(0.05, 1.5, 12) * span, None, True, False),
+    ]
+    kw = dict(precision=prec, reltol=d["reltol"], abstol=d["abstol"], tick_attempts=16, max_saves=16, max_tstops=4)
+    with odemi.EnsembleServer(fun, n, d["p"].shape[0], alg, **kw) as srv:
+        reqs = []
+        for k, (idx, sa, ts, ss, se) in enumerate(specs):
+            if k == 2:   # a plain request in between
+                reqs.append(srv.submit(*_cols(d, np.arange(200, 260))))
+            reqs.append(srv.submit(*_cols(d, idx), saveat=sa, tstops=ts, save_start=ss, save_end=se))
+        for r in reqs:
+            assert r.wait(timeout=120)
+    reqs.pop(2)
+    ea = odemi.EnsembleMI355X(precision=prec)
+    for (idx, sa, ts, ss, se), r in zip(specs, reqs):
+        b = odemi.solve_arrays(fun, alg, *_cols(d, idx), ea, reltol=d["reltol"], abstol=d["abstol"], saveat=sa,
+                               tstops=ts, save_start=ss, save_end=se, max_saves=16)
+        assert np.array_equal(r.u_final, b["u_final"]), name
+        assert np.array_equal(r.stats, b["stats"]) and np.array_equal(r.retcode, b["retcode"])
+        if sa is None and not ss:
+            assert r.t_out is None
+            continue
+        assert np.array_equal(r.nsaved, b["nsaved"])
+        assert int(r.nsaved.max()) > (1 if sa is not None else 0)
+        for j in range(len(idx)):
+            k = int(r.nsaved[j])
+            assert np.array_equal(r.t_out[:k, j], b["t_out"][:k, j])
+            assert np.array_equal(r.u_out[:k, :, j], b["u_out"][:k, :, j])
