@@ -205,9 +205,9 @@ void odemi_default_args(odemi_solve_args* args, int alg);
  * the device between ticks, advances all active trajectories together, and
  * copies a request's results to the caller's buffers when its last trajectory
  * finishes.  A trajectory's result does not depend on what else is in flight.
- * A request may carry its own tolerances, save times and stopping times
- * (odemi_submit_opts; save times and stopping times on lane servers, within
- * the server's max_saves / max_tstops); the other options are the server's.
+ * A request may carry its own tolerances, stopping times and (lane servers)
+ * save times (odemi_submit_opts, within the server's max_tstops / max_saves);
+ * the other options are the server's.
  * ------------------------------------------------------------------------- */
 typedef struct odemi_server odemi_server;
 typedef struct {
@@ -252,7 +252,8 @@ int64_t odemi_submit(odemi_server* server, int64_t N, const int32_t* nvec, const
  *     save_start + save_end must not exceed;
  *   tstops [ntstops]: stopping times, sorted ascending, at most the server's
  *     max_tstops.
- * Save times and stopping times need a lane (Tsit5 / Rodas5P, fixed n) server. */
+ * Save times need a lane (Tsit5 / Rodas5P, fixed n) server; stopping times
+ * are taken by every server. */
 typedef struct {
     double reltol, abstol;
     const double* saveat;

@@ -1183,8 +1183,8 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
         return nullptr;
     }
     if (opts->max_saves < 0 || opts->max_tstops < 0) { ctx->err = "max_saves and max_tstops must be >= 0"; return nullptr; }
-    if (wg && (opts->max_saves > 0 || opts->max_tstops > 0)) {
-        ctx->err = "save times and stopping times need a lane (fixed-n) server";
+    if (wg && opts->max_saves > 0) {
+        ctx->err = "save times need a lane (fixed-n) server";
         return nullptr;
     }
     if (opts->max_saves > 0 && opts->alg == ODEMI_TSIT5 && !rhs->serve_tick_sv[ODEMI_TSIT5][pi]) {
@@ -1357,8 +1357,8 @@ extern "C" int64_t odemi_submit_opts(odemi_server* S, int64_t N, const int32_t* 
         if (ro->nsaveat < 0 || ro->ntstops < 0 || (ro->nsaveat > 0 && !ro->saveat) || (ro->ntstops > 0 && !ro->tstops)) {
             S->err = "saveat / tstops: negative count or NULL array"; delete r; return -3;
         }
-        if ((ro->nsaveat > 0 || ro->ntstops > 0 || saves) && S->wg) {
-            S->err = "save times and stopping times need a lane (fixed-n) server"; delete r; return -3;
+        if ((ro->nsaveat > 0 || saves) && S->wg) {
+            S->err = "save times need a lane (fixed-n) server"; delete r; return -3;
         }
         if (saves) {
             if (!ro->u_out) { S->err = "t_out without u_out"; delete r; return -3; }

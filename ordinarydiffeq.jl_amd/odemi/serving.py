@@ -68,7 +68,7 @@ class EnsembleServer:
 
     `ns` (Rodas5P): the state sizes requests may use (uneven n, workgroup
     kernels, n = max(ns)); each request then passes its per-trajectory `nvec`.
-    `max_saves` / `max_tstops` (lane servers): the saved points per trajectory
+    `max_saves` (lane servers) / `max_tstops`: the saved points per trajectory
     and the stopping times per request a request may ask for (the pool keeps
     max_saves points per slot, counted in the HBM budget)."""
 

@@ -317,3 +317,32 @@ def test_request_saveat_and_tstops_equal_batch(name, alg, prec, fun, pid, dgen):
             k = int(r.nsaved[j])
             assert np.array_equal(r.t_out[:k, j], b["t_out"][:k, j])
             assert np.array_equal(r.u_out[:k, :, j], b["u_out"][:k, :, j])
+
+
+def test_chain_server_request_tstops_equal_batch():
+    """Uneven-n chain (workgroup) server: requests with their own stopping
+    times, among plain requests, give the batch workgroup engine's results with
+    the same tstops, bit for bit (FP64)."""
+    d = P.chain_ensemble(N=96)
+    d["tf"] = np.full(96, 1.0)
+    nv = np.asarray(d["nvec"])
+    n = d["u0"].shape[0]
+    ns = sorted(set(int(x) for x in nv))
+    print("[case] chain server per-request tstops", flush=True)
+    specs = [(np.arange(0, 24), np.array([0.25, 0.5])), (np.arange(24, 48), None),
+             (np.arange(48, 96), np.array([0.1, 0.333, 0.75]))]
+    with odemi.EnsembleServer(P.CHAIN, n, d["p"].shape[0], odemi.Rodas5P(), reltol=d["reltol"], abstol=d["abstol"],
+                              tick_attempts=16, ns=ns, max_tstops=4) as srv:
+        reqs = [srv.submit(*_cols(d, idx), nvec=nv[idx], tstops=ts) for idx, ts in specs]
+        for r in reqs:
+            assert r.wait(timeout=300)
+        with pytest.raises((ValueError, RuntimeError)):   # save times need a lane server
+            srv.submit(*_cols(d, np.arange(4)), nvec=nv[:4], saveat=[0.5])
+    for (idx, ts), r in zip(specs, reqs):
+        b = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), *_cols(d, idx), odemi.EnsembleMI355X(), reltol=d["reltol"],
+                               abstol=d["abstol"], nvec=nv[idx], tstops=ts)
+        for c, j in enumerate(idx):
+            assert np.array_equal(r.u_final[:nv[j], c], b["u_final"][:nv[j], c])
+        assert np.array_equal(r.stats, b["stats"]) and np.array_equal(r.retcode, b["retcode"])
+        if ts is not None:   # every interior stopping time was stepped on: more accepted steps than without
+            assert (r.stats[1] >= 1).all()
