@@ -346,3 +346,82 @@ def test_chain_server_request_tstops_equal_batch():
         assert np.array_equal(r.stats, b["stats"]) and np.array_equal(r.retcode, b["retcode"])
         if ts is not None:   # every interior stopping time was stepped on: more accepted steps than without
             assert (r.stats[1] >= 1).all()
+
+
+_DOSES = [2.0, 4.0, 6.0]
+
+
+def _pk(du, u, p, t):
+    du[0] = -p[0] * u[0]
+    du[1] = p[0] * u[0] - p[1] * u[1]
+
+
+def _pk_jac(J, u, p, t):
+    J[0, 0] = -p[0]
+    J[0, 1] = 0.0
+    J[1, 0] = p[0]
+    J[1, 1] = -p[1]
+
+
+def _dose(itg):
+    itg.u[0] = itg.u[0] + itg.p[2]
+
+
+def _stop(itg):
+    odemi.terminate(itg)
+
+
+_PK_CBS = odemi.CallbackSet(odemi.DiscreteCallback(lambda u, t, itg: odemi.sym_in(t, _DOSES), _dose),
+                            odemi.DiscreteCallback(lambda u, t, itg: u[1] > itg.p[3], _stop,
+                                                   save_positions=(False, True)))
+
+
+def test_lane_servers_with_callbacks_equal_batch():
+    """Servers of problems with callbacks (traced into the served problem):
+    bouncing balls with a ContinuousCallback floor (Tsit5), and a dosing model
+    whose DiscreteCallbacks dose at stopping times and terminate! (Rodas5P,
+    the doses passed as each request's tstops, forced saves kept with saveat)
+    give the batch engine's results bit for bit, several requests in flight."""
+    print("[case] serving with continuous callbacks (balls, tsit5)", flush=True)
+    d = P.ball_ensemble(N=512)
+    with odemi.EnsembleServer(P.BALL, 2, 1, odemi.Tsit5(), reltol=d["reltol"], abstol=d["abstol"], tick_attempts=16,
+                              callback=P.BALL_BOUNCE) as srv:
+        chunks = np.array_split(np.arange(512), 5)
+        reqs = [srv.submit(*_cols(d, idx)) for idx in chunks]
+        for r in reqs:
+            assert r.wait(timeout=120)
+    for idx, r in zip(chunks, reqs):
+        b = odemi.solve_arrays(P.BALL, odemi.Tsit5(), *_cols(d, idx), reltol=d["reltol"], abstol=d["abstol"],
+                               callback=P.BALL_BOUNCE)
+        assert np.array_equal(r.u_final, b["u_final"]) and np.array_equal(r.stats, b["stats"])
+        assert np.array_equal(r.retcode, b["retcode"])
+        assert (r.stats[6] > 0).all()   # conditions were evaluated
+    print("[case] serving with discrete callbacks (dosing, rodas5p)", flush=True)
+    N = 96
+    rng = np.random.default_rng(11)
+    p = np.stack([rng.uniform(0.5, 2.0, N), rng.uniform(0.1, 0.5, N), rng.uniform(1.0, 3.0, N),
+                  rng.uniform(1.5, 4.0, N)])
+    u0 = np.stack([p[2].copy(), np.zeros(N)])
+    fun = odemi.ODEFunction(_pk, jac=_pk_jac)
+    t0, tf = np.zeros(N), np.full(N, 10.0)
+    sa = [1.0, 3.0, 5.0, 7.5]
+    with odemi.EnsembleServer(fun, 2, 4, odemi.Rodas5P(), reltol=1e-6, abstol=1e-8, tick_attempts=16,
+                              callback=_PK_CBS, max_saves=16, max_tstops=4) as srv:
+        halves = (np.arange(0, 48), np.arange(48, 96))
+        reqs = [srv.submit(u0[:, h], p[:, h], t0[h], tf[h], tstops=_DOSES, saveat=sa) for h in halves]
+        plain = srv.submit(u0, p, t0, tf, tstops=_DOSES)
+        for r in reqs + [plain]:
+            assert r.wait(timeout=120)
+    for h, r in zip(halves, reqs):
+        b = odemi.solve_arrays(fun, odemi.Rodas5P(), u0[:, h], p[:, h], t0[h], tf[h], reltol=1e-6, abstol=1e-8,
+                               tstops=_DOSES, callback=_PK_CBS, saveat=sa, max_saves=16)
+        assert np.array_equal(r.retcode, b["retcode"]) and np.array_equal(r.stats, b["stats"])
+        assert np.array_equal(r.nsaved, b["nsaved"])
+        for j in range(len(h)):
+            k = min(int(r.nsaved[j]), 16)
+            assert np.array_equal(r.t_out[:k, j], b["t_out"][:k, j])
+            assert np.array_equal(r.u_out[:k, :, j], b["u_out"][:k, :, j])
+    b = odemi.solve_arrays(fun, odemi.Rodas5P(), u0, p, t0, tf, reltol=1e-6, abstol=1e-8, tstops=_DOSES,
+                           callback=_PK_CBS)
+    assert np.array_equal(plain.u_final, b["u_final"]) and np.array_equal(plain.retcode, b["retcode"])
+    assert set(int(x) for x in plain.retcode) == {1, 2}   # some trajectories terminate
