@@ -46,9 +46,9 @@ constexpr size_t kIntRows = 16;
 // DEStats rows written per trajectory: nf naccept nreject njacs nw nsolve ncondition
 constexpr size_t kStatRows = 7;
 // odemi_features bits: the lane kernels' per-step hooks (isoutofdomain,
-// discrete, continuous callbacks) and the workgroup kernels' callbacks
+// discrete, continuous callbacks); the workgroup kernels' callbacks are bits
+// 8 (discrete) and 16 (continuous, whose event search keeps dense rows in LDS)
 constexpr int kLaneHooks = 1 | 2 | 4;
-constexpr int kWgCallbacks = 8 | 16;
 
 struct Params {  // must match odemi::Params in kernels/odemi_device.h
     int64_t N;
@@ -776,10 +776,6 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
         ctx->err = "the lane kernels' isoutofdomain and callback forms need a lane problem (compile without ODEMI_COMPILE_WG)";
         return -3;
     }
-    if (plain_wg && (rhs->features & kWgCallbacks)) {
-        ctx->err = "callbacks in uneven-size batches run in the persistent engine only";
-        return -3;
-    }
 
     hipStream_t s = a->stream ? (hipStream_t)a->stream : ctx->stream;
     const bool dev = a->device_ptrs != 0;
@@ -823,7 +819,8 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
         const int64_t per_block = use_wg ? 1 : threads;
         grid = std::min<int64_t>(grid, (N + per_block - 1) / per_block);
     } else if (plain_wg) {
-        shmem = wg_shmem_bytes(nmax, false, false, false);
+        // the tick kernels' layout keeps the dense rows when continuous callbacks search events
+        shmem = wg_shmem_bytes(nmax, false, (rhs->features & 16) != 0, false);
         if (shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; return -3; }
         int nb = 0;
         HIPCHK(ctx, hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn_main, 256, shmem));
@@ -1178,8 +1175,8 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
         ctx->err = "the compiled problem has no serving kernels for this algorithm";
         return nullptr;
     }
-    if (wg && rhs->features) {
-        ctx->err = "workgroup servers run problems without isoutofdomain or callbacks";
+    if (wg && (rhs->features & kLaneHooks)) {
+        ctx->err = "the lane kernels' isoutofdomain and callback forms need a lane problem";
         return nullptr;
     }
     if (opts->max_saves < 0 || opts->max_tstops < 0) { ctx->err = "max_saves and max_tstops must be >= 0"; return nullptr; }
@@ -1203,7 +1200,7 @@ extern "C" odemi_server* odemi_server_create(odemi_ctx* ctx, odemi_rhs* rhs, con
     if (wg) {
         // resident workgroups of the tick kernel (the Gauss-Jordan variant needs a
         // Jacobian block per workgroup)
-        S->shmem = wg_shmem_bytes(n, pi == 1, false, S->gj);
+        S->shmem = wg_shmem_bytes(n, pi == 1, (rhs->features & 16) != 0, S->gj);   // dense rows for events
         if (S->shmem > 160 * 1024) { ctx->err = "state dimension too large for the workgroup kernel's LDS"; delete S; return nullptr; }
         int nb = 0;
         if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, rhs->wg_serve_tick[pi], 256, S->shmem) != hipSuccess)

@@ -71,8 +71,8 @@ class EnsembleServer:
     `max_saves` (lane servers) / `max_tstops`: the saved points per trajectory
     and the stopping times per request a request may ask for (the pool keeps
     max_saves points per slot, counted in the HBM budget).
-    `callback` / `isoutofdomain` (lane servers): traced into the served
-    problem, as for solve_arrays; every request of the server runs them."""
+    `callback` / `isoutofdomain`: traced into the served problem, as for
+    solve_arrays; every request of the server runs them."""
 
     def __init__(self, fun, n, np_, alg, *, precision="fp64", reltol=1e-3, abstol=1e-6, dtmax=None, dtmin=0.0,
                  maxiters=1_000_000, max_slots=1 << 20, hbm_budget_bytes=0, tick_attempts=64, refine_sweeps=1,
@@ -83,8 +83,6 @@ class EnsembleServer:
         self.ns = sorted(set(int(x) for x in ns)) if ns is not None else None
         if self.ns is not None and (self.ns[0] < 1 or self.ns[-1] > n):
             raise ValueError("ns entries must lie in [1, n]")
-        if self.ns is not None and (callback is not None or isoutofdomain is not None):
-            raise ValueError("workgroup (uneven-n) servers run problems without callbacks or isoutofdomain")
         self._rhs, _, _ = compile_problem(fun, n, np_, algs=(algname,), ctx=self.ctx, ns=self.ns,
                                           callback=callback, isoutofdomain=isoutofdomain)
         self.n, self.np = n, np_

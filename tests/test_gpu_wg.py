@@ -372,3 +372,39 @@ def test_wg_callbacks_terminate_and_mixed():
         assert abs(omx["t_final"][j] - r["t_final"]) <= 1e-6, (j, omx["t_final"][j], r["t_final"])
         codes.add(r["retcode"])
     assert 2 in codes
+
+
+def test_wg_callbacks_plain_engine_and_server_equal_persistent():
+    """The workgroup callbacks (ContinuousCallback wall + DiscreteCallback kicks
+    at tstops) through the plain FP64 engine (one launch per step attempt, the
+    event state in HBM between launches) and through an uneven-n server (the
+    kicks' tstops passed per request, several requests in flight): final
+    states, retcodes and all seven stats counters equal the persistent
+    engine's, bit for bit."""
+    d = _chain_events(40, 33)
+    nv = np.asarray(d["nvec"])
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], nvec=nv, tstops=[0.25, 0.5], callback=_WG_CBS,
+              save_start=False, save_end=True)
+    print("[case] chain wg callbacks persistent vs plain", flush=True)
+    ref = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], odemi.EnsembleMI355X(),
+                             max_saves=512, **kw)
+    pl = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                            odemi.EnsembleMI355X(engine="plain"), max_saves=512, **kw)
+    for j in range(len(nv)):
+        assert np.array_equal(pl["u_final"][:nv[j], j], ref["u_final"][:nv[j], j]), j
+    assert np.array_equal(pl["stats"], ref["stats"]) and np.array_equal(pl["retcode"], ref["retcode"])
+    assert (ref["stats"][6] > 0).all()
+    print("[case] chain wg callbacks server", flush=True)
+    n = d["u0"].shape[0]
+    ns = sorted(set(int(x) for x in nv))
+    parts = (np.arange(0, 13), np.arange(13, 40))
+    with odemi.EnsembleServer(P.CHAIN, n, d["p"].shape[0], odemi.Rodas5P(), reltol=d["reltol"], abstol=d["abstol"],
+                              tick_attempts=8, ns=ns, max_tstops=2, callback=_WG_CBS) as srv:
+        reqs = [srv.submit(d["u0"][:, idx], d["p"][:, idx], d["t0"][idx], d["tf"][idx], nvec=nv[idx],
+                           tstops=[0.25, 0.5]) for idx in parts]
+        for r in reqs:
+            assert r.wait(timeout=300)
+    for idx, r in zip(parts, reqs):
+        for c, j in enumerate(idx):
+            assert np.array_equal(r.u_final[:nv[j], c], ref["u_final"][:nv[j], j]), j
+        assert np.array_equal(r.stats, ref["stats"][:, idx]) and np.array_equal(r.retcode, ref["retcode"][idx])
