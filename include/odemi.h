@@ -152,10 +152,12 @@ int odemi_abi_version(void);
  *     holding save0, save1, term and
  *     template<typename T> static __device__ bool cond(const T* u, const T* p, T t, int n);
  *     template<typename T> static __device__ void affect(T* u, const T* p, T t, int n);
- *   #define ODEMI_NCCB_WG k with OdemiCcbWg<K> holding OdemiCcb's scalar constants, a
- *     T cond(u, p, t, n), affect and affect_neg (u, p, t, n)
- *     <- the same DiscreteCallbacks and (scalar) ContinuousCallbacks, one case per
- *        state size n of an uneven batch, run by one thread on the state in LDS
+ *   #define ODEMI_NCCB_WG k with OdemiCcbWg<K> holding OdemiCcb's constants, a
+ *     T cond(u, p, t, n), affect and affect_neg (u, p, t, n); with vlen > 0
+ *     (ODEMI_VCB_MAXLEN_WG the largest) instead vcond(T* out, u, p, t, n) and
+ *     bool vaffect(u, p, t, const double* ev, n) returning terminate!
+ *     <- the same DiscreteCallbacks, ContinuousCallbacks and VectorContinuousCallbacks,
+ *        one case per state size n of an uneven batch, run by one thread on the state in LDS
  * n/np are the (maximum) state and parameter counts.  Code objects are cached
  * on disk by source hash ($ODEMI_CACHE_DIR).  compile_seconds may be NULL. */
 odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src, int n, int np, int flags,

@@ -556,6 +556,123 @@ __device__ bool wg_ccb_find_time(const WGPtrs& S, Ctl& c, int n, int nmax, doubl
     residual = wg_ccb_condition<CB>(S, c, n, nmax, cb_t);
     return true;
 }
+#ifndef ODEMI_VCB_MAXLEN_WG
+#define ODEMI_VCB_MAXLEN_WG 1
+#endif
+// get_condition for a VectorContinuousCallback: its len values at abst, by every
+// thread from the state in LDS (uniform values); one condition call each time.
+template <int CB>
+__device__ void wg_vcb_condition(const WGPtrs& S, Ctl& c, int n, int nmax, double abst, double* out) {
+    c.ncond++;
+    const double* x = S.u;
+    if (abst != c.t) {
+        if (abst == c.tprev) {
+            x = S.uprev;
+        } else {
+            wg_interp(S, n, nmax, (abst - c.tprev) / c.dt, S.du);
+            x = S.du;
+        }
+    }
+    OdemiCcbWg<CB>::template vcond<double>(out, x, S.pd, abst, n);
+    __syncthreads();   // S.du is rewritten by the next interpolation
+}
+// findall_events! (callbacks.jl:499-521): event i when the bottom sign is
+// nonzero and bottom sign * value <= 0
+template <int V>
+__device__ __forceinline__ unsigned wg_vcb_events(const double* bs, const double* val) {
+    unsigned m = 0u;
+#pragma unroll
+    for (int i = 0; i < V; i++)
+        if (bs[i] != 0 && bs[i] * val[i] <= 0) m |= 1u << i;
+    return m;
+}
+// find_callback_time for a VectorContinuousCallback (callbacks.jl:239-362), as
+// the lane engines do it: the components that fired last time re-read at the
+// nudged tprev (nudge_tprev on the value closest to zero among them), the
+// interp_points checks unless every component has an event at t, each event
+// component rooted to adjacent floats, the earliest root winning and the
+// components rooted at that same time simultaneous (direction -sign(bottom)).
+template <int CB>
+__device__ bool wg_vcb_find_time(const WGPtrs& S, Ctl& c, int n, int nmax, double& cb_t, double* ev, double& residual) {
+    typedef OdemiCcbWg<CB> C;
+    constexpr int V = C::vlen;
+    double bc[V], bs[V], top[V], tmp[V];
+    const double bottom_t = c.tprev;
+    wg_vcb_condition<CB>(S, c, n, nmax, bottom_t, bc);
+#pragma unroll
+    for (int i = 0; i < V; i++) { bs[i] = wg_sign(bc[i]); ev[i] = 0.0; }
+    double nudged_t = bottom_t;
+    const unsigned prev = (c.event_last == CB + 1) ? c.sim_mask : 0u;
+    if (c.event_last == CB + 1) {
+        double minval = 0.0, minabs = __builtin_inf();
+#pragma unroll
+        for (int i = 0; i < V; i++)
+            if (((prev >> i) & 1u) && fabs(bc[i]) < minabs) { minabs = fabs(bc[i]); minval = bc[i]; }
+        if (fabs(minval - c.last_event_error) <= C::abstol) nudged_t = c.tprev + c.dt * C::repeat_nudge;
+        wg_vcb_condition<CB>(S, c, n, nmax, nudged_t, tmp);
+#pragma unroll
+        for (int i = 0; i < V; i++)
+            if ((prev >> i) & 1u) bs[i] = wg_sign(tmp[i]);
+    }
+    double top_t = c.t;
+    wg_vcb_condition<CB>(S, c, n, nmax, top_t, top);
+    unsigned mask = wg_vcb_events<V>(bs, top);
+    constexpr unsigned all = (V >= 32) ? 0xffffffffu : ((1u << V) - 1u);
+    if (C::interp_points != 0 && mask != all) {
+        const double h = (c.t - c.tprev) / (double)(C::interp_points - 1);
+        for (int i = 2; i <= C::interp_points; i++) {
+            top_t = (i == C::interp_points) ? c.t : fma((double)(i - 1), h, c.tprev);
+            wg_vcb_condition<CB>(S, c, n, nmax, top_t, top);
+            mask = wg_vcb_events<V>(bs, top);
+            if (mask) break;
+        }
+    }
+    if (!mask) { cb_t = c.t; residual = 0.0; return false; }
+    if (C::rootfind == 2) {
+#pragma unroll
+        for (int i = 0; i < V; i++)
+            if ((mask >> i) & 1u) ev[i] = -bs[i];
+        cb_t = top_t; residual = 0.0;
+        return true;
+    }
+    // rightfloat(top_t, tdir): nextfloat forward, prevfloat backward
+    double cbt = (c.tdir > 0) ? __longlong_as_double(__double_as_longlong(top_t) + (top_t >= 0.0 ? 1 : -1))
+                              : __longlong_as_double(__double_as_longlong(top_t) + (top_t > 0.0 ? -1 : 1)), res = 0.0;
+    if (top_t == 0.0) cbt = (c.tdir > 0) ? __longlong_as_double(1) : -__longlong_as_double(1);
+#pragma unroll
+    for (int idx = 0; idx < V; idx++) {
+        if (!((mask >> idx) & 1u)) continue;
+        double cbi;
+        if (wg_sign(top[idx]) == 0.0) {
+            cbi = top_t;
+        } else {
+            double l = ((prev >> idx) & 1u) ? nudged_t : bottom_t, r = top_t;
+            for (int it = 0; it < 2200; it++) {
+                const double m = l + 0.5 * (r - l);
+                if (m == l || m == r) break;
+                wg_vcb_condition<CB>(S, c, n, nmax, m, tmp);
+                const double sm = wg_sign(tmp[idx]);
+                if (sm == 0.0) { l = m; r = m; break; }
+                if (sm == bs[idx]) l = m;
+                else r = m;
+            }
+            cbi = (C::rootfind == 1) ? r : l;
+        }
+        if (c.tdir * cbi < c.tdir * cbt) {
+#pragma unroll
+            for (int i = 0; i < V; i++) ev[i] = 0.0;
+        }
+        if (c.tdir * cbi <= c.tdir * cbt) {
+            cbt = cbi;
+            wg_vcb_condition<CB>(S, c, n, nmax, cbi, tmp);
+            res = tmp[idx];
+            ev[idx] = -bs[idx];
+        }
+    }
+    cb_t = cbt; residual = res;
+    return true;
+}
+
 // _ode_addsteps! for RosenbrockCache (stiff_addsteps.jl:140-224) after
 // change_t_via_interpolation!, in the lane engines' form: the stages of the
 // step from uprev at tprev with the new dt, the step's dT and fsalfirst
@@ -644,7 +761,7 @@ __device__ void wg_addsteps(const WGPtrs& S, const Params& P, const Ctl& c, int 
 // affect! on an upcrossing / affect_neg! on a downcrossing.  Returns terminate!.
 template <int CB, typename F, bool GJ, int R, typename GT, typename SaveFn>
 __device__ bool wg_ccb_apply(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c, double& tlast, double cb_t,
-                             double bs, GT& G, float* Jg, SaveFn& savevalues, bool& saved_in_cb) {
+                             double bs, const double* ev, GT& G, float* Jg, SaveFn& savevalues, bool& saved_in_cb) {
     typedef OdemiCcbWg<CB> C;
     const int tid = threadIdx.x, nmax = P.nmax;
     c.dtpropose = c.tdir * jl_max(nextfloat_pos(P.dtmin), c.tdir * c.dt);
@@ -662,7 +779,16 @@ __device__ bool wg_ccb_apply(const WGPtrs& S, const Params& P, i64 j, int n, Ctl
     if (C::save0 && !savedexactly) wg_save_point(c, tlast, P, j, c.t, S.u, n);
     __syncthreads();
     bool modified = true, term = false;
-    if (bs < 0) {
+    if constexpr (C::vlen > 0) {
+        // VectorContinuousCallback: affect!(integrator, simultaneous_events)
+        if constexpr (C::has_affect != 0) {
+            if (tid == 0) S.red[28] = C::template vaffect<double>(S.u, S.pd, c.t, ev, n) ? 1.0 : 0.0;
+            __syncthreads();
+            term = S.red[28] != 0.0;
+        } else {
+            modified = false;
+        }
+    } else if (bs < 0) {
         if constexpr (C::has_affect != 0) {
             if (tid == 0) C::template affect<double>(S.u, S.pd, c.t, n);
             term = C::term != 0;
@@ -687,22 +813,35 @@ __device__ bool wg_ccb_apply(const WGPtrs& S, const Params& P, i64 j, int n, Ctl
 
 template <int CB>
 __device__ __forceinline__ void wg_ccb_find_first(const WGPtrs& S, Ctl& c, int n, int nmax, int& win, double& tmin,
-                                                  double& bs_win, double& res_win) {
+                                                  double& bs_win, double& res_win, double* ev_win) {
     if constexpr (CB < ODEMI_NCCB_WG) {
+        typedef OdemiCcbWg<CB> C;
         double cbt, bs = 0.0, rs;
-        if (wg_ccb_find_time<CB>(S, c, n, nmax, cbt, bs, rs) && (win < 0 || c.tdir * cbt < c.tdir * tmin)) {
-            win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+        if constexpr (C::vlen > 0) {
+            double ev[C::vlen];
+            if (wg_vcb_find_time<CB>(S, c, n, nmax, cbt, ev, rs) && (win < 0 || c.tdir * cbt < c.tdir * tmin)) {
+                win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+#pragma unroll
+                for (int i = 0; i < ODEMI_VCB_MAXLEN_WG; i++) ev_win[i] = (i < C::vlen) ? ev[i < C::vlen ? i : 0] : 0.0;
+            }
+        } else {
+            if (wg_ccb_find_time<CB>(S, c, n, nmax, cbt, bs, rs) && (win < 0 || c.tdir * cbt < c.tdir * tmin)) {
+                win = CB; tmin = cbt; bs_win = bs; res_win = rs;
+#pragma unroll
+                for (int i = 0; i < ODEMI_VCB_MAXLEN_WG; i++) ev_win[i] = 0.0;
+            }
         }
-        wg_ccb_find_first<CB + 1>(S, c, n, nmax, win, tmin, bs_win, res_win);
+        wg_ccb_find_first<CB + 1>(S, c, n, nmax, win, tmin, bs_win, res_win, ev_win);
     }
 }
 template <int CB, typename F, bool GJ, int R, typename GT, typename SaveFn>
 __device__ __forceinline__ bool wg_ccb_apply_winner(const WGPtrs& S, const Params& P, i64 j, int n, Ctl& c,
-                                                    double& tlast, int win, double cbt, double bs, GT& G, float* Jg,
-                                                    SaveFn& savevalues, bool& saved_in_cb) {
+                                                    double& tlast, int win, double cbt, double bs, const double* ev,
+                                                    GT& G, float* Jg, SaveFn& savevalues, bool& saved_in_cb) {
     if constexpr (CB < ODEMI_NCCB_WG) {
-        if (win == CB) return wg_ccb_apply<CB, F, GJ, R>(S, P, j, n, c, tlast, cbt, bs, G, Jg, savevalues, saved_in_cb);
-        return wg_ccb_apply_winner<CB + 1, F, GJ, R>(S, P, j, n, c, tlast, win, cbt, bs, G, Jg, savevalues,
+        if (win == CB)
+            return wg_ccb_apply<CB, F, GJ, R>(S, P, j, n, c, tlast, cbt, bs, ev, G, Jg, savevalues, saved_in_cb);
+        return wg_ccb_apply_winner<CB + 1, F, GJ, R>(S, P, j, n, c, tlast, win, cbt, bs, ev, G, Jg, savevalues,
                                                      saved_in_cb);
     } else {
         return false;
@@ -749,14 +888,21 @@ __device__ __forceinline__ bool wg_handle_callbacks(const WGPtrs& S, const Param
     {
         int win = -1;
         double tmin = 0.0, bs = 0.0, rs = 0.0;
-        wg_ccb_find_first<0>(S, c, n, P.nmax, win, tmin, bs, rs);
+        double ev[ODEMI_VCB_MAXLEN_WG];
+#pragma unroll
+        for (int i = 0; i < ODEMI_VCB_MAXLEN_WG; i++) ev[i] = 0.0;
+        wg_ccb_find_first<0>(S, c, n, P.nmax, win, tmin, bs, rs, ev);
         if (win < 0) {
             c.event_last = 0;
         } else {
             c.last_event_error = rs;
             c.event_last = win + 1;
-            c.sim_mask = 0u;
-            term = wg_ccb_apply_winner<0, F, GJ, R>(S, P, j, n, c, tlast, win, tmin, bs, G, Jg, savevalues,
+            unsigned m = 0u;
+#pragma unroll
+            for (int i = 0; i < ODEMI_VCB_MAXLEN_WG; i++)
+                if (ev[i] != 0.0) m |= 1u << i;
+            c.sim_mask = m;
+            term = wg_ccb_apply_winner<0, F, GJ, R>(S, P, j, n, c, tlast, win, tmin, bs, ev, G, Jg, savevalues,
                                                     saved_in_cb);
         }
     }

@@ -385,9 +385,6 @@ def trace_for(fun, n, np_, algs=("tsit5", "rodas5p"), ns=None, isoutofdomain=Non
     workgroup kernels.  No device is touched."""
     wg = "rodas5p" in algs and (ns is not None or n > LANE_MAX_RODAS)
     cbs = _callback_list(callback)
-    if wg and any(getattr(c, "len", 0) for c in cbs):
-        raise ValueError("VectorContinuousCallback is supported by the lane kernels (Tsit5; Rodas5P with "
-                         f"n <= {LANE_MAX_RODAS}), not by the workgroup kernels")
     if not wg and "tsit5" in algs and n > LANE_MAX_TSIT5:
         raise ValueError(f"Tsit5 supports state dimension <= {LANE_MAX_TSIT5}")
     if ns is not None and "tsit5" in algs:

@@ -880,6 +880,33 @@ def _wg_affect_case(affect, n, u, p, t):
     return body, bool(itg._term)
 
 
+def _wg_vector_case(cb, n, u, p, t):
+    """A VectorContinuousCallback traced for state size n: (n, the condition's
+    body writing out[0:len], the affect's body with the event directions ev[i]
+    as inputs ending in its terminate! result, "", None, False)."""
+    out = _Recorder((cb.len,))
+    cb.condition(out, list(u), t, _TracedIntegrator(u, p, t))
+    couts = [(i, out.vals.get(i, _lift(0.0))) for i in range(cb.len)]
+    cond = _emit_body(couts, "out", indent="        ", pname="p[{}]") + "\n"
+    aff = "        return false;\n"
+    if cb.affect is not None:
+        g = _g()
+        evs = [g.make("ev", (), i) for i in range(cb.len)]
+        itg = _TracedIntegrator(u, p, t)
+        cb.affect(itg, evs)
+        outs = [(i, v) for i, v in enumerate(itg.u.vals) if v is not u[i]]
+        term = itg._term
+        tsym = term if isinstance(term, Sym) else None
+        allouts = outs + ([(n, tsym)] if tsym is not None else [])
+        aff = ""
+        if allouts:
+            aff += (f"        T nu_[{n + 1}];\n" + _emit_body(allouts, "nu_", indent="        ", pname="p[{}]") + "\n"
+                    + "".join(f"        u[{i}] = nu_[{i}];\n" for i, _ in outs))
+        aff += (f"        return nu_[{n}] != T(0);\n" if tsym is not None
+                else f"        return {'true' if term else 'false'};\n")
+    return (n, cond, aff, "", None, False)
+
+
 def _wg_value_case(value):
     v = value if isinstance(value, Sym) else _lift(float(value))
     return "        T r_[1];\n" + _emit_body([(0, v)], "r_", indent="        ", pname="p[{}]") + "\n"
@@ -893,6 +920,7 @@ def _emit_callbacks_wg(callbacks, cases):
     in LDS."""
     dcbs = [(k, cb) for k, cb in enumerate(callbacks) if not hasattr(cb, "affect_neg")]
     ccbs = [(k, cb) for k, cb in enumerate(callbacks) if hasattr(cb, "affect_neg")]
+    vlens = [int(getattr(cb, "len", 0)) for _, cb in ccbs]
     hdr = "    template <typename T>\n    static __device__ __noinline__ "
     src = []
 
@@ -924,11 +952,29 @@ def _emit_callbacks_wg(callbacks, cases):
         if len(ccbs) > 255:
             raise ValueError("at most 255 continuous callbacks")
         src += [f"#define ODEMI_NCCB_WG {len(ccbs)}", "template <int K> struct OdemiCcbWg;"]
+        if max(vlens) > 0:
+            src.append(f"#define ODEMI_VCB_MAXLEN_WG {max(vlens)}")
         for q, (k, cb) in enumerate(ccbs):
             s0, s1 = (int(bool(x)) for x in cb.save_positions)
             rf = {"left": 0, "right": 1, "none": 2}[cb.rootfind]
+            if vlens[q]:
+                # VectorContinuousCallback: vcond(out, ...), vaffect(..., ev) -> terminate!
+                src += [f"template <> struct OdemiCcbWg<{q}> {{",
+                        f"    static constexpr int vlen = {vlens[q]}, has_affect = {int(cb.affect is not None)}, "
+                        f"has_affect_neg = 0, save0 = {s0}, save1 = {s1};",
+                        f"    static constexpr int rootfind = {rf}, interp_points = {int(cb.interp_points)}, "
+                        "term = 0, term_neg = 0;",
+                        f"    static constexpr double abstol = {float(cb.abstol).hex()}, "
+                        f"repeat_nudge = {float(cb.repeat_nudge).hex()};"]
+                src += fn("void", "vcond", "T* __restrict__ out, const T* __restrict__ u, const T* __restrict__ p, T t",
+                          [(n, c + "        break;\n") for n, c, *_ in cases[k]], "break;")
+                src += fn("bool", "vaffect", "T* __restrict__ u, const T* __restrict__ p, T t, "
+                          "const double* __restrict__ ev",
+                          [(n, a) for n, _, a, *_ in cases[k]], "return false;")
+                src.append("};")
+                continue
             src += [f"template <> struct OdemiCcbWg<{q}> {{",
-                    f"    static constexpr int has_affect = {int(cb.affect is not None)}, "
+                    f"    static constexpr int vlen = 0, has_affect = {int(cb.affect is not None)}, "
                     f"has_affect_neg = {int(cb.affect_neg is not None)}, save0 = {s0}, save1 = {s1};",
                     f"    static constexpr int rootfind = {rf}, interp_points = {int(cb.interp_points)}, "
                     f"term = {terms(k, 4)}, term_neg = {terms(k, 5)};",
@@ -953,9 +999,6 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None, callbacks=Non
     global _CUR
     ns = sorted(set(int(n) for n in ns))
     callbacks = list(callbacks or [])
-    for cb in callbacks:
-        if getattr(cb, "len", 0):
-            raise ValueError("VectorContinuousCallback runs in the lane kernels only, not in the workgroup kernels")
     cbcases = [[] for _ in callbacks]
     fcases, jcases, tcases, vcases, icases = [], [], [], [], []
     flops_f, flops_j, nnz = {}, {}, {}
@@ -1017,6 +1060,9 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None, callbacks=Non
                               + _emit_body([(0, r)], "r_", indent="        ", pname="T(p[{}])")
                               + "\n        return r_[0] != T(0);\n    }")
             for k, cb in enumerate(callbacks):
+                if getattr(cb, "len", 0):
+                    cbcases[k].append(_wg_vector_case(cb, n, u, p, t))
+                    continue
                 cond = cb.condition(list(u), t, _TracedIntegrator(u, p, t))
                 if not hasattr(cb, "affect_neg") and not isinstance(cond, Sym):
                     cond = 1.0 if cond else 0.0

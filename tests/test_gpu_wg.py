@@ -408,3 +408,53 @@ def test_wg_callbacks_plain_engine_and_server_equal_persistent():
         for c, j in enumerate(idx):
             assert np.array_equal(r.u_final[:nv[j], c], ref["u_final"][:nv[j], j]), j
         assert np.array_equal(r.stats, ref["stats"][:, idx]) and np.array_equal(r.retcode, ref["retcode"][idx])
+
+
+def _walls_cond(out, u, t, itg):
+    out[0] = u[0]                 # the first mass at its rest position
+    out[1] = u[1] - 0.02          # the second mass past +0.02
+
+
+def _walls_affect(itg, ev):
+    m = len(itg.u) // 2
+    itg.u[m] = odemi.sym_ifelse(ev[0] == -1, -0.5 * itg.u[m], itg.u[m])
+    itg.u[m + 1] = odemi.sym_ifelse(ev[1] != 0, -itg.u[m + 1], itg.u[m + 1])
+    odemi.terminate(itg, when=itg.t > 0.8)
+
+
+_WG_VCBS = odemi.CallbackSet(odemi.VectorContinuousCallback(_walls_cond, _walls_affect, 2),
+                             odemi.ContinuousCallback(lambda u, t, itg: u[0] - 1.0, None, None))
+
+
+@pytest.mark.parametrize("saves", ["everystep", "final"])
+def test_wg_vector_continuous_callbacks_bit_identical(saves):
+    """A VectorContinuousCallback in the workgroup kernels (uneven chains n =
+    8..24, FP64 LU): two components (the first mass's downcrossing of its rest
+    position halves and reverses its velocity, the second mass crossing +0.02
+    either way reverses its own), simultaneous events as the lane engines
+    handle them, a conditional terminate! once t > 0.8, in a CallbackSet with
+    a scalar ContinuousCallback that is never an event; bit-identical to the
+    CPU restatement: save times and states, final states, retcodes, nf /
+    naccept / nreject / ncondition."""
+    d = _chain_events(40, 35)
+    m = dict(save_everystep=True, save_start=True, save_end=True) if saves == "everystep" else \
+        dict(save_start=False, save_end=True)
+    print(f"[case] chain wg vector callbacks fp64 {saves}", flush=True)
+    out = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], reltol=d["reltol"],
+                             abstol=d["abstol"], nvec=d["nvec"], callback=_WG_VCBS, max_saves=4096, **m)
+    ccb = [_PyCB(c) for c in _WG_VCBS.continuous_callbacks]
+    codes, events = set(), 0
+    for j in range(d["u0"].shape[1]):
+        n = int(d["nvec"][j])
+        r = O.solve_builtin(O.PROB_CHAIN, d["u0"][:n, j], d["p"][:, j], (0.0, float(d["tf"][j])), alg=O.RODAS5P,
+                            reltol=d["reltol"], abstol=d["abstol"], ccallbacks=ccb, **m)
+        ns = int(out["nsaved"][j])
+        assert int(out["retcode"][j]) == r["retcode"] and ns == r["nsaved"], j
+        assert np.array_equal(out["t_out"][:ns, j], r["t"]), j
+        assert np.array_equal(out["u_out"][:ns, :n, j], r["u"]), j
+        assert np.array_equal(out["u_final"][:n, j], r["u_final"]) and out["t_final"][j] == r["t_final"], j
+        st = out["stats"][:, j]
+        assert (st[0], st[1], st[2], st[6]) == (r["nf"], r["naccept"], r["nreject"], r["ncondition"]), j
+        codes.add(r["retcode"])
+    print("retcodes", sorted(codes))
+    assert 2 in codes   # some chains terminate

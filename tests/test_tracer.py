@@ -11,6 +11,7 @@ sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "ordinarydiffeq
 import odemi  # noqa: E402
 from odemi import _lib  # noqa: E402
 from odemi.tracer import trace_problem  # noqa: E402
+from odemi.api import trace_for  # noqa: E402
 
 
 def _decay(du, u, p, t):
@@ -70,16 +71,33 @@ def test_callback_module_compiles_for_gfx950():
         assert rc == 0, log.value.decode()[:2000]
 
 
-def test_vector_callbacks_rejected_by_workgroup_kernels_and_unknown_types():
+def test_vector_callbacks_in_workgroup_kernels_emit_and_compile_and_unknown_types():
+    """A VectorContinuousCallback traced for the workgroup kernels (n = 8 and
+    10): its condition per state size writes out[0:len], its affect reads the
+    event directions ev[i] and returns its conditional terminate!; the source
+    compiles for gfx950 (hipRTC, no device).  Unknown callback types and
+    Python branches on traced values are refused."""
     def vc(out, u, t, itg):
         out[0] = u[0]
-    cb = odemi.VectorContinuousCallback(vc, lambda itg, ev: None, 1)
+        out[1] = u[1] - 0.5
+
+    def va(itg, ev):
+        m = len(itg.u) // 2
+        itg.u[m] = odemi.sym_ifelse(ev[0] == -1, -itg.u[m], itg.u[m])
+        odemi.terminate(itg, when=itg.t > 2.0)
+    cb = odemi.VectorContinuousCallback(vc, va, 2)
 
     def big(du, u, p, t):
-        for i in range(8):
-            du[i] = -u[i]
-    with pytest.raises(ValueError, match="lane kernels"):
-        odemi.compile_problem(odemi.ODEFunction(big), 8, 2, algs=("rodas5p",), callback=cb)
+        for i in range(len(u)):
+            du[i] = -p[0] * u[i]
+    tr, wg = trace_for(odemi.ODEFunction(big), 10, 1, algs=("rodas5p",), ns=[8, 10], callback=cb)
+    assert wg and "#define ODEMI_VCB_MAXLEN_WG 2" in tr.src and "vlen = 2" in tr.src
+    vc_src = tr.src[tr.src.index("void vcond("):tr.src.index("bool vaffect(")]
+    assert "case 8:" in vc_src and "case 10:" in vc_src and "out[1] =" in vc_src
+    va_src = tr.src[tr.src.index("bool vaffect("):]
+    assert "ev[0]" in va_src and "u[4] = " in va_src and "u[5] = " in va_src and "return nu_[8] != T(0);" in va_src
+    path, _ = _lib.precompile(tr.src, 10, 1, _lib.COMPILE_WG)
+    assert os.path.getsize(path) > 0
     with pytest.raises(TypeError):
         odemi.compile_problem(odemi.ODEFunction(_decay), 2, 2, algs=("tsit5",), callback=object())
     with pytest.raises(TypeError):
@@ -242,8 +260,7 @@ def _kick(itg):
 def test_workgroup_callbacks_emit_per_size_and_compile():
     """Discrete and continuous callbacks of an uneven batch are traced once per
     state size into OdemiDcbWg / OdemiCcbWg (a switch on n), with the affect's
-    index following n, and compile with the workgroup kernels; a
-    VectorContinuousCallback is refused there."""
+    index following n, and compile with the workgroup kernels."""
     from odemi import problems as P
     from odemi.api import trace_for
     cbs = odemi.CallbackSet(
@@ -261,12 +278,6 @@ def test_workgroup_callbacks_emit_per_size_and_compile():
     secs = ctypes.c_double()
     rc = L.odemi_rtc_codeobject(src.encode(), 12, 3, _lib.COMPILE_WG, path, 512, ctypes.byref(secs), log, 1 << 16)
     assert rc == 0, log.value.decode()[:3000]
-
-    def vc(out, u, t, itg):
-        out[0] = u[0]
-    with pytest.raises(ValueError, match="VectorContinuousCallback"):
-        trace_for(P.CHAIN, 12, 3, algs=("rodas5p",), ns=[8, 12],
-                  callback=odemi.VectorContinuousCallback(vc, lambda itg, ev: None, 1))
 
 
 def test_mixed_precision_advice_for_tsit5_continuous_callbacks():
