@@ -20,8 +20,8 @@
 # straight-line arithmetic on u, p, t (+ - * / ^, sin cos exp log sqrt tanh
 # abs, comparisons, & | !, ifelse, `t in values`); no `if` on state values.
 # Keywords: saveat, tstops, isoutofdomain, callback (DiscreteCallback,
-# ContinuousCallback, VectorContinuousCallback on the lane kernels; the first
-# two on the workgroup kernels), dt, dtmin, dtmax, maxiters, save_*.  This
+# ContinuousCallback, VectorContinuousCallback; on the workgroup kernels traced
+# per state size), dt, dtmin, dtmax, maxiters, save_*.  This
 # image has no Julia, so this file is not exercised by the test suite; the
 # Python front-end (odemi/api.py, odemi/tracer.py) emits the same forms and is.
 module EnsembleMI355X
@@ -477,13 +477,12 @@ end
 
 """Workgroup forms for the state sizes `ns`: `odemi_f_wg`, `odemi_jac_wg`,
 `odemi_tgrad_wg`, `odemi_jvp_wg` (J v over each row's nonzeros in column
-order), `odemi_isout_wg` and the callbacks' `OdemiDcbWg<K>` / `OdemiCcbWg<K>`,
-each a switch on n; plus empty lane forms and the Gauss-Jordan tile mask."""
+order), `odemi_isout_wg` and the callbacks' `OdemiDcbWg<K>` / `OdemiCcbWg<K>`
+(scalar and vector ContinuousCallbacks), each a switch on n; plus empty lane
+forms and the Gauss-Jordan tile mask."""
 function trace_wg(fun::SciMLBase.ODEFunction, ns, np; isoutofdomain = nothing, callback = nothing)
     ns = sort(unique(ns))
     ccbs, dcbs = split_callbacks(callback)
-    any(cb -> cb isa VectorContinuousCallback, ccbs) &&
-        error("VectorContinuousCallback runs in the lane kernels only, not in the workgroup kernels")
     fc, jc, tc, vc, ic = String[], String[], String[], String[], String[]
     dcase = [String[] for _ in dcbs]; dterm = [Bool[] for _ in dcbs]
     ccase = [String[] for _ in ccbs]; cterm = [Tuple{Bool,Bool}[] for _ in ccbs]
@@ -523,6 +522,20 @@ function trace_wg(fun::SciMLBase.ODEFunction, ns, np; isoutofdomain = nothing, c
             push!(dterm[k], term)
         end
         for (k, cb) in enumerate(ccbs)
+            if cb isa VectorContinuousCallback
+                # vcond writes out[0:len]; vaffect sees the event directions ev[i] and returns terminate!
+                out = Vector{Any}(fill(Sym(0.0), cb.len))
+                cb.condition(out, u, t, TracedIntegrator(u, params_arg(p, np), t))
+                evs = [node(:ev, Int[], Float64(i)) for i in 0:cb.len-1]
+                vo, vt = cb.affect! === nothing ? (Tuple{Int,Sym}[], false) : trace_affect(cb.affect!, u, p, t, np, evs)
+                push!(ccase[k], "        case $n: {\n" *
+                                emit_body(Pair{String,Sym}["out[$(i - 1)]" => convert(Sym, out[i]) for i in 1:cb.len], "        ") *
+                                "        break;\n        }" * "\x00" *
+                                "        case $n: {\n" * affect_body(vo, n, "        ") *
+                                "        return $(vt ? "true" : "false");\n        }")
+                push!(cterm[k], (false, false))
+                continue
+            end
             c = convert(Sym, cb.condition(u, t, TracedIntegrator(u, params_arg(p, np), t)))
             ao, at = cb.affect! === nothing ? (Tuple{Int,Sym}[], false) : trace_affect(cb.affect!, u, p, t, np)
             no, nt = cb.affect_neg! === nothing ? (Tuple{Int,Sym}[], false) : trace_affect(cb.affect_neg!, u, p, t, np)
@@ -570,12 +583,25 @@ function trace_wg(fun::SciMLBase.ODEFunction, ns, np; isoutofdomain = nothing, c
         end
     end
     if !isempty(ccbs)
+        length(ccbs) <= 255 || error("at most 255 continuous callbacks")
         src *= "#define ODEMI_NCCB_WG $(length(ccbs))\ntemplate <int K> struct OdemiCcbWg;\n"
+        vmax = maximum([cb isa VectorContinuousCallback ? cb.len : 0 for cb in ccbs])
+        vmax > 0 && (src *= "#define ODEMI_VCB_MAXLEN_WG $vmax\n")
         for (k, cb) in enumerate(ccbs)
-            all(==(cterm[k][1]), cterm[k]) || error("an affect! must call terminate! for every state size or for none")
             s0, s1 = Int.(cb.save_positions)
+            if cb isa VectorContinuousCallback
+                src *= "template <> struct OdemiCcbWg<$(k - 1)> {\n    static constexpr int vlen = $(cb.len), " *
+                       "has_affect = $(Int(cb.affect! !== nothing)), has_affect_neg = 0, save0 = $s0, save1 = $s1;\n" *
+                       "    static constexpr int rootfind = $(rootfind_code(cb)), interp_points = $(cb.interp_points), " *
+                       "term = 0, term_neg = 0;\n" *
+                       "    static constexpr double abstol = $(cb.abstol), repeat_nudge = $(cb.repeat_nudge);\n" *
+                       mfn("void", "vcond", "T* __restrict__ out, $ua", ccase[k], 1, "break;") *
+                       mfn("bool", "vaffect", "$wa, const double* __restrict__ ev", ccase[k], 2, "return false;") * "};\n"
+                continue
+            end
+            all(==(cterm[k][1]), cterm[k]) || error("an affect! must call terminate! for every state size or for none")
             at, nt = cterm[k][1]
-            src *= "template <> struct OdemiCcbWg<$(k - 1)> {\n    static constexpr int has_affect = $(Int(cb.affect! !== nothing)), " *
+            src *= "template <> struct OdemiCcbWg<$(k - 1)> {\n    static constexpr int vlen = 0, has_affect = $(Int(cb.affect! !== nothing)), " *
                    "has_affect_neg = $(Int(cb.affect_neg! !== nothing)), save0 = $s0, save1 = $s1;\n" *
                    "    static constexpr int rootfind = $(rootfind_code(cb)), interp_points = $(cb.interp_points), " *
                    "term = $(Int(at)), term_neg = $(Int(nt));\n" *

@@ -913,8 +913,8 @@ def _wg_value_case(value):
 
 
 def _emit_callbacks_wg(callbacks, cases):
-    """DiscreteCallbacks (OdemiDcbWg<K>) and ContinuousCallbacks (OdemiCcbWg<K>)
-    for the workgroup kernels: the condition and affects traced once per state
+    """DiscreteCallbacks (OdemiDcbWg<K>) and Continuous / VectorContinuous-
+    Callbacks (OdemiCcbWg<K>) for the workgroup kernels: the condition and affects traced once per state
     size (`cases[k]` = [(n, condition body, affect body, affect_neg body, term,
     term_neg)]), dispatched by a switch on n; one thread runs them on the state
     in LDS."""
@@ -994,8 +994,8 @@ def trace_problem_wg(f, ns, np_, jac=None, tgrad=None, isout=None, callbacks=Non
     """Workgroup forms for every state size in `ns` (distinct, sorted).
     isout(u, p, t): the isoutofdomain predicate, traced per size into
     odemi_isout_wg (one thread evaluates it on the state in LDS).  callbacks:
-    DiscreteCallbacks and (scalar) ContinuousCallbacks, traced per size into
-    OdemiDcbWg<K> / OdemiCcbWg<K>."""
+    DiscreteCallbacks, ContinuousCallbacks and VectorContinuousCallbacks,
+    traced per size into OdemiDcbWg<K> / OdemiCcbWg<K>."""
     global _CUR
     ns = sorted(set(int(n) for n in ns))
     callbacks = list(callbacks or [])

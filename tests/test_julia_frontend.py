@@ -65,15 +65,27 @@ def test_julia_lane_emitter_matches_python_forms():
         assert re.search(rf"#define {macro} \$", JL), macro
 
 
+def _vc(out, u, t, itg):
+    out[0] = u[0]
+    out[1] = u[1] - 0.5
+
+
 def test_julia_workgroup_emitter_matches_python_forms():
     cbs = odemi.CallbackSet(odemi.ContinuousCallback(lambda u, t, itg: u[0], None, affect_neg=lambda itg: None),
+                            odemi.VectorContinuousCallback(_vc, lambda itg, ev: None, 2),
                             odemi.DiscreteCallback(lambda u, t, itg: t == 0.25, lambda itg: None))
     tr, wg = trace_for(P.CHAIN, 12, 3, algs=("rodas5p",), ns=[8, 12], isoutofdomain=lambda u, p, t: t > 5.0,
                        callback=cbs)
     assert wg
     sigs = _signatures(tr.src)
     assert {"odemi_f_wg", "odemi_jac_wg", "odemi_tgrad_wg", "odemi_jvp_wg", "odemi_isout_wg", "cond", "affect",
-            "affect_neg"} <= {name for name, _ in sigs}
+            "affect_neg", "vcond", "vaffect"} <= {name for name, _ in sigs}
+    # every OdemiCcbWg the Python tracer writes opens with vlen (the device code reads it for
+    # scalar and vector callbacks alike); so does each of the Julia emitter's two templates
+    heads = re.findall(r"struct OdemiCcbWg<\d+> \{\n    static constexpr int (\w+) =", tr.src)
+    assert heads == ["vlen", "vlen"], heads
+    jheads = re.findall(r'struct OdemiCcbWg<\$\(k - 1\)> \{\\n    static constexpr int (\w+) =', JL)
+    assert jheads == ["vlen", "vlen"], jheads
     for name, args in sigs:
         assert _declared(name, args), (name, args)
     for member in re.findall(r"static constexpr (?:int|double) ([^;]*);", tr.src):
@@ -83,7 +95,7 @@ def test_julia_workgroup_emitter_matches_python_forms():
                  "template <int K> struct OdemiCcbWg;", "for (int q = tid; q < $cnt; q += nt) {",
                  "const int o = $r0 + q;", "J[o * ld + o + ($d)]", "#define ODEMI_GJ_RMASK 0x$"):
         assert line in JL, line
-    for macro in ("ODEMI_NDCB_WG", "ODEMI_NCCB_WG"):
+    for macro in ("ODEMI_NDCB_WG", "ODEMI_NCCB_WG", "ODEMI_VCB_MAXLEN_WG"):
         assert re.search(rf"#define {macro} \$", JL), macro
 
 
