@@ -233,15 +233,12 @@ __device__ __forceinline__ float fastlog2f(float x) {
     float t5 = t3 / t4;
     return fexp + t5;
 }
-// fastpower's exp2 half, given lg = fastlog2f((float)x) of a finite nonzero x
-__device__ __forceinline__ double fastpower_lg(float lg, double y) {
-    float prod = (float)y * lg;
-    return (double)(float)det_exp((double)prod * kLN2);
-}
 __device__ __forceinline__ double fastpower(double x, double y) {
     if (x == 0.0) return 0.0;
     if (isinf(x) && isinf(y)) return __longlong_as_double(0x7ff0000000000000LL);
-    return fastpower_lg(fastlog2f((float)x), y);
+    float lg = fastlog2f((float)x);
+    float prod = (float)y * lg;
+    return (double)(float)det_exp((double)prod * kLN2);
 }
 // The same fastpower with the hardware reciprocal in Goldberg's rational and
 // the hardware exp2 (v_rcp_f32 / v_exp_f32, ~1 ulp): the mixed-precision lane
@@ -364,8 +361,8 @@ struct Ctl {
     int event_last;
     unsigned sim_mask;   // its components when a vector callback's (prev_simultaneous_events)
     double last_event_error;
-    // fastpower(errold, beta2), kept from the accept that set errold (NaN: not
-    // known yet), and log2(EEst) of the last controller call (both engines)
+    // mixed engines: fastpower(errold, beta2), kept from the accept that set
+    // errold (NaN: not known yet), and log2(EEst) of the last controller call
     float erroldp, lgEEst;
 };
 
@@ -434,13 +431,8 @@ __device__ __forceinline__ double stepsize_controller(Ctl& c, const CtlConst& k)
             if (c.erroldp != c.erroldp) c.erroldp = fastpower_hw(c.errold, k.beta2);
             x = div_nr(q11, (double)c.erroldp * k.gamma);
         } else {
-            // fastpower(EEst, beta1) (EEst nonzero, beta1 finite) with its log2
-            // kept for the accept; fastpower(errold, beta2) as the accept that
-            // set errold left it (the same float either way)
-            c.lgEEst = fastlog2f((float)c.EEst);
-            q11 = fastpower_lg(c.lgEEst, k.beta1);
-            if (c.erroldp != c.erroldp) c.erroldp = (float)fastpower(c.errold, k.beta2);
-            q = q11 / (double)c.erroldp;
+            q11 = fastpower(c.EEst, k.beta1);
+            q = q11 / fastpower(c.errold, k.beta2);
             x = q / k.gamma;
         }
         c.q11 = q11;
@@ -461,11 +453,6 @@ __device__ __forceinline__ void accept_controller(Ctl& c, const CtlConst& k, dou
                                          : fastpower_hw(c.errold, k.beta2);
         dtnew = div_nr(c.dt, q);   // q in [1/qmax, 1/qmin]
     } else {
-        // errold = EEst: its power from the controller's log2 (errold is in
-        // [qoldinit, 1] here, so fastpower's special cases do not arise);
-        // otherwise the next controller call evaluates it
-        if (c.errold == c.EEst) c.erroldp = (float)fastpower_lg(c.lgEEst, k.beta2);
-        else c.erroldp = __builtin_nanf("");
         dtnew = c.dt / q;
     }
     double dp = c.tdir * jl_min(fabs(c.dtmax), fabs(dtnew));
