@@ -540,7 +540,8 @@ def bench_lorenz(args):
     # the same FP64 solve from host arrays: H2D of the inputs through D2H of the outputs
     _, mspc, _ = _timed(lambda: run(fp64, d["u0"], d["p"], d["t0"], d["tf"]), 2, 1, torch)
     res["pcie_inclusive_fp64"] = {"trajectories_per_s": N / (mspc / 1e3), "ms": mspc,
-                                  "note": "host (pageable) inputs and outputs, H2D + solve + D2H"}
+                                  "note": "numpy inputs (pageable), H2D + solve + D2H into the host outputs "
+                                          "solve_arrays allocates (pinned from 1 MiB up)"}
     if not args.no_plain:
         print("[bench] plain fp64 engine", file=sys.stderr, flush=True)
         op, msp, _ = _timed(lambda: run(plain), 1, 0, torch)

@@ -473,6 +473,25 @@ def _is_torch(x):
     return type(x).__module__.startswith("torch")
 
 
+_PIN_MIN_BYTES = 1 << 20
+
+
+def _host_empty(shape, dtype):
+    """np.empty for an output of a host-array solve; from 1 MiB up a view of a
+    pinned (page-locked) tensor from torch's caching host allocator, so the
+    device-to-host copy runs at DMA speed into pages that are already mapped.
+    Fresh pageable pages take a page fault each: 4-17 GB/s for the copy on
+    the MI355X box against 57 GB/s pinned.  The array keeps its tensor (and
+    the block) alive; freed blocks are reused by later solves."""
+    nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    if nbytes >= _PIN_MIN_BYTES:
+        import torch
+        if torch.cuda.is_available():
+            tdt = {np.float64: torch.float64, np.int32: torch.int32}[dtype]
+            return torch.empty(shape, dtype=tdt, pin_memory=True).numpy()
+    return np.empty(shape, dtype=dtype)
+
+
 def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abstol=None, saveat=None,
                  save_start=False, save_end=True, save_everystep=False, save_end_user=-1, max_saves=None,
                  dt=None, dtmax=None, dtmin=0.0, maxiters=1_000_000, ctx=None, outputs=None, nvec=None,
@@ -564,9 +583,10 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
         tfc = np.ascontiguousarray(np.broadcast_to(tf, (N,)), dtype=np.float64)
         sat = np.ascontiguousarray(sa if len(sa) else np.zeros(1))
         tst = np.ascontiguousarray(tsv if len(tsv) else np.zeros(1))
-        o = dict(t_out=np.empty((max(max_saves, 1), N)), u_out=np.empty((max(max_saves, 1), n, N)),
-                 nsaved=np.empty(N, dtype=np.int32), u_final=np.empty((n, N)), t_final=np.empty(N),
-                 retcode=np.empty(N, dtype=np.int32), stats=np.empty((_lib.NSTATS, N), dtype=np.int32))
+        f8, i4 = np.float64, np.int32
+        o = dict(t_out=_host_empty((max(max_saves, 1), N), f8), u_out=_host_empty((max(max_saves, 1), n, N), f8),
+                 nsaved=_host_empty((N,), i4), u_final=_host_empty((n, N), f8), t_final=_host_empty((N,), f8),
+                 retcode=_host_empty((N,), i4), stats=_host_empty((_lib.NSTATS, N), i4))
         keep += [u0c, pc, t0c, tfc, sat, tst]
         if nvec is not None:
             nvc = np.ascontiguousarray(nvec, dtype=np.int32)

@@ -101,3 +101,29 @@ def test_robertson_matches_oracle(engine, precision):
         err = np.abs(out["u_final"] - ref["u_final"]) / (d["abstol"] + d["reltol"] * np.abs(ref["u_final"]))
         print(f"mixed scaled err max {err.max():.3e}")
         assert np.quantile(err.max(axis=0), 0.99) <= 10.0
+
+
+def test_host_arrays_with_pinned_outputs_equal_device_arrays():
+    """A host-array (numpy) solve allocates its outputs of 1 MiB and more as
+    views of pinned torch tensors (the device-to-host copies then run at DMA
+    speed); with an odd N and saveat outputs the results equal the same solve
+    on device-resident arrays bit for bit, on two solves in a row (the second
+    one's pinned blocks may be the first one's, reused)."""
+    import torch
+    N = 200001
+    d = P.lorenz_large(N=N)
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], saveat=np.linspace(0.0, 1.0, 9), save_start=False,
+              save_end=True)
+    ea = odemi.EnsembleMI355X(engine="persistent", precision="fp64")
+    print(f"[case] lorenz N={N} host arrays (pinned outputs) vs device arrays", flush=True)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    ref = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"]), ea, **kw)
+    ref = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in ref.items()}
+    for rep in range(2):
+        out = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"], ea, **kw)
+        for key in ("u_final", "t_final", "retcode", "stats", "t_out", "u_out", "nsaved"):
+            assert isinstance(out[key], np.ndarray), key
+            assert np.array_equal(out[key], ref[key], equal_nan=True), (rep, key)
+        assert isinstance(out["u_out"].base, torch.Tensor) and out["u_out"].base.is_pinned()
+    assert (out["retcode"] == 1).all() and (out["nsaved"] >= 9).all()
