@@ -505,7 +505,9 @@ def solve_arrays(fun, alg, u0, p, t0, tf, ensemblealg=None, *, reltol=None, abst
     max_saves room; nsaved beyond it is reported, the extra points dropped).
 
     Arrays may be numpy (host) or torch CUDA tensors (device-resident, zero
-    copies).  Returns a dict of output arrays of the same kind."""
+    copies).  Returns a dict of output arrays of the same kind; host outputs of
+    1 MiB and more are numpy views of pinned (page-locked) torch tensors, which
+    go back to torch's pinned-block cache when the arrays are freed."""
     ea = ensemblealg or EnsembleMI355X()
     ctx = ctx or get_context(ea.device)
     fun = fun if isinstance(fun, ODEFunction) else ODEFunction(fun)
