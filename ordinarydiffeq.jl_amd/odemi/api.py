@@ -242,7 +242,8 @@ class ContinuousCallback:
 class VectorContinuousCallback:
     """VectorContinuousCallback(condition, affect!, len; rootfind,
     save_positions, interp_points, abstol, repeat_nudge) (SciMLBase; handled by
-    lib/DiffEqBase/src/callbacks.jl:239-362,499-640), lane kernels.
+    lib/DiffEqBase/src/callbacks.jl:239-362,499-640), lane and workgroup
+    kernels (traced per state size in the latter).
     condition(out, u, t, integrator) fills out[0:len] with traced values; a
     sign change of any of them within an accepted step is an event: the step
     is cut back to the earliest root and affect(integrator, events) runs once,

@@ -458,3 +458,46 @@ def test_wg_vector_continuous_callbacks_bit_identical(saves):
         codes.add(r["retcode"])
     print("retcodes", sorted(codes))
     assert 2 in codes   # some chains terminate
+
+
+def _vterm_cond(out, u, t, itg):
+    out[0] = u[0] - 0.02          # the first mass past +0.02
+    out[1] = u[1] - 10.0          # never reached: a second component that stays silent
+
+
+def _vterm_affect(itg, ev):
+    odemi.terminate(itg, when=ev[0] == 1)   # upcrossings only
+
+
+def test_wg_vector_callbacks_mixed_and_plain_engine():
+    """A VectorContinuousCallback (RightRootFind) whose affect terminates on an
+    upcrossing of its first component: the mixed Gauss-Jordan engine ends the
+    same chains with the same retcodes, at event times within 1e-6 of the FP64
+    engine's; and the mirrored-wall vector callbacks through the plain FP64
+    workgroup engine (the event state in HBM between launches) equal the
+    persistent engine's final states, retcodes and stats bit for bit."""
+    d = _chain_events(40, 36)
+    d["u0"][0] = -0.05
+    cb = odemi.VectorContinuousCallback(_vterm_cond, _vterm_affect, 2, rootfind="right")
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], nvec=d["nvec"], callback=cb, save_start=False, save_end=True,
+              max_saves=8)
+    print("[case] chain wg vector callback terminate fp64 + mixed", flush=True)
+    o64 = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], **kw)
+    omx = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                             odemi.EnsembleMI355X(precision="mixed"), **kw)
+    assert 2 in set(int(x) for x in o64["retcode"])
+    assert np.array_equal(omx["retcode"], o64["retcode"])
+    term = o64["retcode"] == 2
+    assert np.abs(omx["t_final"][term] - o64["t_final"][term]).max() <= 1e-6
+    d = _chain_events(40, 37)
+    nv = np.asarray(d["nvec"])
+    kw = dict(reltol=d["reltol"], abstol=d["abstol"], nvec=nv, callback=_WG_VCBS, save_start=False, save_end=True)
+    print("[case] chain wg vector callbacks plain fp64 vs persistent", flush=True)
+    per = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"], **kw)
+    pln = odemi.solve_arrays(P.CHAIN, odemi.Rodas5P(), d["u0"], d["p"], d["t0"], d["tf"],
+                             odemi.EnsembleMI355X(engine="plain"), **kw)
+    for j in range(len(nv)):
+        n = int(nv[j])
+        assert np.array_equal(pln["u_final"][:n, j], per["u_final"][:n, j]), j
+    assert np.array_equal(pln["retcode"], per["retcode"]) and np.array_equal(pln["t_final"], per["t_final"])
+    assert np.array_equal(pln["stats"], per["stats"])
