@@ -91,6 +91,9 @@ struct Params {
     // start-up values computed ahead of the persistent lane kernels (or null):
     double* pre_dt;           // [N] initial dt (initdt or the user's dt)
     double* pre_k1;           // [nmax][N] Tsit5 FSAL stage f(u0), exact in FP64
+    double* rec_in;           // queue-order input records (see rec_in_words) or null
+    double* rec_out;          // queue-order result records (see rec_out_words) or null
+    const int* inv;           // queue slot of each trajectory, with the records
 };
 
 // ---------------------------------------------------------------------------
@@ -684,8 +687,7 @@ __device__ __forceinline__ double user_dt(const Ctl& c, const Params& P) {
 }
 
 // Common per-trajectory initialisation of the controller state.
-__device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
-    double t0 = P.t0[j], tf = P.tf[j];
+__device__ __forceinline__ void ctl_init_t(Ctl& c, const Params& P, double t0, double tf) {
     c.t = t0; c.tprev = t0; c.tf = tf;
     c.tdir = (tf < t0) ? -1.0 : 1.0;
     // dtmax defaults to tf - t0; a positive one takes tdir's sign (solve.jl:401)
@@ -701,6 +703,7 @@ __device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) {
     c.tsp = 0; c.tsend = 0; c.tnext = tf;
     if (P.ntstops > 0) tstops_init(c, P, t0);
 }
+__device__ __forceinline__ void ctl_init(Ctl& c, const Params& P, i64 j) { ctl_init_t(c, P, P.t0[j], P.tf[j]); }
 
 // solution_endpoint_match_cur_integrator! (integrator_utils.jl:540-588) and
 // result write-out.
@@ -730,6 +733,60 @@ __device__ void finish_lane(Ctl& c, const Params& P, i64 j, const S* u, int rc) 
         P.stats[ST_NW * P.N + j] = c.nw;
         P.stats[ST_NSOLVE * P.N + j] = c.nsolve;
         P.stats[ST_NCONDITION * P.N + j] = c.ncond;
+    }
+}
+
+// Queue-order records of the final-state-only Tsit5 persistent engine.  The
+// lanes take trajectories in longest-first queue order, so trajectory j's
+// inputs and results sit at scattered indices of the SoA arrays: read and
+// written in place they cost a 64-128 B line per 8-byte value (1.7 GB per
+// launch for a 1 M Lorenz batch against 0.2 GB of algorithmic bytes).  With
+// records the prep kernel (one thread per trajectory, coalesced SoA reads)
+// writes trajectory j's inputs and start-up results as one record at its
+// queue slot inv[j]; a lane reads the record of the slot it takes (a refill
+// batch takes consecutive slots: contiguous records) and writes its result
+// record whole at that slot; rec_unpack (one thread per trajectory) moves the
+// results to the SoA outputs with coalesced writes.  Input record, doubles:
+// u0[N] p[NP] t0 tf k1[N] dt.  Result record: u_final[N] t_final, then int32
+// nsaved retcode stats[7].  Used only without saves before the end point
+// (no save_start, saveat, save_everystep or callbacks): the end point's saved
+// copy is the final state.
+constexpr int rec_in_words(int n, int np) { return (2 * n + np + 3 + 1) & ~1; }
+constexpr int rec_out_words(int n) { return (n + 1 + 5 + 1) & ~1; }
+
+template <int N, typename S>
+__device__ __forceinline__ void finish_lane_rec(const Ctl& c, const Params& P, i64 slot, const S* u, int rc) {
+    double* r = P.rec_out + slot * rec_out_words(N);
+    const bool save = (rc == RC_SUCCESS || rc == RC_TERMINATED) && P.save_end;   // finish_lane's end point
+#pragma unroll
+    for (int i = 0; i < N; i++) r[i] = (double)u[i];
+    r[N] = c.t;
+    int* q = (int*)(r + N + 1);
+    q[0] = c.nsaved + (save ? 1 : 0);
+    q[1] = rc;
+    q[2 + ST_NF] = c.nf; q[2 + ST_NACCEPT] = c.naccept; q[2 + ST_NREJECT] = c.nreject;
+    q[2 + ST_NJACS] = c.njacs; q[2 + ST_NW] = c.nw; q[2 + ST_NSOLVE] = c.nsolve;
+    q[2 + ST_NCONDITION] = c.ncond;
+}
+template <int N>
+__device__ void rec_unpack(const Params& P) {
+    const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= P.N) return;
+    const double* r = P.rec_out + (i64)P.inv[j] * rec_out_words(N);
+    const int* q = (const int*)(r + N + 1);
+    const int ns = q[0];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        if (P.u_final) P.u_final[(i64)i * P.N + j] = r[i];
+        if (ns > 0 && P.max_saves > 0) P.u_out[(i64)i * P.N + j] = r[i];   // save row 0
+    }
+    if (P.t_final) P.t_final[j] = r[N];
+    if (ns > 0 && P.max_saves > 0) P.t_out[j] = r[N];
+    if (P.nsaved) P.nsaved[j] = ns;
+    P.retcode[j] = q[1];
+    if (P.stats) {
+#pragma unroll
+        for (int k = 0; k < ST_COUNT; k++) P.stats[(i64)k * P.N + j] = q[2 + k];
     }
 }
 
@@ -773,7 +830,21 @@ __device__ __forceinline__ void tsit5_first_stage_dt(Tsit5Lane<S, K, N>& L, cons
     }
 }
 template <typename S, typename K, int N>
-__device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& P, i64 j) {
+__device__ __forceinline__ void tsit5_init(Tsit5Lane<S, K, N>& L, const Params& P, i64 j, i64 slot = -1) {
+    if (P.rec_in) {   // the record prep wrote for this queue slot
+        const double* r = P.rec_in + slot * rec_in_words(N, ODEMI_NP);
+        ctl_init_t(L.c, P, r[N + ODEMI_NP], r[N + ODEMI_NP + 1]);
+#pragma unroll
+        for (int i = 0; i < N; i++) { L.u[i] = (S)r[i]; L.uprev[i] = L.u[i]; }
+#pragma unroll
+        for (int k = 0; k < ODEMI_NP; k++) L.p[k] = (K)r[N + k];
+#pragma unroll
+        for (int i = 0; i < N; i++) L.k1[i] = (K)r[N + ODEMI_NP + 2 + i];
+        L.c.dt = r[2 * N + ODEMI_NP + 2];
+        L.c.nf += (P.dt_user != 0.0) ? 1 : 3;
+        L.c.dtpropose = L.c.dt;
+        return;
+    }
     double u0d[N], pd[ODEMI_NP];
     tsit5_load(L, P, j, u0d, pd);
     if (P.save_start) save_point<N>(L.c, P, j, L.c.t, L.u);
@@ -800,6 +871,19 @@ __device__ void tsit5_prep(const Params& P) {
     double u0d[N], pd[ODEMI_NP];
     tsit5_load(L, P, j, u0d, pd);
     tsit5_first_stage_dt(L, P, u0d, pd);
+    if (P.rec_in) {
+        double* r = P.rec_in + (i64)P.inv[j] * rec_in_words(N, ODEMI_NP);
+#pragma unroll
+        for (int i = 0; i < N; i++) r[i] = u0d[i];
+#pragma unroll
+        for (int k = 0; k < ODEMI_NP; k++) r[N + k] = pd[k];
+        r[N + ODEMI_NP] = L.c.t;
+        r[N + ODEMI_NP + 1] = L.c.tf;
+#pragma unroll
+        for (int i = 0; i < N; i++) r[N + ODEMI_NP + 2 + i] = (double)L.k1[i];
+        r[2 * N + ODEMI_NP + 2] = L.c.dt;
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < N; i++) P.pre_k1[(i64)i * P.N + j] = (double)L.k1[i];
     P.pre_dt[j] = L.c.dt;
@@ -1516,14 +1600,15 @@ __device__ void tsit5_persistent(const Params& P) {
     Tsit5Lane<S, K, N> L;
     WaveQueue q;
     bool active = false, exhausted = false;
-    i64 j = 0;
+    i64 j = 0, js = 0;
     for (;;) {
         bool need = refill_now(!active && !exhausted, active);
         i64 slot = wave_fetch(&P.counters[0], q, need);
         if (need) {
             if (slot < P.N) {
+                js = slot;
                 j = P.order ? P.order[slot] : slot;
-                tsit5_init(L, P, j);
+                tsit5_init(L, P, j, slot);
                 active = true;
             } else {
                 exhausted = true;
@@ -1533,7 +1618,8 @@ __device__ void tsit5_persistent(const Params& P) {
         if (active) {
             int rc = tsit5_attempt<SAVES>(L, P, j);
             if (rc) {
-                finish_lane<N>(L.c, P, j, L.u, rc);
+                if (!SAVES && P.rec_out) finish_lane_rec<N>(L.c, P, js, L.u, rc);
+                else finish_lane<N>(L.c, P, j, L.u, rc);
                 active = false;
             }
         }

@@ -87,7 +87,14 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
     float* wg_jbuf;
     double* pre_dt;
     double* pre_k1;
+    double* rec_in;
+    double* rec_out;
+    const int* inv;
 };
+// Doubles per queue-order record (the device code's rec_in_words /
+// rec_out_words for n states and np parameters, np >= 1).
+constexpr int rec_in_words(int n, int np) { return (2 * n + np + 3 + 1) & ~1; }
+constexpr int rec_out_words(int n) { return (n + 1 + 5 + 1) & ~1; }
 // ints per serving request-table entry (kReqOpts): nsaveat, ntstops,
 // save_start, save_end, save_end_user
 constexpr int kReqOpts = 8;
@@ -147,7 +154,7 @@ __global__ void __launch_bounds__(256) cost_scan_kernel(unsigned* counts) {
     }
 }
 __global__ void __launch_bounds__(256) cost_scatter_kernel(const double* t0, const double* tf, const int* nvec,
-                                                          int64_t N, unsigned* cursor, int* order) {
+                                                          int64_t N, unsigned* cursor, int* order, int* inv) {
     __shared__ unsigned h[kCostBins];
     for (int b = threadIdx.x; b < kCostBins; b += 256) h[b] = 0;
     __syncthreads();
@@ -165,7 +172,11 @@ __global__ void __launch_bounds__(256) cost_scatter_kernel(const double* t0, con
     __syncthreads();
     for (int q = 0; q < kCostItems; q++) {
         const int64_t j = base + (int64_t)q * 256 + threadIdx.x;
-        if (bin[q] >= 0) order[h[bin[q]] + rank[q]] = (int)j;
+        if (bin[q] >= 0) {
+            const unsigned pos = h[bin[q]] + rank[q];
+            order[pos] = (int)j;
+            if (inv) inv[j] = (int)pos;
+        }
     }
 }
 
@@ -263,6 +274,7 @@ struct odemi_rhs {
     hipFunction_t serve_init[2][2] = {}, serve_tick[2][2] = {};  // [alg][fp64, mixed]
     hipFunction_t serve_tick_sv[2][2] = {};                      // the same, requests with save times
     hipFunction_t tsit5_prep[2] = {}, rodas_prep = nullptr;      // start-up kernels [fp64, mixed]
+    hipFunction_t rec_unpack = nullptr;                          // queue-order results -> SoA outputs
     hipFunction_t wg_serve_init[2] = {}, wg_serve_tick[2] = {};  // workgroup serving [fp64, mixed]
 };
 
@@ -387,6 +399,8 @@ static std::string build_source(const char* problem_src, int n, int np, int flag
              " { odemi::tsit5_prep<double, double, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_prep_mixed(odemi::Params P)"
              " { odemi::tsit5_prep<double, float, ODEMI_N>(P); }\n";
+        s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_rec_unpack(odemi::Params P)"
+             " { odemi::rec_unpack<ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_init_f64(odemi::Params P)"
              " { odemi::tsit5_serve_init<double, double, ODEMI_N>(P); }\n";
         s << "extern \"C\" __global__ void __launch_bounds__(256) odemi_tsit5_serve_init_mixed(odemi::Params P)"
@@ -606,6 +620,7 @@ extern "C" odemi_rhs* odemi_compile_rhs(odemi_ctx* ctx, const char* problem_src,
         get("odemi_tsit5_plain_init", &r->tsit5_plain_init);
         get("odemi_tsit5_prep_f64", &r->tsit5_prep[0]);
         get("odemi_tsit5_prep_mixed", &r->tsit5_prep[1]);
+        get("odemi_rec_unpack", &r->rec_unpack);
         get("odemi_tsit5_plain_step", &r->tsit5_plain_step);
     }
     if (flags & ODEMI_COMPILE_RODAS5P) {
@@ -770,6 +785,15 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
         if (!plain) fn_prep = rhs->rodas_prep;
     }
     if (plain && mixed) { ctx->err = "the plain engine is FP64 only"; return -3; }
+    // queue-order records (final-state-only Tsit5 persistent engine, no save
+    // before the end point, no lane callbacks; not in the allocation-free form;
+    // $ODEMI_QUEUE_RECORDS=0 reads and writes the SoA arrays in place instead)
+    const char* qr = getenv("ODEMI_QUEUE_RECORDS");
+    const bool use_rec = !(qr && atoi(qr) == 0) && !rodas && !plain && !noalloc && fn_prep && rhs->rec_unpack &&
+                         a->sort_by_cost && N > 0 &&
+                         a->nsaveat == 0 && !a->save_everystep && !a->save_start && !(rhs->features & 6) &&
+                         nmax == rhs->n;
+    const int rec_np = std::max(rhs->np, 1);
     if (!fn_main || (plain && !fn_init)) { ctx->err = "requested kernel was not compiled for this problem"; return -3; }
     if (a->nvec && !use_wg && !plain_wg) { ctx->err = "uneven batches (nvec) need the workgroup Rodas5P kernels"; return -3; }
     if ((use_wg || plain_wg) && (rhs->features & kLaneHooks)) {
@@ -838,6 +862,7 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
     if (plain_wg) { add(N_ * 4); if (!a->nvec) add(N_ * 4); }   // slot -> request (all 0); nvec = nmax
     if (use_gj) add((size_t)grid * gj_ld(nmax) * gj_ld(nmax) * 4);
     if (fn_prep) { add(N_ * 8); if (!rodas) add(N_ * nmax * 8); }
+    if (use_rec) { add(N_ * 4); add(N_ * rec_in_words(nmax, rec_np) * 8); add(N_ * rec_out_words(nmax) * 8); }
     if (!dev) {
         add(N_ * nmax * 8); add(N_ * std::max(np, 1) * 8); add(N_ * 8); add(N_ * 8);
         add(std::max(a->nsaveat, 1) * 8);
@@ -909,9 +934,11 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
         if (clear_words(ctx, bins, kCostBins, s, noalloc)) return -1;
         hipLaunchKernelGGL(cost_hist_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, bins);
         hipLaunchKernelGGL(cost_scan_kernel, dim3(1), dim3(256), 0, s, bins);
-        hipLaunchKernelGGL(cost_scatter_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, bins, ord);
+        int* inv = use_rec ? (int*)ar.take(N_ * 4) : nullptr;
+        hipLaunchKernelGGL(cost_scatter_kernel, dim3(g), dim3(256), 0, s, P.t0, P.tf, P.nvec, N, bins, ord, inv);
         HIPCHK(ctx, hipGetLastError());
         P.order = ord;
+        P.inv = inv;
     }
     if (plain) {
         P.st_scalar = (double*)ar.take(16 * N_ * 8);
@@ -942,6 +969,11 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
         HIPCHK(ctx, hipGetLastError());
     }
     if (use_gj) P.wg_jbuf = (float*)ar.take((size_t)grid * gj_ld(nmax) * gj_ld(nmax) * 4);
+    if (use_rec) {
+        P.rec_in = (double*)ar.take(N_ * rec_in_words(nmax, rec_np) * 8);
+        P.rec_out = (double*)ar.take(N_ * rec_out_words(nmax) * 8);
+        if (!P.inv || !P.rec_in || !P.rec_out) { ctx->err = "workspace too small for the queue-order records"; return -1; }
+    }
     if (fn_prep) {
         P.pre_dt = (double*)ar.take(N_ * 8);
         if (!rodas) P.pre_k1 = (double*)ar.take(N_ * nmax * 8);
@@ -1036,6 +1068,10 @@ int solve_impl(odemi_ctx* ctx, odemi_rhs* rhs, odemi_solve_args* a, bool noalloc
         return 0;
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev1, s));
+    if (use_rec) {
+        if (launch(ctx, rhs->rec_unpack, (int)((N + 255) / 256), 256, P, s)) return -1;
+        launches++;
+    }
     if (!dev) {
         if (ms_ > 0) {
             HIPCHK(ctx, hipMemcpyAsync(a->t_out, P.t_out, N_ * ms_ * 8, hipMemcpyDeviceToHost, s));
