@@ -127,3 +127,40 @@ def test_host_arrays_with_pinned_outputs_equal_device_arrays():
             assert np.array_equal(out[key], ref[key], equal_nan=True), (rep, key)
         assert isinstance(out["u_out"].base, torch.Tensor) and out["u_out"].base.is_pinned()
     assert (out["retcode"] == 1).all() and (out["nsaved"] >= 9).all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_queue_order_records_equal_in_place(precision, monkeypatch):
+    """The final-state-only Tsit5 persistent engine reads its inputs and writes
+    its results as queue-order records (prep writes each trajectory's record
+    at its queue slot, odemi_rec_unpack restores the SoA outputs); with the
+    records switched off (ODEMI_QUEUE_RECORDS=0, in-place SoA access) every
+    output is the same bit for bit, from device-resident and from host arrays,
+    with and without the saved end point."""
+    import torch
+    N = 50001
+    d = P.lorenz_large(N=N)
+    ea = odemi.EnsembleMI355X(engine="persistent", precision=precision)
+    dev = torch.device("cuda:0")
+    T = lambda a: torch.as_tensor(a, dtype=torch.float64, device=dev).contiguous()  # noqa: E731
+    keys = ("u_final", "t_final", "retcode", "stats", "nsaved", "t_out", "u_out")
+    print(f"[case] lorenz N={N} {precision} queue-order records vs in place", flush=True)
+    for save_end in (True, False):
+        kw = dict(reltol=d["reltol"], abstol=d["abstol"], save_start=False, save_end=save_end)
+        res = []
+        for flag in ("1", "0"):
+            monkeypatch.setenv("ODEMI_QUEUE_RECORDS", flag)
+            od = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), T(d["u0"]), T(d["p"]), T(d["t0"]), T(d["tf"]), ea, **kw)
+            oh = odemi.solve_arrays(P.LORENZ, odemi.Tsit5(), d["u0"], d["p"], d["t0"], d["tf"], ea, **kw)
+            res.append({k: od[k].cpu().numpy() for k in keys})
+            res.append({k: oh[k] for k in keys})
+        ns = res[1]["nsaved"]
+        assert (ns == (1 if save_end else 0)).all()
+        for r in res[1:]:
+            for k in keys:
+                a, b = res[0][k], r[k]
+                if k in ("t_out", "u_out"):   # rows past nsaved are not written
+                    a, b = a[:1], b[:1]
+                    if not save_end:
+                        continue
+                assert np.array_equal(a, b, equal_nan=True), (precision, save_end, k)
