@@ -748,25 +748,24 @@ __device__ void finish_lane(Ctl& c, const Params& P, i64 j, const S* u, int rc) 
 // record whole at that slot; rec_unpack (one thread per trajectory) moves the
 // results to the SoA outputs with coalesced writes.  Input record, doubles:
 // u0[N] p[NP] t0 tf k1[N] dt.  Result record: u_final[N] t_final, then int32
-// nsaved retcode stats[7].  Used only without saves before the end point
-// (no save_start, saveat, save_everystep or callbacks): the end point's saved
-// copy is the final state.
+// retcode stats[7] (64 B for N = 3: one aligned sector per record).  Used only
+// without saves before the end point (no save_start, saveat, save_everystep
+// or callbacks): the end point's saved copy is the final state, and nsaved is
+// 1 exactly when finish_lane saves it.
 constexpr int rec_in_words(int n, int np) { return (2 * n + np + 3 + 1) & ~1; }
-constexpr int rec_out_words(int n) { return (n + 1 + 5 + 1) & ~1; }
+constexpr int rec_out_words(int n) { return (n + 1 + 4 + 1) & ~1; }
 
 template <int N, typename S>
 __device__ __forceinline__ void finish_lane_rec(const Ctl& c, const Params& P, i64 slot, const S* u, int rc) {
     double* r = P.rec_out + slot * rec_out_words(N);
-    const bool save = (rc == RC_SUCCESS || rc == RC_TERMINATED) && P.save_end;   // finish_lane's end point
 #pragma unroll
     for (int i = 0; i < N; i++) r[i] = (double)u[i];
     r[N] = c.t;
     int* q = (int*)(r + N + 1);
-    q[0] = c.nsaved + (save ? 1 : 0);
-    q[1] = rc;
-    q[2 + ST_NF] = c.nf; q[2 + ST_NACCEPT] = c.naccept; q[2 + ST_NREJECT] = c.nreject;
-    q[2 + ST_NJACS] = c.njacs; q[2 + ST_NW] = c.nw; q[2 + ST_NSOLVE] = c.nsolve;
-    q[2 + ST_NCONDITION] = c.ncond;
+    q[0] = rc;
+    q[1 + ST_NF] = c.nf; q[1 + ST_NACCEPT] = c.naccept; q[1 + ST_NREJECT] = c.nreject;
+    q[1 + ST_NJACS] = c.njacs; q[1 + ST_NW] = c.nw; q[1 + ST_NSOLVE] = c.nsolve;
+    q[1 + ST_NCONDITION] = c.ncond;
 }
 template <int N>
 __device__ void rec_unpack(const Params& P) {
@@ -774,7 +773,8 @@ __device__ void rec_unpack(const Params& P) {
     if (j >= P.N) return;
     const double* r = P.rec_out + (i64)P.inv[j] * rec_out_words(N);
     const int* q = (const int*)(r + N + 1);
-    const int ns = q[0];
+    const int rc = q[0];
+    const int ns = ((rc == RC_SUCCESS || rc == RC_TERMINATED) && P.save_end) ? 1 : 0;   // finish_lane's end point
 #pragma unroll
     for (int i = 0; i < N; i++) {
         if (P.u_final) P.u_final[(i64)i * P.N + j] = r[i];
@@ -783,10 +783,10 @@ __device__ void rec_unpack(const Params& P) {
     if (P.t_final) P.t_final[j] = r[N];
     if (ns > 0 && P.max_saves > 0) P.t_out[j] = r[N];
     if (P.nsaved) P.nsaved[j] = ns;
-    P.retcode[j] = q[1];
+    P.retcode[j] = rc;
     if (P.stats) {
 #pragma unroll
-        for (int k = 0; k < ST_COUNT; k++) P.stats[(i64)k * P.N + j] = q[2 + k];
+        for (int k = 0; k < ST_COUNT; k++) P.stats[(i64)k * P.N + j] = q[1 + k];
     }
 }
 

@@ -94,7 +94,7 @@ struct Params {  // must match odemi::Params in kernels/odemi_device.h
 // Doubles per queue-order record (the device code's rec_in_words /
 // rec_out_words for n states and np parameters, np >= 1).
 constexpr int rec_in_words(int n, int np) { return (2 * n + np + 3 + 1) & ~1; }
-constexpr int rec_out_words(int n) { return (n + 1 + 5 + 1) & ~1; }
+constexpr int rec_out_words(int n) { return (n + 1 + 4 + 1) & ~1; }
 // ints per serving request-table entry (kReqOpts): nsaveat, ntstops,
 // save_start, save_end, save_end_user
 constexpr int kReqOpts = 8;
